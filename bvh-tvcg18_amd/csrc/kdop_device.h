@@ -1,0 +1,95 @@
+// kdop_device.h -- device-side data layout shared by the kDOP kernels.
+//
+// Node record (AoS, one per tree node, 4K+8 bytes):
+//   float vol[K]   slab k: vol[2k] = lower, vol[2k+1] = upper
+//   uint32 m0      internal: left child (global index)  | flags
+//                  tet leaf: tet index (body local)      | flags
+//   uint32 m1      internal: right child (global index) | F_ROOT when branch root
+//                  tet leaf: F_ROOT when the leaf is a one-tet chunk's root
+// Flags of m0: F_LEAF (tet leaf), F_BRANCH (node lies in a branch chunk).
+// Super-chunk leaves of the reference tree (copies of a child chunk's root
+// volume) are not stored: a super node points straight at the child chunk's
+// root, resolved through one-element chunks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kdop_amd {
+
+constexpr uint32_t F_LEAF = 0x80000000u;
+constexpr uint32_t F_BRANCH = 0x40000000u;
+constexpr uint32_t F_ROOT = 0x80000000u;
+constexpr uint32_t IDX_MASK = 0x3FFFFFFFu;
+constexpr int kMaxLevels = 24;
+
+// Per-body descriptor, device resident (one entry per body of the context).
+struct BodyDev {
+  uint32_t node_base;  // global index of the root record (body-local node 0)
+  uint32_t nnodes;
+  uint32_t nt, nv;
+  uint32_t top_base;   // offset into the context's table of top-level chunk roots
+  uint32_t ntop;
+  uint32_t nbranch;    // branch chunks C_A
+  uint32_t H, L;
+  uint32_t branch_base;  // global index of the first branch-chunk node
+  uint32_t depth;        // longest root-record-to-leaf path
+  uint32_t pad0;
+  const uint32_t* tets;
+  const uint8_t* surf;
+  const float* X0;
+  const float* Y0;
+  const float* Z0;
+  float* X;
+  float* Y;
+  float* Z;
+};
+
+// Topology parameters of one body (kernel argument).
+struct TopoParams {
+  uint32_t node_base, H, L, nt;
+  uint32_t C[kMaxLevels];          // chunks per level (0 = top, H-1 = branches)
+  uint32_t E[kMaxLevels];          // elements per level (tets at H-1, chunks of level h+1 above)
+  uint32_t level_base[kMaxLevels]; // body-local node index of the level's first chunk
+};
+
+// Direction table as a kernel argument (host computed, bit-exact).
+struct DirTable {
+  float d[39];
+};
+
+// Work item of the traversal: a node pair known to overlap + body-pair index.
+struct Item {
+  uint32_t a, b, pair;
+};
+
+// Leaf pair output: body-pair index and the two tetrahedra.
+struct LeafPair {
+  uint32_t pair, ta, tb;
+};
+
+// Narrow-phase record per surface pair that produced contacts (adjoint input).
+struct NarrowRec {
+  uint32_t pair, ta, tb;
+  uint8_t face, nuniq, cmin, cmax;
+  float depth;
+  float n[3];
+};
+
+// Device counters of one traversal call.
+struct DevCounters {
+  unsigned long long tests;
+  unsigned long long descent;
+  unsigned long long branch_overlaps;
+  unsigned long long p1;
+  unsigned long long p2;
+  unsigned long long contacts;
+  unsigned long long narrow_pairs;
+  unsigned long long branch_pairs;   // C_A*C_B over body pairs whose roots overlap
+  unsigned long long root_tests;
+  unsigned int overflow;             // bit 0 frontier, bit 1 P1, bit 2 P2, bit 3 contacts
+  unsigned int pad;
+  unsigned int level_count[64];      // frontier sizes per level (level-synchronous mode)
+};
+
+}  // namespace kdop_amd

@@ -1,0 +1,680 @@
+// kdop_hip.hip -- host side of the MI355X kDOP library: context, device
+// resident bodies and trees, the refit / traversal / narrow-phase drivers and
+// the C ABI declared in include/kdop_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kdop_amd.h"
+#include "kdop_kernels.h"
+#include "kdop_meshgen.h"
+#include "tree_shape.h"
+
+using namespace kdop_amd;
+
+namespace {
+
+struct BodyHost {
+  uint32_t nv = 0, nt = 0, L = 0, H = 0, depth = 0;
+  std::vector<uint32_t> C, E, level_base;  // per level (0 = top)
+  uint32_t nnodes = 0, node_base = 0;
+  uint32_t top_base = 0, ntop = 0;
+  float *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr;
+  uint32_t* tets = nullptr;
+  uint8_t* surf = nullptr;
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+};
+
+}  // namespace
+
+struct kdop_ctx {
+  kdop_config cfg{};
+  int K = 8;
+  DirTable dt{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  std::vector<BodyHost> bodies;
+  // node pool (records of K+2 floats), body descriptors, top-level roots
+  DevBuf<float> nodes;
+  size_t nodes_used = 0;
+  DevBuf<BodyDev> dbodies;
+  std::vector<uint32_t> tops;
+  DevBuf<uint32_t> dtops;
+  // traversal state
+  DevBuf<uint2> dpairs;
+  DevBuf<Item> front[2];
+  DevBuf<LeafPair> p1, p2;
+  DevBuf<kdop_contact> contacts;
+  DevBuf<NarrowRec> recs;
+  DevCounters* dcnt = nullptr;
+  DevCounters* hcnt = nullptr;  // pinned
+  uint32_t last_npairs = 0;
+  bool have_traversal = false, have_p1 = false, have_narrow = false;
+  std::vector<uint2> last_pairs;
+  // refit scratch
+  DevBuf<uint32_t> plan;
+  DevBuf<float> poses;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double timing[4] = {0, 0, 0, 0};
+  uint32_t last_levels = 0;
+};
+
+namespace {
+
+int fail(kdop_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                               \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return fail((ctx), KDOP_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                             \
+  } while (0)
+
+template <class T>
+int ensure(kdop_ctx* c, DevBuf<T>& b, size_t n, bool keep = false) {
+  if (b.cap >= n) return KDOP_OK;
+  size_t cap = std::max<size_t>(n, b.cap * 2);
+  T* p = nullptr;
+  if (hipMalloc(&p, cap * sizeof(T)) != hipSuccess) {
+    cap = n;
+    hipError_t e = hipMalloc(&p, cap * sizeof(T));
+    if (e != hipSuccess) return fail(c, KDOP_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", cap * sizeof(T), hipGetErrorString(e));
+  }
+  if (keep && b.p && b.cap) {
+    HIPCHK(c, hipMemcpyAsync(p, b.p, b.cap * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (b.p) (void)hipFree(b.p);
+  b.p = p;
+  b.cap = cap;
+  return KDOP_OK;
+}
+template <class T>
+void release(DevBuf<T>& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+bool valid_K(int K) { return K == 6 || K == 8 || K == 12 || K == 14 || K == 18 || K == 20 || K == 26; }
+
+// geometry_direction_table.h:53-172, computed on the host in float so the
+// device sees the reference's bits.
+void make_dir_table(int K, DirTable& dt) {
+  auto unit = [](float x, float y, float z, float* o) {
+    float d = 0.0f;
+    d = d + x * x;
+    d = d + y * y;
+    d = d + z * z;
+    const float l = std::sqrt(d);
+    const float inv = 1.0f / l;
+    o[0] = x * inv; o[1] = y * inv; o[2] = z * inv;
+  };
+  float d3[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  float d4[4][3], d6[6][3];
+  const float raw4[4][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
+  const float raw6[6][3] = {{1, 1, 0}, {1, -1, 0}, {1, 0, 1}, {0, 1, 1}, {-1, 0, 1}, {0, -1, 1}};
+  for (int i = 0; i < 4; ++i) unit(raw4[i][0], raw4[i][1], raw4[i][2], d4[i]);
+  for (int i = 0; i < 6; ++i) unit(raw6[i][0], raw6[i][1], raw6[i][2], d6[i]);
+  int n = 0;
+  auto put = [&](float (*src)[3], int cnt) {
+    for (int i = 0; i < cnt; ++i, ++n) for (int j = 0; j < 3; ++j) dt.d[3 * n + j] = src[i][j];
+  };
+  switch (K / 2) {
+    case 3: put(d3, 3); break;
+    case 4: put(d4, 4); break;
+    case 6: put(d6, 6); break;
+    case 7: put(d3, 3); put(d4, 4); break;
+    case 9: put(d3, 3); put(d6, 6); break;
+    case 10: put(d4, 4); put(d6, 6); break;
+    case 13: put(d3, 3); put(d4, 4); put(d6, 6); break;
+  }
+}
+
+// ----------------------------------------------------------- K dispatch
+template <template <int> class F, class... Args>
+void dispatch_K(int K, Args&&... args) {
+  switch (K) {
+    case 6: F<6>::run(args...); break;
+    case 8: F<8>::run(args...); break;
+    case 12: F<12>::run(args...); break;
+    case 14: F<14>::run(args...); break;
+    case 18: F<18>::run(args...); break;
+    case 20: F<20>::run(args...); break;
+    case 26: F<26>::run(args...); break;
+  }
+}
+
+uint32_t host_chunk_elems(const BodyHost& b, int h, uint32_t c) { return (c + 1 < b.C[h]) ? b.L : b.E[h] - c * b.L; }
+uint32_t host_stride(const BodyHost& b, int h) { return (h == (int)b.H - 1) ? 2 * b.L - 1 : b.L - 1; }
+uint32_t host_resolved_root(const BodyHost& b, int h, uint32_t c) {
+  for (;;) {
+    const uint32_t n = host_chunk_elems(b, h, c);
+    if (h == (int)b.H - 1 || n >= 2) return b.node_base + b.level_base[h] + c * host_stride(b, h);
+    c = c * b.L;
+    ++h;
+  }
+}
+
+int upload_bodies(kdop_ctx* c) {
+  std::vector<BodyDev> h(c->bodies.size());
+  for (size_t i = 0; i < c->bodies.size(); ++i) {
+    const BodyHost& b = c->bodies[i];
+    BodyDev& d = h[i];
+    std::memset(&d, 0, sizeof(d));
+    d.node_base = b.node_base; d.nnodes = b.nnodes; d.nt = b.nt; d.nv = b.nv;
+    d.top_base = b.top_base; d.ntop = b.ntop; d.nbranch = b.C[b.H - 1]; d.H = b.H; d.L = b.L;
+    d.branch_base = b.node_base + b.level_base[b.H - 1];
+    d.depth = b.depth;
+    d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.X = b.X; d.Y = b.Y; d.Z = b.Z;
+  }
+  int rc = ensure(c, c->dbodies, h.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->dbodies.p, h.data(), h.size() * sizeof(BodyDev), hipMemcpyHostToDevice, c->stream));
+  rc = ensure(c, c->dtops, c->tops.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->dtops.p, c->tops.data(), c->tops.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+// ------------------------------------------------------------------- refit
+template <int K>
+struct RefitBranch {
+  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* cpre, uint32_t n) {
+    hipLaunchKernelGGL(k_refit_branch<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, cpre, n, c->nodes.p, c->dt);
+  }
+};
+template <int K>
+struct RefitSuper {
+  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* cpre, const uint32_t* cbase,
+                  const uint32_t* cn, uint32_t n) {
+    hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, cpre, cbase, cn, n, c->nodes.p);
+  }
+};
+template <int K>
+struct RefitRoot {
+  static void run(kdop_ctx* c, uint32_t n, const uint32_t* bb) {
+    hipLaunchKernelGGL(k_refit_root<K>, dim3(n), dim3(64), 0, c->stream, c->dbodies.p, bb, c->dtops.p, c->nodes.p);
+  }
+};
+
+// Transform (or copy, identity) + leaf/union passes for a batch of bodies.
+// pos/quat: device pointers (3n, 4n floats) or null with identity.
+int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
+  // plan layout (uint32): [bb n][vpre n+1][cpre n+1] then per super step s:
+  // [bb m][cpre m+1][cbase m][cn m]
+  std::vector<uint32_t> plan;
+  plan.reserve(8 * (size_t)n + 16);
+  uint32_t maxH = 1;
+  for (uint32_t i = 0; i < n; ++i) maxH = std::max(maxH, c->bodies[bodies[i]].H);
+  const size_t o_bb = 0;
+  for (uint32_t i = 0; i < n; ++i) plan.push_back(bodies[i]);
+  const size_t o_vpre = plan.size();
+  uint64_t acc = 0;
+  plan.push_back(0);
+  for (uint32_t i = 0; i < n; ++i) { acc += c->bodies[bodies[i]].nv; plan.push_back((uint32_t)acc); }
+  const uint64_t total_v = acc;
+  const size_t o_cpre = plan.size();
+  acc = 0;
+  plan.push_back(0);
+  for (uint32_t i = 0; i < n; ++i) { const BodyHost& b = c->bodies[bodies[i]]; acc += b.C[b.H - 1]; plan.push_back((uint32_t)acc); }
+  const uint64_t total_branch = acc;
+  struct Step { size_t o_bb, o_cpre, o_cbase, o_cn; uint32_t m, chunks; };
+  std::vector<Step> steps;
+  for (uint32_t s = 1; s < maxH; ++s) {
+    std::vector<uint32_t> bb, cpre{0}, cbase, cn;
+    uint32_t tot = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const BodyHost& b = c->bodies[bodies[i]];
+      if (b.H < s + 1) continue;
+      const int h = (int)b.H - 1 - (int)s;
+      bb.push_back(bodies[i]);
+      tot += b.C[h];
+      cpre.push_back(tot);
+      cbase.push_back(b.node_base + b.level_base[h]);
+      cn.push_back(host_chunk_elems(b, h, b.C[h] - 1));
+    }
+    if (bb.empty()) continue;
+    Step st;
+    st.m = (uint32_t)bb.size();
+    st.chunks = tot;
+    st.o_bb = plan.size(); plan.insert(plan.end(), bb.begin(), bb.end());
+    st.o_cpre = plan.size(); plan.insert(plan.end(), cpre.begin(), cpre.end());
+    st.o_cbase = plan.size(); plan.insert(plan.end(), cbase.begin(), cbase.end());
+    st.o_cn = plan.size(); plan.insert(plan.end(), cn.begin(), cn.end());
+    steps.push_back(st);
+  }
+  int rc = ensure(c, c->plan, plan.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  const uint32_t* P = c->plan.p;
+  if (total_v) {
+    const uint32_t blocks = (uint32_t)((total_v + 255) / 256);
+    hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, c->stream, c->dbodies.p, P + o_bb, P + o_vpre, n, pos_dev, quat_dev,
+                       identity ? 1 : 0);
+  }
+  dispatch_K<RefitBranch>(c->K, c, (uint32_t)total_branch, P + o_bb, P + o_cpre, n);
+  for (const Step& st : steps)
+    dispatch_K<RefitSuper>(c->K, c, st.chunks, P + st.o_bb, P + st.o_cpre, P + st.o_cbase, P + st.o_cn, st.m);
+  dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+// --------------------------------------------------------------- traversal
+template <int K>
+struct Seed {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t npairs) {
+    DevCounters* d = c->dcnt;
+    hipLaunchKernelGGL(k_seed<K>, dim3(npairs), dim3(64), 0, c->stream, T, c->front[0].p, &d->level_count[0],
+                       (uint32_t)c->front[0].cap);
+  }
+};
+template <int K>
+struct Expand {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t level) {
+    DevCounters* d = c->dcnt;
+    const uint32_t cap = (uint32_t)std::min(c->front[0].cap, c->front[1].cap);
+    hipLaunchKernelGGL(k_expand_level<K>, dim3(2048), dim3(256), 0, c->stream, T, c->front[level & 1].p, &d->level_count[level],
+                       c->front[(level + 1) & 1].p, &d->level_count[level + 1], cap);
+  }
+};
+
+struct Fills {
+  uint32_t p1, p2, c, r;
+};
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void kdop_default_config(kdop_config* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->K = 8;
+  cfg->device = 0;
+  cfg->chunk_bytes = 8000;
+  cfg->traversal = KDOP_TRAVERSAL_AUTO;
+}
+
+int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
+  if (!out) return KDOP_ERR_ARG;
+  *out = nullptr;
+  kdop_config def;
+  kdop_default_config(&def);
+  if (!cfg) cfg = &def;
+  if (!valid_K(cfg->K)) return KDOP_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return KDOP_ERR_HIP;
+  if (cfg->device < 0 || cfg->device >= ndev) return KDOP_ERR_ARG;
+  if (hipSetDevice(cfg->device) != hipSuccess) return KDOP_ERR_HIP;
+  kdop_ctx* c = new kdop_ctx();
+  c->cfg = *cfg;
+  c->K = cfg->K;
+  make_dir_table(c->K, c->dt);
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return KDOP_ERR_HIP; }
+  c->own_stream = true;
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
+  if (hipMalloc(&c->dcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipHostMalloc(&c->hcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  *out = c;
+  return KDOP_OK;
+}
+
+void kdop_destroy(kdop_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& b : c->bodies) {
+    (void)hipFree(b.X0); (void)hipFree(b.Y0); (void)hipFree(b.Z0);
+    (void)hipFree(b.X); (void)hipFree(b.Y); (void)hipFree(b.Z);
+    (void)hipFree(b.tets); (void)hipFree(b.surf);
+  }
+  release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs);
+  release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
+  release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
+  if (c->dcnt) (void)hipFree(c->dcnt);
+  if (c->hcnt) (void)hipHostFree(c->hcnt);
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* kdop_last_error(const kdop_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* kdop_stream(kdop_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int kdop_tree_shape(int32_t K, uint64_t chunk_bytes, uint64_t nt, uint64_t* out) {
+  Shape s;
+  if (!valid_K(K) || !chunk_shape(K, chunk_bytes, nt, s)) return -1;
+  out[0] = s.node_bytes; out[1] = s.n_perfect; out[2] = s.L; out[3] = s.C; out[4] = s.H;
+  return 0;
+}
+
+int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, const float* Z0, uint32_t nt, const uint32_t* tets,
+                  const uint8_t* surface, uint32_t* body_id) {
+  if (!c) return KDOP_ERR_ARG;
+  if (!X0 || !Y0 || !Z0 || !tets || !surface || !body_id) return fail(c, KDOP_ERR_ARG, "kdop_add_body: null argument");
+  if (nv == 0 || nt == 0) return fail(c, KDOP_ERR_ARG, "kdop_add_body: empty mesh (nv=%u nt=%u)", nv, nt);
+  if (nt >= (1u << 30)) return fail(c, KDOP_ERR_ARG, "kdop_add_body: too many tetrahedra (%u)", nt);
+  for (size_t i = 0; i < 4 * (size_t)nt; ++i)
+    if (tets[i] >= nv) return fail(c, KDOP_ERR_ARG, "kdop_add_body: tet vertex index %u out of range (nv=%u)", tets[i], nv);
+  Shape s;
+  if (!chunk_shape(c->K, c->cfg.chunk_bytes, nt, s))
+    return fail(c, KDOP_ERR_ARG, "kdop_add_body: chunk_bytes=%llu holds no two-leaf chunk for K=%d", (unsigned long long)c->cfg.chunk_bytes, c->K);
+  if (s.L > 32) return fail(c, KDOP_ERR_ARG, "kdop_add_body: %llu leaves per chunk; at most 32 supported (chunk_bytes <= 8191 gives at most 32)", (unsigned long long)s.L);
+  if (s.H > (uint64_t)kMaxLevels) return fail(c, KDOP_ERR_ARG, "kdop_add_body: %llu chunk levels", (unsigned long long)s.H);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  BodyHost b;
+  b.nv = nv; b.nt = nt; b.L = (uint32_t)s.L; b.H = (uint32_t)s.H;
+  b.C.assign(b.H, 0); b.E.assign(b.H, 0); b.level_base.assign(b.H, 0);
+  b.C[b.H - 1] = (uint32_t)s.C;
+  b.E[b.H - 1] = nt;
+  for (int h = (int)b.H - 1; h >= 1; --h) {
+    b.C[h - 1] = (uint32_t)chunks_above(b.C[h], b.L);
+    b.E[h - 1] = b.C[h];
+  }
+  // node counts: root record, then level by level
+  uint64_t off = 1;
+  for (uint32_t h = 0; h < b.H; ++h) {
+    b.level_base[h] = (uint32_t)off;
+    const uint32_t nlast = host_chunk_elems(b, (int)h, b.C[h] - 1);
+    const bool branch = (h == b.H - 1);
+    const uint64_t last_nodes = branch ? (2ull * nlast - 1) : (nlast >= 2 ? nlast - 1 : 0);
+    off += (uint64_t)(b.C[h] - 1) * host_stride(b, (int)h) + last_nodes;
+  }
+  if (c->nodes_used + off >= (1ull << 30)) return fail(c, KDOP_ERR_ARG, "kdop_add_body: node pool exceeds 2^30 nodes");
+  b.nnodes = (uint32_t)off;
+  b.node_base = (uint32_t)c->nodes_used;
+  uint32_t lg = 0;
+  while ((1u << lg) < b.L) ++lg;
+  b.depth = 1 + b.H * std::max(lg, 1u);
+  // top-level resolved roots
+  b.top_base = (uint32_t)c->tops.size();
+  for (uint32_t t = 0; t < b.C[0]; ++t) c->tops.push_back(host_resolved_root(b, 0, t));
+  b.ntop = b.C[0];
+  // device buffers
+  const size_t S = (size_t)c->K + 2;
+  int rc = ensure(c, c->nodes, (c->nodes_used + b.nnodes) * S, true);
+  if (rc) { c->tops.resize(b.top_base); return rc; }
+  auto alloc = [&](void** p, size_t bytes) -> int {
+    hipError_t e = hipMalloc(p, bytes);
+    return e == hipSuccess ? KDOP_OK : fail(c, KDOP_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+  };
+  if ((rc = alloc((void**)&b.X0, nv * 4)) || (rc = alloc((void**)&b.Y0, nv * 4)) || (rc = alloc((void**)&b.Z0, nv * 4)) ||
+      (rc = alloc((void**)&b.X, nv * 4)) || (rc = alloc((void**)&b.Y, nv * 4)) || (rc = alloc((void**)&b.Z, nv * 4)) ||
+      (rc = alloc((void**)&b.tets, 16 * (size_t)nt)) || (rc = alloc((void**)&b.surf, nt))) {
+    c->tops.resize(b.top_base);
+    return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(b.X0, X0, nv * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.Y0, Y0, nv * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.Z0, Z0, nv * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.tets, tets, 16 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.surf, surface, nt, hipMemcpyHostToDevice, c->stream));
+  c->nodes_used += b.nnodes;
+  const uint32_t id = (uint32_t)c->bodies.size();
+  c->bodies.push_back(b);
+  rc = upload_bodies(c);
+  if (rc) return rc;
+  // device construction: topology, then refit to the material coordinates
+  TopoParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt;
+  uint32_t total_chunks = 0;
+  for (uint32_t h = 0; h < b.H; ++h) { P.C[h] = b.C[h]; P.E[h] = b.E[h]; P.level_base[h] = b.level_base[h]; total_chunks += b.C[h]; }
+  // root record meta = 0
+  HIPCHK(c, hipMemsetAsync(c->nodes.p + (size_t)b.node_base * S, 0, S * sizeof(float), c->stream));
+  hipLaunchKernelGGL(k_build_topology, dim3((total_chunks + 63) / 64), dim3(64), 0, c->stream, P, c->nodes.p, (int)S, b.surf, total_chunks);
+  HIPCHK(c, hipGetLastError());
+  rc = run_refit(c, 1, &id, nullptr, nullptr, true);
+  if (rc) return rc;
+  *body_id = id;
+  return KDOP_OK;
+}
+
+int kdop_body_shape(kdop_ctx* c, uint32_t body, uint64_t* out) {
+  if (!c) return KDOP_ERR_ARG;
+  if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "unknown body %u", body);
+  const BodyHost& b = c->bodies[body];
+  out[0] = b.L; out[1] = b.C[b.H - 1]; out[2] = b.H; out[3] = b.nnodes; out[4] = b.depth;
+  return KDOP_OK;
+}
+
+int kdop_refit_device(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev) {
+  if (!c) return KDOP_ERR_ARG;
+  if (n == 0) return KDOP_OK;
+  if (!bodies || !pos_dev || !quat_dev) return fail(c, KDOP_ERR_ARG, "kdop_refit: null argument");
+  for (uint32_t i = 0; i < n; ++i)
+    if (bodies[i] >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_refit: unknown body %u", bodies[i]);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  return run_refit(c, n, bodies, pos_dev, quat_dev, false);
+}
+
+int kdop_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos, const float* quat) {
+  if (!c) return KDOP_ERR_ARG;
+  if (n == 0) return KDOP_OK;
+  if (!bodies || !pos || !quat) return fail(c, KDOP_ERR_ARG, "kdop_refit: null argument");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  int rc = ensure(c, c->poses, 7 * (size_t)n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->poses.p, pos, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->poses.p + 3 * (size_t)n, quat, 4 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  return kdop_refit_device(c, n, bodies, c->poses.p, c->poses.p + 3 * (size_t)n);
+}
+
+int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, kdop_counters* out) {
+  if (!c) return KDOP_ERR_ARG;
+  if (npairs && !body_pairs) return fail(c, KDOP_ERR_ARG, "kdop_traverse: null body_pairs");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  uint32_t levels = 0;
+  uint64_t est = 0;
+  std::vector<uint2> pairs(npairs);
+  for (uint32_t i = 0; i < npairs; ++i) {
+    const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
+    if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
+    pairs[i] = make_uint2(a, b);
+    levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
+    est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
+  }
+  if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
+  c->have_traversal = c->have_p1 = c->have_narrow = false;
+  int rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1));
+  if (rc) return rc;
+  if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
+  const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
+  const bool narrow = flags & KDOP_NARROW;
+  est = std::max<uint64_t>(est, 1 << 16);
+  if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
+  if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
+  if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
+  if (narrow) {
+    if ((rc = ensure(c, c->contacts, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
+    if ((rc = ensure(c, c->recs, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
+  }
+  Fills* dfill = reinterpret_cast<Fills*>(c->dcnt + 1);
+  uint64_t overflowed = 0;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(DevCounters) + sizeof(Fills), c->stream));
+    TravArgs T;
+    T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p; T.cnt = c->dcnt;
+    T.p1 = keep_p1 ? c->p1.p : nullptr; T.p2 = c->p2.p;
+    T.p1_fill = &dfill->p1; T.p2_fill = &dfill->p2;
+    T.p1_cap = keep_p1 ? (uint32_t)std::min<size_t>(c->p1.cap, 0xFFFFFFFFu) : 0u;
+    T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
+    T.npairs = npairs;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if (npairs) {
+      dispatch_K<Seed>(c->K, c, T, npairs);
+      for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    if (narrow && npairs) {
+      NarrowArgs A;
+      A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &dfill->p2; A.p2_cap = T.p2_cap;
+      A.contacts = c->contacts.p; A.c_fill = &dfill->c; A.c_cap = (uint32_t)std::min<size_t>(c->contacts.cap, 0xFFFFFFFFu);
+      A.recs = c->recs.p; A.r_fill = &dfill->r; A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
+      A.cnt = c->dcnt;
+      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, A);
+      HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const DevCounters& h = *c->hcnt;
+    const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+    bool again = false;
+    if (h.overflow & 1u) {
+      uint32_t mx = 0;
+      for (int l = 0; l < 64; ++l) mx = std::max(mx, h.level_count[l]);
+      const size_t want = std::max<size_t>(4 * c->front[0].cap, (size_t)mx * 2);
+      if ((rc = ensure(c, c->front[0], want)) || (rc = ensure(c, c->front[1], want))) return rc;
+      again = true;
+    } else {
+      if (keep_p1 && f.p1 > T.p1_cap) { if ((rc = ensure(c, c->p1, (size_t)f.p1 + f.p1 / 4))) return rc; again = true; }
+      if (f.p2 > T.p2_cap) { if ((rc = ensure(c, c->p2, (size_t)f.p2 + f.p2 / 4))) return rc; again = true; }
+      if (narrow && f.c > c->contacts.cap) { if ((rc = ensure(c, c->contacts, (size_t)f.c + f.c / 4))) return rc; again = true; }
+      if (narrow && f.r > c->recs.cap) { if ((rc = ensure(c, c->recs, (size_t)f.r + f.r / 4))) return rc; again = true; }
+    }
+    if (again) { ++overflowed; continue; }
+    float ms[3] = {0, 0, 0};
+    HIPCHK(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&ms[2], c->ev[0], c->ev[2]));
+    c->timing[0] = ms[0]; c->timing[1] = ms[1]; c->timing[2] = ms[2];
+    c->timing[3] = levels ? ms[0] / (double)(levels + 1) : 0.0;
+    c->last_levels = levels;
+    c->last_npairs = npairs;
+    c->last_pairs = pairs;
+    c->have_traversal = true;
+    c->have_p1 = keep_p1;
+    c->have_narrow = narrow;
+    if (out) {
+      std::memset(out, 0, sizeof(*out));
+      out->tests = h.tests;
+      out->descent_tests = h.descent;
+      out->branch_overlaps = h.branch_overlaps;
+      out->leaf_pairs = h.p1;
+      out->surface_pairs = f.p2;
+      out->contacts = narrow ? f.c : 0;
+      out->narrow_pairs = narrow ? h.narrow_pairs : 0;
+      out->branch_pairs = h.branch_pairs;
+      out->reference_tests = h.root_tests + h.branch_pairs + h.descent;
+      out->overflow = overflowed;
+    }
+    return KDOP_OK;
+  }
+  return fail(c, KDOP_ERR_CAPACITY, "kdop_traverse: buffers kept overflowing");
+}
+
+int kdop_get_pairs(kdop_ctx* c, int which, uint32_t* dst, uint64_t cap, int dst_is_device, uint64_t* n) {
+  if (!c || !n) return KDOP_ERR_ARG;
+  if (!c->have_traversal) return fail(c, KDOP_ERR_STATE, "kdop_get_pairs: no traversal yet");
+  if (which == 0 && !c->have_p1) return fail(c, KDOP_ERR_STATE, "kdop_get_pairs: P1 not kept (pass KDOP_KEEP_ALL_PAIRS)");
+  const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+  const uint64_t cnt = which == 0 ? f.p1 : f.p2;
+  *n = cnt;
+  if (!dst) return KDOP_OK;
+  if (cap < cnt) return fail(c, KDOP_ERR_CAPACITY, "kdop_get_pairs: need %llu pairs", (unsigned long long)cnt);
+  if (cnt == 0) return KDOP_OK;
+  const LeafPair* src = which == 0 ? c->p1.p : c->p2.p;
+  HIPCHK(c, hipMemcpyAsync(dst, src, cnt * sizeof(LeafPair), dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+int kdop_get_contacts(kdop_ctx* c, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n) {
+  if (!c || !n) return KDOP_ERR_ARG;
+  if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_get_contacts: last traversal ran without KDOP_NARROW");
+  const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+  *n = f.c;
+  if (!dst) return KDOP_OK;
+  if (cap < f.c) return fail(c, KDOP_ERR_CAPACITY, "kdop_get_contacts: need %u contacts", f.c);
+  if (f.c == 0) return KDOP_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, c->contacts.p, (size_t)f.c * sizeof(kdop_contact), dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+int kdop_get_world(kdop_ctx* c, uint32_t body, float* X, float* Y, float* Z) {
+  if (!c || !X || !Y || !Z) return KDOP_ERR_ARG;
+  if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "unknown body %u", body);
+  const BodyHost& b = c->bodies[body];
+  HIPCHK(c, hipMemcpyAsync(X, b.X, b.nv * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Y, b.Y, b.nv * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(Z, b.Z, b.nv * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint32_t* range, uint8_t* kind, float* root, uint64_t* n) {
+  if (!c || !n) return KDOP_ERR_ARG;
+  if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "unknown body %u", body);
+  const BodyHost& b = c->bodies[body];
+  *n = b.nnodes - 1;
+  if (!vol && !range && !kind && !root) return KDOP_OK;
+  if (cap < *n) return fail(c, KDOP_ERR_CAPACITY, "kdop_export_tree: need %u nodes", b.nnodes - 1);
+  const int K = c->K, S = K + 2;
+  std::vector<float> rec((size_t)b.nnodes * S);
+  HIPCHK(c, hipMemcpyAsync(rec.data(), c->nodes.p + (size_t)b.node_base * S, rec.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto meta = [&](uint32_t local, int w) {
+    uint32_t u;
+    std::memcpy(&u, &rec[(size_t)local * S + K + w], 4);
+    return u;
+  };
+  // tet ranges: children always have larger local indices within a level, and
+  // lower levels follow upper ones, so a reverse sweep sees children first.
+  std::vector<uint32_t> lo(b.nnodes, 0), hi(b.nnodes, 0);
+  for (int64_t i = (int64_t)b.nnodes - 1; i >= 1; --i) {
+    const uint32_t m0 = meta((uint32_t)i, 0), m1 = meta((uint32_t)i, 1);
+    if (m0 & F_LEAF) { lo[i] = hi[i] = m0 & IDX_MASK; continue; }
+    const uint32_t c0 = (m0 & IDX_MASK) - b.node_base, c1 = (m1 & IDX_MASK) - b.node_base;
+    lo[i] = lo[c0]; hi[i] = hi[c1];
+  }
+  for (uint32_t i = 1; i < b.nnodes; ++i) {
+    const uint32_t m0 = meta(i, 0);
+    if (vol) for (int k = 0; k < K; ++k) vol[(size_t)(i - 1) * K + k] = rec[(size_t)i * S + k];
+    if (range) { range[2 * (i - 1)] = lo[i]; range[2 * (i - 1) + 1] = hi[i]; }
+    if (kind) kind[i - 1] = (m0 & F_LEAF) ? 2 : ((m0 & F_BRANCH) ? 1 : 0);
+  }
+  if (root) for (int k = 0; k < K; ++k) root[k] = rec[k];
+  return KDOP_OK;
+}
+
+int kdop_last_timing(kdop_ctx* c, double* out) {
+  if (!c || !out) return KDOP_ERR_ARG;
+  for (int i = 0; i < 4; ++i) out[i] = c->timing[i];
+  return KDOP_OK;
+}
+
+int kdop_narrow_adjoint(kdop_ctx* c, uint32_t, const uint32_t*, const float*, float*, float*, float*) {
+  return fail(c, KDOP_ERR_STATE, "kdop_narrow_adjoint: not available in this build");
+}
+
+}  // extern "C"

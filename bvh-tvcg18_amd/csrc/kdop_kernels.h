@@ -1,0 +1,608 @@
+// kdop_kernels.h -- HIP kernels of the kDOP pipeline (build, refit, traversal,
+// narrow phase).  Included once, by kdop_hip.hip.
+#pragma once
+
+#include "kdop_device.h"
+#include "kdop_math.h"
+#include "kdop_narrow.h"
+#include "kdop_amd.h"
+
+namespace kdop_amd {
+
+// ------------------------------------------------------------- wave helpers
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, (unsigned)d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+__device__ inline unsigned long long wave_sum64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+// Reserve `cnt` slots per lane in a buffer filled through *ctr (one atomic per
+// wave).  Every lane of the wave must call it.  Returns this lane's first slot.
+__device__ inline uint32_t wave_reserve(uint32_t* ctr, uint32_t cnt) {
+  const uint32_t incl = wave_incl_scan(cnt);
+  const uint32_t total = __shfl(incl, 63, 64);
+  uint32_t base = 0;
+  if (lane_id() == 0 && total) base = atomicAdd(ctr, total);
+  base = __shfl(base, 0, 64);
+  return base + incl - cnt;
+}
+__device__ inline void wave_add64(unsigned long long* dst, unsigned long long v) {
+  const unsigned long long s = wave_sum64(v);
+  if (lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
+// ------------------------------------------------------------ node records
+template <int K>
+struct NodeRec {
+  float v[K];
+  uint32_t m0, m1;
+};
+
+template <int K>
+__device__ inline void load_rec(const float* __restrict__ nodes, uint32_t idx, NodeRec<K>& r) {
+  constexpr int S = K + 2;
+  const float* p = nodes + (size_t)idx * S;
+  if constexpr (S % 4 == 0) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < S / 4; ++i) {
+      const float4 x = q[i];
+      const float t[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int f = 4 * i + j;
+        if (f < K) r.v[f] = t[j];
+        else if (f == K) r.m0 = __float_as_uint(t[j]);
+        else r.m1 = __float_as_uint(t[j]);
+      }
+    }
+  } else {
+    const float2* q = reinterpret_cast<const float2*>(p);
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i) {
+      const float2 x = q[i];
+      const float t[2] = {x.x, x.y};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int f = 2 * i + j;
+        if (f < K) r.v[f] = t[j];
+        else if (f == K) r.m0 = __float_as_uint(t[j]);
+        else r.m1 = __float_as_uint(t[j]);
+      }
+    }
+  }
+}
+__device__ inline uint2 load_meta(const float* __restrict__ nodes, uint32_t idx, int S) {
+  return *reinterpret_cast<const uint2*>(nodes + (size_t)idx * S + (S - 2));
+}
+
+// geometry_overlap_dop_dop.h:12-24 with the strict interval test of
+// geometry_overlap_interval_interval.h:11-21 (no early exit; same result).
+template <int K>
+__device__ inline bool dop_overlap(const float* A, const float* B) {
+  bool sep = false;
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) sep |= (A[2 * k + 1] < B[2 * k]) | (B[2 * k + 1] < A[2 * k]);
+  return !sep;
+}
+
+// ============================================================== build
+// One thread per chunk (all levels of one body): writes the meta words of the
+// chunk's nodes in BFS order of the reference's median split
+// (kdop_make_tree.h:57-121 branch chunks, :123-186 super chunks).
+__device__ inline uint32_t chunk_elems(const TopoParams& P, int h, uint32_t c) {
+  return (c + 1 < P.C[h]) ? P.L : P.E[h] - c * P.L;
+}
+__device__ inline uint32_t chunk_stride(const TopoParams& P, int h) { return (h == (int)P.H - 1) ? 2 * P.L - 1 : P.L - 1; }
+// Root node of chunk (h, c) after skipping one-element super chunks.
+__device__ inline uint32_t resolved_root(const TopoParams& P, int h, uint32_t c) {
+  for (;;) {
+    const uint32_t n = chunk_elems(P, h, c);
+    if (h == (int)P.H - 1 || n >= 2) return P.node_base + P.level_base[h] + c * chunk_stride(P, h);
+    c = c * P.L;
+    ++h;
+  }
+}
+
+__global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int S, const uint8_t* __restrict__ surf,
+                                 uint32_t total_chunks) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_chunks) return;
+  int h = 0;
+  uint32_t c = g;
+  while (h < (int)P.H && c >= P.C[h]) { c -= P.C[h]; ++h; }
+  const bool branch = (h == (int)P.H - 1);
+  const uint32_t n = chunk_elems(P, h, c);
+  const uint32_t base = P.node_base + P.level_base[h] + c * chunk_stride(P, h);
+  const uint32_t first = c * P.L;  // first element (tet or child chunk) of the chunk
+  if (!branch && n < 2) return;    // no nodes: parents point at the resolved child
+  // BFS over element ranges; branch chunks store every range, super chunks
+  // only ranges of two or more elements.
+  uint16_t qlo[64], qhi[64];
+  int head = 0, tail = 0;
+  qlo[tail] = 0; qhi[tail] = (uint16_t)(n - 1); ++tail;
+  while (head < tail) {
+    const uint32_t lo = qlo[head], hi = qhi[head];
+    const uint32_t node = base + head;
+    uint32_t* meta = reinterpret_cast<uint32_t*>(nodes + (size_t)node * S + (S - 2));
+    const uint32_t rootflag = (branch && head == 0) ? F_ROOT : 0u;
+    ++head;
+    if (lo == hi) {  // branch leaf (super chunks never enqueue single ranges)
+      const uint32_t tet = first + lo;
+      meta[0] = F_LEAF | F_BRANCH | tet;
+      meta[1] = rootflag | ((surf[tet] & 15u) ? 0x40000000u : 0u);
+      continue;
+    }
+    const uint32_t mid = (lo + hi) >> 1;  // floor((first+last)/2): first is even (c*L), see header
+    uint32_t child[2];
+    const uint32_t rlo[2] = {lo, mid + 1}, rhi[2] = {mid, hi};
+    for (int s = 0; s < 2; ++s) {
+      if (branch || rlo[s] < rhi[s]) {
+        child[s] = base + tail;
+        qlo[tail] = (uint16_t)rlo[s]; qhi[tail] = (uint16_t)rhi[s]; ++tail;
+      } else {
+        child[s] = resolved_root(P, h + 1, first + rlo[s]);
+      }
+    }
+    meta[0] = (branch ? F_BRANCH : 0u) | child[0];
+    meta[1] = rootflag | child[1];
+  }
+}
+
+// ============================================================== refit
+// Vertex transform r = rotate(q, r0) + p (narrow_update_kdop_bvh.h:154-165),
+// batched over bodies: item i of the batch covers vertices [vpre[i], vpre[i+1]).
+__global__ void k_transform(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+                            const uint32_t* __restrict__ vpre, uint32_t nbatch, const float* __restrict__ pos,
+                            const float* __restrict__ quat, int identity) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= vpre[nbatch]) return;
+  uint32_t lo = 0, hi = nbatch;  // find i with vpre[i] <= g < vpre[i+1]
+  while (hi - lo > 1) {
+    const uint32_t m = (lo + hi) >> 1;
+    if (vpre[m] <= g) lo = m; else hi = m;
+  }
+  const BodyDev& B = bodies[batch_body[lo]];
+  const uint32_t v = g - vpre[lo];
+  const f3 r0 = f3{B.X0[v], B.Y0[v], B.Z0[v]};
+  if (identity) {  // make_tree refits to the material coordinates themselves
+    B.X[v] = r0.x; B.Y[v] = r0.y; B.Z[v] = r0.z;
+    return;
+  }
+  const float* q = quat + 4 * lo;
+  const float* p = pos + 3 * lo;
+  const f3 r = rotate3(q[0], q[1], q[2], q[3], r0);
+  B.X[v] = r.x + p[0];
+  B.Y[v] = r.y + p[1];
+  B.Z[v] = r.z + p[2];
+}
+
+// Locate the batch item owning global chunk g (cpre: prefix of chunk counts).
+__device__ inline uint32_t find_item(const uint32_t* __restrict__ cpre, uint32_t n, uint32_t g) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t m = (lo + hi) >> 1;
+    if (cpre[m] <= g) lo = m; else hi = m;
+  }
+  return lo;
+}
+
+// One 64-lane block per branch chunk: leaf DOPs (geometry_dop.h:48-74) from
+// the world vertices, unions (geometry_make_union.h:9-25) bottom-up in LDS,
+// then one coalesced write of the chunk's volumes.
+template <int K>
+__global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+                                                     const uint32_t* __restrict__ cpre, uint32_t nbatch, float* __restrict__ nodes,
+                                                     DirTable dt) {
+  constexpr int N = K / 2, S = K + 2;
+  __shared__ float vol[63][K];
+  __shared__ uint32_t meta0[63], meta1[63];
+  const uint32_t g = blockIdx.x;
+  if (g >= cpre[nbatch]) return;
+  const uint32_t item = find_item(cpre, nbatch, g);
+  const BodyDev& B = bodies[batch_body[item]];
+  const uint32_t c = g - cpre[item];
+  const uint32_t L = B.L;
+  const uint32_t n = (c + 1 < B.nbranch) ? L : B.nt - c * L;
+  const uint32_t nn = 2 * n - 1;
+  const uint32_t base = B.branch_base + c * (2 * L - 1);
+  const int lane = threadIdx.x;
+  for (uint32_t i = lane; i < nn; i += 64) {
+    const uint2 m = load_meta(nodes, base + i, S);
+    meta0[i] = m.x; meta1[i] = m.y;
+  }
+  __syncthreads();
+  // leaf slabs: lanes over (node, slab)
+  for (uint32_t w = lane; w < nn * N; w += 64) {
+    const uint32_t i = w / N, k = w - i * N;
+    const uint32_t m0 = meta0[i];
+    if (!(m0 & F_LEAF)) continue;
+    const uint32_t tet = m0 & IDX_MASK;
+    const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)tet);
+    const f3 d = f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]};
+    const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
+    float lo = FLT_MAX, hi = -FLT_MAX;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const f3 P = f3{B.X[vi[p]], B.Y[vi[p]], B.Z[vi[p]]};
+      const float proj = dot3(d, P);
+      lo = smin(lo, proj);
+      hi = smax(hi, proj);
+    }
+    vol[i][2 * k] = lo;
+    vol[i][2 * k + 1] = hi;
+  }
+  __syncthreads();
+  if (n == L && (L & (L - 1)) == 0) {
+    // perfect chunk: BFS order is the implicit heap, children of i at 2i+1, 2i+2
+    for (int first = (int)(L / 2) - 1, cnt = (int)(L / 2); cnt >= 1; cnt >>= 1, first = (first - 1) / 2) {
+      for (int w = lane; w < cnt * K; w += 64) {
+        const int i = first + w / K, f = w % K;
+        const float a = vol[2 * i + 1][f], b = vol[2 * i + 2][f];
+        vol[i][f] = (f & 1) ? smax(a, b) : smin(a, b);
+      }
+      __syncthreads();
+      if (first == 0) break;
+    }
+  } else {
+    // partial chunk: reverse BFS sweep (children always follow their parent)
+    for (int i = (int)nn - 1; i >= 0; --i) {
+      const uint32_t m0 = meta0[i];
+      if (!(m0 & F_LEAF)) {
+        const uint32_t c0 = (m0 & IDX_MASK) - base, c1 = (meta1[i] & IDX_MASK) - base;
+        if (lane < K) {
+          const float a = vol[c0][lane], b = vol[c1][lane];
+          vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t w = lane; w < nn * K; w += 64) {
+    const uint32_t i = w / K, f = w - i * K;
+    nodes[(size_t)(base + i) * S + f] = vol[i][f];
+  }
+}
+
+// One 64-lane block per super chunk of level h (per body h = H-1-step):
+// reverse BFS sweep over the chunk's internal nodes; children outside the
+// chunk are resolved roots of level h+1, refitted by the previous launch
+// (kdop_refit_tree.h:82-115).
+template <int K>
+__global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+                                                    const uint32_t* __restrict__ cpre, const uint32_t* __restrict__ chunk_base,
+                                                    const uint32_t* __restrict__ chunk_n, uint32_t nbatch, float* __restrict__ nodes) {
+  constexpr int S = K + 2;
+  __shared__ float vol[32][K];
+  __shared__ uint32_t meta0[32], meta1[32];
+  const uint32_t g = blockIdx.x;
+  if (g >= cpre[nbatch]) return;
+  const uint32_t item = find_item(cpre, nbatch, g);
+  const uint32_t c = g - cpre[item];
+  const BodyDev& B = bodies[batch_body[item]];
+  const uint32_t L = B.L;
+  const uint32_t nlast = chunk_n[item];  // elements of the level's last chunk
+  const uint32_t ncnk = cpre[item + 1] - cpre[item];
+  const uint32_t n = (c + 1 < ncnk) ? L : nlast;
+  if (n < 2) return;
+  const uint32_t nn = n - 1;
+  const uint32_t base = chunk_base[item] + c * (L - 1);
+  const int lane = threadIdx.x;
+  for (uint32_t i = lane; i < nn; i += 64) {
+    const uint2 m = load_meta(nodes, base + i, S);
+    meta0[i] = m.x; meta1[i] = m.y;
+  }
+  __syncthreads();
+  for (int i = (int)nn - 1; i >= 0; --i) {
+    const uint32_t c0 = meta0[i] & IDX_MASK, c1 = meta1[i] & IDX_MASK;
+    if (lane < K) {
+      const float a = (c0 >= base && c0 < base + nn) ? vol[c0 - base][lane] : nodes[(size_t)c0 * S + lane];
+      const float b = (c1 >= base && c1 < base + nn) ? vol[c1 - base][lane] : nodes[(size_t)c1 * S + lane];
+      vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
+    }
+    __syncthreads();
+  }
+  for (uint32_t w = lane; w < nn * K; w += 64) {
+    const uint32_t i = w / K, f = w - i * K;
+    nodes[(size_t)(base + i) * S + f] = vol[i][f];
+  }
+  (void)B;
+}
+
+// Root volume of each body = union of its top-level chunk roots
+// (kdop_refit_tree.h:154-161).  One 64-lane block per batch item.
+template <int K>
+__global__ __launch_bounds__(64) void k_refit_root(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+                                                   const uint32_t* __restrict__ tops, float* __restrict__ nodes) {
+  constexpr int S = K + 2;
+  const BodyDev& B = bodies[batch_body[blockIdx.x]];
+  const int lane = threadIdx.x;
+  if (lane >= K) return;
+  float acc = nodes[(size_t)tops[B.top_base] * S + lane];
+  for (uint32_t t = 1; t < B.ntop; ++t) {
+    const float x = nodes[(size_t)tops[B.top_base + t] * S + lane];
+    acc = (lane & 1) ? smax(acc, x) : smin(acc, x);
+  }
+  nodes[(size_t)B.node_base * S + lane] = acc;
+}
+
+// ============================================================== traversal
+struct TravArgs {
+  const float* nodes;
+  const BodyDev* bodies;
+  const uint2* pairs;        // body pair ids
+  const uint32_t* tops;
+  DevCounters* cnt;
+  LeafPair* p1;              // may be null (P1 not kept)
+  LeafPair* p2;
+  uint32_t* p1_fill;
+  uint32_t* p2_fill;
+  uint32_t p1_cap, p2_cap;
+  uint32_t npairs;
+};
+
+// Result of expanding one work item: the child node ids and three 4-bit masks
+// over the child combinations k = 2*i + j (child i of a, child j of b).
+struct Expansion {
+  uint32_t ca[2], cb[2];  // child node ids
+  uint32_t ta[2], tb[2];  // tet ids when the child is a tet leaf
+  uint32_t push;          // overlapping combinations to push as new items
+  uint32_t leaf;          // overlapping tet-leaf pairs (P1)
+  uint32_t surf;          // subset of leaf where both tets have a surface face (P2)
+  uint32_t tests, descent, broot;
+};
+
+// Test the child pairs of work item (a, b), following the reference's
+// expansion rule inside branch chunks (kdop_tandem_traversal.h:122-154) and
+// treating branch roots as leaves while either side is still above the
+// branch level (super-chunk levels, which the reference does not traverse).
+template <int K>
+__device__ inline void expand_item(const float* __restrict__ nodes, uint32_t a, uint32_t b, Expansion& x) {
+  constexpr int S = K + 2;
+  const uint2 ma = load_meta(nodes, a, S);
+  const uint2 mb = load_meta(nodes, b, S);
+  const bool a_br = ma.x & F_BRANCH, b_br = mb.x & F_BRANCH;
+  bool exp_a, exp_b;
+  if (!a_br && !b_br) { exp_a = true; exp_b = true; }
+  else if (!a_br) { exp_a = true; exp_b = false; }
+  else if (!b_br) { exp_a = false; exp_b = true; }
+  else { exp_a = !(ma.x & F_LEAF); exp_b = !(mb.x & F_LEAF); }
+  x.ca[0] = exp_a ? (ma.x & IDX_MASK) : a;
+  x.ca[1] = ma.y & IDX_MASK;
+  x.cb[0] = exp_b ? (mb.x & IDX_MASK) : b;
+  x.cb[1] = mb.y & IDX_MASK;
+  NodeRec<K> A0, A1, B0, B1;
+  load_rec<K>(nodes, x.ca[0], A0);
+  if (exp_a) load_rec<K>(nodes, x.ca[1], A1);
+  load_rec<K>(nodes, x.cb[0], B0);
+  if (exp_b) load_rec<K>(nodes, x.cb[1], B1);
+  x.ta[0] = A0.m0 & IDX_MASK; x.ta[1] = A1.m0 & IDX_MASK;
+  x.tb[0] = B0.m0 & IDX_MASK; x.tb[1] = B1.m0 & IDX_MASK;
+  x.push = 0; x.leaf = 0; x.surf = 0;
+  x.tests = 0; x.descent = 0; x.broot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = k >> 1, j = k & 1;
+    if ((i == 1 && !exp_a) || (j == 1 && !exp_b)) continue;
+    const NodeRec<K>& X = i ? A1 : A0;
+    const NodeRec<K>& Y = j ? B1 : B0;
+    ++x.tests;
+    const bool xb = X.m0 & F_BRANCH, yb = Y.m0 & F_BRANCH;
+    const bool both_roots = xb && yb && (X.m1 & F_ROOT) && (Y.m1 & F_ROOT);
+    if (xb && yb && !both_roots) ++x.descent;
+    if (!dop_overlap<K>(X.v, Y.v)) continue;
+    if (both_roots) ++x.broot;
+    if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
+      x.leaf |= 1u << k;
+      if ((X.m1 & 0x40000000u) && (Y.m1 & 0x40000000u)) x.surf |= 1u << k;
+    } else {
+      x.push |= 1u << k;
+    }
+  }
+}
+
+// Write the leaf pairs of one lane (P1 when kept, P2 for the surface subset).
+// Every lane of the wave must call it.
+__device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expansion& x) {
+  if (T.p1) {
+    const uint32_t o1 = wave_reserve(T.p1_fill, __popc(x.leaf));
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if ((x.leaf >> k) & 1u) {
+        const uint32_t o = o1 + __popc(x.leaf & ((1u << k) - 1u));
+        if (o < T.p1_cap) T.p1[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+      }
+  }
+  const uint32_t o2 = wave_reserve(T.p2_fill, __popc(x.surf));
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if ((x.surf >> k) & 1u) {
+      const uint32_t o = o2 + __popc(x.surf & ((1u << k) - 1u));
+      if (o < T.p2_cap) T.p2[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+    }
+}
+
+// Seed: one 64-lane block per body pair.  Root-vs-root test
+// (kdop_tandem_traversal.h:164-165), then every pair of top-level chunk roots.
+template <int K>
+__global__ __launch_bounds__(64) void k_seed(TravArgs T, Item* __restrict__ out, uint32_t* __restrict__ out_fill, uint32_t out_cap) {
+  constexpr int S = K + 2;
+  const uint32_t pair = blockIdx.x;
+  const uint2 bp = T.pairs[pair];
+  const BodyDev& BA = T.bodies[bp.x];
+  const BodyDev& BB = T.bodies[bp.y];
+  NodeRec<K> ra, rb;
+  load_rec<K>(T.nodes, BA.node_base, ra);
+  load_rec<K>(T.nodes, BB.node_base, rb);
+  const bool ov = dop_overlap<K>(ra.v, rb.v);
+  if (lane_id() == 0) {
+    atomicAdd(&T.cnt->root_tests, 1ull);
+    atomicAdd(&T.cnt->tests, 1ull);
+    if (ov) atomicAdd(&T.cnt->branch_pairs, (unsigned long long)BA.nbranch * BB.nbranch);
+  }
+  if (!ov) return;
+  const uint32_t combos = BA.ntop * BB.ntop;
+  unsigned long long tests = 0, broot = 0, p1s = 0;
+  for (uint32_t base = 0; base < combos; base += 64) {
+    const uint32_t w = base + lane_id();
+    Expansion x;
+    x.push = x.leaf = x.surf = 0;
+    x.ca[0] = x.cb[0] = 0;
+    if (w < combos) {
+      const uint32_t i = w / BB.ntop, j = w - i * BB.ntop;
+      x.ca[0] = T.tops[BA.top_base + i];
+      x.cb[0] = T.tops[BB.top_base + j];
+      NodeRec<K> X, Y;
+      load_rec<K>(T.nodes, x.ca[0], X);
+      load_rec<K>(T.nodes, x.cb[0], Y);
+      x.ta[0] = X.m0 & IDX_MASK;
+      x.tb[0] = Y.m0 & IDX_MASK;
+      ++tests;
+      const bool both_roots = (X.m0 & F_BRANCH) && (Y.m0 & F_BRANCH) && (X.m1 & F_ROOT) && (Y.m1 & F_ROOT);
+      if (dop_overlap<K>(X.v, Y.v)) {
+        if (both_roots) ++broot;
+        if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
+          x.leaf = 1u;
+          x.surf = ((X.m1 & 0x40000000u) && (Y.m1 & 0x40000000u)) ? 1u : 0u;
+        } else {
+          x.push = 1u;
+        }
+      }
+    }
+    const uint32_t o = wave_reserve(out_fill, x.push);
+    if (x.push) {
+      if (o < out_cap) out[o] = Item{x.ca[0], x.cb[0], pair};
+      else atomicOr(&T.cnt->overflow, 1u);
+    }
+    p1s += x.leaf;
+    emit_leaves(T, pair, x);
+  }
+  wave_add64(&T.cnt->tests, tests);
+  wave_add64(&T.cnt->branch_overlaps, broot);
+  wave_add64(&T.cnt->p1, p1s);
+  (void)S;
+}
+
+// One level of the level-synchronous traversal: expand every item of `in`.
+template <int K>
+__global__ __launch_bounds__(256) void k_expand_level(TravArgs T, const Item* __restrict__ in, const uint32_t* __restrict__ in_fill,
+                                                      Item* __restrict__ out, uint32_t* __restrict__ out_fill, uint32_t cap) {
+  const uint32_t n = min(*in_fill, cap);
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
+  for (uint32_t base = wave * 64; base < n; base += waves * 64) {
+    const uint32_t i = base + lane_id();
+    Expansion x;
+    x.push = x.leaf = x.surf = 0;
+    x.tests = x.descent = x.broot = 0;
+    Item it{0, 0, 0};
+    if (i < n) {
+      it = in[i];
+      expand_item<K>(T.nodes, it.a, it.b, x);
+    }
+    tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
+    const uint32_t o = wave_reserve(out_fill, __popc(x.push));
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if ((x.push >> k) & 1u) {
+        const uint32_t slot = o + __popc(x.push & ((1u << k) - 1u));
+        if (slot < cap) out[slot] = Item{x.ca[k >> 1], x.cb[k & 1], it.pair};
+        else atomicOr(&T.cnt->overflow, 1u);
+      }
+    emit_leaves(T, it.pair, x);
+  }
+  wave_add64(&T.cnt->tests, tests);
+  wave_add64(&T.cnt->descent, descent);
+  wave_add64(&T.cnt->branch_overlaps, broot);
+  wave_add64(&T.cnt->p1, p1s);
+}
+
+// ============================================================== narrow phase
+struct NarrowArgs {
+  const BodyDev* bodies;
+  const uint2* pairs;
+  const LeafPair* p2;
+  const uint32_t* p2_fill;
+  uint32_t p2_cap;
+  kdop_contact* contacts;
+  uint32_t* c_fill;
+  uint32_t c_cap;
+  NarrowRec* recs;
+  uint32_t* r_fill;
+  uint32_t r_cap;
+  DevCounters* cnt;
+};
+
+__device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
+  const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)t);
+  const uint32_t v[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) P[q] = f3{B.X[v[q]], B.Y[v[q]], B.Z[v[q]]};
+}
+
+// One thread per surface pair: SAT, most opposing normal, contact points
+// (contacts/geometry_contacts_tetrahedron_tetrahedron.h:1312-1345).
+__global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
+  const uint32_t n = min(*A.p2_fill, A.p2_cap);
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  unsigned long long hits = 0;
+  for (uint32_t base = wave * 64; base < n; base += waves * 64) {
+    const uint32_t i = base + lane_id();
+    f3 raw[kMaxCand];
+    uint64_t mask = 0;
+    NarrowResult r;
+    r.nuniq = 0; r.face = -1;
+    LeafPair lp{0, 0, 0};
+    if (i < n) {
+      lp = A.p2[i];
+      const uint2 bp = A.pairs[lp.pair];
+      const BodyDev& BA = A.bodies[bp.x];
+      const BodyDev& BB = A.bodies[bp.y];
+      f3 PA[4], PB[4];
+      gather_tet(BA, lp.ta, PA);
+      gather_tet(BB, lp.tb, PB);
+      r = narrow_pair(PA, PB, BA.surf[lp.ta] & 15u, BB.surf[lp.tb] & 15u, raw, &mask);
+    }
+    const uint32_t nu = (uint32_t)r.nuniq;
+    const uint32_t oc = wave_reserve(A.c_fill, nu);
+    const uint32_t orr = wave_reserve(A.r_fill, nu ? 1u : 0u);
+    if (nu) {
+      ++hits;
+      uint32_t k = 0;
+      for (int c = 0; c < r.ncand; ++c) {
+        if (!((mask >> c) & 1ull)) continue;
+        if (oc + k < A.c_cap) {
+          kdop_contact ct;
+          ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = k;
+          ct.p[0] = raw[c].x; ct.p[1] = raw[c].y; ct.p[2] = raw[c].z;
+          ct.n[0] = r.n.x; ct.n[1] = r.n.y; ct.n[2] = r.n.z;
+          ct.depth = r.depth;
+          A.contacts[oc + k] = ct;
+        }
+        ++k;
+      }
+      if (orr < A.r_cap) {
+        NarrowRec rec;
+        rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
+        rec.face = (uint8_t)r.face; rec.nuniq = (uint8_t)nu; rec.cmin = r.cmin; rec.cmax = r.cmax;
+        rec.depth = r.depth; rec.n[0] = r.n.x; rec.n[1] = r.n.y; rec.n[2] = r.n.z;
+        A.recs[orr] = rec;
+      }
+    }
+  }
+  wave_add64(&A.cnt->narrow_pairs, hits);
+}
+
+}  // namespace kdop_amd

@@ -1,0 +1,266 @@
+"""Python harness over the MI355X kDOP library (libkdop_amd.so, C ABI).
+
+Mirrors the reference's KDOP call shapes (diku-dk/bvh-tvcg18):
+
+* ``Context.make_tree(mesh)``            ~ kdop::make_tree (kdop_make_tree.h:190-288)
+* ``Context.refit(bodies, pos, quat)``   ~ narrow::update_kdop_bvh (narrow_update_kdop_bvh.h:126-176)
+* ``Context.tandem_traversal(pairs)``    ~ kdop::tandem_traversal (kdop_tandem_traversal.h:198-226)
+* ``box_mesh`` / ``mesh_reorder`` / ``compute_surface_map`` prepare inputs as
+  narrow::Tetramesh::set_tetramesh_shape does (narrow_shape_types.h:161-210).
+
+Every compute call goes to the HIP library; a missing library raises.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import (CONTACT_DTYPE, KDOP_KEEP_ALL_PAIRS, KDOP_NARROW, KDOP_TRAVERSAL_AUTO,  # noqa: F401
+                   KDOP_TRAVERSAL_LEVELS, KDOP_TRAVERSAL_PERSISTENT, Config, Counters, lib, ptr)
+
+SUPPORTED_K = (6, 8, 12, 14, 18, 20, 26)
+
+
+class KdopError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kdop error {code}: {msg}")
+        self.code = code
+
+
+@dataclass
+class Mesh:
+    """Tetrahedral mesh: SoA vertex coordinates, tets (nt x 4, vertices i j k m),
+    surface bits (bit v: face opposite vertex v lies on the surface)."""
+    X: np.ndarray
+    Y: np.ndarray
+    Z: np.ndarray
+    tets: np.ndarray
+    surface: np.ndarray
+
+    @property
+    def nv(self):
+        return int(self.X.shape[0])
+
+    @property
+    def nt(self):
+        return int(self.tets.shape[0])
+
+    def points(self):
+        return np.stack([self.X, self.Y, self.Z], axis=1)
+
+
+# ------------------------------------------------------------ mesh preparation
+def box_mesh_raw(n, size=1.0, jitter=0.0, seed=0):
+    """Box grid of n^3 cubes x 5 tets in grid order (no reorder, no surface map)."""
+    L = lib()
+    nv = ctypes.c_uint32()
+    nt = ctypes.c_uint32()
+    L.kdop_box_mesh_size(n, ctypes.byref(nv), ctypes.byref(nt))
+    X = np.empty(nv.value, np.float32)
+    Y = np.empty(nv.value, np.float32)
+    Z = np.empty(nv.value, np.float32)
+    T = np.empty((nt.value, 4), np.uint32)
+    rc = L.kdop_make_box_mesh(n, float(size), float(jitter), int(seed), ptr(X), ptr(Y), ptr(Z), ptr(T))
+    if rc:
+        raise KdopError(rc, "kdop_make_box_mesh failed")
+    return X, Y, Z, T
+
+
+def mesh_reorder(X, Y, Z, tets):
+    """Morton reorder of tets, first-touch vertex renumbering (kdop_mesh_reorder.h:172-353)."""
+    L = lib()
+    nv, nt = X.shape[0], tets.shape[0]
+    Xo = np.empty(nv, np.float32)
+    Yo = np.empty(nv, np.float32)
+    Zo = np.empty(nv, np.float32)
+    To = np.empty((nt, 4), np.uint32)
+    tets = np.ascontiguousarray(tets, np.uint32)
+    rc = L.kdop_mesh_reorder(nv, ptr(X), ptr(Y), ptr(Z), nt, ptr(tets), ptr(Xo), ptr(Yo), ptr(Zo), ptr(To))
+    if rc:
+        raise KdopError(rc, "kdop_mesh_reorder failed")
+    return Xo, Yo, Zo, To
+
+
+def compute_surface_map(nv, tets):
+    """Surface bits per tet (mesh_array_compute_surface_map.h:41-76)."""
+    L = lib()
+    tets = np.ascontiguousarray(tets, np.uint32)
+    out = np.empty(tets.shape[0], np.uint8)
+    rc = L.kdop_compute_surface_map(nv, tets.shape[0], ptr(tets), ptr(out))
+    if rc:
+        raise KdopError(rc, "kdop_compute_surface_map failed")
+    return out
+
+
+def box_mesh(n, size=1.0, jitter=0.0, seed=0, reorder=True):
+    """Morton-ordered box-grid mesh with its surface map."""
+    X, Y, Z, T = box_mesh_raw(n, size, jitter, seed)
+    if reorder:
+        X, Y, Z, T = mesh_reorder(X, Y, Z, T)
+    S = compute_surface_map(X.shape[0], T)
+    return Mesh(X, Y, Z, T, S)
+
+
+def tree_shape(K, chunk_bytes, nt):
+    """Reference chunked-tree shape: dict(node_bytes, n_perfect, L, C, H)."""
+    out = np.zeros(5, np.uint64)
+    if lib().kdop_tree_shape(K, chunk_bytes, nt, ptr(out)):
+        raise KdopError(-1, f"no tree shape for K={K} chunk_bytes={chunk_bytes} nt={nt}")
+    return dict(zip(("node_bytes", "n_perfect", "L", "C", "H"), (int(v) for v in out)))
+
+
+def rng_uniform(seed, stream, n):
+    out = np.empty(n, np.float64)
+    lib().kdop_rng_uniform(seed, stream, n, ptr(out))
+    return out
+
+
+def rng_normal(seed, stream, n):
+    out = np.empty(n, np.float64)
+    lib().kdop_rng_normal(seed, stream, n, ptr(out))
+    return out
+
+
+def rng_quaternions(seed, stream, n):
+    out = np.empty((n, 4), np.float32)
+    lib().kdop_rng_quaternions(seed, stream, n, ptr(out))
+    return out
+
+
+# -------------------------------------------------------------------- context
+class Context:
+    """One HIP stream + device-resident bodies and trees (kdop_create)."""
+
+    def __init__(self, K=8, chunk_bytes=8000, device=0, traversal=KDOP_TRAVERSAL_AUTO, queue_depth=0, waves_per_cu=0):
+        if K not in SUPPORTED_K:
+            raise ValueError(f"K={K} not in {SUPPORTED_K}")
+        self._lib = lib()
+        cfg = Config()
+        self._lib.kdop_default_config(ctypes.byref(cfg))
+        cfg.K = K
+        cfg.chunk_bytes = chunk_bytes
+        cfg.device = device
+        cfg.traversal = traversal
+        cfg.queue_depth = queue_depth
+        cfg.waves_per_cu = waves_per_cu
+        h = ctypes.c_void_p()
+        rc = self._lib.kdop_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc:
+            raise KdopError(rc, "kdop_create failed (no HIP device?)")
+        self._h = h
+        self.K = K
+        self.meshes = []
+        self._last_npairs = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.kdop_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise KdopError(rc, self._lib.kdop_last_error(self._h).decode())
+
+    # -- kdop::make_tree
+    def make_tree(self, mesh):
+        X = np.ascontiguousarray(mesh.X, np.float32)
+        Y = np.ascontiguousarray(mesh.Y, np.float32)
+        Z = np.ascontiguousarray(mesh.Z, np.float32)
+        T = np.ascontiguousarray(mesh.tets, np.uint32)
+        S = np.ascontiguousarray(mesh.surface, np.uint8)
+        bid = ctypes.c_uint32()
+        self._check(self._lib.kdop_add_body(self._h, X.shape[0], ptr(X), ptr(Y), ptr(Z), T.shape[0], ptr(T), ptr(S),
+                                            ctypes.byref(bid)))
+        self.meshes.append(mesh)
+        return bid.value
+
+    def shape(self, body):
+        out = np.zeros(5, np.uint64)
+        self._check(self._lib.kdop_body_shape(self._h, body, ptr(out)))
+        return dict(zip(("L", "C", "H", "nodes", "depth"), (int(v) for v in out)))
+
+    # -- narrow::update_kdop_bvh
+    def refit(self, bodies, pos, quat):
+        b = np.ascontiguousarray(np.atleast_1d(bodies), np.uint32)
+        p = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(-1, 3))
+        q = np.ascontiguousarray(np.asarray(quat, np.float32).reshape(-1, 4))
+        assert p.shape[0] == b.shape[0] == q.shape[0]
+        self._check(self._lib.kdop_refit(self._h, b.shape[0], ptr(b), ptr(p), ptr(q)))
+
+    def refit_device(self, bodies, pos_ptr, quat_ptr):
+        b = np.ascontiguousarray(np.atleast_1d(bodies), np.uint32)
+        self._check(self._lib.kdop_refit_device(self._h, b.shape[0], ptr(b), ctypes.c_void_p(pos_ptr),
+                                                ctypes.c_void_p(quat_ptr)))
+
+    # -- kdop::tandem_traversal
+    def tandem_traversal(self, pairs, narrow=True, keep_all_pairs=False):
+        P = np.ascontiguousarray(np.asarray(pairs, np.uint32).reshape(-1, 2))
+        flags = (KDOP_NARROW if narrow else 0) | (KDOP_KEEP_ALL_PAIRS if keep_all_pairs else 0)
+        c = Counters()
+        self._check(self._lib.kdop_traverse(self._h, P.shape[0], ptr(P), flags, ctypes.byref(c)))
+        self._last_npairs = P.shape[0]
+        return c.as_dict()
+
+    def pairs(self, which=1):
+        """Leaf pairs of the last traversal as an (n, 3) uint32 array (pair, tet_a, tet_b)."""
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_get_pairs(self._h, which, None, 0, 0, ctypes.byref(n)))
+        out = np.empty((n.value, 3), np.uint32)
+        if n.value:
+            self._check(self._lib.kdop_get_pairs(self._h, which, ptr(out), n.value, 0, ctypes.byref(n)))
+        return out
+
+    def contacts(self):
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_get_contacts(self._h, None, 0, 0, ctypes.byref(n)))
+        out = np.empty(n.value, CONTACT_DTYPE)
+        if n.value:
+            self._check(self._lib.kdop_get_contacts(self._h, ptr(out), n.value, 0, ctypes.byref(n)))
+        return out
+
+    def world(self, body):
+        nv = self.meshes[body].nv
+        X = np.empty(nv, np.float32)
+        Y = np.empty(nv, np.float32)
+        Z = np.empty(nv, np.float32)
+        self._check(self._lib.kdop_get_world(self._h, body, ptr(X), ptr(Y), ptr(Z)))
+        return X, Y, Z
+
+    def export_tree(self, body):
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_export_tree(self._h, body, 0, None, None, None, None, ctypes.byref(n)))
+        vol = np.empty((n.value, self.K), np.float32)
+        rng = np.empty((n.value, 2), np.uint32)
+        kind = np.empty(n.value, np.uint8)
+        root = np.empty(self.K, np.float32)
+        self._check(self._lib.kdop_export_tree(self._h, body, n.value, ptr(vol), ptr(rng), ptr(kind), ptr(root),
+                                               ctypes.byref(n)))
+        return dict(vol=vol, range=rng, kind=kind, root=root)
+
+    def last_timing(self):
+        out = np.zeros(4, np.float64)
+        self._check(self._lib.kdop_last_timing(self._h, ptr(out)))
+        return dict(traversal_ms=out[0], narrow_ms=out[1], total_ms=out[2], per_launch_ms=out[3])
+
+
+def _quat_rotate_np(q, P):
+    """Rotate points P (n x 3) by quaternion q = (x, y, z, w) in float64 (for
+    workload set-up only: pose sampling, AABB estimates)."""
+    u = np.asarray(q[:3], np.float64)
+    s = float(q[3])
+    P = np.asarray(P, np.float64)
+    uv = np.cross(u, P)
+    return (s * s - u @ u) * P + 2.0 * np.outer(P @ u, u) + 2.0 * s * uv

@@ -1,0 +1,168 @@
+/*
+ * kdop_amd.h -- C ABI of the MI355X kDOP BVH collision library.
+ *
+ * Drop-in boundary for the KDOP subsystem of diku-dk/bvh-tvcg18 (the PROX
+ * simulator).  The reference exposes a tag-dispatched C++ template API
+ * (kdop::make_tree / refit_tree / tandem_traversal, narrow::update_kdop_bvh);
+ * each entry point below names the reference call it replaces.  All entry
+ * points are synchronous: they return once the device work they start has
+ * finished, and return 0 (KDOP_OK) or a negative KDOP_ERR_* code; the message
+ * of the last failure is kdop_last_error(ctx).  One context per host thread.
+ *
+ * Pointers are host pointers unless a parameter says "device".
+ */
+#ifndef KDOP_AMD_H
+#define KDOP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KDOP_OK 0
+#define KDOP_ERR_ARG (-1)      /* invalid argument (bad K, empty mesh, unknown body id, ...) */
+#define KDOP_ERR_HIP (-2)      /* HIP runtime error (no device, launch failure, ...)        */
+#define KDOP_ERR_NOMEM (-3)    /* device allocation failed                                  */
+#define KDOP_ERR_STATE (-4)    /* call out of order (e.g. kdop_narrow before a traversal)   */
+#define KDOP_ERR_CAPACITY (-5) /* caller buffer too small; the required size is reported    */
+
+typedef struct kdop_ctx kdop_ctx;
+
+/* Traversal strategies (kdop_config.traversal). */
+#define KDOP_TRAVERSAL_AUTO 0        /* autotuned choice                                   */
+#define KDOP_TRAVERSAL_LEVELS 1      /* level-synchronous frontier kernels                 */
+#define KDOP_TRAVERSAL_PERSISTENT 2  /* one persistent launch, wave stacks + work stealing */
+
+typedef struct kdop_config {
+  int32_t K;             /* slab count of the kDOP: 6, 8, 14, 18 or 26 (reference template K)  */
+  int32_t device;        /* HIP device ordinal                                                  */
+  uint64_t chunk_bytes;  /* reference make_tree mem_bytes (narrow_params.h:54 default 8000)     */
+  int32_t traversal;     /* KDOP_TRAVERSAL_*                                                    */
+  int32_t queue_depth;   /* persistent kernel: LDS stack entries per wave (0 = autotune)       */
+  int32_t waves_per_cu;  /* persistent kernel: resident waves per CU (0 = autotune)            */
+  int32_t reserved[5];
+} kdop_config;
+
+/* Counters of one traversal (+ narrow phase) call. */
+typedef struct kdop_counters {
+  uint64_t tests;            /* node-pair DOP tests the device executed (incl. the root test)   */
+  uint64_t reference_tests;  /* tests the reference CPU traversal makes for the same input:      */
+                             /*   1 + C_A*C_B + descent_tests when the roots overlap, else 1     */
+  uint64_t descent_tests;    /* tests below branch-root pairs (identical on both sides)          */
+  uint64_t branch_overlaps;  /* overlapping branch-root pairs                                    */
+  uint64_t leaf_pairs;       /* overlapping leaf (tet) pairs, P1                                  */
+  uint64_t surface_pairs;    /* P1 pairs whose tets both have a surface face, P2                  */
+  uint64_t contacts;         /* contact points reported by the narrow phase                      */
+  uint64_t narrow_pairs;     /* P2 pairs for which the narrow phase generated contacts           */
+  uint64_t branch_pairs;     /* C_A*C_B summed over body pairs whose roots overlap               */
+  uint64_t overflow;         /* nonzero if an internal buffer had to grow (the call was re-run)  */
+} kdop_counters;
+
+/* One contact point (geometry::ContactsCallback arguments, contacts/geometry_contacts_callback.h:7-24). */
+typedef struct kdop_contact {
+  uint32_t pair;      /* index into the body-pair list of the traversal call                 */
+  uint32_t tet_a;     /* tetrahedron of body A                                               */
+  uint32_t tet_b;     /* tetrahedron of body B                                               */
+  uint32_t ordinal;   /* position among this tet pair's reported contacts (reference order)  */
+  float p[3];         /* contact point, world coordinates                                    */
+  float n[3];         /* unit contact normal (a face normal of tet_a)                        */
+  float depth;        /* penetration measure, negative when overlapping                      */
+} kdop_contact;
+
+/* Library defaults: K=8, chunk_bytes=8000, device 0, autotuned traversal. */
+void kdop_default_config(kdop_config* cfg);
+
+/* Create / destroy a context (owns a HIP stream on cfg->device and all device
+ * buffers).  Replaces the DIKUCL singletons (FOUNDATION/DIKUCL dikucl_*_manager.hpp). */
+int kdop_create(kdop_ctx** ctx, const kdop_config* cfg);
+void kdop_destroy(kdop_ctx* ctx);
+const char* kdop_last_error(const kdop_ctx* ctx);
+
+/* Upload a tetrahedral mesh (material coordinates X0/Y0/Z0, tets as 4 vertex
+ * indices i,j,k,m, surface bits: bit v set when the face opposite vertex v is on
+ * the surface) and build its chunked kDOP tree on the device, refitted to the
+ * material coordinates.
+ * Replaces kdop::make_tree<V,K,T>(mem_bytes, mesh, X, Y, Z, sequential)
+ * (SIMULATION/KDOP/KDOP/include/kdop_make_tree.h:190-288) as called from
+ * narrow::make_kdop_bvh (SIMULATION/NARROW/NARROW/include/narrow_object.h:96-134). */
+int kdop_add_body(kdop_ctx* ctx, uint32_t nv, const float* X0, const float* Y0, const float* Z0,
+                  uint32_t nt, const uint32_t* tets, const uint8_t* surface, uint32_t* body_id);
+
+/* Tree shape of a body: out[0..4] = {L leaves per chunk, C branch chunks, H levels,
+ * device node count, depth (longest root-to-leaf path)}. */
+int kdop_body_shape(kdop_ctx* ctx, uint32_t body, uint64_t* out);
+
+/* Rigid transform + refit of n bodies: r = rotate(q, r0) + p per vertex, then
+ * the leaf and union passes.  pos: 3n floats, quat: 4n floats (x, y, z, w).
+ * Replaces narrow::update_kdop_bvh(work_pool, sequential)
+ * (SIMULATION/NARROW/NARROW/include/narrow_update_kdop_bvh.h:126-176), whose
+ * per-body refit is kdop::refit_tree (SIMULATION/KDOP/KDOP/include/kdop_refit_tree.h:119-162). */
+int kdop_refit(kdop_ctx* ctx, uint32_t n, const uint32_t* bodies, const float* pos, const float* quat);
+
+/* Same with the poses already in device memory (pos, quat: device pointers). */
+int kdop_refit_device(kdop_ctx* ctx, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev);
+
+/* Flags of kdop_traverse. */
+#define KDOP_NARROW 1u         /* also run the tet-tet narrow phase on the surface pairs    */
+#define KDOP_KEEP_ALL_PAIRS 2u /* keep the full leaf-pair list P1 (else only P2 is kept)    */
+
+/* Tandem traversal of npairs body pairs (body_pairs: 2*npairs ids, A then B)
+ * in one device pass; with KDOP_NARROW the MOST_OPPOSING_SURFACES contact
+ * generator runs on every surface pair.  Results stay in device memory until
+ * the next traversal (kdop_get_pairs / kdop_get_contacts).
+ * Replaces kdop::tandem_traversal(std::vector<TestPair>&, sequential|dikucl)
+ * (SIMULATION/KDOP/KDOP/include/kdop_tandem_traversal.h:198-226,
+ *  SIMULATION/KDOP/KDOP/include/cl/kdop_cl_tandem_traversal.h:622-626) as called from
+ * narrow::details::dispatch_tetramesh_tetramesh (narrow_tetramesh_tetramesh.h:20-93). */
+int kdop_traverse(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, kdop_counters* out);
+
+/* Copy the leaf pairs of the last traversal: which = 0 for P1 (needs
+ * KDOP_KEEP_ALL_PAIRS), 1 for P2.  dst receives 3 uint32 per pair (pair index,
+ * tet_a, tet_b) in device order (unsorted).  *n receives the pair count; returns
+ * KDOP_ERR_CAPACITY (nothing copied) when cap < *n.  dst_is_device: dst is a
+ * device pointer. */
+int kdop_get_pairs(kdop_ctx* ctx, int which, uint32_t* dst, uint64_t cap, int dst_is_device, uint64_t* n);
+
+/* Copy the contacts of the last traversal (unsorted; sort by (pair, tet_a, tet_b,
+ * ordinal) for the reference's per-pair report order). */
+int kdop_get_contacts(kdop_ctx* ctx, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n);
+
+/* World coordinates of a body after the last refit (3 arrays of nv floats). */
+int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);
+
+/* Export a body's device tree: node count in *n; when the buffers are non-null
+ * (cap >= *n), per node: vol (K floats lo0,hi0,lo1,...), range (2 uint32: first
+ * and last tet covered), kind (1 byte: 0 super-internal, 1 branch-internal,
+ * 2 tet leaf), and the root volume (K floats).  For parity tests. */
+int kdop_export_tree(kdop_ctx* ctx, uint32_t body, uint64_t cap, float* vol, uint32_t* range, uint8_t* kind,
+                     float* root, uint64_t* n);
+
+/* Adjoint of the penetration measure Phi = sum over reported contacts of depth,
+ * with the contact set and every discrete choice (normal face, contact-point
+ * provenance, argmin/argmax) frozen from the last KDOP_NARROW traversal.
+ * poses: per body pair used (one entry per body in bodies[]), with
+ * dphi: scalar seed dL/dPhi per body pair (device, npairs floats, may be NULL =
+ * all ones).  Writes grad_pos (3 per body) and grad_quat (4 per body, x,y,z,w)
+ * for the bodies listed in bodies[] (device pointers, overwritten), and
+ * phi (device, npairs floats, may be NULL).  The quaternion gradient is that of
+ * the literal r = Im(q r0 q*) + p (q not renormalised).
+ * No counterpart in the reference (it has no differentiable narrow phase). */
+int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies, const float* dphi_dev,
+                        float* phi_dev, float* grad_pos_dev, float* grad_quat_dev);
+
+/* Device wall time (milliseconds) of the phases of the last traversal call,
+ * measured with HIP events on the context stream: out[0] traversal kernels,
+ * out[1] narrow phase, out[2] whole call.  out[3] = average duration of one
+ * launch of the dominant traversal kernel. */
+int kdop_last_timing(kdop_ctx* ctx, double* out);
+
+/* Stream of the context (hipStream_t), for callers that order their own work. */
+void* kdop_stream(kdop_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KDOP_AMD_H */

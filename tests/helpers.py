@@ -1,0 +1,68 @@
+"""Shared helpers of the test suite: run one pair workload through the oracle
+and through the HIP library and return both results side by side."""
+import numpy as np
+
+import kdop_amd as K
+from oracle import oracle as O
+
+
+def world_coords(mesh, pose):
+    """World vertex coordinates by the oracle's transform (reference op order)."""
+    return O.transform_f32(mesh.X, mesh.Y, mesh.Z, pose.p, pose.q)
+
+
+def oracle_run(wl, narrow=True, threads=1):
+    """Oracle traversal of every body pair of a workload.  Returns a list of
+    (counters, p1, p2, contacts) per pair and the oracle trees per mesh."""
+    trees, worlds = [], []
+    for m, pose in zip(wl.meshes, wl.poses):
+        X, Y, Z = world_coords(m, pose)
+        t = O.Tree(wl.K, 8000, m.tets, m.X, m.Y, m.Z)  # make_tree on material coordinates
+        t.refit(X, Y, Z)                                 # update_kdop_bvh
+        trees.append(t)
+        worlds.append((X, Y, Z))
+    out = []
+    for a, b in wl.pairs:
+        ma, mb = wl.meshes[a], wl.meshes[b]
+        res = O.traverse(trees[a], (ma.tets, *worlds[a], ma.surface), trees[b], (mb.tets, *worlds[b], mb.surface),
+                         narrow=narrow, threads=threads)
+        out.append(res[:4])
+    return out, trees, worlds
+
+
+def device_run(wl, narrow=True, keep_all=True, ctx=None, **ctx_kw):
+    """Build, refit and traverse a workload on the device."""
+    own = ctx is None
+    if own:
+        ctx = K.Context(K=wl.K, **ctx_kw)
+    ids = [ctx.make_tree(m) for m in wl.meshes]
+    ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
+    pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
+    cnt = ctx.tandem_traversal(pairs, narrow=narrow, keep_all_pairs=keep_all)
+    res = dict(counters=cnt, p2=ctx.pairs(1), ids=ids, ctx=ctx)
+    if keep_all:
+        res["p1"] = ctx.pairs(0)
+    if narrow:
+        res["contacts"] = ctx.contacts()
+    return res
+
+
+def sort_pairs(p):
+    """Sort an (n, k) uint32 pair array lexicographically (last column fastest)."""
+    if p.shape[0] == 0:
+        return p
+    keys = tuple(p[:, i] for i in range(p.shape[1] - 1, -1, -1))
+    return p[np.lexsort(keys)]
+
+
+def oracle_contacts_with_ordinals(con):
+    """Attach per-(ta, tb) ordinals to the oracle's contacts (report order)."""
+    ords = np.zeros(con.shape[0], np.uint32)
+    last = None
+    k = 0
+    for i, c in enumerate(con):
+        key = (int(c["ta"]), int(c["tb"]))
+        k = k + 1 if key == last else 0
+        last = key
+        ords[i] = k
+    return ords

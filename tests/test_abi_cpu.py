@@ -1,0 +1,53 @@
+"""CPU-side checks of the C ABI library: it loads, and exports every entry
+point the public headers declare (no device calls)."""
+import ctypes
+import os
+import re
+
+import kdop_amd as K
+from kdop_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(kdop_[A-Za-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = K.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    bound = {s[0] for s in _abi.SIGNATURES}
+    assert set(names) == bound, (set(names) ^ bound)
+
+
+def test_default_config_and_error_on_missing_device():
+    L = K.lib()
+    cfg = _abi.Config()
+    L.kdop_default_config(ctypes.byref(cfg))
+    assert cfg.K == 8 and cfg.chunk_bytes == 8000 and cfg.device == 0
+
+
+def test_tree_shape_matches_quoted_workload_shapes():
+    # chunks C and levels H quoted for the reference formula (kdop_make_tree.h:216-229)
+    assert K.tree_shape(8, 8000, 5000) == dict(node_bytes=56, n_perfect=63, L=32, C=157, H=2)
+    s = K.tree_shape(26, 8000, 320000)
+    assert (s["L"], s["C"], s["H"]) == (8, 40000, 6)
+    s = K.tree_shape(6, 8000, 320000)
+    assert (s["L"], s["C"], s["H"]) == (32, 10000, 3)
+    for k in (14, 18):
+        s = K.tree_shape(k, 8000, 320000)
+        assert (s["L"], s["C"], s["H"]) == (16, 20000, 4)
+    s = K.tree_shape(14, 8000, 50000)
+    assert (s["L"], s["C"], s["H"]) == (16, 3125, 3)

@@ -1,0 +1,152 @@
+"""Parity of the HIP pipeline (device build + refit + traversal + narrow phase)
+against the CPU oracle, through the C ABI.
+
+Bar: bit-exact for the tree volumes, world coordinates, leaf-pair sets and
+test counts; contact points / normals / depths within 1e-5 relative (they
+are expected bit-exact as well; the number of inexact values is asserted 0
+for the default workloads).
+"""
+import numpy as np
+import pytest
+
+import kdop_amd as K
+from kdop_amd import workloads as W
+from helpers import (device_run, oracle_contacts_with_ordinals, oracle_run, sort_pairs)
+
+pytestmark = pytest.mark.gpu
+
+CONTACT_RTOL = 1e-5
+
+
+def _check_pair_results(wl, dev, orc, exact_contacts=True):
+    cnt = dev["counters"]
+    tot = {k: 0 for k in ("tests", "descent_tests", "branch_overlaps", "leaf_pairs", "surface_pairs", "contacts",
+                          "narrow_pairs")}
+    p1_o, p2_o, con_o = [], [], []
+    for pi, (c, p1, p2, con) in enumerate(orc):
+        for k in tot:
+            tot[k] += c[k]
+        p1_o.append(np.column_stack([np.full(p1.shape[0], pi, np.uint32), p1]))
+        p2_o.append(np.column_stack([np.full(p2.shape[0], pi, np.uint32), p2]))
+        if con is not None and con.shape[0]:
+            con_o.append((pi, con))
+    # node-pair test count of the reference traversal (kdop_tandem_traversal.h)
+    assert cnt["reference_tests"] == tot["tests"], (cnt, tot)
+    assert cnt["descent_tests"] == tot["descent_tests"]
+    assert cnt["branch_overlaps"] == tot["branch_overlaps"]
+    assert cnt["leaf_pairs"] == tot["leaf_pairs"]
+    assert cnt["surface_pairs"] == tot["surface_pairs"]
+    # pair sets, bit exact after sorting
+    p1_o = sort_pairs(np.concatenate(p1_o)) if p1_o else np.zeros((0, 3), np.uint32)
+    p2_o = sort_pairs(np.concatenate(p2_o)) if p2_o else np.zeros((0, 3), np.uint32)
+    if "p1" in dev:
+        np.testing.assert_array_equal(sort_pairs(dev["p1"]), p1_o)
+    np.testing.assert_array_equal(sort_pairs(dev["p2"]), p2_o)
+    if "contacts" not in dev:
+        return
+    assert cnt["contacts"] == tot["contacts"]
+    assert cnt["narrow_pairs"] == tot["narrow_pairs"]
+    dc = dev["contacts"]
+    rows = []
+    for pi, con in con_o:
+        ords = oracle_contacts_with_ordinals(con)
+        for c, o in zip(con, ords):
+            rows.append((pi, int(c["ta"]), int(c["tb"]), int(o), *c["p"], *c["n"], float(c["d"])))
+    if not rows:
+        assert dc.shape[0] == 0
+        return
+    ref = np.array(rows, dtype=np.float64)
+    ref = ref[np.lexsort((ref[:, 3], ref[:, 2], ref[:, 1], ref[:, 0]))]
+    key = np.lexsort((dc["ordinal"], dc["tet_b"], dc["tet_a"], dc["pair"]))
+    dc = dc[key]
+    np.testing.assert_array_equal(dc["pair"], ref[:, 0].astype(np.uint32))
+    np.testing.assert_array_equal(dc["tet_a"], ref[:, 1].astype(np.uint32))
+    np.testing.assert_array_equal(dc["tet_b"], ref[:, 2].astype(np.uint32))
+    np.testing.assert_array_equal(dc["ordinal"], ref[:, 3].astype(np.uint32))
+    got = np.concatenate([dc["p"], dc["n"], dc["depth"][:, None]], axis=1).astype(np.float64)
+    want = ref[:, 4:]
+    scale = np.maximum(np.abs(want), 1.0)
+    assert np.all(np.abs(got - want) <= CONTACT_RTOL * scale)
+    if exact_contacts:
+        assert np.count_nonzero(got.astype(np.float32) != want.astype(np.float32)) == 0
+
+
+def test_single_pair_box10_k8():
+    """Two copies of the 10^3-cube box (5000 tets), K=8: full parity."""
+    wl = W.single_pair(n=10, K=8)
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+    assert dev["counters"]["leaf_pairs"] > 100000
+
+
+@pytest.mark.parametrize("k", [6, 8, 12, 14, 18, 20, 26])
+def test_jittered_pair_all_k(k):
+    """Jittered 12^3-cube boxes (8640 tets), random pose, every supported K."""
+    wl = W.large_pair(n=12, K=k, seed=3)
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+
+
+def test_refit_bits_and_world_coordinates():
+    """Device world coordinates and every tree volume equal the oracle's bits."""
+    from helpers import world_coords
+    wl = W.large_pair(n=8, K=26, seed=5)
+    orc, trees, worlds = oracle_run(wl, narrow=False)
+    dev = device_run(wl, narrow=False)
+    ctx = dev["ctx"]
+    for i, bid in enumerate(dev["ids"]):
+        X, Y, Z = ctx.world(bid)
+        np.testing.assert_array_equal(X, worlds[i][0])
+        np.testing.assert_array_equal(Y, worlds[i][1])
+        np.testing.assert_array_equal(Z, worlds[i][2])
+        t = ctx.export_tree(bid)
+        np.testing.assert_array_equal(t["root"], trees[i].root())
+        ex = trees[i].export()
+        ref = {}
+        for (lv, ch, sl, leaf, st, en), (f, l), v in zip(ex["info"], ex["range"], ex["vol"]):
+            ref[(int(f), int(l))] = v
+        for (f, l), v in zip(t["range"], t["vol"]):
+            np.testing.assert_array_equal(v, ref[(int(f), int(l))])
+
+
+def test_batched_pairs_one_launch():
+    """Several body pairs of different sizes (some separated) in one call."""
+    wl = W.batched_pairs(npairs=12, K=14, seed=4, tmin=200, tmax=6000)
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+
+
+def test_separated_bodies():
+    """Bodies far apart: one root test per pair, nothing else."""
+    wl = W.single_pair(n=4, K=8)
+    wl.poses[1] = W.Pose(np.array([5.0, 0.0, 0.0], np.float32), wl.poses[1].q)
+    dev = device_run(wl)
+    c = dev["counters"]
+    assert c["tests"] == 1 and c["reference_tests"] == 1
+    assert c["leaf_pairs"] == 0 and dev["p2"].shape[0] == 0 and dev["contacts"].shape[0] == 0
+
+
+def test_refit_series_small():
+    """Three steps of rigid motion with refit (not rebuild): pair sets per step."""
+    wl = W.large_pair(n=8, K=18, seed=9)
+    ctx = K.Context(K=18)
+    ids = [ctx.make_tree(m) for m in wl.meshes]
+    P, Q = W.motion(2, 3, seed=31, dt=0.05, drift=0.01)
+    from oracle import oracle as O
+    from helpers import world_coords
+    trees = [O.Tree(18, 8000, m.tets, m.X, m.Y, m.Z) for m in wl.meshes]
+    for s in range(3):
+        poses = [W.Pose(P[s, i] + wl.poses[i].p, Q[s, i]) for i in range(2)]
+        ctx.refit(ids, np.stack([p.p for p in poses]), np.stack([p.q for p in poses]))
+        cnt = ctx.tandem_traversal([[ids[0], ids[1]]], narrow=True, keep_all_pairs=True)
+        ws = [world_coords(m, p) for m, p in zip(wl.meshes, poses)]
+        for t, w in zip(trees, ws):
+            t.refit(*w)
+        c, p1, p2, con, _ = O.traverse(trees[0], (wl.meshes[0].tets, *ws[0], wl.meshes[0].surface),
+                                       trees[1], (wl.meshes[1].tets, *ws[1], wl.meshes[1].surface))
+        assert cnt["reference_tests"] == c["tests"]
+        np.testing.assert_array_equal(sort_pairs(ctx.pairs(0)[:, 1:]), sort_pairs(p1))
+        assert cnt["contacts"] == c["contacts"]
