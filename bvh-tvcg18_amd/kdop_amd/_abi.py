@@ -97,11 +97,21 @@ _lib = None
 def lib():
     """Load libkdop_amd.so (built in-tree by bvh-tvcg18_amd/build.py).
 
+    PyTorch-ROCm wheels carry their own HIP runtime under the same soname
+    (libamdhip64.so.7).  torch is imported first when it is installed, so the
+    library binds to that already loaded runtime and one process never holds
+    two HIP runtimes.  Set KDOP_NO_TORCH=1 to skip the import.
+
     Raises RuntimeError when the library is missing: there is no fallback path.
     """
     global _lib
     if _lib is not None:
         return _lib
+    if os.environ.get("KDOP_NO_TORCH", "0") != "1":
+        try:
+            import torch  # noqa: F401  (binds the shared HIP runtime first)
+        except ImportError:
+            pass
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built: run `python bvh-tvcg18_amd/build.py`")
     L = ctypes.CDLL(LIB_PATH)

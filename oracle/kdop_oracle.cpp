@@ -754,6 +754,29 @@ int orc_traverse(void* hA, int ntA_unused, const uint32_t* tetsA, const float* X
   return 0;
 }
 
+// Timing sample for the CPU baseline: the branch rows [row0, row1) of A
+// against every branch of B, single threaded, narrow phase included, results
+// counted but not stored.  counters as orc_traverse (without the root test).
+int orc_traverse_rows(void* hA, const uint32_t* tetsA, const float* XA, const float* YA, const float* ZA, const uint8_t* surfA,
+                      void* hB, const uint32_t* tetsB, const float* XB, const float* YB, const float* ZB, const uint8_t* surfB,
+                      uint64_t row0, uint64_t row1, int do_narrow, uint64_t* counters, double* seconds) {
+  const Tree& A = *(Tree*)hA;
+  const Tree& B = *(Tree*)hB;
+  if (A.K != B.K) return -1;
+  const Mesh mA{tetsA, XA, YA, ZA, surfA}, mB{tetsB, XB, YB, ZB, surfB};
+  const size_t CA = A.levels.back().size();
+  if (row1 > CA) row1 = CA;
+  Sink s;
+  s.do_narrow = do_narrow != 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  tandem_rows(A, mA, B, mB, (size_t)row0, (size_t)row1, s);
+  const auto t1 = std::chrono::steady_clock::now();
+  counters[0] = s.tests; counters[1] = s.branch_tests; counters[2] = s.descent_tests; counters[3] = s.branch_overlaps;
+  counters[4] = s.p1; counters[5] = s.p2; counters[6] = s.ncontacts; counters[7] = s.narrow_hits; counters[8] = 0;
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
+}
+
 // Brute force: every (leaf of A, leaf of B) whose leaf DOPs overlap.  The leaf
 // DOP of tet t comes from the tree (so it is the refit value).  Independent of
 // the tree traversal; pins the traversal's pair set.

@@ -47,6 +47,7 @@ def lib():
         L.orc_make_dop_f32.argtypes = [ctypes.c_int, _p, _p]
         L.orc_traverse.argtypes = [_p, ctypes.c_int, _p, _p, _p, _p, _p, _p, ctypes.c_int, _p, _p, _p, _p, _p,
                                    ctypes.c_int, ctypes.c_int, _p, _u64, _p, _u64, _p, _u64, _p, _p]
+        L.orc_traverse_rows.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int, _p, _p]
         L.orc_brute_pairs.restype = _u64
         L.orc_brute_pairs.argtypes = [_p, _p, _p, _u64]
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
@@ -171,6 +172,19 @@ def traverse(tree_a, mesh_a, tree_b, mesh_b, narrow=True, threads=1, want_p1=Tru
     n1, n2, nc = counters["leaf_pairs"], counters["surface_pairs"], counters["contacts"]
     return (counters, p1[:n1] if p1 is not None else None, p2[:n2] if p2 is not None else None,
             con[:nc] if con is not None else None, sec.value)
+
+
+def traverse_rows(tree_a, mesh_a, tree_b, mesh_b, row0, row1, narrow=True):
+    """Single-threaded timing sample: branch rows [row0, row1) of A against all of B.
+    Returns (counters dict, seconds)."""
+    ta, Xa, Ya, Za, Sa = (np.ascontiguousarray(a) for a in mesh_a)
+    tb, Xb, Yb, Zb, Sb = (np.ascontiguousarray(a) for a in mesh_b)
+    cnt = np.zeros(9, np.uint64)
+    sec = ctypes.c_double()
+    lib().orc_traverse_rows(tree_a.h, _ptr(ta), _ptr(Xa), _ptr(Ya), _ptr(Za), _ptr(Sa),
+                            tree_b.h, _ptr(tb), _ptr(Xb), _ptr(Yb), _ptr(Zb), _ptr(Sb),
+                            int(row0), int(row1), int(narrow), _ptr(cnt), ctypes.byref(sec))
+    return dict(zip(COUNTER_NAMES, (int(v) for v in cnt))), sec.value
 
 
 def brute_pairs(tree_a, tree_b, cap):
