@@ -5,7 +5,8 @@
 //   uint32 m0      internal: left child (global index)  | flags
 //                  tet leaf: tet index (body local)      | flags
 //   uint32 m1      internal: right child (global index) | F_ROOT when branch root
-//                  tet leaf: F_ROOT when the leaf is a one-tet chunk's root
+//                  tet leaf: body index | F_SURF when the tet has a surface face
+//                            | F_ROOT when the leaf is a one-tet chunk's root
 // Flags of m0: F_LEAF (tet leaf), F_BRANCH (node lies in a branch chunk).
 // Super-chunk leaves of the reference tree (copies of a child chunk's root
 // volume) are not stored: a super node points straight at the child chunk's
@@ -20,6 +21,7 @@ namespace kdop_amd {
 constexpr uint32_t F_LEAF = 0x80000000u;
 constexpr uint32_t F_BRANCH = 0x40000000u;
 constexpr uint32_t F_ROOT = 0x80000000u;
+constexpr uint32_t F_SURF = 0x40000000u;
 constexpr uint32_t IDX_MASK = 0x3FFFFFFFu;
 constexpr int kMaxLevels = 24;
 
@@ -47,7 +49,7 @@ struct BodyDev {
 
 // Topology parameters of one body (kernel argument).
 struct TopoParams {
-  uint32_t node_base, H, L, nt;
+  uint32_t node_base, H, L, nt, body;
   uint32_t C[kMaxLevels];          // chunks per level (0 = top, H-1 = branches)
   uint32_t E[kMaxLevels];          // elements per level (tets at H-1, chunks of level h+1 above)
   uint32_t level_base[kMaxLevels]; // body-local node index of the level's first chunk
@@ -74,6 +76,17 @@ struct NarrowRec {
   uint8_t face, nuniq, cmin, cmax;
   float depth;
   float n[3];
+};
+
+// Shared state of the persistent traversal's global work queue: a ring of
+// 64-bit items (a << 32 | b, node indices), ~0 marks a free slot.
+struct QueueState {
+  unsigned long long head;  // next slot to hand out (monotone)
+  unsigned long long tail;  // next slot to reserve for a push (monotone)
+  int pending;              // items pushed but not yet expanded
+  unsigned int error;       // nonzero: a bounded wait ran out (the call is re-run level by level)
+  unsigned long long steals; // batches taken from the global queue
+  unsigned long long donations; // batches pushed to the global queue
 };
 
 // Device counters of one traversal call.

@@ -13,6 +13,7 @@
 
 #include "kdop_amd.h"
 #include "kdop_kernels.h"
+#include "kdop_persistent.h"
 #include "kdop_meshgen.h"
 #include "tree_shape.h"
 
@@ -66,6 +67,16 @@ struct kdop_ctx {
   // refit scratch
   DevBuf<uint32_t> plan;
   DevBuf<float> poses;
+  // persistent traversal: global work ring, its state, the pair-key table
+  DevBuf<unsigned long long> ring;
+  QueueState* qstate = nullptr;
+  QueueState* hqstate = nullptr;  // pinned
+  DevBuf<unsigned long long> pkeys;
+  DevBuf<uint32_t> pindex;
+  int num_cus = 0;
+  int ls = 1024, wpc = 8, donate_min = 128, ring_low = 0;
+  uint64_t persistent_fallbacks = 0;
+  bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double timing[4] = {0, 0, 0, 0};
   uint32_t last_levels = 0;
@@ -306,6 +317,72 @@ struct Fills {
   uint32_t p1, p2, c, r;
 };
 
+template <int K>
+struct SeedRing {
+  static void run(kdop_ctx* c, const TravArgs& T, const PersistArgs& P, uint32_t npairs) {
+    hipLaunchKernelGGL(k_seed_ring<K>, dim3(npairs), dim3(64), 0, c->stream, T, P);
+  }
+};
+template <int K>
+struct Persist {
+  static void run(kdop_ctx* c, const PersistArgs& P) {
+    const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
+    switch (c->ls) {
+      case 256: hipLaunchKernelGGL((k_traverse_persistent<K, 256>), grid, block, 0, c->stream, P); break;
+      case 512: hipLaunchKernelGGL((k_traverse_persistent<K, 512>), grid, block, 0, c->stream, P); break;
+      case 2048: hipLaunchKernelGGL((k_traverse_persistent<K, 2048>), grid, block, 0, c->stream, P); break;
+      default: hipLaunchKernelGGL((k_traverse_persistent<K, 1024>), grid, block, 0, c->stream, P); break;
+    }
+  }
+};
+
+struct Fills;
+
+// Ring capacity, pair-key table and queue state for one persistent call.
+int prepare_persistent(kdop_ctx* c, const std::vector<std::pair<unsigned long long, uint32_t>>& keys, uint64_t seed_bound) {
+  uint64_t q = 1ull << 22;
+  while (q < 2 * seed_bound + (uint64_t)c->num_cus * c->wpc * 512) q <<= 1;
+  if (c->ring.cap < q) {
+    release(c->ring);
+    int rc = ensure(c, c->ring, q);
+    if (rc) return rc;
+    HIPCHK(c, hipMemsetAsync(c->ring.p, 0xFF, q * sizeof(unsigned long long), c->stream));
+  }
+  const size_t n = keys.size();
+  int rc = ensure(c, c->pkeys, std::max<size_t>(n, 1));
+  if (rc) return rc;
+  if ((rc = ensure(c, c->pindex, std::max<size_t>(n, 1)))) return rc;
+  std::vector<unsigned long long> k(n);
+  std::vector<uint32_t> idx(n);
+  for (size_t i = 0; i < n; ++i) { k[i] = keys[i].first; idx[i] = keys[i].second; }
+  if (n) {
+    HIPCHK(c, hipMemcpyAsync(c->pkeys.p, k.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->pindex.p, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  }
+  return KDOP_OK;
+}
+
+PersistArgs persist_args(kdop_ctx* c, const TravArgs& T, uint32_t npairs) {
+  PersistArgs P;
+  P.nodes = c->nodes.p;
+  P.cnt = c->dcnt;
+  P.qs = c->qstate;
+  P.ring = c->ring.p;
+  P.qmask = c->ring.cap - 1;
+  P.p1 = T.p1; P.p2 = T.p2; P.p1_fill = T.p1_fill; P.p2_fill = T.p2_fill; P.p1_cap = T.p1_cap; P.p2_cap = T.p2_cap;
+  P.pair_keys = c->pkeys.p;
+  P.pair_index = c->pindex.p;
+  P.npairs = npairs;
+  P.donate_min = (uint32_t)std::max(64, c->donate_min);
+  P.ring_low = (uint32_t)(c->ring_low > 0 ? c->ring_low : c->num_cus * c->wpc * 2);
+  P.spin_limit = 1u << 22;
+  return P;
+}
+
+void launch_persistent(kdop_ctx* c, const PersistArgs& P) { dispatch_K<Persist>(c->K, c, P); }
+
+
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -340,6 +417,15 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
     if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   if (hipMalloc(&c->dcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
   if (hipHostMalloc(&c->hcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipMalloc(&c->qstate, sizeof(QueueState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipHostMalloc(&c->hqstate, sizeof(QueueState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
+    c->num_cus = prop.multiProcessorCount;
+  }
+  if (cfg->queue_depth == 256 || cfg->queue_depth == 512 || cfg->queue_depth == 1024 || cfg->queue_depth == 2048) c->ls = cfg->queue_depth;
+  if (cfg->waves_per_cu > 0) c->wpc = std::min(cfg->waves_per_cu, 32);
   *out = c;
   return KDOP_OK;
 }
@@ -356,6 +442,9 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
+  release(c->ring); release(c->pkeys); release(c->pindex);
+  if (c->qstate) (void)hipFree(c->qstate);
+  if (c->hqstate) (void)hipHostFree(c->hqstate);
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -443,7 +532,7 @@ int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, co
   // device construction: topology, then refit to the material coordinates
   TopoParams P;
   std::memset(&P, 0, sizeof(P));
-  P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt;
+  P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt; P.body = id;
   uint32_t total_chunks = 0;
   for (uint32_t h = 0; h < b.H; ++h) { P.C[h] = b.C[h]; P.E[h] = b.E[h]; P.level_base[h] = b.level_base[h]; total_chunks += b.C[h]; }
   // root record meta = 0
@@ -491,15 +580,21 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (npairs && !body_pairs) return fail(c, KDOP_ERR_ARG, "kdop_traverse: null body_pairs");
   HIPCHK(c, hipSetDevice(c->cfg.device));
   uint32_t levels = 0;
-  uint64_t est = 0;
+  uint64_t est = 0, seed_bound = 0;
   std::vector<uint2> pairs(npairs);
+  std::vector<std::pair<unsigned long long, uint32_t>> keys(npairs);
   for (uint32_t i = 0; i < npairs; ++i) {
     const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
     if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
     pairs[i] = make_uint2(a, b);
+    keys[i] = {((unsigned long long)a << 32) | b, i};
     levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
     est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
+    seed_bound += (uint64_t)c->bodies[a].ntop * c->bodies[b].ntop;
   }
+  std::sort(keys.begin(), keys.end());
+  bool unique_pairs = true;
+  for (uint32_t i = 1; i < npairs; ++i) if (keys[i].first == keys[i - 1].first) unique_pairs = false;
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
   c->have_traversal = c->have_p1 = c->have_narrow = false;
   int rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1));
@@ -507,17 +602,24 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
+  // strategy: the persistent kernel recovers the body pair of a leaf pair from
+  // the leaf records, so it needs distinct (A, B) pairs in one call
+  bool persistent = (c->cfg.traversal != KDOP_TRAVERSAL_LEVELS) && unique_pairs;
   est = std::max<uint64_t>(est, 1 << 16);
-  if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
   if (narrow) {
     if ((rc = ensure(c, c->contacts, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
     if ((rc = ensure(c, c->recs, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
   }
+  if (persistent) {
+    if ((rc = prepare_persistent(c, keys, seed_bound))) return rc;
+  } else {
+    if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
+  }
   Fills* dfill = reinterpret_cast<Fills*>(c->dcnt + 1);
   uint64_t overflowed = 0;
-  for (int attempt = 0; attempt < 8; ++attempt) {
+  for (int attempt = 0; attempt < 10; ++attempt) {
     HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(DevCounters) + sizeof(Fills), c->stream));
     TravArgs T;
     T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p; T.cnt = c->dcnt;
@@ -528,8 +630,17 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.npairs = npairs;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
-      dispatch_K<Seed>(c->K, c, T, npairs);
-      for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
+      if (persistent) {
+        HIPCHK(c, hipMemsetAsync(c->qstate, 0, sizeof(QueueState), c->stream));
+        PersistArgs P = persist_args(c, T, npairs);
+        dispatch_K<SeedRing>(c->K, c, T, P, npairs);
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        launch_persistent(c, P);
+      } else {
+        dispatch_K<Seed>(c->K, c, T, npairs);
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
+      }
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
@@ -544,11 +655,19 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
+    if (persistent) HIPCHK(c, hipMemcpyAsync(c->hqstate, c->qstate, sizeof(QueueState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const DevCounters& h = *c->hcnt;
     const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
     bool again = false;
-    if (h.overflow & 1u) {
+    if (persistent && c->hqstate->error) {
+      // a bounded wait ran out: scrub the ring and run this call level by level
+      HIPCHK(c, hipMemsetAsync(c->ring.p, 0xFF, c->ring.cap * sizeof(unsigned long long), c->stream));
+      persistent = false;
+      ++c->persistent_fallbacks;
+      if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
+      again = true;
+    } else if (!persistent && (h.overflow & 1u)) {
       uint32_t mx = 0;
       for (int l = 0; l < 64; ++l) mx = std::max(mx, h.level_count[l]);
       const size_t want = std::max<size_t>(4 * c->front[0].cap, (size_t)mx * 2);
@@ -561,13 +680,15 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       if (narrow && f.r > c->recs.cap) { if ((rc = ensure(c, c->recs, (size_t)f.r + f.r / 4))) return rc; again = true; }
     }
     if (again) { ++overflowed; continue; }
-    float ms[3] = {0, 0, 0};
+    float ms[4] = {0, 0, 0, 0};
     HIPCHK(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
     HIPCHK(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
     HIPCHK(c, hipEventElapsedTime(&ms[2], c->ev[0], c->ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&ms[3], c->ev[3], c->ev[1]));
     c->timing[0] = ms[0]; c->timing[1] = ms[1]; c->timing[2] = ms[2];
-    c->timing[3] = levels ? ms[0] / (double)(levels + 1) : 0.0;
-    c->last_levels = levels;
+    c->timing[3] = persistent ? ms[3] : (levels ? ms[3] / (double)levels : 0.0);
+    c->last_levels = persistent ? 0 : levels;
+    c->last_persistent = persistent;
     c->last_npairs = npairs;
     c->last_pairs = pairs;
     c->have_traversal = true;

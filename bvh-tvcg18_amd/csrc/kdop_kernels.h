@@ -140,7 +140,7 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int S,
     if (lo == hi) {  // branch leaf (super chunks never enqueue single ranges)
       const uint32_t tet = first + lo;
       meta[0] = F_LEAF | F_BRANCH | tet;
-      meta[1] = rootflag | ((surf[tet] & 15u) ? 0x40000000u : 0u);
+      meta[1] = rootflag | ((surf[tet] & 15u) ? F_SURF : 0u) | P.body;
       continue;
     }
     const uint32_t mid = (lo + hi) >> 1;  // floor((first+last)/2): first is even (c*L), see header
@@ -356,6 +356,7 @@ struct TravArgs {
 struct Expansion {
   uint32_t ca[2], cb[2];  // child node ids
   uint32_t ta[2], tb[2];  // tet ids when the child is a tet leaf
+  uint32_t body_a, body_b;  // bodies of the tet leaves (from the leaf records)
   uint32_t push;          // overlapping combinations to push as new items
   uint32_t leaf;          // overlapping tet-leaf pairs (P1)
   uint32_t surf;          // subset of leaf where both tets have a surface face (P2)
@@ -382,12 +383,15 @@ __device__ inline void expand_item(const float* __restrict__ nodes, uint32_t a, 
   x.cb[0] = exp_b ? (mb.x & IDX_MASK) : b;
   x.cb[1] = mb.y & IDX_MASK;
   NodeRec<K> A0, A1, B0, B1;
+  A1.m0 = A1.m1 = B1.m0 = B1.m1 = 0u;
   load_rec<K>(nodes, x.ca[0], A0);
   if (exp_a) load_rec<K>(nodes, x.ca[1], A1);
   load_rec<K>(nodes, x.cb[0], B0);
   if (exp_b) load_rec<K>(nodes, x.cb[1], B1);
   x.ta[0] = A0.m0 & IDX_MASK; x.ta[1] = A1.m0 & IDX_MASK;
   x.tb[0] = B0.m0 & IDX_MASK; x.tb[1] = B1.m0 & IDX_MASK;
+  x.body_a = ((A0.m0 & F_LEAF) ? A0.m1 : A1.m1) & IDX_MASK;
+  x.body_b = ((B0.m0 & F_LEAF) ? B0.m1 : B1.m1) & IDX_MASK;
   x.push = 0; x.leaf = 0; x.surf = 0;
   x.tests = 0; x.descent = 0; x.broot = 0;
 #pragma unroll
@@ -404,7 +408,7 @@ __device__ inline void expand_item(const float* __restrict__ nodes, uint32_t a, 
     if (both_roots) ++x.broot;
     if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
       x.leaf |= 1u << k;
-      if ((X.m1 & 0x40000000u) && (Y.m1 & 0x40000000u)) x.surf |= 1u << k;
+      if ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) x.surf |= 1u << k;
     } else {
       x.push |= 1u << k;
     }
@@ -473,7 +477,7 @@ __global__ __launch_bounds__(64) void k_seed(TravArgs T, Item* __restrict__ out,
         if (both_roots) ++broot;
         if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
           x.leaf = 1u;
-          x.surf = ((X.m1 & 0x40000000u) && (Y.m1 & 0x40000000u)) ? 1u : 0u;
+          x.surf = ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) ? 1u : 0u;
         } else {
           x.push = 1u;
         }

@@ -431,6 +431,9 @@ template <class T> inline int pick_most_opposing(const Tet<T>& A, const Tet<T>& 
 //   kind 0: vertex v of B inside A      kind 1: vertex v of A inside B
 //   kind 2: edge e of B crossing plane f of A   kind 3: edge e of A crossing plane f of B
 struct Cand { int kind, v, e, f; };
+// One-byte provenance code, the same encoding the HIP narrow phase records:
+// kind | (vertex or edge) << 2 | plane << 5.
+inline uint8_t cand_code(const Cand& c) { return (uint8_t)(c.kind | ((c.kind < 2 ? c.v : c.e) << 2) | (c.f < 0 ? 0 : c.f << 5)); }
 
 template <class T> struct PairContacts {
   int face_a = -1;               // A face whose unit normal is the contact normal
@@ -501,6 +504,93 @@ template <class T> inline bool contacts_most_opposing(const Tet<T>& A, const Tet
   out.face_a = fa;
   out.n = n;
   return generate_contacts(A, B, n, out);
+}
+
+// ------------------------------------------- frozen-choice evaluation (FP64)
+// The penetration measure of one tet pair with every discrete choice of the
+// forward pass frozen: the A face whose unit normal is n, and the provenance
+// of the candidates attaining min and max of p.n.  depth = (p_min - p_max).n
+// (contacts ...tetrahedron.h:543-681 restricted to those two points).  This is
+// what the adjoint differentiates; it is not part of the reference.
+// Edge-plane crossing with the signed parameter t = sA / (sA - sB): equal to
+// make_intersection's |sA| / (|sA| + |sB|) whenever the edge crosses the
+// plane (the frozen choice), and smooth where the unsigned form has a kink.
+template <class T> inline V3<T> crossing_frozen(const V3<T>& a, const V3<T>& b, const UPlane<T>& pl) {
+  const T sA = usd(a, pl), sB = usd(b, pl);
+  const T t = sA / (sA - sB);
+  return add(a, mul(sub(b, a), t));
+}
+template <class T> inline V3<T> cand_point(const Tet<T>& A, const Tet<T>& B, uint8_t code) {
+  const int kind = code & 3, ve = (code >> 2) & 7, f = (code >> 5) & 3;
+  if (kind == 0) return B.p[ve];
+  if (kind == 1) return A.p[ve];
+  if (kind == 2) return crossing_frozen(B.p[EDGE[ve][0]], B.p[EDGE[ve][1]], face_uplane(A, f));
+  return crossing_frozen(A.p[EDGE[ve][0]], A.p[EDGE[ve][1]], face_uplane(B, f));
+}
+template <class T> inline T frozen_depth(const Tet<T>& A, const Tet<T>& B, int face, uint8_t cmin, uint8_t cmax) {
+  const V3<T> n = unit(face_uplane(A, face).n);
+  return dot(cand_point(A, B, cmin), n) - dot(cand_point(A, B, cmax), n);
+}
+
+// Reverse mode of the frozen depth: adds d(depth)/d(vertex) * w to gA, gB.
+// Plane through (p0,p1,p2) as N = (p1-p0) x (p2-p0), point p0 (the three
+// choices make_unscaled_plane picks between are equal in exact arithmetic).
+inline void grad_cross_plane(const V3<double>& p0, const V3<double>& p1, const V3<double>& p2, const V3<double>& gN,
+                             V3<double>& g0, V3<double>& g1, V3<double>& g2) {
+  const V3<double> e1 = sub(p1, p0), e2 = sub(p2, p0);
+  const V3<double> ge1 = cross(e2, gN), ge2 = cross(gN, e1);
+  g1 = add(g1, ge1); g2 = add(g2, ge2); g0 = sub(g0, add(ge1, ge2));
+}
+inline void grad_cand(const Tet<double>& A, const Tet<double>& B, uint8_t code, const V3<double>& gq, V3<double>* gA, V3<double>* gB) {
+  const int kind = code & 3, ve = (code >> 2) & 7, f = (code >> 5) & 3;
+  if (kind == 0) { gB[ve] = add(gB[ve], gq); return; }
+  if (kind == 1) { gA[ve] = add(gA[ve], gq); return; }
+  // edge (a,b) of one tet crossing the face plane f of the other
+  const Tet<double>& E = (kind == 2) ? B : A;   // tet owning the edge
+  const Tet<double>& F = (kind == 2) ? A : B;   // tet owning the plane
+  V3<double>* gE = (kind == 2) ? gB : gA;
+  V3<double>* gF = (kind == 2) ? gA : gB;
+  const int ia = EDGE[ve][0], ib = EDGE[ve][1];
+  const V3<double>& a = E.p[ia];
+  const V3<double>& b = E.p[ib];
+  const int f0 = FACE[f][0], f1 = FACE[f][1], f2 = FACE[f][2];
+  const V3<double> N = cross(sub(F.p[f1], F.p[f0]), sub(F.p[f2], F.p[f0]));
+  const double sA = dot(N, sub(a, F.p[f0])), sB = dot(N, sub(b, F.p[f0]));
+  const double den = sA - sB;
+  const double t = sA / den;
+  gE[ia] = add(gE[ia], mul(gq, 1.0 - t));
+  gE[ib] = add(gE[ib], mul(gq, t));
+  const double gt = dot(sub(b, a), gq);
+  const double gsA = gt * (-sB / (den * den)), gsB = gt * (sA / (den * den));
+  gE[ia] = add(gE[ia], mul(N, gsA));
+  gE[ib] = add(gE[ib], mul(N, gsB));
+  const V3<double> gN = add(mul(sub(a, F.p[f0]), gsA), mul(sub(b, F.p[f0]), gsB));
+  gF[f0] = sub(gF[f0], mul(N, gsA + gsB));
+  grad_cross_plane(F.p[f0], F.p[f1], F.p[f2], gN, gF[f0], gF[f1], gF[f2]);
+}
+inline double frozen_depth_grad(const Tet<double>& A, const Tet<double>& B, int face, uint8_t cmin, uint8_t cmax, double w,
+                                V3<double>* gA, V3<double>* gB) {
+  const int f0 = FACE[face][0], f1 = FACE[face][1], f2 = FACE[face][2];
+  const V3<double> N = cross(sub(A.p[f1], A.p[f0]), sub(A.p[f2], A.p[f0]));
+  const double len = std::sqrt(dot(N, N));
+  const V3<double> n = mul(N, 1.0 / len);
+  const V3<double> pmin = cand_point(A, B, cmin), pmax = cand_point(A, B, cmax);
+  const double depth = dot(pmin, n) - dot(pmax, n);
+  grad_cand(A, B, cmin, mul(n, w), gA, gB);
+  grad_cand(A, B, cmax, mul(n, -w), gA, gB);
+  const V3<double> gn = mul(sub(pmin, pmax), w);
+  const V3<double> gN = mul(sub(gn, mul(n, dot(n, gn))), 1.0 / len);
+  grad_cross_plane(A.p[f0], A.p[f1], A.p[f2], gN, gA[f0], gA[f1], gA[f2]);
+  return depth;
+}
+// Pose gradient of r = Im(q r0 q*) + p = (s^2 - u.u) r0 + 2 (u.r0) u + 2 s (u x r0) + p.
+inline void pose_grad(const double q[4], const V3<double>& r0, const V3<double>& g, double* gp, double* gq) {
+  const V3<double> u = mk(q[0], q[1], q[2]);
+  const double s = q[3];
+  gp[0] += g.x; gp[1] += g.y; gp[2] += g.z;
+  const V3<double> gu = add(add(mul(u, -2.0 * dot(g, r0)), mul(r0, 2.0 * dot(g, u))), add(mul(g, 2.0 * dot(u, r0)), mul(cross(r0, g), 2.0 * s)));
+  gq[0] += gu.x; gq[1] += gu.y; gq[2] += gu.z;
+  gq[3] += dot(g, add(mul(r0, 2.0 * s), mul(cross(u, r0), 2.0)));
 }
 
 // -------------------------------------------------------------- traversal
@@ -802,7 +892,8 @@ uint64_t orc_brute_pairs(void* hA, void* hB, uint32_t* out, uint64_t cap) {
 }  // extern "C"
 
 template <class T>
-static int contacts_pair(const T* pa, const T* pb, int sa_bits, int sb_bits, T* n_out, T* depth_out, int* face_out, T* pts_out, int cap, int* ncand_out) {
+static int contacts_pair(const T* pa, const T* pb, int sa_bits, int sb_bits, T* n_out, T* depth_out, int* face_out, T* pts_out, int cap, int* ncand_out,
+                         uint8_t* codes = nullptr, uint8_t* uniq = nullptr, int* iminmax = nullptr) {
   Tet<T> A, B;
   for (int v = 0; v < 4; ++v) { A.p[v] = mk(pa[3 * v], pa[3 * v + 1], pa[3 * v + 2]); B.p[v] = mk(pb[3 * v], pb[3 * v + 1], pb[3 * v + 2]); }
   bool sA[4], sB[4];
@@ -815,6 +906,11 @@ static int contacts_pair(const T* pa, const T* pb, int sa_bits, int sb_bits, T* 
   n_out[0] = pc.n.x; n_out[1] = pc.n.y; n_out[2] = pc.n.z;
   *depth_out = pc.depth;
   if (ncand_out) *ncand_out = (int)pc.raw.size();
+  for (size_t c = 0; c < pc.cand.size() && c < 56; ++c) {
+    if (codes) codes[c] = cand_code(pc.cand[c]);
+    if (uniq) uniq[c] = pc.unique[c];
+  }
+  if (iminmax) { iminmax[0] = pc.imin; iminmax[1] = pc.imax; }
   int k = 0;
   for (size_t c = 0; c < pc.proj.size(); ++c) {
     if (!pc.unique[c]) continue;
@@ -827,6 +923,71 @@ extern "C" {
 
 int orc_contacts_pair_f32(const float* pa, const float* pb, int sa, int sb, float* n, float* depth, int* face, float* pts, int cap, int* ncand) {
   return contacts_pair<float>(pa, pb, sa, sb, n, depth, face, pts, cap, ncand);
+}
+// Same, also returning the provenance code and report flag of every candidate
+// and the indices of the candidates attaining min / max of p.n.
+int orc_contacts_pair_codes_f32(const float* pa, const float* pb, int sa, int sb, float* n, float* depth, int* face, float* pts, int cap,
+                                int* ncand, uint8_t* codes, uint8_t* uniq, int* iminmax) {
+  return contacts_pair<float>(pa, pb, sa, sb, n, depth, face, pts, cap, ncand, codes, uniq, iminmax);
+}
+
+// Frozen-choice FP64 geometry of one pair (see frozen_depth): unit normal,
+// depth, and the projected points of the reported candidates.
+int orc_frozen_pair_f64(const double* pa, const double* pb, int face, int ncand, const uint8_t* codes, const uint8_t* uniq,
+                        int imin, int imax, double* n_out, double* depth_out, double* pts_out) {
+  Tet<double> A, B;
+  for (int v = 0; v < 4; ++v) { A.p[v] = mk(pa[3 * v], pa[3 * v + 1], pa[3 * v + 2]); B.p[v] = mk(pb[3 * v], pb[3 * v + 1], pb[3 * v + 2]); }
+  const V3<double> n = unit(face_uplane(A, face).n);
+  const V3<double> pmin = cand_point(A, B, codes[imin]), pmax = cand_point(A, B, codes[imax]);
+  const double mn = dot(pmin, n), mx = dot(pmax, n);
+  n_out[0] = n.x; n_out[1] = n.y; n_out[2] = n.z;
+  *depth_out = mn - mx;
+  const V3<double> mid = mul(mul(n, mx + mn), 0.5);
+  int k = 0;
+  for (int c = 0; c < ncand; ++c) {
+    if (!uniq[c]) continue;
+    const V3<double> p = cand_point(A, B, codes[c]);
+    const double s = dot(n, sub(p, mid));
+    const V3<double> q = sub(p, mul(n, s));
+    pts_out[3 * k] = q.x; pts_out[3 * k + 1] = q.y; pts_out[3 * k + 2] = q.z;
+    ++k;
+  }
+  return k;
+}
+
+// Penetration measure of one body pair, Phi = sum over narrow records of
+// nuniq * depth, evaluated in FP64 from the material coordinates and the
+// poses with every discrete choice frozen (records: ta, tb, face, nuniq,
+// cmin, cmax); with grads non-null also dPhi/d(pose) by reverse mode:
+// grads = {gpA[3], gqA[4], gpB[3], gqB[4]} (q as x, y, z, w).
+double orc_phi_frozen_f64(uint32_t nrec, const uint32_t* ta, const uint32_t* tb, const uint8_t* face, const uint8_t* nuniq,
+                          const uint8_t* cmin, const uint8_t* cmax,
+                          const uint32_t* tetsA, const double* XA, const double* YA, const double* ZA, const double* pA, const double* qA,
+                          const uint32_t* tetsB, const double* XB, const double* YB, const double* ZB, const double* pB, const double* qB,
+                          double* grads) {
+  double phi = 0.0;
+  if (grads) for (int i = 0; i < 14; ++i) grads[i] = 0.0;
+  for (uint32_t r = 0; r < nrec; ++r) {
+    Tet<double> A, B;
+    V3<double> A0[4], B0[4];
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t ia = tetsA[4 * ta[r] + v], ib = tetsB[4 * tb[r] + v];
+      A0[v] = mk(XA[ia], YA[ia], ZA[ia]);
+      B0[v] = mk(XB[ib], YB[ib], ZB[ib]);
+      A.p[v] = add(rotate<double>(qA, A0[v]), mk(pA[0], pA[1], pA[2]));
+      B.p[v] = add(rotate<double>(qB, B0[v]), mk(pB[0], pB[1], pB[2]));
+    }
+    const double w = (double)nuniq[r];
+    if (!grads) { phi += w * frozen_depth(A, B, face[r], cmin[r], cmax[r]); continue; }
+    V3<double> gA[4], gB[4];
+    for (int v = 0; v < 4; ++v) { gA[v] = mk(0.0, 0.0, 0.0); gB[v] = mk(0.0, 0.0, 0.0); }
+    phi += w * frozen_depth_grad(A, B, face[r], cmin[r], cmax[r], w, gA, gB);
+    for (int v = 0; v < 4; ++v) {
+      pose_grad(qA, A0[v], gA[v], grads, grads + 3);
+      pose_grad(qB, B0[v], gB[v], grads + 7, grads + 10);
+    }
+  }
+  return phi;
 }
 int orc_contacts_pair_f64(const double* pa, const double* pb, int sa, int sb, double* n, double* depth, int* face, double* pts, int cap, int* ncand) {
   return contacts_pair<double>(pa, pb, sa, sb, n, depth, face, pts, cap, ncand);

@@ -53,6 +53,12 @@ def lib():
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
         L.orc_contacts_pair_f64.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
         L.orc_overlap_tet_tet_f32.argtypes = [_p, _p]
+        L.orc_contacts_pair_codes_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p,
+                                                  _p, _p, _p]
+        L.orc_frozen_pair_f64.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p]
+        L.orc_phi_frozen_f64.restype = ctypes.c_double
+        L.orc_phi_frozen_f64.argtypes = [_u32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                         _p]
         _lib = L
     return _lib
 
@@ -211,3 +217,58 @@ def overlap_tet_tet(pa, pb):
     pa = np.ascontiguousarray(pa, np.float32).reshape(12)
     pb = np.ascontiguousarray(pb, np.float32).reshape(12)
     return bool(lib().orc_overlap_tet_tet_f32(_ptr(pa), _ptr(pb)))
+
+
+def contacts_pair_codes(pa, pb, sa, sb):
+    """Float32 narrow phase of one pair plus its discrete choices:
+    dict(n, depth, face, pts, codes, uniq, imin, imax)."""
+    pa = np.ascontiguousarray(pa, np.float32).reshape(12)
+    pb = np.ascontiguousarray(pb, np.float32).reshape(12)
+    n = np.zeros(3, np.float32)
+    d = np.zeros(1, np.float32)
+    face = ctypes.c_int()
+    ncand = ctypes.c_int()
+    pts = np.zeros((56, 3), np.float32)
+    codes = np.zeros(56, np.uint8)
+    uniq = np.zeros(56, np.uint8)
+    mm = np.zeros(2, np.int32)
+    k = lib().orc_contacts_pair_codes_f32(_ptr(pa), _ptr(pb), int(sa), int(sb), _ptr(n), _ptr(d), ctypes.byref(face),
+                                          _ptr(pts), 56, ctypes.byref(ncand), _ptr(codes), _ptr(uniq), _ptr(mm))
+    nc = ncand.value
+    return dict(n=n, depth=float(d[0]), face=face.value, pts=pts[:k], codes=codes[:nc], uniq=uniq[:nc],
+                imin=int(mm[0]), imax=int(mm[1]))
+
+
+def frozen_pair_f64(pa, pb, face, codes, uniq, imin, imax):
+    """FP64 evaluation of one pair with the float run's discrete choices."""
+    pa = np.ascontiguousarray(pa, np.float64).reshape(12)
+    pb = np.ascontiguousarray(pb, np.float64).reshape(12)
+    codes = np.ascontiguousarray(codes, np.uint8)
+    uniq = np.ascontiguousarray(uniq, np.uint8)
+    n = np.zeros(3)
+    d = np.zeros(1)
+    pts = np.zeros((56, 3))
+    k = lib().orc_frozen_pair_f64(_ptr(pa), _ptr(pb), int(face), codes.shape[0], _ptr(codes), _ptr(uniq), int(imin),
+                                  int(imax), _ptr(n), _ptr(d), _ptr(pts))
+    return n, float(d[0]), pts[:k]
+
+
+def phi_frozen_f64(recs, mesh_a, pose_a, mesh_b, pose_b, grad=False):
+    """Phi = sum nuniq * depth over narrow records (fields ta, tb, face, nuniq,
+    cmin, cmax) of one body pair, in FP64 from material coordinates and poses
+    (p[3], q[4] as x y z w).  With grad: (phi, gpA, gqA, gpB, gqB)."""
+    cols = [np.ascontiguousarray(recs[f], dt) for f, dt in
+            (("ta", np.uint32), ("tb", np.uint32), ("face", np.uint8), ("nuniq", np.uint8), ("cmin", np.uint8),
+             ("cmax", np.uint8))]
+    def m(mesh):
+        return (np.ascontiguousarray(mesh.tets, np.uint32),) + tuple(np.ascontiguousarray(a, np.float64)
+                                                                    for a in (mesh.X, mesh.Y, mesh.Z))
+    ta, Xa, Ya, Za = m(mesh_a)
+    tb, Xb, Yb, Zb = m(mesh_b)
+    pa, qa, pb, qb = (np.ascontiguousarray(v, np.float64) for v in (pose_a[0], pose_a[1], pose_b[0], pose_b[1]))
+    g = np.zeros(14) if grad else None
+    phi = lib().orc_phi_frozen_f64(cols[0].shape[0], *(_ptr(c) for c in cols), _ptr(ta), _ptr(Xa), _ptr(Ya), _ptr(Za),
+                                   _ptr(pa), _ptr(qa), _ptr(tb), _ptr(Xb), _ptr(Yb), _ptr(Zb), _ptr(pb), _ptr(qb), _ptr(g))
+    if not grad:
+        return phi
+    return phi, g[0:3], g[3:7], g[7:10], g[10:14]

@@ -71,21 +71,26 @@ def _check_pair_results(wl, dev, orc, exact_contacts=True):
         assert np.count_nonzero(got.astype(np.float32) != want.astype(np.float32)) == 0
 
 
-def test_single_pair_box10_k8():
+MODES = [K.KDOP_TRAVERSAL_PERSISTENT, K.KDOP_TRAVERSAL_LEVELS]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["persistent", "levels"])
+def test_single_pair_box10_k8(mode):
     """Two copies of the 10^3-cube box (5000 tets), K=8: full parity."""
     wl = W.single_pair(n=10, K=8)
     orc, _, _ = oracle_run(wl)
-    dev = device_run(wl)
+    dev = device_run(wl, traversal=mode)
     _check_pair_results(wl, dev, orc)
     assert dev["counters"]["leaf_pairs"] > 100000
 
 
+@pytest.mark.parametrize("mode", MODES, ids=["persistent", "levels"])
 @pytest.mark.parametrize("k", [6, 8, 12, 14, 18, 20, 26])
-def test_jittered_pair_all_k(k):
+def test_jittered_pair_all_k(k, mode):
     """Jittered 12^3-cube boxes (8640 tets), random pose, every supported K."""
     wl = W.large_pair(n=12, K=k, seed=3)
     orc, _, _ = oracle_run(wl)
-    dev = device_run(wl)
+    dev = device_run(wl, traversal=mode)
     _check_pair_results(wl, dev, orc)
 
 
@@ -111,9 +116,29 @@ def test_refit_bits_and_world_coordinates():
             np.testing.assert_array_equal(v, ref[(int(f), int(l))])
 
 
-def test_batched_pairs_one_launch():
+@pytest.mark.parametrize("mode", MODES, ids=["persistent", "levels"])
+def test_batched_pairs_one_launch(mode):
     """Several body pairs of different sizes (some separated) in one call."""
     wl = W.batched_pairs(npairs=12, K=14, seed=4, tmin=200, tmax=6000)
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl, traversal=mode)
+    _check_pair_results(wl, dev, orc)
+
+
+@pytest.mark.parametrize("ls,wpc", [(256, 1), (512, 4), (2048, 16)])
+def test_persistent_queue_shapes(ls, wpc):
+    """Persistent kernel with tiny and large LDS stacks / few and many waves:
+    stack overflow into the ring and work stealing must not change results."""
+    wl = W.large_pair(n=10, K=18, seed=13)
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl, traversal=K.KDOP_TRAVERSAL_PERSISTENT, queue_depth=ls, waves_per_cu=wpc)
+    _check_pair_results(wl, dev, orc)
+
+
+def test_duplicate_body_pairs_fall_back_to_levels():
+    """The same body pair twice in one call: both copies are reported."""
+    wl = W.single_pair(n=5, K=8)
+    wl.pairs = np.array([[0, 1], [0, 1]], np.uint32)
     orc, _, _ = oracle_run(wl)
     dev = device_run(wl)
     _check_pair_results(wl, dev, orc)
