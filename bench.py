@@ -56,7 +56,7 @@ def headline_leg(args):
     import kdop_amd as KA
     from kdop_amd import workloads as W
     wl = W.large_pair(n=args.n, K=args.k, seed=7)
-    ctx = KA.Context(K=args.k)
+    ctx = KA.Context(K=args.k, traversal=args.traversal, queue_depth=args.queue_depth, waves_per_cu=args.waves_per_cu)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
 
@@ -91,7 +91,8 @@ def batched_leg(args):
     import kdop_amd as KA
     from kdop_amd import workloads as W
     wl = W.batched_pairs(npairs=args.batched_pairs, K=args.batched_k, seed=11)
-    ctx = KA.Context(K=args.batched_k)
+    ctx = KA.Context(K=args.batched_k, traversal=args.traversal, queue_depth=args.queue_depth,
+                     waves_per_cu=args.waves_per_cu)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
     ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
@@ -166,6 +167,9 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 level-synchronous, 2 persistent")
+    ap.add_argument("--queue-depth", type=int, default=0, help="persistent kernel: LDS stack entries per wave")
+    ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent kernel: waves per CU")
     ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -78,17 +78,6 @@ struct NarrowRec {
   float n[3];
 };
 
-// Shared state of the persistent traversal's global work queue: a ring of
-// 64-bit items (a << 32 | b, node indices), ~0 marks a free slot.
-struct QueueState {
-  unsigned long long head;  // next slot to hand out (monotone)
-  unsigned long long tail;  // next slot to reserve for a push (monotone)
-  int pending;              // items pushed but not yet expanded
-  unsigned int error;       // nonzero: a bounded wait ran out (the call is re-run level by level)
-  unsigned long long steals; // batches taken from the global queue
-  unsigned long long donations; // batches pushed to the global queue
-};
-
 // Device counters of one traversal call.
 struct DevCounters {
   unsigned long long tests;

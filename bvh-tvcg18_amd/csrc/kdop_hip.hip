@@ -67,15 +67,13 @@ struct kdop_ctx {
   // refit scratch
   DevBuf<uint32_t> plan;
   DevBuf<float> poses;
-  // persistent traversal: global work ring, its state, the pair-key table
-  DevBuf<unsigned long long> ring;
-  QueueState* qstate = nullptr;
-  QueueState* hqstate = nullptr;  // pinned
-  DevBuf<unsigned long long> pkeys;
-  DevBuf<uint32_t> pindex;
+  // persistent traversal: ramp / round state (device + pinned host copy)
+  PersistState* pstate = nullptr;
+  PersistState* hpstate = nullptr;
   int num_cus = 0;
-  int ls = 1024, wpc = 8, donate_min = 128, ring_low = 0;
-  uint64_t persistent_fallbacks = 0;
+  int ls = 1024, wpc = 8;
+  uint32_t ramp_target = 0;  // 0: 2 x persistent waves
+  uint32_t last_rounds = 0;
   bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double timing[4] = {0, 0, 0, 0};
@@ -297,10 +295,8 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
 // --------------------------------------------------------------- traversal
 template <int K>
 struct Seed {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t npairs) {
-    DevCounters* d = c->dcnt;
-    hipLaunchKernelGGL(k_seed<K>, dim3(npairs), dim3(64), 0, c->stream, T, c->front[0].p, &d->level_count[0],
-                       (uint32_t)c->front[0].cap);
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t npairs, uint32_t* fill, uint32_t cap) {
+    hipLaunchKernelGGL(k_seed<K>, dim3(npairs), dim3(64), 0, c->stream, T, c->front[0].p, fill, cap);
   }
 };
 template <int K>
@@ -318,68 +314,25 @@ struct Fills {
 };
 
 template <int K>
-struct SeedRing {
-  static void run(kdop_ctx* c, const TravArgs& T, const PersistArgs& P, uint32_t npairs) {
-    hipLaunchKernelGGL(k_seed_ring<K>, dim3(npairs), dim3(64), 0, c->stream, T, P);
+struct Ramp {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t target, uint32_t max_levels) {
+    hipLaunchKernelGGL(k_ramp<K>, dim3(1), dim3(512), 0, c->stream, T, c->front[0].p, c->front[1].p, cap, c->pstate, target, max_levels);
   }
 };
 template <int K>
-struct Persist {
-  static void run(kdop_ctx* c, const PersistArgs& P) {
+struct PersistRound {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) {
     const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
+    Item* b0 = c->front[0].p;
+    Item* b1 = c->front[1].p;
     switch (c->ls) {
-      case 256: hipLaunchKernelGGL((k_traverse_persistent<K, 256>), grid, block, 0, c->stream, P); break;
-      case 512: hipLaunchKernelGGL((k_traverse_persistent<K, 512>), grid, block, 0, c->stream, P); break;
-      case 2048: hipLaunchKernelGGL((k_traverse_persistent<K, 2048>), grid, block, 0, c->stream, P); break;
-      default: hipLaunchKernelGGL((k_traverse_persistent<K, 1024>), grid, block, 0, c->stream, P); break;
+      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
+      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
+      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
+      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
     }
   }
 };
-
-struct Fills;
-
-// Ring capacity, pair-key table and queue state for one persistent call.
-int prepare_persistent(kdop_ctx* c, const std::vector<std::pair<unsigned long long, uint32_t>>& keys, uint64_t seed_bound) {
-  uint64_t q = 1ull << 22;
-  while (q < 2 * seed_bound + (uint64_t)c->num_cus * c->wpc * 512) q <<= 1;
-  if (c->ring.cap < q) {
-    release(c->ring);
-    int rc = ensure(c, c->ring, q);
-    if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(c->ring.p, 0xFF, q * sizeof(unsigned long long), c->stream));
-  }
-  const size_t n = keys.size();
-  int rc = ensure(c, c->pkeys, std::max<size_t>(n, 1));
-  if (rc) return rc;
-  if ((rc = ensure(c, c->pindex, std::max<size_t>(n, 1)))) return rc;
-  std::vector<unsigned long long> k(n);
-  std::vector<uint32_t> idx(n);
-  for (size_t i = 0; i < n; ++i) { k[i] = keys[i].first; idx[i] = keys[i].second; }
-  if (n) {
-    HIPCHK(c, hipMemcpyAsync(c->pkeys.p, k.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->pindex.p, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-  }
-  return KDOP_OK;
-}
-
-PersistArgs persist_args(kdop_ctx* c, const TravArgs& T, uint32_t npairs) {
-  PersistArgs P;
-  P.nodes = c->nodes.p;
-  P.cnt = c->dcnt;
-  P.qs = c->qstate;
-  P.ring = c->ring.p;
-  P.qmask = c->ring.cap - 1;
-  P.p1 = T.p1; P.p2 = T.p2; P.p1_fill = T.p1_fill; P.p2_fill = T.p2_fill; P.p1_cap = T.p1_cap; P.p2_cap = T.p2_cap;
-  P.pair_keys = c->pkeys.p;
-  P.pair_index = c->pindex.p;
-  P.npairs = npairs;
-  P.donate_min = (uint32_t)std::max(64, c->donate_min);
-  P.ring_low = (uint32_t)(c->ring_low > 0 ? c->ring_low : c->num_cus * c->wpc * 2);
-  P.spin_limit = 1u << 22;
-  return P;
-}
-
-void launch_persistent(kdop_ctx* c, const PersistArgs& P) { dispatch_K<Persist>(c->K, c, P); }
 
 
 
@@ -417,8 +370,8 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
     if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   if (hipMalloc(&c->dcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
   if (hipHostMalloc(&c->hcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipMalloc(&c->qstate, sizeof(QueueState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipHostMalloc(&c->hqstate, sizeof(QueueState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipMalloc(&c->pstate, sizeof(PersistState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipHostMalloc(&c->hpstate, sizeof(PersistState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
@@ -442,9 +395,8 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
-  release(c->ring); release(c->pkeys); release(c->pindex);
-  if (c->qstate) (void)hipFree(c->qstate);
-  if (c->hqstate) (void)hipHostFree(c->hqstate);
+  if (c->pstate) (void)hipFree(c->pstate);
+  if (c->hpstate) (void)hipHostFree(c->hpstate);
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -575,26 +527,37 @@ int kdop_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos
   return kdop_refit_device(c, n, bodies, c->poses.p, c->poses.p + 3 * (size_t)n);
 }
 
+}  // extern "C"
+
+namespace {
+
+NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, Fills* dfill) {
+  NarrowArgs A;
+  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &dfill->p2; A.p2_cap = T.p2_cap;
+  A.contacts = c->contacts.p; A.c_fill = &dfill->c; A.c_cap = (uint32_t)std::min<size_t>(c->contacts.cap, 0xFFFFFFFFu);
+  A.recs = c->recs.p; A.r_fill = &dfill->r; A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
+  A.cnt = c->dcnt;
+  return A;
+}
+
+}  // namespace
+
+extern "C" {
+
 int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, kdop_counters* out) {
   if (!c) return KDOP_ERR_ARG;
   if (npairs && !body_pairs) return fail(c, KDOP_ERR_ARG, "kdop_traverse: null body_pairs");
   HIPCHK(c, hipSetDevice(c->cfg.device));
   uint32_t levels = 0;
-  uint64_t est = 0, seed_bound = 0;
+  uint64_t est = 0;
   std::vector<uint2> pairs(npairs);
-  std::vector<std::pair<unsigned long long, uint32_t>> keys(npairs);
   for (uint32_t i = 0; i < npairs; ++i) {
     const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
     if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
     pairs[i] = make_uint2(a, b);
-    keys[i] = {((unsigned long long)a << 32) | b, i};
     levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
-    est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
-    seed_bound += (uint64_t)c->bodies[a].ntop * c->bodies[b].ntop;
+    est += 4ull * (c->bodies[a].nt + c->bodies[b].nt) + (uint64_t)c->bodies[a].ntop * c->bodies[b].ntop;
   }
-  std::sort(keys.begin(), keys.end());
-  bool unique_pairs = true;
-  for (uint32_t i = 1; i < npairs; ++i) if (keys[i].first == keys[i - 1].first) unique_pairs = false;
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
   c->have_traversal = c->have_p1 = c->have_narrow = false;
   int rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1));
@@ -602,9 +565,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
-  // strategy: the persistent kernel recovers the body pair of a leaf pair from
-  // the leaf records, so it needs distinct (A, B) pairs in one call
-  bool persistent = (c->cfg.traversal != KDOP_TRAVERSAL_LEVELS) && unique_pairs;
+  const bool persistent = c->cfg.traversal != KDOP_TRAVERSAL_LEVELS;
   est = std::max<uint64_t>(est, 1 << 16);
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
@@ -612,11 +573,10 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if ((rc = ensure(c, c->contacts, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
     if ((rc = ensure(c, c->recs, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
   }
-  if (persistent) {
-    if ((rc = prepare_persistent(c, keys, seed_bound))) return rc;
-  } else {
-    if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
-  }
+  if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
+  const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
+  const uint32_t target = c->ramp_target ? c->ramp_target : 2 * pwaves;
+  const uint32_t rounds_async = 2;  // persistent rounds queued before the first host sync
   Fills* dfill = reinterpret_cast<Fills*>(c->dcnt + 1);
   uint64_t overflowed = 0;
   for (int attempt = 0; attempt < 10; ++attempt) {
@@ -628,48 +588,64 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.p1_cap = keep_p1 ? (uint32_t)std::min<size_t>(c->p1.cap, 0xFFFFFFFFu) : 0u;
     T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
     T.npairs = npairs;
+    const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
+    uint32_t rounds = 0;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
       if (persistent) {
-        HIPCHK(c, hipMemsetAsync(c->qstate, 0, sizeof(QueueState), c->stream));
-        PersistArgs P = persist_args(c, T, npairs);
-        dispatch_K<SeedRing>(c->K, c, T, P, npairs);
+        HIPCHK(c, hipMemsetAsync(c->pstate, 0, sizeof(PersistState), c->stream));
+        dispatch_K<Seed>(c->K, c, T, npairs, &c->pstate->level_fill[0], cap);
+        dispatch_K<Ramp>(c->K, c, T, cap, target, levels + 1);
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        launch_persistent(c, P);
+        for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
-        dispatch_K<Seed>(c->K, c, T, npairs);
+        dispatch_K<Seed>(c->K, c, T, npairs, &c->dcnt->level_count[0], cap);
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
       }
+    } else {
+      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (narrow && npairs) {
-      NarrowArgs A;
-      A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &dfill->p2; A.p2_cap = T.p2_cap;
-      A.contacts = c->contacts.p; A.c_fill = &dfill->c; A.c_cap = (uint32_t)std::min<size_t>(c->contacts.cap, 0xFFFFFFFFu);
-      A.recs = c->recs.p; A.r_fill = &dfill->r; A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
-      A.cnt = c->dcnt;
-      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, dfill));
       HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
-    if (persistent) HIPCHK(c, hipMemcpyAsync(c->hqstate, c->qstate, sizeof(QueueState), hipMemcpyDeviceToHost, c->stream));
+    if (persistent) HIPCHK(c, hipMemcpyAsync(c->hpstate, c->pstate, sizeof(PersistState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    bool grow_front = persistent ? (c->hpstate->overflow != 0) : ((c->hcnt->overflow & 1u) != 0);
+    if (persistent && !grow_front && npairs && c->hpstate->count[rounds] > 0) {
+      // spill left after the queued rounds: drain it, then redo the narrow phase
+      while (c->hpstate->count[rounds] > 0 && !c->hpstate->overflow) {
+        if (rounds + 1 >= (uint32_t)kMaxRounds) return fail(c, KDOP_ERR_CAPACITY, "kdop_traverse: spill rounds exhausted");
+        dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
+        ++rounds;
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(c->hpstate, c->pstate, sizeof(PersistState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+      }
+      grow_front = c->hpstate->overflow != 0;
+      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      if (narrow && !grow_front) {
+        HIPCHK(c, hipMemsetAsync(&dfill->c, 0, 2 * sizeof(uint32_t), c->stream));
+        HIPCHK(c, hipMemsetAsync(&c->dcnt->narrow_pairs, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, dfill));
+        HIPCHK(c, hipGetLastError());
+      }
+      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     const DevCounters& h = *c->hcnt;
     const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
     bool again = false;
-    if (persistent && c->hqstate->error) {
-      // a bounded wait ran out: scrub the ring and run this call level by level
-      HIPCHK(c, hipMemsetAsync(c->ring.p, 0xFF, c->ring.cap * sizeof(unsigned long long), c->stream));
-      persistent = false;
-      ++c->persistent_fallbacks;
-      if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
-      again = true;
-    } else if (!persistent && (h.overflow & 1u)) {
+    if (grow_front) {
       uint32_t mx = 0;
-      for (int l = 0; l < 64; ++l) mx = std::max(mx, h.level_count[l]);
+      for (int l = 0; l < 64; ++l) mx = std::max(mx, persistent ? c->hpstate->level_fill[l] : h.level_count[l]);
+      if (persistent) for (int r = 0; r < kMaxRounds; ++r) mx = std::max(mx, c->hpstate->count[r]);
       const size_t want = std::max<size_t>(4 * c->front[0].cap, (size_t)mx * 2);
       if ((rc = ensure(c, c->front[0], want)) || (rc = ensure(c, c->front[1], want))) return rc;
       again = true;
@@ -688,6 +664,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->timing[0] = ms[0]; c->timing[1] = ms[1]; c->timing[2] = ms[2];
     c->timing[3] = persistent ? ms[3] : (levels ? ms[3] / (double)levels : 0.0);
     c->last_levels = persistent ? 0 : levels;
+    c->last_rounds = persistent ? rounds : 0;
     c->last_persistent = persistent;
     c->last_npairs = npairs;
     c->last_pairs = pairs;

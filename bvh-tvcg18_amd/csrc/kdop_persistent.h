@@ -1,280 +1,171 @@
-// kdop_persistent.h -- persistent tandem traversal: one launch, per-wave work
-// stacks in LDS, a global ring of work items for seeding and work stealing.
+// kdop_persistent.h -- persistent tandem traversal.
 //
-// Every wave loops: take up to 64 node pairs (from the top of its own LDS
-// stack, else a batch from the global ring), expand them with the same rule
-// as the level-synchronous kernel (expand_item), push the overlapping child
-// pairs with one wave-wide prefix sum (to its stack, or to the ring when the
-// stack is full), write leaf pairs with one atomic per wave, and donate the
-// bottom of a deep stack to the ring when the ring runs low.  Termination: a
-// global count of items pushed but not yet expanded; a wave with no work
-// exits once it reads zero.  Every wait is bounded; running out sets
-// QueueState::error and the host re-runs the call level by level.
+// Three launches per call instead of one per BVTT level:
+//   k_seed  (one block per body pair)  root tests + top-level chunk-root pairs
+//   k_ramp  (one 512-thread block)     breadth-first levels, synchronised with
+//                                      __syncthreads, while the frontier is
+//                                      narrower than the persistent grid needs
+//   k_persist (persistent grid)        waves take guided chunks of that
+//                                      frontier (one atomic per chunk) and
+//                                      expand them depth first from a stack
+//                                      in LDS; a full stack spills to a
+//                                      global buffer that the next round
+//                                      (same kernel) takes as its frontier.
+// Agent-scope atomics and polling loads on one word are slow on a multi-XCD
+// part (they bypass the per-XCD L2s), so nothing here polls shared words:
+// the only shared RMWs are the chunk counter, the output counters (one per
+// wave and batch) and the spill counter.
 #pragma once
 
 #include "kdop_kernels.h"
 
 namespace kdop_amd {
 
-constexpr unsigned long long kFreeSlot = ~0ull;
+constexpr int kMaxRounds = 8;
 
-__device__ inline unsigned long long ld_relaxed(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_relaxed(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline int ld_relaxed_i(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
-struct PersistArgs {
-  const float* nodes;
-  DevCounters* cnt;
-  QueueState* qs;
-  unsigned long long* ring;
-  unsigned long long qmask;          // ring capacity - 1 (power of two)
-  LeafPair* p1;                      // null when P1 is not kept
-  LeafPair* p2;
-  uint32_t* p1_fill;
-  uint32_t* p2_fill;
-  uint32_t p1_cap, p2_cap;
-  const unsigned long long* pair_keys;  // sorted (body_a << 32 | body_b)
-  const uint32_t* pair_index;           // call-order pair index per sorted key
-  uint32_t npairs;
-  uint32_t donate_min;   // donate when the local stack holds more than this
-  uint32_t ring_low;     // ... and the ring holds fewer items than this
-  uint32_t spin_limit;   // bound of every wait loop (iterations)
+struct PersistState {
+  unsigned int level_fill[64];      // ramp: items per level (level 0 = seeds)
+  unsigned int final_level;         // ramp: level whose buffer holds the frontier
+  unsigned int final_count;
+  unsigned int ramp_done;           // ramp: 1 when the frontier is complete (empty)
+  unsigned int next[kMaxRounds];    // persist: chunk counter per round
+  unsigned int count[kMaxRounds];   // persist: frontier size per round (round r > 0: spill of round r-1)
+  unsigned int overflow;
+  unsigned int pad;
 };
 
-__device__ inline uint32_t pair_of(const PersistArgs& P, uint32_t body_a, uint32_t body_b) {
-  if (P.npairs <= 1) return 0;
-  const unsigned long long key = ((unsigned long long)body_a << 32) | body_b;
-  uint32_t lo = 0, hi = P.npairs;
-  while (hi - lo > 1) {
-    const uint32_t m = (lo + hi) >> 1;
-    if (P.pair_keys[m] <= key) lo = m; else hi = m;
-  }
-  return P.pair_index[lo];
-}
-
-// Seed the ring: one 64-lane block per body pair; root test, then all pairs
-// of top-level chunk roots (kdop_tandem_traversal.h:164-195).
+// Breadth-first levels in one workgroup until the frontier reaches `target`
+// items, runs empty, or `max_levels` levels are done.  Buffers ping-pong:
+// level l is read from buf[l & 1] and written to buf[(l + 1) & 1].
 template <int K>
-__global__ __launch_bounds__(64) void k_seed_ring(TravArgs T, PersistArgs P) {
-  const uint32_t pair = blockIdx.x;
-  const uint2 bp = T.pairs[pair];
-  const BodyDev& BA = T.bodies[bp.x];
-  const BodyDev& BB = T.bodies[bp.y];
-  NodeRec<K> ra, rb;
-  load_rec<K>(T.nodes, BA.node_base, ra);
-  load_rec<K>(T.nodes, BB.node_base, rb);
-  const bool ov = dop_overlap<K>(ra.v, rb.v);
-  if (lane_id() == 0) {
-    atomicAdd(&T.cnt->root_tests, 1ull);
-    atomicAdd(&T.cnt->tests, 1ull);
-    if (ov) atomicAdd(&T.cnt->branch_pairs, (unsigned long long)BA.nbranch * BB.nbranch);
-  }
-  if (!ov) return;
-  const uint32_t combos = BA.ntop * BB.ntop;
-  unsigned long long tests = 0, broot = 0, p1s = 0;
-  for (uint32_t base = 0; base < combos; base += 64) {
-    const uint32_t w = base + lane_id();
-    Expansion x;
-    x.push = x.leaf = x.surf = 0;
-    x.ca[0] = x.cb[0] = 0;
-    if (w < combos) {
-      const uint32_t i = w / BB.ntop, j = w - i * BB.ntop;
-      x.ca[0] = T.tops[BA.top_base + i];
-      x.cb[0] = T.tops[BB.top_base + j];
-      NodeRec<K> X, Y;
-      load_rec<K>(T.nodes, x.ca[0], X);
-      load_rec<K>(T.nodes, x.cb[0], Y);
-      x.ta[0] = X.m0 & IDX_MASK;
-      x.tb[0] = Y.m0 & IDX_MASK;
-      ++tests;
-      const bool both_roots = (X.m0 & F_BRANCH) && (Y.m0 & F_BRANCH) && (X.m1 & F_ROOT) && (Y.m1 & F_ROOT);
-      if (dop_overlap<K>(X.v, Y.v)) {
-        if (both_roots) ++broot;
-        if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
-          x.leaf = 1u;
-          x.surf = ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) ? 1u : 0u;
-        } else {
-          x.push = 1u;
-        }
+__global__ __launch_bounds__(512) void k_ramp(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+                                               PersistState* __restrict__ S, uint32_t target, uint32_t max_levels) {
+  const int lane = lane_id();
+  const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
+  for (uint32_t l = 0;; ++l) {
+    const uint32_t n = min(__hip_atomic_load(&S->level_fill[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
+    if (n == 0 || n >= target || l >= max_levels || l >= 62) {
+      if (threadIdx.x == 0) {
+        S->final_level = l;
+        S->final_count = n;
+        S->count[0] = n;
+        S->ramp_done = (n == 0) ? 1u : 0u;
       }
+      break;
     }
-    const uint32_t incl = wave_incl_scan(x.push);
-    const uint32_t total = __shfl(incl, 63, 64);
-    unsigned long long slot0 = 0;
-    if (lane_id() == 0 && total) {
-      atomicAdd(&P.qs->pending, (int)total);
-      slot0 = atomicAdd(&P.qs->tail, (unsigned long long)total);
+    const Item* in = (l & 1) ? buf1 : buf0;
+    Item* out = (l & 1) ? buf0 : buf1;
+    for (uint32_t base = wave * 64; base < n; base += nwaves * 64) {
+      const uint32_t i = base + lane;
+      Expansion x;
+      x.push = x.leaf = x.surf = 0;
+      x.tests = x.descent = x.broot = 0;
+      Item it{0, 0, 0};
+      if (i < n) {
+        it = in[i];
+        expand_item<K>(T.nodes, it.a, it.b, x);
+      }
+      tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
+      const uint32_t o = wave_reserve(&S->level_fill[l + 1], __popc(x.push));
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((x.push >> k) & 1u) {
+          const uint32_t slot = o + __popc(x.push & ((1u << k) - 1u));
+          if (slot < cap) out[slot] = Item{x.ca[k >> 1], x.cb[k & 1], it.pair};
+          else atomicOr(&S->overflow, 1u);
+        }
+      emit_leaves(T, it.pair, x);
     }
-    slot0 = __shfl(slot0, 0, 64);
-    if (x.push) P.ring[(slot0 + incl - 1) & P.qmask] = ((unsigned long long)x.ca[0] << 32) | x.cb[0];
-    p1s += x.leaf;
-    emit_leaves(T, pair, x);
+    __syncthreads();
   }
   wave_add64(&T.cnt->tests, tests);
+  wave_add64(&T.cnt->descent, descent);
   wave_add64(&T.cnt->branch_overlaps, broot);
   wave_add64(&T.cnt->p1, p1s);
 }
 
-// Write this lane's leaf pairs; the body pair comes from the leaf records.
-__device__ inline void emit_leaves_p(const PersistArgs& P, const Expansion& x, uint32_t body_a, uint32_t body_b) {
-  uint32_t pair = 0;
-  if (x.leaf) pair = pair_of(P, body_a, body_b);
-  if (P.p1) {
-    const uint32_t o1 = wave_reserve(P.p1_fill, __popc(x.leaf));
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if ((x.leaf >> k) & 1u) {
-        const uint32_t o = o1 + __popc(x.leaf & ((1u << k) - 1u));
-        if (o < P.p1_cap) P.p1[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
-      }
-  }
-  const uint32_t o2 = wave_reserve(P.p2_fill, __popc(x.surf));
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if ((x.surf >> k) & 1u) {
-      const uint32_t o = o2 + __popc(x.surf & ((1u << k) - 1u));
-      if (o < P.p2_cap) P.p2[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
-    }
-}
-
-// Push `cnt` items of this lane (item k of `items`) to the ring.  Every lane
-// of the wave must call it.  Waits (bounded) for a slot still being read.
-__device__ inline bool ring_push(const PersistArgs& P, const unsigned long long* items, uint32_t cnt, uint32_t total) {
-  unsigned long long base = 0;
-  if (lane_id() == 0 && total) base = atomicAdd(&P.qs->tail, (unsigned long long)total);
-  base = __shfl(base, 0, 64);
-  const uint32_t incl = wave_incl_scan(cnt);
-  bool ok = true;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    unsigned long long* slot = P.ring + ((base + incl - cnt + k) & P.qmask);
-    uint32_t spins = 0;
-    while (ld_relaxed(slot) != kFreeSlot) {
-      if (++spins > P.spin_limit) { ok = false; break; }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    if (!ok) break;
-    st_relaxed(slot, items[k]);
-  }
-  return ok;
-}
-
+// One round of the persistent traversal over frontier `in` (S->count[round]
+// items); children that do not fit the LDS stack go to `spill`, which becomes
+// the frontier of round + 1.
 template <int K, int LS>
-__global__ __launch_bounds__(64) void k_traverse_persistent(PersistArgs P) {
-  static_assert((LS & (LS - 1)) == 0, "LDS stack size must be a power of two");
-  __shared__ unsigned long long stack[LS];
+__global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+                                                PersistState* __restrict__ S, uint32_t round) {
+  static_assert((LS & (LS - 1)) == 0 && LS >= 256, "LDS stack: power of two, >= 256");
+  __shared__ uint32_t sa[LS], sb[LS], sp[LS];
   const int lane = lane_id();
-  uint32_t top = 0, bot = 0;  // stack holds items [bot, top) modulo LS (wave-uniform)
-  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0, steals = 0, donations = 0;
-  uint32_t idle = 0;
-  bool failed = false;
+  // the ramp left its frontier in buf[final_level & 1]; each round's spill
+  // goes to the other buffer and is the next round's frontier
+  const bool odd = ((S->final_level + round) & 1u) != 0u;
+  const Item* __restrict__ in = odd ? buf1 : buf0;
+  Item* __restrict__ spill = odd ? buf0 : buf1;
+  const uint32_t n_front = min(S->count[round], cap);
+  const uint32_t nwaves = gridDim.x;
+  uint32_t top = 0;
+  uint32_t next_est = 0;  // last chunk start seen, for the guided chunk size
+  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   for (;;) {
-    // ---- take up to 64 items
-    unsigned long long item = kFreeSlot;
+    Item it{0, 0, 0};
     uint32_t n = 0;
-    if (top != bot) {
-      n = min(64u, top - bot);
-      if ((uint32_t)lane < n) item = stack[(top - n + lane) & (LS - 1)];
-      top -= n;
+    if (top == 0) {
+      // guided self-scheduling: chunks shrink as the frontier runs out
+      const uint32_t remaining = n_front > next_est ? n_front - next_est : 0u;
+      uint32_t want = remaining / (2u * nwaves);
+      want = want < 1u ? 1u : (want > 64u ? 64u : want);
+      uint32_t start = 0;
+      if (lane == 0) start = atomicAdd(&S->next[round], want);
+      start = __shfl(start, 0, 64);
+      if (start >= n_front) break;
+      next_est = start + want;
+      n = min(want, n_front - start);
+      if ((uint32_t)lane < n) it = in[start + lane];
     } else {
-      unsigned long long h = 0;
-      if (lane == 0) {
-        for (uint32_t tries = 0;; ++tries) {
-          const unsigned long long hh = ld_relaxed(&P.qs->head);
-          const unsigned long long tt = ld_relaxed(&P.qs->tail);
-          if (hh >= tt || tries > 4096) { n = 0; break; }  // contended: retry after the idle check
-          const unsigned long long avail = tt - hh;
-          n = (uint32_t)(avail < 64 ? avail : 64);
-          if (atomicCAS(&P.qs->head, hh, hh + n) == hh) { h = hh; break; }
-        }
-      }
-      n = __shfl(n, 0, 64);
-      h = __shfl(h, 0, 64);
-      if (n == 0) {
-        int pend = 0;
-        if (lane == 0) pend = ld_relaxed_i(&P.qs->pending);
-        pend = __shfl(pend, 0, 64);
-        if (pend == 0) break;  // every pushed item has been expanded: done
-        if (++idle > P.spin_limit) { failed = true; break; }
-        __builtin_amdgcn_s_sleep(2);
-        continue;
-      }
-      ++steals;
+      n = min(64u, top);
       if ((uint32_t)lane < n) {
-        unsigned long long* slot = P.ring + ((h + lane) & P.qmask);
-        uint32_t spins = 0;
-        for (;;) {
-          item = ld_relaxed(slot);
-          if (item != kFreeSlot) break;
-          if (++spins > P.spin_limit) { failed = true; break; }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        st_relaxed(slot, kFreeSlot);
+        const uint32_t s = top - n + lane;
+        it = Item{sa[s], sb[s], sp[s]};
       }
-      if (__any(failed)) { failed = true; break; }
+      top -= n;
     }
-    idle = 0;
-    // ---- expand
     Expansion x;
     x.push = x.leaf = x.surf = 0;
     x.tests = x.descent = x.broot = 0;
-    uint32_t body_a = 0, body_b = 0;
-    if ((uint32_t)lane < n) {
-      expand_item<K>(P.nodes, (uint32_t)(item >> 32), (uint32_t)item, x);
-      body_a = x.body_a;
-      body_b = x.body_b;
-    }
+    if ((uint32_t)lane < n) expand_item<K>(T.nodes, it.a, it.b, x);
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-    // ---- push children: local stack when it has room, else the ring
     const uint32_t m = __popc(x.push);
     const uint32_t incl = wave_incl_scan(m);
     const uint32_t total = __shfl(incl, 63, 64);
-    if (total && lane == 0) atomicAdd(&P.qs->pending, (int)total);  // before they become visible
-    unsigned long long kids[4];
-    uint32_t nk = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if ((x.push >> k) & 1u) kids[nk++] = ((unsigned long long)x.ca[k >> 1] << 32) | x.cb[k & 1];
     if (total) {
-      if ((top - bot) + total <= (uint32_t)LS) {
+      if (top + total <= (uint32_t)LS) {
         uint32_t o = top + incl - m;
-        for (uint32_t k = 0; k < nk; ++k) stack[(o + k) & (LS - 1)] = kids[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((x.push >> k) & 1u) {
+            sa[o] = x.ca[k >> 1]; sb[o] = x.cb[k & 1]; sp[o] = it.pair;
+            ++o;
+          }
         top += total;
-        __syncthreads();  // one-wave block: orders the stack writes before the next pop
       } else {
-        ++donations;
-        if (!ring_push(P, kids, nk, total)) failed = true;
-        if (__any(failed)) { failed = true; }
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&S->count[round + 1], total);
+        base = __shfl(base, 0, 64);
+        uint32_t o = base + incl - m;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((x.push >> k) & 1u) {
+            if (o < cap) spill[o] = Item{x.ca[k >> 1], x.cb[k & 1], it.pair};
+            else atomicOr(&S->overflow, 1u);
+            ++o;
+          }
       }
     }
-    emit_leaves_p(P, x, body_a, body_b);
-    // ---- donate the bottom of a deep stack while the ring runs low
-    if (!failed && (top - bot) > P.donate_min) {
-      unsigned long long occ = 0;
-      if (lane == 0) occ = ld_relaxed(&P.qs->tail) - ld_relaxed(&P.qs->head);
-      occ = __shfl(occ, 0, 64);
-      if (occ < P.ring_low) {
-        const unsigned long long it = stack[(bot + lane) & (LS - 1)];
-        bot += 64;
-        ++donations;
-        if (!ring_push(P, &it, 1u, 64u)) failed = true;
-      }
-    }
-    if (lane == 0 && n) atomicSub(&P.qs->pending, (int)n);
-    if (__any(failed)) { failed = true; break; }
+    __syncthreads();  // one-wave block: stack writes land before the next pop
+    emit_leaves(T, it.pair, x);
   }
-  if (failed && lane == 0) atomicOr(&P.qs->error, 1u);
-  wave_add64(&P.cnt->tests, tests);
-  wave_add64(&P.cnt->descent, descent);
-  wave_add64(&P.cnt->branch_overlaps, broot);
-  wave_add64(&P.cnt->p1, p1s);
-  if (lane == 0) {
-    atomicAdd(&P.qs->steals, steals);
-    atomicAdd(&P.qs->donations, donations);
-  }
+  wave_add64(&T.cnt->tests, tests);
+  wave_add64(&T.cnt->descent, descent);
+  wave_add64(&T.cnt->branch_overlaps, broot);
+  wave_add64(&T.cnt->p1, p1s);
 }
 
 }  // namespace kdop_amd
