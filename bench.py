@@ -56,7 +56,8 @@ def headline_leg(args):
     import kdop_amd as KA
     from kdop_amd import workloads as W
     wl = W.large_pair(n=args.n, K=args.k, seed=7)
-    ctx = KA.Context(K=args.k, traversal=args.traversal, queue_depth=args.queue_depth, waves_per_cu=args.waves_per_cu)
+    ctx = KA.Context(K=args.k, traversal=args.traversal, queue_depth=args.queue_depth, waves_per_cu=args.waves_per_cu,
+                     ramp_target=args.ramp_target, keep=args.keep, seed_budget=args.seed_budget)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
 
@@ -82,7 +83,7 @@ def headline_leg(args):
     alg_bytes = counters["tests"] * 2 * node_bytes + counters["surface_pairs"] * 12
     achieved = alg_bytes / (trav_ms * 1e-3) / 1e9
     return dict(wl=wl, ctx=ctx, ids=ids, counters=counters, ms_per_step=ms_per_step, trav_ms=trav_ms,
-                narrow_ms=narrow_ms, alg_bytes=alg_bytes, achieved_gbs=achieved)
+                narrow_ms=narrow_ms, alg_bytes=alg_bytes, achieved_gbs=achieved, sched=ctx.traversal_stats())
 
 
 def batched_leg(args):
@@ -170,8 +171,23 @@ def main():
     ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 level-synchronous, 2 persistent")
     ap.add_argument("--queue-depth", type=int, default=0, help="persistent kernel: LDS stack entries per wave")
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent kernel: waves per CU")
+    ap.add_argument("--ramp-target", type=int, default=0, help="persistent kernel: frontier size that ends the ramp")
+    ap.add_argument("--keep", type=int, default=0, help="persistent kernel: stack kept once the frontier is gone")
+    ap.add_argument("--seed-budget", type=int, default=0, help="chunk-root pairs tested to seed a call")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: the kDOP pipeline here is a single-device build "
@@ -219,6 +235,7 @@ def main():
         "counters": cnt,
         "reference_tests_per_s": cnt["reference_tests"] * steps_per_s,
         "traversal_ms": head["trav_ms"],
+        "schedule": head["sched"],
         "narrow_ms": head["narrow_ms"],
         "batched": batched,
         "cpu_baseline": cpu,

@@ -1,6 +1,8 @@
 // kdop_device.h -- device-side data layout shared by the kDOP kernels.
 //
-// Node record (AoS, one per tree node, 4K+8 bytes):
+// Node record (AoS, one per tree node slot, rec_stride(K) floats: the power
+// of two >= K + 2, i.e. 32 B for K = 6, 64 B for K = 8..14, 128 B for K =
+// 18..26, so a record never straddles a 128-byte cache line):
 //   float vol[K]   slab k: vol[2k] = lower, vol[2k+1] = upper
 //   uint32 m0      internal: left child (global index)  | flags
 //                  tet leaf: tet index (body local)      | flags
@@ -11,6 +13,10 @@
 // Super-chunk leaves of the reference tree (copies of a child chunk's root
 // volume) are not stored: a super node points straight at the child chunk's
 // root, resolved through one-element chunks.
+// Slots: a chunk of stride 2L (branch) or L (super) keeps slot 0 free and its
+// BFS node j in slot j + 1, so siblings (BFS 2j+1, 2j+2, or any later pair the
+// median split appends) start on an even slot and share a line where records
+// are <= 64 bytes.  Free slots hold meta words 0xFFFFFFFF.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +30,8 @@ constexpr uint32_t F_ROOT = 0x80000000u;
 constexpr uint32_t F_SURF = 0x40000000u;
 constexpr uint32_t IDX_MASK = 0x3FFFFFFFu;
 constexpr int kMaxLevels = 24;
+
+__host__ __device__ constexpr int rec_stride(int K) { return K + 2 <= 8 ? 8 : (K + 2 <= 16 ? 16 : 32); }
 
 // Per-body descriptor, device resident (one entry per body of the context).
 struct BodyDev {
@@ -60,9 +68,13 @@ struct DirTable {
   float d[39];
 };
 
-// Work item of the traversal: a node pair known to overlap + body-pair index.
+// Work item of the traversal: a node pair known to overlap, with what its
+// expansion needs already resolved by the producer (which held both records):
+// per side either I_EXP | first child, second child (expand this side) or the
+// node itself (a0 = node, a1 unused).  pair: index into the call's pair list.
+constexpr uint32_t I_EXP = 0x80000000u;
 struct Item {
-  uint32_t a, b, pair;
+  uint32_t a0, a1, b0, b1, pair;
 };
 
 // Leaf pair output: body-pair index and the two tetrahedra.

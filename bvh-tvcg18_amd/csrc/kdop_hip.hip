@@ -24,8 +24,10 @@ namespace {
 struct BodyHost {
   uint32_t nv = 0, nt = 0, L = 0, H = 0, depth = 0;
   std::vector<uint32_t> C, E, level_base;  // per level (0 = top)
-  uint32_t nnodes = 0, node_base = 0;
+  uint32_t nnodes = 0, node_base = 0;  // slots (nodes + free slots), first slot
+  uint64_t real_nodes = 0;             // tree nodes below the root record
   uint32_t top_base = 0, ntop = 0;
+  std::vector<uint32_t> roots_off;  // per level: offset of the level's chunk roots in the context's root table
   float *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr;
   uint32_t* tets = nullptr;
   uint8_t* surf = nullptr;
@@ -55,6 +57,8 @@ struct kdop_ctx {
   DevBuf<uint32_t> dtops;
   // traversal state
   DevBuf<uint2> dpairs;
+  DevBuf<SeedDesc> dseeds;
+  uint32_t seed_budget = 65536;  // seeds (chunk-root pairs tested first) per call
   DevBuf<Item> front[2];
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
@@ -71,8 +75,9 @@ struct kdop_ctx {
   PersistState* pstate = nullptr;
   PersistState* hpstate = nullptr;
   int num_cus = 0;
-  int ls = 1024, wpc = 8;
+  int ls = 512, wpc = 4;
   uint32_t ramp_target = 0;  // 0: 2 x persistent waves
+  uint32_t keep = 128;       // items a wave keeps once the frontier is gone
   uint32_t last_rounds = 0;
   bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -175,11 +180,11 @@ void dispatch_K(int K, Args&&... args) {
 }
 
 uint32_t host_chunk_elems(const BodyHost& b, int h, uint32_t c) { return (c + 1 < b.C[h]) ? b.L : b.E[h] - c * b.L; }
-uint32_t host_stride(const BodyHost& b, int h) { return (h == (int)b.H - 1) ? 2 * b.L - 1 : b.L - 1; }
+uint32_t host_stride(const BodyHost& b, int h) { return (h == (int)b.H - 1) ? 2 * b.L : b.L; }
 uint32_t host_resolved_root(const BodyHost& b, int h, uint32_t c) {
   for (;;) {
     const uint32_t n = host_chunk_elems(b, h, c);
-    if (h == (int)b.H - 1 || n >= 2) return b.node_base + b.level_base[h] + c * host_stride(b, h);
+    if (h == (int)b.H - 1 || n >= 2) return b.node_base + b.level_base[h] + c * host_stride(b, h) + 1;
     c = c * b.L;
     ++h;
   }
@@ -295,8 +300,9 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
 // --------------------------------------------------------------- traversal
 template <int K>
 struct Seed {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t npairs, uint32_t* fill, uint32_t cap) {
-    hipLaunchKernelGGL(k_seed<K>, dim3(npairs), dim3(64), 0, c->stream, T, c->front[0].p, fill, cap);
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t total, uint32_t* fill, uint32_t cap) {
+    const uint32_t blocks = std::max(1u, std::min(4096u, (total + 255) / 256));
+    hipLaunchKernelGGL(k_seed<K>, dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[0].p, fill, cap);
   }
 };
 template <int K>
@@ -326,10 +332,10 @@ struct PersistRound {
     Item* b0 = c->front[0].p;
     Item* b1 = c->front[1].p;
     switch (c->ls) {
-      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
-      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
-      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
-      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round); break;
+      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
+      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
+      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
+      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
     }
   }
 };
@@ -379,6 +385,9 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
   }
   if (cfg->queue_depth == 256 || cfg->queue_depth == 512 || cfg->queue_depth == 1024 || cfg->queue_depth == 2048) c->ls = cfg->queue_depth;
   if (cfg->waves_per_cu > 0) c->wpc = std::min(cfg->waves_per_cu, 32);
+  if (cfg->ramp_target > 0) c->ramp_target = (uint32_t)cfg->ramp_target;
+  if (cfg->stack_keep > 0) c->keep = (uint32_t)cfg->stack_keep;
+  if (cfg->seed_budget > 0) c->seed_budget = (uint32_t)cfg->seed_budget;
   *out = c;
   return KDOP_OK;
 }
@@ -392,7 +401,7 @@ void kdop_destroy(kdop_ctx* c) {
     (void)hipFree(b.X); (void)hipFree(b.Y); (void)hipFree(b.Z);
     (void)hipFree(b.tets); (void)hipFree(b.surf);
   }
-  release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs);
+  release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
   if (c->pstate) (void)hipFree(c->pstate);
@@ -438,27 +447,36 @@ int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, co
     b.C[h - 1] = (uint32_t)chunks_above(b.C[h], b.L);
     b.E[h - 1] = b.C[h];
   }
-  // node counts: root record, then level by level
-  uint64_t off = 1;
+  // slots: root record (+ a free slot), then level by level; a chunk is one
+  // free slot + its nodes, every level starts on an even slot
+  uint64_t off = 2;
   for (uint32_t h = 0; h < b.H; ++h) {
     b.level_base[h] = (uint32_t)off;
     const uint32_t nlast = host_chunk_elems(b, (int)h, b.C[h] - 1);
     const bool branch = (h == b.H - 1);
-    const uint64_t last_nodes = branch ? (2ull * nlast - 1) : (nlast >= 2 ? nlast - 1 : 0);
-    off += (uint64_t)(b.C[h] - 1) * host_stride(b, (int)h) + last_nodes;
+    const uint64_t last_slots = branch ? 2ull * nlast : (nlast >= 2 ? nlast : 0);
+    off += (uint64_t)(b.C[h] - 1) * host_stride(b, (int)h) + last_slots;
+    b.real_nodes += (uint64_t)(b.C[h] - 1) * (host_stride(b, (int)h) - 1) + (last_slots ? last_slots - 1 : 0);
+    off += off & 1u;
   }
   if (c->nodes_used + off >= (1ull << 30)) return fail(c, KDOP_ERR_ARG, "kdop_add_body: node pool exceeds 2^30 nodes");
   b.nnodes = (uint32_t)off;
+  c->nodes_used += c->nodes_used & 1u;
   b.node_base = (uint32_t)c->nodes_used;
   uint32_t lg = 0;
   while ((1u << lg) < b.L) ++lg;
   b.depth = 1 + b.H * std::max(lg, 1u);
   // top-level resolved roots
+  // resolved chunk roots of every level (level 0 first: the refit's root union)
   b.top_base = (uint32_t)c->tops.size();
-  for (uint32_t t = 0; t < b.C[0]; ++t) c->tops.push_back(host_resolved_root(b, 0, t));
+  b.roots_off.assign(b.H, 0);
+  for (uint32_t h = 0; h < b.H; ++h) {
+    b.roots_off[h] = (uint32_t)c->tops.size();
+    for (uint32_t t = 0; t < b.C[h]; ++t) c->tops.push_back(host_resolved_root(b, (int)h, t));
+  }
   b.ntop = b.C[0];
   // device buffers
-  const size_t S = (size_t)c->K + 2;
+  const size_t S = (size_t)rec_stride(c->K);
   int rc = ensure(c, c->nodes, (c->nodes_used + b.nnodes) * S, true);
   if (rc) { c->tops.resize(b.top_base); return rc; }
   auto alloc = [&](void** p, size_t bytes) -> int {
@@ -487,9 +505,10 @@ int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, co
   P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt; P.body = id;
   uint32_t total_chunks = 0;
   for (uint32_t h = 0; h < b.H; ++h) { P.C[h] = b.C[h]; P.E[h] = b.E[h]; P.level_base[h] = b.level_base[h]; total_chunks += b.C[h]; }
-  // root record meta = 0
+  // free slots read as meta 0xFFFFFFFF; the root record's meta is 0
+  HIPCHK(c, hipMemsetAsync(c->nodes.p + (size_t)b.node_base * S, 0xFF, (size_t)b.nnodes * S * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->nodes.p + (size_t)b.node_base * S, 0, S * sizeof(float), c->stream));
-  hipLaunchKernelGGL(k_build_topology, dim3((total_chunks + 63) / 64), dim3(64), 0, c->stream, P, c->nodes.p, (int)S, b.surf, total_chunks);
+  hipLaunchKernelGGL(k_build_topology, dim3((total_chunks + 63) / 64), dim3(64), 0, c->stream, P, c->nodes.p, c->K, b.surf, total_chunks);
   HIPCHK(c, hipGetLastError());
   rc = run_refit(c, 1, &id, nullptr, nullptr, true);
   if (rc) return rc;
@@ -501,7 +520,7 @@ int kdop_body_shape(kdop_ctx* c, uint32_t body, uint64_t* out) {
   if (!c) return KDOP_ERR_ARG;
   if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "unknown body %u", body);
   const BodyHost& b = c->bodies[body];
-  out[0] = b.L; out[1] = b.C[b.H - 1]; out[2] = b.H; out[3] = b.nnodes; out[4] = b.depth;
+  out[0] = b.L; out[1] = b.C[b.H - 1]; out[2] = b.H; out[3] = b.real_nodes; out[4] = b.depth;
   return KDOP_OK;
 }
 
@@ -556,13 +575,38 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
     pairs[i] = make_uint2(a, b);
     levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
-    est += 4ull * (c->bodies[a].nt + c->bodies[b].nt) + (uint64_t)c->bodies[a].ntop * c->bodies[b].ntop;
+    est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
   }
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
   c->have_traversal = c->have_p1 = c->have_narrow = false;
   int rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1));
   if (rc) return rc;
   if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
+  // seed levels: per body pair the deepest pair of chunk levels whose root
+  // pairs fit the per-pair share of the seed budget
+  std::vector<SeedDesc> seeds(npairs);
+  uint64_t total_seeds = 0;
+  {
+    const uint64_t share = std::max<uint64_t>(1, c->seed_budget / std::max<uint32_t>(npairs, 1));
+    for (uint32_t i = 0; i < npairs; ++i) {
+      const BodyHost& A = c->bodies[pairs[i].x];
+      const BodyHost& B = c->bodies[pairs[i].y];
+      uint32_t ha = 0, hb = 0;
+      uint64_t best = (uint64_t)A.C[0] * B.C[0];
+      for (uint32_t x = 0; x < A.H; ++x)
+        for (uint32_t y = 0; y < B.H; ++y) {
+          const uint64_t prod = (uint64_t)A.C[x] * B.C[y];
+          if (prod > share || prod <= best) continue;
+          best = prod; ha = x; hb = y;
+        }
+      seeds[i] = SeedDesc{A.roots_off[ha], A.C[ha], B.roots_off[hb], B.C[hb], (uint32_t)total_seeds};
+      total_seeds += best;
+      est += best;
+    }
+  }
+  if (total_seeds >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_traverse: too many seed pairs");
+  if ((rc = ensure(c, c->dseeds, std::max<uint32_t>(npairs, 1)))) return rc;
+  if (npairs) HIPCHK(c, hipMemcpyAsync(c->dseeds.p, seeds.data(), npairs * sizeof(SeedDesc), hipMemcpyHostToDevice, c->stream));
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
   const bool persistent = c->cfg.traversal != KDOP_TRAVERSAL_LEVELS;
@@ -576,7 +620,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
   const uint32_t target = c->ramp_target ? c->ramp_target : 2 * pwaves;
-  const uint32_t rounds_async = 2;  // persistent rounds queued before the first host sync
+  const uint32_t rounds_async = 3;  // persistent rounds queued before the first host sync
   Fills* dfill = reinterpret_cast<Fills*>(c->dcnt + 1);
   uint64_t overflowed = 0;
   for (int attempt = 0; attempt < 10; ++attempt) {
@@ -594,12 +638,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if (npairs) {
       if (persistent) {
         HIPCHK(c, hipMemsetAsync(c->pstate, 0, sizeof(PersistState), c->stream));
-        dispatch_K<Seed>(c->K, c, T, npairs, &c->pstate->level_fill[0], cap);
+        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &c->pstate->level_fill[0], cap);
         dispatch_K<Ramp>(c->K, c, T, cap, target, levels + 1);
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
-        dispatch_K<Seed>(c->K, c, T, npairs, &c->dcnt->level_count[0], cap);
+        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &c->dcnt->level_count[0], cap);
         HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
       }
@@ -616,7 +660,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
     if (persistent) HIPCHK(c, hipMemcpyAsync(c->hpstate, c->pstate, sizeof(PersistState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    bool grow_front = persistent ? (c->hpstate->overflow != 0) : ((c->hcnt->overflow & 1u) != 0);
+    bool grow_front = ((c->hcnt->overflow & 1u) != 0) || (persistent && c->hpstate->overflow != 0);
     if (persistent && !grow_front && npairs && c->hpstate->count[rounds] > 0) {
       // spill left after the queued rounds: drain it, then redo the narrow phase
       while (c->hpstate->count[rounds] > 0 && !c->hpstate->overflow) {
@@ -734,10 +778,10 @@ int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint3
   if (!c || !n) return KDOP_ERR_ARG;
   if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "unknown body %u", body);
   const BodyHost& b = c->bodies[body];
-  *n = b.nnodes - 1;
+  *n = b.real_nodes;
   if (!vol && !range && !kind && !root) return KDOP_OK;
-  if (cap < *n) return fail(c, KDOP_ERR_CAPACITY, "kdop_export_tree: need %u nodes", b.nnodes - 1);
-  const int K = c->K, S = K + 2;
+  if (cap < *n) return fail(c, KDOP_ERR_CAPACITY, "kdop_export_tree: need %llu nodes", (unsigned long long)b.real_nodes);
+  const int K = c->K, S = rec_stride(K);
   std::vector<float> rec((size_t)b.nnodes * S);
   HIPCHK(c, hipMemcpyAsync(rec.data(), c->nodes.p + (size_t)b.node_base * S, rec.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -746,22 +790,36 @@ int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint3
     std::memcpy(&u, &rec[(size_t)local * S + K + w], 4);
     return u;
   };
-  // tet ranges: children always have larger local indices within a level, and
-  // lower levels follow upper ones, so a reverse sweep sees children first.
+  // tet ranges: children always sit in later slots (within a chunk in BFS
+  // order, across chunks on lower levels), so a reverse sweep sees them first
   std::vector<uint32_t> lo(b.nnodes, 0), hi(b.nnodes, 0);
   for (int64_t i = (int64_t)b.nnodes - 1; i >= 1; --i) {
     const uint32_t m0 = meta((uint32_t)i, 0), m1 = meta((uint32_t)i, 1);
+    if (m0 == 0xFFFFFFFFu) continue;  // free slot
     if (m0 & F_LEAF) { lo[i] = hi[i] = m0 & IDX_MASK; continue; }
     const uint32_t c0 = (m0 & IDX_MASK) - b.node_base, c1 = (m1 & IDX_MASK) - b.node_base;
     lo[i] = lo[c0]; hi[i] = hi[c1];
   }
+  uint64_t k_out = 0;
   for (uint32_t i = 1; i < b.nnodes; ++i) {
     const uint32_t m0 = meta(i, 0);
-    if (vol) for (int k = 0; k < K; ++k) vol[(size_t)(i - 1) * K + k] = rec[(size_t)i * S + k];
-    if (range) { range[2 * (i - 1)] = lo[i]; range[2 * (i - 1) + 1] = hi[i]; }
-    if (kind) kind[i - 1] = (m0 & F_LEAF) ? 2 : ((m0 & F_BRANCH) ? 1 : 0);
+    if (m0 == 0xFFFFFFFFu) continue;
+    if (vol) for (int k = 0; k < K; ++k) vol[(size_t)k_out * K + k] = rec[(size_t)i * S + k];
+    if (range) { range[2 * k_out] = lo[i]; range[2 * k_out + 1] = hi[i]; }
+    if (kind) kind[k_out] = (m0 & F_LEAF) ? 2 : ((m0 & F_BRANCH) ? 1 : 0);
+    ++k_out;
   }
   if (root) for (int k = 0; k < K; ++k) root[k] = rec[k];
+  return KDOP_OK;
+}
+
+int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
+  if (!c || !out) return KDOP_ERR_ARG;
+  std::memset(out, 0, 8 * sizeof(uint64_t));
+  if (!c->have_traversal || !c->last_persistent) return KDOP_OK;
+  const PersistState& S = *c->hpstate;
+  out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.iters;
+  out[4] = S.lanes; out[5] = S.max_iters; out[6] = S.grabs; out[7] = (uint64_t)c->num_cus * c->wpc;
   return KDOP_OK;
 }
 

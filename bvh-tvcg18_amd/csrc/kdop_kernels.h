@@ -48,42 +48,29 @@ struct NodeRec {
   uint32_t m0, m1;
 };
 
+// Load the used part of a record (K + 2 floats) with 16-byte loads; the
+// record stride is a multiple of 4 floats, so the last load stays inside it.
 template <int K>
 __device__ inline void load_rec(const float* __restrict__ nodes, uint32_t idx, NodeRec<K>& r) {
-  constexpr int S = K + 2;
-  const float* p = nodes + (size_t)idx * S;
-  if constexpr (S % 4 == 0) {
-    const float4* q = reinterpret_cast<const float4*>(p);
+  constexpr int S = rec_stride(K);
+  constexpr int NV = (K + 2 + 3) / 4;
+  const float4* q = reinterpret_cast<const float4*>(nodes + (size_t)idx * S);
 #pragma unroll
-    for (int i = 0; i < S / 4; ++i) {
-      const float4 x = q[i];
-      const float t[4] = {x.x, x.y, x.z, x.w};
+  for (int i = 0; i < NV; ++i) {
+    const float4 x = q[i];
+    const float t[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int f = 4 * i + j;
-        if (f < K) r.v[f] = t[j];
-        else if (f == K) r.m0 = __float_as_uint(t[j]);
-        else r.m1 = __float_as_uint(t[j]);
-      }
-    }
-  } else {
-    const float2* q = reinterpret_cast<const float2*>(p);
-#pragma unroll
-    for (int i = 0; i < S / 2; ++i) {
-      const float2 x = q[i];
-      const float t[2] = {x.x, x.y};
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int f = 2 * i + j;
-        if (f < K) r.v[f] = t[j];
-        else if (f == K) r.m0 = __float_as_uint(t[j]);
-        else r.m1 = __float_as_uint(t[j]);
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int f = 4 * i + j;
+      if (f < K) r.v[f] = t[j];
+      else if (f == K) r.m0 = __float_as_uint(t[j]);
+      else if (f == K + 1) r.m1 = __float_as_uint(t[j]);
     }
   }
 }
-__device__ inline uint2 load_meta(const float* __restrict__ nodes, uint32_t idx, int S) {
-  return *reinterpret_cast<const uint2*>(nodes + (size_t)idx * S + (S - 2));
+template <int K>
+__device__ inline uint2 load_meta(const float* __restrict__ nodes, uint32_t idx) {
+  return *reinterpret_cast<const uint2*>(nodes + (size_t)idx * rec_stride(K) + K);
 }
 
 // geometry_overlap_dop_dop.h:12-24 with the strict interval test of
@@ -103,19 +90,20 @@ __device__ inline bool dop_overlap(const float* A, const float* B) {
 __device__ inline uint32_t chunk_elems(const TopoParams& P, int h, uint32_t c) {
   return (c + 1 < P.C[h]) ? P.L : P.E[h] - c * P.L;
 }
-__device__ inline uint32_t chunk_stride(const TopoParams& P, int h) { return (h == (int)P.H - 1) ? 2 * P.L - 1 : P.L - 1; }
+__device__ inline uint32_t chunk_stride(const TopoParams& P, int h) { return (h == (int)P.H - 1) ? 2 * P.L : P.L; }
 // Root node of chunk (h, c) after skipping one-element super chunks.
 __device__ inline uint32_t resolved_root(const TopoParams& P, int h, uint32_t c) {
   for (;;) {
     const uint32_t n = chunk_elems(P, h, c);
-    if (h == (int)P.H - 1 || n >= 2) return P.node_base + P.level_base[h] + c * chunk_stride(P, h);
+    if (h == (int)P.H - 1 || n >= 2) return P.node_base + P.level_base[h] + c * chunk_stride(P, h) + 1;
     c = c * P.L;
     ++h;
   }
 }
 
-__global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int S, const uint8_t* __restrict__ surf,
+__global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int K, const uint8_t* __restrict__ surf,
                                  uint32_t total_chunks) {
+  const int S = rec_stride(K);
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total_chunks) return;
   int h = 0;
@@ -123,7 +111,7 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int S,
   while (h < (int)P.H && c >= P.C[h]) { c -= P.C[h]; ++h; }
   const bool branch = (h == (int)P.H - 1);
   const uint32_t n = chunk_elems(P, h, c);
-  const uint32_t base = P.node_base + P.level_base[h] + c * chunk_stride(P, h);
+  const uint32_t base = P.node_base + P.level_base[h] + c * chunk_stride(P, h) + 1;  // slot of BFS node 0
   const uint32_t first = c * P.L;  // first element (tet or child chunk) of the chunk
   if (!branch && n < 2) return;    // no nodes: parents point at the resolved child
   // BFS over element ranges; branch chunks store every range, super chunks
@@ -134,7 +122,7 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int S,
   while (head < tail) {
     const uint32_t lo = qlo[head], hi = qhi[head];
     const uint32_t node = base + head;
-    uint32_t* meta = reinterpret_cast<uint32_t*>(nodes + (size_t)node * S + (S - 2));
+    uint32_t* meta = reinterpret_cast<uint32_t*>(nodes + (size_t)node * S + K);
     const uint32_t rootflag = (branch && head == 0) ? F_ROOT : 0u;
     ++head;
     if (lo == hi) {  // branch leaf (super chunks never enqueue single ranges)
@@ -204,7 +192,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                      const uint32_t* __restrict__ cpre, uint32_t nbatch, float* __restrict__ nodes,
                                                      DirTable dt) {
-  constexpr int N = K / 2, S = K + 2;
+  constexpr int N = K / 2, S = rec_stride(K);
   __shared__ float vol[63][K];
   __shared__ uint32_t meta0[63], meta1[63];
   const uint32_t g = blockIdx.x;
@@ -215,10 +203,10 @@ __global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__
   const uint32_t L = B.L;
   const uint32_t n = (c + 1 < B.nbranch) ? L : B.nt - c * L;
   const uint32_t nn = 2 * n - 1;
-  const uint32_t base = B.branch_base + c * (2 * L - 1);
+  const uint32_t base = B.branch_base + c * (2 * L) + 1;  // slot of BFS node 0
   const int lane = threadIdx.x;
   for (uint32_t i = lane; i < nn; i += 64) {
-    const uint2 m = load_meta(nodes, base + i, S);
+    const uint2 m = load_meta<K>(nodes, base + i);
     meta0[i] = m.x; meta1[i] = m.y;
   }
   __syncthreads();
@@ -282,7 +270,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                     const uint32_t* __restrict__ cpre, const uint32_t* __restrict__ chunk_base,
                                                     const uint32_t* __restrict__ chunk_n, uint32_t nbatch, float* __restrict__ nodes) {
-  constexpr int S = K + 2;
+  constexpr int S = rec_stride(K);
   __shared__ float vol[32][K];
   __shared__ uint32_t meta0[32], meta1[32];
   const uint32_t g = blockIdx.x;
@@ -296,10 +284,10 @@ __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ 
   const uint32_t n = (c + 1 < ncnk) ? L : nlast;
   if (n < 2) return;
   const uint32_t nn = n - 1;
-  const uint32_t base = chunk_base[item] + c * (L - 1);
+  const uint32_t base = chunk_base[item] + c * L + 1;  // slot of BFS node 0
   const int lane = threadIdx.x;
   for (uint32_t i = lane; i < nn; i += 64) {
-    const uint2 m = load_meta(nodes, base + i, S);
+    const uint2 m = load_meta<K>(nodes, base + i);
     meta0[i] = m.x; meta1[i] = m.y;
   }
   __syncthreads();
@@ -324,7 +312,7 @@ __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ 
 template <int K>
 __global__ __launch_bounds__(64) void k_refit_root(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                    const uint32_t* __restrict__ tops, float* __restrict__ nodes) {
-  constexpr int S = K + 2;
+  constexpr int S = rec_stride(K);
   const BodyDev& B = bodies[batch_body[blockIdx.x]];
   const int lane = threadIdx.x;
   if (lane >= K) return;
@@ -351,47 +339,58 @@ struct TravArgs {
   uint32_t npairs;
 };
 
-// Result of expanding one work item: the child node ids and three 4-bit masks
-// over the child combinations k = 2*i + j (child i of a, child j of b).
+// Which sides of an overlapping node pair (x, y) expand: the reference's rule
+// inside branch chunks (kdop_tandem_traversal.h:122-154: both internal -> 4
+// pairs, one leaf -> the other side's 2 children), with branch roots treated
+// as leaves while the other side is still above the branch level
+// (super-chunk levels, which the reference does not traverse).
+__device__ inline void expand_rule(uint32_t mx0, uint32_t my0, bool& ex, bool& ey) {
+  const bool xb = mx0 & F_BRANCH, yb = my0 & F_BRANCH;
+  if (!xb && !yb) { ex = true; ey = true; }
+  else if (!xb) { ex = true; ey = false; }
+  else if (!yb) { ex = false; ey = true; }
+  else { ex = !(mx0 & F_LEAF); ey = !(my0 & F_LEAF); }
+}
+// Work item for the overlapping pair (x at slot xi, y at slot yi).
+template <int K>
+__device__ inline Item make_item(const NodeRec<K>& X, uint32_t xi, const NodeRec<K>& Y, uint32_t yi, uint32_t pair) {
+  bool ex, ey;
+  expand_rule(X.m0, Y.m0, ex, ey);
+  Item it;
+  it.a0 = ex ? (I_EXP | (X.m0 & IDX_MASK)) : xi;
+  it.a1 = ex ? (X.m1 & IDX_MASK) : 0u;
+  it.b0 = ey ? (I_EXP | (Y.m0 & IDX_MASK)) : yi;
+  it.b1 = ey ? (Y.m1 & IDX_MASK) : 0u;
+  it.pair = pair;
+  return it;
+}
+
+// Result of expanding one work item: new items and three 4-bit masks over the
+// child combinations k = 2 i + j (child i of side a, child j of side b).
 struct Expansion {
-  uint32_t ca[2], cb[2];  // child node ids
+  Item kids[4];           // kids[k] valid where bit k of push is set
   uint32_t ta[2], tb[2];  // tet ids when the child is a tet leaf
-  uint32_t body_a, body_b;  // bodies of the tet leaves (from the leaf records)
-  uint32_t push;          // overlapping combinations to push as new items
+  uint32_t push;          // overlapping combinations that become new items
   uint32_t leaf;          // overlapping tet-leaf pairs (P1)
   uint32_t surf;          // subset of leaf where both tets have a surface face (P2)
   uint32_t tests, descent, broot;
 };
 
-// Test the child pairs of work item (a, b), following the reference's
-// expansion rule inside branch chunks (kdop_tandem_traversal.h:122-154) and
-// treating branch roots as leaves while either side is still above the
-// branch level (super-chunk levels, which the reference does not traverse).
+// Test the child pairs of one item (every test is one node-pair test of the
+// BVTT; descent counts those below branch-root pairs, broot the overlapping
+// branch-root pairs).
 template <int K>
-__device__ inline void expand_item(const float* __restrict__ nodes, uint32_t a, uint32_t b, Expansion& x) {
-  constexpr int S = K + 2;
-  const uint2 ma = load_meta(nodes, a, S);
-  const uint2 mb = load_meta(nodes, b, S);
-  const bool a_br = ma.x & F_BRANCH, b_br = mb.x & F_BRANCH;
-  bool exp_a, exp_b;
-  if (!a_br && !b_br) { exp_a = true; exp_b = true; }
-  else if (!a_br) { exp_a = true; exp_b = false; }
-  else if (!b_br) { exp_a = false; exp_b = true; }
-  else { exp_a = !(ma.x & F_LEAF); exp_b = !(mb.x & F_LEAF); }
-  x.ca[0] = exp_a ? (ma.x & IDX_MASK) : a;
-  x.ca[1] = ma.y & IDX_MASK;
-  x.cb[0] = exp_b ? (mb.x & IDX_MASK) : b;
-  x.cb[1] = mb.y & IDX_MASK;
+__device__ inline void expand_item(const float* __restrict__ nodes, const Item& it, Expansion& x) {
+  const bool exp_a = it.a0 & I_EXP, exp_b = it.b0 & I_EXP;
+  const uint32_t ca0 = it.a0 & IDX_MASK, ca1 = it.a1, cb0 = it.b0 & IDX_MASK, cb1 = it.b1;
   NodeRec<K> A0, A1, B0, B1;
   A1.m0 = A1.m1 = B1.m0 = B1.m1 = 0u;
-  load_rec<K>(nodes, x.ca[0], A0);
-  if (exp_a) load_rec<K>(nodes, x.ca[1], A1);
-  load_rec<K>(nodes, x.cb[0], B0);
-  if (exp_b) load_rec<K>(nodes, x.cb[1], B1);
+  load_rec<K>(nodes, ca0, A0);
+  if (exp_a) load_rec<K>(nodes, ca1, A1);
+  load_rec<K>(nodes, cb0, B0);
+  if (exp_b) load_rec<K>(nodes, cb1, B1);
   x.ta[0] = A0.m0 & IDX_MASK; x.ta[1] = A1.m0 & IDX_MASK;
   x.tb[0] = B0.m0 & IDX_MASK; x.tb[1] = B1.m0 & IDX_MASK;
-  x.body_a = ((A0.m0 & F_LEAF) ? A0.m1 : A1.m1) & IDX_MASK;
-  x.body_b = ((B0.m0 & F_LEAF) ? B0.m1 : B1.m1) & IDX_MASK;
   x.push = 0; x.leaf = 0; x.surf = 0;
   x.tests = 0; x.descent = 0; x.broot = 0;
 #pragma unroll
@@ -411,6 +410,7 @@ __device__ inline void expand_item(const float* __restrict__ nodes, uint32_t a, 
       if ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) x.surf |= 1u << k;
     } else {
       x.push |= 1u << k;
+      x.kids[k] = make_item<K>(X, i ? ca1 : ca0, Y, j ? cb1 : cb0, it.pair);
     }
   }
 }
@@ -436,65 +436,92 @@ __device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expan
     }
 }
 
-// Seed: one 64-lane block per body pair.  Root-vs-root test
-// (kdop_tandem_traversal.h:164-165), then every pair of top-level chunk roots.
+// Write this lane's new items at slot o of `out` (one wave-wide reservation).
+__device__ inline void emit_items(const Expansion& x, Item* __restrict__ out, uint32_t o, uint32_t cap, unsigned int* overflow) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if ((x.push >> k) & 1u) {
+      const uint32_t slot = o + __popc(x.push & ((1u << k) - 1u));
+      if (slot < cap) out[slot] = x.kids[k];
+      else atomicOr(overflow, 1u);
+    }
+}
+
+// Seed descriptor of one body pair: every chunk root of level hA of body A
+// (tops[a_off .. a_off+na)) against every chunk root of level hB of body B.
+struct SeedDesc {
+  uint32_t a_off, na, b_off, nb, base;  // base: first seed index of the pair
+};
+
+// Seed: one thread per (chunk root of A, chunk root of B) pair of the chosen
+// levels, for every body pair whose root volumes overlap (the root-vs-root
+// test is kdop_tandem_traversal.h:164-165).  Seeding below the top level
+// only skips tests among super-chunk nodes, which the reference never
+// makes; the pairs found and every test at or below the branch roots are
+// unchanged.
 template <int K>
-__global__ __launch_bounds__(64) void k_seed(TravArgs T, Item* __restrict__ out, uint32_t* __restrict__ out_fill, uint32_t out_cap) {
-  constexpr int S = K + 2;
-  const uint32_t pair = blockIdx.x;
-  const uint2 bp = T.pairs[pair];
-  const BodyDev& BA = T.bodies[bp.x];
-  const BodyDev& BB = T.bodies[bp.y];
-  NodeRec<K> ra, rb;
-  load_rec<K>(T.nodes, BA.node_base, ra);
-  load_rec<K>(T.nodes, BB.node_base, rb);
-  const bool ov = dop_overlap<K>(ra.v, rb.v);
-  if (lane_id() == 0) {
-    atomicAdd(&T.cnt->root_tests, 1ull);
-    atomicAdd(&T.cnt->tests, 1ull);
-    if (ov) atomicAdd(&T.cnt->branch_pairs, (unsigned long long)BA.nbranch * BB.nbranch);
-  }
-  if (!ov) return;
-  const uint32_t combos = BA.ntop * BB.ntop;
-  unsigned long long tests = 0, broot = 0, p1s = 0;
-  for (uint32_t base = 0; base < combos; base += 64) {
-    const uint32_t w = base + lane_id();
+__global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __restrict__ sd, uint32_t total, Item* __restrict__ out,
+                                             uint32_t* __restrict__ out_fill, uint32_t out_cap) {
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  unsigned long long tests = 0, broot = 0, p1s = 0, roots = 0, bpairs = 0;
+  for (uint32_t base = wave * 64; base < total; base += waves * 64) {
+    const uint32_t g = base + lane_id();
     Expansion x;
     x.push = x.leaf = x.surf = 0;
-    x.ca[0] = x.cb[0] = 0;
-    if (w < combos) {
-      const uint32_t i = w / BB.ntop, j = w - i * BB.ntop;
-      x.ca[0] = T.tops[BA.top_base + i];
-      x.cb[0] = T.tops[BB.top_base + j];
-      NodeRec<K> X, Y;
-      load_rec<K>(T.nodes, x.ca[0], X);
-      load_rec<K>(T.nodes, x.cb[0], Y);
-      x.ta[0] = X.m0 & IDX_MASK;
-      x.tb[0] = Y.m0 & IDX_MASK;
-      ++tests;
-      const bool both_roots = (X.m0 & F_BRANCH) && (Y.m0 & F_BRANCH) && (X.m1 & F_ROOT) && (Y.m1 & F_ROOT);
-      if (dop_overlap<K>(X.v, Y.v)) {
-        if (both_roots) ++broot;
-        if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
-          x.leaf = 1u;
-          x.surf = ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) ? 1u : 0u;
-        } else {
-          x.push = 1u;
+    uint32_t pair = 0;
+    if (g < total) {
+      uint32_t lo = 0, hi = T.npairs;
+      while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (sd[m].base <= g) lo = m; else hi = m;
+      }
+      pair = lo;
+      const SeedDesc d = sd[pair];
+      const uint32_t local = g - d.base;
+      const uint2 bp = T.pairs[pair];
+      const BodyDev& BA = T.bodies[bp.x];
+      const BodyDev& BB = T.bodies[bp.y];
+      NodeRec<K> ra, rb;
+      load_rec<K>(T.nodes, BA.node_base, ra);
+      load_rec<K>(T.nodes, BB.node_base, rb);
+      const bool ov = dop_overlap<K>(ra.v, rb.v);
+      if (local == 0) {
+        ++roots;
+        if (ov) bpairs += (unsigned long long)BA.nbranch * BB.nbranch;
+      }
+      if (ov) {
+        const uint32_t i = local / d.nb, j = local - i * d.nb;
+        const uint32_t xa = T.tops[d.a_off + i], yb = T.tops[d.b_off + j];
+        NodeRec<K> X, Y;
+        load_rec<K>(T.nodes, xa, X);
+        load_rec<K>(T.nodes, yb, Y);
+        x.ta[0] = X.m0 & IDX_MASK;
+        x.tb[0] = Y.m0 & IDX_MASK;
+        ++tests;
+        const bool both_roots = (X.m0 & F_BRANCH) && (Y.m0 & F_BRANCH) && (X.m1 & F_ROOT) && (Y.m1 & F_ROOT);
+        if (dop_overlap<K>(X.v, Y.v)) {
+          if (both_roots) ++broot;
+          if ((X.m0 & F_LEAF) && (Y.m0 & F_LEAF)) {
+            x.leaf = 1u;
+            x.surf = ((X.m1 & F_SURF) && (Y.m1 & F_SURF)) ? 1u : 0u;
+          } else {
+            x.push = 1u;
+            x.kids[0] = make_item<K>(X, xa, Y, yb, pair);
+          }
         }
       }
     }
     const uint32_t o = wave_reserve(out_fill, x.push);
-    if (x.push) {
-      if (o < out_cap) out[o] = Item{x.ca[0], x.cb[0], pair};
-      else atomicOr(&T.cnt->overflow, 1u);
-    }
+    emit_items(x, out, o, out_cap, &T.cnt->overflow);
     p1s += x.leaf;
     emit_leaves(T, pair, x);
   }
-  wave_add64(&T.cnt->tests, tests);
+  wave_add64(&T.cnt->root_tests, roots);
+  wave_add64(&T.cnt->tests, tests + roots);
+  wave_add64(&T.cnt->branch_pairs, bpairs);
   wave_add64(&T.cnt->branch_overlaps, broot);
   wave_add64(&T.cnt->p1, p1s);
-  (void)S;
 }
 
 // One level of the level-synchronous traversal: expand every item of `in`.
@@ -510,20 +537,14 @@ __global__ __launch_bounds__(256) void k_expand_level(TravArgs T, const Item* __
     Expansion x;
     x.push = x.leaf = x.surf = 0;
     x.tests = x.descent = x.broot = 0;
-    Item it{0, 0, 0};
+    Item it{0, 0, 0, 0, 0};
     if (i < n) {
       it = in[i];
-      expand_item<K>(T.nodes, it.a, it.b, x);
+      expand_item<K>(T.nodes, it, x);
     }
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
     const uint32_t o = wave_reserve(out_fill, __popc(x.push));
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if ((x.push >> k) & 1u) {
-        const uint32_t slot = o + __popc(x.push & ((1u << k) - 1u));
-        if (slot < cap) out[slot] = Item{x.ca[k >> 1], x.cb[k & 1], it.pair};
-        else atomicOr(&T.cnt->overflow, 1u);
-      }
+    emit_items(x, out, o, cap, &T.cnt->overflow);
     emit_leaves(T, it.pair, x);
   }
   wave_add64(&T.cnt->tests, tests);

@@ -132,7 +132,8 @@ def rng_quaternions(seed, stream, n):
 class Context:
     """One HIP stream + device-resident bodies and trees (kdop_create)."""
 
-    def __init__(self, K=8, chunk_bytes=8000, device=0, traversal=KDOP_TRAVERSAL_AUTO, queue_depth=0, waves_per_cu=0):
+    def __init__(self, K=8, chunk_bytes=8000, device=0, traversal=KDOP_TRAVERSAL_AUTO, queue_depth=0, waves_per_cu=0,
+                 ramp_target=0, keep=0, seed_budget=0):
         if K not in SUPPORTED_K:
             raise ValueError(f"K={K} not in {SUPPORTED_K}")
         self._lib = lib()
@@ -144,6 +145,9 @@ class Context:
         cfg.traversal = traversal
         cfg.queue_depth = queue_depth
         cfg.waves_per_cu = waves_per_cu
+        cfg.ramp_target = ramp_target
+        cfg.stack_keep = keep
+        cfg.seed_budget = seed_budget
         h = ctypes.c_void_p()
         rc = self._lib.kdop_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc:
@@ -249,6 +253,12 @@ class Context:
         self._check(self._lib.kdop_export_tree(self._h, body, n.value, ptr(vol), ptr(rng), ptr(kind), ptr(root),
                                                ctypes.byref(n)))
         return dict(vol=vol, range=rng, kind=kind, root=root)
+
+    def traversal_stats(self):
+        out = np.zeros(8, np.uint64)
+        self._check(self._lib.kdop_traversal_stats(self._h, ptr(out)))
+        keys = ("ramp_levels", "frontier", "rounds", "iters", "lanes", "max_iters", "grabs", "waves")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def last_timing(self):
         out = np.zeros(4, np.float64)

@@ -30,7 +30,10 @@ class Config(ctypes.Structure):
         ("traversal", ctypes.c_int32),
         ("queue_depth", ctypes.c_int32),
         ("waves_per_cu", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("ramp_target", ctypes.c_int32),
+        ("stack_keep", ctypes.c_int32),
+        ("seed_budget", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -80,6 +83,7 @@ SIGNATURES = [
     ("kdop_export_tree", ctypes.c_int, [_p, _u32, _u64, _p, _p, _p, _p, _p]),
     ("kdop_narrow_adjoint", ctypes.c_int, [_p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_last_timing", ctypes.c_int, [_p, _p]),
+    ("kdop_traversal_stats", ctypes.c_int, [_p, _p]),
     ("kdop_stream", _p, [_p]),
     ("kdop_box_mesh_size", None, [_u32, _p, _p]),
     ("kdop_make_box_mesh", ctypes.c_int, [_u32, ctypes.c_float, ctypes.c_double, _u64, _p, _p, _p, _p]),

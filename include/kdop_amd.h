@@ -42,7 +42,11 @@ typedef struct kdop_config {
   int32_t traversal;     /* KDOP_TRAVERSAL_*                                                    */
   int32_t queue_depth;   /* persistent kernel: LDS stack entries per wave (0 = autotune)       */
   int32_t waves_per_cu;  /* persistent kernel: resident waves per CU (0 = autotune)            */
-  int32_t reserved[5];
+  int32_t ramp_target;   /* persistent kernel: frontier size that ends the breadth-first ramp   */
+  int32_t stack_keep;    /* persistent kernel: stack entries a wave keeps once the frontier is  */
+                         /*   exhausted; deeper stacks go to the next round (0 = default 128)  */
+  int32_t seed_budget;   /* chunk-root pairs tested to seed one call (0 = default 65536)        */
+  int32_t reserved[2];
 } kdop_config;
 
 /* Counters of one traversal (+ narrow phase) call. */
@@ -157,6 +161,13 @@ int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies,
  * out[1] narrow phase, out[2] whole call.  out[3] = average duration of one
  * launch of the dominant traversal kernel. */
 int kdop_last_timing(kdop_ctx* ctx, double* out);
+
+/* Scheduling statistics of the last traversal (persistent strategy; zeros for
+ * the level-synchronous one): out[0] ramp levels, out[1] frontier handed to the
+ * persistent rounds, out[2] rounds run, out[3] wave loop iterations (sum),
+ * out[4] items expanded (sum of active lanes), out[5] most iterations of one
+ * wave, out[6] chunks taken, out[7] persistent waves. */
+int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 
 /* Stream of the context (hipStream_t), for callers that order their own work. */
 void* kdop_stream(kdop_ctx* ctx);
