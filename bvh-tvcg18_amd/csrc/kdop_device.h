@@ -90,20 +90,38 @@ struct NarrowRec {
   float n[3];
 };
 
-// Device counters of one traversal call.
-struct DevCounters {
-  unsigned long long tests;
-  unsigned long long descent;
+// Device counters of one call, striped: a kernel adds its per-wave sums to
+// stripe (global wave index mod kStripes).  Agent-scope atomics on one word
+// serialize across the eight XCDs (they are performed beyond the per-XCD
+// L2), so thousands of waves adding to one counter cost more than the
+// traversal itself; 64 stripes on separate 256-byte lines remove that hot
+// spot.  k_sum_counters folds the stripes into `total`.
+constexpr int kStripes = 64;
+struct alignas(256) CounterStripe {
+  unsigned long long tests;          // node-pair tests executed
+  unsigned long long descent;        // tests below branch-root pairs
   unsigned long long branch_overlaps;
-  unsigned long long p1;
-  unsigned long long p2;
-  unsigned long long contacts;
-  unsigned long long narrow_pairs;
+  unsigned long long p1;             // overlapping tet pairs
+  unsigned long long narrow_pairs;   // surface pairs with contacts
   unsigned long long branch_pairs;   // C_A*C_B over body pairs whose roots overlap
   unsigned long long root_tests;
-  unsigned int overflow;             // bit 0 frontier, bit 1 P1, bit 2 P2, bit 3 contacts
-  unsigned int pad;
-  unsigned int level_count[64];      // frontier sizes per level (level-synchronous mode)
+  unsigned long long iters;          // persistent: wave loop iterations
+  unsigned long long lanes;          // persistent: items expanded
+  unsigned long long grabs;          // persistent: frontier chunks taken
+  unsigned long long max_iters;      // persistent: most iterations of one wave (max, not sum)
+};
+constexpr int kStripeFields = 11;
+
+// Segments of the persistent traversal's frontier: each round's input is
+// split into kSegs segments with their own chunk counters (one lane of a
+// wave per segment).
+constexpr int kSegs = 64;
+
+__device__ inline CounterStripe* stripe_of(CounterStripe* s, uint32_t wave) { return s + (wave & (kStripes - 1)); }
+
+// Output fills of one call (single words: their waves reserve whole batches).
+struct Fills {
+  uint32_t p1, p2, c, r;
 };
 
 }  // namespace kdop_amd

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -63,19 +64,16 @@ struct kdop_ctx {
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
   DevBuf<NarrowRec> recs;
-  DevCounters* dcnt = nullptr;
-  DevCounters* hcnt = nullptr;  // pinned
+  CallState* dstate = nullptr;    // counters and frontier segments of one call
+  CallSummary* hsum = nullptr;    // pinned copy of dstate->sum
   uint32_t last_npairs = 0;
   bool have_traversal = false, have_p1 = false, have_narrow = false;
   std::vector<uint2> last_pairs;
   // refit scratch
   DevBuf<uint32_t> plan;
   DevBuf<float> poses;
-  // persistent traversal: ramp / round state (device + pinned host copy)
-  PersistState* pstate = nullptr;
-  PersistState* hpstate = nullptr;
   int num_cus = 0;
-  int ls = 512, wpc = 4;
+  int ls = 512, wpc = 8;
   uint32_t ramp_target = 0;  // 0: 2 x persistent waves
   uint32_t keep = 128;       // items a wave keeps once the frontier is gone
   uint32_t last_rounds = 0;
@@ -307,22 +305,17 @@ struct Seed {
 };
 template <int K>
 struct Expand {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t level) {
-    DevCounters* d = c->dcnt;
-    const uint32_t cap = (uint32_t)std::min(c->front[0].cap, c->front[1].cap);
-    hipLaunchKernelGGL(k_expand_level<K>, dim3(2048), dim3(256), 0, c->stream, T, c->front[level & 1].p, &d->level_count[level],
-                       c->front[(level + 1) & 1].p, &d->level_count[level + 1], cap);
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t level, uint32_t cap) {
+    CallState* d = c->dstate;
+    hipLaunchKernelGGL(k_expand_level<K>, dim3(2048), dim3(256), 0, c->stream, T, c->front[level & 1].p, &d->level_fill[level],
+                       c->front[(level + 1) & 1].p, &d->level_fill[level + 1], cap);
   }
-};
-
-struct Fills {
-  uint32_t p1, p2, c, r;
 };
 
 template <int K>
 struct Ramp {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t target, uint32_t max_levels) {
-    hipLaunchKernelGGL(k_ramp<K>, dim3(1), dim3(512), 0, c->stream, T, c->front[0].p, c->front[1].p, cap, c->pstate, target, max_levels);
+    hipLaunchKernelGGL(k_ramp<K>, dim3(1), dim3(512), 0, c->stream, T, c->front[0].p, c->front[1].p, cap, c->dstate, target, max_levels);
   }
 };
 template <int K>
@@ -332,10 +325,10 @@ struct PersistRound {
     Item* b0 = c->front[0].p;
     Item* b1 = c->front[1].p;
     switch (c->ls) {
-      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
-      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
-      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
-      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->pstate, round, c->keep); break;
+      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
+      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
+      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
+      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
     }
   }
 };
@@ -374,10 +367,9 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
   c->own_stream = true;
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
-  if (hipMalloc(&c->dcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipHostMalloc(&c->hcnt, sizeof(DevCounters) + sizeof(Fills)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipMalloc(&c->pstate, sizeof(PersistState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipHostMalloc(&c->hpstate, sizeof(PersistState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipMalloc(&c->dstate, sizeof(CallState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipHostMalloc(&c->hsum, sizeof(CallSummary)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  std::memset(c->hsum, 0, sizeof(CallSummary));
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
@@ -404,10 +396,8 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
-  if (c->pstate) (void)hipFree(c->pstate);
-  if (c->hpstate) (void)hipHostFree(c->hpstate);
-  if (c->dcnt) (void)hipFree(c->dcnt);
-  if (c->hcnt) (void)hipHostFree(c->hcnt);
+  if (c->dstate) (void)hipFree(c->dstate);
+  if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -550,13 +540,28 @@ int kdop_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos
 
 namespace {
 
-NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, Fills* dfill) {
+// Narrow-phase arguments; with pending_round >= 0 the kernel runs only when
+// the persistent traversal left nothing in that round's segments.
+NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round) {
+  CallState* d = c->dstate;
   NarrowArgs A;
-  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &dfill->p2; A.p2_cap = T.p2_cap;
-  A.contacts = c->contacts.p; A.c_fill = &dfill->c; A.c_cap = (uint32_t)std::min<size_t>(c->contacts.cap, 0xFFFFFFFFu);
-  A.recs = c->recs.p; A.r_fill = &dfill->r; A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
-  A.cnt = c->dcnt;
+  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &d->fills.p2; A.p2_cap = T.p2_cap;
+  A.contacts = c->contacts.p; A.c_fill = &d->fills.c; A.c_cap = (uint32_t)std::min<size_t>(c->contacts.cap, 0xFFFFFFFFu);
+  A.recs = c->recs.p; A.r_fill = &d->fills.r; A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
+  A.stripes = d->stripe;
+  A.overflow = &d->overflow;
+  A.pending = pending_round >= 0 ? &d->seg[pending_round][0].fill : nullptr;
+  A.pending_stride = sizeof(SegCtr) / sizeof(uint32_t);
   return A;
+}
+
+// Fold the call's counters on the device and copy the summary to the host.
+int summarize(kdop_ctx* c, uint32_t cap) {
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(64), 0, c->stream, c->dstate, cap);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->hsum, &c->dstate->sum, sizeof(CallSummary), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
 }
 
 }  // namespace
@@ -621,14 +626,15 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
   const uint32_t target = c->ramp_target ? c->ramp_target : 2 * pwaves;
   const uint32_t rounds_async = 3;  // persistent rounds queued before the first host sync
-  Fills* dfill = reinterpret_cast<Fills*>(c->dcnt + 1);
   uint64_t overflowed = 0;
+  CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {
-    HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(DevCounters) + sizeof(Fills), c->stream));
+    HIPCHK(c, hipMemsetAsync(d, 0, offsetof(CallState, sum), c->stream));
     TravArgs T;
-    T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p; T.cnt = c->dcnt;
+    T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p;
+    T.stripes = d->stripe; T.overflow = &d->overflow;
     T.p1 = keep_p1 ? c->p1.p : nullptr; T.p2 = c->p2.p;
-    T.p1_fill = &dfill->p1; T.p2_fill = &dfill->p2;
+    T.p1_fill = &d->fills.p1; T.p2_fill = &d->fills.p2;
     T.p1_cap = keep_p1 ? (uint32_t)std::min<size_t>(c->p1.cap, 0xFFFFFFFFu) : 0u;
     T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
     T.npairs = npairs;
@@ -636,16 +642,13 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     uint32_t rounds = 0;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
+      dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
+      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
       if (persistent) {
-        HIPCHK(c, hipMemsetAsync(c->pstate, 0, sizeof(PersistState), c->stream));
-        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &c->pstate->level_fill[0], cap);
         dispatch_K<Ramp>(c->K, c, T, cap, target, levels + 1);
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
-        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &c->dcnt->level_count[0], cap);
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l);
+        for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l, cap);
       }
     } else {
       HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
@@ -653,44 +656,39 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (narrow && npairs) {
-      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, dfill));
+      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, persistent ? (int)rounds : -1));
       HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
-    if (persistent) HIPCHK(c, hipMemcpyAsync(c->hpstate, c->pstate, sizeof(PersistState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    bool grow_front = ((c->hcnt->overflow & 1u) != 0) || (persistent && c->hpstate->overflow != 0);
-    if (persistent && !grow_front && npairs && c->hpstate->count[rounds] > 0) {
-      // spill left after the queued rounds: drain it, then redo the narrow phase
-      while (c->hpstate->count[rounds] > 0 && !c->hpstate->overflow) {
+    if ((rc = summarize(c, cap))) return rc;
+    bool grow_front = (c->hsum->overflow & 1u) != 0;
+    if (persistent && !grow_front && npairs && c->hsum->round_total[rounds] > 0) {
+      // spill left after the queued rounds (the narrow phase above saw it and
+      // did nothing): drain it, then run the narrow phase
+      while (c->hsum->round_total[rounds] > 0 && !(c->hsum->overflow & 1u)) {
         if (rounds + 1 >= (uint32_t)kMaxRounds) return fail(c, KDOP_ERR_CAPACITY, "kdop_traverse: spill rounds exhausted");
         dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
         ++rounds;
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(c->hpstate, c->pstate, sizeof(PersistState), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((rc = summarize(c, cap))) return rc;
       }
-      grow_front = c->hpstate->overflow != 0;
+      grow_front = (c->hsum->overflow & 1u) != 0;
       HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
       if (narrow && !grow_front) {
-        HIPCHK(c, hipMemsetAsync(&dfill->c, 0, 2 * sizeof(uint32_t), c->stream));
-        HIPCHK(c, hipMemsetAsync(&c->dcnt->narrow_pairs, 0, sizeof(unsigned long long), c->stream));
-        hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, dfill));
+        hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, (int)rounds));
         HIPCHK(c, hipGetLastError());
       }
       HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters) + sizeof(Fills), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if ((rc = summarize(c, cap))) return rc;
     }
-    const DevCounters& h = *c->hcnt;
-    const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+    const CallSummary& h = *c->hsum;
+    const Fills& f = h.fills;
     bool again = false;
     if (grow_front) {
-      uint32_t mx = 0;
-      for (int l = 0; l < 64; ++l) mx = std::max(mx, persistent ? c->hpstate->level_fill[l] : h.level_count[l]);
-      if (persistent) for (int r = 0; r < kMaxRounds; ++r) mx = std::max(mx, c->hpstate->count[r]);
-      const size_t want = std::max<size_t>(4 * c->front[0].cap, (size_t)mx * 2);
+      uint64_t mx = 0;
+      for (int l = 0; l < 64; ++l) mx = std::max<uint64_t>(mx, h.level_fill[l]);
+      for (int r = 1; r < kMaxRounds; ++r) mx = std::max<uint64_t>(mx, (uint64_t)h.round_maxseg[r] * kSegs);
+      const size_t want = std::max<size_t>(4 * (size_t)cap, (size_t)mx * 2);
       if ((rc = ensure(c, c->front[0], want)) || (rc = ensure(c, c->front[1], want))) return rc;
       again = true;
     } else {
@@ -716,16 +714,17 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->have_p1 = keep_p1;
     c->have_narrow = narrow;
     if (out) {
+      const unsigned long long* t = h.total;  // CounterStripe field order
       std::memset(out, 0, sizeof(*out));
-      out->tests = h.tests;
-      out->descent_tests = h.descent;
-      out->branch_overlaps = h.branch_overlaps;
-      out->leaf_pairs = h.p1;
+      out->tests = t[0];
+      out->descent_tests = t[1];
+      out->branch_overlaps = t[2];
+      out->leaf_pairs = t[3];
       out->surface_pairs = f.p2;
       out->contacts = narrow ? f.c : 0;
-      out->narrow_pairs = narrow ? h.narrow_pairs : 0;
-      out->branch_pairs = h.branch_pairs;
-      out->reference_tests = h.root_tests + h.branch_pairs + h.descent;
+      out->narrow_pairs = narrow ? t[4] : 0;
+      out->branch_pairs = t[5];
+      out->reference_tests = t[6] + t[5] + t[1];
       out->overflow = overflowed;
     }
     return KDOP_OK;
@@ -737,7 +736,7 @@ int kdop_get_pairs(kdop_ctx* c, int which, uint32_t* dst, uint64_t cap, int dst_
   if (!c || !n) return KDOP_ERR_ARG;
   if (!c->have_traversal) return fail(c, KDOP_ERR_STATE, "kdop_get_pairs: no traversal yet");
   if (which == 0 && !c->have_p1) return fail(c, KDOP_ERR_STATE, "kdop_get_pairs: P1 not kept (pass KDOP_KEEP_ALL_PAIRS)");
-  const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+  const Fills& f = c->hsum->fills;
   const uint64_t cnt = which == 0 ? f.p1 : f.p2;
   *n = cnt;
   if (!dst) return KDOP_OK;
@@ -752,7 +751,7 @@ int kdop_get_pairs(kdop_ctx* c, int which, uint32_t* dst, uint64_t cap, int dst_
 int kdop_get_contacts(kdop_ctx* c, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n) {
   if (!c || !n) return KDOP_ERR_ARG;
   if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_get_contacts: last traversal ran without KDOP_NARROW");
-  const Fills& f = *reinterpret_cast<const Fills*>(c->hcnt + 1);
+  const Fills& f = c->hsum->fills;
   *n = f.c;
   if (!dst) return KDOP_OK;
   if (cap < f.c) return fail(c, KDOP_ERR_CAPACITY, "kdop_get_contacts: need %u contacts", f.c);
@@ -817,9 +816,9 @@ int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
   if (!c || !out) return KDOP_ERR_ARG;
   std::memset(out, 0, 8 * sizeof(uint64_t));
   if (!c->have_traversal || !c->last_persistent) return KDOP_OK;
-  const PersistState& S = *c->hpstate;
-  out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.iters;
-  out[4] = S.lanes; out[5] = S.max_iters; out[6] = S.grabs; out[7] = (uint64_t)c->num_cus * c->wpc;
+  const CallSummary& S = *c->hsum;
+  out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.total[7];
+  out[4] = S.total[8]; out[5] = S.total[10]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * c->wpc;
   return KDOP_OK;
 }
 

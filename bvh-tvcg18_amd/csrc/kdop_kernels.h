@@ -40,6 +40,31 @@ __device__ inline void wave_add64(unsigned long long* dst, unsigned long long v)
   const unsigned long long s = wave_sum64(v);
   if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
+// Global index of the calling wave.
+__device__ inline uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+// Reserve `cnt` slots per thread for the whole block with one atomic (every
+// thread of the block must call it; lds: nwaves + 1 words).  Returns this
+// thread's first slot.
+__device__ inline uint32_t block_reserve(uint32_t* ctr, uint32_t cnt, uint32_t* lds) {
+  const uint32_t incl = wave_incl_scan(cnt);
+  const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane_id() == 63) lds[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < nw; ++i) {
+      const uint32_t t = lds[i];
+      lds[i] = acc;
+      acc += t;
+    }
+    lds[nw] = acc ? atomicAdd(ctr, acc) : 0u;
+  }
+  __syncthreads();
+  const uint32_t r = lds[nw] + lds[w] + incl - cnt;
+  __syncthreads();
+  return r;
+}
 
 // ------------------------------------------------------------ node records
 template <int K>
@@ -330,7 +355,8 @@ struct TravArgs {
   const BodyDev* bodies;
   const uint2* pairs;        // body pair ids
   const uint32_t* tops;
-  DevCounters* cnt;
+  CounterStripe* stripes;    // kStripes striped counters
+  unsigned int* overflow;    // bit 0 frontier, 1 P1, 2 P2, 3 contacts
   LeafPair* p1;              // may be null (P1 not kept)
   LeafPair* p2;
   uint32_t* p1_fill;
@@ -425,6 +451,7 @@ __device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expan
       if ((x.leaf >> k) & 1u) {
         const uint32_t o = o1 + __popc(x.leaf & ((1u << k) - 1u));
         if (o < T.p1_cap) T.p1[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+        else atomicOr(T.overflow, 2u);
       }
   }
   const uint32_t o2 = wave_reserve(T.p2_fill, __popc(x.surf));
@@ -433,6 +460,7 @@ __device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expan
     if ((x.surf >> k) & 1u) {
       const uint32_t o = o2 + __popc(x.surf & ((1u << k) - 1u));
       if (o < T.p2_cap) T.p2[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+      else atomicOr(T.overflow, 4u);
     }
 }
 
@@ -462,11 +490,10 @@ struct SeedDesc {
 template <int K>
 __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __restrict__ sd, uint32_t total, Item* __restrict__ out,
                                              uint32_t* __restrict__ out_fill, uint32_t out_cap) {
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  __shared__ uint32_t red[5];
   unsigned long long tests = 0, broot = 0, p1s = 0, roots = 0, bpairs = 0;
-  for (uint32_t base = wave * 64; base < total; base += waves * 64) {
-    const uint32_t g = base + lane_id();
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t g = base + threadIdx.x;
     Expansion x;
     x.push = x.leaf = x.surf = 0;
     uint32_t pair = 0;
@@ -512,28 +539,28 @@ __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __rest
         }
       }
     }
-    const uint32_t o = wave_reserve(out_fill, x.push);
-    emit_items(x, out, o, out_cap, &T.cnt->overflow);
+    const uint32_t o = block_reserve(out_fill, x.push, red);
+    emit_items(x, out, o, out_cap, T.overflow);
     p1s += x.leaf;
     emit_leaves(T, pair, x);
   }
-  wave_add64(&T.cnt->root_tests, roots);
-  wave_add64(&T.cnt->tests, tests + roots);
-  wave_add64(&T.cnt->branch_pairs, bpairs);
-  wave_add64(&T.cnt->branch_overlaps, broot);
-  wave_add64(&T.cnt->p1, p1s);
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  wave_add64(&st->root_tests, roots);
+  wave_add64(&st->tests, tests + roots);
+  wave_add64(&st->branch_pairs, bpairs);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
 }
 
 // One level of the level-synchronous traversal: expand every item of `in`.
 template <int K>
 __global__ __launch_bounds__(256) void k_expand_level(TravArgs T, const Item* __restrict__ in, const uint32_t* __restrict__ in_fill,
                                                       Item* __restrict__ out, uint32_t* __restrict__ out_fill, uint32_t cap) {
+  __shared__ uint32_t red[5];
   const uint32_t n = min(*in_fill, cap);
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
-  for (uint32_t base = wave * 64; base < n; base += waves * 64) {
-    const uint32_t i = base + lane_id();
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
     Expansion x;
     x.push = x.leaf = x.surf = 0;
     x.tests = x.descent = x.broot = 0;
@@ -543,14 +570,15 @@ __global__ __launch_bounds__(256) void k_expand_level(TravArgs T, const Item* __
       expand_item<K>(T.nodes, it, x);
     }
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-    const uint32_t o = wave_reserve(out_fill, __popc(x.push));
-    emit_items(x, out, o, cap, &T.cnt->overflow);
+    const uint32_t o = block_reserve(out_fill, __popc(x.push), red);
+    emit_items(x, out, o, cap, T.overflow);
     emit_leaves(T, it.pair, x);
   }
-  wave_add64(&T.cnt->tests, tests);
-  wave_add64(&T.cnt->descent, descent);
-  wave_add64(&T.cnt->branch_overlaps, broot);
-  wave_add64(&T.cnt->p1, p1s);
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  wave_add64(&st->tests, tests);
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
 }
 
 // ============================================================== narrow phase
@@ -566,7 +594,13 @@ struct NarrowArgs {
   NarrowRec* recs;
   uint32_t* r_fill;
   uint32_t r_cap;
-  DevCounters* cnt;
+  CounterStripe* stripes;
+  unsigned int* overflow;
+  // when non-null: run only if every word of pending[0..kSegs) is zero
+  // (the persistent traversal left no spill behind; else the host drains it
+  // and launches the narrow phase again)
+  const uint32_t* pending;
+  uint32_t pending_stride;  // words between consecutive pending entries
 };
 
 __device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
@@ -579,6 +613,11 @@ __device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
 // One thread per surface pair: SAT, most opposing normal, contact points
 // (contacts/geometry_contacts_tetrahedron_tetrahedron.h:1312-1345).
 __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
+  if (A.pending) {
+    const uint32_t lane = lane_id();
+    const uint32_t v = A.pending[(size_t)lane * A.pending_stride];
+    if (__any(v != 0u)) return;
+  }
   const uint32_t n = min(*A.p2_fill, A.p2_cap);
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -608,7 +647,8 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
       uint32_t k = 0;
       for (int c = 0; c < r.ncand; ++c) {
         if (!((mask >> c) & 1ull)) continue;
-        if (oc + k < A.c_cap) {
+        if (oc + k >= A.c_cap) atomicOr(A.overflow, 8u);
+        else {
           kdop_contact ct;
           ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = k;
           ct.p[0] = raw[c].x; ct.p[1] = raw[c].y; ct.p[2] = raw[c].z;
@@ -627,7 +667,7 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
       }
     }
   }
-  wave_add64(&A.cnt->narrow_pairs, hits);
+  wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
 }
 
 }  // namespace kdop_amd

@@ -1,20 +1,22 @@
 // kdop_persistent.h -- persistent tandem traversal.
 //
 // Three launches per call instead of one per BVTT level:
-//   k_seed  (one block per body pair)  root tests + top-level chunk-root pairs
+//   k_seed  (grid over seed pairs)     root tests + chunk-root pairs
 //   k_ramp  (one 512-thread block)     breadth-first levels, synchronised with
 //                                      __syncthreads, while the frontier is
 //                                      narrower than the persistent grid needs
 //   k_persist (persistent grid)        waves take guided chunks of that
-//                                      frontier (one atomic per chunk) and
-//                                      expand them depth first from a stack
-//                                      in LDS; a full stack spills to a
-//                                      global buffer that the next round
-//                                      (same kernel) takes as its frontier.
-// Agent-scope atomics and polling loads on one word are slow on a multi-XCD
-// part (they bypass the per-XCD L2s), so nothing here polls shared words:
-// the only shared RMWs are the chunk counter, the output counters (one per
-// wave and batch) and the spill counter.
+//                                      frontier and expand them depth first
+//                                      from a stack in LDS; a full stack
+//                                      spills to a global buffer that the
+//                                      next round (same kernel) takes as its
+//                                      frontier.
+// Agent-scope atomics on one word are slow on a multi-XCD part: they are
+// performed beyond the per-XCD L2s and serialize, so a counter that a
+// thousand waves hit costs more than the traversal.  Every hot counter is
+// therefore spread: each round's frontier is cut into kSegs segments, each
+// with its own chunk and spill counters on a line of its own; statistics go
+// to striped counters; leaf pairs are staged in LDS and written in batches.
 #pragma once
 
 #include "kdop_kernels.h"
@@ -23,18 +25,33 @@ namespace kdop_amd {
 
 constexpr int kMaxRounds = 8;
 
-struct PersistState {
-  unsigned int level_fill[64];      // ramp: items per level (level 0 = seeds)
-  unsigned int final_level;         // ramp: level whose buffer holds the frontier
-  unsigned int final_count;
-  unsigned int ramp_done;           // ramp: 1 when the frontier is complete (empty)
-  unsigned int next[kMaxRounds];    // persist: chunk counter per round
-  unsigned int count[kMaxRounds];   // persist: frontier size per round (round r > 0: spill of round r-1)
-  unsigned int overflow;
-  unsigned int max_iters;            // persist: most loop iterations of one wave (all rounds)
-  unsigned long long iters;          // persist: loop iterations summed over waves
-  unsigned long long lanes;          // persist: items expanded (active lanes) summed over waves
-  unsigned long long grabs;          // persist: chunks taken from the frontier
+// Counters of one segment of one round's frontier.
+struct alignas(256) SegCtr {
+  uint32_t fill;  // items spilled into the segment by the previous round
+  uint32_t next;  // chunk counter of the round that reads the segment
+};
+
+// Summary of one call, written by k_sum_counters and copied to the host.
+struct CallSummary {
+  unsigned long long total[kStripeFields];  // CounterStripe fields summed (max_iters: max)
+  uint32_t overflow;
+  Fills fills;
+  uint32_t level_fill[64];
+  uint32_t final_level, final_count;
+  uint32_t round_total[kMaxRounds];   // items in round r's frontier
+  uint32_t round_maxseg[kMaxRounds];  // largest segment fill of round r (r >= 1)
+};
+
+// Device state of one traversal call, zeroed by the host before the call.
+struct CallState {
+  CounterStripe stripe[kStripes];
+  SegCtr seg[kMaxRounds][kSegs];
+  alignas(256) unsigned long long done[kMaxRounds];  // per round: segments whose last chunk was taken
+  alignas(256) unsigned int overflow;                // bit 0 frontier, 1 P1, 2 P2, 3 contacts
+  alignas(256) Fills fills;
+  alignas(256) uint32_t level_fill[64];  // items per BFS level (level 0 = seeds)
+  uint32_t final_level, final_count;     // ramp: level whose buffer holds the frontier, its size
+  alignas(256) CallSummary sum;
 };
 
 // Breadth-first levels in one workgroup until the frontier reaches `target`
@@ -42,18 +59,16 @@ struct PersistState {
 // level l is read from buf[l & 1] and written to buf[(l + 1) & 1].
 template <int K>
 __global__ __launch_bounds__(512) void k_ramp(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                               PersistState* __restrict__ S, uint32_t target, uint32_t max_levels) {
+                                               CallState* __restrict__ C, uint32_t target, uint32_t max_levels) {
   const int lane = lane_id();
   const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   for (uint32_t l = 0;; ++l) {
-    const uint32_t n = min(__hip_atomic_load(&S->level_fill[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
+    const uint32_t n = min(__hip_atomic_load(&C->level_fill[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
     if (n == 0 || n >= target || l >= max_levels || l >= 62) {
       if (threadIdx.x == 0) {
-        S->final_level = l;
-        S->final_count = n;
-        S->count[0] = n;
-        S->ramp_done = (n == 0) ? 1u : 0u;
+        C->final_level = l;
+        C->final_count = n;
       }
       break;
     }
@@ -70,81 +85,171 @@ __global__ __launch_bounds__(512) void k_ramp(TravArgs T, Item* __restrict__ buf
         expand_item<K>(T.nodes, it, x);
       }
       tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-      const uint32_t o = wave_reserve(&S->level_fill[l + 1], __popc(x.push));
-      emit_items(x, out, o, cap, &S->overflow);
+      const uint32_t o = wave_reserve(&C->level_fill[l + 1], __popc(x.push));
+      emit_items(x, out, o, cap, T.overflow);
       emit_leaves(T, it.pair, x);
     }
     __syncthreads();
   }
-  wave_add64(&T.cnt->tests, tests);
-  wave_add64(&T.cnt->descent, descent);
-  wave_add64(&T.cnt->branch_overlaps, broot);
-  wave_add64(&T.cnt->p1, p1s);
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  wave_add64(&st->tests, tests);
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
 }
 
-// One round of the persistent traversal over frontier `in` (S->count[round]
-// items).  Each iteration a wave expands up to 64 pairs: the top of its LDS
-// stack, topped up with a guided chunk of the frontier while it lasts.
-// Children go on the stack, or to `spill` (the next round's frontier) when it
-// is full.  Once the frontier is gone, a wave keeps at most `keep` items and
-// spills the bottom (oldest, largest subtrees) of a deeper stack, so the next
-// round spreads the tail over every wave.
+// First segment after `cur` (cyclically) whose bit in `known` is clear;
+// `known` must have a clear bit.
+__device__ inline uint32_t next_open(unsigned long long known, uint32_t cur) {
+  const unsigned long long open = ~known;
+  const unsigned long long above = cur >= 63u ? 0ull : (open & (~0ull << (cur + 1u)));
+  return (uint32_t)(__ffsll((unsigned long long)(above ? above : open)) - 1);
+}
+
+// Write the staged leaf pairs of one wave: all of them to P1 (when kept),
+// the surface subset (bit 31 of the pair word) to P2.  One atomic per list.
+__device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, const uint32_t* lp_ta, const uint32_t* lp_tb,
+                                    uint32_t nst) {
+  const uint32_t lane = lane_id();
+  if (T.p1) {
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(T.p1_fill, nst);
+    b = __shfl(b, 0, 64);
+    for (uint32_t i = lane; i < nst; i += 64) {
+      const uint32_t o = b + i;
+      if (o < T.p1_cap) T.p1[o] = LeafPair{lp_pair[i] & 0x7FFFFFFFu, lp_ta[i], lp_tb[i]};
+      else atomicOr(T.overflow, 2u);
+    }
+  }
+  uint32_t n2 = 0;
+  for (uint32_t i0 = 0; i0 < nst; i0 += 64) {
+    const uint32_t i = i0 + lane;
+    n2 += (uint32_t)__popcll(__ballot(i < nst && (lp_pair[i] >> 31)));
+  }
+  if (n2 == 0) return;
+  uint32_t b2 = 0;
+  if (lane == 0) b2 = atomicAdd(T.p2_fill, n2);
+  b2 = __shfl(b2, 0, 64);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (uint32_t i0 = 0; i0 < nst; i0 += 64) {
+    const uint32_t i = i0 + lane;
+    const bool s = i < nst && (lp_pair[i] >> 31);
+    const unsigned long long m = __ballot(s);
+    if (s) {
+      const uint32_t o = b2 + (uint32_t)__popcll(m & below);
+      if (o < T.p2_cap) T.p2[o] = LeafPair{lp_pair[i] & 0x7FFFFFFFu, lp_ta[i], lp_tb[i]};
+      else atomicOr(T.overflow, 4u);
+    }
+    b2 += (uint32_t)__popcll(m);
+  }
+}
+
+// Orders this wave's LDS accesses in program order (a one-wave workgroup:
+// the LDS executes one wave's instructions in issue order, so only the
+// compiler has to be kept from reordering them).
+__device__ inline void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// One round of the persistent traversal over this round's frontier (round 0:
+// the ramp's last level, contiguous; round r > 0: the segments that round
+// r - 1 spilled).  Each iteration a wave expands up to 64 pairs: the top of
+// its LDS stack, topped up with a guided chunk of its current segment while
+// the frontier lasts.  Children go on the stack, or to the next round's
+// segments when it is full.  Once the frontier is gone, a wave keeps at most
+// `keep` items and spills the bottom (oldest, largest subtrees) of a deeper
+// stack, so the next round spreads the tail over every wave.
 template <int K, int LS>
 __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                                PersistState* __restrict__ S, uint32_t round, uint32_t keep) {
+                                                CallState* __restrict__ C, uint32_t round, uint32_t keep) {
   static_assert((LS & (LS - 1)) == 0 && LS >= 256, "LDS stack: power of two, >= 256");
+  static_assert(kSegs == 64, "one lane per segment");
+  constexpr uint32_t SB = 512;  // staged leaf pairs; flushed above SB - 256 (one iteration adds <= 256)
   __shared__ uint32_t sa0[LS], sa1[LS], sb0[LS], sb1[LS], sp[LS];
-  const int lane = lane_id();
+  __shared__ uint32_t lp_pair[SB], lp_ta[SB], lp_tb[SB];
+  const uint32_t lane = lane_id();
   // the ramp left its frontier in buf[final_level & 1]; each round's spill
   // goes to the other buffer and is the next round's frontier
-  const bool odd = ((S->final_level + round) & 1u) != 0u;
+  const bool odd = ((C->final_level + round) & 1u) != 0u;
   const Item* __restrict__ in = odd ? buf1 : buf0;
   Item* __restrict__ spill = odd ? buf0 : buf1;
-  const uint32_t n_front = min(S->count[round], cap);
-  const uint32_t nwaves = gridDim.x;
+  const uint32_t segcap = cap / kSegs;
+  const uint32_t wave = blockIdx.x;
+  const uint32_t wps = max(1u, gridDim.x / (uint32_t)kSegs);  // waves per segment
+  // lane s holds segment s of this round's frontier
+  uint32_t seg_base, seg_len;
+  if (round == 0) {
+    const uint32_t n = min(C->final_count, cap);
+    seg_base = (uint32_t)(((unsigned long long)n * lane) / kSegs);
+    seg_len = (uint32_t)(((unsigned long long)n * (lane + 1)) / kSegs) - seg_base;
+  } else {
+    seg_base = lane * segcap;
+    seg_len = min(C->seg[round][lane].fill, segcap);
+  }
+  unsigned long long known = __ballot(seg_len == 0u);  // segments known to be exhausted
+  const uint32_t home = wave & (kSegs - 1);
+  uint32_t cur = home;
+  bool front_done = known == ~0ull;
+  if (!front_done && ((known >> cur) & 1ull)) cur = next_open(known, cur);
   uint32_t top = 0, bot = 0;  // stack = [bot, top) modulo LS, wave-uniform
-  uint32_t next_est = 0;      // last chunk start seen, for the guided chunk size
-  bool front_done = n_front == 0;
+  uint32_t next_est = 0;      // end of the last chunk taken from `cur`
+  uint32_t nsp = 0;           // spills so far (rotates the target segment)
+  uint32_t nstage = 0;        // staged leaf pairs
+  const bool keep_p1 = T.p1 != nullptr;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   uint32_t iters = 0, lanes = 0, grabs = 0;
   for (;;) {
-    // ---- batch: top of the stack, then a chunk of the frontier
+    // ---- batch: top of the stack, then a chunk of the current segment
     const uint32_t ns = min(64u, top - bot);
     Item it{0, 0, 0, 0, 0};
-    if ((uint32_t)lane < ns) {
+    if (lane < ns) {
       const uint32_t s = (top - ns + lane) & (LS - 1);
       it = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
     }
     top -= ns;
     uint32_t ng = 0;
+    bool refresh = false;
     if (ns < 64 && !front_done) {
-      const uint32_t remaining = n_front > next_est ? n_front - next_est : 0u;
-      uint32_t want = remaining / (2u * nwaves);
+      const uint32_t len = __shfl(seg_len, cur, 64), sbase = __shfl(seg_base, cur, 64);
+      const uint32_t remaining = len > next_est ? len - next_est : 0u;
+      uint32_t want = remaining / (2u * wps);
       want = want < 1u ? 1u : (want > 64u - ns ? 64u - ns : want);
       uint32_t start = 0;
-      if (lane == 0) start = atomicAdd(&S->next[round], want);
+      if (lane == 0) start = atomicAdd(&C->seg[round][cur].next, want);
       start = __shfl(start, 0, 64);
-      if (start >= n_front) {
-        front_done = true;
-      } else {
+      if (start < len) {
         ++grabs;
         next_est = start + want;
-        ng = min(want, n_front - start);
-        if ((uint32_t)lane >= ns && (uint32_t)lane < ns + ng) it = in[start + lane - ns];
+        ng = min(want, len - start);
+        if (lane >= ns && lane < ns + ng) it = in[sbase + start + lane - ns];
+        if (start + want >= len) {  // this wave took the segment's last chunk
+          if (lane == 0) atomicOr(&C->done[round], 1ull << cur);
+          known |= 1ull << cur;
+        }
+      } else {
+        known |= 1ull << cur;
+        refresh = true;
       }
     } else if (!front_done && (iters & 15u) == 15u) {
-      // a wave living off its own stack looks at the frontier now and then
-      uint32_t nx = 0;
-      if (lane == 0) nx = __hip_atomic_load(&S->next[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      front_done = __shfl(nx, 0, 64) >= n_front;
+      refresh = true;  // a wave living off its own stack looks at the frontier now and then
+    }
+    if (refresh) {
+      unsigned long long d = 0;
+      if (lane == 0) d = __hip_atomic_load(&C->done[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      known |= __shfl(d, 0, 64);
+    }
+    if (!front_done && ((known >> cur) & 1ull)) {
+      if (known == ~0ull) front_done = true;
+      else { cur = next_open(known, cur); next_est = 0; }
     }
     const uint32_t n = ns + ng;
-    if (n == 0) break;
+    if (n == 0) {
+      if (front_done) break;
+      continue;  // the current segment ran dry: try the next one
+    }
     // ---- expand
     Expansion x;
     x.push = x.leaf = x.surf = 0;
     x.tests = x.descent = x.broot = 0;
-    if ((uint32_t)lane < n) expand_item<K>(T.nodes, it, x);
+    if (lane < n) expand_item<K>(T.nodes, it, x);
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
     ++iters;
     lanes += n;
@@ -164,10 +269,11 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
           }
         top += total;
       } else {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&S->count[round + 1], total);
-        base = __shfl(base, 0, 64);
-        emit_items(x, spill, base + incl - nk, cap, &S->overflow);
+        const uint32_t sg = (home + nsp++) & (kSegs - 1);
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, total);
+        b = __shfl(b, 0, 64);
+        emit_items(x, spill + (size_t)sg * segcap, b + incl - nk, segcap, T.overflow);
       }
     }
     // ---- tail: hand the bottom of a deep stack to the next round
@@ -175,25 +281,85 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
       const uint32_t s3 = (bot + lane) & (LS - 1);
       const Item mine = Item{sa0[s3], sa1[s3], sb0[s3], sb1[s3], sp[s3]};
       bot += 64;
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&S->count[round + 1], 64u);
-      base = __shfl(base, 0, 64) + lane;
-      if (base < cap) spill[base] = mine;
-      else atomicOr(&S->overflow, 1u);
+      const uint32_t sg = (home + nsp++) & (kSegs - 1);
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, 64u);
+      b = __shfl(b, 0, 64) + lane;
+      if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
+      else atomicOr(T.overflow, 1u);
     }
-    __syncthreads();  // one-wave block: stack writes land before the next pop
-    emit_leaves(T, it.pair, x);
+    // ---- leaf pairs: stage in LDS (all of P1 when it is kept, else only P2)
+    const uint32_t lm = keep_p1 ? x.leaf : x.surf;
+    const uint32_t nl = __popc(lm);
+    const uint32_t linc = wave_incl_scan(nl);
+    const uint32_t ltot = __shfl(linc, 63, 64);
+    if (ltot) {
+      uint32_t o = nstage + linc - nl;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((lm >> k) & 1u) {
+          lp_pair[o] = it.pair | (((x.surf >> k) & 1u) << 31);
+          lp_ta[o] = x.ta[k >> 1];
+          lp_tb[o] = x.tb[k & 1];
+          ++o;
+        }
+      nstage += ltot;
+      if (nstage > SB - 256) {
+        wave_lds_order();
+        flush_leaves(T, lp_pair, lp_ta, lp_tb, nstage);
+        nstage = 0;
+      }
+    }
+    wave_lds_order();  // stack and staging writes land before the next reads
+  }
+  wave_lds_order();
+  if (nstage) flush_leaves(T, lp_pair, lp_ta, lp_tb, nstage);
+  CounterStripe* st = stripe_of(T.stripes, wave);
+  wave_add64(&st->tests, tests);
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
+  if (lane == 0) {
+    atomicAdd(&st->iters, (unsigned long long)iters);
+    atomicAdd(&st->lanes, (unsigned long long)lanes);
+    atomicAdd(&st->grabs, (unsigned long long)grabs);
+    atomicMax(&st->max_iters, (unsigned long long)iters);
+  }
+}
+
+// Fold the striped counters and the segment fills into C->sum (one wave).
+__global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, uint32_t cap) {
+  const uint32_t lane = threadIdx.x;
+  CallSummary& S = C->sum;
+  if (lane < (uint32_t)kStripeFields) {
+    unsigned long long acc = 0;
+    for (int s = 0; s < kStripes; ++s) {
+      const unsigned long long v = reinterpret_cast<const unsigned long long*>(&C->stripe[s])[lane];
+      acc = (lane == (uint32_t)kStripeFields - 1u) ? max(acc, v) : acc + v;
+    }
+    S.total[lane] = acc;
   }
   if (lane == 0) {
-    atomicMax(&S->max_iters, iters);
-    atomicAdd(&S->iters, (unsigned long long)iters);
-    atomicAdd(&S->lanes, (unsigned long long)lanes);
-    atomicAdd(&S->grabs, (unsigned long long)grabs);
+    S.overflow = C->overflow;
+    S.fills = C->fills;
+    S.final_level = C->final_level;
+    S.final_count = C->final_count;
   }
-  wave_add64(&T.cnt->tests, tests);
-  wave_add64(&T.cnt->descent, descent);
-  wave_add64(&T.cnt->branch_overlaps, broot);
-  wave_add64(&T.cnt->p1, p1s);
+  S.level_fill[lane] = C->level_fill[lane];
+  const uint32_t segcap = cap / kSegs;
+  for (int r = 0; r < kMaxRounds; ++r) {
+    const uint32_t f = C->seg[r][lane].fill;
+    uint32_t tot = min(f, segcap), mx = f;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      tot += __shfl_xor(tot, d, 64);
+      mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+    }
+    if (lane == 0) {
+      S.round_total[r] = r == 0 ? C->final_count : tot;
+      S.round_maxseg[r] = mx;
+    }
+  }
 }
 
 }  // namespace kdop_amd
