@@ -74,8 +74,9 @@ struct kdop_ctx {
   DevBuf<float> poses;
   int num_cus = 0;
   int ls = 512, wpc = 8;
-  uint32_t ramp_target = 0;  // 0: 2 x persistent waves
-  uint32_t keep = 128;       // items a wave keeps once the frontier is gone
+  uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
+  uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
+  uint32_t keep = 512;       // items a wave keeps once the frontier is gone
   uint32_t last_rounds = 0;
   bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -304,18 +305,10 @@ struct Seed {
   }
 };
 template <int K>
-struct Expand {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t level, uint32_t cap) {
-    CallState* d = c->dstate;
-    hipLaunchKernelGGL(k_expand_level<K>, dim3(2048), dim3(256), 0, c->stream, T, c->front[level & 1].p, &d->level_fill[level],
-                       c->front[(level + 1) & 1].p, &d->level_fill[level + 1], cap);
-  }
-};
-
-template <int K>
-struct Ramp {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t target, uint32_t max_levels) {
-    hipLaunchKernelGGL(k_ramp<K>, dim3(1), dim3(512), 0, c->stream, T, c->front[0].p, c->front[1].p, cap, c->dstate, target, max_levels);
+struct Bfs {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t level, uint32_t target, uint32_t last) {
+    hipLaunchKernelGGL(k_bfs_level<K>, dim3((unsigned)(c->num_cus * 4)), dim3(256), 0, c->stream, T, c->front[0].p, c->front[1].p, cap,
+                       c->dstate, level, target, last);
   }
 };
 template <int K>
@@ -624,7 +617,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
-  const uint32_t target = c->ramp_target ? c->ramp_target : 2 * pwaves;
+  const uint32_t target = c->ramp_target ? c->ramp_target : 16 * pwaves;
   const uint32_t rounds_async = 3;  // persistent rounds queued before the first host sync
   uint64_t overflowed = 0;
   CallState* d = c->dstate;
@@ -645,10 +638,11 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
       HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
       if (persistent) {
-        dispatch_K<Ramp>(c->K, c, T, cap, target, levels + 1);
+        const uint32_t last = std::min(levels, c->bfs_max);
+        for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
-        for (uint32_t l = 0; l < levels; ++l) dispatch_K<Expand>(c->K, c, T, l, cap);
+        for (uint32_t l = 0; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
       }
     } else {
       HIPCHK(c, hipEventRecord(c->ev[3], c->stream));

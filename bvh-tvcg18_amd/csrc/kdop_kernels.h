@@ -40,6 +40,24 @@ __device__ inline void wave_add64(unsigned long long* dst, unsigned long long v)
   const unsigned long long s = wave_sum64(v);
   if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
+// Exclusive prefix sum across the wave of a per-lane count below 8, from
+// three ballots (no LDS round trips); *total receives the wave sum.
+__device__ inline uint32_t wave_prefix_small(uint32_t v, uint32_t* total) {
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const unsigned long long m = __ballot((v >> b) & 1u);
+    pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+    tot += (uint32_t)__popcll(m) << b;
+  }
+  *total = tot;
+  return pre;
+}
+// Value of lane 0 (a scalar register read, not an LDS permute).
+__device__ inline uint32_t lane0(uint32_t v) { return __builtin_amdgcn_readlane(v, 0); }
+// A value the caller knows to be equal on every lane, as a scalar.
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // Global index of the calling wave.
 __device__ inline uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 
@@ -548,35 +566,6 @@ __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __rest
   wave_add64(&st->root_tests, roots);
   wave_add64(&st->tests, tests + roots);
   wave_add64(&st->branch_pairs, bpairs);
-  wave_add64(&st->branch_overlaps, broot);
-  wave_add64(&st->p1, p1s);
-}
-
-// One level of the level-synchronous traversal: expand every item of `in`.
-template <int K>
-__global__ __launch_bounds__(256) void k_expand_level(TravArgs T, const Item* __restrict__ in, const uint32_t* __restrict__ in_fill,
-                                                      Item* __restrict__ out, uint32_t* __restrict__ out_fill, uint32_t cap) {
-  __shared__ uint32_t red[5];
-  const uint32_t n = min(*in_fill, cap);
-  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const uint32_t i = base + threadIdx.x;
-    Expansion x;
-    x.push = x.leaf = x.surf = 0;
-    x.tests = x.descent = x.broot = 0;
-    Item it{0, 0, 0, 0, 0};
-    if (i < n) {
-      it = in[i];
-      expand_item<K>(T.nodes, it, x);
-    }
-    tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-    const uint32_t o = block_reserve(out_fill, __popc(x.push), red);
-    emit_items(x, out, o, cap, T.overflow);
-    emit_leaves(T, it.pair, x);
-  }
-  CounterStripe* st = stripe_of(T.stripes, wave_global());
-  wave_add64(&st->tests, tests);
-  wave_add64(&st->descent, descent);
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);
 }

@@ -1,10 +1,10 @@
 // kdop_persistent.h -- persistent tandem traversal.
 //
-// Three launches per call instead of one per BVTT level:
+// A few launches per call instead of one per BVTT level:
 //   k_seed  (grid over seed pairs)     root tests + chunk-root pairs
-//   k_ramp  (one 512-thread block)     breadth-first levels, synchronised with
-//                                      __syncthreads, while the frontier is
-//                                      narrower than the persistent grid needs
+//   k_bfs_level (grid, a few launches) breadth-first levels while the frontier
+//                                      is narrower than the persistent grid
+//                                      needs (decided on the device)
 //   k_persist (persistent grid)        waves take guided chunks of that
 //                                      frontier and expand them depth first
 //                                      from a stack in LDS; a full stack
@@ -49,47 +49,50 @@ struct CallState {
   alignas(256) unsigned long long done[kMaxRounds];  // per round: segments whose last chunk was taken
   alignas(256) unsigned int overflow;                // bit 0 frontier, 1 P1, 2 P2, 3 contacts
   alignas(256) Fills fills;
-  alignas(256) uint32_t level_fill[64];  // items per BFS level (level 0 = seeds)
-  uint32_t final_level, final_count;     // ramp: level whose buffer holds the frontier, its size
+  alignas(256) uint32_t level_fill[64];     // items per BFS level (level 0 = seeds)
+  uint32_t final_level, final_count, final_set;  // level whose buffer holds the frontier, its size
   alignas(256) CallSummary sum;
 };
 
-// Breadth-first levels in one workgroup until the frontier reaches `target`
-// items, runs empty, or `max_levels` levels are done.  Buffers ping-pong:
-// level l is read from buf[l & 1] and written to buf[(l + 1) & 1].
+// One breadth-first level of the BVTT over the whole grid: expand every item
+// of level `level` (buf[level & 1]) into level + 1.  The launch that finds the
+// level empty, at least `target` items wide, or equal to `last` expands
+// nothing and records it as the frontier (final_level / final_count); later
+// launches of the same call return at once.  The host queues `last` + 1 of
+// them without reading any size back.  With target = ~0u and last = BVTT
+// depth this is the whole level-synchronous traversal.
 template <int K>
-__global__ __launch_bounds__(512) void k_ramp(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                               CallState* __restrict__ C, uint32_t target, uint32_t max_levels) {
-  const int lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+__global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+                                                   CallState* __restrict__ C, uint32_t level, uint32_t target, uint32_t last) {
+  __shared__ uint32_t red[5];
+  if (C->final_set) return;
+  const uint32_t fill = C->level_fill[level];
+  const uint32_t n = min(fill, cap);
+  if (n == 0 || n >= target || level >= last) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      C->final_level = level;
+      C->final_count = n;
+      C->final_set = 1u;
+    }
+    return;
+  }
+  const Item* __restrict__ in = (level & 1) ? buf1 : buf0;
+  Item* __restrict__ out = (level & 1) ? buf0 : buf1;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
-  for (uint32_t l = 0;; ++l) {
-    const uint32_t n = min(__hip_atomic_load(&C->level_fill[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), cap);
-    if (n == 0 || n >= target || l >= max_levels || l >= 62) {
-      if (threadIdx.x == 0) {
-        C->final_level = l;
-        C->final_count = n;
-      }
-      break;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    Expansion x;
+    x.push = x.leaf = x.surf = 0;
+    x.tests = x.descent = x.broot = 0;
+    Item it{0, 0, 0, 0, 0};
+    if (i < n) {
+      it = in[i];
+      expand_item<K>(T.nodes, it, x);
     }
-    const Item* in = (l & 1) ? buf1 : buf0;
-    Item* out = (l & 1) ? buf0 : buf1;
-    for (uint32_t base = wave * 64; base < n; base += nwaves * 64) {
-      const uint32_t i = base + lane;
-      Expansion x;
-      x.push = x.leaf = x.surf = 0;
-      x.tests = x.descent = x.broot = 0;
-      Item it{0, 0, 0, 0, 0};
-      if (i < n) {
-        it = in[i];
-        expand_item<K>(T.nodes, it, x);
-      }
-      tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-      const uint32_t o = wave_reserve(&C->level_fill[l + 1], __popc(x.push));
-      emit_items(x, out, o, cap, T.overflow);
-      emit_leaves(T, it.pair, x);
-    }
-    __syncthreads();
+    tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
+    const uint32_t o = block_reserve(&C->level_fill[level + 1], __popc(x.push), red);
+    emit_items(x, out, o, cap, T.overflow);
+    emit_leaves(T, it.pair, x);
   }
   CounterStripe* st = stripe_of(T.stripes, wave_global());
   wave_add64(&st->tests, tests);
@@ -114,7 +117,7 @@ __device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, 
   if (T.p1) {
     uint32_t b = 0;
     if (lane == 0) b = atomicAdd(T.p1_fill, nst);
-    b = __shfl(b, 0, 64);
+    b = lane0(b);
     for (uint32_t i = lane; i < nst; i += 64) {
       const uint32_t o = b + i;
       if (o < T.p1_cap) T.p1[o] = LeafPair{lp_pair[i] & 0x7FFFFFFFu, lp_ta[i], lp_tb[i]};
@@ -129,7 +132,7 @@ __device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, 
   if (n2 == 0) return;
   uint32_t b2 = 0;
   if (lane == 0) b2 = atomicAdd(T.p2_fill, n2);
-  b2 = __shfl(b2, 0, 64);
+  b2 = lane0(b2);
   const unsigned long long below = (1ull << lane) - 1ull;
   for (uint32_t i0 = 0; i0 < nst; i0 += 64) {
     const uint32_t i = i0 + lane;
@@ -150,7 +153,7 @@ __device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, 
 __device__ inline void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
 
 // One round of the persistent traversal over this round's frontier (round 0:
-// the ramp's last level, contiguous; round r > 0: the segments that round
+// the last breadth-first level, contiguous; round r > 0: the segments that round
 // r - 1 spilled).  Each iteration a wave expands up to 64 pairs: the top of
 // its LDS stack, topped up with a guided chunk of its current segment while
 // the frontier lasts.  Children go on the stack, or to the next round's
@@ -158,7 +161,7 @@ __device__ inline void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST
 // `keep` items and spills the bottom (oldest, largest subtrees) of a deeper
 // stack, so the next round spreads the tail over every wave.
 template <int K, int LS>
-__global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_persist(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
                                                 CallState* __restrict__ C, uint32_t round, uint32_t keep) {
   static_assert((LS & (LS - 1)) == 0 && LS >= 256, "LDS stack: power of two, >= 256");
   static_assert(kSegs == 64, "one lane per segment");
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
   __shared__ uint32_t sa0[LS], sa1[LS], sb0[LS], sb1[LS], sp[LS];
   __shared__ uint32_t lp_pair[SB], lp_ta[SB], lp_tb[SB];
   const uint32_t lane = lane_id();
-  // the ramp left its frontier in buf[final_level & 1]; each round's spill
+  // the breadth-first levels left the frontier in buf[final_level & 1]; each round's spill
   // goes to the other buffer and is the next round's frontier
   const bool odd = ((C->final_level + round) & 1u) != 0u;
   const Item* __restrict__ in = odd ? buf1 : buf0;
@@ -197,6 +200,7 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   uint32_t iters = 0, lanes = 0, grabs = 0;
   for (;;) {
+    top = uniform(top); bot = uniform(bot); cur = uniform(cur); next_est = uniform(next_est);
     // ---- batch: top of the stack, then a chunk of the current segment
     const uint32_t ns = min(64u, top - bot);
     Item it{0, 0, 0, 0, 0};
@@ -208,13 +212,13 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
     uint32_t ng = 0;
     bool refresh = false;
     if (ns < 64 && !front_done) {
-      const uint32_t len = __shfl(seg_len, cur, 64), sbase = __shfl(seg_base, cur, 64);
+      const uint32_t len = __builtin_amdgcn_readlane(seg_len, cur), sbase = __builtin_amdgcn_readlane(seg_base, cur);
       const uint32_t remaining = len > next_est ? len - next_est : 0u;
       uint32_t want = remaining / (2u * wps);
       want = want < 1u ? 1u : (want > 64u - ns ? 64u - ns : want);
       uint32_t start = 0;
       if (lane == 0) start = atomicAdd(&C->seg[round][cur].next, want);
-      start = __shfl(start, 0, 64);
+      start = lane0(start);
       if (start < len) {
         ++grabs;
         next_est = start + want;
@@ -234,7 +238,7 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
     if (refresh) {
       unsigned long long d = 0;
       if (lane == 0) d = __hip_atomic_load(&C->done[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      known |= __shfl(d, 0, 64);
+      known |= ((unsigned long long)lane0((uint32_t)(d >> 32)) << 32) | lane0((uint32_t)d);
     }
     if (!front_done && ((known >> cur) & 1ull)) {
       if (known == ~0ull) front_done = true;
@@ -254,12 +258,11 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
     ++iters;
     lanes += n;
     // ---- children: stack when they fit, else the next round
-    const uint32_t nk = __popc(x.push);
-    const uint32_t incl = wave_incl_scan(nk);
-    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t total;
+    const uint32_t kpre = wave_prefix_small(__popc(x.push), &total);
     if (total) {
       if ((top - bot) + total <= (uint32_t)LS) {
-        uint32_t o = top + incl - nk;
+        uint32_t o = top + kpre;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           if ((x.push >> k) & 1u) {
@@ -272,8 +275,8 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
         const uint32_t sg = (home + nsp++) & (kSegs - 1);
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, total);
-        b = __shfl(b, 0, 64);
-        emit_items(x, spill + (size_t)sg * segcap, b + incl - nk, segcap, T.overflow);
+        b = lane0(b);
+        emit_items(x, spill + (size_t)sg * segcap, b + kpre, segcap, T.overflow);
       }
     }
     // ---- tail: hand the bottom of a deep stack to the next round
@@ -284,17 +287,16 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
       const uint32_t sg = (home + nsp++) & (kSegs - 1);
       uint32_t b = 0;
       if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, 64u);
-      b = __shfl(b, 0, 64) + lane;
+      b = lane0(b) + lane;
       if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
       else atomicOr(T.overflow, 1u);
     }
     // ---- leaf pairs: stage in LDS (all of P1 when it is kept, else only P2)
     const uint32_t lm = keep_p1 ? x.leaf : x.surf;
-    const uint32_t nl = __popc(lm);
-    const uint32_t linc = wave_incl_scan(nl);
-    const uint32_t ltot = __shfl(linc, 63, 64);
+    uint32_t ltot;
+    const uint32_t lpre = wave_prefix_small(__popc(lm), &ltot);
     if (ltot) {
-      uint32_t o = nstage + linc - nl;
+      uint32_t o = nstage + lpre;
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if ((lm >> k) & 1u) {
@@ -327,29 +329,34 @@ __global__ __launch_bounds__(64) void k_persist(TravArgs T, Item* __restrict__ b
   }
 }
 
-// Fold the striped counters and the segment fills into C->sum (one wave).
+// Fold the striped counters and the segment fills into C->sum (one wave;
+// lane s reads stripe s, so every load is independent).
 __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, uint32_t cap) {
+  static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
   const uint32_t lane = threadIdx.x;
   CallSummary& S = C->sum;
-  if (lane < (uint32_t)kStripeFields) {
-    unsigned long long acc = 0;
-    for (int s = 0; s < kStripes; ++s) {
-      const unsigned long long v = reinterpret_cast<const unsigned long long*>(&C->stripe[s])[lane];
-      acc = (lane == (uint32_t)kStripeFields - 1u) ? max(acc, v) : acc + v;
+  const unsigned long long* row = reinterpret_cast<const unsigned long long*>(&C->stripe[lane]);
+  unsigned long long v[kStripeFields];
+#pragma unroll
+  for (int f = 0; f < kStripeFields; ++f) v[f] = row[f];
+  uint32_t fill[kMaxRounds];
+#pragma unroll
+  for (int r = 0; r < kMaxRounds; ++r) fill[r] = C->seg[r][lane].fill;
+  const uint32_t lf = C->level_fill[lane];
+#pragma unroll
+  for (int f = 0; f < kStripeFields; ++f) {
+    unsigned long long x = v[f];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const unsigned long long y = __shfl_xor(x, d, 64);
+      x = (f == kStripeFields - 1) ? max(x, y) : x + y;
     }
-    S.total[lane] = acc;
+    if (lane == 0) S.total[f] = x;
   }
-  if (lane == 0) {
-    S.overflow = C->overflow;
-    S.fills = C->fills;
-    S.final_level = C->final_level;
-    S.final_count = C->final_count;
-  }
-  S.level_fill[lane] = C->level_fill[lane];
   const uint32_t segcap = cap / kSegs;
+#pragma unroll
   for (int r = 0; r < kMaxRounds; ++r) {
-    const uint32_t f = C->seg[r][lane].fill;
-    uint32_t tot = min(f, segcap), mx = f;
+    uint32_t tot = min(fill[r], segcap), mx = fill[r];
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
       tot += __shfl_xor(tot, d, 64);
@@ -359,6 +366,13 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
       S.round_total[r] = r == 0 ? C->final_count : tot;
       S.round_maxseg[r] = mx;
     }
+  }
+  S.level_fill[lane] = lf;
+  if (lane == 0) {
+    S.overflow = C->overflow;
+    S.fills = C->fills;
+    S.final_level = C->final_level;
+    S.final_count = C->final_count;
   }
 }
 
