@@ -58,16 +58,22 @@ __device__ inline uint32_t lane0(uint32_t v) { return __builtin_amdgcn_readlane(
 // A value the caller knows to be equal on every lane, as a scalar.
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Orders one wave's LDS accesses: the LDS executes a wave's instructions in
+// issue order, so only the compiler must be kept from reordering them (no
+// barrier: other waves of the workgroup are not involved).
+__device__ inline void wave_lds_sync() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
 // Global index of the calling wave.
 __device__ inline uint32_t wave_global() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 
-// Reserve `cnt` slots per thread for the whole block with one atomic (every
-// thread of the block must call it; lds: nwaves + 1 words).  Returns this
-// thread's first slot.
+// Reserve `cnt` (< 8) slots per thread for the whole block with one atomic
+// (every thread of the block must call it; lds: nwaves + 1 words).  Returns
+// this thread's first slot.
 __device__ inline uint32_t block_reserve(uint32_t* ctr, uint32_t cnt, uint32_t* lds) {
-  const uint32_t incl = wave_incl_scan(cnt);
+  uint32_t wtot;
+  const uint32_t incl = wave_prefix_small(cnt, &wtot) + cnt;
   const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane_id() == 63) lds[w] = incl;
+  if (lane_id() == 0) lds[w] = wtot;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t acc = 0;
@@ -592,69 +598,166 @@ struct NarrowArgs {
   uint32_t pending_stride;  // words between consecutive pending entries
 };
 
-__device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
-  const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)t);
-  const uint32_t v[4] = {tv.x, tv.y, tv.z, tv.w};
+__device__ inline float wave_min_f(float v) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) P[q] = f3{B.X[v[q]], B.Y[v[q]], B.Z[v[q]]};
+  for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
 }
+__device__ inline float wave_max_f(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ inline float readlane_f(float v, uint32_t l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
-// One thread per surface pair: SAT, most opposing normal, contact points
-// (contacts/geometry_contacts_tetrahedron_tetrahedron.h:1312-1345).
+// One wave per surface pair (four per 256-thread block): SAT, most opposing
+// normal and contact points (contacts/geometry_contacts_tetrahedron_tetrahedron.h:
+// 1312-1345).  Lanes take the 44 SAT axes, the 16 face pairs and the 56
+// contact candidates; each of them is computed with the serial operation
+// order, and the serial loops' choices (first minimum, first maximum, first
+// opposing face pair with the smallest dot product, a candidate dropped when
+// within working precision of ANY earlier candidate) are made with ballots
+// over lanes in that order, so the results are those of the serial code.
 __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
+  __shared__ NarrowLds lds[4];
+  __shared__ uint32_t red[5];
   if (A.pending) {
-    const uint32_t lane = lane_id();
-    const uint32_t v = A.pending[(size_t)lane * A.pending_stride];
+    const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
     if (__any(v != 0u)) return;
   }
   const uint32_t n = min(*A.p2_fill, A.p2_cap);
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  NarrowLds& L = lds[w];
   unsigned long long hits = 0;
-  for (uint32_t base = wave * 64; base < n; base += waves * 64) {
-    const uint32_t i = base + lane_id();
-    f3 raw[kMaxCand];
-    uint64_t mask = 0;
-    NarrowResult r;
-    r.nuniq = 0; r.face = -1;
+  for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
+    const uint32_t i = base + w;
+    bool live = i < n;  // wave-uniform
     LeafPair lp{0, 0, 0};
-    if (i < n) {
+    uint32_t sa = 0, sb = 0;
+    if (live) {
       lp = A.p2[i];
       const uint2 bp = A.pairs[lp.pair];
       const BodyDev& BA = A.bodies[bp.x];
       const BodyDev& BB = A.bodies[bp.y];
-      f3 PA[4], PB[4];
-      gather_tet(BA, lp.ta, PA);
-      gather_tet(BB, lp.tb, PB);
-      r = narrow_pair(PA, PB, BA.surf[lp.ta] & 15u, BB.surf[lp.tb] & 15u, raw, &mask);
-    }
-    const uint32_t nu = (uint32_t)r.nuniq;
-    const uint32_t oc = wave_reserve(A.c_fill, nu);
-    const uint32_t orr = wave_reserve(A.r_fill, nu ? 1u : 0u);
-    if (nu) {
-      ++hits;
-      uint32_t k = 0;
-      for (int c = 0; c < r.ncand; ++c) {
-        if (!((mask >> c) & 1ull)) continue;
-        if (oc + k >= A.c_cap) atomicOr(A.overflow, 8u);
-        else {
-          kdop_contact ct;
-          ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = k;
-          ct.p[0] = raw[c].x; ct.p[1] = raw[c].y; ct.p[2] = raw[c].z;
-          ct.n[0] = r.n.x; ct.n[1] = r.n.y; ct.n[2] = r.n.z;
-          ct.depth = r.depth;
-          A.contacts[oc + k] = ct;
-        }
-        ++k;
+      sa = BA.surf[lp.ta] & 15u;
+      sb = BB.surf[lp.tb] & 15u;
+      if (lane < 8u) {
+        const BodyDev& Bd = lane < 4u ? BA : BB;
+        const uint32_t t = lane < 4u ? lp.ta : lp.tb;
+        const uint32_t vi = Bd.tets[4 * (size_t)t + (lane & 3u)];
+        L.v[lane] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
       }
+    }
+    wave_lds_sync();
+    // face planes (contacts ...tetrahedron.h:30-62)
+    if (live && lane < 8u) {
+      const f3* P = L.v + (lane & 4u);
+      const uint32_t f = lane & 3u;
+      L.u[lane] = make_uplane(P[face_v(f, 0)], P[face_v(f, 1)], P[face_v(f, 2)]);
+    }
+    wave_lds_sync();
+    // SAT gate (overlap ...tetrahedron.h:50-290)
+    const bool sep = live && lane < 44u && sat_axis(L, lane);
+    live = live && !__any(sep);
+    // most opposing face pair (contacts ...tetrahedron.h:464-533)
+    bool valid = false;
+    float test = 0.0f;
+    if (live && lane < 16u) {
+      const uint32_t a = lane >> 2, b = lane & 3u;
+      if (((sa >> a) & 1u) && ((sb >> b) & 1u)) {
+        const UPlane plA = L.u[a], plB = L.u[4 + b];
+        const f3 nA = unit3(plA.n), nB = unit3(plB.n);
+        test = dot3(nA, nB);
+        const float testA = dot3(plA.n, sub3(L.v[4 + b], plA.p));
+        const float testB = dot3(plB.n, sub3(L.v[a], plB.p));
+        valid = test < 0.0f && !(testA < 0.0f) && !(testB < 0.0f);
+      }
+    }
+    const float best = wave_min_f(valid ? test : INFINITY);
+    const unsigned long long pick = __ballot(valid && test == best);
+    live = live && pick != 0ull;
+    uint32_t face = 0;
+    f3 nrm{0.f, 0.f, 0.f};
+    bool ex = false;
+    f3 q{0.f, 0.f, 0.f};
+    uint8_t code = 0;
+    if (live) {
+      face = (uint32_t)(__ffsll(pick) - 1) >> 2;
+      nrm = unit3(L.u[face].n);
+      if (lane < (uint32_t)kMaxCand) ex = candidate(L, lane, &q, &code);
+    }
+    const unsigned long long exm = __ballot(ex);
+    live = live && exm != 0ull;
+    uint32_t nu = 0, ordinal = 0;
+    bool keep = false;
+    float depth = 0.0f;
+    uint32_t cmin = 0, cmax = 0;
+    if (live) {
+      // penetration depth from the extreme projections onto the normal
+      const float d = dot3(q, nrm);
+      const bool has = ex && d == d;
+      // serial: mn = FLT_MAX, mx = -FLT_MAX, first candidate attaining each
+      // (strict comparisons); the value is read from that candidate's lane
+      const float lo = wave_min_f(has ? d : FLT_MAX);
+      const float hi = wave_max_f(has ? d : -FLT_MAX);
+      const unsigned long long at_mn = __ballot(has && d == lo && d < FLT_MAX);
+      const unsigned long long at_mx = __ballot(has && d == hi && -FLT_MAX < d);
+      const uint32_t first = (uint32_t)(__ffsll(exm) - 1);
+      const uint32_t lmin = at_mn ? (uint32_t)(__ffsll(at_mn) - 1) : first;
+      const uint32_t lmax = at_mx ? (uint32_t)(__ffsll(at_mx) - 1) : first;
+      const float mn = at_mn ? readlane_f(d, lmin) : FLT_MAX;
+      const float mx = at_mx ? readlane_f(d, lmax) : -FLT_MAX;
+      cmin = __builtin_amdgcn_readlane((uint32_t)code, lmin);
+      cmax = __builtin_amdgcn_readlane((uint32_t)code, lmax);
+      depth = mn - mx;
+      // project onto the mid plane, drop points within working precision
+      const f3 mid = mul3(mul3(nrm, mx + mn), 0.5f);
+      const float s = dot3(nrm, sub3(q, mid));
+      q = sub3(q, f3{s * nrm.x, s * nrm.y, s * nrm.z});
+      const float wp = FLT_EPSILON * 10.0f;
+      bool dup = false;
+      unsigned long long rem = exm & (exm - 1ull) ? exm & ~(1ull << (63 - __clzll(exm))) : 0ull;  // all but the last
+      while (rem) {
+        const uint32_t src = (uint32_t)(__ffsll(rem) - 1);
+        rem &= rem - 1ull;
+        const f3 o{readlane_f(q.x, src), readlane_f(q.y, src), readlane_f(q.z, src)};
+        if (lane > src) {
+          const f3 dd = sub3(o, q);
+          const float n1 = smax(smax(fabsf(dd.x), fabsf(dd.y)), fabsf(dd.z));
+          if (n1 < wp) dup = true;
+        }
+      }
+      keep = ex && !dup;
+      const unsigned long long km = __ballot(keep);
+      nu = (uint32_t)__popcll(km);
+      ordinal = (uint32_t)__popcll(km & below);
+    }
+    const uint32_t oc = block_reserve(A.c_fill, keep ? 1u : 0u, red);
+    const uint32_t orr = block_reserve(A.r_fill, (lane == 0u && nu) ? 1u : 0u, red);
+    if (keep) {
+      if (oc >= A.c_cap) {
+        atomicOr(A.overflow, 8u);
+      } else {
+        kdop_contact ct;
+        ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = ordinal;
+        ct.p[0] = q.x; ct.p[1] = q.y; ct.p[2] = q.z;
+        ct.n[0] = nrm.x; ct.n[1] = nrm.y; ct.n[2] = nrm.z;
+        ct.depth = depth;
+        A.contacts[oc] = ct;
+      }
+    }
+    if (lane == 0u && nu) {
+      ++hits;
       if (orr < A.r_cap) {
         NarrowRec rec;
         rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
-        rec.face = (uint8_t)r.face; rec.nuniq = (uint8_t)nu; rec.cmin = r.cmin; rec.cmax = r.cmax;
-        rec.depth = r.depth; rec.n[0] = r.n.x; rec.n[1] = r.n.y; rec.n[2] = r.n.z;
+        rec.face = (uint8_t)face; rec.nuniq = (uint8_t)nu; rec.cmin = (uint8_t)cmin; rec.cmax = (uint8_t)cmax;
+        rec.depth = depth; rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
         A.recs[orr] = rec;
       }
     }
+    wave_lds_sync();  // this wave's LDS is rewritten by its next pair
   }
   wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
 }

@@ -11,17 +11,32 @@
 
 namespace kdop_amd {
 
+// Small index tables of the reference, packed two bits per entry so that a
+// lane-dependent lookup is a shift, not a memory access.
+template <int N>
+constexpr unsigned long long pack2(const int (&t)[N]) {
+  unsigned long long v = 0;
+  for (int i = 0; i < N; ++i) v |= (unsigned long long)t[i] << (2 * i);
+  return v;
+}
 // faces opposite vertex 0..3 (types/geometry_tetrahedron.h:231-257)
-__constant__ static const int8_t kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
-// edges 01 02 03 12 13 23 (overlap ...tetrahedron.h:102-107)
-__constant__ static const int8_t kEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+constexpr int kFaceT[12] = {1, 2, 3, 0, 3, 2, 0, 1, 3, 0, 2, 1};
+// edges 01 02 03 12 13 23 (overlap/geometry_overlap_tetrahedron_tetrahedron.h:102-107)
+constexpr int kEdgeT[12] = {0, 1, 0, 2, 0, 3, 1, 2, 1, 3, 2, 3};
 // Vertex triples SAT_line projects for each edge-edge axis, as the reference
-// hard-codes them (overlap ...tetrahedron.h:109-287): A side by A edge, B side
-// by (A edge, B edge); triple ids 0={1,2,3} 1={0,1,3} 2={0,1,2}.
-__constant__ static const int8_t kTriple[3][3] = {{1, 2, 3}, {0, 1, 3}, {0, 1, 2}};
-__constant__ static const int8_t kSatA[6] = {0, 0, 0, 1, 2, 2};
-__constant__ static const int8_t kSatB[6][6] = {{0, 1, 2, 1, 2, 2}, {0, 0, 2, 1, 2, 2}, {0, 0, 0, 1, 2, 2},
-                                                 {0, 0, 0, 1, 2, 2}, {0, 0, 0, 1, 2, 2}, {0, 0, 0, 1, 2, 2}};
+// hard-codes them (overlap ...tetrahedron.h:109-287): triple ids 0={1,2,3}
+// 1={0,1,3} 2={0,1,2}; A side by A edge, B side by (min(A edge, 2), B edge).
+constexpr int kTripleT[9] = {1, 2, 3, 0, 1, 3, 0, 1, 2};
+constexpr int kSatAT[6] = {0, 0, 0, 1, 2, 2};
+constexpr int kSatBT[18] = {0, 1, 2, 1, 2, 2, 0, 0, 2, 1, 2, 2, 0, 0, 0, 1, 2, 2};
+constexpr unsigned long long kFaceP = pack2(kFaceT), kEdgeP = pack2(kEdgeT), kTripleP = pack2(kTripleT);
+constexpr unsigned long long kSatAP = pack2(kSatAT), kSatBP = pack2(kSatBT);
+__device__ inline uint32_t tab2(unsigned long long t, uint32_t i) { return (uint32_t)(t >> (2 * i)) & 3u; }
+__device__ inline uint32_t face_v(uint32_t f, uint32_t k) { return tab2(kFaceP, 3 * f + k); }
+__device__ inline uint32_t edge_v(uint32_t e, uint32_t k) { return tab2(kEdgeP, 2 * e + k); }
+__device__ inline uint32_t triple_v(uint32_t t, uint32_t k) { return tab2(kTripleP, 3 * t + k); }
+__device__ inline uint32_t sat_a(uint32_t ea) { return tab2(kSatAP, ea); }
+__device__ inline uint32_t sat_b(uint32_t ea, uint32_t eb) { return tab2(kSatBP, 6 * (ea < 2u ? ea : 2u) + eb); }
 
 struct UPlane {  // unscaled plane: normal + a point on it
   f3 n, p;
@@ -39,7 +54,6 @@ __device__ inline UPlane make_uplane(const f3& p0, const f3& p1, const f3& p2) {
   if (l2 >= l1 && l2 >= l0) { pl.n = m2; pl.p = p2; }
   return pl;
 }
-__device__ inline UPlane face_uplane(const f3* t, int f) { return make_uplane(t[kFace[f][0]], t[kFace[f][1]], t[kFace[f][2]]); }
 __device__ inline float usd(const f3& q, const UPlane& pl) { return dot3(pl.n, sub3(q, pl.p)); }
 __device__ inline bool inside_planes(const f3& q, const UPlane* P) {
   const float d0 = usd(q, P[0]), d1 = usd(q, P[1]), d2 = usd(q, P[2]), d3 = usd(q, P[3]);
@@ -75,61 +89,6 @@ __device__ inline bool sat_line(const f3& a0, const f3& a1, const f3& a2, const 
   return (t0_max < t1_min) || (t0_min > t1_max);
 }
 
-// overlap/geometry_overlap_tetrahedron_tetrahedron.h:50-290
-__device__ inline bool overlap_tet_tet(const f3* A, const f3* B) {
-  for (int i = 0; i < 4; ++i) {
-    f3 n; float w;
-    make_plane(A[kFace[i][0]], A[kFace[i][1]], A[kFace[i][2]], n, w);
-    const float b0 = dot3(n, B[0]) - w, b1 = dot3(n, B[1]) - w, b2 = dot3(n, B[2]) - w, b3 = dot3(n, B[3]) - w;
-    if (b0 > 0.0f && b1 > 0.0f && b2 > 0.0f && b3 > 0.0f) return false;
-  }
-  for (int i = 0; i < 4; ++i) {
-    f3 n; float w;
-    make_plane(B[kFace[i][0]], B[kFace[i][1]], B[kFace[i][2]], n, w);
-    const float a0 = dot3(n, A[0]) - w, a1 = dot3(n, A[1]) - w, a2 = dot3(n, A[2]) - w, a3 = dot3(n, A[3]) - w;
-    if (a0 > 0.0f && a1 > 0.0f && a2 > 0.0f && a3 > 0.0f) return false;
-  }
-  for (int ea = 0; ea < 6; ++ea) {
-    const f3 eA = sub3(A[kEdge[ea][1]], A[kEdge[ea][0]]);
-    const int8_t* ta = kTriple[kSatA[ea]];
-    for (int eb = 0; eb < 6; ++eb) {
-      const f3 eB = sub3(B[kEdge[eb][1]], B[kEdge[eb][0]]);
-      const f3 n = cross3(eA, eB);
-      const int8_t* tb = kTriple[kSatB[ea][eb]];
-      if (sat_line(A[ta[0]], A[ta[1]], A[ta[2]], B[tb[0]], B[tb[1]], B[tb[2]], n)) return false;
-    }
-  }
-  return true;
-}
-
-// contacts ...tetrahedron.h:464-533; returns the chosen face of A or -1
-__device__ inline int pick_most_opposing(const f3* A, const f3* B, uint32_t sa, uint32_t sb, f3& n) {
-  int found = -1;
-  float best = 0.0f;
-  for (int a = 0; a < 4; ++a) {
-    if (!((sa >> a) & 1u)) continue;
-    const UPlane plA = face_uplane(A, a);
-    const f3 nA = unit3(plA.n);
-    for (int b = 0; b < 4; ++b) {
-      if (!((sb >> b) & 1u)) continue;
-      const UPlane plB = face_uplane(B, b);
-      const f3 nB = unit3(plB.n);
-      const float test = dot3(nA, nB);
-      if (test >= 0.0f) continue;
-      const float testA = dot3(plA.n, sub3(B[b], plA.p));
-      const float testB = dot3(plB.n, sub3(A[a], plB.p));
-      if (testA < 0.0f) continue;
-      if (testB < 0.0f) continue;
-      if (test < best) {
-        n = nA;
-        best = test;
-        found = a;
-      }
-    }
-  }
-  return found;
-}
-
 // Candidate provenance, one byte: kind (bits 0-1), vertex or edge (bits 2-4),
 // plane (bits 5-6).  kind 0: vertex of B inside A; 1: vertex of A inside B;
 // 2: edge of B crossing a face plane of A; 3: edge of A crossing a face plane of B.
@@ -137,79 +96,67 @@ __device__ inline uint8_t cand_code(int kind, int ve, int f) { return (uint8_t)(
 
 constexpr int kMaxCand = 56;  // 8 vertices + 6 edges x 4 planes x 2 directions
 
-struct NarrowResult {
-  int face;        // A face giving the normal, -1 when no contact
-  int ncand;       // candidates generated
-  int nuniq;       // reported contacts
-  f3 n;
-  float depth;
-  uint8_t cmin, cmax;  // provenance of the candidates attaining min / max of p.n
+// Per-wave LDS of the narrow phase: the two tets (A: v[0..3], B: v[4..7]) and
+// their face planes (A faces u[0..3], B faces u[4..7]).
+struct NarrowLds {
+  f3 v[8];
+  UPlane u[8];
 };
 
-// contacts ...tetrahedron.h:543-681 + :1312-1345.  On return raw[0..ncand)
-// holds the projected candidates and bit c of *mask is set when candidate c is
-// reported (not within working precision of an earlier one).
-__device__ inline NarrowResult narrow_pair(const f3* A, const f3* B, uint32_t sa, uint32_t sb, f3* raw, uint64_t* mask) {
-  NarrowResult r;
-  r.face = -1; r.ncand = 0; r.nuniq = 0; r.depth = 0.0f; r.n = f3{0.f, 0.f, 0.f}; r.cmin = r.cmax = 0;
-  *mask = 0;
-  if (!overlap_tet_tet(A, B)) return r;
-  f3 n{0.f, 0.f, 0.f};
-  const int fa = pick_most_opposing(A, B, sa, sb, n);
-  if (fa < 0) return r;
-  UPlane PA[4], PB[4];
-  for (int v = 0; v < 4; ++v) { PA[v] = face_uplane(A, v); PB[v] = face_uplane(B, v); }
-  uint8_t code[kMaxCand];
-  int nc = 0;
-  for (int v = 0; v < 4; ++v) {
-    if (inside_planes(B[v], PA)) { raw[nc] = B[v]; code[nc++] = cand_code(0, v, 0); }
-    if (inside_planes(A[v], PB)) { raw[nc] = A[v]; code[nc++] = cand_code(1, v, 0); }
+// SAT axis `ax` (0..43) in the reference's order (overlap ...tetrahedron.h:
+// 50-290): 0-3 faces of A against B's vertices, 4-7 faces of B against A's,
+// 8-43 edge pairs (ea, eb) = ((ax-8)/6, (ax-8)%6).  True when it separates.
+__device__ inline bool sat_axis(const NarrowLds& L, uint32_t ax) {
+  const f3* A = L.v;
+  const f3* B = L.v + 4;
+  if (ax < 8u) {
+    const f3* P = ax < 4u ? A : B;
+    const f3* Q = ax < 4u ? B : A;
+    const uint32_t f = ax & 3u;
+    f3 n; float w;
+    make_plane(P[face_v(f, 0)], P[face_v(f, 1)], P[face_v(f, 2)], n, w);
+    const float d0 = dot3(n, Q[0]) - w, d1 = dot3(n, Q[1]) - w, d2 = dot3(n, Q[2]) - w, d3 = dot3(n, Q[3]) - w;
+    return d0 > 0.0f && d1 > 0.0f && d2 > 0.0f && d3 > 0.0f;
   }
-  for (int e = 0; e < 6; ++e) {
-    const int i = kEdge[e][0], j = kEdge[e][1];
-    for (int p = 0; p < 4; ++p) {
-      if (is_crossing(B[i], B[j], PA[p])) {
-        const f3 q = make_intersection(B[i], B[j], PA[p]);
-        if (inside_planes(q, PA)) { raw[nc] = q; code[nc++] = cand_code(2, e, p); }
-      }
-      if (is_crossing(A[i], A[j], PB[p])) {
-        const f3 q = make_intersection(A[i], A[j], PB[p]);
-        if (inside_planes(q, PB)) { raw[nc] = q; code[nc++] = cand_code(3, e, p); }
-      }
+  const uint32_t k = ax - 8u, ea = k / 6u, eb = k - 6u * ea;
+  const f3 eA = sub3(A[edge_v(ea, 1)], A[edge_v(ea, 0)]);
+  const f3 eB = sub3(B[edge_v(eb, 1)], B[edge_v(eb, 0)]);
+  const f3 n = cross3(eA, eB);
+  const uint32_t ta = sat_a(ea), tb = sat_b(ea, eb);
+  return sat_line(A[triple_v(ta, 0)], A[triple_v(ta, 1)], A[triple_v(ta, 2)], B[triple_v(tb, 0)], B[triple_v(tb, 1)],
+                  B[triple_v(tb, 2)], n);
+}
+
+// Contact candidate `c` (0..55) in the reference's generation order
+// (contacts ...tetrahedron.h:543-640): c < 8: vertex c/2, B's inside A (even c)
+// or A's inside B (odd c); c >= 8: k = c - 8, edge k/8 against face plane
+// (k/2)%4, B's edge on A's plane (even k) or A's edge on B's plane (odd k).
+// Returns whether it exists; *q its point, *code its provenance.
+__device__ inline bool candidate(const NarrowLds& L, uint32_t c, f3* q, uint8_t* code) {
+  const f3* A = L.v;
+  const f3* B = L.v + 4;
+  const UPlane* PA = L.u;
+  const UPlane* PB = L.u + 4;
+  if (c < 8u) {
+    const uint32_t v = c >> 1;
+    if (c & 1u) {
+      *q = A[v];
+      *code = cand_code(1, (int)v, 0);
+      return inside_planes(*q, PB);
     }
+    *q = B[v];
+    *code = cand_code(0, (int)v, 0);
+    return inside_planes(*q, PA);
   }
-  r.face = fa;
-  r.n = n;
-  r.ncand = nc;
-  if (nc == 0) { r.face = -1; return r; }
-  float mn = FLT_MAX, mx = -FLT_MAX;
-  int imin = 0, imax = 0;
-  for (int c = 0; c < nc; ++c) {
-    const float d = dot3(raw[c], n);
-    if (d < mn) { mn = d; imin = c; }
-    if (mx < d) { mx = d; imax = c; }
-  }
-  r.depth = mn - mx;
-  r.cmin = code[imin];
-  r.cmax = code[imax];
-  const f3 mid = mul3(mul3(n, mx + mn), 0.5f);
-  const float wp = FLT_EPSILON * 10.0f;
-  int nu = 0;
-  uint64_t m = 0;
-  for (int c = 0; c < nc; ++c) {
-    const float s = dot3(n, sub3(raw[c], mid));
-    raw[c] = sub3(raw[c], f3{s * n.x, s * n.y, s * n.z});
-    bool uniq = true;
-    for (int q = 0; q < c; ++q) {
-      const f3 d = sub3(raw[q], raw[c]);
-      const float n1 = smax(smax(fabsf(d.x), fabsf(d.y)), fabsf(d.z));
-      if (n1 < wp) { uniq = false; break; }
-    }
-    if (uniq) { m |= 1ull << c; ++nu; }
-  }
-  *mask = m;
-  r.nuniq = nu;
-  return r;
+  const uint32_t k = c - 8u, e = k >> 3, p = (k >> 1) & 3u;
+  const uint32_t i = edge_v(e, 0), j = edge_v(e, 1);
+  const bool a_edge = k & 1u;
+  const f3* E = a_edge ? A : B;
+  const UPlane* P = a_edge ? PB : PA;
+  *code = cand_code(a_edge ? 3 : 2, (int)e, (int)p);
+  if (!is_crossing(E[i], E[j], P[p])) return false;
+  *q = make_intersection(E[i], E[j], P[p]);
+  return inside_planes(*q, P);
 }
 
 }  // namespace kdop_amd

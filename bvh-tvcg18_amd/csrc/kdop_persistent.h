@@ -147,11 +147,6 @@ __device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, 
   }
 }
 
-// Orders this wave's LDS accesses in program order (a one-wave workgroup:
-// the LDS executes one wave's instructions in issue order, so only the
-// compiler has to be kept from reordering them).
-__device__ inline void wave_lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
-
 // One round of the persistent traversal over this round's frontier (round 0:
 // the last breadth-first level, contiguous; round r > 0: the segments that round
 // r - 1 spilled).  Each iteration a wave expands up to 64 pairs: the top of
@@ -307,14 +302,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }
       nstage += ltot;
       if (nstage > SB - 256) {
-        wave_lds_order();
+        wave_lds_sync();
         flush_leaves(T, lp_pair, lp_ta, lp_tb, nstage);
         nstage = 0;
       }
     }
-    wave_lds_order();  // stack and staging writes land before the next reads
+    wave_lds_sync();  // stack and staging writes land before the next reads
   }
-  wave_lds_order();
+  wave_lds_sync();
   if (nstage) flush_leaves(T, lp_pair, lp_ta, lp_tb, nstage);
   CounterStripe* st = stripe_of(T.stripes, wave);
   wave_add64(&st->tests, tests);
