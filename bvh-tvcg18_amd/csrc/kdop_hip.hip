@@ -213,9 +213,16 @@ int upload_bodies(kdop_ctx* c) {
 
 // ------------------------------------------------------------------- refit
 template <int K>
+struct RefitLeaves {
+  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, uint32_t n) {
+    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, n, c->nodes.p,
+                       c->dt);
+  }
+};
+template <int K>
 struct RefitBranch {
-  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* cpre, uint32_t n) {
-    hipLaunchKernelGGL(k_refit_branch<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, cpre, n, c->nodes.p, c->dt);
+  static void run(kdop_ctx* c, uint32_t nlist, const uint32_t* lbody, const uint32_t* lchunk) {
+    hipLaunchKernelGGL(k_refit_branch<K>, dim3(nlist), dim3(64), 0, c->stream, c->dbodies.p, lbody, lchunk, nlist, c->nodes.p, c->dt);
   }
 };
 template <int K>
@@ -235,8 +242,8 @@ struct RefitRoot {
 // Transform (or copy, identity) + leaf/union passes for a batch of bodies.
 // pos/quat: device pointers (3n, 4n floats) or null with identity.
 int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
-  // plan layout (uint32): [bb n][vpre n+1][cpre n+1] then per super step s:
-  // [bb m][cpre m+1][cbase m][cn m]
+  // plan layout (uint32): [bb n][vpre n+1][tpre n+1][partial chunks: body m,
+  // chunk m] then per super step s: [bb m][cpre m+1][cbase m][cn m]
   std::vector<uint32_t> plan;
   plan.reserve(8 * (size_t)n + 16);
   uint32_t maxH = 1;
@@ -248,11 +255,24 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   plan.push_back(0);
   for (uint32_t i = 0; i < n; ++i) { acc += c->bodies[bodies[i]].nv; plan.push_back((uint32_t)acc); }
   const uint64_t total_v = acc;
-  const size_t o_cpre = plan.size();
+  // full branch chunks: one thread per tet, each body's share padded to 64
+  const size_t o_tpre = plan.size();
   acc = 0;
   plan.push_back(0);
-  for (uint32_t i = 0; i < n; ++i) { const BodyHost& b = c->bodies[bodies[i]]; acc += b.C[b.H - 1]; plan.push_back((uint32_t)acc); }
-  const uint64_t total_branch = acc;
+  std::vector<uint32_t> pbody, pchunk;
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    const uint64_t full = (uint64_t)(b.nt / b.L) * b.L;
+    acc += (full + 63) / 64 * 64;
+    plan.push_back((uint32_t)acc);
+    if (full < b.nt) { pbody.push_back(bodies[i]); pchunk.push_back(b.C[b.H - 1] - 1); }
+  }
+  const uint64_t total_leaf_threads = acc;
+  if (total_leaf_threads >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many tetrahedra in one batch");
+  const size_t o_pbody = plan.size();
+  plan.insert(plan.end(), pbody.begin(), pbody.end());
+  const size_t o_pchunk = plan.size();
+  plan.insert(plan.end(), pchunk.begin(), pchunk.end());
   struct Step { size_t o_bb, o_cpre, o_cbase, o_cn; uint32_t m, chunks; };
   std::vector<Step> steps;
   for (uint32_t s = 1; s < maxH; ++s) {
@@ -287,7 +307,8 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, c->stream, c->dbodies.p, P + o_bb, P + o_vpre, n, pos_dev, quat_dev,
                        identity ? 1 : 0);
   }
-  dispatch_K<RefitBranch>(c->K, c, (uint32_t)total_branch, P + o_bb, P + o_cpre, n);
+  if (total_leaf_threads) dispatch_K<RefitLeaves>(c->K, c, (uint32_t)total_leaf_threads, P + o_bb, P + o_tpre, n);
+  if (!pbody.empty()) dispatch_K<RefitBranch>(c->K, c, (uint32_t)pbody.size(), P + o_pbody, P + o_pchunk);
   for (const Step& st : steps)
     dispatch_K<RefitSuper>(c->K, c, st.chunks, P + st.o_bb, P + st.o_cpre, P + st.o_cbase, P + st.o_cn, st.m);
   dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);

@@ -234,21 +234,97 @@ __device__ inline uint32_t find_item(const uint32_t* __restrict__ cpre, uint32_t
   return lo;
 }
 
-// One 64-lane block per branch chunk: leaf DOPs (geometry_dop.h:48-74) from
-// the world vertices, unions (geometry_make_union.h:9-25) bottom-up in LDS,
-// then one coalesced write of the chunk's volumes.
+// Store the K volume floats of one record (16-byte stores; K is even, so the
+// tail is one 8-byte store and the meta words after it stay untouched).
 template <int K>
-__global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                     const uint32_t* __restrict__ cpre, uint32_t nbatch, float* __restrict__ nodes,
+__device__ inline void store_vol(float* __restrict__ nodes, uint32_t slot, const float* v) {
+  float* r = nodes + (size_t)slot * rec_stride(K);
+#pragma unroll
+  for (int f = 0; f + 4 <= K; f += 4) *reinterpret_cast<float4*>(r + f) = make_float4(v[f], v[f + 1], v[f + 2], v[f + 3]);
+  if (K % 4) *reinterpret_cast<float2*>(r + (K & ~3)) = make_float2(v[K - 2], v[K - 1]);
+}
+
+// Leaf DOP of one tetrahedron (geometry_dop.h:48-74): per slab direction the
+// min / max projection of its four world vertices.
+template <int K>
+__device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& dt, float* v) {
+  const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)tet);
+  const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
+  f3 P[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) P[p] = f3{B.X[vi[p]], B.Y[vi[p]], B.Z[vi[p]]};
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) {
+    const f3 d = f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]};
+    float lo = FLT_MAX, hi = -FLT_MAX;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const float proj = dot3(d, P[p]);
+      lo = smin(lo, proj);
+      hi = smax(hi, proj);
+    }
+    v[2 * k] = lo;
+    v[2 * k + 1] = hi;
+  }
+}
+
+// Full branch chunks, one thread per tetrahedron: the leaf DOP, then the
+// chunk's internal nodes by a butterfly over its L lanes.  A full chunk's
+// median-split tree over L = 2^logL leaves is the implicit heap (BFS node
+// 2^d - 1 + i at depth d, leaf j at L - 1 + j), so level t of the butterfly
+// is depth logL - t, and every union takes (left, right) in the order the
+// reference's bottom-up pass does (geometry_make_union.h:9-25).  tpre: prefix
+// over the batch of the tets in full chunks, each body's share rounded up to
+// 64 so that a wave never mixes bodies (L, a power of two <= 32, then divides
+// every wave's tet range and an aligned L-lane group is one chunk).
+template <int K>
+__global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+                                                      const uint32_t* __restrict__ tpre, uint32_t nbatch, float* __restrict__ nodes,
+                                                      DirTable dt) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= tpre[nbatch]) return;  // whole waves (tpre[nbatch] is a multiple of 64)
+  const uint32_t item = find_item(tpre, nbatch, g);
+  const BodyDev& B = bodies[batch_body[item]];
+  const uint32_t logL = (uint32_t)(__ffs((int)B.L) - 1), L = B.L;
+  const uint32_t tet = g - tpre[item];
+  const bool act = tet < ((B.nt >> logL) << logL);  // whole L-lane groups are active or not
+  float v[K];
+#pragma unroll
+  for (int f = 0; f < K; ++f) v[f] = 0.0f;
+  uint32_t base = 0, j = 0;
+  if (act) {
+    j = tet & (L - 1u);
+    base = B.branch_base + (tet >> logL) * (2u * L) + 1u;  // slot of BFS node 0
+    tet_dop<K>(B, tet, dt, v);
+    store_vol<K>(nodes, base + L - 1u + j, v);
+  }
+  for (uint32_t t = 1; t <= logL; ++t) {
+    const uint32_t s = 1u << (t - 1u);
+    const bool left = (j & s) == 0u;
+#pragma unroll
+    for (int f = 0; f < K; ++f) {
+      const float y = __shfl_xor(v[f], (int)s, 64);
+      const float a = left ? v[f] : y, b = left ? y : v[f];
+      v[f] = (f & 1) ? smax(a, b) : smin(a, b);
+    }
+    if (act && (j & ((s << 1) - 1u)) == 0u) store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), v);
+  }
+}
+
+// Branch chunks that are not full (the last chunk of a body when L does not
+// divide its tet count), one 64-lane block per chunk of the list: leaf DOPs,
+// unions bottom-up in LDS, then one coalesced write of the chunk's volumes.
+template <int K>
+__global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ lbody,
+                                                     const uint32_t* __restrict__ lchunk, uint32_t nlist, float* __restrict__ nodes,
                                                      DirTable dt) {
   constexpr int N = K / 2, S = rec_stride(K);
   __shared__ float vol[63][K];
   __shared__ uint32_t meta0[63], meta1[63];
   const uint32_t g = blockIdx.x;
-  if (g >= cpre[nbatch]) return;
-  const uint32_t item = find_item(cpre, nbatch, g);
-  const BodyDev& B = bodies[batch_body[item]];
-  const uint32_t c = g - cpre[item];
+  if (g >= nlist) return;
+  const BodyDev& B = bodies[lbody[g]];
+  const uint32_t c = lchunk[g];
   const uint32_t L = B.L;
   const uint32_t n = (c + 1 < B.nbranch) ? L : B.nt - c * L;
   const uint32_t nn = 2 * n - 1;
