@@ -63,15 +63,15 @@ def headline_leg(args):
 
     def step():
         ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
-        return ctx.tandem_traversal(pairs, narrow=True), ctx.last_timing()
+        return ctx.tandem_traversal(pairs, narrow=True), ctx.last_timing(), ctx.traversal_stats()
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    trav_ms, narrow_ms, counters = 0.0, 0.0, None
+    trav_ms, narrow_ms, counters, sched = 0.0, 0.0, None, None
     for _ in range(args.steps):
-        counters, tm = step()
+        counters, tm, sched = step()
         trav_ms += tm["traversal_ms"]
         narrow_ms += tm["narrow_ms"]
     torch.cuda.synchronize()
@@ -79,11 +79,46 @@ def headline_leg(args):
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     trav_ms /= args.steps
     narrow_ms /= args.steps
+    # dominant kernel (the persistent rounds): a second timed pass of the same
+    # steps with HIP events attached to each round's dispatch packet on the
+    # context stream (they cost a few us per call, so the pass above runs
+    # without them)
+    ctx.set_option("kernel_timing", 1)
+    step()
+    torch.cuda.synchronize()
+    launch_ms, ptests, launches, round_ms = 0.0, 0, 0, None
+    for _ in range(args.steps):
+        _, tm, st = step()
+        launch_ms += tm["per_launch_ms"] * st["persist_launches"]
+        ptests += st["persist_tests"]
+        launches += st["persist_launches"]
+        r = tm["round_ms"][:st["persist_launches"]]
+        round_ms = r if round_ms is None or len(round_ms) != len(r) else [a + b for a, b in zip(round_ms, r)]
+    torch.cuda.synchronize()
+    ctx.set_option("kernel_timing", 0)
     node_bytes = 4 * args.k + 8
-    alg_bytes = counters["tests"] * 2 * node_bytes + counters["surface_pairs"] * 12
-    achieved = alg_bytes / (trav_ms * 1e-3) / 1e9
+    # algorithmic bytes per launch of the dominant kernel: its node-pair tests x 2 (4K + 8)
+    per_launch_bytes = ptests * 2 * node_bytes / max(launches, 1)
+    per_launch_ms = launch_ms / max(launches, 1)
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     return dict(wl=wl, ctx=ctx, ids=ids, counters=counters, ms_per_step=ms_per_step, trav_ms=trav_ms,
-                narrow_ms=narrow_ms, alg_bytes=alg_bytes, achieved_gbs=achieved, sched=ctx.traversal_stats())
+                narrow_ms=narrow_ms, per_launch_bytes=per_launch_bytes, per_launch_ms=per_launch_ms,
+                launches_per_step=launches / args.steps, achieved_gbs=achieved, sched=sched,
+                round_ms=[x / args.steps for x in round_ms] if round_ms else None)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy calibration);
+    None when no summary for it is present."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return {"bytes": float(f[2]), "source": "profiles/pmc_traffic.txt"}
+    return None
 
 
 def batched_leg(args):
@@ -212,6 +247,8 @@ def main():
         cpu = cpu_leg(args, head)
         log(f"[bench] cpu {cpu['value']:.3e} tests/s, est. full query {cpu['est_full_query_s']:.1f} s")
     steps_per_s = 1e3 / head["ms_per_step"]
+    kernel = f"k_persist<{args.k},{head['sched']['queue_depth']}>"
+    traffic = measured_traffic(kernel)
     out = {
         "metric": METRIC,
         "value": cnt["tests"] * steps_per_s,
@@ -229,9 +266,13 @@ def main():
                    "K": args.k, "chunk_bytes": 8000, "aabb_overlap": round(head["wl"].info["aabb_overlap"], 3),
                    "step": "refit both bodies + tandem traversal + narrow phase"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "traversal (seed + expand launches)", "kernel_ms": head["trav_ms"],
-                     "algorithmic_bytes": head["alg_bytes"]},
+                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "kernel": kernel, "launch_ms": head["per_launch_ms"], "round_ms": head["round_ms"],
+                     "launches_per_step": head["launches_per_step"],
+                     "algorithmic_bytes_per_launch": head["per_launch_bytes"],
+                     "bytes_per_test": 2 * (4 * args.k + 8)},
         "counters": cnt,
         "reference_tests_per_s": cnt["reference_tests"] * steps_per_s,
         "traversal_ms": head["trav_ms"],

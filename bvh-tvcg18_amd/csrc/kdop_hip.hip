@@ -2,6 +2,7 @@
 // resident bodies and trees, the refit / traversal / narrow-phase drivers and
 // the C ABI declared in include/kdop_amd.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -76,11 +77,13 @@ struct kdop_ctx {
   int ls = 512, wpc = 8;
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
+  bool kernel_timing = false;  // time each persistent round by dispatch events
   uint32_t keep = 512;       // items a wave keeps once the frontier is gone
   uint32_t last_rounds = 0;
   bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  double timing[4] = {0, 0, 0, 0};
+  hipEvent_t evr[2 * kMaxRounds] = {};  // start / end of each persistent round
+  double timing[4 + kMaxRounds] = {};  // see kdop_last_timing
   uint32_t last_levels = 0;
 };
 
@@ -332,22 +335,33 @@ struct Bfs {
                        c->dstate, level, target, last);
   }
 };
+// A persistent round, launched with start / stop events attached to its
+// dispatch packet (the events time exactly this kernel, with no extra packets
+// on the stream).
+// A persistent round.  With kernel timing on, start / stop events ride on its
+// dispatch packet and time exactly this kernel.
+template <int K, int LS>
+void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) {
+  const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
+  Item* b0 = c->front[0].p;
+  Item* b1 = c->front[1].p;
+  if (c->kernel_timing)
+    hipExtLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0, b1,
+                          cap, c->dstate, round, c->keep);
+  else
+    hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep);
+}
 template <int K>
 struct PersistRound {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) {
-    const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
-    Item* b0 = c->front[0].p;
-    Item* b1 = c->front[1].p;
     switch (c->ls) {
-      case 256: hipLaunchKernelGGL((k_persist<K, 256>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
-      case 512: hipLaunchKernelGGL((k_persist<K, 512>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
-      case 2048: hipLaunchKernelGGL((k_persist<K, 2048>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
-      default: hipLaunchKernelGGL((k_persist<K, 1024>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep); break;
+      case 256: launch_round<K, 256>(c, T, cap, round); break;
+      case 512: launch_round<K, 512>(c, T, cap, round); break;
+      case 2048: launch_round<K, 2048>(c, T, cap, round); break;
+      default: launch_round<K, 1024>(c, T, cap, round); break;
     }
   }
 };
-
-
 
 }  // namespace
 
@@ -381,6 +395,8 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
   c->own_stream = true;
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
+  for (auto& e : c->evr)
+    if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   if (hipMalloc(&c->dstate, sizeof(CallState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
   if (hipHostMalloc(&c->hsum, sizeof(CallSummary)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
   std::memset(c->hsum, 0, sizeof(CallSummary));
@@ -389,11 +405,13 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
     c->num_cus = prop.multiProcessorCount;
   }
-  if (cfg->queue_depth == 256 || cfg->queue_depth == 512 || cfg->queue_depth == 1024 || cfg->queue_depth == 2048) c->ls = cfg->queue_depth;
-  if (cfg->waves_per_cu > 0) c->wpc = std::min(cfg->waves_per_cu, 32);
-  if (cfg->ramp_target > 0) c->ramp_target = (uint32_t)cfg->ramp_target;
-  if (cfg->stack_keep > 0) c->keep = (uint32_t)cfg->stack_keep;
-  if (cfg->seed_budget > 0) c->seed_budget = (uint32_t)cfg->seed_budget;
+  {
+    const int64_t opts[5][2] = {{KDOP_OPT_QUEUE_DEPTH, cfg->queue_depth}, {KDOP_OPT_WAVES_PER_CU, cfg->waves_per_cu},
+                                {KDOP_OPT_RAMP_TARGET, cfg->ramp_target}, {KDOP_OPT_STACK_KEEP, cfg->stack_keep},
+                                {KDOP_OPT_SEED_BUDGET, cfg->seed_budget}};
+    for (const auto& o : opts)
+      if (o[1] != 0 && kdop_set_option(c, (int)o[0], o[1]) != KDOP_OK) { kdop_destroy(c); return KDOP_ERR_ARG; }
+  }
   *out = c;
   return KDOP_OK;
 }
@@ -413,6 +431,7 @@ void kdop_destroy(kdop_ctx* c) {
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -420,6 +439,67 @@ void kdop_destroy(kdop_ctx* c) {
 const char* kdop_last_error(const kdop_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 void* kdop_stream(kdop_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
+  if (!c) return KDOP_ERR_ARG;
+  switch (option) {
+    case KDOP_OPT_TRAVERSAL:
+      if (v < 0 || v > 2) break;
+      c->cfg.traversal = (int32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_QUEUE_DEPTH:
+      if (v == 0) v = 512;
+      if (v != 256 && v != 512 && v != 1024 && v != 2048) break;
+      c->ls = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_WAVES_PER_CU:
+      if (v == 0) v = 8;
+      if (v < 1 || v > 32) break;
+      c->wpc = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_RAMP_TARGET:
+      if (v < 0 || v > (1ll << 30)) break;
+      c->ramp_target = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_STACK_KEEP:
+      if (v == 0) v = 512;
+      if (v < 1 || v > (1ll << 20)) break;
+      c->keep = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_SEED_BUDGET:
+      if (v == 0) v = 65536;
+      if (v < 1 || v > (1ll << 30)) break;
+      c->seed_budget = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_KERNEL_TIMING:
+      if (v != 0 && v != 1) break;
+      c->kernel_timing = v != 0;
+      return KDOP_OK;
+    case KDOP_OPT_BFS_MAX:
+      if (v == 0) v = 6;
+      if (v < 1 || v > 62) break;
+      c->bfs_max = (uint32_t)v;
+      return KDOP_OK;
+    default:
+      return fail(c, KDOP_ERR_ARG, "kdop_set_option: unknown option %d", option);
+  }
+  return fail(c, KDOP_ERR_ARG, "kdop_set_option: value %lld out of range for option %d", (long long)v, option);
+}
+
+int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
+  if (!c || !v) return KDOP_ERR_ARG;
+  switch (option) {
+    case KDOP_OPT_TRAVERSAL: *v = c->cfg.traversal; return KDOP_OK;
+    case KDOP_OPT_QUEUE_DEPTH: *v = c->ls; return KDOP_OK;
+    case KDOP_OPT_WAVES_PER_CU: *v = c->wpc; return KDOP_OK;
+    case KDOP_OPT_RAMP_TARGET: *v = c->ramp_target; return KDOP_OK;
+    case KDOP_OPT_STACK_KEEP: *v = c->keep; return KDOP_OK;
+    case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget; return KDOP_OK;
+    case KDOP_OPT_KERNEL_TIMING: *v = c->kernel_timing ? 1 : 0; return KDOP_OK;
+    case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
+    default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
+  }
+}
 
 int kdop_tree_shape(int32_t K, uint64_t chunk_bytes, uint64_t nt, uint64_t* out) {
   Shape s;
@@ -657,12 +737,13 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
       dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
-      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
       if (persistent) {
         const uint32_t last = std::min(levels, c->bfs_max);
         for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (uint32_t l = 0; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
       }
     } else {
@@ -719,7 +800,23 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     HIPCHK(c, hipEventElapsedTime(&ms[2], c->ev[0], c->ev[2]));
     HIPCHK(c, hipEventElapsedTime(&ms[3], c->ev[3], c->ev[1]));
     c->timing[0] = ms[0]; c->timing[1] = ms[1]; c->timing[2] = ms[2];
-    c->timing[3] = persistent ? ms[3] : (levels ? ms[3] / (double)levels : 0.0);
+    // dominant kernel: k_persist (each round timed by its own dispatch events) or
+    // k_bfs_level (levels + 1 launches between two events)
+    for (int r = 0; r < kMaxRounds; ++r) c->timing[4 + r] = 0.0;
+    if (persistent && c->kernel_timing) {
+      double sum = 0.0;
+      for (uint32_t r = 0; r < rounds; ++r) {
+        float t = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->evr[2 * r], c->evr[2 * r + 1]));
+        c->timing[4 + r] = t;
+        sum += t;
+      }
+      c->timing[3] = rounds ? sum / (double)rounds : 0.0;
+    } else if (persistent) {
+      c->timing[3] = rounds ? ms[3] / (double)rounds : 0.0;  // includes the gaps between rounds
+    } else {
+      c->timing[3] = ms[3] / (double)(levels + 1);
+    }
     c->last_levels = persistent ? 0 : levels;
     c->last_rounds = persistent ? rounds : 0;
     c->last_persistent = persistent;
@@ -829,17 +926,20 @@ int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint3
 
 int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
   if (!c || !out) return KDOP_ERR_ARG;
-  std::memset(out, 0, 8 * sizeof(uint64_t));
+  std::memset(out, 0, 12 * sizeof(uint64_t));
+  out[10] = (uint64_t)c->ls;
+  out[11] = (uint64_t)c->wpc;
   if (!c->have_traversal || !c->last_persistent) return KDOP_OK;
   const CallSummary& S = *c->hsum;
   out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.total[7];
-  out[4] = S.total[8]; out[5] = S.total[10]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * c->wpc;
+  out[4] = S.total[8]; out[5] = S.total[11]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * c->wpc;
+  out[8] = S.total[10]; out[9] = c->last_rounds;
   return KDOP_OK;
 }
 
 int kdop_last_timing(kdop_ctx* c, double* out) {
   if (!c || !out) return KDOP_ERR_ARG;
-  for (int i = 0; i < 4; ++i) out[i] = c->timing[i];
+  for (int i = 0; i < 4 + kMaxRounds; ++i) out[i] = c->timing[i];
   return KDOP_OK;
 }
 

@@ -243,7 +243,6 @@ __device__ inline void store_vol(float* __restrict__ nodes, uint32_t slot, const
   for (int f = 0; f + 4 <= K; f += 4) *reinterpret_cast<float4*>(r + f) = make_float4(v[f], v[f + 1], v[f + 2], v[f + 3]);
   if (K % 4) *reinterpret_cast<float2*>(r + (K & ~3)) = make_float2(v[K - 2], v[K - 1]);
 }
-
 // Leaf DOP of one tetrahedron (geometry_dop.h:48-74): per slab direction the
 // min / max projection of its four world vertices.
 template <int K>
@@ -268,15 +267,45 @@ __device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& d
   }
 }
 
+// Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16}: DPP row moves within
+// 16-lane rows (full-rate VALU, no LDS), a swizzle for 16.
+template <int S>
+__device__ inline float lane_xor(float v) {
+  const int x = __float_as_int(v);
+  if constexpr (S == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+  if constexpr (S == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+  if constexpr (S == 4 || S == 8) {
+    const int up = __builtin_amdgcn_mov_dpp(x, 0x100 + S, 0xF, 0xF, false);  // row_shl:S, lane i <- i + S
+    const int dn = __builtin_amdgcn_mov_dpp(x, 0x110 + S, 0xF, 0xF, false);  // row_shr:S, lane i <- i - S
+    return __int_as_float((threadIdx.x & S) ? dn : up);
+  }
+  if constexpr (S == 16) return __int_as_float(__builtin_amdgcn_ds_swizzle(x, 0x401F));  // bitmask mode, xor 16 in 32 lanes
+  return v;
+}
+
+// One butterfly level of the chunk unions: lanes j and j ^ S hold the two
+// children (left = the one without bit S) and both end with their union.
+template <int K, int S>
+__device__ inline void union_level(float* v, uint32_t j) {
+  const bool left = (j & S) == 0u;
+#pragma unroll
+  for (int f = 0; f < K; ++f) {
+    const float y = lane_xor<S>(v[f]);
+    const float a = left ? v[f] : y, b = left ? y : v[f];
+    v[f] = (f & 1) ? smax(a, b) : smin(a, b);
+  }
+}
+
 // Full branch chunks, one thread per tetrahedron: the leaf DOP, then the
 // chunk's internal nodes by a butterfly over its L lanes.  A full chunk's
 // median-split tree over L = 2^logL leaves is the implicit heap (BFS node
-// 2^d - 1 + i at depth d, leaf j at L - 1 + j), so level t of the butterfly
-// is depth logL - t, and every union takes (left, right) in the order the
-// reference's bottom-up pass does (geometry_make_union.h:9-25).  tpre: prefix
-// over the batch of the tets in full chunks, each body's share rounded up to
-// 64 so that a wave never mixes bodies (L, a power of two <= 32, then divides
-// every wave's tet range and an aligned L-lane group is one chunk).
+// 2^d - 1 + i at depth d, children of BFS node i at 2i+1 and 2i+2, leaf j at
+// L - 1 + j), so level t of the butterfly is depth logL - t, and every union
+// takes (left, right) in the order the reference's bottom-up pass does
+// (geometry_make_union.h:9-25).  tpre: prefix over the batch of
+// the tets in full chunks, each body's share rounded up to 64 so that a wave
+// never mixes bodies (L, a power of two <= 32, then divides every wave's tet
+// range and an aligned L-lane group is one chunk).
 template <int K>
 __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                       const uint32_t* __restrict__ tpre, uint32_t nbatch, float* __restrict__ nodes,
@@ -299,15 +328,17 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     store_vol<K>(nodes, base + L - 1u + j, v);
   }
   for (uint32_t t = 1; t <= logL; ++t) {
-    const uint32_t s = 1u << (t - 1u);
-    const bool left = (j & s) == 0u;
-#pragma unroll
-    for (int f = 0; f < K; ++f) {
-      const float y = __shfl_xor(v[f], (int)s, 64);
-      const float a = left ? v[f] : y, b = left ? y : v[f];
-      v[f] = (f & 1) ? smax(a, b) : smin(a, b);
+    switch (t) {
+      case 1: union_level<K, 1>(v, j); break;
+      case 2: union_level<K, 2>(v, j); break;
+      case 3: union_level<K, 4>(v, j); break;
+      case 4: union_level<K, 8>(v, j); break;
+      default: union_level<K, 16>(v, j); break;
     }
-    if (act && (j & ((s << 1) - 1u)) == 0u) store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), v);
+    const uint32_t s = 1u << (t - 1u);
+    if (act && (j & ((s << 1) - 1u)) == 0u) {
+      store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), v);  // BFS index 2^d - 1 + i
+    }
   }
 }
 

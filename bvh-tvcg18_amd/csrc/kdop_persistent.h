@@ -312,7 +312,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   wave_lds_sync();
   if (nstage) flush_leaves(T, lp_pair, lp_ta, lp_tb, nstage);
   CounterStripe* st = stripe_of(T.stripes, wave);
-  wave_add64(&st->tests, tests);
+  const unsigned long long wtests = wave_sum64(tests);
+  if (lane == 0 && wtests) {
+    atomicAdd(&st->tests, wtests);
+    atomicAdd(&st->ptests, wtests);
+  }
   wave_add64(&st->descent, descent);
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);

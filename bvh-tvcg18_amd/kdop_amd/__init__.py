@@ -178,6 +178,18 @@ class Context:
         if rc:
             raise KdopError(rc, self._lib.kdop_last_error(self._h).decode())
 
+    # -- options of the live context (kdop_set_option / kdop_get_option)
+    def set_option(self, name, value):
+        self._check(self._lib.kdop_set_option(self._h, _abi.OPTIONS[name], int(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        self._check(self._lib.kdop_get_option(self._h, _abi.OPTIONS[name], ctypes.byref(v)))
+        return v.value
+
+    def options(self):
+        return {k: self.get_option(k) for k in _abi.OPTIONS}
+
     # -- kdop::make_tree
     def make_tree(self, mesh):
         X = np.ascontiguousarray(mesh.X, np.float32)
@@ -255,15 +267,17 @@ class Context:
         return dict(vol=vol, range=rng, kind=kind, root=root)
 
     def traversal_stats(self):
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(12, np.uint64)
         self._check(self._lib.kdop_traversal_stats(self._h, ptr(out)))
-        keys = ("ramp_levels", "frontier", "rounds", "iters", "lanes", "max_iters", "grabs", "waves")
+        keys = ("ramp_levels", "frontier", "rounds", "iters", "lanes", "max_iters", "grabs", "waves",
+                "persist_tests", "persist_launches", "queue_depth", "waves_per_cu")
         return dict(zip(keys, (int(v) for v in out)))
 
     def last_timing(self):
-        out = np.zeros(4, np.float64)
+        out = np.zeros(12, np.float64)
         self._check(self._lib.kdop_last_timing(self._h, ptr(out)))
-        return dict(traversal_ms=out[0], narrow_ms=out[1], total_ms=out[2], per_launch_ms=out[3])
+        return dict(traversal_ms=out[0], narrow_ms=out[1], total_ms=out[2], per_launch_ms=out[3],
+                    round_ms=[float(v) for v in out[4:]])
 
 
 def _quat_rotate_np(q, P):

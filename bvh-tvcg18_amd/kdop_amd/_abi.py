@@ -18,6 +18,9 @@ KDOP_TRAVERSAL_AUTO = 0
 KDOP_TRAVERSAL_LEVELS = 1
 KDOP_TRAVERSAL_PERSISTENT = 2
 
+# kdop_set_option / kdop_get_option (include/kdop_amd.h)
+OPTIONS = {"traversal": 1, "queue_depth": 2, "waves_per_cu": 3, "ramp_target": 4, "stack_keep": 5,
+           "seed_budget": 6, "kernel_timing": 7, "bfs_max": 8}
 KDOP_NARROW = 1
 KDOP_KEEP_ALL_PAIRS = 2
 
@@ -84,6 +87,8 @@ SIGNATURES = [
     ("kdop_narrow_adjoint", ctypes.c_int, [_p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_last_timing", ctypes.c_int, [_p, _p]),
     ("kdop_traversal_stats", ctypes.c_int, [_p, _p]),
+    ("kdop_set_option", ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int64]),
+    ("kdop_get_option", ctypes.c_int, [_p, ctypes.c_int, _p]),
     ("kdop_stream", _p, [_p]),
     ("kdop_box_mesh_size", None, [_u32, _p, _p]),
     ("kdop_make_box_mesh", ctypes.c_int, [_u32, ctypes.c_float, ctypes.c_double, _u64, _p, _p, _p, _p]),

@@ -81,9 +81,12 @@ def large_pair(n=40, K=26, seed=7, jitter=0.1, lo=0.3, hi=0.6):
                         [IDENTITY, Pose(p, q)], dict(n=n, nt=ma.nt, nv=ma.nv, aabb_overlap=frac))
 
 
-def batched_pairs(npairs=512, K=8, seed=11, tmin=1e3, tmax=1e5, separated=0.4, jitter=0.1):
-    """npairs independent body pairs; tet counts log-uniform in [tmin, tmax];
-    poses inside the unit box, about `separated` of the pairs far apart."""
+def batched_pairs(npairs=512, K=8, seed=11, tmin=1e3, tmax=1e5, separated=0.4, jitter=0.1, lo=0.3, hi=0.6):
+    """npairs independent body pairs of jittered boxes (edge 0.25); tet counts
+    log-uniform in [tmin, tmax]; body A placed uniformly in the unit box; about
+    `separated` of the pairs far apart, the others with a seeded rotation and a
+    translation drawn (as for large_pair) until the world-AABB overlap fraction
+    is in [lo, hi]."""
     u = rng_uniform(seed, 0, 2 * npairs)
     counts = np.exp(np.log(tmin) + u * (np.log(tmax) - np.log(tmin)))
     ncubes = np.maximum(1, np.round((counts / 5.0) ** (1.0 / 3.0))).astype(int)
@@ -91,7 +94,8 @@ def batched_pairs(npairs=512, K=8, seed=11, tmin=1e3, tmax=1e5, separated=0.4, j
     meshes, poses = [], []
     qs = rng_quaternions(seed, 1, 2 * npairs)
     sep = rng_uniform(seed, 2, npairs) < separated
-    offs = rng_uniform(seed, 3, 6 * npairs).reshape(npairs, 2, 3)
+    offs = rng_uniform(seed, 3, 3 * npairs).reshape(npairs, 3)
+    fracs = []
     pairs = np.zeros((npairs, 2), np.uint32)
     for i in range(npairs):
         for s in range(2):
@@ -100,16 +104,29 @@ def batched_pairs(npairs=512, K=8, seed=11, tmin=1e3, tmax=1e5, separated=0.4, j
             if key not in cache:
                 cache[key] = box_mesh(nc, size=0.25, jitter=jitter, seed=key[1])
             meshes.append(cache[key])
-        base = offs[i, 0] * 0.5
+        base = (offs[i] * 0.75).astype(np.float32)
+        pa = Pose(base, qs[2 * i])
         if sep[i]:
-            pb = base + np.array([2.0, 0.0, 0.0])
+            pb = Pose((base + np.array([2.0, 0.0, 0.0])).astype(np.float32), qs[2 * i + 1])
         else:
-            pb = base + (offs[i, 1] - 0.5) * 0.2
-        poses.append(Pose(base.astype(np.float32), qs[2 * i]))
-        poses.append(Pose(pb.astype(np.float32), qs[2 * i + 1]))
+            lo_a, hi_a = _world_aabb(meshes[2 * i], pa)
+            draws = rng_uniform(seed, 16 + i, 3 * 256).reshape(-1, 3)
+            best, pb = None, None
+            for t in draws:
+                cand = Pose((base + (t * 2.0 - 1.0) * 0.3).astype(np.float32), qs[2 * i + 1])
+                lo_b, hi_b = _world_aabb(meshes[2 * i + 1], cand)
+                f = aabb_overlap_fraction(lo_a, hi_a, lo_b, hi_b)
+                if best is None or abs(f - 0.5 * (lo + hi)) < abs(best - 0.5 * (lo + hi)):
+                    best, pb = f, cand
+                if lo <= f <= hi:
+                    break
+            fracs.append(best)
+        poses.append(pa)
+        poses.append(pb)
         pairs[i] = (2 * i, 2 * i + 1)
     return PairWorkload(f"batched{npairs}_k{K}", K, meshes, pairs, poses,
-                        dict(tets=[m.nt for m in meshes], separated=int(sep.sum())))
+                        dict(tets=[m.nt for m in meshes], separated=int(sep.sum()),
+                             aabb_overlap=(float(np.min(fracs)), float(np.max(fracs))) if fracs else None))
 
 
 def motion(nbodies, steps, seed=23, dt=1e-3, drift=0.01):

@@ -159,15 +159,40 @@ int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies,
 /* Device wall time (milliseconds) of the phases of the last traversal call,
  * measured with HIP events on the context stream: out[0] traversal kernels,
  * out[1] narrow phase, out[2] whole call.  out[3] = average duration of one
- * launch of the dominant traversal kernel. */
+ * launch of the dominant traversal kernel (persistent strategy: a persistent
+ * round; level-synchronous: a breadth-first level).  out[4..12): with
+ * KDOP_OPT_KERNEL_TIMING on, the time of each persistent round (zeros past the
+ * last round).  out holds 12 doubles. */
 int kdop_last_timing(kdop_ctx* ctx, double* out);
 
 /* Scheduling statistics of the last traversal (persistent strategy; zeros for
- * the level-synchronous one): out[0] ramp levels, out[1] frontier handed to the
- * persistent rounds, out[2] rounds run, out[3] wave loop iterations (sum),
- * out[4] items expanded (sum of active lanes), out[5] most iterations of one
- * wave, out[6] chunks taken, out[7] persistent waves. */
+ * the level-synchronous one), 12 entries: out[0] breadth-first levels before
+ * the persistent rounds, out[1] frontier handed to them, out[2] rounds run,
+ * out[3] wave loop iterations (sum), out[4] items expanded (sum of active
+ * lanes), out[5] most iterations of one wave in one round, out[6] frontier
+ * chunks taken, out[7] persistent waves, out[8] node-pair tests made by the
+ * persistent rounds, out[9] persistent launches; out[10] LDS stack entries
+ * per wave and out[11] waves per CU in effect (set for either strategy). */
 int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
+
+/* Options of a live context (kdop_set_option / kdop_get_option).  The
+ * traversal knobs take effect at the next kdop_traverse; 0 restores the
+ * library default of a knob. */
+#define KDOP_OPT_TRAVERSAL 1      /* KDOP_TRAVERSAL_*                                        */
+#define KDOP_OPT_QUEUE_DEPTH 2    /* LDS stack entries per wave: 256, 512, 1024 or 2048      */
+#define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32                           */
+#define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels        */
+#define KDOP_OPT_STACK_KEEP 5     /* stack entries a wave keeps once the frontier is gone    */
+#define KDOP_OPT_SEED_BUDGET 6    /* chunk-root pairs tested to seed one call                */
+#define KDOP_OPT_KERNEL_TIMING 7  /* 1: time every persistent round by events attached to its */
+                                  /*   dispatch (kdop_last_timing out[3]); costs a few us per call */
+#define KDOP_OPT_BFS_MAX 8        /* most breadth-first levels before the persistent rounds  */
+
+/* Set / read one option (no counterpart in the reference, whose OpenCL
+ * path fixes these sizes at compile time).  Returns KDOP_ERR_ARG for an
+ * unknown option or a value out of range. */
+int kdop_set_option(kdop_ctx* ctx, int option, int64_t value);
+int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
 
 /* Stream of the context (hipStream_t), for callers that order their own work. */
 void* kdop_stream(kdop_ctx* ctx);
