@@ -16,6 +16,7 @@
 #include "kdop_amd.h"
 #include "kdop_kernels.h"
 #include "kdop_persistent.h"
+#include "kdop_adjoint.h"
 #include "kdop_meshgen.h"
 #include "tree_shape.h"
 
@@ -55,6 +56,7 @@ struct kdop_ctx {
   DevBuf<float> nodes;
   size_t nodes_used = 0;
   DevBuf<BodyDev> dbodies;
+  DevBuf<float> dposes;  // 8 floats per body: the pose of its last refit (p, q xyzw, 0)
   std::vector<uint32_t> tops;
   DevBuf<uint32_t> dtops;
   // traversal state
@@ -65,6 +67,8 @@ struct kdop_ctx {
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
   DevBuf<NarrowRec> recs;
+  DevBuf<double> adj_acc;      // adjoint accumulators [npairs][kAdjStripes][kAdjVals]
+  DevBuf<uint32_t> adj_bodies;
   CallState* dstate = nullptr;    // counters and frontier segments of one call
   CallSummary* hsum = nullptr;    // pinned copy of dstate->sum
   uint32_t last_npairs = 0;
@@ -205,6 +209,7 @@ int upload_bodies(kdop_ctx* c) {
     d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.X = b.X; d.Y = b.Y; d.Z = b.Z;
   }
   int rc = ensure(c, c->dbodies, h.size());
+  if (!rc) rc = ensure(c, c->dposes, 8 * h.size(), true);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(c->dbodies.p, h.data(), h.size() * sizeof(BodyDev), hipMemcpyHostToDevice, c->stream));
   rc = ensure(c, c->dtops, c->tops.size());
@@ -308,7 +313,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   if (total_v) {
     const uint32_t blocks = (uint32_t)((total_v + 255) / 256);
     hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, c->stream, c->dbodies.p, P + o_bb, P + o_vpre, n, pos_dev, quat_dev,
-                       identity ? 1 : 0);
+                       identity ? 1 : 0, c->dposes.p);
   }
   if (total_leaf_threads) dispatch_K<RefitLeaves>(c->K, c, (uint32_t)total_leaf_threads, P + o_bb, P + o_tpre, n);
   if (!pbody.empty()) dispatch_K<RefitBranch>(c->K, c, (uint32_t)pbody.size(), P + o_pbody, P + o_pchunk);
@@ -425,9 +430,9 @@ void kdop_destroy(kdop_ctx* c) {
     (void)hipFree(b.X); (void)hipFree(b.Y); (void)hipFree(b.Z);
     (void)hipFree(b.tets); (void)hipFree(b.surf);
   }
-  release(c->nodes); release(c->dbodies); release(c->dtops); release(c->dpairs); release(c->dseeds);
+  release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
-  release(c->contacts); release(c->recs); release(c->plan); release(c->poses);
+  release(c->contacts); release(c->recs); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -943,8 +948,32 @@ int kdop_last_timing(kdop_ctx* c, double* out) {
   return KDOP_OK;
 }
 
-int kdop_narrow_adjoint(kdop_ctx* c, uint32_t, const uint32_t*, const float*, float*, float*, float*) {
-  return fail(c, KDOP_ERR_STATE, "kdop_narrow_adjoint: not available in this build");
+int kdop_narrow_adjoint(kdop_ctx* c, uint32_t nbodies, const uint32_t* bodies, const float* dphi_dev, float* phi_dev,
+                        float* grad_pos_dev, float* grad_quat_dev) {
+  if (!c) return KDOP_ERR_ARG;
+  if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_narrow_adjoint: needs a kdop_traverse with KDOP_NARROW first");
+  if (nbodies && (!bodies || !grad_pos_dev || !grad_quat_dev)) return fail(c, KDOP_ERR_ARG, "kdop_narrow_adjoint: null argument");
+  for (uint32_t i = 0; i < nbodies; ++i)
+    if (bodies[i] >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_narrow_adjoint: unknown body %u", bodies[i]);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t npairs = c->last_npairs;
+  int rc = ensure(c, c->adj_acc, std::max<size_t>((size_t)npairs * kAdjStripes * kAdjVals, 1));
+  if (!rc) rc = ensure(c, c->adj_bodies, std::max<uint32_t>(nbodies, 1));
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->adj_acc.p, 0, (size_t)npairs * kAdjStripes * kAdjVals * sizeof(double), c->stream));
+  if (nbodies) HIPCHK(c, hipMemcpyAsync(c->adj_bodies.p, bodies, nbodies * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  AdjArgs A;
+  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.recs = c->recs.p; A.r_fill = &c->dstate->fills.r;
+  A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
+  A.poses = c->dposes.p; A.dphi = dphi_dev; A.acc = c->adj_acc.p;
+  if (npairs && c->hsum->fills.r) hipLaunchKernelGGL(k_adjoint, dim3((unsigned)(c->num_cus * 4)), dim3(256), 0, c->stream, A);
+  const uint32_t nthreads = std::max(7 * nbodies, npairs);
+  if (nthreads)
+    hipLaunchKernelGGL(k_adjoint_finish, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dpairs.p, npairs, c->adj_acc.p,
+                       c->adj_bodies.p, nbodies, grad_pos_dev, grad_quat_dev, phi_dev);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
 }
 
 }  // extern "C"

@@ -199,9 +199,11 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int K,
 // ============================================================== refit
 // Vertex transform r = rotate(q, r0) + p (narrow_update_kdop_bvh.h:154-165),
 // batched over bodies: item i of the batch covers vertices [vpre[i], vpre[i+1]).
+// The pose of every body is recorded in body_pose (8 floats per body: p,
+// q as x y z w, 0) for the adjoint.
 __global__ void k_transform(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                             const uint32_t* __restrict__ vpre, uint32_t nbatch, const float* __restrict__ pos,
-                            const float* __restrict__ quat, int identity) {
+                            const float* __restrict__ quat, int identity, float* __restrict__ body_pose) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= vpre[nbatch]) return;
   uint32_t lo = 0, hi = nbatch;  // find i with vpre[i] <= g < vpre[i+1]
@@ -211,6 +213,18 @@ __global__ void k_transform(const BodyDev* __restrict__ bodies, const uint32_t* 
   }
   const BodyDev& B = bodies[batch_body[lo]];
   const uint32_t v = g - vpre[lo];
+  if (v == 0u) {
+    float* bp = body_pose + 8 * (size_t)batch_body[lo];
+    if (identity) {
+      bp[0] = bp[1] = bp[2] = 0.f;
+      bp[3] = bp[4] = bp[5] = 0.f;
+      bp[6] = 1.f;
+    } else {
+      bp[0] = pos[3 * lo]; bp[1] = pos[3 * lo + 1]; bp[2] = pos[3 * lo + 2];
+      bp[3] = quat[4 * lo]; bp[4] = quat[4 * lo + 1]; bp[5] = quat[4 * lo + 2]; bp[6] = quat[4 * lo + 3];
+    }
+    bp[7] = 0.f;
+  }
   const f3 r0 = f3{B.X0[v], B.Y0[v], B.Z0[v]};
   if (identity) {  // make_tree refits to the material coordinates themselves
     B.X[v] = r0.x; B.Y[v] = r0.y; B.Z[v] = r0.z;

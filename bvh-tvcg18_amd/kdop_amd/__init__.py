@@ -156,6 +156,8 @@ class Context:
         self.K = K
         self.meshes = []
         self._last_npairs = 0
+        self.device = device
+        self.generation = 0  # bumped by every refit and traversal (the adjoint's inputs)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -210,6 +212,7 @@ class Context:
 
     # -- narrow::update_kdop_bvh
     def refit(self, bodies, pos, quat):
+        self.generation += 1
         b = np.ascontiguousarray(np.atleast_1d(bodies), np.uint32)
         p = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(-1, 3))
         q = np.ascontiguousarray(np.asarray(quat, np.float32).reshape(-1, 4))
@@ -217,6 +220,7 @@ class Context:
         self._check(self._lib.kdop_refit(self._h, b.shape[0], ptr(b), ptr(p), ptr(q)))
 
     def refit_device(self, bodies, pos_ptr, quat_ptr):
+        self.generation += 1
         b = np.ascontiguousarray(np.atleast_1d(bodies), np.uint32)
         self._check(self._lib.kdop_refit_device(self._h, b.shape[0], ptr(b), ctypes.c_void_p(pos_ptr),
                                                 ctypes.c_void_p(quat_ptr)))
@@ -228,7 +232,32 @@ class Context:
         c = Counters()
         self._check(self._lib.kdop_traverse(self._h, P.shape[0], ptr(P), flags, ctypes.byref(c)))
         self._last_npairs = P.shape[0]
+        self.generation += 1
         return c.as_dict()
+
+    # -- adjoint of the penetration measure (kdop_narrow_adjoint)
+    def narrow_adjoint(self, bodies, dphi=None):
+        """Phi per body pair of the last KDOP_NARROW traversal and dL/d(pose)
+        of L = sum(dphi * Phi) for `bodies` (frozen choices, FP64 inside):
+        (phi (npairs,), grad_pos (n, 3), grad_quat (n, 4)) as float32 torch
+        tensors on the context's device.  dphi: torch tensor (npairs,) or None
+        (all ones)."""
+        import torch
+        ids = np.ascontiguousarray(np.asarray(bodies, np.uint32).reshape(-1))
+        n, npairs = ids.shape[0], self._last_npairs
+        dev = torch.device("cuda", self.device)
+        phi = torch.empty(max(npairs, 1), dtype=torch.float32, device=dev)
+        gp = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        gq = torch.empty((max(n, 1), 4), dtype=torch.float32, device=dev)
+        dp = None
+        if dphi is not None:
+            dphi = dphi.detach().to(device=dev, dtype=torch.float32).contiguous()
+            if dphi.numel() != npairs:
+                raise ValueError(f"dphi has {dphi.numel()} entries, the last traversal {npairs} pairs")
+            dp = dphi.data_ptr()
+        torch.cuda.current_stream(dev).synchronize()  # torch work producing dphi is done
+        self._check(self._lib.kdop_narrow_adjoint(self._h, n, ptr(ids), dp, phi.data_ptr(), gp.data_ptr(), gq.data_ptr()))
+        return phi[:npairs], gp[:n], gq[:n]
 
     def pairs(self, which=1):
         """Leaf pairs of the last traversal as an (n, 3) uint32 array (pair, tet_a, tet_b)."""

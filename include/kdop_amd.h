@@ -143,15 +143,18 @@ int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);
 int kdop_export_tree(kdop_ctx* ctx, uint32_t body, uint64_t cap, float* vol, uint32_t* range, uint8_t* kind,
                      float* root, uint64_t* n);
 
-/* Adjoint of the penetration measure Phi = sum over reported contacts of depth,
- * with the contact set and every discrete choice (normal face, contact-point
- * provenance, argmin/argmax) frozen from the last KDOP_NARROW traversal.
- * poses: per body pair used (one entry per body in bodies[]), with
- * dphi: scalar seed dL/dPhi per body pair (device, npairs floats, may be NULL =
- * all ones).  Writes grad_pos (3 per body) and grad_quat (4 per body, x,y,z,w)
- * for the bodies listed in bodies[] (device pointers, overwritten), and
- * phi (device, npairs floats, may be NULL).  The quaternion gradient is that of
- * the literal r = Im(q r0 q*) + p (q not renormalised).
+/* Adjoint of the narrow-phase penetration measure.  Phi(pair) = sum of the
+ * depths of the contacts reported for that body pair by the last KDOP_NARROW
+ * traversal, with every discrete choice of that traversal frozen (contact
+ * set, normal face, provenance of the extreme candidates), evaluated in FP64
+ * from the material coordinates and the poses of the last refit, as a smooth
+ * function of the poses.  For each body in bodies[] (host array) writes
+ * dL/dp (grad_pos: 3 floats per body) and dL/dq (grad_quat: 4 floats, x y z w)
+ * with L = sum over pairs of dphi[pair] * Phi(pair) (dphi: device, one float
+ * per body pair of the traversal, or NULL for all ones); a body in several
+ * pairs gets the sum.  phi (device, one float per pair, may be NULL) receives
+ * Phi.  The quaternion gradient is that of the literal r = Im(q r0 q*) + p (q
+ * not renormalised).  grad_pos / grad_quat / phi are device pointers.
  * No counterpart in the reference (it has no differentiable narrow phase). */
 int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies, const float* dphi_dev,
                         float* phi_dev, float* grad_pos_dev, float* grad_quat_dev);

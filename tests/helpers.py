@@ -66,3 +66,18 @@ def oracle_contacts_with_ordinals(con):
         last = key
         ords[i] = k
     return ords
+
+
+def narrow_records(wl, pair_index=0):
+    """Float32 narrow records of one body pair of a workload, via the oracle."""
+    orc, trees, worlds = oracle_run(wl)
+    a, b = wl.pairs[pair_index]
+    ma, mb = wl.meshes[a], wl.meshes[b]
+    PA = np.stack(worlds[a], 1)
+    PB = np.stack(worlds[b], 1)
+    recs = []
+    for ta_i, tb_i in orc[pair_index][2]:
+        r = O.contacts_pair_codes(PA[ma.tets[ta_i]], PB[mb.tets[tb_i]], ma.surface[ta_i], mb.surface[tb_i])
+        if r["pts"].shape[0]:
+            recs.append((int(ta_i), int(tb_i), r))
+    return recs, PA, PB, ma, mb

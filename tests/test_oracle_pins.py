@@ -10,6 +10,7 @@ import pytest
 
 import kdop_amd as K
 from oracle import oracle as O
+from helpers import narrow_records
 
 F32 = np.float32
 
@@ -383,22 +384,6 @@ def test_contacts_rejected_without_opposing_surfaces():
     assert face == -1 and pts.shape[0] == 0
 
 
-def _narrow_records(wl, pair_index=0):
-    """Float32 narrow records of one body pair of a workload, via the oracle."""
-    from helpers import oracle_run
-    orc, trees, worlds = oracle_run(wl)
-    a, b = wl.pairs[pair_index]
-    ma, mb = wl.meshes[a], wl.meshes[b]
-    PA = np.stack(worlds[a], 1)
-    PB = np.stack(worlds[b], 1)
-    recs = []
-    for ta_i, tb_i in orc[pair_index][2]:
-        r = O.contacts_pair_codes(PA[ma.tets[ta_i]], PB[mb.tets[tb_i]], ma.surface[ta_i], mb.surface[tb_i])
-        if r["pts"].shape[0]:
-            recs.append((int(ta_i), int(tb_i), r))
-    return recs, PA, PB, ma, mb
-
-
 @pytest.mark.parametrize("jitter", [0.0, 0.1])
 def test_float32_contacts_agree_with_frozen_float64(jitter):
     """Every float32 contact vs the FP64 restatement of the same pair with the
@@ -407,7 +392,7 @@ def test_float32_contacts_agree_with_frozen_float64(jitter):
     unjittered grid, where vertices lie exactly on face planes."""
     from kdop_amd import workloads as W
     wl = W.single_pair(n=10, K=8) if jitter == 0.0 else W.large_pair(n=10, K=8, seed=21, jitter=jitter)
-    recs, PA, PB, ma, mb = _narrow_records(wl)
+    recs, PA, PB, ma, mb = narrow_records(wl)
     assert len(recs) > 5
     worst = 0.0
     for ta_i, tb_i, r in recs:
@@ -424,7 +409,7 @@ def test_frozen_gradient_matches_central_differences():
     literally, not renormalised)."""
     from kdop_amd import workloads as W
     wl = W.large_pair(n=8, K=14, seed=17, jitter=0.1)
-    recs, PA, PB, ma, mb = _narrow_records(wl)
+    recs, PA, PB, ma, mb = narrow_records(wl)
     assert len(recs) > 3
     R = np.zeros(len(recs), dtype=[("ta", np.uint32), ("tb", np.uint32), ("face", np.uint8), ("nuniq", np.uint8),
                                    ("cmin", np.uint8), ("cmax", np.uint8)])
