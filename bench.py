@@ -133,6 +133,8 @@ def batched_leg(args):
     dp, dq = device_poses(pose_p, pose_q)
     ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
     reps = max(1, args.batched_reps)
+    # the project's autotuner picks the traversal options on this very call
+    tuned, tuned_ms = ctx.autotune(pairs, narrow=True, budget_ms=args.tune_ms) if args.tune_ms > 0 else (ctx.options(), None)
     # batched: every pair in one traversal call
     for _ in range(2):
         ctx.tandem_traversal(pairs, narrow=True)
@@ -158,7 +160,7 @@ def batched_leg(args):
                 separated_pairs=wl.info["separated"],
                 pair_queries_per_s=n / t_batched, unbatched_pair_queries_per_s=n / t_single,
                 batched_speedup=t_single / t_batched, batched_ms=t_batched * 1e3, unbatched_ms=t_single * 1e3,
-                tests=c["tests"], contacts=c["contacts"])
+                tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms)
 
 
 def cpu_leg(args, head):
@@ -201,6 +203,7 @@ def main():
     ap.add_argument("--batched-pairs", type=int, default=512)
     ap.add_argument("--batched-k", type=int, default=8)
     ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--tune-ms", type=float, default=3000.0, help="autotuner budget for the batched leg (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traversal", type=int, default=0, help="0 auto, 1 level-synchronous, 2 persistent")

@@ -5,6 +5,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstddef>
@@ -491,6 +492,65 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
   return fail(c, KDOP_ERR_ARG, "kdop_set_option: value %lld out of range for option %d", (long long)v, option);
 }
 
+int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, double budget_ms, int64_t* chosen,
+                  double* best_ms) {
+  if (!c) return KDOP_ERR_ARG;
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  auto elapsed_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t_start).count(); };
+  // median wall time of 3 calls with the context's current options
+  auto measure = [&](double* ms) -> int {
+    double t[3];
+    for (int r = 0; r < 3; ++r) {
+      const auto a = clk::now();
+      const int rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr);
+      if (rc) return rc;
+      t[r] = std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    }
+    std::sort(t, t + 3);
+    *ms = t[1];
+    return KDOP_OK;
+  };
+  int rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr);  // warm up (buffers sized)
+  if (rc) return rc;
+  double best = 0.0;
+  if ((rc = measure(&best))) return rc;
+  const int64_t waves = (int64_t)c->num_cus;
+  struct Knob {
+    int opt;
+    std::vector<int64_t> values;
+  };
+  const std::vector<Knob> knobs = {
+      {KDOP_OPT_TRAVERSAL, {KDOP_TRAVERSAL_PERSISTENT, KDOP_TRAVERSAL_LEVELS}},
+      {KDOP_OPT_WAVES_PER_CU, {4, 8, 12}},
+      {KDOP_OPT_QUEUE_DEPTH, {256, 512, 1024}},
+      {KDOP_OPT_STACK_KEEP, {128, 512, 2048}},
+      {KDOP_OPT_RAMP_TARGET, {4 * waves * 8, 16 * waves * 8, 64 * waves * 8}},
+      {KDOP_OPT_SEED_BUDGET, {16384, 65536, 262144}},
+  };
+  for (const Knob& k : knobs) {
+    int64_t cur = 0;
+    kdop_get_option(c, k.opt, &cur);
+    int64_t best_v = cur;
+    for (const int64_t v : k.values) {
+      if (v == cur) continue;
+      if (elapsed_ms() > budget_ms) break;
+      if (kdop_set_option(c, k.opt, v) != KDOP_OK) continue;
+      double ms = 0.0;
+      if ((rc = measure(&ms))) { kdop_set_option(c, k.opt, cur); return rc; }
+      if (ms < best) { best = ms; best_v = v; }
+    }
+    kdop_set_option(c, k.opt, best_v);
+    // the levels strategy ignores the persistent knobs
+    if (k.opt == KDOP_OPT_TRAVERSAL && best_v == KDOP_TRAVERSAL_LEVELS) break;
+  }
+  if ((rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr))) return rc;  // results of the chosen options
+  if (chosen)
+    for (int o = 1; o <= 8; ++o) kdop_get_option(c, o, &chosen[o - 1]);
+  if (best_ms) *best_ms = best;
+  return KDOP_OK;
+}
+
 int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
   if (!c || !v) return KDOP_ERR_ARG;
   switch (option) {
@@ -713,7 +773,13 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (npairs) HIPCHK(c, hipMemcpyAsync(c->dseeds.p, seeds.data(), npairs * sizeof(SeedDesc), hipMemcpyHostToDevice, c->stream));
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
-  const bool persistent = c->cfg.traversal != KDOP_TRAVERSAL_LEVELS;
+  // AUTO: the persistent strategy where its depth-first waves beat one launch
+  // per level (tight DOPs, up to a few million tets per call); the
+  // level-synchronous one for wide frontiers (small K, big batches)
+  uint64_t call_tets = 0;
+  for (uint32_t i = 0; i < npairs; ++i) call_tets += (uint64_t)c->bodies[pairs[i].x].nt + c->bodies[pairs[i].y].nt;
+  const bool persistent = c->cfg.traversal == KDOP_TRAVERSAL_PERSISTENT ||
+                          (c->cfg.traversal == KDOP_TRAVERSAL_AUTO && c->K >= 14 && call_tets <= (2ull << 20));
   est = std::max<uint64_t>(est, 1 << 16);
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;

@@ -192,6 +192,18 @@ class Context:
     def options(self):
         return {k: self.get_option(k) for k in _abi.OPTIONS}
 
+    def autotune(self, pairs, narrow=True, budget_ms=2000.0):
+        """kdop_autotune on this pair list: returns (chosen options, best median ms)."""
+        P = np.ascontiguousarray(np.asarray(pairs, np.uint32).reshape(-1, 2))
+        chosen = np.zeros(8, np.int64)
+        best = ctypes.c_double()
+        self._check(self._lib.kdop_autotune(self._h, P.shape[0], ptr(P), KDOP_NARROW if narrow else 0, float(budget_ms),
+                                            ptr(chosen), ctypes.byref(best)))
+        self._last_npairs = P.shape[0]
+        self.generation += 1
+        names = sorted(_abi.OPTIONS, key=_abi.OPTIONS.get)
+        return dict(zip(names, (int(v) for v in chosen))), best.value
+
     # -- kdop::make_tree
     def make_tree(self, mesh):
         X = np.ascontiguousarray(mesh.X, np.float32)

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("kdop_last_timing", ctypes.c_int, [_p, _p]),
     ("kdop_traversal_stats", ctypes.c_int, [_p, _p]),
     ("kdop_set_option", ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int64]),
+    ("kdop_autotune", ctypes.c_int, [_p, _u32, _p, _u32, ctypes.c_double, _p, _p]),
     ("kdop_get_option", ctypes.c_int, [_p, ctypes.c_int, _p]),
     ("kdop_stream", _p, [_p]),
     ("kdop_box_mesh_size", None, [_u32, _p, _p]),

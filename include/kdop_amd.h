@@ -31,7 +31,8 @@ extern "C" {
 typedef struct kdop_ctx kdop_ctx;
 
 /* Traversal strategies (kdop_config.traversal). */
-#define KDOP_TRAVERSAL_AUTO 0        /* autotuned choice                                   */
+#define KDOP_TRAVERSAL_AUTO 0        /* persistent for K >= 14 and <= 2M tets per call,    */
+                                     /* level-synchronous otherwise (kdop_autotune refines) */
 #define KDOP_TRAVERSAL_LEVELS 1      /* level-synchronous frontier kernels                 */
 #define KDOP_TRAVERSAL_PERSISTENT 2  /* one persistent launch, wave stacks + work stealing */
 
@@ -196,6 +197,18 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
  * unknown option or a value out of range. */
 int kdop_set_option(kdop_ctx* ctx, int option, int64_t value);
 int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
+
+/* Autotuner: picks the traversal options (strategy, waves per CU, LDS queue
+ * depth, stack keep, breadth-first target, seed budget) for this context by
+ * timing kdop_traverse(npairs, body_pairs, flags) itself: coordinate descent
+ * over the options in that order, each candidate measured as the median host
+ * wall time of 3 calls, until budget_ms of tuning time is spent.  The best
+ * options stay set on the context, and the last traversal's results are those
+ * of a run with them.  chosen (may be NULL) receives 8 values indexed by
+ * KDOP_OPT_* - 1; *best_ms (may be NULL) the median time of the best.
+ * No counterpart in the reference (its OpenCL sizes are compile-time). */
+int kdop_autotune(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, double budget_ms, int64_t* chosen,
+                  double* best_ms);
 
 /* Stream of the context (hipStream_t), for callers that order their own work. */
 void* kdop_stream(kdop_ctx* ctx);

@@ -30,12 +30,13 @@ def oracle_run(wl, narrow=True, threads=1):
     return out, trees, worlds
 
 
-def device_run(wl, narrow=True, keep_all=True, ctx=None, **ctx_kw):
-    """Build, refit and traverse a workload on the device."""
+def device_run(wl, narrow=True, keep_all=True, ctx=None, ids=None, **ctx_kw):
+    """Build (unless the body ids are given), refit and traverse a workload on the device."""
     own = ctx is None
     if own:
         ctx = K.Context(K=wl.K, **ctx_kw)
-    ids = [ctx.make_tree(m) for m in wl.meshes]
+    if ids is None:
+        ids = [ctx.make_tree(m) for m in wl.meshes]
     ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
     pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
     cnt = ctx.tandem_traversal(pairs, narrow=narrow, keep_all_pairs=keep_all)

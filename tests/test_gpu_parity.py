@@ -175,3 +175,18 @@ def test_refit_series_small():
         assert cnt["reference_tests"] == c["tests"]
         np.testing.assert_array_equal(sort_pairs(ctx.pairs(0)[:, 1:]), sort_pairs(p1))
         assert cnt["contacts"] == c["contacts"]
+
+
+def test_autotuned_batched_parity():
+    """The autotuner's final run (its chosen options) reports the oracle's
+    results, and the options it leaves on the context are valid ones."""
+    wl = W.batched_pairs(npairs=10, K=8, seed=14, tmin=300, tmax=4000)
+    orc, _, _ = oracle_run(wl)
+    ctx = K.Context(K=8)
+    ids = [ctx.make_tree(m) for m in wl.meshes]
+    ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
+    pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
+    chosen, best_ms = ctx.autotune(pairs, narrow=True, budget_ms=1500)
+    assert best_ms > 0 and chosen["queue_depth"] in (256, 512, 1024, 2048) and 1 <= chosen["waves_per_cu"] <= 32
+    dev = device_run(wl, ctx=ctx, ids=ids)
+    _check_pair_results(wl, dev, orc)
