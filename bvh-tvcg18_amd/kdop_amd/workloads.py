@@ -159,3 +159,31 @@ def _qmul(a, b):
                      aw * by - ax * bz + ay * bw + az * bx,
                      aw * bz + ax * by - ay * bx + az * bw,
                      aw * bw - ax * bx - ay * by - az * bz], axis=1)
+
+
+def refit_series(npairs=64, K=18, steps=200, seed=41, tmin=2e4, tmax=8e4, dt=1e-3, drift=0.01):
+    """Refit + re-traverse time series: npairs overlapping body pairs (tet
+    counts log-uniform in [tmin, tmax], poses of batched_pairs with no
+    separated pairs), then `steps` steps of seeded rigid motion per body:
+    constant angular velocity w ~ N(0, 1) rad/s per axis, q(s) = q0 * exp(w s dt / 2),
+    and a drift of `drift` per step along a seeded unit direction.
+    Returns (workload at step 0, P (steps, nbodies, 3), Q (steps, nbodies, 4))."""
+    wl = batched_pairs(npairs=npairs, K=K, seed=seed, tmin=tmin, tmax=tmax, separated=0.0)
+    nb = len(wl.meshes)
+    w = rng_normal(seed, 5, 3 * nb).reshape(nb, 3)
+    d = rng_normal(seed, 6, 3 * nb).reshape(nb, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p0 = np.stack([p.p for p in wl.poses]).astype(np.float64)
+    q0 = np.stack([p.q for p in wl.poses]).astype(np.float64)
+    wn = np.linalg.norm(w, axis=1)
+    ax = w / np.maximum(wn[:, None], 1e-30)
+    P = np.empty((steps, nb, 3), np.float32)
+    Q = np.empty((steps, nb, 4), np.float32)
+    for s in range(steps):
+        ang = wn * (s * dt) / 2
+        dq = np.concatenate([ax * np.sin(ang)[:, None], np.cos(ang)[:, None]], axis=1)
+        q = _qmul(q0, dq)
+        Q[s] = q / np.linalg.norm(q, axis=1, keepdims=True)
+        P[s] = p0 + drift * s * d
+    wl.name = f"refit_series{npairs}_k{K}"
+    return wl, P, Q

@@ -1,0 +1,127 @@
+"""Parity at the benchmark configurations' real sizes, and edge cases.
+
+Every comparison is against the CPU oracle on the same inputs, to the same
+bar as tests/test_gpu_parity.py: bit-exact pair sets (after sorting), node-pair
+test counts and contact values.
+"""
+import numpy as np
+import pytest
+
+import kdop_amd as K
+from kdop_amd import workloads as W
+from helpers import device_run, oracle_run, sort_pairs, world_coords
+from oracle import oracle as O
+from test_gpu_parity import _check_pair_results
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_THREADS = 8
+
+
+@pytest.mark.parametrize("k", [26, 6, 14, 18])
+def test_c2_full_size(k):
+    """Two jittered 40^3 boxes (320,000 tets each), the headline pose: full
+    parity (P1, P2, counts incl. the reference's 1.6e9-test count at K=26,
+    contacts) for the headline K and the comparison K values."""
+    wl = W.large_pair(n=40, K=k, seed=7)
+    orc, _, _ = oracle_run(wl, threads=ORACLE_THREADS)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+    if k == 26:
+        assert dev["counters"]["reference_tests"] == 1603611489
+
+
+def test_c3_batched_subset_full_sizes():
+    """128 pairs drawn like the batched benchmark (1k-100k tets, ~40 %
+    separated, 30-60 % AABB overlap), one call, both traversal strategies."""
+    wl = W.batched_pairs(npairs=128, K=8, seed=12)
+    orc, _, _ = oracle_run(wl, threads=ORACLE_THREADS)
+    for mode in (K.KDOP_TRAVERSAL_PERSISTENT, K.KDOP_TRAVERSAL_LEVELS):
+        dev = device_run(wl, keep_all=False, traversal=mode)
+        _check_pair_results(wl, dev, orc)
+
+
+def test_c4_refit_series():
+    """64 overlapping pairs (20k-80k tets, K=18), 200 steps of rigid motion,
+    refit (never rebuilt) and re-traversed every step on the device; the
+    pair lists, counts and contacts of steps 0, 7, 40 and 199 equal the
+    oracle's, and every step reports a consistent count set."""
+    wl, P, Q = W.refit_series(npairs=64, K=18, steps=200)
+    ctx = K.Context(K=18)
+    ids = np.array([ctx.make_tree(m) for m in wl.meshes], np.uint32)
+    pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
+    check = {0, 7, 40, 199}
+    for s in range(P.shape[0]):
+        ctx.refit(ids, P[s], Q[s])
+        cnt = ctx.tandem_traversal(pairs, narrow=True, keep_all_pairs=s in check)
+        assert cnt["surface_pairs"] <= cnt["leaf_pairs"] and cnt["narrow_pairs"] <= cnt["surface_pairs"]
+        if s not in check:
+            continue
+        wl.poses = [W.Pose(P[s, i], Q[s, i]) for i in range(len(wl.meshes))]
+        orc, _, _ = oracle_run(wl, threads=ORACLE_THREADS)
+        dev = dict(counters=cnt, p1=ctx.pairs(0), p2=ctx.pairs(1), contacts=ctx.contacts())
+        _check_pair_results(wl, dev, orc)
+
+
+def _tet_mesh(points, tets):
+    pts = np.asarray(points, np.float32)
+    T = np.asarray(tets, np.uint32).reshape(-1, 4)
+    return K.Mesh(pts[:, 0].copy(), pts[:, 1].copy(), pts[:, 2].copy(), T, K.compute_surface_map(pts.shape[0], T))
+
+
+def test_empty_call_and_single_tet_bodies():
+    """No pairs at all; then two one-tet bodies (L = 1, one chunk, one level)
+    that overlap, that touch only by translation, and one against itself."""
+    ctx = K.Context(K=8)
+    assert ctx.tandem_traversal(np.zeros((0, 2), np.uint32))["tests"] == 0
+    assert ctx.pairs(1).shape[0] == 0 and ctx.contacts().shape[0] == 0
+    tet = _tet_mesh([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], [[0, 1, 2, 3]])
+    for dp in ([0.1, 0.1, 0.1], [0.3, 0.0, 0.0], [5.0, 0.0, 0.0]):
+        wl = W.PairWorkload("tets", 8, [tet, tet], np.array([[0, 1]], np.uint32),
+                            [W.IDENTITY, W.Pose(np.array(dp, np.float32), np.array([0, 0, 0, 1], np.float32))])
+        orc, _, _ = oracle_run(wl)
+        _check_pair_results(wl, device_run(wl), orc)
+    wl = W.PairWorkload("self", 8, [tet], np.array([[0, 0]], np.uint32), [W.IDENTITY])
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+    assert dev["counters"]["leaf_pairs"] == 1
+
+
+@pytest.mark.parametrize("k", [8, 26])
+def test_chunk_boundary_tet_counts(k):
+    """Bodies whose tet counts sit on chunk boundaries (L - 1, L, L + 1,
+    2L - 1, 2L + 1 for the body's own L), cut from a jittered box in Morton
+    order: partial and full last chunks, one and two chunk levels."""
+    base = K.box_mesh(6, size=1.0, jitter=0.1, seed=3)
+    meshes = []
+    for nt in (7, 8, 9, 15, 17, 31, 32, 33, 63, 65, 500, 513):
+        T = base.tets[:nt]
+        meshes.append(K.Mesh(base.X, base.Y, base.Z, T, K.compute_surface_map(base.nv, T)))
+    q = K.rng_quaternions(5, 0, len(meshes))
+    poses = [W.Pose(np.array([0.02 * i, 0.01, 0.0], np.float32), q[i]) for i in range(len(meshes))]
+    pairs = np.array([[i, (i + 1) % len(meshes)] for i in range(len(meshes))], np.uint32)
+    wl = W.PairWorkload("boundary", k, meshes, pairs, poses)
+    orc, trees, worlds = oracle_run(wl)
+    dev = device_run(wl)
+    _check_pair_results(wl, dev, orc)
+    ctx = dev["ctx"]
+    for i, bid in enumerate(dev["ids"]):   # every tree volume, bit for bit
+        t = ctx.export_tree(bid)
+        np.testing.assert_array_equal(t["root"], trees[i].root())
+        ref = {(int(f), int(l)): v for (f, l), v in zip(trees[i].export()["range"], trees[i].export()["vol"])}
+        for (f, l), v in zip(t["range"], t["vol"]):
+            np.testing.assert_array_equal(v, ref[(int(f), int(l))])
+
+
+def test_invalid_inputs_fail_loudly():
+    """Unknown body ids and out-of-range options are errors, not crashes."""
+    ctx = K.Context(K=8)
+    m = K.box_mesh(2)
+    a = ctx.make_tree(m)
+    with pytest.raises(K.KdopError):
+        ctx.tandem_traversal([[a, a + 5]])
+    with pytest.raises(K.KdopError):
+        ctx.set_option("queue_depth", 300)
+    with pytest.raises(K.KdopError):
+        ctx.narrow_adjoint([a])   # no narrow traversal yet
