@@ -60,6 +60,9 @@ def headline_leg(args):
                      ramp_target=args.ramp_target, keep=args.keep, seed_budget=args.seed_budget)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
+    # the step chains refit and traversal on the context stream: the refit
+    # returns once queued and the traversal call's completion covers both
+    ctx.set_option("async_refit", 1)
 
     def step():
         ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
@@ -69,26 +72,24 @@ def headline_leg(args):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    trav_ms, narrow_ms, counters, sched = 0.0, 0.0, None, None
+    counters, sched = None, None
     for _ in range(args.steps):
-        counters, tm, sched = step()
-        trav_ms += tm["traversal_ms"]
-        narrow_ms += tm["narrow_ms"]
+        counters, _, sched = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    trav_ms /= args.steps
-    narrow_ms /= args.steps
-    # dominant kernel (the persistent rounds): a second timed pass of the same
-    # steps with HIP events attached to each round's dispatch packet on the
-    # context stream (they cost a few us per call, so the pass above runs
-    # without them)
-    ctx.set_option("kernel_timing", 1)
+    # phase times and the dominant kernel (the persistent rounds): a second
+    # timed pass of the same steps with HIP events on the context stream (phase
+    # events, and one pair attached to each round's dispatch packet); events
+    # cost a few us each, so the pass above runs without them
+    ctx.set_option("kernel_timing", 2)
     step()
     torch.cuda.synchronize()
-    launch_ms, ptests, launches, round_ms = 0.0, 0, 0, None
+    trav_ms, narrow_ms, launch_ms, ptests, launches, round_ms = 0.0, 0.0, 0.0, 0, 0, None
     for _ in range(args.steps):
         _, tm, st = step()
+        trav_ms += tm["traversal_ms"]
+        narrow_ms += tm["narrow_ms"]
         launch_ms += tm["per_launch_ms"] * st["persist_launches"]
         ptests += st["persist_tests"]
         launches += st["persist_launches"]
@@ -96,6 +97,8 @@ def headline_leg(args):
         round_ms = r if round_ms is None or len(round_ms) != len(r) else [a + b for a, b in zip(round_ms, r)]
     torch.cuda.synchronize()
     ctx.set_option("kernel_timing", 0)
+    trav_ms /= args.steps
+    narrow_ms /= args.steps
     node_bytes = 4 * args.k + 8
     # algorithmic bytes per launch of the dominant kernel: its node-pair tests x 2 (4K + 8)
     per_launch_bytes = ptests * 2 * node_bytes / max(launches, 1)

@@ -135,8 +135,7 @@ struct AdjArgs {
   const BodyDev* bodies;
   const uint2* pairs;
   const NarrowRec* recs;
-  const uint32_t* r_fill;
-  uint32_t r_cap;
+  uint32_t nrec;
   const float* poses;  // 8 floats per body
   const float* dphi;   // per body pair, or null (all ones)
   double* acc;         // [npairs][kAdjStripes][kAdjVals]
@@ -147,7 +146,7 @@ struct AdjArgs {
 // records belong to one pair, the usual case since a pair's records are
 // written together).
 __global__ __launch_bounds__(256) void k_adjoint(AdjArgs A) {
-  const uint32_t n = min(*A.r_fill, A.r_cap);
+  const uint32_t n = A.nrec;
   const uint32_t lane = lane_id();
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;

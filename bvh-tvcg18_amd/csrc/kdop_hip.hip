@@ -82,7 +82,16 @@ struct kdop_ctx {
   int ls = 512, wpc = 8;
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
-  bool kernel_timing = false;  // time each persistent round by dispatch events
+  int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
+  bool async_refit = false;    // refit returns once queued (the next call orders after it)
+  bool state_clean = false;    // CallState zeroed (by the last call's final fold)
+  std::vector<uint32_t> plan_key;   // bodies of the refit plan in c->plan
+  std::vector<uint32_t> plan_host;  // that plan (host copy)
+  std::vector<uint2> seed_key;      // pair list whose pairs / seeds are on the device
+  uint32_t seed_key_budget = 0;
+  size_t seed_key_bodies = 0;
+  uint64_t seed_total = 0;
+  std::vector<uint64_t> seed_plan_est;  // per pair: seeds tested
   uint32_t keep = 512;       // items a wave keeps once the frontier is gone
   uint32_t last_rounds = 0;
   bool last_persistent = false;
@@ -252,7 +261,8 @@ struct RefitRoot {
 // pos/quat: device pointers (3n, 4n floats) or null with identity.
 int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
   // plan layout (uint32): [bb n][vpre n+1][tpre n+1][partial chunks: body m,
-  // chunk m] then per super step s: [bb m][cpre m+1][cbase m][cn m]
+  // chunk m] then per super step s: [bb m][cpre m+1][cbase m][cn m].  It only
+  // depends on the body list, so the device copy is reused while that repeats.
   std::vector<uint32_t> plan;
   plan.reserve(8 * (size_t)n + 16);
   uint32_t maxH = 1;
@@ -307,9 +317,14 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     st.o_cn = plan.size(); plan.insert(plan.end(), cn.begin(), cn.end());
     steps.push_back(st);
   }
-  int rc = ensure(c, c->plan, plan.size());
-  if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  const bool same = c->plan_key.size() == n && std::equal(bodies, bodies + n, c->plan_key.begin()) && c->plan_host == plan;
+  if (!same) {
+    int rc = ensure(c, c->plan, plan.size());
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    c->plan_key.assign(bodies, bodies + n);
+    c->plan_host = plan;
+  }
   const uint32_t* P = c->plan.p;
   if (total_v) {
     const uint32_t blocks = (uint32_t)((total_v + 255) / 256);
@@ -322,7 +337,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     dispatch_K<RefitSuper>(c->K, c, st.chunks, P + st.o_bb, P + st.o_cpre, P + st.o_cbase, P + st.o_cn, st.m);
   dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
 }
 
@@ -351,7 +366,7 @@ void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) 
   const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
-  if (c->kernel_timing)
+  if (c->timing_level >= 2)
     hipExtLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0, b1,
                           cap, c->dstate, round, c->keep);
   else
@@ -478,8 +493,12 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->seed_budget = (uint32_t)v;
       return KDOP_OK;
     case KDOP_OPT_KERNEL_TIMING:
+      if (v < 0 || v > 2) break;
+      c->timing_level = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_ASYNC_REFIT:
       if (v != 0 && v != 1) break;
-      c->kernel_timing = v != 0;
+      c->async_refit = v != 0;
       return KDOP_OK;
     case KDOP_OPT_BFS_MAX:
       if (v == 0) v = 6;
@@ -546,7 +565,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   if ((rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr))) return rc;  // results of the chosen options
   if (chosen)
-    for (int o = 1; o <= 8; ++o) kdop_get_option(c, o, &chosen[o - 1]);
+    for (int o = 1; o <= 8; ++o) kdop_get_option(c, o, &chosen[o - 1]);  // the traversal options
   if (best_ms) *best_ms = best;
   return KDOP_OK;
 }
@@ -560,7 +579,8 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_RAMP_TARGET: *v = c->ramp_target; return KDOP_OK;
     case KDOP_OPT_STACK_KEEP: *v = c->keep; return KDOP_OK;
     case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget; return KDOP_OK;
-    case KDOP_OPT_KERNEL_TIMING: *v = c->kernel_timing ? 1 : 0; return KDOP_OK;
+    case KDOP_OPT_KERNEL_TIMING: *v = c->timing_level; return KDOP_OK;
+    case KDOP_OPT_ASYNC_REFIT: *v = c->async_refit ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
@@ -715,11 +735,15 @@ NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round) {
 }
 
 // Fold the call's counters on the device and copy the summary to the host.
-int summarize(kdop_ctx* c, uint32_t cap) {
-  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(64), 0, c->stream, c->dstate, cap);
+// final_round >= 0: the fold also zeroes the call state (for the next call,
+// in place of a memset) when the call is complete: no overflow and nothing
+// left in that round's segments (-1: levels strategy, no rounds).  -2: never.
+int summarize(kdop_ctx* c, uint32_t cap, int final_round) {
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(64), 0, c->stream, c->dstate, cap, final_round);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->hsum, &c->dstate->sum, sizeof(CallSummary), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->state_clean = c->hsum->reset != 0;
   return KDOP_OK;
 }
 
@@ -743,14 +767,22 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
   c->have_traversal = c->have_p1 = c->have_narrow = false;
-  int rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1));
-  if (rc) return rc;
-  if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
-  // seed levels: per body pair the deepest pair of chunk levels whose root
-  // pairs fit the per-pair share of the seed budget
-  std::vector<SeedDesc> seeds(npairs);
+  int rc = KDOP_OK;
+  // pair list and seed plan on the device: uploaded again only when they change
+  const bool same_pairs = c->seed_key.size() == pairs.size() &&
+                          (pairs.empty() || std::memcmp(c->seed_key.data(), pairs.data(), pairs.size() * sizeof(uint2)) == 0) &&
+                          c->seed_key_budget == c->seed_budget && c->seed_key_bodies == c->bodies.size();
   uint64_t total_seeds = 0;
-  {
+  if (same_pairs) {
+    total_seeds = c->seed_total;
+    for (uint32_t i = 0; i < npairs; ++i) est += c->seed_plan_est[i];
+  } else {
+    if ((rc = ensure(c, c->dpairs, std::max<uint32_t>(npairs, 1)))) return rc;
+    if (npairs) HIPCHK(c, hipMemcpyAsync(c->dpairs.p, pairs.data(), npairs * sizeof(uint2), hipMemcpyHostToDevice, c->stream));
+    // seed levels: per body pair the deepest pair of chunk levels whose root
+    // pairs fit the per-pair share of the seed budget
+    std::vector<SeedDesc> seeds(npairs);
+    c->seed_plan_est.assign(npairs, 0);
     const uint64_t share = std::max<uint64_t>(1, c->seed_budget / std::max<uint32_t>(npairs, 1));
     for (uint32_t i = 0; i < npairs; ++i) {
       const BodyHost& A = c->bodies[pairs[i].x];
@@ -765,12 +797,17 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
         }
       seeds[i] = SeedDesc{A.roots_off[ha], A.C[ha], B.roots_off[hb], B.C[hb], (uint32_t)total_seeds};
       total_seeds += best;
+      c->seed_plan_est[i] = best;
       est += best;
     }
+    if (total_seeds >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_traverse: too many seed pairs");
+    if ((rc = ensure(c, c->dseeds, std::max<uint32_t>(npairs, 1)))) return rc;
+    if (npairs) HIPCHK(c, hipMemcpyAsync(c->dseeds.p, seeds.data(), npairs * sizeof(SeedDesc), hipMemcpyHostToDevice, c->stream));
+    c->seed_key = pairs;
+    c->seed_key_budget = c->seed_budget;
+    c->seed_key_bodies = c->bodies.size();
+    c->seed_total = total_seeds;
   }
-  if (total_seeds >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_traverse: too many seed pairs");
-  if ((rc = ensure(c, c->dseeds, std::max<uint32_t>(npairs, 1)))) return rc;
-  if (npairs) HIPCHK(c, hipMemcpyAsync(c->dseeds.p, seeds.data(), npairs * sizeof(SeedDesc), hipMemcpyHostToDevice, c->stream));
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
   // AUTO: the persistent strategy where its depth-first waves beat one launch
@@ -794,7 +831,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   uint64_t overflowed = 0;
   CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {
-    HIPCHK(c, hipMemsetAsync(d, 0, offsetof(CallState, sum), c->stream));
+    if (!c->state_clean) HIPCHK(c, hipMemsetAsync(d, 0, offsetof(CallState, sum), c->stream));
+    c->state_clean = false;
     TravArgs T;
     T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p;
     T.stripes = d->stripe; T.overflow = &d->overflow;
@@ -805,29 +843,29 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.npairs = npairs;
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
     uint32_t rounds = 0;
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
       dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
       if (persistent) {
         const uint32_t last = std::min(levels, c->bfs_max);
         for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (uint32_t l = 0; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
       }
     } else {
-      HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     if (narrow && npairs) {
       hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, persistent ? (int)rounds : -1));
       HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if ((rc = summarize(c, cap))) return rc;
+    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if ((rc = summarize(c, cap, persistent ? (int)rounds : -1))) return rc;
     bool grow_front = (c->hsum->overflow & 1u) != 0;
     if (persistent && !grow_front && npairs && c->hsum->round_total[rounds] > 0) {
       // spill left after the queued rounds (the narrow phase above saw it and
@@ -837,16 +875,16 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
         dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
         ++rounds;
         HIPCHK(c, hipGetLastError());
-        if ((rc = summarize(c, cap))) return rc;
+        if ((rc = summarize(c, cap, -2))) return rc;
       }
       grow_front = (c->hsum->overflow & 1u) != 0;
-      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
       if (narrow && !grow_front) {
         hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, (int)rounds));
         HIPCHK(c, hipGetLastError());
       }
-      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-      if ((rc = summarize(c, cap))) return rc;
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      if ((rc = summarize(c, cap, (int)rounds))) return rc;
     }
     const CallSummary& h = *c->hsum;
     const Fills& f = h.fills;
@@ -866,15 +904,17 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     if (again) { ++overflowed; continue; }
     float ms[4] = {0, 0, 0, 0};
-    HIPCHK(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
-    HIPCHK(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
-    HIPCHK(c, hipEventElapsedTime(&ms[2], c->ev[0], c->ev[2]));
-    HIPCHK(c, hipEventElapsedTime(&ms[3], c->ev[3], c->ev[1]));
+    if (c->timing_level >= 1) {
+      HIPCHK(c, hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
+      HIPCHK(c, hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
+      HIPCHK(c, hipEventElapsedTime(&ms[2], c->ev[0], c->ev[2]));
+      HIPCHK(c, hipEventElapsedTime(&ms[3], c->ev[3], c->ev[1]));
+    }
     c->timing[0] = ms[0]; c->timing[1] = ms[1]; c->timing[2] = ms[2];
     // dominant kernel: k_persist (each round timed by its own dispatch events) or
     // k_bfs_level (levels + 1 launches between two events)
     for (int r = 0; r < kMaxRounds; ++r) c->timing[4 + r] = 0.0;
-    if (persistent && c->kernel_timing) {
+    if (persistent && c->timing_level >= 2) {
       double sum = 0.0;
       for (uint32_t r = 0; r < rounds; ++r) {
         float t = 0.0f;
@@ -884,7 +924,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       }
       c->timing[3] = rounds ? sum / (double)rounds : 0.0;
     } else if (persistent) {
-      c->timing[3] = rounds ? ms[3] / (double)rounds : 0.0;  // includes the gaps between rounds
+      c->timing[3] = rounds ? ms[3] / (double)rounds : 0.0;  // includes the gaps between rounds (0 without events)
     } else {
       c->timing[3] = ms[3] / (double)(levels + 1);
     }
@@ -1029,8 +1069,8 @@ int kdop_narrow_adjoint(kdop_ctx* c, uint32_t nbodies, const uint32_t* bodies, c
   HIPCHK(c, hipMemsetAsync(c->adj_acc.p, 0, (size_t)npairs * kAdjStripes * kAdjVals * sizeof(double), c->stream));
   if (nbodies) HIPCHK(c, hipMemcpyAsync(c->adj_bodies.p, bodies, nbodies * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   AdjArgs A;
-  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.recs = c->recs.p; A.r_fill = &c->dstate->fills.r;
-  A.r_cap = (uint32_t)std::min<size_t>(c->recs.cap, 0xFFFFFFFFu);
+  A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.recs = c->recs.p;
+  A.nrec = (uint32_t)std::min<size_t>(c->hsum->fills.r, c->recs.cap);
   A.poses = c->dposes.p; A.dphi = dphi_dev; A.acc = c->adj_acc.p;
   if (npairs && c->hsum->fills.r) hipLaunchKernelGGL(k_adjoint, dim3((unsigned)(c->num_cus * 4)), dim3(256), 0, c->stream, A);
   const uint32_t nthreads = std::max(7 * nbodies, npairs);

@@ -40,6 +40,7 @@ struct CallSummary {
   uint32_t final_level, final_count;
   uint32_t round_total[kMaxRounds];   // items in round r's frontier
   uint32_t round_maxseg[kMaxRounds];  // largest segment fill of round r (r >= 1)
+  uint32_t reset;                      // 1: the fold zeroed the call state
 };
 
 // Device state of one traversal call, zeroed by the host before the call.
@@ -329,9 +330,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 }
 
 // Fold the striped counters and the segment fills into C->sum (one wave;
-// lane s reads stripe s, so every load is independent).
-__global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, uint32_t cap) {
+// lane s reads stripe s, so every load is independent).  With final_round >=
+// -1 and a complete call (no overflow, nothing spilled into final_round's
+// segments; -1: no rounds), every lane then zeroes exactly what it read, so the
+// next call needs no memset.
+__global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round) {
   static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
+  static_assert(sizeof(CounterStripe) % 16 == 0, "stripes are cleared with 16-byte stores");
   const uint32_t lane = threadIdx.x;
   CallSummary& S = C->sum;
   const unsigned long long* row = reinterpret_cast<const unsigned long long*>(&C->stripe[lane]);
@@ -342,6 +347,8 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
 #pragma unroll
   for (int r = 0; r < kMaxRounds; ++r) fill[r] = C->seg[r][lane].fill;
   const uint32_t lf = C->level_fill[lane];
+  const uint32_t ovf = C->overflow;
+  const uint32_t fin_count = C->final_count;
 #pragma unroll
   for (int f = 0; f < kStripeFields; ++f) {
     unsigned long long x = v[f];
@@ -353,6 +360,7 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
     if (lane == 0) S.total[f] = x;
   }
   const uint32_t segcap = cap / kSegs;
+  uint32_t pending = 0;
 #pragma unroll
   for (int r = 0; r < kMaxRounds; ++r) {
     uint32_t tot = min(fill[r], segcap), mx = fill[r];
@@ -361,17 +369,37 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
       tot += __shfl_xor(tot, d, 64);
       mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
     }
+    if (r == final_round) pending = tot;
     if (lane == 0) {
-      S.round_total[r] = r == 0 ? C->final_count : tot;
+      S.round_total[r] = r == 0 ? fin_count : tot;
       S.round_maxseg[r] = mx;
     }
   }
   S.level_fill[lane] = lf;
   if (lane == 0) {
-    S.overflow = C->overflow;
+    S.overflow = ovf;
     S.fills = C->fills;
     S.final_level = C->final_level;
-    S.final_count = C->final_count;
+    S.final_count = fin_count;
+  }
+  const bool reset = final_round >= -1 && ovf == 0u && pending == 0u;
+  if (lane == 0) S.reset = reset ? 1u : 0u;
+  if (!reset) return;
+  float4* st = reinterpret_cast<float4*>(&C->stripe[lane]);
+  for (uint32_t w = 0; w < sizeof(CounterStripe) / 16; ++w) st[w] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int r = 0; r < kMaxRounds; ++r) {
+    C->seg[r][lane].fill = 0u;
+    C->seg[r][lane].next = 0u;
+  }
+  C->level_fill[lane] = 0u;
+  if (lane < (uint32_t)kMaxRounds) C->done[lane] = 0ull;
+  if (lane == 0) {
+    C->overflow = 0u;
+    C->fills = Fills{0u, 0u, 0u, 0u};
+    C->final_level = 0u;
+    C->final_count = 0u;
+    C->final_set = 0u;
   }
 }
 

@@ -5,9 +5,10 @@
  * simulator).  The reference exposes a tag-dispatched C++ template API
  * (kdop::make_tree / refit_tree / tandem_traversal, narrow::update_kdop_bvh);
  * each entry point below names the reference call it replaces.  All entry
- * points are synchronous: they return once the device work they start has
- * finished, and return 0 (KDOP_OK) or a negative KDOP_ERR_* code; the message
- * of the last failure is kdop_last_error(ctx).  One context per host thread.
+ * points are synchronous (they return once the device work they start has
+ * finished; refit can opt out, KDOP_OPT_ASYNC_REFIT) and return 0 (KDOP_OK) or
+ * a negative KDOP_ERR_* code; the message of the last failure is
+ * kdop_last_error(ctx).  One context per host thread.
  *
  * Pointers are host pointers unless a parameter says "device".
  */
@@ -165,8 +166,9 @@ int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies,
  * out[1] narrow phase, out[2] whole call.  out[3] = average duration of one
  * launch of the dominant traversal kernel (persistent strategy: a persistent
  * round; level-synchronous: a breadth-first level).  out[4..12): with
- * KDOP_OPT_KERNEL_TIMING on, the time of each persistent round (zeros past the
- * last round).  out holds 12 doubles. */
+ * KDOP_OPT_KERNEL_TIMING 2, the time of each persistent round (zeros past the
+ * last round).  All zeros with KDOP_OPT_KERNEL_TIMING 0 (the default).  out
+ * holds 12 doubles. */
 int kdop_last_timing(kdop_ctx* ctx, double* out);
 
 /* Scheduling statistics of the last traversal (persistent strategy; zeros for
@@ -188,9 +190,14 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 #define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels        */
 #define KDOP_OPT_STACK_KEEP 5     /* stack entries a wave keeps once the frontier is gone    */
 #define KDOP_OPT_SEED_BUDGET 6    /* chunk-root pairs tested to seed one call                */
-#define KDOP_OPT_KERNEL_TIMING 7  /* 1: time every persistent round by events attached to its */
-                                  /*   dispatch (kdop_last_timing out[3]); costs a few us per call */
+#define KDOP_OPT_KERNEL_TIMING 7  /* HIP events for kdop_last_timing: 0 none (default; the     */
+                                  /*   timings read 0), 1 phase events, 2 also one pair on each */
+                                  /*   persistent round's dispatch; each event costs a few us   */
 #define KDOP_OPT_BFS_MAX 8        /* most breadth-first levels before the persistent rounds  */
+#define KDOP_OPT_ASYNC_REFIT 9    /* 1: kdop_refit / kdop_refit_device return once the work is */
+                                  /*   queued on the context stream (later calls order after it; */
+                                  /*   device pose buffers must stay valid until the next call   */
+                                  /*   that synchronises, e.g. kdop_traverse)                    */
 
 /* Set / read one option (no counterpart in the reference, whose OpenCL
  * path fixes these sizes at compile time).  Returns KDOP_ERR_ARG for an

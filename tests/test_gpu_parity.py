@@ -190,3 +190,26 @@ def test_autotuned_batched_parity():
     assert best_ms > 0 and chosen["queue_depth"] in (256, 512, 1024, 2048) and 1 <= chosen["waves_per_cu"] <= 32
     dev = device_run(wl, ctx=ctx, ids=ids)
     _check_pair_results(wl, dev, orc)
+
+
+def test_async_refit_and_state_reuse():
+    """Refits that return once queued, chained with traversals, give the same
+    results as synchronous ones; call after call (the per-call state is
+    cleared by the previous call's final fold, not by a memset) as well."""
+    wl = W.large_pair(n=10, K=18, seed=23)
+    P, Q = W.motion(2, 4, seed=5, dt=0.05, drift=0.01)
+    ctxs = [K.Context(K=18), K.Context(K=18)]
+    ctxs[1].set_option("async_refit", 1)
+    ids = [[c.make_tree(m) for m in wl.meshes] for c in ctxs]
+    for s in range(4):
+        pos = np.stack([P[s, i] + wl.poses[i].p for i in range(2)])
+        quat = Q[s]
+        res = []
+        for c, b in zip(ctxs, ids):
+            c.refit(b, pos, quat)
+            cnt = c.tandem_traversal([[b[0], b[1]]], narrow=True, keep_all_pairs=True)
+            res.append((cnt, sort_pairs(c.pairs(0)), c.contacts()))
+        assert res[0][0] == res[1][0]
+        np.testing.assert_array_equal(res[0][1], res[1][1])
+        key = lambda con: np.lexsort((con["ordinal"], con["tet_b"], con["tet_a"]))  # noqa: E731
+        np.testing.assert_array_equal(res[0][2][key(res[0][2])], res[1][2][key(res[1][2])])
