@@ -58,6 +58,8 @@ def headline_leg(args):
     wl = W.large_pair(n=args.n, K=args.k, seed=7)
     ctx = KA.Context(K=args.k, traversal=args.traversal, queue_depth=args.queue_depth, waves_per_cu=args.waves_per_cu,
                      ramp_target=args.ramp_target, keep=args.keep, seed_budget=args.seed_budget)
+    if args.bfs_max:
+        ctx.set_option("bfs_max", args.bfs_max)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
     # the step chains refit and traversal on the context stream: the refit
@@ -215,6 +217,7 @@ def main():
     ap.add_argument("--ramp-target", type=int, default=0, help="persistent kernel: frontier size that ends the ramp")
     ap.add_argument("--keep", type=int, default=0, help="persistent kernel: stack kept once the frontier is gone")
     ap.add_argument("--seed-budget", type=int, default=0, help="chunk-root pairs tested to seed a call")
+    ap.add_argument("--bfs-max", type=int, default=0, help="persistent strategy: most breadth-first levels first")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")

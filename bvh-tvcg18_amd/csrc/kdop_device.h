@@ -123,6 +123,7 @@ __device__ inline CounterStripe* stripe_of(CounterStripe* s, uint32_t wave) { re
 // Output fills of one call (single words: their waves reserve whole batches).
 struct Fills {
   uint32_t p1, p2, c, r;
+  uint32_t g;  // two-pass narrow phase: surface pairs through the SAT gate
 };
 
 }  // namespace kdop_amd

@@ -68,6 +68,9 @@ struct kdop_ctx {
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
   DevBuf<NarrowRec> recs;
+  DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
+  uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
+  int narrow_path = 0;         // KDOP_OPT_NARROW_PATH
   DevBuf<double> adj_acc;      // adjoint accumulators [npairs][kAdjStripes][kAdjVals]
   DevBuf<uint32_t> adj_bodies;
   CallState* dstate = nullptr;    // counters and frontier segments of one call
@@ -448,7 +451,7 @@ void kdop_destroy(kdop_ctx* c) {
   }
   release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
-  release(c->contacts); release(c->recs); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
+  release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -495,6 +498,10 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
     case KDOP_OPT_KERNEL_TIMING:
       if (v < 0 || v > 2) break;
       c->timing_level = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_NARROW_PATH:
+      if (v < 0 || v > 2) break;
+      c->narrow_path = (int)v;
       return KDOP_OK;
     case KDOP_OPT_ASYNC_REFIT:
       if (v != 0 && v != 1) break;
@@ -581,6 +588,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget; return KDOP_OK;
     case KDOP_OPT_KERNEL_TIMING: *v = c->timing_level; return KDOP_OK;
     case KDOP_OPT_ASYNC_REFIT: *v = c->async_refit ? 1 : 0; return KDOP_OK;
+    case KDOP_OPT_NARROW_PATH: *v = c->narrow_path; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
@@ -719,9 +727,10 @@ int kdop_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos
 
 namespace {
 
-// Narrow-phase arguments; with pending_round >= 0 the kernel runs only when
-// the persistent traversal left nothing in that round's segments.
-NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round) {
+// Narrow-phase arguments; with pending_round >= 0 the gate runs only when
+// the persistent traversal left nothing in that round's segments, with
+// levels_done only when the level-synchronous traversal finished.
+NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels_done) {
   CallState* d = c->dstate;
   NarrowArgs A;
   A.bodies = c->dbodies.p; A.pairs = c->dpairs.p; A.p2 = c->p2.p; A.p2_fill = &d->fills.p2; A.p2_cap = T.p2_cap;
@@ -731,7 +740,29 @@ NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round) {
   A.overflow = &d->overflow;
   A.pending = pending_round >= 0 ? &d->seg[pending_round][0].fill : nullptr;
   A.pending_stride = sizeof(SegCtr) / sizeof(uint32_t);
+  A.done = levels_done ? &d->final_set : nullptr;
+  A.gate = c->gate.p;
+  A.g_fill = &d->fills.g;
   return A;
+}
+
+// The narrow phase.  Few surface pairs (as many as the GPU holds waves): one
+// wave per pair in one pass; many: the SAT gate with 8 lanes per pair, then
+// one wave per surviving pair (most do not survive).  The choice follows the
+// previous call's surface-pair count (or the size estimate on the first
+// call); both paths give the same results.
+int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels_done, uint64_t p2_guess) {
+  const NarrowArgs A = narrow_args(c, T, pending_round, levels_done);
+  const dim3 grid((unsigned)(c->num_cus * 4)), block(256);
+  const bool one_pass = c->narrow_path == 1 || (c->narrow_path == 0 && p2_guess <= (uint64_t)c->num_cus * 32);
+  if (one_pass) {
+    hipLaunchKernelGGL(k_narrow_wave, grid, block, 0, c->stream, A);
+  } else {
+    hipLaunchKernelGGL(k_narrow_gate, grid, block, 0, c->stream, A);
+    hipLaunchKernelGGL(k_narrow_gated, grid, block, 0, c->stream, A);
+  }
+  HIPCHK(c, hipGetLastError());
+  return KDOP_OK;
 }
 
 // Fold the call's counters on the device and copy the summary to the host.
@@ -755,7 +786,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (!c) return KDOP_ERR_ARG;
   if (npairs && !body_pairs) return fail(c, KDOP_ERR_ARG, "kdop_traverse: null body_pairs");
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  uint32_t levels = 0;
+  uint32_t levels = 0, level_guess = 0;  // BVTT depth bound; likely depth (deeper tree + 4)
   uint64_t est = 0;
   std::vector<uint2> pairs(npairs);
   for (uint32_t i = 0; i < npairs; ++i) {
@@ -763,6 +794,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
     pairs[i] = make_uint2(a, b);
     levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
+    level_guess = std::max(level_guess, std::max(c->bodies[a].depth, c->bodies[b].depth) + 4);
     est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
   }
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
@@ -820,6 +852,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   est = std::max<uint64_t>(est, 1 << 16);
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
+  if ((rc = ensure(c, c->gate, c->p2.cap))) return rc;
+  const uint64_t p2_guess = c->last_p2 ? c->last_p2 : est / 64;
   if (narrow) {
     if ((rc = ensure(c, c->contacts, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
     if ((rc = ensure(c, c->recs, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
@@ -842,7 +876,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
     T.npairs = npairs;
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
-    uint32_t rounds = 0;
+    uint32_t rounds = 0, levels_queued = 0;
     if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (npairs) {
       dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
@@ -852,18 +886,18 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
         if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
         for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
       } else {
+        // first the levels the deeper tree suggests; more below if the
+        // frontier is not empty by then
         if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        for (uint32_t l = 0; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
+        levels_queued = std::min(levels, level_guess);
+        for (uint32_t l = 0; l <= levels_queued; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
       }
     } else {
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     }
     HIPCHK(c, hipGetLastError());
     if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    if (narrow && npairs) {
-      hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, persistent ? (int)rounds : -1));
-      HIPCHK(c, hipGetLastError());
-    }
+    if (narrow && npairs && (rc = launch_narrow(c, T, persistent ? (int)rounds : -1, !persistent, p2_guess))) return rc;
     if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     if ((rc = summarize(c, cap, persistent ? (int)rounds : -1))) return rc;
     bool grow_front = (c->hsum->overflow & 1u) != 0;
@@ -879,12 +913,19 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       }
       grow_front = (c->hsum->overflow & 1u) != 0;
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      if (narrow && !grow_front) {
-        hipLaunchKernelGGL(k_narrow, dim3(1024), dim3(256), 0, c->stream, narrow_args(c, T, (int)rounds));
-        HIPCHK(c, hipGetLastError());
-      }
+      if (narrow && !grow_front && (rc = launch_narrow(c, T, (int)rounds, false, p2_guess))) return rc;
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
       if ((rc = summarize(c, cap, (int)rounds))) return rc;
+    }
+    if (!persistent && npairs && !grow_front && !c->hsum->final_set) {
+      // the level-synchronous frontier outlived the first batch of launches
+      for (uint32_t l = levels_queued + 1; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
+      HIPCHK(c, hipGetLastError());
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      if (narrow && (rc = launch_narrow(c, T, -1, true, p2_guess))) return rc;
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      if ((rc = summarize(c, cap, -1))) return rc;
+      grow_front = (c->hsum->overflow & 1u) != 0;
     }
     const CallSummary& h = *c->hsum;
     const Fills& f = h.fills;
@@ -898,7 +939,10 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       again = true;
     } else {
       if (keep_p1 && f.p1 > T.p1_cap) { if ((rc = ensure(c, c->p1, (size_t)f.p1 + f.p1 / 4))) return rc; again = true; }
-      if (f.p2 > T.p2_cap) { if ((rc = ensure(c, c->p2, (size_t)f.p2 + f.p2 / 4))) return rc; again = true; }
+      if (f.p2 > T.p2_cap) {
+        if ((rc = ensure(c, c->p2, (size_t)f.p2 + f.p2 / 4)) || (rc = ensure(c, c->gate, c->p2.cap))) return rc;
+        again = true;
+      }
       if (narrow && f.c > c->contacts.cap) { if ((rc = ensure(c, c->contacts, (size_t)f.c + f.c / 4))) return rc; again = true; }
       if (narrow && f.r > c->recs.cap) { if ((rc = ensure(c, c->recs, (size_t)f.r + f.r / 4))) return rc; again = true; }
     }
@@ -932,6 +976,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->last_rounds = persistent ? rounds : 0;
     c->last_persistent = persistent;
     c->last_npairs = npairs;
+    c->last_p2 = f.p2;
     c->last_pairs = pairs;
     c->have_traversal = true;
     c->have_p1 = keep_p1;

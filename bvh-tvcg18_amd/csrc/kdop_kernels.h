@@ -712,12 +712,84 @@ struct NarrowArgs {
   uint32_t r_cap;
   CounterStripe* stripes;
   unsigned int* overflow;
-  // when non-null: run only if every word of pending[0..kSegs) is zero
-  // (the persistent traversal left no spill behind; else the host drains it
-  // and launches the narrow phase again)
+  // gate list of the two-pass narrow phase (P2 index | face << 30) and its fill
+  uint32_t* gate;
+  uint32_t* g_fill;
+  // when non-null: runs only if every word of pending[0..kSegs) is zero (the
+  // persistent traversal left no spill behind; else the host drains it and
+  // launches the narrow phase again)
   const uint32_t* pending;
   uint32_t pending_stride;  // words between consecutive pending entries
+  // when non-null: runs only if *done is nonzero (the level-synchronous
+  // traversal reached an empty frontier)
+  const uint32_t* done;
 };
+
+__device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
+  const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)t);
+  const uint32_t v[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) P[q] = f3{B.X[v[q]], B.Y[v[q]], B.Z[v[q]]};
+}
+
+// SAT gate (overlap/geometry_overlap_tetrahedron_tetrahedron.h:50-290) and
+// the most opposing face pair (contacts ...tetrahedron.h:464-533) of surface
+// pair i with the 8 lanes of its group: each lane takes 5-6 of the 44 SAT axes
+// and 2 of the 16 face pairs; the serial loops' results (any separating axis;
+// the first face pair, in a-major order, with the smallest dot product) come
+// from ballots over the group.  V, U: the group's LDS for the two tets'
+// vertices and face planes.  Returns the A face, or -1 (same in all 8 lanes).
+__device__ inline int gate_group(const NarrowArgs& A, f3* V, UPlane* U, uint32_t i, bool live) {
+  const uint32_t lane = lane_id(), r = lane & 7u, shift = lane & ~7u;
+  uint32_t sa = 0, sb = 0;
+  if (live) {
+    const LeafPair lp = A.p2[i];
+    const uint2 bp = A.pairs[lp.pair];
+    const BodyDev& BA = A.bodies[bp.x];
+    const BodyDev& BB = A.bodies[bp.y];
+    sa = BA.surf[lp.ta] & 15u;
+    sb = BB.surf[lp.tb] & 15u;
+    const BodyDev& Bd = r < 4u ? BA : BB;
+    const uint32_t vi = Bd.tets[4 * (size_t)(r < 4u ? lp.ta : lp.tb) + (r & 3u)];
+    V[r] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
+  }
+  wave_lds_sync();
+  if (live) {
+    const f3* P = V + (r & 4u);
+    const uint32_t f = r & 3u;
+    U[r] = make_uplane(P[face_v(f, 0)], P[face_v(f, 1)], P[face_v(f, 2)]);
+  }
+  wave_lds_sync();
+  bool sep = false;
+  if (live)
+    for (uint32_t ax = r; ax < 44u; ax += 8u) sep = sep || sat_axis(V, ax);
+  const bool alive = live && (((__ballot(sep) >> shift) & 0xFFull) == 0ull);
+  // face pairs c = 4a + b: lane r takes c = r and c = r + 8
+  bool ok[2] = {false, false};
+  float t[2] = {INFINITY, INFINITY};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t a = (r >> 2) + 2u * h, b = r & 3u;
+    if (alive && ((sa >> a) & 1u) && ((sb >> b) & 1u)) {
+      const UPlane plA = U[a], plB = U[4 + b];
+      const f3 nA = unit3(plA.n), nB = unit3(plB.n);
+      const float test = dot3(nA, nB);
+      const float testA = dot3(plA.n, sub3(V[4 + b], plA.p));
+      const float testB = dot3(plB.n, sub3(V[a], plB.p));
+      ok[h] = test < 0.0f && !(testA < 0.0f) && !(testB < 0.0f);
+      t[h] = test;
+    }
+  }
+  float best = fminf(ok[0] ? t[0] : INFINITY, ok[1] ? t[1] : INFINITY);
+#pragma unroll
+  for (int d = 1; d < 8; d <<= 1) best = fminf(best, __shfl_xor(best, d, 64));
+  const uint32_t m0 = (uint32_t)(__ballot(ok[0] && t[0] == best) >> shift) & 0xFFu;
+  const uint32_t m1 = (uint32_t)(__ballot(ok[1] && t[1] == best) >> shift) & 0xFFu;
+  wave_lds_sync();  // the group's LDS is rewritten by its next pair
+  if (m0) return (__ffs((int)m0) - 1) >> 2;
+  if (m1) return (8 + __ffs((int)m1) - 1) >> 2;
+  return -1;
+}
 
 __device__ inline float wave_min_f(float v) {
 #pragma unroll
@@ -731,33 +803,141 @@ __device__ inline float wave_max_f(float v) {
 }
 __device__ inline float readlane_f(float v, uint32_t l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
-// One wave per surface pair (four per 256-thread block): SAT, most opposing
-// normal and contact points (contacts/geometry_contacts_tetrahedron_tetrahedron.h:
-// 1312-1345).  Lanes take the 44 SAT axes, the 16 face pairs and the 56
-// contact candidates; each of them is computed with the serial operation
-// order, and the serial loops' choices (first minimum, first maximum, first
-// opposing face pair with the smallest dot product, a candidate dropped when
-// within working precision of ANY earlier candidate) are made with ballots
-// over lanes in that order, so the results are those of the serial code.
-__global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
+// Contact points of one gated surface pair, one wave (contacts/geometry_
+// contacts_tetrahedron_tetrahedron.h:543-681, as :1312-1345 calls it): lanes
+// take the 56 contact candidates, each computed with the serial operation
+// order; the serial loops' choices (first minimum, first maximum, a candidate
+// dropped when within working precision of ANY earlier candidate) are made
+// with ballots over lanes in that order, so the results are those of the
+// serial code.  gate_word: P2 index | face << 30.  Every thread of the block
+// calls it (block-level output reservation); live is wave-uniform.
+__device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, uint32_t gate_word, bool live, uint32_t* red,
+                                            unsigned long long& hits) {
+  const uint32_t lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  LeafPair lp{0, 0, 0};
+  const uint32_t face = gate_word >> 30;
+  if (live) {
+    const uint32_t g = gate_word;
+    lp = A.p2[g & 0x3FFFFFFFu];
+    const uint2 bp = A.pairs[lp.pair];
+    if (lane < 8u) {
+      const BodyDev& Bd = A.bodies[lane < 4u ? bp.x : bp.y];
+      const uint32_t t = lane < 4u ? lp.ta : lp.tb;
+      const uint32_t vi = Bd.tets[4 * (size_t)t + (lane & 3u)];
+      L.v[lane] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
+    }
+  }
+  wave_lds_sync();
+  // face planes (contacts ...tetrahedron.h:30-62)
+  if (live && lane < 8u) {
+    const f3* P = L.v + (lane & 4u);
+    const uint32_t f = lane & 3u;
+    L.u[lane] = make_uplane(P[face_v(f, 0)], P[face_v(f, 1)], P[face_v(f, 2)]);
+  }
+  wave_lds_sync();
+  f3 nrm{0.f, 0.f, 0.f};
+  bool ex = false;
+  f3 q{0.f, 0.f, 0.f};
+  uint8_t code = 0;
+  if (live) {
+    nrm = unit3(L.u[face].n);
+    if (lane < (uint32_t)kMaxCand) ex = candidate(L, lane, &q, &code);
+  }
+  const unsigned long long exm = __ballot(ex);
+  live = live && exm != 0ull;
+  uint32_t nu = 0, ordinal = 0;
+  bool keep = false;
+  float depth = 0.0f;
+  uint32_t cmin = 0, cmax = 0;
+  if (live) {
+  // penetration depth from the extreme projections onto the normal
+    const float d = dot3(q, nrm);
+    const bool has = ex && d == d;
+    // serial: mn = FLT_MAX, mx = -FLT_MAX, first candidate attaining each
+    // (strict comparisons); the value is read from that candidate's lane
+    const float lo = wave_min_f(has ? d : FLT_MAX);
+    const float hi = wave_max_f(has ? d : -FLT_MAX);
+    const unsigned long long at_mn = __ballot(has && d == lo && d < FLT_MAX);
+    const unsigned long long at_mx = __ballot(has && d == hi && -FLT_MAX < d);
+    const uint32_t first = (uint32_t)(__ffsll(exm) - 1);
+    const uint32_t lmin = at_mn ? (uint32_t)(__ffsll(at_mn) - 1) : first;
+    const uint32_t lmax = at_mx ? (uint32_t)(__ffsll(at_mx) - 1) : first;
+    const float mn = at_mn ? readlane_f(d, lmin) : FLT_MAX;
+    const float mx = at_mx ? readlane_f(d, lmax) : -FLT_MAX;
+    cmin = __builtin_amdgcn_readlane((uint32_t)code, lmin);
+    cmax = __builtin_amdgcn_readlane((uint32_t)code, lmax);
+    depth = mn - mx;
+    // project onto the mid plane, drop points within working precision
+    const f3 mid = mul3(mul3(nrm, mx + mn), 0.5f);
+    const float s = dot3(nrm, sub3(q, mid));
+    q = sub3(q, f3{s * nrm.x, s * nrm.y, s * nrm.z});
+    const float wp = FLT_EPSILON * 10.0f;
+    bool dup = false;
+    unsigned long long rem = exm & (exm - 1ull) ? exm & ~(1ull << (63 - __clzll(exm))) : 0ull;  // all but the last
+    while (rem) {
+      const uint32_t src = (uint32_t)(__ffsll(rem) - 1);
+      rem &= rem - 1ull;
+      const f3 o{readlane_f(q.x, src), readlane_f(q.y, src), readlane_f(q.z, src)};
+      if (lane > src) {
+        const f3 dd = sub3(o, q);
+        const float n1 = smax(smax(fabsf(dd.x), fabsf(dd.y)), fabsf(dd.z));
+        if (n1 < wp) dup = true;
+      }
+    }
+    keep = ex && !dup;
+    const unsigned long long km = __ballot(keep);
+    nu = (uint32_t)__popcll(km);
+    ordinal = (uint32_t)__popcll(km & below);
+  }
+  const uint32_t oc = block_reserve(A.c_fill, keep ? 1u : 0u, red);
+  const uint32_t orr = block_reserve(A.r_fill, (lane == 0u && nu) ? 1u : 0u, red);
+  if (keep) {
+    if (oc >= A.c_cap) {
+      atomicOr(A.overflow, 8u);
+    } else {
+      kdop_contact ct;
+      ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = ordinal;
+      ct.p[0] = q.x; ct.p[1] = q.y; ct.p[2] = q.z;
+      ct.n[0] = nrm.x; ct.n[1] = nrm.y; ct.n[2] = nrm.z;
+      ct.depth = depth;
+      A.contacts[oc] = ct;
+    }
+  }
+  if (lane == 0u && nu) {
+    ++hits;
+    if (orr < A.r_cap) {
+      NarrowRec rec;
+      rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
+      rec.face = (uint8_t)face; rec.nuniq = (uint8_t)nu; rec.cmin = (uint8_t)cmin; rec.cmax = (uint8_t)cmax;
+      rec.depth = depth; rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
+      A.recs[orr] = rec;
+    }
+  }
+  wave_lds_sync();  // this wave's LDS is rewritten by its next pair
+}
+
+// Narrow phase for few surface pairs: one wave per pair.  Lanes 0..43 test the
+// SAT axes and lanes 0..15 the face pairs (the serial loops' results by
+// ballots), then narrow_contacts_wave on the same wave.
+__global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
   __shared__ NarrowLds lds[4];
   __shared__ uint32_t red[5];
   if (A.pending) {
     const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
     if (__any(v != 0u)) return;
   }
+  if (A.done && *A.done == 0u) return;
   const uint32_t n = min(*A.p2_fill, A.p2_cap);
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-  const unsigned long long below = (1ull << lane) - 1ull;
   NarrowLds& L = lds[w];
   unsigned long long hits = 0;
   for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
     const uint32_t i = base + w;
     bool live = i < n;  // wave-uniform
-    LeafPair lp{0, 0, 0};
     uint32_t sa = 0, sb = 0;
     if (live) {
-      lp = A.p2[i];
+      const LeafPair lp = A.p2[i];
       const uint2 bp = A.pairs[lp.pair];
       const BodyDev& BA = A.bodies[bp.x];
       const BodyDev& BB = A.bodies[bp.y];
@@ -765,24 +945,20 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
       sb = BB.surf[lp.tb] & 15u;
       if (lane < 8u) {
         const BodyDev& Bd = lane < 4u ? BA : BB;
-        const uint32_t t = lane < 4u ? lp.ta : lp.tb;
-        const uint32_t vi = Bd.tets[4 * (size_t)t + (lane & 3u)];
+        const uint32_t vi = Bd.tets[4 * (size_t)(lane < 4u ? lp.ta : lp.tb) + (lane & 3u)];
         L.v[lane] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
       }
     }
     wave_lds_sync();
-    // face planes (contacts ...tetrahedron.h:30-62)
     if (live && lane < 8u) {
       const f3* P = L.v + (lane & 4u);
       const uint32_t f = lane & 3u;
       L.u[lane] = make_uplane(P[face_v(f, 0)], P[face_v(f, 1)], P[face_v(f, 2)]);
     }
     wave_lds_sync();
-    // SAT gate (overlap ...tetrahedron.h:50-290)
-    const bool sep = live && lane < 44u && sat_axis(L, lane);
+    const bool sep = live && lane < 44u && sat_axis(L.v, lane);
     live = live && !__any(sep);
-    // most opposing face pair (contacts ...tetrahedron.h:464-533)
-    bool valid = false;
+    bool ok = false;
     float test = 0.0f;
     if (live && lane < 16u) {
       const uint32_t a = lane >> 2, b = lane & 3u;
@@ -792,93 +968,52 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs A) {
         test = dot3(nA, nB);
         const float testA = dot3(plA.n, sub3(L.v[4 + b], plA.p));
         const float testB = dot3(plB.n, sub3(L.v[a], plB.p));
-        valid = test < 0.0f && !(testA < 0.0f) && !(testB < 0.0f);
+        ok = test < 0.0f && !(testA < 0.0f) && !(testB < 0.0f);
       }
     }
-    const float best = wave_min_f(valid ? test : INFINITY);
-    const unsigned long long pick = __ballot(valid && test == best);
-    live = live && pick != 0ull;
-    uint32_t face = 0;
-    f3 nrm{0.f, 0.f, 0.f};
-    bool ex = false;
-    f3 q{0.f, 0.f, 0.f};
-    uint8_t code = 0;
-    if (live) {
-      face = (uint32_t)(__ffsll(pick) - 1) >> 2;
-      nrm = unit3(L.u[face].n);
-      if (lane < (uint32_t)kMaxCand) ex = candidate(L, lane, &q, &code);
-    }
-    const unsigned long long exm = __ballot(ex);
-    live = live && exm != 0ull;
-    uint32_t nu = 0, ordinal = 0;
-    bool keep = false;
-    float depth = 0.0f;
-    uint32_t cmin = 0, cmax = 0;
-    if (live) {
-      // penetration depth from the extreme projections onto the normal
-      const float d = dot3(q, nrm);
-      const bool has = ex && d == d;
-      // serial: mn = FLT_MAX, mx = -FLT_MAX, first candidate attaining each
-      // (strict comparisons); the value is read from that candidate's lane
-      const float lo = wave_min_f(has ? d : FLT_MAX);
-      const float hi = wave_max_f(has ? d : -FLT_MAX);
-      const unsigned long long at_mn = __ballot(has && d == lo && d < FLT_MAX);
-      const unsigned long long at_mx = __ballot(has && d == hi && -FLT_MAX < d);
-      const uint32_t first = (uint32_t)(__ffsll(exm) - 1);
-      const uint32_t lmin = at_mn ? (uint32_t)(__ffsll(at_mn) - 1) : first;
-      const uint32_t lmax = at_mx ? (uint32_t)(__ffsll(at_mx) - 1) : first;
-      const float mn = at_mn ? readlane_f(d, lmin) : FLT_MAX;
-      const float mx = at_mx ? readlane_f(d, lmax) : -FLT_MAX;
-      cmin = __builtin_amdgcn_readlane((uint32_t)code, lmin);
-      cmax = __builtin_amdgcn_readlane((uint32_t)code, lmax);
-      depth = mn - mx;
-      // project onto the mid plane, drop points within working precision
-      const f3 mid = mul3(mul3(nrm, mx + mn), 0.5f);
-      const float s = dot3(nrm, sub3(q, mid));
-      q = sub3(q, f3{s * nrm.x, s * nrm.y, s * nrm.z});
-      const float wp = FLT_EPSILON * 10.0f;
-      bool dup = false;
-      unsigned long long rem = exm & (exm - 1ull) ? exm & ~(1ull << (63 - __clzll(exm))) : 0ull;  // all but the last
-      while (rem) {
-        const uint32_t src = (uint32_t)(__ffsll(rem) - 1);
-        rem &= rem - 1ull;
-        const f3 o{readlane_f(q.x, src), readlane_f(q.y, src), readlane_f(q.z, src)};
-        if (lane > src) {
-          const f3 dd = sub3(o, q);
-          const float n1 = smax(smax(fabsf(dd.x), fabsf(dd.y)), fabsf(dd.z));
-          if (n1 < wp) dup = true;
-        }
-      }
-      keep = ex && !dup;
-      const unsigned long long km = __ballot(keep);
-      nu = (uint32_t)__popcll(km);
-      ordinal = (uint32_t)__popcll(km & below);
-    }
-    const uint32_t oc = block_reserve(A.c_fill, keep ? 1u : 0u, red);
-    const uint32_t orr = block_reserve(A.r_fill, (lane == 0u && nu) ? 1u : 0u, red);
-    if (keep) {
-      if (oc >= A.c_cap) {
-        atomicOr(A.overflow, 8u);
-      } else {
-        kdop_contact ct;
-        ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = ordinal;
-        ct.p[0] = q.x; ct.p[1] = q.y; ct.p[2] = q.z;
-        ct.n[0] = nrm.x; ct.n[1] = nrm.y; ct.n[2] = nrm.z;
-        ct.depth = depth;
-        A.contacts[oc] = ct;
-      }
-    }
-    if (lane == 0u && nu) {
-      ++hits;
-      if (orr < A.r_cap) {
-        NarrowRec rec;
-        rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
-        rec.face = (uint8_t)face; rec.nuniq = (uint8_t)nu; rec.cmin = (uint8_t)cmin; rec.cmax = (uint8_t)cmax;
-        rec.depth = depth; rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
-        A.recs[orr] = rec;
-      }
-    }
-    wave_lds_sync();  // this wave's LDS is rewritten by its next pair
+    const float best = wave_min_f(ok ? test : INFINITY);
+    const unsigned long long pick = __ballot(ok && test == best);
+    const uint32_t face = pick ? (uint32_t)(__ffsll(pick) - 1) >> 2 : 0u;
+    wave_lds_sync();
+    narrow_contacts_wave(A, L, i | (face << 30), live && pick != 0ull, red, hits);
+  }
+  wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
+}
+
+// Narrow phase for many surface pairs, pass 1: the SAT gate and face pick
+// with 8 lanes per pair (gate_group, 32 pairs per block at a time); the
+// survivors go to the gate list (P2 index | face << 30).
+__global__ __launch_bounds__(256) void k_narrow_gate(NarrowArgs A) {
+  __shared__ f3 sv[32][8];
+  __shared__ UPlane su[32][8];
+  __shared__ uint32_t red[5];
+  if (A.pending) {
+    const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
+    if (__any(v != 0u)) return;
+  }
+  if (A.done && *A.done == 0u) return;
+  const uint32_t n = min(*A.p2_fill, A.p2_cap);
+  const uint32_t lane = lane_id(), grp = threadIdx.x >> 3;
+  for (uint32_t base = blockIdx.x * 32u; base < n; base += gridDim.x * 32u) {
+    const uint32_t i = base + grp;
+    const int face = gate_group(A, sv[grp], su[grp], i, i < n);
+    const bool emit = (lane & 7u) == 0u && face >= 0;
+    const uint32_t o = block_reserve(A.g_fill, emit ? 1u : 0u, red);
+    if (emit) A.gate[o] = i | ((uint32_t)face << 30);  // o < n <= P2 capacity = gate capacity
+  }
+}
+
+// Narrow phase for many surface pairs, pass 2: contact points of the gated
+// pairs, one wave each.
+__global__ __launch_bounds__(256) void k_narrow_gated(NarrowArgs A) {
+  __shared__ NarrowLds lds[4];
+  __shared__ uint32_t red[5];
+  const uint32_t n = *A.g_fill;
+  const uint32_t w = threadIdx.x >> 6;
+  unsigned long long hits = 0;
+  for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
+    const uint32_t k = base + w;
+    narrow_contacts_wave(A, lds[w], k < n ? A.gate[k] : 0u, k < n, red, hits);
   }
   wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
 }

@@ -105,10 +105,11 @@ struct NarrowLds {
 
 // SAT axis `ax` (0..43) in the reference's order (overlap ...tetrahedron.h:
 // 50-290): 0-3 faces of A against B's vertices, 4-7 faces of B against A's,
-// 8-43 edge pairs (ea, eb) = ((ax-8)/6, (ax-8)%6).  True when it separates.
-__device__ inline bool sat_axis(const NarrowLds& L, uint32_t ax) {
-  const f3* A = L.v;
-  const f3* B = L.v + 4;
+// 8-43 edge pairs (ea, eb) = ((ax-8)/6, (ax-8)%6).  V: A's vertices, then
+// B's.  True when the axis separates.
+__device__ inline bool sat_axis(const f3* V, uint32_t ax) {
+  const f3* A = V;
+  const f3* B = V + 4;
   if (ax < 8u) {
     const f3* P = ax < 4u ? A : B;
     const f3* Q = ax < 4u ? B : A;

@@ -41,6 +41,7 @@ struct CallSummary {
   uint32_t round_total[kMaxRounds];   // items in round r's frontier
   uint32_t round_maxseg[kMaxRounds];  // largest segment fill of round r (r >= 1)
   uint32_t reset;                      // 1: the fold zeroed the call state
+  uint32_t final_set;                  // a breadth-first launch recorded the frontier
 };
 
 // Device state of one traversal call, zeroed by the host before the call.
@@ -381,8 +382,10 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
     S.fills = C->fills;
     S.final_level = C->final_level;
     S.final_count = fin_count;
+    S.final_set = C->final_set;
   }
-  const bool reset = final_round >= -1 && ovf == 0u && pending == 0u;
+  // levels strategy (final_round -1): complete once a breadth-first launch found the frontier empty
+  const bool reset = final_round >= -1 && ovf == 0u && pending == 0u && (final_round >= 0 || C->final_set != 0u);
   if (lane == 0) S.reset = reset ? 1u : 0u;
   if (!reset) return;
   float4* st = reinterpret_cast<float4*>(&C->stripe[lane]);
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, 
   if (lane < (uint32_t)kMaxRounds) C->done[lane] = 0ull;
   if (lane == 0) {
     C->overflow = 0u;
-    C->fills = Fills{0u, 0u, 0u, 0u};
+    C->fills = Fills{0u, 0u, 0u, 0u, 0u};
     C->final_level = 0u;
     C->final_count = 0u;
     C->final_set = 0u;

@@ -194,6 +194,9 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   timings read 0), 1 phase events, 2 also one pair on each */
                                   /*   persistent round's dispatch; each event costs a few us   */
 #define KDOP_OPT_BFS_MAX 8        /* most breadth-first levels before the persistent rounds  */
+#define KDOP_OPT_NARROW_PATH 10   /* narrow phase: 0 auto (by the last surface-pair count), */
+                                  /*   1 one wave per pair, 2 SAT gate (8 lanes per pair) then */
+                                  /*   one wave per surviving pair; same results either way     */
 #define KDOP_OPT_ASYNC_REFIT 9    /* 1: kdop_refit / kdop_refit_device return once the work is */
                                   /*   queued on the context stream (later calls order after it; */
                                   /*   device pose buffers must stay valid until the next call   */
@@ -212,7 +215,8 @@ int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
  * wall time of 3 calls, until budget_ms of tuning time is spent.  The best
  * options stay set on the context, and the last traversal's results are those
  * of a run with them.  chosen (may be NULL) receives 8 values indexed by
- * KDOP_OPT_* - 1; *best_ms (may be NULL) the median time of the best.
+ * KDOP_OPT_* - 1 (options 1..8); *best_ms (may be NULL) the median time of
+ * the best.
  * No counterpart in the reference (its OpenCL sizes are compile-time). */
 int kdop_autotune(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, double budget_ms, int64_t* chosen,
                   double* best_ms);

@@ -213,3 +213,15 @@ def test_async_refit_and_state_reuse():
         np.testing.assert_array_equal(res[0][1], res[1][1])
         key = lambda con: np.lexsort((con["ordinal"], con["tet_b"], con["tet_a"]))  # noqa: E731
         np.testing.assert_array_equal(res[0][2][key(res[0][2])], res[1][2][key(res[1][2])])
+
+
+@pytest.mark.parametrize("path", [1, 2], ids=["wave_per_pair", "gate_then_contacts"])
+def test_both_narrow_paths(path):
+    """The narrow phase's two paths (chosen by the surface-pair count) give
+    the oracle's contacts: forced one-pass and two-pass on the same input."""
+    wl = W.large_pair(n=12, K=14, seed=19)
+    orc, _, _ = oracle_run(wl)
+    ctx = K.Context(K=14)
+    ctx.set_option("narrow_path", path)
+    dev = device_run(wl, ctx=ctx)
+    _check_pair_results(wl, dev, orc)
