@@ -71,6 +71,11 @@ struct kdop_ctx {
   DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
   uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
   int narrow_path = 0;         // KDOP_OPT_NARROW_PATH
+  bool use_graphs = true;      // KDOP_OPT_GRAPHS: replay the traversal's launches from a HIP graph
+  std::vector<uint64_t> tg_sig;    // signature (every launch argument) of the captured traversal
+  std::vector<uint64_t> prev_sig;  // signature of the previous call (capture once it repeats)
+  hipGraph_t tg = nullptr;
+  hipGraphExec_t tg_exec = nullptr;
   DevBuf<double> adj_acc;      // adjoint accumulators [npairs][kAdjStripes][kAdjVals]
   DevBuf<uint32_t> adj_bodies;
   CallState* dstate = nullptr;    // counters and frontier segments of one call
@@ -456,6 +461,8 @@ void kdop_destroy(kdop_ctx* c) {
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
+  if (c->tg_exec) (void)hipGraphExecDestroy(c->tg_exec);
+  if (c->tg) (void)hipGraphDestroy(c->tg);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -498,6 +505,10 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
     case KDOP_OPT_KERNEL_TIMING:
       if (v < 0 || v > 2) break;
       c->timing_level = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_GRAPHS:
+      if (v != 0 && v != 1) break;
+      c->use_graphs = v != 0;
       return KDOP_OK;
     case KDOP_OPT_NARROW_PATH:
       if (v < 0 || v > 2) break;
@@ -589,6 +600,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_KERNEL_TIMING: *v = c->timing_level; return KDOP_OK;
     case KDOP_OPT_ASYNC_REFIT: *v = c->async_refit ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_NARROW_PATH: *v = c->narrow_path; return KDOP_OK;
+    case KDOP_OPT_GRAPHS: *v = c->use_graphs ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
@@ -751,10 +763,12 @@ NarrowArgs narrow_args(kdop_ctx* c, const TravArgs& T, int pending_round, bool l
 // one wave per surviving pair (most do not survive).  The choice follows the
 // previous call's surface-pair count (or the size estimate on the first
 // call); both paths give the same results.
-int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels_done, uint64_t p2_guess) {
+bool narrow_one_pass(const kdop_ctx* c, uint64_t p2_guess) {
+  return c->narrow_path == 1 || (c->narrow_path == 0 && p2_guess <= (uint64_t)c->num_cus * 32);
+}
+int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels_done, bool one_pass) {
   const NarrowArgs A = narrow_args(c, T, pending_round, levels_done);
   const dim3 grid((unsigned)(c->num_cus * 4)), block(256);
-  const bool one_pass = c->narrow_path == 1 || (c->narrow_path == 0 && p2_guess <= (uint64_t)c->num_cus * 32);
   if (one_pass) {
     hipLaunchKernelGGL(k_narrow_wave, grid, block, 0, c->stream, A);
   } else {
@@ -769,13 +783,20 @@ int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels
 // final_round >= 0: the fold also zeroes the call state (for the next call,
 // in place of a memset) when the call is complete: no overflow and nothing
 // left in that round's segments (-1: levels strategy, no rounds).  -2: never.
-int summarize(kdop_ctx* c, uint32_t cap, int final_round) {
-  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(64), 0, c->stream, c->dstate, cap, final_round);
+int enqueue_summary(kdop_ctx* c, uint32_t cap, int final_round) {
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(kSumThreads), 0, c->stream, c->dstate, cap, final_round);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->hsum, &c->dstate->sum, sizeof(CallSummary), hipMemcpyDeviceToHost, c->stream));
+  return KDOP_OK;
+}
+int finish_summary(kdop_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->state_clean = c->hsum->reset != 0;
   return KDOP_OK;
+}
+int summarize(kdop_ctx* c, uint32_t cap, int final_round) {
+  int rc = enqueue_summary(c, cap, final_round);
+  return rc ? rc : finish_summary(c);
 }
 
 }  // namespace
@@ -877,29 +898,72 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.npairs = npairs;
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
     uint32_t rounds = 0, levels_queued = 0;
-    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    const uint32_t last = std::min(levels, c->bfs_max);
     if (npairs) {
-      dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
-      if (persistent) {
-        const uint32_t last = std::min(levels, c->bfs_max);
-        for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
-        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        for (; rounds < rounds_async; ++rounds) dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
-      } else {
-        // first the levels the deeper tree suggests; more below if the
-        // frontier is not empty by then
-        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-        levels_queued = std::min(levels, level_guess);
-        for (uint32_t l = 0; l <= levels_queued; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
-      }
-    } else {
-      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      if (persistent) rounds = rounds_async;
+      else levels_queued = std::min(levels, level_guess);
     }
-    HIPCHK(c, hipGetLastError());
-    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    if (narrow && npairs && (rc = launch_narrow(c, T, persistent ? (int)rounds : -1, !persistent, p2_guess))) return rc;
-    if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if ((rc = summarize(c, cap, persistent ? (int)rounds : -1))) return rc;
+    const bool one_pass = narrow_one_pass(c, p2_guess);
+    // the launches up to the summary copy: recorded once into a HIP graph per
+    // set of launch arguments and replayed (host submission of a dozen small
+    // kernels otherwise leaves the GPU waiting); direct with timing events
+    auto enqueue_first = [&]() -> int {
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      if (npairs) {
+        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
+        if (persistent) {
+          for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
+          if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+          for (uint32_t r = 0; r < rounds_async; ++r) dispatch_K<PersistRound>(c->K, c, T, cap, r);
+        } else {
+          // first the levels the deeper tree suggests; more below if the
+          // frontier is not empty by then
+          if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+          for (uint32_t l = 0; l <= levels_queued; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
+        }
+      } else {
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+      }
+      HIPCHK(c, hipGetLastError());
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      int r2 = KDOP_OK;
+      if (narrow && npairs && (r2 = launch_narrow(c, T, persistent ? (int)rounds : -1, !persistent, one_pass))) return r2;
+      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      return enqueue_summary(c, cap, persistent ? (int)rounds : -1);
+    };
+    std::vector<uint64_t> sig;
+    if (c->use_graphs && c->timing_level == 0 && npairs) {
+      sig = {
+          (uint64_t)c->K, npairs, persistent, last, levels_queued, levels, target, cap, total_seeds, narrow, one_pass,
+          (uint64_t)(uintptr_t)T.p1, (uint64_t)(uintptr_t)T.p2, T.p1_cap, T.p2_cap, (uint64_t)(uintptr_t)c->nodes.p,
+          (uint64_t)(uintptr_t)c->dbodies.p, (uint64_t)(uintptr_t)c->dpairs.p, (uint64_t)(uintptr_t)c->dtops.p,
+          (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
+          (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
+          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus};
+    }
+    const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
+    c->prev_sig = sig;
+    if (replay) {
+      // a capture costs far more than a call: only for launch arguments that repeat
+      if (!c->tg_exec || sig != c->tg_sig) {
+        if (c->tg_exec) { (void)hipGraphExecDestroy(c->tg_exec); c->tg_exec = nullptr; }
+        if (c->tg) { (void)hipGraphDestroy(c->tg); c->tg = nullptr; }
+        c->tg_sig.clear();
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rcap = enqueue_first();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (rcap) { if (g) (void)hipGraphDestroy(g); return rcap; }
+        if (ec != hipSuccess) return fail(c, KDOP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        c->tg = g;
+        HIPCHK(c, hipGraphInstantiate(&c->tg_exec, c->tg, nullptr, nullptr, 0));
+        c->tg_sig = sig;
+      }
+      HIPCHK(c, hipGraphLaunch(c->tg_exec, c->stream));
+    } else if ((rc = enqueue_first())) {
+      return rc;
+    }
+    if ((rc = finish_summary(c))) return rc;
     bool grow_front = (c->hsum->overflow & 1u) != 0;
     if (persistent && !grow_front && npairs && c->hsum->round_total[rounds] > 0) {
       // spill left after the queued rounds (the narrow phase above saw it and
@@ -913,7 +977,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       }
       grow_front = (c->hsum->overflow & 1u) != 0;
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      if (narrow && !grow_front && (rc = launch_narrow(c, T, (int)rounds, false, p2_guess))) return rc;
+      if (narrow && !grow_front && (rc = launch_narrow(c, T, (int)rounds, false, one_pass))) return rc;
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
       if ((rc = summarize(c, cap, (int)rounds))) return rc;
     }
@@ -922,7 +986,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       for (uint32_t l = levels_queued + 1; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
       HIPCHK(c, hipGetLastError());
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      if (narrow && (rc = launch_narrow(c, T, -1, true, p2_guess))) return rc;
+      if (narrow && (rc = launch_narrow(c, T, -1, true, one_pass))) return rc;
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
       if ((rc = summarize(c, cap, -1))) return rc;
       grow_front = (c->hsum->overflow & 1u) != 0;

@@ -330,74 +330,75 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
-// Fold the striped counters and the segment fills into C->sum (one wave;
-// lane s reads stripe s, so every load is independent).  With final_round >=
-// -1 and a complete call (no overflow, nothing spilled into final_round's
-// segments; -1: no rounds), every lane then zeroes exactly what it read, so the
-// next call needs no memset.
-__global__ __launch_bounds__(64) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round) {
+// Fold the striped counters and the segment fills into C->sum, one 1024-thread
+// block: wave f < kStripeFields reduces counter field f over the stripes (lane
+// s reads stripe s), waves 12..15 the segment fills of two rounds each, so
+// every load is independent and each reduction is one wave's.  With
+// final_round >= -1 and a complete call (no overflow, nothing spilled into
+// final_round's segments; -1: levels strategy, frontier found empty) the
+// block then zeroes the call state, so the next call needs no memset.
+constexpr int kSumThreads = 1024;
+__global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round) {
   static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
-  static_assert(sizeof(CounterStripe) % 16 == 0, "stripes are cleared with 16-byte stores");
-  const uint32_t lane = threadIdx.x;
+  static_assert(kStripeFields <= 12 && kMaxRounds == 8, "12 field waves, 4 round waves");
+  static_assert(sizeof(CounterStripe) * kStripes == 16 * kSumThreads, "one 16-byte clear per thread");
+  __shared__ uint32_t pending_s;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   CallSummary& S = C->sum;
-  const unsigned long long* row = reinterpret_cast<const unsigned long long*>(&C->stripe[lane]);
-  unsigned long long v[kStripeFields];
-#pragma unroll
-  for (int f = 0; f < kStripeFields; ++f) v[f] = row[f];
-  uint32_t fill[kMaxRounds];
-#pragma unroll
-  for (int r = 0; r < kMaxRounds; ++r) fill[r] = C->seg[r][lane].fill;
-  const uint32_t lf = C->level_fill[lane];
-  const uint32_t ovf = C->overflow;
-  const uint32_t fin_count = C->final_count;
-#pragma unroll
-  for (int f = 0; f < kStripeFields; ++f) {
-    unsigned long long x = v[f];
+  if (tid == 0) pending_s = 0u;
+  __syncthreads();
+  if (w < (uint32_t)kStripeFields) {
+    unsigned long long x = reinterpret_cast<const unsigned long long*>(&C->stripe[lane])[w];
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
       const unsigned long long y = __shfl_xor(x, d, 64);
-      x = (f == kStripeFields - 1) ? max(x, y) : x + y;
+      x = (w == (uint32_t)kStripeFields - 1u) ? max(x, y) : x + y;
     }
-    if (lane == 0) S.total[f] = x;
-  }
-  const uint32_t segcap = cap / kSegs;
-  uint32_t pending = 0;
+    if (lane == 0) S.total[w] = x;
+  } else if (w >= 12u) {
+    const uint32_t segcap = cap / kSegs;
 #pragma unroll
-  for (int r = 0; r < kMaxRounds; ++r) {
-    uint32_t tot = min(fill[r], segcap), mx = fill[r];
+    for (int h = 0; h < 2; ++h) {
+      const int r = 2 * (int)(w - 12u) + h;
+      const uint32_t f = C->seg[r][lane].fill;
+      uint32_t tot = min(f, segcap), mx = f;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      tot += __shfl_xor(tot, d, 64);
-      mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+      for (int d = 32; d >= 1; d >>= 1) {
+        tot += __shfl_xor(tot, d, 64);
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+      }
+      if (lane == 0) {
+        S.round_total[r] = r == 0 ? C->final_count : tot;
+        S.round_maxseg[r] = mx;
+        if (r == final_round) pending_s = tot;
+      }
     }
-    if (r == final_round) pending = tot;
-    if (lane == 0) {
-      S.round_total[r] = r == 0 ? fin_count : tot;
-      S.round_maxseg[r] = mx;
+    if (w == 12u) {
+      S.level_fill[lane] = C->level_fill[lane];
+      if (lane == 0) {
+        S.overflow = C->overflow;
+        S.fills = C->fills;
+        S.final_level = C->final_level;
+        S.final_count = C->final_count;
+        S.final_set = C->final_set;
+      }
     }
   }
-  S.level_fill[lane] = lf;
-  if (lane == 0) {
-    S.overflow = ovf;
-    S.fills = C->fills;
-    S.final_level = C->final_level;
-    S.final_count = fin_count;
-    S.final_set = C->final_set;
-  }
-  // levels strategy (final_round -1): complete once a breadth-first launch found the frontier empty
-  const bool reset = final_round >= -1 && ovf == 0u && pending == 0u && (final_round >= 0 || C->final_set != 0u);
-  if (lane == 0) S.reset = reset ? 1u : 0u;
+  __syncthreads();
+  const uint32_t ovf = C->overflow;
+  const bool reset = final_round >= -1 && ovf == 0u && pending_s == 0u && (final_round >= 0 || C->final_set != 0u);
+  __syncthreads();  // every thread has read what it needs before anything is cleared
+  if (tid == 0) S.reset = reset ? 1u : 0u;
   if (!reset) return;
-  float4* st = reinterpret_cast<float4*>(&C->stripe[lane]);
-  for (uint32_t w = 0; w < sizeof(CounterStripe) / 16; ++w) st[w] = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-  for (int r = 0; r < kMaxRounds; ++r) {
-    C->seg[r][lane].fill = 0u;
-    C->seg[r][lane].next = 0u;
+  reinterpret_cast<float4*>(C->stripe)[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (tid < (uint32_t)(kMaxRounds * kSegs)) {
+    SegCtr& sc = C->seg[tid / kSegs][tid % kSegs];
+    sc.fill = 0u;
+    sc.next = 0u;
   }
-  C->level_fill[lane] = 0u;
-  if (lane < (uint32_t)kMaxRounds) C->done[lane] = 0ull;
-  if (lane == 0) {
+  if (tid < 64u) C->level_fill[tid] = 0u;
+  if (tid < (uint32_t)kMaxRounds) C->done[tid] = 0ull;
+  if (tid == 0) {
     C->overflow = 0u;
     C->fills = Fills{0u, 0u, 0u, 0u, 0u};
     C->final_level = 0u;

@@ -192,16 +192,18 @@ def test_autotuned_batched_parity():
     _check_pair_results(wl, dev, orc)
 
 
-def test_async_refit_and_state_reuse():
-    """Refits that return once queued, chained with traversals, give the same
-    results as synchronous ones; call after call (the per-call state is
-    cleared by the previous call's final fold, not by a memset) as well."""
+def test_async_refit_graph_replay_and_state_reuse():
+    """Refits that return once queued and traversals replayed from the
+    context's HIP graph give, call after call, the results of synchronous
+    refits and direct launches (the per-call state is cleared by the previous
+    call's final fold, not by a memset); every replay is checked."""
     wl = W.large_pair(n=10, K=18, seed=23)
-    P, Q = W.motion(2, 4, seed=5, dt=0.05, drift=0.01)
+    P, Q = W.motion(2, 6, seed=5, dt=0.05, drift=0.01)
     ctxs = [K.Context(K=18), K.Context(K=18)]
+    ctxs[0].set_option("graphs", 0)
     ctxs[1].set_option("async_refit", 1)
     ids = [[c.make_tree(m) for m in wl.meshes] for c in ctxs]
-    for s in range(4):
+    for s in range(6):
         pos = np.stack([P[s, i] + wl.poses[i].p for i in range(2)])
         quat = Q[s]
         res = []
