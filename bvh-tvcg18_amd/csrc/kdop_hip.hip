@@ -223,6 +223,7 @@ int upload_bodies(kdop_ctx* c) {
     d.node_base = b.node_base; d.nnodes = b.nnodes; d.nt = b.nt; d.nv = b.nv;
     d.top_base = b.top_base; d.ntop = b.ntop; d.nbranch = b.C[b.H - 1]; d.H = b.H; d.L = b.L;
     d.branch_base = b.node_base + b.level_base[b.H - 1];
+    d.super_base = b.H >= 2 ? b.node_base + b.level_base[b.H - 2] : 0u;
     d.depth = b.depth;
     d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.X = b.X; d.Y = b.Y; d.Z = b.Z;
   }
@@ -253,9 +254,10 @@ struct RefitBranch {
 };
 template <int K>
 struct RefitSuper {
-  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* cpre, const uint32_t* cbase,
-                  const uint32_t* cn, uint32_t n) {
-    hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, cpre, cbase, cn, n, c->nodes.p);
+  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* gpre, const uint32_t* cfirst, const uint32_t* ncnk,
+                  const uint32_t* cbase, const uint32_t* cn, uint32_t n) {
+    hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, gpre, cfirst, ncnk, cbase, cn, n,
+                       c->nodes.p);
   }
 };
 template <int K>
@@ -269,8 +271,9 @@ struct RefitRoot {
 // pos/quat: device pointers (3n, 4n floats) or null with identity.
 int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
   // plan layout (uint32): [bb n][vpre n+1][tpre n+1][partial chunks: body m,
-  // chunk m] then per super step s: [bb m][cpre m+1][cbase m][cn m].  It only
-  // depends on the body list, so the device copy is reused while that repeats.
+  // chunk m] then per super step s: [bb m][gpre m+1][cfirst m][ncnk m]
+  // [cbase m][cn m].  It only depends on the body list, so the device copy is
+  // reused while that repeats.
   std::vector<uint32_t> plan;
   plan.reserve(8 * (size_t)n + 16);
   uint32_t maxH = 1;
@@ -300,27 +303,38 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   plan.insert(plan.end(), pbody.begin(), pbody.end());
   const size_t o_pchunk = plan.size();
   plan.insert(plan.end(), pchunk.begin(), pchunk.end());
-  struct Step { size_t o_bb, o_cpre, o_cbase, o_cn; uint32_t m, chunks; };
+  struct Step { size_t o_bb, o_gpre, o_cfirst, o_ncnk, o_cbase, o_cn; uint32_t m, blocks; };
   std::vector<Step> steps;
   for (uint32_t s = 1; s < maxH; ++s) {
-    std::vector<uint32_t> bb, cpre{0}, cbase, cn;
+    std::vector<uint32_t> bb, gpre{0}, cfirst, ncnk, cbase, cn;
     uint32_t tot = 0;
     for (uint32_t i = 0; i < n; ++i) {
       const BodyHost& b = c->bodies[bodies[i]];
       if (b.H < s + 1) continue;
       const int h = (int)b.H - 1 - (int)s;
+      // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64)
+      const uint32_t cf = (s == 1 && b.L * b.L <= 64) ? (b.nt / b.L) / b.L : 0u;
+      const uint32_t nc = b.C[h], nl = host_chunk_elems(b, h, nc - 1);
+      if (cf >= nc) continue;
+      const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u), G = 64 / b.L;
+      const uint32_t blocks = (nfc + G - 1) / G + (nl < b.L && nl >= 2 ? 1u : 0u);
+      if (!blocks) continue;
       bb.push_back(bodies[i]);
-      tot += b.C[h];
-      cpre.push_back(tot);
+      tot += blocks;
+      gpre.push_back(tot);
+      cfirst.push_back(cf);
+      ncnk.push_back(nc);
       cbase.push_back(b.node_base + b.level_base[h]);
-      cn.push_back(host_chunk_elems(b, h, b.C[h] - 1));
+      cn.push_back(nl);
     }
     if (bb.empty()) continue;
     Step st;
     st.m = (uint32_t)bb.size();
-    st.chunks = tot;
+    st.blocks = tot;
     st.o_bb = plan.size(); plan.insert(plan.end(), bb.begin(), bb.end());
-    st.o_cpre = plan.size(); plan.insert(plan.end(), cpre.begin(), cpre.end());
+    st.o_gpre = plan.size(); plan.insert(plan.end(), gpre.begin(), gpre.end());
+    st.o_cfirst = plan.size(); plan.insert(plan.end(), cfirst.begin(), cfirst.end());
+    st.o_ncnk = plan.size(); plan.insert(plan.end(), ncnk.begin(), ncnk.end());
     st.o_cbase = plan.size(); plan.insert(plan.end(), cbase.begin(), cbase.end());
     st.o_cn = plan.size(); plan.insert(plan.end(), cn.begin(), cn.end());
     steps.push_back(st);
@@ -342,7 +356,8 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   if (total_leaf_threads) dispatch_K<RefitLeaves>(c->K, c, (uint32_t)total_leaf_threads, P + o_bb, P + o_tpre, n);
   if (!pbody.empty()) dispatch_K<RefitBranch>(c->K, c, (uint32_t)pbody.size(), P + o_pbody, P + o_pchunk);
   for (const Step& st : steps)
-    dispatch_K<RefitSuper>(c->K, c, st.chunks, P + st.o_bb, P + st.o_cpre, P + st.o_cbase, P + st.o_cn, st.m);
+    dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
+                           st.m);
   dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);
   HIPCHK(c, hipGetLastError());
   if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));

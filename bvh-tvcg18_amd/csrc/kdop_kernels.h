@@ -281,8 +281,8 @@ __device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& d
   }
 }
 
-// Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16}: DPP row moves within
-// 16-lane rows (full-rate VALU, no LDS), a swizzle for 16.
+// Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16, 32}: DPP row moves within
+// 16-lane rows (full-rate VALU, no LDS), a swizzle for 16, a permute for 32.
 template <int S>
 __device__ inline float lane_xor(float v) {
   const int x = __float_as_int(v);
@@ -294,6 +294,7 @@ __device__ inline float lane_xor(float v) {
     return __int_as_float((threadIdx.x & S) ? dn : up);
   }
   if constexpr (S == 16) return __int_as_float(__builtin_amdgcn_ds_swizzle(x, 0x401F));  // bitmask mode, xor 16 in 32 lanes
+  if constexpr (S == 32) return __int_as_float(__builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ 32u) << 2), x));
   return v;
 }
 
@@ -310,6 +311,49 @@ __device__ inline void union_level(float* v, uint32_t j) {
   }
 }
 
+// Level H-2 of a wave's full super chunks (L = 2^LOGL, L * L <= 64): the
+// wave's 64 / L chunk roots are in LDS rows 0.. of `st` (row stride RS); lane
+// (element e = lane >> LOGL, component group cg = lane & (L-1)) takes the
+// components cg, cg + L, ... of element e, so the butterfly over the
+// elements (lane xor L, 2L, ...) holds no copies; even components are
+// minima, odd ones maxima, and L is even, so a lane does one kind only.  The
+// super chunks' nodes go to rows sc * L + BFS index (sact: this lane's super
+// chunk is full), overwriting the roots.
+template <int K, int LOGL, uint32_t RS>
+__device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bool sact) {
+  constexpr uint32_t L = 1u << LOGL, NC = (K + L - 1) / L;
+  const uint32_t e = lane >> LOGL, cg = lane & (L - 1u), sc = e >> LOGL, ei = e & (L - 1u);
+  const bool hi = (cg & 1u) != 0u;
+  float w[NC];
+#pragma unroll
+  for (uint32_t m = 0; m < NC; ++m) w[m] = (cg + m * L < (uint32_t)K) ? st[e * RS + cg + m * L] : 0.0f;
+  wave_lds_sync();
+#pragma unroll
+  for (uint32_t t = 1; t <= LOGL; ++t) {
+    const uint32_t S = L << (t - 1u);
+    const bool left = (lane & S) == 0u;
+#pragma unroll
+    for (uint32_t m = 0; m < NC; ++m) {
+      float y;
+      switch (S) {
+        case 2: y = lane_xor<2>(w[m]); break;
+        case 4: y = lane_xor<4>(w[m]); break;
+        case 8: y = lane_xor<8>(w[m]); break;
+        case 16: y = lane_xor<16>(w[m]); break;
+        default: y = lane_xor<32>(w[m]); break;
+      }
+      const float a = left ? w[m] : y, b = left ? y : w[m];
+      w[m] = hi ? smax(a, b) : smin(a, b);
+    }
+    if (sact && (ei & ((2u << (t - 1u)) - 1u)) == 0u) {
+      float* row = st + (sc * L + (1u << (LOGL - t)) - 1u + (ei >> t)) * RS;
+#pragma unroll
+      for (uint32_t m = 0; m < NC; ++m)
+        if (cg + m * L < (uint32_t)K) row[cg + m * L] = w[m];
+    }
+  }
+}
+
 // Full branch chunks, one thread per tetrahedron: the leaf DOP, then the
 // chunk's internal nodes by a butterfly over its L lanes.  A full chunk's
 // median-split tree over L = 2^logL leaves is the implicit heap (BFS node
@@ -320,27 +364,70 @@ __device__ inline void union_level(float* v, uint32_t j) {
 // the tets in full chunks, each body's share rounded up to 64 so that a wave
 // never mixes bodies (L, a power of two <= 32, then divides every wave's tet
 // range and an aligned L-lane group is one chunk).
+//
+// When L * L <= 64 a wave also holds whole super chunks of level H-2 (L full
+// branch chunks each, whose chunk roots are the super chunk's elements; a
+// full super chunk is the same heap), and the butterfly goes on over them.
+// Super chunks that are not full, or that take a chunk that is not, are left
+// to k_refit_super.
+//
+// The volumes are staged in LDS (the leaves, the branch chunks' internal
+// nodes, then the super chunks' nodes; each a contiguous run of records) and
+// written out with consecutive 16-byte words on consecutive lanes, so that a
+// store instruction covers whole lines instead of one word of 64 lines.
 template <int K>
 __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                       const uint32_t* __restrict__ tpre, uint32_t nbatch, float* __restrict__ nodes,
                                                       DirTable dt) {
+  constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
+  __shared__ __align__(16) float stage[4][64 * RS];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= tpre[nbatch]) return;  // whole waves (tpre[nbatch] is a multiple of 64)
+  float* __restrict__ st = stage[threadIdx.x >> 6];
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t item = find_item(tpre, nbatch, g);
   const BodyDev& B = bodies[batch_body[item]];
   const uint32_t logL = (uint32_t)(__ffs((int)B.L) - 1), L = B.L;
   const uint32_t tet = g - tpre[item];
-  const bool act = tet < ((B.nt >> logL) << logL);  // whole L-lane groups are active or not
+  const uint32_t tet0 = tet - lane;                   // first tet of the wave
+  const uint32_t nfull = (B.nt >> logL) << logL;      // tets in full chunks
+  const bool act = tet < nfull;                       // whole L-lane groups are active or not
+  const uint32_t j = tet & (L - 1u), q = lane >> logL;  // leaf of the chunk, chunk of the wave
+  // level H-2 in this wave: super chunk lane >> 2logL covers tets [tet & ~(L*L-1), +L*L)
+  const bool fuse = B.H >= 2u && logL >= 1u && logL <= 3u;
+  const uint32_t logLL = 2u * logL;
+  const bool sact = fuse && (tet >> logLL << logLL) + (1u << logLL) <= nfull;
+  float* __restrict__ chunk_nodes = nodes + ((size_t)B.branch_base + (size_t)(tet0 >> logL) * (2u * L) + 1u) * S;  // BFS node 0 of chunk 0
   float v[K];
 #pragma unroll
   for (int f = 0; f < K; ++f) v[f] = 0.0f;
-  uint32_t base = 0, j = 0;
+  auto stage_row = [&](uint32_t row) {
+    float* r = st + row * RS;
+#pragma unroll
+    for (int f = 0; f + 4 <= K; f += 4) *reinterpret_cast<float4*>(r + f) = make_float4(v[f], v[f + 1], v[f + 2], v[f + 3]);
+    if (K % 4) *reinterpret_cast<float2*>(r + (K & ~3)) = make_float2(v[K - 2], v[K - 1]);
+  };
+  // staged rows [0, nrows) -> records; dst(r) is row r's record or null
+  auto flush = [&](uint32_t nrows, auto&& dst) {
+    wave_lds_sync();
+    for (uint32_t w = lane; w < nrows * W; w += 64u) {
+      const uint32_t r = w / W, p = w % W;
+      float* d = dst(r);
+      if (!d || 4u * p >= (uint32_t)K) continue;
+      const float* src = st + r * RS + 4u * p;
+      if (4u * p + 4u <= (uint32_t)K) *reinterpret_cast<float4*>(d + 4u * p) = *reinterpret_cast<const float4*>(src);
+      else *reinterpret_cast<float2*>(d + 4u * p) = *reinterpret_cast<const float2*>(src);
+    }
+    wave_lds_sync();
+  };
   if (act) {
-    j = tet & (L - 1u);
-    base = B.branch_base + (tet >> logL) * (2u * L) + 1u;  // slot of BFS node 0
     tet_dop<K>(B, tet, dt, v);
-    store_vol<K>(nodes, base + L - 1u + j, v);
+    stage_row(lane);
   }
+  flush(64u, [&](uint32_t r) -> float* {  // row r: leaf r & (L-1) of chunk r >> logL
+    if (tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
+    return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + L - 1u + (r & (L - 1u))) * S;
+  });
   for (uint32_t t = 1; t <= logL; ++t) {
     switch (t) {
       case 1: union_level<K, 1>(v, j); break;
@@ -349,11 +436,26 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
       case 4: union_level<K, 8>(v, j); break;
       default: union_level<K, 16>(v, j); break;
     }
-    const uint32_t s = 1u << (t - 1u);
-    if (act && (j & ((s << 1) - 1u)) == 0u) {
-      store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), v);  // BFS index 2^d - 1 + i
-    }
+    if (act && (j & ((2u << (t - 1u)) - 1u)) == 0u) stage_row((q << logL) + (1u << (logL - t)) - 1u + (j >> t));  // BFS 2^d - 1 + i
   }
+  flush(64u, [&](uint32_t r) -> float* {  // row r: internal node r & (L-1) (< L-1) of chunk r >> logL
+    const uint32_t i = r & (L - 1u);
+    if (i == L - 1u || tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
+    return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + i) * S;
+  });
+  if (!fuse || tet0 + (1u << logLL) > nfull) return;  // no full super chunk in this wave (sact is false on every lane)
+  if (j == 0u) stage_row(q);  // chunk roots -> rows 0 .. 64/L - 1
+  wave_lds_sync();
+  switch (logL) {
+    case 1: super_butterfly<K, 1, RS>(st, lane, sact); break;
+    case 2: super_butterfly<K, 2, RS>(st, lane, sact); break;
+    default: super_butterfly<K, 3, RS>(st, lane, sact); break;
+  }
+  flush(64u >> logL, [&](uint32_t r) -> float* {  // row r: node r & (L-1) (< L-1) of super chunk r >> logL
+    const uint32_t i = r & (L - 1u), ks = r >> logL;
+    if (i == L - 1u || tet0 + ((ks + 1u) << logLL) > nfull) return nullptr;
+    return nodes + ((size_t)B.super_base + (size_t)((tet0 >> logLL) + ks) * L + 1u + i) * S;
+  });
 }
 
 // Branch chunks that are not full (the last chunk of a body when L does not
@@ -432,30 +534,61 @@ __global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__
   }
 }
 
-// One 64-lane block per super chunk of level h (per body h = H-1-step):
-// reverse BFS sweep over the chunk's internal nodes; children outside the
-// chunk are resolved roots of level h+1, refitted by the previous launch
-// (kdop_refit_tree.h:82-115).
+// Super chunks of level h (per body h = H-1-step), chunks [cfirst, C[h]) of
+// each batch item; their children outside the chunk are resolved roots of
+// level h+1, refitted before (kdop_refit_tree.h:82-115).  A full chunk (L
+// elements) is the implicit heap over its elements, so a 64-lane block takes
+// 64 / L of them at once: lane j loads element j (the child of bottom node
+// L/2 - 1 + j/2) and a butterfly builds the internal nodes, as in
+// k_refit_leaves.  The level's last chunk, when it is not full, gets a block
+// of its own and the reference's reverse BFS sweep.  gpre: prefix of blocks
+// per item.
 template <int K>
 __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                    const uint32_t* __restrict__ cpre, const uint32_t* __restrict__ chunk_base,
+                                                    const uint32_t* __restrict__ gpre, const uint32_t* __restrict__ cfirst,
+                                                    const uint32_t* __restrict__ ncnk, const uint32_t* __restrict__ chunk_base,
                                                     const uint32_t* __restrict__ chunk_n, uint32_t nbatch, float* __restrict__ nodes) {
   constexpr int S = rec_stride(K);
   __shared__ float vol[32][K];
   __shared__ uint32_t meta0[32], meta1[32];
   const uint32_t g = blockIdx.x;
-  if (g >= cpre[nbatch]) return;
-  const uint32_t item = find_item(cpre, nbatch, g);
-  const uint32_t c = g - cpre[item];
+  if (g >= gpre[nbatch]) return;
+  const uint32_t item = find_item(gpre, nbatch, g);
+  const uint32_t k = g - gpre[item];
   const BodyDev& B = bodies[batch_body[item]];
-  const uint32_t L = B.L;
-  const uint32_t nlast = chunk_n[item];  // elements of the level's last chunk
-  const uint32_t ncnk = cpre[item + 1] - cpre[item];
-  const uint32_t n = (c + 1 < ncnk) ? L : nlast;
+  const uint32_t L = B.L, logL = (uint32_t)(__ffs((int)L) - 1);
+  const uint32_t cf = cfirst[item], nc = ncnk[item], nlast = chunk_n[item];
+  const uint32_t nfc = (nc - 1u - cf) + (nlast == L ? 1u : 0u);  // full chunks from cf (cf < nc)
+  const uint32_t G = 64u >> logL;                                 // full chunks per block
+  const uint32_t lane = threadIdx.x;
+  if (k < (nfc + G - 1u) / G) {
+    const uint32_t j = lane & (L - 1u), slot = k * G + (lane >> logL);
+    const bool ok = slot < nfc;
+    const uint32_t base = chunk_base[item] + (cf + slot) * L + 1u;  // slot of BFS node 0
+    NodeRec<K> r;
+#pragma unroll
+    for (int f = 0; f < K; ++f) r.v[f] = 0.0f;
+    if (ok) {
+      const uint2 m = load_meta<K>(nodes, base + (L >> 1) - 1u + (j >> 1));
+      load_rec<K>(nodes, ((j & 1u) ? m.y : m.x) & IDX_MASK, r);
+    }
+    for (uint32_t t = 1; t <= logL; ++t) {
+      switch (t) {
+        case 1: union_level<K, 1>(r.v, j); break;
+        case 2: union_level<K, 2>(r.v, j); break;
+        case 3: union_level<K, 4>(r.v, j); break;
+        case 4: union_level<K, 8>(r.v, j); break;
+        default: union_level<K, 16>(r.v, j); break;
+      }
+      if (ok && (j & ((2u << (t - 1u)) - 1u)) == 0u) store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), r.v);
+    }
+    return;
+  }
+  // the level's last chunk, not full
+  const uint32_t n = nlast;
   if (n < 2) return;
   const uint32_t nn = n - 1;
-  const uint32_t base = chunk_base[item] + c * L + 1;  // slot of BFS node 0
-  const int lane = threadIdx.x;
+  const uint32_t base = chunk_base[item] + (nc - 1u) * L + 1;  // slot of BFS node 0
   for (uint32_t i = lane; i < nn; i += 64) {
     const uint2 m = load_meta<K>(nodes, base + i);
     meta0[i] = m.x; meta1[i] = m.y;
@@ -474,7 +607,6 @@ __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ 
     const uint32_t i = w / K, f = w - i * K;
     nodes[(size_t)(base + i) * S + f] = vol[i][f];
   }
-  (void)B;
 }
 
 // Root volume of each body = union of its top-level chunk roots

@@ -11,13 +11,13 @@ def world_coords(mesh, pose):
     return O.transform_f32(mesh.X, mesh.Y, mesh.Z, pose.p, pose.q)
 
 
-def oracle_run(wl, narrow=True, threads=1):
+def oracle_run(wl, narrow=True, threads=1, chunk_bytes=8000):
     """Oracle traversal of every body pair of a workload.  Returns a list of
     (counters, p1, p2, contacts) per pair and the oracle trees per mesh."""
     trees, worlds = [], []
     for m, pose in zip(wl.meshes, wl.poses):
         X, Y, Z = world_coords(m, pose)
-        t = O.Tree(wl.K, 8000, m.tets, m.X, m.Y, m.Z)  # make_tree on material coordinates
+        t = O.Tree(wl.K, chunk_bytes, m.tets, m.X, m.Y, m.Z)  # make_tree on material coordinates
         t.refit(X, Y, Z)                                 # update_kdop_bvh
         trees.append(t)
         worlds.append((X, Y, Z))

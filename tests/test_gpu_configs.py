@@ -114,6 +114,39 @@ def test_chunk_boundary_tet_counts(k):
             np.testing.assert_array_equal(v, ref[(int(f), int(l))])
 
 
+@pytest.mark.parametrize("k,chunk_bytes", [(26, 2000), (26, 4000), (26, 8000), (14, 8000), (8, 8000)],
+                         ids=["L2", "L4", "L8", "L16", "L32"])
+def test_refit_tree_bits_around_super_chunks(k, chunk_bytes):
+    """Tree volumes bit for bit for every chunk width L, on tet counts around
+    L and L*L: level H-2 is refitted together with the leaves where L*L <= 64
+    (whole super chunks in a wave) and by the super-chunk pass where a chunk
+    or super chunk is not full."""
+    L = K.tree_shape(k, chunk_bytes, 4096)["L"]
+    base = K.box_mesh(9, size=1.0, jitter=0.1, seed=11)
+    counts = sorted({min(c, base.tets.shape[0]) for c in
+                     (L * L - 1, L * L, L * L + 1, 2 * L * L + L - 1, 3 * L * L + 1, 4 * L * L + L + 1, 64 * L + 3, 1000, 3645)})
+    meshes = []
+    for nt in counts:
+        T = base.tets[:nt]
+        meshes.append(K.Mesh(base.X, base.Y, base.Z, T, K.compute_surface_map(base.nv, T)))
+    q = K.rng_quaternions(7, 0, len(meshes))
+    poses = [W.Pose(np.array([0.01 * i, 0.02, 0.0], np.float32), q[i]) for i in range(len(meshes))]
+    pairs = np.array([[i, (i + 1) % len(meshes)] for i in range(len(meshes))], np.uint32)
+    wl = W.PairWorkload("superchunks", k, meshes, pairs, poses)
+    orc, trees, _ = oracle_run(wl, chunk_bytes=chunk_bytes)
+    dev = device_run(wl, chunk_bytes=chunk_bytes)
+    _check_pair_results(wl, dev, orc)
+    ctx = dev["ctx"]
+    for i, bid in enumerate(dev["ids"]):
+        t = ctx.export_tree(bid)
+        np.testing.assert_array_equal(t["root"], trees[i].root())
+        ex = trees[i].export()
+        ref = {(int(f), int(l)): v for (f, l), v in zip(ex["range"], ex["vol"])}
+        assert len(ref) == len(t["range"])
+        for (f, l), v in zip(t["range"], t["vol"]):
+            np.testing.assert_array_equal(v, ref[(int(f), int(l))])
+
+
 def test_invalid_inputs_fail_loudly():
     """Unknown body ids and out-of-range options are errors, not crashes."""
     ctx = K.Context(K=8)
