@@ -79,7 +79,8 @@ struct kdop_ctx {
   DevBuf<double> adj_acc;      // adjoint accumulators [npairs][kAdjStripes][kAdjVals]
   DevBuf<uint32_t> adj_bodies;
   CallState* dstate = nullptr;    // counters and frontier segments of one call
-  CallSummary* hsum = nullptr;    // pinned copy of dstate->sum
+  CallSummary* hsum = nullptr;    // call summary, pinned host memory written by k_sum_counters
+  CallSummary* hsum_dev = nullptr;  // its device address
   uint32_t last_npairs = 0;
   bool have_traversal = false, have_p1 = false, have_narrow = false;
   std::vector<uint2> last_pairs;
@@ -442,8 +443,12 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
   for (auto& e : c->evr)
     if (hipEventCreate(&e) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   if (hipMalloc(&c->dstate, sizeof(CallState)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
-  if (hipHostMalloc(&c->hsum, sizeof(CallSummary)) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_NOMEM; }
+  if (hipHostMalloc(&c->hsum, sizeof(CallSummary), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    kdop_destroy(c);
+    return KDOP_ERR_NOMEM;
+  }
   std::memset(c->hsum, 0, sizeof(CallSummary));
+  if (hipHostGetDevicePointer((void**)&c->hsum_dev, c->hsum, 0) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
@@ -799,13 +804,26 @@ int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels
 // in place of a memset) when the call is complete: no overflow and nothing
 // left in that round's segments (-1: levels strategy, no rounds).  -2: never.
 int enqueue_summary(kdop_ctx* c, uint32_t cap, int final_round) {
-  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(kSumThreads), 0, c->stream, c->dstate, cap, final_round);
+  __atomic_store_n(&c->hsum->ready, 0u, __ATOMIC_RELEASE);
+  hipLaunchKernelGGL(k_sum_counters, dim3(1), dim3(kSumThreads), 0, c->stream, c->dstate, cap, final_round, c->hsum_dev);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(c->hsum, &c->dstate->sum, sizeof(CallSummary), hipMemcpyDeviceToHost, c->stream));
   return KDOP_OK;
 }
+// Wait for the summary's ready word (k_sum_counters writes it last); the
+// stream is polled now and then, so that a failed launch is an error and not
+// a hang.  The fold's own clearing of the call state may still run: it is
+// ordered before anything queued next on the stream.
 int finish_summary(kdop_ctx* c) {
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (uint64_t spin = 1;; ++spin) {
+    if (__atomic_load_n(&c->hsum->ready, __ATOMIC_ACQUIRE)) break;
+    if ((spin & 4095u) == 0u) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(c, KDOP_ERR_HIP, "HIP error %d (%s) in the traversal", (int)e, hipGetErrorString(e));
+      if (!__atomic_load_n(&c->hsum->ready, __ATOMIC_ACQUIRE)) return fail(c, KDOP_ERR_HIP, "the call summary was not written");
+      break;
+    }
+  }
   c->state_clean = c->hsum->reset != 0;
   return KDOP_OK;
 }
@@ -901,7 +919,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   uint64_t overflowed = 0;
   CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {
-    if (!c->state_clean) HIPCHK(c, hipMemsetAsync(d, 0, offsetof(CallState, sum), c->stream));
+    if (!c->state_clean) HIPCHK(c, hipMemsetAsync(d, 0, sizeof(CallState), c->stream));
     c->state_clean = false;
     TravArgs T;
     T.nodes = c->nodes.p; T.bodies = c->dbodies.p; T.pairs = c->dpairs.p; T.tops = c->dtops.p;
@@ -974,6 +992,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
         HIPCHK(c, hipGraphInstantiate(&c->tg_exec, c->tg, nullptr, nullptr, 0));
         c->tg_sig = sig;
       }
+      __atomic_store_n(&c->hsum->ready, 0u, __ATOMIC_RELEASE);
       HIPCHK(c, hipGraphLaunch(c->tg_exec, c->stream));
     } else if ((rc = enqueue_first())) {
       return rc;

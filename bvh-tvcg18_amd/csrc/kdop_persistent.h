@@ -31,7 +31,7 @@ struct alignas(256) SegCtr {
   uint32_t next;  // chunk counter of the round that reads the segment
 };
 
-// Summary of one call, written by k_sum_counters and copied to the host.
+// Summary of one call, written by k_sum_counters into pinned host memory.
 struct CallSummary {
   unsigned long long total[kStripeFields];  // CounterStripe fields summed (max_iters: max)
   uint32_t overflow;
@@ -42,6 +42,7 @@ struct CallSummary {
   uint32_t round_maxseg[kMaxRounds];  // largest segment fill of round r (r >= 1)
   uint32_t reset;                      // 1: the fold zeroed the call state
   uint32_t final_set;                  // a breadth-first launch recorded the frontier
+  uint32_t ready;                      // set to 1 last (system-scope release); the host clears it
 };
 
 // Device state of one traversal call, zeroed by the host before the call.
@@ -53,7 +54,6 @@ struct CallState {
   alignas(256) Fills fills;
   alignas(256) uint32_t level_fill[64];     // items per BFS level (level 0 = seeds)
   uint32_t final_level, final_count, final_set;  // level whose buffer holds the frontier, its size
-  alignas(256) CallSummary sum;
 };
 
 // One breadth-first level of the BVTT over the whole grid: expand every item
@@ -330,21 +330,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
-// Fold the striped counters and the segment fills into C->sum, one 1024-thread
-// block: wave f < kStripeFields reduces counter field f over the stripes (lane
-// s reads stripe s), waves 12..15 the segment fills of two rounds each, so
-// every load is independent and each reduction is one wave's.  With
-// final_round >= -1 and a complete call (no overflow, nothing spilled into
-// final_round's segments; -1: levels strategy, frontier found empty) the
-// block then zeroes the call state, so the next call needs no memset.
+// Fold the striped counters and the segment fills into the call summary, one
+// 1024-thread block: wave f < kStripeFields reduces counter field f over the
+// stripes (lane s reads stripe s), waves 12..15 the segment fills of two
+// rounds each, so every load is independent and each reduction is one
+// wave's.  The summary is written straight to pinned host memory (`host`,
+// mapped, coherent), and its `ready` word last, after a system-scope fence by
+// every thread, so the host waits for that word instead of a copy and a
+// stream synchronisation.  With final_round >= -1 and a complete call (no
+// overflow, nothing spilled into final_round's segments; -1: levels strategy,
+// frontier found empty) the block then zeroes the call state, so the next
+// call needs no memset.
 constexpr int kSumThreads = 1024;
-__global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round) {
+__global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round,
+                                                              CallSummary* __restrict__ host) {
   static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
   static_assert(kStripeFields <= 12 && kMaxRounds == 8, "12 field waves, 4 round waves");
   static_assert(sizeof(CounterStripe) * kStripes == 16 * kSumThreads, "one 16-byte clear per thread");
   __shared__ uint32_t pending_s;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  CallSummary& S = C->sum;
+  CallSummary& S = *host;
   if (tid == 0) pending_s = 0u;
   __syncthreads();
   if (w < (uint32_t)kStripeFields) {
@@ -387,8 +392,10 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
   __syncthreads();
   const uint32_t ovf = C->overflow;
   const bool reset = final_round >= -1 && ovf == 0u && pending_s == 0u && (final_round >= 0 || C->final_set != 0u);
-  __syncthreads();  // every thread has read what it needs before anything is cleared
   if (tid == 0) S.reset = reset ? 1u : 0u;
+  __threadfence_system();  // this thread's summary writes are visible to the host
+  __syncthreads();         // ... for every thread, and every read of the state is done before it is cleared
+  if (tid == 0) __hip_atomic_store(&S.ready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   if (!reset) return;
   reinterpret_cast<float4*>(C->stripe)[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (tid < (uint32_t)(kMaxRounds * kSegs)) {
