@@ -163,11 +163,51 @@ def batched_leg(args):
     t_single = time.perf_counter() - t0
     assert tot == c["contacts"], (tot, c["contacts"])
     n = pairs.shape[0]
+    # device contact sort of the batched call's contacts, in both report
+    # orders (after a fresh traversal each time; the sort alone is timed),
+    # beside a host lexsort of the same records
+    ctx.tandem_traversal(pairs, narrow=True)
+    con = ctx.contacts()
+    t0 = time.perf_counter()
+    np.lexsort((con["depth"], con["n"][:, 2], con["n"][:, 1], con["n"][:, 0], con["p"][:, 2], con["p"][:, 1], con["p"][:, 0],
+                con["pair"]))
+    t_host = time.perf_counter() - t0
+    sort_ms = {}
+    for order in ("sequential", "opencl"):
+        tt = 0.0
+        for _ in range(reps):
+            ctx.tandem_traversal(pairs, narrow=True)
+            t0 = time.perf_counter()
+            ctx.sort_contacts(order)
+            tt += time.perf_counter() - t0
+        sort_ms[order] = tt / reps * 1e3
     return dict(workload=wl.name, pairs=int(n), K=args.batched_k, tets_total=int(sum(wl.info["tets"])),
                 separated_pairs=wl.info["separated"],
                 pair_queries_per_s=n / t_batched, unbatched_pair_queries_per_s=n / t_single,
                 batched_speedup=t_single / t_batched, batched_ms=t_batched * 1e3, unbatched_ms=t_single * 1e3,
-                tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms)
+                tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms,
+                contact_sort={"contacts": int(con.shape[0]), "device_ms": sort_ms,
+                              "host_numpy_lexsort_opencl_order_ms": t_host * 1e3})
+
+
+def reorder_leg(args):
+    """Morton reorder of the headline mesh (40^3 box, 320k tets) on the device
+    (host arrays in and out, copies included) against the library's host
+    implementation of the same order."""
+    import kdop_amd as KA
+    X, Y, Z, T = KA.box_mesh_raw(args.n, size=1.0, jitter=0.1, seed=7)
+    ctx = KA.Context(K=args.k)
+    ctx.mesh_reorder(X, Y, Z, T)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = ctx.mesh_reorder(X, Y, Z, T)
+    t_dev = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    host = KA.mesh_reorder(X, Y, Z, T)
+    t_host = time.perf_counter() - t0
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(out, host))
+    return dict(tets=int(T.shape[0]), verts=int(X.shape[0]), device_ms=t_dev * 1e3, host_ms=t_host * 1e3, identical=same)
 
 
 def cpu_leg(args, head):
@@ -221,6 +261,7 @@ def main():
     ap.add_argument("--seed-budget", type=int, default=0, help="chunk-root pairs tested to seed a call")
     ap.add_argument("--bfs-max", type=int, default=0, help="persistent strategy: most breadth-first levels first")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-reorder", action="store_true", help="skip the device Morton reorder leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
@@ -252,6 +293,10 @@ def main():
         log(f"[bench] batched leg: {args.batched_pairs} pairs")
         batched = batched_leg(args)
         log(f"[bench] batched {batched['pair_queries_per_s']:.1f} pair-queries/s, speedup {batched['batched_speedup']:.1f}x")
+    reorder = None
+    if not args.no_reorder:
+        reorder = reorder_leg(args)
+        log(f"[bench] mesh reorder {reorder['tets']} tets: device {reorder['device_ms']:.2f} ms, host {reorder['host_ms']:.2f} ms")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline sample")
@@ -290,6 +335,7 @@ def main():
         "schedule": head["sched"],
         "narrow_ms": head["narrow_ms"],
         "batched": batched,
+        "mesh_reorder": reorder,
         "cpu_baseline": cpu,
     }
     if cpu:

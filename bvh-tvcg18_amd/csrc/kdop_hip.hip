@@ -18,6 +18,7 @@
 #include "kdop_kernels.h"
 #include "kdop_persistent.h"
 #include "kdop_adjoint.h"
+#include "kdop_sort.h"
 #include "kdop_meshgen.h"
 #include "tree_shape.h"
 
@@ -67,6 +68,8 @@ struct kdop_ctx {
   DevBuf<Item> front[2];
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
+  DevBuf<kdop_contact> contacts_tmp;  // kdop_sort_contacts: the permuted copy
+  DevBuf<unsigned char> sortbuf;       // kdop_sort_contacts: two key arrays
   DevBuf<NarrowRec> recs;
   DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
   uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
@@ -477,6 +480,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
+  release(c->contacts_tmp); release(c->sortbuf);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -830,6 +834,49 @@ int finish_summary(kdop_ctx* c) {
 int summarize(kdop_ctx* c, uint32_t cap, int final_round) {
   int rc = enqueue_summary(c, cap, final_round);
   return rc ? rc : finish_summary(c);
+}
+
+// Sort n keys (a: the keys, b: as many scratch); *result: a or b, whichever
+// holds them sorted.
+template <class Key>
+int device_sort(kdop_ctx* c, Key* a, Key* b, uint32_t n, Key** result) {
+  if (n > 1) {
+    hipLaunchKernelGGL(k_sort_tiles<Key>, dim3((n + kSortTile - 1) / kSortTile), dim3(kSortThreads), 0, c->stream, a, n);
+    for (uint64_t w = kSortTile; w < n; w <<= 1) {
+      hipLaunchKernelGGL(k_merge_pass<Key>, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const Key*)a, b, n, (uint32_t)w);
+      std::swap(a, b);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  *result = a;
+  return KDOP_OK;
+}
+
+// Exclusive scan of n uint32 (sums: ceil(n / kScanBlock) words of scratch;
+// total: the sum, on the device).
+int device_scan(kdop_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* total) {
+  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, c->stream, in, out, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, c->stream, sums, nb, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, c->stream, out, n, (const uint32_t*)sums);
+  HIPCHK(c, hipGetLastError());
+  return KDOP_OK;
+}
+
+// Sort the n contacts by Key (keys: filled by the caller's launch into the
+// first half of sortbuf), permute them into contacts_tmp and copy them back.
+template <class Key>
+int sort_contacts_keys(kdop_ctx* c, uint32_t n) {
+  Key* a = reinterpret_cast<Key*>(c->sortbuf.p);
+  Key* sorted = nullptr;
+  int rc = device_sort(c, a, a + n, n, &sorted);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_contact_permute<Key>, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p,
+                     (const Key*)sorted, n, c->contacts_tmp.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->contacts.p, c->contacts_tmp.p, (size_t)n * sizeof(kdop_contact), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
 }
 
 }  // namespace
@@ -1223,6 +1270,111 @@ int kdop_narrow_adjoint(kdop_ctx* c, uint32_t nbodies, const uint32_t* bodies, c
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
+}
+
+int kdop_sort_contacts(kdop_ctx* c, int order) {
+  if (!c) return KDOP_ERR_ARG;
+  if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_sort_contacts: last traversal ran without KDOP_NARROW");
+  if (order != KDOP_ORDER_SEQUENTIAL && order != KDOP_ORDER_OPENCL) return fail(c, KDOP_ERR_ARG, "kdop_sort_contacts: unknown order %d", order);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t n = (uint32_t)std::min<size_t>(c->hsum->fills.c, c->contacts.cap);
+  if (n < 2) return KDOP_OK;
+  int rc = ensure(c, c->sortbuf, 2 * (size_t)n * std::max(sizeof(ContactSeqKey), sizeof(ContactClKey)));
+  if (!rc) rc = ensure(c, c->contacts_tmp, n);
+  if (rc) return rc;
+  const dim3 grid((n + 255) / 256);
+  if (order == KDOP_ORDER_SEQUENTIAL) {
+    hipLaunchKernelGGL(k_contact_keys_seq, grid, dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p, n, (const uint2*)c->dpairs.p,
+                       (const BodyDev*)c->dbodies.p, reinterpret_cast<ContactSeqKey*>(c->sortbuf.p));
+    return sort_contacts_keys<ContactSeqKey>(c, n);
+  }
+  hipLaunchKernelGGL(k_contact_keys_cl, grid, dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p, n,
+                     reinterpret_cast<ContactClKey*>(c->sortbuf.p));
+  return sort_contacts_keys<ContactClKey>(c, n);
+}
+
+int kdop_mesh_reorder_gpu(kdop_ctx* c, uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt, const uint32_t* tets,
+                          float* Xo, float* Yo, float* Zo, uint32_t* tets_o) {
+  if (!c) return KDOP_ERR_ARG;
+  if (!X || !Y || !Z || !tets || !Xo || !Yo || !Zo || !tets_o) return fail(c, KDOP_ERR_ARG, "kdop_mesh_reorder_gpu: null argument");
+  if (nv == 0 || nt == 0) return fail(c, KDOP_ERR_ARG, "kdop_mesh_reorder_gpu: empty mesh (nv=%u nt=%u)", nv, nt);
+  if (nt >= (1u << 29) || nv >= (1u << 31)) return fail(c, KDOP_ERR_ARG, "kdop_mesh_reorder_gpu: mesh too large (nv=%u nt=%u)", nv, nt);
+  for (size_t i = 0; i < 4 * (size_t)nt; ++i)
+    if (tets[i] >= nv) return fail(c, KDOP_ERR_ARG, "kdop_mesh_reorder_gpu: tet vertex index %u out of range (nv=%u)", tets[i], nv);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t ns = 4 * nt, nbt = (nt + 255) / 256;
+  const uint32_t nsum = (std::max(ns, nv) + kScanBlock - 1) / kScanBlock;
+  // one allocation, carved into 256-byte aligned pieces
+  size_t off = 0;
+  auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+  const size_t oX = take(3 * (size_t)nv * 4), oXo = take(3 * (size_t)nv * 4), oT = take((size_t)ns * 4), oTo = take((size_t)ns * 4);
+  const size_t oK = take(2 * (size_t)nt * sizeof(MortonKey)), oPart = take((size_t)nbt * 6 * 4), oP = take(sizeof(ReorderParams));
+  const size_t oFirst = take((size_t)nv * 4), oFlag = take((size_t)ns * 4), oFlagS = take((size_t)ns * 4);
+  const size_t oUnused = take((size_t)nv * 4), oUnusedS = take((size_t)nv * 4), oSums = take((size_t)nsum * 4), oTot = take(8);
+  unsigned char* m = nullptr;
+  {
+    const hipError_t e = hipMalloc(&m, off);
+    if (e != hipSuccess) return fail(c, KDOP_ERR_NOMEM, "kdop_mesh_reorder_gpu: hipMalloc(%zu bytes) failed", off);
+  }
+  float* dX = reinterpret_cast<float*>(m + oX);
+  float* dY = dX + nv;
+  float* dZ = dY + nv;
+  float* dXo = reinterpret_cast<float*>(m + oXo);
+  float* dYo = dXo + nv;
+  float* dZo = dYo + nv;
+  uint32_t* dT = reinterpret_cast<uint32_t*>(m + oT);
+  uint32_t* dTo = reinterpret_cast<uint32_t*>(m + oTo);
+  MortonKey* ka = reinterpret_cast<MortonKey*>(m + oK);
+  MortonKey* kb = ka + nt;
+  float* part = reinterpret_cast<float*>(m + oPart);
+  ReorderParams* P = reinterpret_cast<ReorderParams*>(m + oP);
+  uint32_t* first = reinterpret_cast<uint32_t*>(m + oFirst);
+  uint32_t* flag = reinterpret_cast<uint32_t*>(m + oFlag);
+  uint32_t* flag_s = reinterpret_cast<uint32_t*>(m + oFlagS);
+  uint32_t* unused = reinterpret_cast<uint32_t*>(m + oUnused);
+  uint32_t* unused_s = reinterpret_cast<uint32_t*>(m + oUnusedS);
+  uint32_t* sums = reinterpret_cast<uint32_t*>(m + oSums);
+  uint32_t* tot = reinterpret_cast<uint32_t*>(m + oTot);
+  int rc = KDOP_OK;
+  auto run = [&]() -> int {
+    HIPCHK(c, hipMemcpyAsync(dX, X, (size_t)nv * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dY, Y, (size_t)nv * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dZ, Z, (size_t)nv * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dT, tets, (size_t)ns * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(first, 0xFF, (size_t)nv * 4, c->stream));
+    hipLaunchKernelGGL(k_reorder_bounds, dim3(nbt), dim3(256), 0, c->stream, (const float*)dX, (const float*)dY, (const float*)dZ,
+                       (const uint32_t*)dT, nt, part);
+    hipLaunchKernelGGL(k_reorder_scale, dim3(1), dim3(256), 0, c->stream, (const float*)part, nbt, P);
+    hipLaunchKernelGGL(k_reorder_codes, dim3(nbt), dim3(256), 0, c->stream, (const float*)dX, (const float*)dY, (const float*)dZ,
+                       (const uint32_t*)dT, nt, (const ReorderParams*)P, ka);
+    HIPCHK(c, hipGetLastError());
+    MortonKey* sorted = nullptr;
+    int r = device_sort(c, ka, kb, nt, &sorted);
+    if (r) return r;
+    const uint32_t nmax = std::max(ns, nv);
+    hipLaunchKernelGGL(k_reorder_first, dim3((ns + 255) / 256), dim3(256), 0, c->stream, (const MortonKey*)sorted, (const uint32_t*)dT, nt,
+                       first);
+    hipLaunchKernelGGL(k_reorder_flags, dim3((nmax + 255) / 256), dim3(256), 0, c->stream, (const MortonKey*)sorted, (const uint32_t*)dT, nt,
+                       (const uint32_t*)first, nv, flag, unused);
+    HIPCHK(c, hipGetLastError());
+    if ((r = device_scan(c, flag, flag_s, ns, sums, tot))) return r;
+    if ((r = device_scan(c, unused, unused_s, nv, sums, tot + 1))) return r;
+    hipLaunchKernelGGL(k_reorder_tets, dim3((ns + 255) / 256), dim3(256), 0, c->stream, (const MortonKey*)sorted, (const uint32_t*)dT, nt,
+                       (const uint32_t*)first, (const uint32_t*)flag_s, dTo);
+    hipLaunchKernelGGL(k_reorder_verts, dim3((nv + 255) / 256), dim3(256), 0, c->stream, (const float*)dX, (const float*)dY, (const float*)dZ,
+                       nv, (const uint32_t*)first, (const uint32_t*)flag_s, (const uint32_t*)unused_s, (const uint32_t*)tot, dXo, dYo, dZo);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(Xo, dXo, (size_t)nv * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Yo, dYo, (size_t)nv * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Zo, dZo, (size_t)nv * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tets_o, dTo, (size_t)ns * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return KDOP_OK;
+  };
+  rc = run();
+  if (rc) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(m);
+  return rc;
 }
 
 }  // extern "C"

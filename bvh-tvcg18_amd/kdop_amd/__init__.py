@@ -288,6 +288,26 @@ class Context:
             self._check(self._lib.kdop_get_contacts(self._h, ptr(out), n.value, 0, ctypes.byref(n)))
         return out
 
+    def sort_contacts(self, order="sequential"):
+        """Sort the last traversal's contacts on the device (kdop_sort_contacts):
+        "sequential" = (pair, tet_a, tet_b, ordinal), the sequential reference's
+        report order; "opencl" = (pair, p, n, depth), the OpenCL path's qsort order."""
+        code = {"sequential": _abi.KDOP_ORDER_SEQUENTIAL, "opencl": _abi.KDOP_ORDER_OPENCL}[order]
+        self._check(self._lib.kdop_sort_contacts(self._h, code))
+
+    def mesh_reorder(self, X, Y, Z, tets):
+        """Morton reorder of a mesh on the device (kdop_mesh_reorder_gpu): (X, Y, Z, tets) reordered."""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        Z = np.ascontiguousarray(Z, np.float32)
+        T = np.ascontiguousarray(np.asarray(tets, np.uint32).reshape(-1, 4))
+        nv, nt = X.shape[0], T.shape[0]
+        Xo, Yo, Zo = (np.empty(nv, np.float32) for _ in range(3))
+        To = np.empty((nt, 4), np.uint32)
+        self._check(self._lib.kdop_mesh_reorder_gpu(self._h, nv, ptr(X), ptr(Y), ptr(Z), nt, ptr(T), ptr(Xo), ptr(Yo), ptr(Zo),
+                                                    ptr(To)))
+        return Xo, Yo, Zo, To
+
     def world(self, body):
         nv = self.meshes[body].nv
         X = np.empty(nv, np.float32)

@@ -23,6 +23,8 @@ OPTIONS = {"traversal": 1, "queue_depth": 2, "waves_per_cu": 3, "ramp_target": 4
            "seed_budget": 6, "kernel_timing": 7, "bfs_max": 8, "async_refit": 9,
            "narrow_path": 10, "graphs": 11}
 KDOP_NARROW = 1
+KDOP_ORDER_SEQUENTIAL = 0
+KDOP_ORDER_OPENCL = 1
 KDOP_KEEP_ALL_PAIRS = 2
 
 
@@ -83,6 +85,8 @@ SIGNATURES = [
     ("kdop_traverse", ctypes.c_int, [_p, _u32, _p, _u32, _p]),
     ("kdop_get_pairs", ctypes.c_int, [_p, ctypes.c_int, _p, _u64, ctypes.c_int, _p]),
     ("kdop_get_contacts", ctypes.c_int, [_p, _p, _u64, ctypes.c_int, _p]),
+    ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
+    ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_get_world", ctypes.c_int, [_p, _u32, _p, _p, _p]),
     ("kdop_export_tree", ctypes.c_int, [_p, _u32, _u64, _p, _p, _p, _p, _p]),
     ("kdop_narrow_adjoint", ctypes.c_int, [_p, _u32, _p, _p, _p, _p, _p]),

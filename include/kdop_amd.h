@@ -131,9 +131,36 @@ int kdop_traverse(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, ui
  * device pointer. */
 int kdop_get_pairs(kdop_ctx* ctx, int which, uint32_t* dst, uint64_t cap, int dst_is_device, uint64_t* n);
 
-/* Copy the contacts of the last traversal (unsorted; sort by (pair, tet_a, tet_b,
- * ordinal) for the reference's per-pair report order). */
+/* Copy the contacts of the last traversal, in device order unless
+ * kdop_sort_contacts ran after it. */
 int kdop_get_contacts(kdop_ctx* ctx, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n);
+
+/* Orders of kdop_sort_contacts. */
+#define KDOP_ORDER_SEQUENTIAL 0 /* the callback order of kdop::tandem_traversal(..., sequential) */
+#define KDOP_ORDER_OPENCL 1     /* (pair, p, n, depth), float comparisons: the OpenCL path's report order */
+
+/* Sort the contacts of the last KDOP_NARROW traversal on the device, in place,
+ * so that kdop_get_contacts returns them in that order.  KDOP_ORDER_SEQUENTIAL
+ * is the order the sequential traversal reports them through each pair's
+ * ContactsCallback: body pair, branch chunk pairs row-major
+ * (SIMULATION/KDOP/KDOP/include/kdop_tandem_traversal.h:160-196), the
+ * depth-first descent inside a chunk pair (:18-156), then the contact
+ * generator's order (the ordinal); KDOP_ORDER_OPENCL replaces the
+ * host qsort of the OpenCL path (SIMULATION/KDOP/KDOP/include/cl/
+ * kdop_cl_tandem_traversal.h:61-104 compare_kernel_contact_points, applied at
+ * :1772-1788), with the source position breaking ties that qsort leaves open. */
+int kdop_sort_contacts(kdop_ctx* ctx, int order);
+
+/* Morton reorder of a tetrahedral mesh on the device (host arrays in and
+ * out; outputs must not alias inputs): tets sorted by the Morton code of their
+ * centroid, vertices renumbered in first-touch order, vertices no tet names
+ * appended in input order.  Replaces kdop::mesh_reorder (SIMULATION/KDOP/KDOP/
+ * include/kdop_mesh_reorder.h:172-353, called from narrow_shape_types.h:172-180)
+ * with equal codes kept in input order (the reference's std::sort leaves them in
+ * an implementation-defined order); same result as the host kdop_mesh_reorder
+ * of kdop_meshgen.h. */
+int kdop_mesh_reorder_gpu(kdop_ctx* ctx, uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt,
+                          const uint32_t* tets, float* Xo, float* Yo, float* Zo, uint32_t* tets_o);
 
 /* World coordinates of a body after the last refit (3 arrays of nv floats). */
 int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);

@@ -665,6 +665,72 @@ inline void tandem_rows(const Tree& A, const Mesh& mA, const Tree& B, const Mesh
     for (size_t b = 0; b < brB.size(); ++b) traversal(0, brA[a], mA, 0, brB[b], mB, A.N, 0, s);
 }
 
+// kdop_mesh_reorder.h:172-353 (Morton order; the default of :364-387).  The
+// reference's std::sort leaves equal codes in an implementation-defined
+// order; here they keep their input order (a stable sort), the one change.
+// Vertices that no tetrahedron names are appended in input order (the
+// reference's output mesh drops them).
+inline void mesh_reorder(uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt, const uint32_t* T, float* Xo, float* Yo,
+                         float* Zo, uint32_t* To) {
+  std::vector<float> cx(nt), cy(nt), cz(nt);
+  float min_X = std::numeric_limits<float>::max(), min_Y = min_X, min_Z = min_X;
+  float max_X = std::numeric_limits<float>::min(), max_Y = max_X, max_Z = max_X;  // :197-199 as written
+  for (uint32_t i = 0; i < nt; ++i) {                                               // :201-231
+    const uint32_t* t = T + 4 * (size_t)i;
+    cx[i] = (float)(0.25 * (X[t[0]] + X[t[1]] + X[t[2]] + X[t[3]]));
+    cy[i] = (float)(0.25 * (Y[t[0]] + Y[t[1]] + Y[t[2]] + Y[t[3]]));
+    cz[i] = (float)(0.25 * (Z[t[0]] + Z[t[1]] + Z[t[2]] + Z[t[3]]));
+    min_X = std::min(min_X, cx[i]); min_Y = std::min(min_Y, cy[i]); min_Z = std::min(min_Z, cz[i]);
+    max_X = std::max(max_X, cx[i]); max_Y = std::max(max_Y, cy[i]); max_Z = std::max(max_Z, cz[i]);
+  }
+  const int KB = (int)(sizeof(unsigned long long) * 8) / 3;  // :235
+  // :237-249: int scale = ((1 << K) - 1) / (max - min), the smallest over the
+  // axes; a zero-width axis (undefined in the reference) counts as INT_MAX
+  auto axis_scale = [&](float lo, float hi) {
+    const float q = ((1 << KB) - 1) / (hi - lo);
+    return q < 2147483647.0f ? (int)q : std::numeric_limits<int>::max();
+  };
+  int scale = axis_scale(min_X, max_X);
+  scale = std::min(scale, axis_scale(min_Y, max_Y));
+  scale = std::min(scale, axis_scale(min_Z, max_Z));
+  std::vector<std::pair<unsigned long long, uint32_t>> codes(nt);
+  for (uint32_t i = 0; i < nt; ++i) {
+    // :251-258: the integer coordinate is stored back into the float attribute
+    const float qx = (float)static_cast<int>((cx[i] - min_X) * scale);
+    const float qy = (float)static_cast<int>((cy[i] - min_Y) * scale);
+    const float qz = (float)static_cast<int>((cz[i] - min_Z) * scale);
+    const unsigned long long x = (unsigned long long)qx, y = (unsigned long long)qy, z = (unsigned long long)qz;
+    unsigned long long code = 0;
+    for (unsigned long k = 0; k < (unsigned long)KB; ++k) {  // :274-280
+      code |= ((x >> k) & 1) << (3 * k);
+      code |= ((y >> k) & 1) << (3 * k + 1);
+      code |= ((z >> k) & 1) << (3 * k + 2);
+    }
+    codes[i] = std::make_pair(code, i);
+  }
+  std::stable_sort(codes.begin(), codes.end(),
+                   [](const std::pair<unsigned long long, uint32_t>& a, const std::pair<unsigned long long, uint32_t>& b) { return a.first < b.first; });
+  const uint32_t UNASSIGNED = 0xFFFFFFFFu;  // :300-350: vertices in first-touch order
+  std::vector<uint32_t> lookup(nv, UNASSIGNED);
+  uint32_t pushed = 0;
+  for (uint32_t i = 0; i < nt; ++i) {
+    const uint32_t* t = T + 4 * (size_t)codes[i].second;
+    for (int q = 0; q < 4; ++q) {
+      if (lookup[t[q]] == UNASSIGNED) {
+        lookup[t[q]] = pushed;
+        Xo[pushed] = X[t[q]]; Yo[pushed] = Y[t[q]]; Zo[pushed] = Z[t[q]];
+        ++pushed;
+      }
+      To[4 * (size_t)i + q] = lookup[t[q]];
+    }
+  }
+  for (uint32_t v = 0; v < nv; ++v)
+    if (lookup[v] == UNASSIGNED) {
+      Xo[pushed] = X[v]; Yo[pushed] = Y[v]; Zo[pushed] = Z[v];
+      ++pushed;
+    }
+}
+
 }  // namespace orc
 
 // =================================================================== C ABI
@@ -678,6 +744,14 @@ int orc_dir_table_f32(int K, float* out) {
   const int n = dir_table<float>(K, D);
   for (int i = 0; i < n; ++i) { out[3 * i] = D[i].x; out[3 * i + 1] = D[i].y; out[3 * i + 2] = D[i].z; }
   return n;
+}
+
+// Morton reorder of a mesh (kdop_mesh_reorder.h:172-353, stable on equal codes).
+int orc_mesh_reorder(uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt, const uint32_t* tets, float* Xo, float* Yo,
+                     float* Zo, uint32_t* To) {
+  if (nv == 0 || nt == 0) return -1;
+  mesh_reorder(nv, X, Y, Z, nt, tets, Xo, Yo, Zo, To);
+  return 0;
 }
 
 // Chunking shape (kdop_make_tree.h:216-229): out = {node_bytes, N_perfect, L, C, H}.

@@ -32,6 +32,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_dir_table_f32.argtypes = [ctypes.c_int, _p]
         L.orc_tree_shape.argtypes = [ctypes.c_int, _u64, _u64, _p]
+        L.orc_mesh_reorder.argtypes = [_u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]
         L.orc_tree_build.restype = _p
         L.orc_tree_build.argtypes = [ctypes.c_int, _u64, _u32, _p, _p, _p, _p]
         L.orc_tree_free.argtypes = [_p]
@@ -74,6 +75,18 @@ def dir_table(K):
     out = np.zeros((13, 3), np.float32)
     n = lib().orc_dir_table_f32(K, _ptr(out))
     return out[:n]
+
+
+def mesh_reorder(X, Y, Z, tets):
+    """Morton reorder (kdop_mesh_reorder.h:172-353, stable on equal codes): (X, Y, Z, tets) reordered."""
+    X, Y, Z = (np.ascontiguousarray(a, np.float32) for a in (X, Y, Z))
+    T = np.ascontiguousarray(np.asarray(tets, np.uint32).reshape(-1, 4))
+    nv, nt = X.shape[0], T.shape[0]
+    Xo, Yo, Zo = (np.empty(nv, np.float32) for _ in range(3))
+    To = np.empty((nt, 4), np.uint32)
+    if lib().orc_mesh_reorder(nv, _ptr(X), _ptr(Y), _ptr(Z), nt, _ptr(T), _ptr(Xo), _ptr(Yo), _ptr(Zo), _ptr(To)):
+        raise ValueError("orc_mesh_reorder: empty mesh")
+    return Xo, Yo, Zo, To
 
 
 def tree_shape(K, chunk_bytes, nt):
