@@ -234,10 +234,17 @@ def cpu_leg(args, head):
     c, s = O.traverse_rows(trees[0], meshA, trees[1], meshB, 0, rows)
     tests_per_s = c["tests"] / s
     full_query_s = s * rows_total / rows
-    return dict(value=tests_per_s, unit="node-pair tests/s", cores=1, kind="port",
-                sample=f"oracle sequential tandem traversal + narrow phase, branch rows 0..{rows} of {rows_total} "
-                       f"of body A vs all {trees[1].levels()[-1]} branches of body B ({c['tests']} tests in {s:.2f} s)",
-                est_full_query_s=full_query_s, sample_seconds=s)
+    one = dict(value=tests_per_s, unit="node-pair tests/s", cores=1, kind="port",
+               sample=f"oracle sequential tandem traversal + narrow phase, branch rows 0..{rows} of {rows_total} "
+                      f"of body A vs all {trees[1].levels()[-1]} branches of body B ({c['tests']} tests in {s:.2f} s)",
+               est_full_query_s=full_query_s, sample_seconds=s)
+    # the same query on every host core this job may use (rows taken one at a time), whole query
+    cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+    cm, sm = O.traverse_rows_mt(trees[0], meshA, trees[1], meshB, 0, rows_total, cores)
+    allc = dict(value=cm["tests"] / sm, unit="node-pair tests/s", cores=cores, kind="port",
+                sample=f"oracle tandem traversal + narrow phase, all {rows_total} branch rows of body A on {cores} threads "
+                       f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm)
+    return one, allc
 
 
 def main():
@@ -297,11 +304,12 @@ def main():
     if not args.no_reorder:
         reorder = reorder_leg(args)
         log(f"[bench] mesh reorder {reorder['tets']} tets: device {reorder['device_ms']:.2f} ms, host {reorder['host_ms']:.2f} ms")
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         log("[bench] cpu baseline sample")
-        cpu = cpu_leg(args, head)
-        log(f"[bench] cpu {cpu['value']:.3e} tests/s, est. full query {cpu['est_full_query_s']:.1f} s")
+        cpu, cpu_all = cpu_leg(args, head)
+        log(f"[bench] cpu {cpu['value']:.3e} tests/s, est. full query {cpu['est_full_query_s']:.1f} s; "
+            f"{cpu_all['cores']} cores: {cpu_all['full_query_s']:.2f} s")
     steps_per_s = 1e3 / head["ms_per_step"]
     kernel = f"k_persist<{args.k},{head['sched']['queue_depth']}>"
     traffic = measured_traffic(kernel)
@@ -337,9 +345,11 @@ def main():
         "batched": batched,
         "mesh_reorder": reorder,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     if cpu:
         out["vs_cpu_query_speedup"] = cpu["est_full_query_s"] * 1e3 / head["ms_per_step"]
+        out["vs_cpu_all_cores_query_speedup"] = cpu_all["full_query_s"] * 1e3 / head["ms_per_step"]
     print(json.dumps(out), flush=True)
 
 

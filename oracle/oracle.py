@@ -49,6 +49,8 @@ def lib():
         L.orc_traverse.argtypes = [_p, ctypes.c_int, _p, _p, _p, _p, _p, _p, ctypes.c_int, _p, _p, _p, _p, _p,
                                    ctypes.c_int, ctypes.c_int, _p, _u64, _p, _u64, _p, _u64, _p, _p]
         L.orc_traverse_rows.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int, _p, _p]
+        L.orc_traverse_rows_mt.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int,
+                                           ctypes.c_int, _p, _p]
         L.orc_brute_pairs.restype = _u64
         L.orc_brute_pairs.argtypes = [_p, _p, _p, _u64]
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
@@ -203,6 +205,19 @@ def traverse_rows(tree_a, mesh_a, tree_b, mesh_b, row0, row1, narrow=True):
     lib().orc_traverse_rows(tree_a.h, _ptr(ta), _ptr(Xa), _ptr(Ya), _ptr(Za), _ptr(Sa),
                             tree_b.h, _ptr(tb), _ptr(Xb), _ptr(Yb), _ptr(Zb), _ptr(Sb),
                             int(row0), int(row1), int(narrow), _ptr(cnt), ctypes.byref(sec))
+    return dict(zip(COUNTER_NAMES, (int(v) for v in cnt))), sec.value
+
+
+def traverse_rows_mt(tree_a, mesh_a, tree_b, mesh_b, row0, row1, threads, narrow=True):
+    """All-cores timing sample: branch rows [row0, row1) of A against all of B on
+    `threads` threads taking rows dynamically.  Returns (counters dict, seconds)."""
+    ta, Xa, Ya, Za, Sa = (np.ascontiguousarray(a) for a in mesh_a)
+    tb, Xb, Yb, Zb, Sb = (np.ascontiguousarray(a) for a in mesh_b)
+    cnt = np.zeros(9, np.uint64)
+    sec = ctypes.c_double()
+    lib().orc_traverse_rows_mt(tree_a.h, _ptr(ta), _ptr(Xa), _ptr(Ya), _ptr(Za), _ptr(Sa),
+                               tree_b.h, _ptr(tb), _ptr(Xb), _ptr(Yb), _ptr(Zb), _ptr(Sb),
+                               int(row0), int(row1), int(narrow), int(threads), _ptr(cnt), ctypes.byref(sec))
     return dict(zip(COUNTER_NAMES, (int(v) for v in cnt))), sec.value
 
 
