@@ -63,10 +63,21 @@ def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle")])
 
 
+def build_glue_test():
+    """tests/glue/glue_test: a C++ caller of include/kdop_amd.hpp, checked against the oracle."""
+    src = os.path.join(ROOT, "tests", "glue", "glue_test.cpp")
+    out = os.path.join(ROOT, "tests", "glue", "glue_test")
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, src, "-o", out,
+         "-L", LIBDIR, "-lkdop_amd", "-L", os.path.join(ROOT, "oracle"), "-loracle",
+         f"-Wl,-rpath,{LIBDIR}", f"-Wl,-rpath,{os.path.join(ROOT, 'oracle')}", "-Wl,-rpath,$ORIGIN/../../bvh-tvcg18_amd/lib",
+         "-Wl,-rpath,$ORIGIN/../../oracle", "-lpthread"])
+
+
 def main(argv):
     build_library()
     if "--oracle" in argv:
         build_oracle()
+        build_glue_test()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,142 @@
+/* kdop_amd.hpp -- the reference's KDOP call shapes in C++ over the C ABI of
+ * kdop_amd.h (header only).
+ *
+ * The reference drives its KDOP path from three call sites:
+ *   narrow::make_kdop_bvh         -> kdop::make_tree      (kdop_make_tree.h:190-288)
+ *   narrow::update_kdop_bvh(pool) -> transform + refit per body (narrow_update_kdop_bvh.h:126-176)
+ *   dispatch_tetramesh_tetramesh  -> kdop::tandem_traversal(std::vector<TestPair>&)
+ *                                    (kdop_tandem_traversal.h:198-226), contacts
+ *                                    reported through each pair's ContactsCallback
+ * This header gives the same three operations for a caller that keeps its own
+ * mesh arrays: trees stay in device memory across steps, a whole pool is
+ * refitted in one call, and a whole pair list is traversed in one call, after
+ * which every pair's callback receives its contacts in the order the
+ * sequential reference reports them (kdop_sort_contacts, KDOP_ORDER_SEQUENTIAL).
+ * Errors throw kdop_amd::Error (the reference exits the process instead).
+ * One Context per host thread.
+ */
+#ifndef KDOP_AMD_HPP
+#define KDOP_AMD_HPP
+
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kdop_amd.h"
+
+namespace kdop_amd {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int rc, const std::string& what) : std::runtime_error(what), code(rc) {}
+};
+
+/* The tree of one body: kdop::Tree<T,K> (kdop_tree.h:12-200) is held by value
+ * in narrow::Object; here the context holds it and the caller keeps the id. */
+struct Tree {
+  uint32_t body = 0xFFFFFFFFu;
+};
+
+/* geometry::ContactsCallback (contacts/geometry_contacts_callback.h:7-24). */
+struct ContactsCallback {
+  virtual ~ContactsCallback() {}
+  virtual void operator()(const float p[3], const float n[3], float depth) = 0;
+};
+
+/* kdop::TestPair (kdop_test_pair.h:15-87): two trees and the pair's callback
+ * (the meshes and world coordinates live with the trees here). */
+struct TestPair {
+  const Tree* tree_a;
+  const Tree* tree_b;
+  ContactsCallback* callback;
+};
+
+/* narrow::KDopBvhUpdateWorkItem (narrow_update_kdop_bvh.h:28-120): the body
+ * and its pose, r = rotate(q, r0) + p with q = (x, y, z, w). */
+struct UpdateWorkItem {
+  const Tree* tree;
+  float p[3];
+  float q[4];
+};
+
+class Context {
+ public:
+  explicit Context(int K, uint64_t chunk_bytes = 8000, int device = 0) {
+    kdop_config cfg;
+    kdop_default_config(&cfg);
+    cfg.K = K;
+    cfg.chunk_bytes = chunk_bytes;
+    cfg.device = device;
+    const int rc = kdop_create(&c_, &cfg);
+    if (rc != KDOP_OK) throw Error(rc, "kdop_create failed");
+  }
+  ~Context() { kdop_destroy(c_); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  kdop_ctx* get() const { return c_; }
+
+  /* kdop::make_tree on the material coordinates: tets holds 4 vertex
+   * indices per tetrahedron, surface one byte per tetrahedron (bit v: the face
+   * opposite its v-th vertex has no neighbour, compute_surface_map's flags). */
+  Tree make_tree(const std::vector<float>& X, const std::vector<float>& Y, const std::vector<float>& Z, const std::vector<uint32_t>& tets,
+                 const std::vector<uint8_t>& surface) {
+    if (X.size() != Y.size() || X.size() != Z.size() || tets.size() % 4 != 0 || surface.size() != tets.size() / 4)
+      throw Error(KDOP_ERR_ARG, "make_tree: inconsistent mesh arrays");
+    Tree t;
+    check(kdop_add_body(c_, (uint32_t)X.size(), X.data(), Y.data(), Z.data(), (uint32_t)(tets.size() / 4), tets.data(), surface.data(),
+                        &t.body));
+    return t;
+  }
+
+  /* narrow::update_kdop_bvh(pool): every body of the pool transformed and
+   * refitted in one call. */
+  void update_kdop_bvh(const std::vector<UpdateWorkItem>& pool) {
+    std::vector<uint32_t> bodies;
+    std::vector<float> pos, quat;
+    bodies.reserve(pool.size());
+    for (const UpdateWorkItem& w : pool) {
+      bodies.push_back(w.tree->body);
+      pos.insert(pos.end(), w.p, w.p + 3);
+      quat.insert(quat.end(), w.q, w.q + 4);
+    }
+    check(kdop_refit(c_, (uint32_t)bodies.size(), bodies.data(), pos.data(), quat.data()));
+  }
+
+  /* kdop::tandem_traversal(pool) with MOST_OPPOSING_SURFACES contacts: one
+   * traversal of every pair, then each pair's callback is called with its
+   * contacts in `order` (KDOP_ORDER_SEQUENTIAL: the sequential reference's
+   * order; KDOP_ORDER_OPENCL: the OpenCL path's).  Returns the counters. */
+  kdop_counters tandem_traversal(const std::vector<TestPair>& pool, int order = KDOP_ORDER_SEQUENTIAL) {
+    std::vector<uint32_t> ids;
+    ids.reserve(2 * pool.size());
+    for (const TestPair& tp : pool) {
+      ids.push_back(tp.tree_a->body);
+      ids.push_back(tp.tree_b->body);
+    }
+    kdop_counters cnt;
+    check(kdop_traverse(c_, (uint32_t)pool.size(), ids.data(), KDOP_NARROW, &cnt));
+    check(kdop_sort_contacts(c_, order));
+    uint64_t n = 0;
+    check(kdop_get_contacts(c_, nullptr, 0, 0, &n));
+    contacts_.resize((size_t)n);
+    if (n) check(kdop_get_contacts(c_, contacts_.data(), n, 0, &n));
+    for (const kdop_contact& k : contacts_) {
+      ContactsCallback* cb = pool[k.pair].callback;
+      if (cb) (*cb)(k.p, k.n, k.depth);
+    }
+    return cnt;
+  }
+
+ private:
+  kdop_ctx* c_ = nullptr;
+  std::vector<kdop_contact> contacts_;
+  void check(int rc) const {
+    if (rc != KDOP_OK) throw Error(rc, kdop_last_error(c_));
+  }
+};
+
+}  // namespace kdop_amd
+
+#endif  // KDOP_AMD_HPP

@@ -1,0 +1,127 @@
+// C++ caller of include/kdop_amd.hpp against the oracle (test infrastructure):
+// three box meshes in the reference's call shapes (make_tree, a pool refit,
+// a pair-list tandem traversal with one ContactsCallback per pair), two pose
+// steps; every pair's callback must receive exactly the contacts the oracle
+// reports for that pair, in the oracle's order, bit for bit.  Exit status 0
+// when all match.  Build: python bvh-tvcg18_amd/build.py --oracle.
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kdop_amd.hpp"
+#include "kdop_meshgen.h"
+
+// oracle/liboracle.so (the CPU restatement of the sequential reference)
+extern "C" {
+struct OrcContact {
+  uint32_t ta, tb;
+  float p[3], n[3], d;
+};
+void* orc_tree_build(int K, uint64_t mem_bytes, uint32_t nt, const uint32_t* tets, const float* X, const float* Y, const float* Z);
+void orc_tree_free(void* h);
+void orc_tree_refit(void* h, const uint32_t* tets, const float* X, const float* Y, const float* Z);
+void orc_transform_f32(uint32_t nv, const float* X0, const float* Y0, const float* Z0, const float* p, const float* q, float* X, float* Y,
+                       float* Z);
+int orc_traverse(void* hA, int ntA, const uint32_t* tetsA, const float* XA, const float* YA, const float* ZA, const uint8_t* surfA, void* hB,
+                 int ntB, const uint32_t* tetsB, const float* XB, const float* YB, const float* ZB, const uint8_t* surfB, int do_narrow,
+                 int threads, uint32_t* p1, uint64_t p1_cap, uint32_t* p2, uint64_t p2_cap, OrcContact* contacts, uint64_t c_cap,
+                 uint64_t* counters, double* seconds);
+}
+
+struct MeshData {
+  std::vector<float> X, Y, Z;
+  std::vector<uint32_t> tets;
+  std::vector<uint8_t> surf;
+};
+
+static MeshData box(uint32_t n, float size, double jitter, uint64_t seed) {
+  uint32_t nv = 0, nt = 0;
+  kdop_box_mesh_size(n, &nv, &nt);
+  MeshData raw, m;
+  raw.X.resize(nv); raw.Y.resize(nv); raw.Z.resize(nv); raw.tets.resize(4 * (size_t)nt);
+  kdop_make_box_mesh(n, size, jitter, seed, raw.X.data(), raw.Y.data(), raw.Z.data(), raw.tets.data());
+  m.X.resize(nv); m.Y.resize(nv); m.Z.resize(nv); m.tets.resize(4 * (size_t)nt); m.surf.resize(nt);
+  kdop_mesh_reorder(nv, raw.X.data(), raw.Y.data(), raw.Z.data(), nt, raw.tets.data(), m.X.data(), m.Y.data(), m.Z.data(), m.tets.data());
+  kdop_compute_surface_map(nv, nt, m.tets.data(), m.surf.data());
+  return m;
+}
+
+struct Collect : kdop_amd::ContactsCallback {
+  std::vector<float> v;  // 7 floats per contact: p, n, depth
+  void operator()(const float p[3], const float n[3], float depth) override {
+    v.insert(v.end(), p, p + 3);
+    v.insert(v.end(), n, n + 3);
+    v.push_back(depth);
+  }
+};
+
+int main() {
+  const int K = 26;
+  std::vector<MeshData> meshes = {box(8, 1.0f, 0.1, 1), box(10, 1.2f, 0.1, 2), box(6, 0.8f, 0.1, 3)};
+  const uint32_t pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};
+  std::vector<float> quats(4 * meshes.size() * 2);
+  kdop_rng_quaternions(21, 0, (uint64_t)(meshes.size() * 2), quats.data());
+  try {
+    kdop_amd::Context ctx(K);
+    std::vector<kdop_amd::Tree> trees;
+    std::vector<void*> orc;
+    for (const MeshData& m : meshes) {
+      trees.push_back(ctx.make_tree(m.X, m.Y, m.Z, m.tets, m.surf));
+      orc.push_back(orc_tree_build(K, 8000, (uint32_t)m.surf.size(), m.tets.data(), m.X.data(), m.Y.data(), m.Z.data()));
+    }
+    int bad = 0;
+    size_t total = 0;
+    for (int step = 0; step < 2; ++step) {
+      std::vector<kdop_amd::UpdateWorkItem> pool;
+      std::vector<MeshData> world(meshes.size());
+      for (size_t i = 0; i < meshes.size(); ++i) {
+        kdop_amd::UpdateWorkItem w;
+        w.tree = &trees[i];
+        // side by side with partial overlap (surfaces facing each other), moving a little per step
+        static const float base[3][3] = {{0.0f, 0.0f, 0.0f}, {0.95f, 0.05f, 0.0f}, {0.3f, 0.75f, 0.1f}};
+        w.p[0] = base[i][0] + 0.02f * (float)step; w.p[1] = base[i][1]; w.p[2] = base[i][2] - 0.01f * (float)step;
+        std::memcpy(w.q, &quats[4 * (i + meshes.size() * step)], 4 * sizeof(float));
+        pool.push_back(w);
+        const MeshData& m = meshes[i];
+        world[i].X.resize(m.X.size()); world[i].Y.resize(m.X.size()); world[i].Z.resize(m.X.size());
+        orc_transform_f32((uint32_t)m.X.size(), m.X.data(), m.Y.data(), m.Z.data(), w.p, w.q, world[i].X.data(), world[i].Y.data(),
+                          world[i].Z.data());
+        orc_tree_refit(orc[i], m.tets.data(), world[i].X.data(), world[i].Y.data(), world[i].Z.data());
+      }
+      ctx.update_kdop_bvh(pool);
+      std::vector<Collect> got(3);
+      std::vector<kdop_amd::TestPair> tps;
+      for (int k = 0; k < 3; ++k) tps.push_back(kdop_amd::TestPair{&trees[pairs[k][0]], &trees[pairs[k][1]], &got[k]});
+      ctx.tandem_traversal(tps);
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t a = pairs[k][0], b = pairs[k][1];
+        std::vector<OrcContact> want(1 << 16);
+        uint64_t cnt[9];
+        orc_traverse(orc[a], 0, meshes[a].tets.data(), world[a].X.data(), world[a].Y.data(), world[a].Z.data(), meshes[a].surf.data(), orc[b], 0,
+                     meshes[b].tets.data(), world[b].X.data(), world[b].Y.data(), world[b].Z.data(), meshes[b].surf.data(), 1, 1, nullptr, 0,
+                     nullptr, 0, want.data(), want.size(), cnt, nullptr);
+        const size_t nw = (size_t)cnt[6];
+        const size_t ng = got[k].v.size() / 7;
+        bool same = nw == ng && nw <= want.size();
+        for (size_t i = 0; same && i < nw; ++i) {
+          const float w7[7] = {want[i].p[0], want[i].p[1], want[i].p[2], want[i].n[0], want[i].n[1], want[i].n[2], want[i].d};
+          same = std::memcmp(w7, &got[k].v[7 * i], sizeof(w7)) == 0;
+        }
+        std::printf("step %d pair %d: %zu contacts (oracle %zu) %s\n", step, k, ng, nw, same ? "match" : "MISMATCH");
+        bad += same ? 0 : 1;
+        total += ng;
+      }
+    }
+    for (void* h : orc) orc_tree_free(h);
+    if (total == 0) {
+      std::printf("no contacts: the workload does not test anything\n");
+      return 2;
+    }
+    return bad ? 1 : 0;
+  } catch (const kdop_amd::Error& e) {
+    std::printf("error %d: %s\n", e.code, e.what());
+    return 3;
+  }
+}

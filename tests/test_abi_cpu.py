@@ -51,3 +51,19 @@ def test_tree_shape_matches_quoted_workload_shapes():
         assert (s["L"], s["C"], s["H"]) == (16, 20000, 4)
     s = K.tree_shape(14, 8000, 50000)
     assert (s["L"], s["C"], s["H"]) == (16, 3125, 3)
+
+
+def test_cpp_header_compiles():
+    """include/kdop_amd.hpp compiles on its own with the host compiler."""
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with tempfile.NamedTemporaryFile("w", suffix=".cpp", delete=False) as f:
+        f.write('#include "kdop_amd.hpp"\nint main() { kdop_amd::Tree t; return (int)(t.body & 0u); }\n')
+        src = f.name
+    try:
+        r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(root, "include"), src],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    finally:
+        os.unlink(src)
