@@ -190,6 +190,34 @@ def batched_leg(args):
                               "host_numpy_lexsort_opencl_order_ms": t_host * 1e3})
 
 
+def ray_set(n, seed=5):
+    """Rays at the headline body A: origins on a sphere of radius 2 around its
+    pose, aimed at points of its box (a few start inside or point away)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = 2.0 * u
+    d = rng.uniform(-0.5, 0.5, size=(n, 3)) - o
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def raycast_leg(args, head):
+    """kdop_raycast of many rays at body A of the headline pair (its refitted pose)."""
+    ctx, body = head["ctx"], head["ids"][0]
+    pose = head["wl"].poses[0]
+    o, d = ray_set(args.rays)
+    o = o + pose.p[None, :]
+    ctx.raycast(body, o, d)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hits, lengths, flags = ctx.raycast(body, o, d)
+    dt = (time.perf_counter() - t0) / reps
+    return dict(rays=int(o.shape[0]), hits=int(flags.sum()), ms=dt * 1e3, rays_per_s=o.shape[0] / dt,
+                body_tets=int(head["wl"].info["nt"]), note="host arrays in and out, copies included")
+
+
 def reorder_leg(args):
     """Morton reorder of the headline mesh (40^3 box, 320k tets) on the device
     (host arrays in and out, copies included) against the library's host
@@ -238,13 +266,28 @@ def cpu_leg(args, head):
                sample=f"oracle sequential tandem traversal + narrow phase, branch rows 0..{rows} of {rows_total} "
                       f"of body A vs all {trees[1].levels()[-1]} branches of body B ({c['tests']} tests in {s:.2f} s)",
                est_full_query_s=full_query_s, sample_seconds=s)
+    # kdop::raycast as the reference runs it (every branch root per ray), one thread, on a sample of the rays
+    ray = None
+    if head.get("raycast"):
+        o, d = ray_set(head["raycast"]["rays"])
+        o = o + wl.poses[0].p[None, :]
+        nr = 8
+        import time as _t
+        t0 = _t.perf_counter()
+        O.raycast(trees[0], meshA, o[:nr], d[:nr])
+        per = (_t.perf_counter() - t0) / nr
+        nr = int(min(o.shape[0], max(nr, args.cpu_seconds / 3 / max(per, 1e-6))))
+        t0 = _t.perf_counter()
+        O.raycast(trees[0], meshA, o[:nr], d[:nr])
+        sr = _t.perf_counter() - t0
+        ray = dict(value=nr / sr, unit="rays/s", cores=1, kind="port", sample=f"oracle kdop::raycast, first {nr} rays of the set ({sr:.2f} s)")
     # the same query on every host core this job may use (rows taken one at a time), whole query
     cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
     cm, sm = O.traverse_rows_mt(trees[0], meshA, trees[1], meshB, 0, rows_total, cores)
     allc = dict(value=cm["tests"] / sm, unit="node-pair tests/s", cores=cores, kind="port",
                 sample=f"oracle tandem traversal + narrow phase, all {rows_total} branch rows of body A on {cores} threads "
                        f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm)
-    return one, allc
+    return one, allc, ray
 
 
 def main():
@@ -269,6 +312,7 @@ def main():
     ap.add_argument("--bfs-max", type=int, default=0, help="persistent strategy: most breadth-first levels first")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reorder", action="store_true", help="skip the device Morton reorder leg")
+    ap.add_argument("--rays", type=int, default=65536, help="rays of the raycast leg (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
@@ -300,6 +344,9 @@ def main():
         log(f"[bench] batched leg: {args.batched_pairs} pairs")
         batched = batched_leg(args)
         log(f"[bench] batched {batched['pair_queries_per_s']:.1f} pair-queries/s, speedup {batched['batched_speedup']:.1f}x")
+    if args.rays:
+        head["raycast"] = raycast_leg(args, head)
+        log(f"[bench] raycast {head['raycast']['rays']} rays: {head['raycast']['ms']:.2f} ms")
     reorder = None
     if not args.no_reorder:
         reorder = reorder_leg(args)
@@ -307,7 +354,10 @@ def main():
     cpu = cpu_all = None
     if not args.no_cpu:
         log("[bench] cpu baseline sample")
-        cpu, cpu_all = cpu_leg(args, head)
+        cpu, cpu_all, cpu_ray = cpu_leg(args, head)
+        if cpu_ray and head.get("raycast"):
+            head["raycast"]["cpu_baseline"] = cpu_ray
+            head["raycast"]["vs_cpu"] = head["raycast"]["rays_per_s"] / cpu_ray["value"]
         log(f"[bench] cpu {cpu['value']:.3e} tests/s, est. full query {cpu['est_full_query_s']:.1f} s; "
             f"{cpu_all['cores']} cores: {cpu_all['full_query_s']:.2f} s")
     steps_per_s = 1e3 / head["ms_per_step"]
@@ -344,6 +394,7 @@ def main():
         "narrow_ms": head["narrow_ms"],
         "batched": batched,
         "mesh_reorder": reorder,
+        "raycast": head.get("raycast"),
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

@@ -19,6 +19,7 @@
 #include "kdop_persistent.h"
 #include "kdop_adjoint.h"
 #include "kdop_sort.h"
+#include "kdop_raycast.h"
 #include "kdop_meshgen.h"
 #include "tree_shape.h"
 
@@ -70,6 +71,7 @@ struct kdop_ctx {
   DevBuf<kdop_contact> contacts;
   DevBuf<kdop_contact> contacts_tmp;  // kdop_sort_contacts: the permuted copy
   DevBuf<unsigned char> sortbuf;       // kdop_sort_contacts: two key arrays
+  DevBuf<unsigned char> raybuf;        // kdop_raycast: rays in, hits out
   DevBuf<NarrowRec> recs;
   DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
   uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
@@ -262,6 +264,13 @@ struct RefitSuper {
                   const uint32_t* cbase, const uint32_t* cn, uint32_t n) {
     hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, gpre, cfirst, ncnk, cbase, cn, n,
                        c->nodes.p);
+  }
+};
+template <int K>
+struct Raycast {
+  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* o, const float* d, float* hits, float* lengths, uint8_t* flags) {
+    hipLaunchKernelGGL(k_raycast<K>, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->dbodies.p, body, c->dtops.p, c->nodes.p, c->dt, n, o, d,
+                       hits, lengths, flags);
   }
 };
 template <int K>
@@ -480,7 +489,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
-  release(c->contacts_tmp); release(c->sortbuf);
+  release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -1375,6 +1384,37 @@ int kdop_mesh_reorder_gpu(kdop_ctx* c, uint32_t nv, const float* X, const float*
   if (rc) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(m);
   return rc;
+}
+
+int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, const float* directions, float* hit_points, float* lengths,
+                 uint8_t* hits) {
+  if (!c) return KDOP_ERR_ARG;
+  if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_raycast: unknown body %u", body);
+  if (n == 0) return KDOP_OK;
+  if (!origins || !directions || !hit_points || !lengths || !hits) return fail(c, KDOP_ERR_ARG, "kdop_raycast: null argument");
+  if (n >= (1u << 28)) return fail(c, KDOP_ERR_ARG, "kdop_raycast: too many rays (%u)", n);
+  const BodyHost& b = c->bodies[body];
+  uint32_t lg = 0;
+  while ((1u << lg) < b.L) ++lg;
+  if (b.H * std::max(lg, 1u) + 2 > kRayStack) return fail(c, KDOP_ERR_ARG, "kdop_raycast: tree too deep for the ray stack");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const size_t bytes_in = 6 * (size_t)n * sizeof(float), bytes_out = 4 * (size_t)n * sizeof(float);
+  int rc = ensure(c, c->raybuf, bytes_in + bytes_out + n);
+  if (rc) return rc;
+  float* d_o = reinterpret_cast<float*>(c->raybuf.p);
+  float* d_d = d_o + 3 * (size_t)n;
+  float* d_h = d_d + 3 * (size_t)n;
+  float* d_l = d_h + 3 * (size_t)n;
+  uint8_t* d_f = reinterpret_cast<uint8_t*>(d_l + n);
+  HIPCHK(c, hipMemcpyAsync(d_o, origins, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_d, directions, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  dispatch_K<Raycast>(c->K, c, body, n, (const float*)d_o, (const float*)d_d, d_h, d_l, d_f);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(hit_points, d_h, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lengths, d_l, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hits, d_f, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
 }
 
 }  // extern "C"

@@ -308,6 +308,19 @@ class Context:
                                                     ptr(To)))
         return Xo, Yo, Zo, To
 
+    def raycast(self, body, origins, directions):
+        """kdop_raycast: (hit points (n, 3), lengths (n,), hit flags (n,) bool) of n rays at `body`."""
+        o = np.ascontiguousarray(np.asarray(origins, np.float32).reshape(-1, 3))
+        d = np.ascontiguousarray(np.asarray(directions, np.float32).reshape(-1, 3))
+        n = o.shape[0]
+        if d.shape[0] != n:
+            raise ValueError("origins and directions differ in length")
+        hits = np.zeros((n, 3), np.float32)
+        lengths = np.zeros(n, np.float32)
+        flags = np.zeros(n, np.uint8)
+        self._check(self._lib.kdop_raycast(self._h, body, n, ptr(o), ptr(d), ptr(hits), ptr(lengths), ptr(flags)))
+        return hits, lengths, flags.astype(bool)
+
     def world(self, body):
         nv = self.meshes[body].nv
         X = np.empty(nv, np.float32)

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("kdop_get_pairs", ctypes.c_int, [_p, ctypes.c_int, _p, _u64, ctypes.c_int, _p]),
     ("kdop_get_contacts", ctypes.c_int, [_p, _p, _u64, ctypes.c_int, _p]),
     ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
+    ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_get_world", ctypes.c_int, [_p, _u32, _p, _p, _p]),
     ("kdop_export_tree", ctypes.c_int, [_p, _u32, _u64, _p, _p, _p, _p, _p]),

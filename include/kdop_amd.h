@@ -162,6 +162,19 @@ int kdop_sort_contacts(kdop_ctx* ctx, int order);
 int kdop_mesh_reorder_gpu(kdop_ctx* ctx, uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt,
                           const uint32_t* tets, float* Xo, float* Yo, float* Zo, uint32_t* tets_o);
 
+/* Cast n rays at one body in its last refitted pose (host arrays; origins and
+ * directions 3 floats each, a direction need not be unit: it is normalised as
+ * geometry::make_ray does).  Per ray: the nearest hit on a front-facing
+ * surface face (hit_points, 3 floats), its ray length, and hits[i] = 1; a
+ * miss gives the point (0, 0, 0), length FLT_MAX and 0.  Replaces
+ * kdop::raycast(ray, tree, mesh, X, Y, Z, surface_map, hit_point, length)
+ * (SIMULATION/KDOP/KDOP/include/kdop_raycast.h:102-155) with the same results:
+ * the same tests in the same order at and below the branch-chunk roots, with
+ * the super levels culling only chunks whose root the reference's test would
+ * reject or prune. */
+int kdop_raycast(kdop_ctx* ctx, uint32_t body, uint32_t n, const float* origins, const float* directions, float* hit_points,
+                 float* lengths, uint8_t* hits);
+
 /* World coordinates of a body after the last refit (3 arrays of nv floats). */
 int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);
 

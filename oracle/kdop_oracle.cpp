@@ -732,6 +732,128 @@ inline void mesh_reorder(uint32_t nv, const float* X, const float* Y, const floa
     }
 }
 
+// ------------------------------------------------------------------ raycast
+// geometry_inside.h:19-40
+inline bool outside_dop(const V3<float>& p, const Dop& d, const V3<float>* D, int N, float threshold) {
+  for (int k = 0; k < N; ++k) {
+    const float o = dot(D[k], p);
+    if (o < d.lo[k] - threshold) return true;
+    if (o > d.hi[k] + threshold) return true;
+  }
+  return false;
+}
+// raycast/geometry_raycast_dop.h:19-99 (VT::lowest/highest = -/+FLT_MAX,
+// VT::infinity = FLT_MAX, tiny_constants.h:139-145)
+inline bool raycast_dop(const V3<float>& p, const V3<float>& r, const Dop& dop, const V3<float>* D, int N, float threshold, V3<float>& point,
+                        float& length) {
+  const float FMAX = std::numeric_limits<float>::max();
+  length = FMAX;
+  point = mk(0.0f, 0.0f, 0.0f);
+  float t_min = -FMAX, t_max = FMAX;
+  for (int k = 0; k < N; ++k) {
+    const V3<float> d0 = neg(D[k]);
+    const float w0 = -dop.lo[k];
+    const float f0 = dot(d0, r);
+    const V3<float> d1 = D[k];
+    const float w1 = dop.hi[k];
+    const float f1 = dot(d1, r);
+    if (f0 == 0.0f) continue;
+    float t_near = (w0 - dot(d0, p)) / f0;
+    float t_far = (w1 - dot(d1, p)) / f1;
+    if (t_near > t_far) std::swap(t_near, t_far);
+    t_min = (t_near > t_min) ? t_near : t_min;
+    t_max = (t_far < t_max) ? t_far : t_max;
+  }
+  if (t_min > 0.0f && t_min <= t_max) {
+    const V3<float> q = add(p, mul(r, t_min));
+    if (outside_dop(q, dop, D, N, threshold)) return false;
+    length = t_min;
+    point = q;
+    return true;
+  }
+  return false;
+}
+// raycast/geometry_raycast_tetrahedron.h:27-131 with the faces of
+// get_opposite_face (types/geometry_tetrahedron.h:231-257)
+inline bool raycast_tet(const V3<float>& o, const V3<float>& r, const Tet<float>& tet, const bool surf[4], V3<float>& hit, float& length) {
+  static const int F[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+  length = std::numeric_limits<float>::max();
+  hit = mk(0.0f, 0.0f, 0.0f);
+  bool status = false;
+  V3<float> n[4];
+  float w[4];
+  for (int m = 0; m < 4; ++m) {
+    const V3<float>& a = tet.p[F[m][0]];
+    const V3<float>& b = tet.p[F[m][1]];
+    const V3<float>& c = tet.p[F[m][2]];
+    const V3<float> n0 = cross(sub(b, a), sub(c, a));
+    const V3<float> n1 = cross(sub(c, b), sub(a, b));
+    const V3<float> n2 = cross(sub(a, c), sub(b, c));
+    const float w0 = dot(n0, a), w1 = dot(n1, b), w2 = dot(n2, c);
+    const float t0 = dot(n0, n0), t1 = dot(n1, n1), t2 = dot(n2, n2);
+    n[m] = mk(0.0f, 0.0f, 0.0f);
+    w[m] = 0.0f;
+    if (t0 >= t1 && t0 >= t2) { n[m] = n0; w[m] = w0; }
+    else if (t1 >= t0 && t1 >= t2) { n[m] = n1; w[m] = w1; }
+    else if (t2 >= t0 && t2 >= t1) { n[m] = n2; w[m] = w2; }
+  }
+  for (int m = 0; m < 4; ++m) {
+    if (!surf[m]) continue;
+    const float rn = dot(r, n[m]);
+    const float on = dot(o, n[m]);
+    if (rn >= 0.0f) continue;
+    const float t = (w[m] - on) / rn;
+    if (t < 0.0f) continue;
+    const V3<float> q = add(o, mul(r, t));
+    const int i = (m + 1) % 4, j = (m + 2) % 4, k = (m + 3) % 4;
+    if (dot(n[i], q) - w[i] > 0.0f) continue;
+    if (dot(n[j], q) - w[j] > 0.0f) continue;
+    if (dot(n[k], q) - w[k] > 0.0f) continue;
+    if (t < length) {
+      length = t;
+      hit = q;
+      status = true;
+    }
+  }
+  return status;
+}
+// kdop_raycast.h:19-99
+inline void raycast_node(const V3<float>& o, const V3<float>& r, size_t idx, const SubTree& br, const Mesh& m, const V3<float>* D, int N,
+                         V3<float>& hit_point, float& length) {
+  const float threshold = 0.01f;
+  const Node& node = br.nodes[idx];
+  V3<float> p;
+  float t;
+  if (!raycast_dop(o, r, node.vol, D, N, threshold, p, t)) return;
+  if (t >= length) return;
+  if (node.is_leaf()) {
+    const uint32_t tet = (uint32_t)node.start;
+    const Tet<float> T = gather_tet(m, tet);
+    bool surf[4];
+    for (int v = 0; v < 4; ++v) surf[v] = (m.surf[tet] >> v) & 1u;
+    V3<float> q;
+    float s;
+    if (raycast_tet(o, r, T, surf, q, s)) {
+      hit_point = s < length ? q : hit_point;
+      length = s < length ? s : length;
+    }
+  } else {
+    for (size_t c = node.start; c <= node.end; ++c) raycast_node(o, r, c, br, m, D, N, hit_point, length);
+  }
+}
+// kdop_raycast.h:102-155; the ray is made by geometry::make_ray (direction
+// normalised, types/geometry_ray.h:39-42).
+inline bool raycast(const Tree& t, const Mesh& m, const V3<float>& origin, const V3<float>& direction, V3<float>& hit_point, float& length) {
+  const V3<float> r = unit(direction);
+  hit_point = mk(0.0f, 0.0f, 0.0f);
+  length = std::numeric_limits<float>::max();
+  V3<float> p;
+  float tt;
+  if (!raycast_dop(origin, r, t.root, t.D.data(), t.N, 0.01f, p, tt)) return false;
+  for (const SubTree& br : t.levels.back()) raycast_node(origin, r, 0, br, m, t.D.data(), t.N, hit_point, length);
+  return length < std::numeric_limits<float>::max();
+}
+
 }  // namespace orc
 
 // =================================================================== C ABI
@@ -976,6 +1098,23 @@ int orc_traverse_rows_mt(void* hA, const uint32_t* tetsA, const float* XA, const
     counters[4] += s.p1; counters[5] += s.p2; counters[6] += s.ncontacts; counters[7] += s.narrow_hits;
   }
   *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
+}
+
+// kdop::raycast for n rays against one refitted tree (world coordinates X, Y, Z).
+int orc_raycast(void* h, const uint32_t* tets, const float* X, const float* Y, const float* Z, const uint8_t* surf, uint32_t n,
+                const float* origins, const float* directions, float* hits, float* lengths, uint8_t* did_hit) {
+  const Tree& t = *(Tree*)h;
+  const Mesh m{tets, X, Y, Z, surf};
+  for (uint32_t i = 0; i < n; ++i) {
+    V3<float> hp;
+    float len;
+    const bool hit = raycast(t, m, mk(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]),
+                             mk(directions[3 * i], directions[3 * i + 1], directions[3 * i + 2]), hp, len);
+    hits[3 * i] = hp.x; hits[3 * i + 1] = hp.y; hits[3 * i + 2] = hp.z;
+    lengths[i] = len;
+    did_hit[i] = hit ? 1 : 0;
+  }
   return 0;
 }
 

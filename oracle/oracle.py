@@ -51,6 +51,7 @@ def lib():
         L.orc_traverse_rows.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int, _p, _p]
         L.orc_traverse_rows_mt.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int,
                                            ctypes.c_int, _p, _p]
+        L.orc_raycast.argtypes = [_p, _p, _p, _p, _p, _p, _u32, _p, _p, _p, _p, _p]
         L.orc_brute_pairs.restype = _u64
         L.orc_brute_pairs.argtypes = [_p, _p, _p, _u64]
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
@@ -219,6 +220,20 @@ def traverse_rows_mt(tree_a, mesh_a, tree_b, mesh_b, row0, row1, threads, narrow
                                tree_b.h, _ptr(tb), _ptr(Xb), _ptr(Yb), _ptr(Zb), _ptr(Sb),
                                int(row0), int(row1), int(narrow), int(threads), _ptr(cnt), ctypes.byref(sec))
     return dict(zip(COUNTER_NAMES, (int(v) for v in cnt))), sec.value
+
+
+def raycast(tree, mesh, origins, directions):
+    """kdop::raycast (kdop_raycast.h:102-155) of n rays against a refitted tree:
+    (hit points (n, 3), lengths (n,), hit flags (n,) bool)."""
+    t, X, Y, Z, S = (np.ascontiguousarray(a) for a in mesh)
+    o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(directions, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    hits = np.zeros((n, 3), np.float32)
+    lengths = np.zeros(n, np.float32)
+    flags = np.zeros(n, np.uint8)
+    lib().orc_raycast(tree.h, _ptr(t), _ptr(X), _ptr(Y), _ptr(Z), _ptr(S), n, _ptr(o), _ptr(d), _ptr(hits), _ptr(lengths), _ptr(flags))
+    return hits, lengths, flags.astype(bool)
 
 
 def brute_pairs(tree_a, tree_b, cap):

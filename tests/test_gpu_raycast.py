@@ -1,0 +1,58 @@
+"""kdop_raycast against the oracle's kdop::raycast, through the C ABI: hit
+flags, lengths and hit points bit for bit, for several K, chunk widths (the
+super levels the device culls with), posed bodies and rays that miss, hit,
+graze and start inside."""
+import numpy as np
+import pytest
+
+import kdop_amd as K
+from kdop_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def _rays(rng, n, center, radius, spread):
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = center + radius * u
+    target = center + rng.uniform(-spread, spread, size=(n, 3))
+    d = target - o
+    # a few rays from inside the body and a few pointing away
+    o[: n // 16] = center + rng.uniform(-0.1, 0.1, size=(n // 16, 3))
+    d[n // 16: n // 8] = -d[n // 16: n // 8]
+    return o.astype(F32), d.astype(F32)
+
+
+@pytest.mark.parametrize("k,n,chunk_bytes,nrays", [(8, 10, 8000, 4096), (26, 12, 8000, 4096), (14, 12, 8000, 2048),
+                                                   (26, 12, 2000, 2048), (26, 40, 8000, 256)],
+                         ids=["k8_box10", "k26_box12", "k14_box12", "k26_L2", "k26_box40"])
+def test_raycast_matches_oracle(k, n, chunk_bytes, nrays):
+    rng = np.random.default_rng(n * 31 + k)
+    mesh = K.box_mesh(n, size=1.0, jitter=0.1, seed=7)
+    pose = W.Pose(np.array([0.3, -0.2, 0.1], F32), K.rng_quaternions(5, 1, 1)[0])
+    X, Y, Z = O.transform_f32(mesh.X, mesh.Y, mesh.Z, pose.p, pose.q)
+    tree = O.Tree(k, chunk_bytes, mesh.tets, mesh.X, mesh.Y, mesh.Z)
+    tree.refit(X, Y, Z)
+    ctx = K.Context(K=k, chunk_bytes=chunk_bytes)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], pose.p[None, :], pose.q[None, :])
+    o, d = _rays(rng, nrays, pose.p.astype(np.float64), 2.0, 0.55)
+    h_d, l_d, f_d = ctx.raycast(b, o, d)
+    h_o, l_o, f_o = O.raycast(tree, (mesh.tets, X, Y, Z, mesh.surface), o, d)
+    print(k, n, chunk_bytes, "hits", int(f_o.sum()), "of", nrays)
+    assert f_o.sum() > nrays // 4
+    np.testing.assert_array_equal(f_d, f_o)
+    assert l_d.tobytes() == l_o.tobytes()
+    assert h_d.tobytes() == h_o.tobytes()
+
+
+def test_raycast_errors_and_empty():
+    ctx = K.Context(K=8)
+    b = ctx.make_tree(K.box_mesh(2))
+    h, l, f = ctx.raycast(b, np.zeros((0, 3), F32), np.zeros((0, 3), F32))
+    assert h.shape == (0, 3)
+    with pytest.raises(K.KdopError):
+        ctx.raycast(b + 5, np.zeros((1, 3), F32), np.ones((1, 3), F32))
