@@ -218,6 +218,31 @@ def raycast_leg(args, head):
                 body_tets=int(head["wl"].info["nt"]), note="host arrays in and out, copies included")
 
 
+def sphere_set(n, seed=6):
+    """Spheres around body A of the headline pair: centres in a box of half
+    width 0.75 around its pose (near its surface and inside), radii 0.01-0.15."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.75, 0.75, size=(n, 3))
+    r = rng.uniform(0.01, 0.15, size=(n, 1))
+    return np.concatenate([c, r], 1).astype(np.float32)
+
+
+def single_leg(args, head):
+    """kdop_single_traversal of many spheres against body A of the headline pair."""
+    ctx, body = head["ctx"], head["ids"][0]
+    sp = sphere_set(args.spheres)
+    sp[:, :3] += head["wl"].poses[0].p[None, :]
+    out = ctx.single_traversal(body, sp)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = ctx.single_traversal(body, sp)
+    dt = (time.perf_counter() - t0) / reps
+    return dict(spheres=int(sp.shape[0]), contacts=int(out.shape[0]), ms=dt * 1e3, spheres_per_s=sp.shape[0] / dt,
+                body_tets=int(head["wl"].info["nt"]), note="host arrays in and out, copies included")
+
+
 def reorder_leg(args):
     """Morton reorder of the headline mesh (40^3 box, 320k tets) on the device
     (host arrays in and out, copies included) against the library's host
@@ -281,13 +306,28 @@ def cpu_leg(args, head):
         O.raycast(trees[0], meshA, o[:nr], d[:nr])
         sr = _t.perf_counter() - t0
         ray = dict(value=nr / sr, unit="rays/s", cores=1, kind="port", sample=f"oracle kdop::raycast, first {nr} rays of the set ({sr:.2f} s)")
+    sph = None
+    if head.get("single"):
+        sp = sphere_set(head["single"]["spheres"])
+        sp[:, :3] += wl.poses[0].p[None, :]
+        import time as _t
+        nr = 8
+        t0 = _t.perf_counter()
+        O.single_traversal(trees[0], meshA, sp[:nr])
+        per = (_t.perf_counter() - t0) / nr
+        nr = int(min(sp.shape[0], max(nr, args.cpu_seconds / 3 / max(per, 1e-6))))
+        t0 = _t.perf_counter()
+        O.single_traversal(trees[0], meshA, sp[:nr])
+        ss = _t.perf_counter() - t0
+        sph = dict(value=nr / ss, unit="spheres/s", cores=1, kind="port",
+                   sample=f"oracle kdop::single_traversal, first {nr} spheres of the set ({ss:.2f} s)")
     # the same query on every host core this job may use (rows taken one at a time), whole query
     cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
     cm, sm = O.traverse_rows_mt(trees[0], meshA, trees[1], meshB, 0, rows_total, cores)
     allc = dict(value=cm["tests"] / sm, unit="node-pair tests/s", cores=cores, kind="port",
                 sample=f"oracle tandem traversal + narrow phase, all {rows_total} branch rows of body A on {cores} threads "
                        f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm)
-    return one, allc, ray
+    return one, allc, ray, sph
 
 
 def main():
@@ -313,6 +353,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reorder", action="store_true", help="skip the device Morton reorder leg")
     ap.add_argument("--rays", type=int, default=65536, help="rays of the raycast leg (0: skip)")
+    ap.add_argument("--spheres", type=int, default=65536, help="spheres of the single-traversal leg (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
@@ -347,6 +388,9 @@ def main():
     if args.rays:
         head["raycast"] = raycast_leg(args, head)
         log(f"[bench] raycast {head['raycast']['rays']} rays: {head['raycast']['ms']:.2f} ms")
+    if args.spheres:
+        head["single"] = single_leg(args, head)
+        log(f"[bench] single traversal {head['single']['spheres']} spheres: {head['single']['ms']:.2f} ms")
     reorder = None
     if not args.no_reorder:
         reorder = reorder_leg(args)
@@ -354,7 +398,10 @@ def main():
     cpu = cpu_all = None
     if not args.no_cpu:
         log("[bench] cpu baseline sample")
-        cpu, cpu_all, cpu_ray = cpu_leg(args, head)
+        cpu, cpu_all, cpu_ray, cpu_sph = cpu_leg(args, head)
+        if cpu_sph and head.get("single"):
+            head["single"]["cpu_baseline"] = cpu_sph
+            head["single"]["vs_cpu"] = head["single"]["spheres_per_s"] / cpu_sph["value"]
         if cpu_ray and head.get("raycast"):
             head["raycast"]["cpu_baseline"] = cpu_ray
             head["raycast"]["vs_cpu"] = head["raycast"]["rays_per_s"] / cpu_ray["value"]
@@ -395,6 +442,7 @@ def main():
         "batched": batched,
         "mesh_reorder": reorder,
         "raycast": head.get("raycast"),
+        "single_traversal": head.get("single"),
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

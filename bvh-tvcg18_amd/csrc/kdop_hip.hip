@@ -20,6 +20,7 @@
 #include "kdop_adjoint.h"
 #include "kdop_sort.h"
 #include "kdop_raycast.h"
+#include "kdop_single.h"
 #include "kdop_meshgen.h"
 #include "tree_shape.h"
 
@@ -71,7 +72,8 @@ struct kdop_ctx {
   DevBuf<kdop_contact> contacts;
   DevBuf<kdop_contact> contacts_tmp;  // kdop_sort_contacts: the permuted copy
   DevBuf<unsigned char> sortbuf;       // kdop_sort_contacts: two key arrays
-  DevBuf<unsigned char> raybuf;        // kdop_raycast: rays in, hits out
+  DevBuf<unsigned char> raybuf;        // kdop_raycast / kdop_single_traversal: inputs and counts
+  DevBuf<kdop_sphere_contact> sphere_out;  // kdop_single_traversal: the contacts
   DevBuf<NarrowRec> recs;
   DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
   uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
@@ -271,6 +273,14 @@ struct Raycast {
   static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* o, const float* d, float* hits, float* lengths, uint8_t* flags) {
     hipLaunchKernelGGL(k_raycast<K>, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->dbodies.p, body, c->dtops.p, c->nodes.p, c->dt, n, o, d,
                        hits, lengths, flags);
+  }
+};
+template <int K>
+struct Single {
+  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* spheres, int flip, int write, uint32_t* count, const uint32_t* offset,
+                  kdop_sphere_contact* out) {
+    hipLaunchKernelGGL(k_single<K>, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->dbodies.p, body, c->dtops.p, c->nodes.p, c->dt, n,
+                       spheres, flip, write, count, offset, out);
   }
 };
 template <int K>
@@ -489,7 +499,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
-  release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf);
+  release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -1413,6 +1423,48 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   HIPCHK(c, hipMemcpyAsync(hit_points, d_h, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(lengths, d_l, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(hits, d_f, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* spheres, int flip, kdop_sphere_contact* out, uint64_t cap,
+                          uint64_t* count) {
+  if (!c || !count) return KDOP_ERR_ARG;
+  *count = 0;
+  if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: unknown body %u", body);
+  if (n == 0) return KDOP_OK;
+  if (!spheres) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: null spheres");
+  if (n >= (1u << 28)) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: too many spheres (%u)", n);
+  const BodyHost& b = c->bodies[body];
+  uint32_t lg = 0;
+  while ((1u << lg) < b.L) ++lg;
+  if (b.H * std::max(lg, 1u) + 2 > kRayStack) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: tree too deep for the stack");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  // [spheres 4n floats][count n][offset n][block sums nb][total 1] in raybuf
+  const size_t o_cnt = 16 * (size_t)n, o_off = o_cnt + 4 * (size_t)n, o_sums = o_off + 4 * (size_t)n, o_tot = o_sums + 4 * (size_t)nb;
+  int rc = ensure(c, c->raybuf, o_tot + 4);
+  if (rc) return rc;
+  float* d_s = reinterpret_cast<float*>(c->raybuf.p);
+  uint32_t* d_cnt = reinterpret_cast<uint32_t*>(c->raybuf.p + o_cnt);
+  uint32_t* d_off = reinterpret_cast<uint32_t*>(c->raybuf.p + o_off);
+  uint32_t* d_sums = reinterpret_cast<uint32_t*>(c->raybuf.p + o_sums);
+  uint32_t* d_tot = reinterpret_cast<uint32_t*>(c->raybuf.p + o_tot);
+  HIPCHK(c, hipMemcpyAsync(d_s, spheres, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  dispatch_K<Single>(c->K, c, body, n, (const float*)d_s, flip, 0, d_cnt, (const uint32_t*)nullptr, (kdop_sphere_contact*)nullptr);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = device_scan(c, d_cnt, d_off, n, d_sums, d_tot))) return rc;
+  uint32_t total = 0;
+  HIPCHK(c, hipMemcpyAsync(&total, d_tot, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *count = total;
+  if (!out) return KDOP_OK;
+  if (cap < total) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: need %u contacts", total);
+  if (total == 0) return KDOP_OK;
+  if ((rc = ensure(c, c->sphere_out, total))) return rc;
+  dispatch_K<Single>(c->K, c, body, n, (const float*)d_s, flip, 1, d_cnt, (const uint32_t*)d_off, c->sphere_out.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, c->sphere_out.p, (size_t)total * sizeof(kdop_sphere_contact), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
 }

@@ -1,0 +1,193 @@
+// kdop_single.h -- kdop::single_traversal of spheres against one body
+// (SIMULATION/KDOP/KDOP/include/kdop_single_traversal.h:41-161) with the
+// sphere-tetrahedron contact of contacts/geometry_contacts_sphere_tetrahedron.h.
+//
+// One thread per sphere.  At and below the branch-chunk roots every node gets
+// the reference's overlap test in the reference's depth-first order, and a
+// leaf the reference's contact routine, so each sphere's contacts come out in
+// the order the sequential code reports them.  The reference tests all C
+// branch roots per sphere; here the super levels above them are walked first
+// with the same overlap test, which can only fail for a branch root when it
+// fails for every super node above it (a super DOP contains the DOPs below).
+// Two passes: the first counts each sphere's contacts, the second writes them
+// at the sphere's offset, so the output is grouped by sphere without a sort.
+#pragma once
+
+#include "kdop_kernels.h"
+#include "kdop_raycast.h"
+
+namespace kdop_amd {
+
+struct Plane3 {
+  f3 n;
+  float w;
+};
+// types/geometry_plane.h:103-111 (unit normal of the triangle, offset p0 . n)
+__device__ inline Plane3 plane3(const f3& p0, const f3& p1, const f3& p2) {
+  const f3 n = unit3(cross3(sub3(p1, p0), sub3(p2, p0)));
+  return Plane3{n, dot3(p0, n)};
+}
+// types/geometry_plane.h:156-163 (a normal and a point on the plane)
+__device__ inline Plane3 plane3_np(const f3& n, const f3& p) { return Plane3{n, dot3(p, n)}; }
+// types/geometry_plane.h:172-175
+__device__ inline float signed_distance(const f3& p, const Plane3& P) { return dot3(P.n, p) - P.w; }
+// closest_points/geometry_closest_point_on_plane.h:14-28
+__device__ inline f3 closest_on_plane(const f3& p, const Plane3& P) {
+  return sub3(p, mul3(P.n, dot3(P.n, sub3(p, mul3(P.n, P.w)))));
+}
+// closest_points/geometry_closest_point_on_line.h:16-31 with make_line(a, b)
+// (types/geometry_line.h:55-70: direction unit(b - a))
+__device__ inline f3 closest_on_line(const f3& p, const f3& a, const f3& b) {
+  const f3 d = unit3(sub3(b, a));
+  return add3(a, mul3(d, dot3(d, sub3(p, a))));
+}
+// geometry_inside.h:135-161 with test_face_plane = false
+__device__ inline bool inside_triangle_walls(const f3& p, const f3* t) {
+  const Plane3 pl = plane3(t[0], t[1], t[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const f3& p0 = t[k];
+    const f3& p1 = t[(k + 1) % 3];
+    const f3 p2 = add3(p1, pl.n);
+    if (signed_distance(p, plane3(p0, p1, p2)) > 0.0f) return false;
+  }
+  return true;
+}
+
+// contacts/geometry_contacts_sphere_tetrahedron.h:35-195: at most one contact
+// (true, with its point, normal and depth); surf bit v: the face opposite
+// vertex v is on the surface.
+__device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uint32_t surf, bool flip, f3& p, f3& n, float& depth) {
+  constexpr int FACE[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};  // get_opposite_face
+  Plane3 plane[4];
+  float d[4];
+  for (int k = 0; k < 4; ++k) {  // :51-60, stopping at the first face too far away
+    plane[k] = plane3(B[FACE[k][0]], B[FACE[k][1]], B[FACE[k][2]]);
+    d[k] = signed_distance(c, plane[k]);
+    if (d[k] > r) return false;
+  }
+  if (d[0] < 0.0f && d[1] < 0.0f && d[2] < 0.0f && d[3] < 0.0f) {  // :63-84 center inside
+    for (int k = 0; k < 4; ++k) {
+      if (d[k] < d[(k + 1) % 4] || d[k] < d[(k + 2) % 4] || d[k] < d[(k + 3) % 4]) continue;
+      n = flip ? plane[k].n : f3{-plane[k].n.x, -plane[k].n.y, -plane[k].n.z};
+      p = closest_on_plane(c, plane[k]);
+      depth = d[k];
+      return true;
+    }
+  }
+  for (int i = 0; i < 4; ++i) {  // :87-112 vertex regions
+    if (!(((surf >> ((i + 1) & 3)) | (surf >> ((i + 2) & 3)) | (surf >> ((i + 3) & 3))) & 1u)) continue;
+    const Plane3 v1 = plane3_np(unit3(sub3(B[i], B[(i + 1) & 3])), B[i]);
+    const Plane3 v2 = plane3_np(unit3(sub3(B[i], B[(i + 2) & 3])), B[i]);
+    const Plane3 v3 = plane3_np(unit3(sub3(B[i], B[(i + 3) & 3])), B[i]);
+    if (signed_distance(c, v1) >= 0.0f && signed_distance(c, v2) >= 0.0f && signed_distance(c, v3) >= 0.0f) {
+      const f3 m = sub3(c, B[i]);
+      const float dist = sqrtf(dot3(m, m));
+      if (dist > r) return false;
+      const f3 u = unit3(m);
+      n = flip ? u : f3{-u.x, -u.y, -u.z};
+      p = B[i];
+      depth = dist - r;
+      return true;
+    }
+  }
+  constexpr int EDGE[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 2, 0, 3}, {1, 3, 2, 0}, {2, 3, 0, 1}};
+  for (int v = 0; v < 6; ++v) {  // :115-160 edge regions
+    const int i = EDGE[v][0], j = EDGE[v][1], k = EDGE[v][2], m = EDGE[v][3];
+    if (!(((surf >> k) | (surf >> m)) & 1u)) continue;
+    const f3 e = sub3(B[j], B[i]);
+    const f3 nk = unit3(cross3(e, plane[k].n));
+    const f3 nm = unit3(cross3(f3{-e.x, -e.y, -e.z}, plane[m].n));
+    if (signed_distance(c, plane3_np(nk, B[i])) >= 0.0f && signed_distance(c, plane3_np(nm, B[i])) >= 0.0f) {
+      const f3 q = closest_on_line(c, B[i], B[j]);
+      const f3 mm = sub3(c, q);
+      const float dist = sqrtf(dot3(mm, mm));
+      if (dist > r) return false;
+      const f3 u = unit3(mm);
+      n = flip ? u : f3{-u.x, -u.y, -u.z};
+      p = q;
+      depth = dist - r;
+      return true;
+    }
+  }
+  for (int v = 0; v < 4; ++v) {  // :163-185 face regions
+    if (!((surf >> v) & 1u)) continue;
+    const float dist = signed_distance(c, plane[v]);
+    if (dist < 0.0f) continue;
+    const f3 tri[3] = {B[FACE[v][0]], B[FACE[v][1]], B[FACE[v][2]]};
+    if (inside_triangle_walls(c, tri)) {
+      n = flip ? plane[v].n : f3{-plane[v].n.x, -plane[v].n.y, -plane[v].n.z};
+      p = closest_on_plane(c, plane[v]);
+      depth = dist - r;
+      return true;
+    }
+  }
+  return false;
+}
+
+// One thread per sphere; write = false counts its contacts into count[i],
+// write = true writes them from out[offset[i]] on.  geometry_convert.h:13-31
+// makes the sphere's DOP (o -/+ radius per direction), kdop_single_traversal.h:
+// 139-160 the root test and the walk over the branch chunks in order.
+template <int K>
+__global__ __launch_bounds__(64) void k_single(const BodyDev* __restrict__ bodies, uint32_t body, const uint32_t* __restrict__ tops,
+                                               const float* __restrict__ nodes, DirTable dt, uint32_t n, const float* __restrict__ spheres,
+                                               int flip, int write, uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
+                                               kdop_sphere_contact* __restrict__ out) {
+  __shared__ uint32_t stk[64][kRayStack];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* st = stk[threadIdx.x];
+  const BodyDev& B = bodies[body];
+  const f3 c = f3{spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]};
+  const float r = spheres[4 * i + 3];
+  float sd[K];
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) {
+    const float o = dot3(f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]}, c);
+    sd[2 * k] = o - r;
+    sd[2 * k + 1] = o + r;
+  }
+  uint32_t found = 0;
+  const uint32_t base = write ? offset[i] : 0u;
+  NodeRec<K> rec;
+  load_rec<K>(nodes, B.node_base, rec);
+  if (dop_overlap<K>(rec.v, sd)) {
+    for (uint32_t top = 0; top < B.ntop; ++top) {
+      uint32_t sp = 0;
+      st[sp++] = tops[B.top_base + top];
+      while (sp) {
+        const uint32_t s = st[--sp];
+        load_rec<K>(nodes, s, rec);
+        if (!dop_overlap<K>(rec.v, sd)) continue;  // kdop_single_traversal.h:63-64 (and the super-level cull)
+        const uint32_t c0 = rec.m0 & IDX_MASK, c1 = rec.m1 & IDX_MASK;
+        if ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) {  // :68-89
+          const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)c0);
+          const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
+          f3 P[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
+          f3 p, nn;
+          float depth;
+          if (sphere_tet_contact(c, r, P, B.surf[c0], flip != 0, p, nn, depth)) {
+            if (write) {
+              kdop_sphere_contact& o = out[base + found];
+              o.sphere = i;
+              o.tet = c0;
+              o.p[0] = p.x; o.p[1] = p.y; o.p[2] = p.z;
+              o.n[0] = nn.x; o.n[1] = nn.y; o.n[2] = nn.z;
+              o.depth = depth;
+            }
+            ++found;
+          }
+        } else {  // :91-110 children in order (super nodes: the lower chunks' roots)
+          st[sp++] = c1;
+          st[sp++] = c0;
+        }
+      }
+    }
+  }
+  if (!write) count[i] = found;
+}
+
+}  // namespace kdop_amd

@@ -321,6 +321,18 @@ class Context:
         self._check(self._lib.kdop_raycast(self._h, body, n, ptr(o), ptr(d), ptr(hits), ptr(lengths), ptr(flags)))
         return hits, lengths, flags.astype(bool)
 
+    def single_traversal(self, body, spheres, flip=False):
+        """kdop_single_traversal: contacts of spheres (n, 4: x, y, z, radius) with `body`,
+        grouped by sphere, each sphere's in the reference's report order."""
+        sp = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1, 4))
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, _abi.SPHERE_CONTACT_DTYPE)
+        if n.value:
+            self._check(self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), n.value,
+                                                        ctypes.byref(n)))
+        return out
+
     def world(self, body):
         nv = self.meshes[body].nv
         X = np.empty(nv, np.float32)

@@ -62,6 +62,9 @@ class Counters(ctypes.Structure):
 
 
 # numpy view of kdop_contact (44 bytes)
+SPHERE_CONTACT_DTYPE = np.dtype([
+    ("sphere", np.uint32), ("tet", np.uint32), ("p", np.float32, (3,)), ("n", np.float32, (3,)), ("depth", np.float32),
+])
 CONTACT_DTYPE = np.dtype([
     ("pair", np.uint32), ("tet_a", np.uint32), ("tet_b", np.uint32), ("ordinal", np.uint32),
     ("p", np.float32, (3,)), ("n", np.float32, (3,)), ("depth", np.float32),
@@ -87,6 +90,7 @@ SIGNATURES = [
     ("kdop_get_contacts", ctypes.c_int, [_p, _p, _u64, ctypes.c_int, _p]),
     ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
     ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p]),
+    ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_get_world", ctypes.c_int, [_p, _u32, _p, _p, _p]),
     ("kdop_export_tree", ctypes.c_int, [_p, _u32, _u64, _p, _p, _p, _p, _p]),

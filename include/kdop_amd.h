@@ -77,6 +77,16 @@ typedef struct kdop_contact {
   float depth;        /* penetration measure, negative when overlapping                      */
 } kdop_contact;
 
+/* One sphere-tetrahedron contact of kdop_single_traversal (the arguments of
+ * the ContactsCallback call in contacts/geometry_contacts_sphere_tetrahedron.h). */
+typedef struct kdop_sphere_contact {
+  uint32_t sphere;    /* index into the sphere list of the call        */
+  uint32_t tet;       /* tetrahedron of the body                       */
+  float p[3];         /* contact point                                 */
+  float n[3];         /* unit normal (sphere to tetrahedron unless flip) */
+  float depth;        /* negative when penetrating                     */
+} kdop_sphere_contact;
+
 /* Library defaults: K=8, chunk_bytes=8000, device 0, autotuned traversal. */
 void kdop_default_config(kdop_config* cfg);
 
@@ -174,6 +184,17 @@ int kdop_mesh_reorder_gpu(kdop_ctx* ctx, uint32_t nv, const float* X, const floa
  * reject or prune. */
 int kdop_raycast(kdop_ctx* ctx, uint32_t body, uint32_t n, const float* origins, const float* directions, float* hit_points,
                  float* lengths, uint8_t* hits);
+
+/* Spheres against one body in its last refitted pose (host arrays: spheres
+ * holds x, y, z, radius per sphere).  Contacts go to out grouped by sphere in
+ * sphere order, each sphere's in the order the reference reports them through
+ * its ContactsCallback; *n receives the contact count, and KDOP_ERR_CAPACITY
+ * (nothing written) means cap < *n.  flip: the reference's should_flip.
+ * Replaces kdop::single_traversal(sphere, tree, mesh, X, Y, Z, surface_map,
+ * callback, should_flip) (SIMULATION/KDOP/KDOP/include/kdop_single_traversal.h:
+ * 123-161) for n spheres at once, with the same contacts in the same order. */
+int kdop_single_traversal(kdop_ctx* ctx, uint32_t body, uint32_t nspheres, const float* spheres, int flip,
+                          kdop_sphere_contact* out, uint64_t cap, uint64_t* n);
 
 /* World coordinates of a body after the last refit (3 arrays of nv floats). */
 int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);

@@ -854,6 +854,133 @@ inline bool raycast(const Tree& t, const Mesh& m, const V3<float>& origin, const
   return length < std::numeric_limits<float>::max();
 }
 
+// ------------------------------------------------------------ single traversal
+struct SPlane { V3<float> n; float w; };
+// types/geometry_plane.h:103-111, :156-163, :172-175
+inline SPlane make_plane3(const V3<float>& p0, const V3<float>& p1, const V3<float>& p2) {
+  const V3<float> n = unit(cross(sub(p1, p0), sub(p2, p0)));
+  return SPlane{n, dot(p0, n)};
+}
+inline SPlane make_plane_np(const V3<float>& n, const V3<float>& p) { return SPlane{n, dot(p, n)}; }
+inline float signed_dist(const V3<float>& p, const SPlane& P) { return dot(P.n, p) - P.w; }
+// closest_points/geometry_closest_point_on_plane.h:14-28
+inline V3<float> closest_point_on_plane(const V3<float>& p, const SPlane& P) {
+  return sub(p, mul(P.n, dot(P.n, sub(p, mul(P.n, P.w)))));
+}
+// closest_points/geometry_closest_point_on_line.h:16-31, line from points (types/geometry_line.h:29-32,55-70)
+inline V3<float> closest_point_on_line(const V3<float>& p, const V3<float>& a, const V3<float>& b) {
+  const V3<float> d = unit(sub(b, a));
+  return add(a, mul(d, dot(d, sub(p, a))));
+}
+// geometry_inside.h:135-161 (test_face_plane = false)
+inline bool inside_triangle_nf(const V3<float>& p, const V3<float>& t0, const V3<float>& t1, const V3<float>& t2) {
+  const V3<float> t[3] = {t0, t1, t2};
+  const SPlane pl = make_plane3(t0, t1, t2);
+  for (int k = 0; k < 3; ++k) {
+    const V3<float> p2 = add(t[(k + 1) % 3], pl.n);
+    if (signed_dist(p, make_plane3(t[k], t[(k + 1) % 3], p2)) > 0.0f) return false;
+  }
+  return true;
+}
+struct SphereContact { uint32_t sphere, tet; float p[3], n[3], d; };
+// contacts/geometry_contacts_sphere_tetrahedron.h:35-195
+inline bool contacts_sphere_tet(const V3<float>& c, float r, const Tet<float>& B, const bool surf[4], bool flip, V3<float>& out_p,
+                                V3<float>& out_n, float& out_d) {
+  static const int F[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+  float d[4];
+  SPlane plane[4];
+  for (int k = 0; k < 4; ++k) {
+    plane[k] = make_plane3(B.p[F[k][0]], B.p[F[k][1]], B.p[F[k][2]]);
+    d[k] = signed_dist(c, plane[k]);
+    if (d[k] > r) return false;
+  }
+  if (d[0] < 0.0f && d[1] < 0.0f && d[2] < 0.0f && d[3] < 0.0f) {
+    for (int k = 0; k < 4; ++k) {
+      if (d[k] < d[(k + 1) % 4]) continue;
+      if (d[k] < d[(k + 2) % 4]) continue;
+      if (d[k] < d[(k + 3) % 4]) continue;
+      out_n = flip ? plane[k].n : neg(plane[k].n);
+      out_p = closest_point_on_plane(c, plane[k]);
+      out_d = d[k];
+      return true;
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (!(surf[(i + 1) % 4] || surf[(i + 2) % 4] || surf[(i + 3) % 4])) continue;
+    const SPlane v1 = make_plane_np(unit(sub(B.p[i], B.p[(i + 1) % 4])), B.p[i]);
+    const SPlane v2 = make_plane_np(unit(sub(B.p[i], B.p[(i + 2) % 4])), B.p[i]);
+    const SPlane v3 = make_plane_np(unit(sub(B.p[i], B.p[(i + 3) % 4])), B.p[i]);
+    if (signed_dist(c, v1) >= 0.0f && signed_dist(c, v2) >= 0.0f && signed_dist(c, v3) >= 0.0f) {
+      const V3<float> m = sub(c, B.p[i]);
+      const float dist = std::sqrt(dot(m, m));
+      if (dist > r) return false;
+      out_n = flip ? unit(m) : neg(unit(m));
+      out_p = B.p[i];
+      out_d = dist - r;
+      return true;
+    }
+  }
+  static const int E[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 2, 0, 3}, {1, 3, 2, 0}, {2, 3, 0, 1}};
+  for (int v = 0; v < 6; ++v) {
+    const int i = E[v][0], j = E[v][1], k = E[v][2], m = E[v][3];
+    if (!(surf[k] || surf[m])) continue;
+    const V3<float> e = sub(B.p[j], B.p[i]);
+    const V3<float> nk = unit(cross(e, plane[k].n));
+    const V3<float> nm = unit(cross(neg(e), plane[m].n));
+    if (signed_dist(c, make_plane_np(nk, B.p[i])) >= 0.0f && signed_dist(c, make_plane_np(nm, B.p[i])) >= 0.0f) {
+      const V3<float> q = closest_point_on_line(c, B.p[i], B.p[j]);
+      const V3<float> mm = sub(c, q);
+      const float dist = std::sqrt(dot(mm, mm));
+      if (dist > r) return false;
+      out_n = flip ? unit(mm) : neg(unit(mm));
+      out_p = q;
+      out_d = dist - r;
+      return true;
+    }
+  }
+  for (int v = 0; v < 4; ++v) {
+    if (!surf[v]) continue;
+    const float dist = signed_dist(c, plane[v]);
+    if (dist < 0.0f) continue;
+    if (inside_triangle_nf(c, B.p[F[v][0]], B.p[F[v][1]], B.p[F[v][2]])) {
+      out_n = flip ? plane[v].n : neg(plane[v].n);
+      out_p = closest_point_on_plane(c, plane[v]);
+      out_d = dist - r;
+      return true;
+    }
+  }
+  return false;
+}
+// kdop_single_traversal.h:41-110
+inline void single_node(const V3<float>& c, float r, const Dop& sd, size_t idx, const SubTree& br, const Mesh& m, int N, bool flip,
+                        uint32_t sphere, std::vector<SphereContact>& out) {
+  const Node& node = br.nodes[idx];
+  if (!dop_overlap(node.vol, sd, N)) return;
+  if (node.is_leaf()) {
+    const uint32_t tet = (uint32_t)node.start;
+    const Tet<float> T = gather_tet(m, tet);
+    bool surf[4];
+    for (int v = 0; v < 4; ++v) surf[v] = (m.surf[tet] >> v) & 1u;
+    V3<float> p, n;
+    float d;
+    if (contacts_sphere_tet(c, r, T, surf, flip, p, n, d)) out.push_back(SphereContact{sphere, tet, {p.x, p.y, p.z}, {n.x, n.y, n.z}, d});
+  } else {
+    for (size_t ch = node.start; ch <= node.end; ++ch) single_node(c, r, sd, ch, br, m, N, flip, sphere, out);
+  }
+}
+// kdop_single_traversal.h:123-161 with geometry::convert of the sphere (geometry_convert.h:13-31)
+inline void single_traversal(const Tree& t, const Mesh& m, const V3<float>& c, float r, bool flip, uint32_t sphere,
+                             std::vector<SphereContact>& out) {
+  Dop sd;
+  for (int k = 0; k < t.N; ++k) {
+    const float o = dot(t.D[k], c);
+    sd.lo[k] = o - r;
+    sd.hi[k] = o + r;
+  }
+  if (!dop_overlap(t.root, sd, t.N)) return;
+  for (const SubTree& br : t.levels.back()) single_node(c, r, sd, 0, br, m, t.N, flip, sphere, out);
+}
+
 }  // namespace orc
 
 // =================================================================== C ABI
@@ -1116,6 +1243,20 @@ int orc_raycast(void* h, const uint32_t* tets, const float* X, const float* Y, c
     did_hit[i] = hit ? 1 : 0;
   }
   return 0;
+}
+
+// kdop::single_traversal for n spheres (x, y, z, radius each) against one
+// refitted tree; contacts of sphere 0 first, each in report order.  Returns
+// the count (records beyond cap are not written).
+uint64_t orc_single_traversal(void* h, const uint32_t* tets, const float* X, const float* Y, const float* Z, const uint8_t* surf, uint32_t n,
+                              const float* spheres, int flip, SphereContact* out, uint64_t cap) {
+  const Tree& t = *(Tree*)h;
+  const Mesh m{tets, X, Y, Z, surf};
+  std::vector<SphereContact> all;
+  for (uint32_t i = 0; i < n; ++i)
+    single_traversal(t, m, mk(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3], flip != 0, i, all);
+  for (size_t i = 0; i < all.size() && i < cap; ++i) out[i] = all[i];
+  return all.size();
 }
 
 // Brute force: every (leaf of A, leaf of B) whose leaf DOPs overlap.  The leaf

@@ -52,6 +52,8 @@ def lib():
         L.orc_traverse_rows_mt.argtypes = [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _u64, ctypes.c_int,
                                            ctypes.c_int, _p, _p]
         L.orc_raycast.argtypes = [_p, _p, _p, _p, _p, _p, _u32, _p, _p, _p, _p, _p]
+        L.orc_single_traversal.restype = _u64
+        L.orc_single_traversal.argtypes = [_p, _p, _p, _p, _p, _p, _u32, _p, ctypes.c_int, _p, _u64]
         L.orc_brute_pairs.restype = _u64
         L.orc_brute_pairs.argtypes = [_p, _p, _p, _u64]
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
@@ -234,6 +236,22 @@ def raycast(tree, mesh, origins, directions):
     flags = np.zeros(n, np.uint8)
     lib().orc_raycast(tree.h, _ptr(t), _ptr(X), _ptr(Y), _ptr(Z), _ptr(S), n, _ptr(o), _ptr(d), _ptr(hits), _ptr(lengths), _ptr(flags))
     return hits, lengths, flags.astype(bool)
+
+
+SPHERE_CONTACT_DTYPE = np.dtype([("sphere", np.uint32), ("tet", np.uint32), ("p", np.float32, (3,)), ("n", np.float32, (3,)),
+                                 ("depth", np.float32)])
+
+
+def single_traversal(tree, mesh, spheres, flip=False):
+    """kdop::single_traversal (kdop_single_traversal.h:123-161) of spheres (n, 4: x, y, z, radius)
+    against a refitted tree: contact records grouped by sphere, each sphere's in report order."""
+    t, X, Y, Z, S = (np.ascontiguousarray(a) for a in mesh)
+    sp = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+    n = int(lib().orc_single_traversal(tree.h, _ptr(t), _ptr(X), _ptr(Y), _ptr(Z), _ptr(S), sp.shape[0], _ptr(sp), int(flip), None, 0))
+    out = np.zeros(n, SPHERE_CONTACT_DTYPE)
+    if n:
+        lib().orc_single_traversal(tree.h, _ptr(t), _ptr(X), _ptr(Y), _ptr(Z), _ptr(S), sp.shape[0], _ptr(sp), int(flip), _ptr(out), n)
+    return out
 
 
 def brute_pairs(tree_a, tree_b, cap):

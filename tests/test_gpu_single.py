@@ -1,0 +1,61 @@
+"""kdop_single_traversal against the oracle's kdop::single_traversal, through
+the C ABI: the contact records (sphere, tet, point, normal, depth) bit for bit
+and in the same order, for several K, chunk widths, posed bodies, radii and
+both normal directions."""
+import numpy as np
+import pytest
+
+import kdop_amd as K
+from kdop_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+@pytest.mark.parametrize("k,n,chunk_bytes,nsph,flip", [(8, 10, 8000, 4096, False), (26, 12, 8000, 4096, True),
+                                                       (14, 12, 8000, 2048, False), (26, 12, 2000, 2048, False),
+                                                       (26, 40, 8000, 512, False)],
+                         ids=["k8_box10", "k26_box12_flip", "k14_box12", "k26_L2", "k26_box40"])
+def test_single_traversal_matches_oracle(k, n, chunk_bytes, nsph, flip):
+    rng = np.random.default_rng(n * 13 + k)
+    mesh = K.box_mesh(n, size=1.0, jitter=0.1, seed=4)
+    pose = W.Pose(np.array([-0.2, 0.1, 0.3], F32), K.rng_quaternions(6, 2, 1)[0])
+    X, Y, Z = O.transform_f32(mesh.X, mesh.Y, mesh.Z, pose.p, pose.q)
+    tree = O.Tree(k, chunk_bytes, mesh.tets, mesh.X, mesh.Y, mesh.Z)
+    tree.refit(X, Y, Z)
+    ctx = K.Context(K=k, chunk_bytes=chunk_bytes)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], pose.p[None, :], pose.q[None, :])
+    # centres around the posed box (near its surface and inside), radii from small to large
+    c = pose.p[None, :] + rng.uniform(-0.75, 0.75, size=(nsph, 3))
+    r = rng.uniform(0.01, 0.15, size=(nsph, 1))
+    sp = np.concatenate([c, r], 1).astype(F32)
+    got = ctx.single_traversal(b, sp, flip=flip)
+    want = O.single_traversal(tree, (mesh.tets, X, Y, Z, mesh.surface), sp, flip=flip)
+    print(k, n, chunk_bytes, "contacts", want.shape[0])
+    assert want.shape[0] > nsph // 4
+    assert got.tobytes() == want.tobytes()
+
+
+def test_single_traversal_capacity_and_errors():
+    ctx = K.Context(K=8)
+    b = ctx.make_tree(K.box_mesh(3))
+    sp = np.array([[0, 0, 0.5, 0.2], [5, 5, 5, 0.1]], F32)
+    n = ctypes_count(ctx, b, sp)
+    assert n >= 1
+    with pytest.raises(K.KdopError):
+        ctx.single_traversal(b + 9, sp)
+
+
+def ctypes_count(ctx, b, sp):
+    import ctypes
+    cnt = ctypes.c_uint64()
+    ctx._check(ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], K.ptr(sp), 0, None, 0, ctypes.byref(cnt)))
+    out = np.zeros(max(cnt.value - 1, 0), K._abi.SPHERE_CONTACT_DTYPE)
+    if cnt.value:  # one record short: a capacity error, nothing written
+        rc = ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], K.ptr(sp), 0, K.ptr(out) if out.size else None,
+                                            out.shape[0], ctypes.byref(cnt))
+        assert rc == K._abi.KDOP_ERR_CAPACITY if out.size else rc == 0
+    return cnt.value
