@@ -408,24 +408,24 @@ struct Bfs {
 // A persistent round.  With kernel timing on, start / stop events ride on its
 // dispatch packet and time exactly this kernel.
 template <int K, int LS>
-void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) {
+void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
   const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
   if (c->timing_level >= 2)
     hipExtLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0, b1,
-                          cap, c->dstate, round, c->keep);
+                          cap, c->dstate, round, c->keep, last);
   else
-    hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep);
+    hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep, last);
 }
 template <int K>
 struct PersistRound {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round) {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
     switch (c->ls) {
-      case 256: launch_round<K, 256>(c, T, cap, round); break;
-      case 512: launch_round<K, 512>(c, T, cap, round); break;
-      case 2048: launch_round<K, 2048>(c, T, cap, round); break;
-      default: launch_round<K, 1024>(c, T, cap, round); break;
+      case 256: launch_round<K, 256>(c, T, cap, round, last); break;
+      case 512: launch_round<K, 512>(c, T, cap, round, last); break;
+      case 2048: launch_round<K, 2048>(c, T, cap, round, last); break;
+      default: launch_round<K, 1024>(c, T, cap, round, last); break;
     }
   }
 };
@@ -1011,9 +1011,10 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       if (npairs) {
         dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
         if (persistent) {
-          for (uint32_t l = 0; l <= last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
+          // levels 0 .. last-1; round 0 takes level `last` when no launch stopped earlier
+          for (uint32_t l = 0; l < last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
           if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-          for (uint32_t r = 0; r < rounds_async; ++r) dispatch_K<PersistRound>(c->K, c, T, cap, r);
+          for (uint32_t r = 0; r < rounds_async; ++r) dispatch_K<PersistRound>(c->K, c, T, cap, r, last);
         } else {
           // first the levels the deeper tree suggests; more below if the
           // frontier is not empty by then
@@ -1070,7 +1071,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       // did nothing): drain it, then run the narrow phase
       while (c->hsum->round_total[rounds] > 0 && !(c->hsum->overflow & 1u)) {
         if (rounds + 1 >= (uint32_t)kMaxRounds) return fail(c, KDOP_ERR_CAPACITY, "kdop_traverse: spill rounds exhausted");
-        dispatch_K<PersistRound>(c->K, c, T, cap, rounds);
+        dispatch_K<PersistRound>(c->K, c, T, cap, rounds, last);
         ++rounds;
         HIPCHK(c, hipGetLastError());
         if ((rc = summarize(c, cap, -2))) return rc;

@@ -159,7 +159,7 @@ __device__ inline void flush_leaves(const TravArgs& T, const uint32_t* lp_pair, 
 // stack, so the next round spreads the tail over every wave.
 template <int K, int LS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_persist(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                                CallState* __restrict__ C, uint32_t round, uint32_t keep) {
+                                                CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last) {
   static_assert((LS & (LS - 1)) == 0 && LS >= 256, "LDS stack: power of two, >= 256");
   static_assert(kSegs == 64, "one lane per segment");
   constexpr uint32_t SB = 512;  // staged leaf pairs; flushed above SB - 256 (one iteration adds <= 256)
@@ -167,8 +167,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   __shared__ uint32_t lp_pair[SB], lp_ta[SB], lp_tb[SB];
   const uint32_t lane = lane_id();
   // the breadth-first levels left the frontier in buf[final_level & 1]; each round's spill
-  // goes to the other buffer and is the next round's frontier
-  const bool odd = ((C->final_level + round) & 1u) != 0u;
+  // goes to the other buffer and is the next round's frontier.  A breadth-first launch
+  // that stopped early recorded the frontier; when none did, every queued level was
+  // expanded and round 0 takes level `last` (every block derives it the same way, and
+  // block 0 records it for the later rounds and the summary)
+  uint32_t final_level = C->final_level, final_count = 0;
+  if (round == 0) {
+    const bool recorded = C->final_set != 0u;
+    if (!recorded) final_level = last;
+    final_count = recorded ? C->final_count : min(C->level_fill[last], cap);
+    if (!recorded && blockIdx.x == 0 && threadIdx.x == 0) {
+      C->final_level = final_level;
+      C->final_count = final_count;
+    }
+  }
+  const bool odd = ((final_level + round) & 1u) != 0u;
   const Item* __restrict__ in = odd ? buf1 : buf0;
   Item* __restrict__ spill = odd ? buf0 : buf1;
   const uint32_t segcap = cap / kSegs;
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   // lane s holds segment s of this round's frontier
   uint32_t seg_base, seg_len;
   if (round == 0) {
-    const uint32_t n = min(C->final_count, cap);
+    const uint32_t n = min(final_count, cap);
     seg_base = (uint32_t)(((unsigned long long)n * lane) / kSegs);
     seg_len = (uint32_t)(((unsigned long long)n * (lane + 1)) / kSegs) - seg_base;
   } else {
