@@ -181,13 +181,23 @@ def batched_leg(args):
             ctx.sort_contacts(order)
             tt += time.perf_counter() - t0
         sort_ms[order] = tt / reps * 1e3
+    # device PROX contact reduction of the same contacts (sort by the PROX key,
+    # std::unique with the working precision), after a fresh traversal each time
+    tt = 0.0
+    for _ in range(reps):
+        ctx.tandem_traversal(pairs, narrow=True)
+        t0 = time.perf_counter()
+        kept = ctx.reduce_contacts()
+        tt += time.perf_counter() - t0
+    reduce_ms = tt / reps * 1e3
     return dict(workload=wl.name, pairs=int(n), K=args.batched_k, tets_total=int(sum(wl.info["tets"])),
                 separated_pairs=wl.info["separated"],
                 pair_queries_per_s=n / t_batched, unbatched_pair_queries_per_s=n / t_single,
                 batched_speedup=t_single / t_batched, batched_ms=t_batched * 1e3, unbatched_ms=t_single * 1e3,
                 tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms,
                 contact_sort={"contacts": int(con.shape[0]), "device_ms": sort_ms,
-                              "host_numpy_lexsort_opencl_order_ms": t_host * 1e3})
+                              "host_numpy_lexsort_opencl_order_ms": t_host * 1e3},
+                contact_reduction={"contacts": int(con.shape[0]), "kept": int(kept), "device_ms": reduce_ms})
 
 
 def ray_set(n, seed=5):

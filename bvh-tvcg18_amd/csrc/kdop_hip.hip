@@ -74,6 +74,10 @@ struct kdop_ctx {
   DevBuf<unsigned char> sortbuf;       // kdop_sort_contacts: two key arrays
   DevBuf<unsigned char> raybuf;        // kdop_raycast / kdop_single_traversal: inputs and counts
   DevBuf<kdop_sphere_contact> sphere_out;  // kdop_single_traversal: the contacts
+  DevBuf<uint32_t> redbuf;                 // contact reduction: keep flags, offsets, scan sums, total
+  DevBuf<kdop_prox_contact> prox_io;       // kdop_reduce_prox_contacts: staged input and output
+  uint32_t reduced_n = 0;                  // contacts left by kdop_reduce_contacts
+  bool reduced = false;                    // kdop_reduce_contacts ran since the last traversal
   DevBuf<NarrowRec> recs;
   DevBuf<uint32_t> gate;       // two-pass narrow phase: surface pairs through the SAT gate
   uint64_t last_p2 = 0;        // surface pairs of the last traversal (picks the narrow path)
@@ -500,6 +504,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
+  release(c->redbuf); release(c->prox_io);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -898,6 +903,45 @@ int sort_contacts_keys(kdop_ctx* c, uint32_t n) {
   return KDOP_OK;
 }
 
+// Contacts held in c->contacts: the narrow phase's, or what the reduction left.
+uint32_t contact_count(const kdop_ctx* c) {
+  return c->reduced ? c->reduced_n : (uint32_t)std::min<size_t>(c->hsum->fills.c, c->contacts.cap);
+}
+
+// PROX reduction of n records whose keys fill the first half of sortbuf: sort
+// the keys, mark the points std::unique keeps, and write them to out in
+// sorted order; *kept receives their count (the call ends synchronised).
+template <class Rec>
+int reduce_keys(kdop_ctx* c, const Rec* in, Rec* out, uint32_t n, uint32_t* kept) {
+  ProxKey* a = reinterpret_cast<ProxKey*>(c->sortbuf.p);
+  ProxKey* sorted = nullptr;
+  int rc = device_sort(c, a, a + n, n, &sorted);
+  if (rc) return rc;
+  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  uint32_t* keep = c->redbuf.p;
+  uint32_t* off = keep + n;
+  uint32_t* sums = off + n;
+  uint32_t* total = sums + nb;
+  const dim3 grid((n + 255) / 256);
+  hipLaunchKernelGGL(k_prox_unique, grid, dim3(256), 0, c->stream, (const ProxKey*)sorted, n, keep);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = device_scan(c, keep, off, n, sums, total))) return rc;
+  hipLaunchKernelGGL(k_prox_compact<Rec>, grid, dim3(256), 0, c->stream, in, (const ProxKey*)sorted, (const uint32_t*)keep,
+                     (const uint32_t*)off, n, out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(kept, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KDOP_OK;
+}
+
+// Scratch of a reduction of n records: two key arrays, then keep, offsets,
+// block sums and the total.
+int reduce_scratch(kdop_ctx* c, uint32_t n) {
+  int rc = ensure(c, c->sortbuf, 2 * (size_t)n * sizeof(ProxKey));
+  if (!rc) rc = ensure(c, c->redbuf, 2 * (size_t)n + (n + kScanBlock - 1) / kScanBlock + 1);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1146,6 +1190,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->have_traversal = true;
     c->have_p1 = keep_p1;
     c->have_narrow = narrow;
+    c->reduced = false;
     if (out) {
       const unsigned long long* t = h.total;  // CounterStripe field order
       std::memset(out, 0, sizeof(*out));
@@ -1184,12 +1229,12 @@ int kdop_get_pairs(kdop_ctx* c, int which, uint32_t* dst, uint64_t cap, int dst_
 int kdop_get_contacts(kdop_ctx* c, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n) {
   if (!c || !n) return KDOP_ERR_ARG;
   if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_get_contacts: last traversal ran without KDOP_NARROW");
-  const Fills& f = c->hsum->fills;
-  *n = f.c;
+  const uint32_t nc = c->reduced ? c->reduced_n : c->hsum->fills.c;
+  *n = nc;
   if (!dst) return KDOP_OK;
-  if (cap < f.c) return fail(c, KDOP_ERR_CAPACITY, "kdop_get_contacts: need %u contacts", f.c);
-  if (f.c == 0) return KDOP_OK;
-  HIPCHK(c, hipMemcpyAsync(dst, c->contacts.p, (size_t)f.c * sizeof(kdop_contact), dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+  if (cap < nc) return fail(c, KDOP_ERR_CAPACITY, "kdop_get_contacts: need %u contacts", nc);
+  if (nc == 0) return KDOP_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, c->contacts.p, (size_t)nc * sizeof(kdop_contact), dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
@@ -1297,7 +1342,7 @@ int kdop_sort_contacts(kdop_ctx* c, int order) {
   if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_sort_contacts: last traversal ran without KDOP_NARROW");
   if (order != KDOP_ORDER_SEQUENTIAL && order != KDOP_ORDER_OPENCL) return fail(c, KDOP_ERR_ARG, "kdop_sort_contacts: unknown order %d", order);
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const uint32_t n = (uint32_t)std::min<size_t>(c->hsum->fills.c, c->contacts.cap);
+  const uint32_t n = contact_count(c);
   if (n < 2) return KDOP_OK;
   int rc = ensure(c, c->sortbuf, 2 * (size_t)n * std::max(sizeof(ContactSeqKey), sizeof(ContactClKey)));
   if (!rc) rc = ensure(c, c->contacts_tmp, n);
@@ -1311,6 +1356,62 @@ int kdop_sort_contacts(kdop_ctx* c, int order) {
   hipLaunchKernelGGL(k_contact_keys_cl, grid, dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p, n,
                      reinterpret_cast<ContactClKey*>(c->sortbuf.p));
   return sort_contacts_keys<ContactClKey>(c, n);
+}
+
+int kdop_reduce_contacts(kdop_ctx* c, uint64_t* n) {
+  if (!c || !n) return KDOP_ERR_ARG;
+  if (!c->have_narrow) return fail(c, KDOP_ERR_STATE, "kdop_reduce_contacts: last traversal ran without KDOP_NARROW");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t nc = contact_count(c);
+  if (nc == 0) {
+    c->reduced_n = 0;
+    c->reduced = true;
+    *n = 0;
+    return KDOP_OK;
+  }
+  int rc = reduce_scratch(c, nc);
+  if (!rc) rc = ensure(c, c->contacts_tmp, nc);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_prox_keys_pairs, dim3((nc + 255) / 256), dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p, nc,
+                     (const uint2*)c->dpairs.p, reinterpret_cast<ProxKey*>(c->sortbuf.p));
+  uint32_t kept = 0;
+  if ((rc = reduce_keys<kdop_contact>(c, c->contacts.p, c->contacts_tmp.p, nc, &kept))) return rc;
+  if (kept) HIPCHK(c, hipMemcpyAsync(c->contacts.p, c->contacts_tmp.p, (size_t)kept * sizeof(kdop_contact), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->reduced_n = kept;
+  c->reduced = true;
+  *n = kept;
+  return KDOP_OK;
+}
+
+int kdop_reduce_prox_contacts(kdop_ctx* c, uint64_t n, const kdop_prox_contact* in, kdop_prox_contact* out, int is_device, uint64_t* n_out) {
+  if (!c || !n_out) return KDOP_ERR_ARG;
+  *n_out = 0;
+  if (n == 0) return KDOP_OK;
+  if (!in || !out) return fail(c, KDOP_ERR_ARG, "kdop_reduce_prox_contacts: null argument");
+  if (n >= 0xFFFFFFFFull) return fail(c, KDOP_ERR_ARG, "kdop_reduce_prox_contacts: %llu contacts (at most 2^32 - 2)", (unsigned long long)n);
+  if (in == out) return fail(c, KDOP_ERR_ARG, "kdop_reduce_prox_contacts: out aliases in");
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t nc = (uint32_t)n;
+  int rc = reduce_scratch(c, nc);
+  if (!rc && !is_device) rc = ensure(c, c->prox_io, 2 * (size_t)nc);
+  if (rc) return rc;
+  const kdop_prox_contact* din = in;
+  kdop_prox_contact* dout = out;
+  if (!is_device) {
+    HIPCHK(c, hipMemcpyAsync(c->prox_io.p, in, (size_t)nc * sizeof(kdop_prox_contact), hipMemcpyHostToDevice, c->stream));
+    din = c->prox_io.p;
+    dout = c->prox_io.p + nc;
+  }
+  hipLaunchKernelGGL(k_prox_keys_points, dim3((nc + 255) / 256), dim3(256), 0, c->stream, din, nc, reinterpret_cast<ProxKey*>(c->sortbuf.p));
+  uint32_t kept = 0;
+  if ((rc = reduce_keys<kdop_prox_contact>(c, din, dout, nc, &kept))) return rc;
+  if (!is_device && kept) {
+    HIPCHK(c, hipMemcpyAsync(out, dout, (size_t)kept * sizeof(kdop_prox_contact), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  *n_out = kept;
+  return KDOP_OK;
 }
 
 int kdop_mesh_reorder_gpu(kdop_ctx* c, uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt, const uint32_t* tets,

@@ -129,13 +129,12 @@ __global__ __launch_bounds__(64) void k_raycast(const BodyDev* __restrict__ bodi
                                                 const float* __restrict__ nodes, DirTable dt, uint32_t n, const float* __restrict__ orig,
                                                 const float* __restrict__ dir, float* __restrict__ hits, float* __restrict__ lengths,
                                                 uint8_t* __restrict__ flags) {
-  constexpr int S = rec_stride(K);
   __shared__ uint32_t stk[64][kRayStack];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t* st = stk[threadIdx.x];
   const BodyDev& B = bodies[body];
-  const f3 o = f3{orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
+  const f3 o =f3{orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
   const f3 r = unit3(f3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]});
   const float threshold = 0.01f;
   f3 hit = f3{0.0f, 0.0f, 0.0f};

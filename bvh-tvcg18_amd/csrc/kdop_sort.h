@@ -77,6 +77,20 @@ __device__ inline uint32_t descent_path(uint32_t ja, uint32_t na, uint32_t jb, u
   return steps ? key << (2u * (kMaxPathSteps - steps)) : 0u;
 }
 
+// A float as a uint32 whose unsigned order is the float order, with -0 and +0
+// mapped to one value (they compare equal as floats) and NaNs placed past the
+// infinities, so a key built from it always has a strict total order (a float
+// comparator with a NaN in it has none, and the rank merge would then leave
+// slots unwritten).  For every non-NaN pair, a < b as floats iff ordered(a) <
+// ordered(b), and a == b iff ordered(a) == ordered(b).
+__device__ inline uint32_t ordered_bits(float f) {
+  const uint32_t u = f == 0.0f ? 0u : __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ordered_float(uint32_t o) {  // the float back (+0 for either zero)
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
 // The OpenCL path's report order: qsort by compare_kernel_contact_points
 // (cl/kdop_cl_tandem_traversal.h:61-104, applied at :1772-1788): test pair,
 // then p, n (component-wise) and depth with float comparisons, equal when
@@ -84,22 +98,46 @@ __device__ inline uint32_t descent_path(uint32_t ja, uint32_t na, uint32_t jb, u
 // (qsort leaves them in no particular order).
 struct ContactClKey {
   uint32_t pair;
-  float v[7];  // p[3], n[3], depth
+  uint32_t v[7];  // ordered_bits of p[3], n[3], depth
   uint32_t idx;
   __device__ bool operator<(const ContactClKey& o) const {
     if (pair != o.pair) return pair < o.pair;
 #pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      if (v[i] < o.v[i]) return true;
-      if (o.v[i] < v[i]) return false;
-    }
+    for (int i = 0; i < 7; ++i)
+      if (v[i] != o.v[i]) return v[i] < o.v[i];
     return idx < o.idx;
   }
   __device__ static ContactClKey top() {
     ContactClKey k;
     k.pair = ~0u;
 #pragma unroll
-    for (int i = 0; i < 7; ++i) k.v[i] = 0.0f;
+    for (int i = 0; i < 7; ++i) k.v[i] = ~0u;
+    k.idx = ~0u;
+    return k;
+  }
+};
+
+// PROX contact reduction (SIMULATION/PROX/PROX/include/prox_collision_detection.h:
+// 340-345: std::sort, then std::unique) over prox::ContactPoint
+// (prox_contact_point.h).  Its operator< (:140-190) orders by body i, body j,
+// then p.y, p.x, p.z, n.y, n.x, n.z, depth with float comparisons; the source
+// position breaks the remaining ties, which leaves points equal in every
+// compared field (std::sort leaves them in no particular order; they can
+// differ only in the sign of a zero).  w: body i, body j, then ordered_bits
+// of the seven floats in that order.
+struct ProxKey {
+  uint32_t w[9];
+  uint32_t idx;
+  __device__ bool operator<(const ProxKey& o) const {
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (w[i] != o.w[i]) return w[i] < o.w[i];
+    return idx < o.idx;
+  }
+  __device__ static ProxKey top() {
+    ProxKey k;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) k.w[i] = ~0u;
     k.idx = ~0u;
     return k;
   }
@@ -231,9 +269,9 @@ __global__ void k_contact_keys_cl(const kdop_contact* __restrict__ c, uint32_t n
   const kdop_contact& x = c[i];
   ContactClKey k;
   k.pair = x.pair;
-  k.v[0] = x.p[0]; k.v[1] = x.p[1]; k.v[2] = x.p[2];
-  k.v[3] = x.n[0]; k.v[4] = x.n[1]; k.v[5] = x.n[2];
-  k.v[6] = x.depth;
+  k.v[0] = ordered_bits(x.p[0]); k.v[1] = ordered_bits(x.p[1]); k.v[2] = ordered_bits(x.p[2]);
+  k.v[3] = ordered_bits(x.n[0]); k.v[4] = ordered_bits(x.n[1]); k.v[5] = ordered_bits(x.n[2]);
+  k.v[6] = ordered_bits(x.depth);
   k.idx = i;
   keys[i] = k;
 }
@@ -243,6 +281,86 @@ __global__ void k_contact_permute(const kdop_contact* __restrict__ in, const Key
                                   kdop_contact* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[keys[i].idx];
+}
+
+// ----------------------------------------------------- contact reduction
+__device__ inline ProxKey prox_key(uint32_t bi, uint32_t bj, const float* p, const float* n, float depth, uint32_t idx) {
+  ProxKey k;
+  k.w[0] = bi; k.w[1] = bj;
+  k.w[2] = ordered_bits(p[1]); k.w[3] = ordered_bits(p[0]); k.w[4] = ordered_bits(p[2]);
+  k.w[5] = ordered_bits(n[1]); k.w[6] = ordered_bits(n[0]); k.w[7] = ordered_bits(n[2]);
+  k.w[8] = ordered_bits(depth);
+  k.idx = idx;
+  return k;
+}
+// Contacts of the last traversal: body i and body j are the pair's bodies A
+// and B (ContactCallbackFunctor, prox_collision_detection.h:38-110).
+__global__ void k_prox_keys_pairs(const kdop_contact* __restrict__ c, uint32_t n, const uint2* __restrict__ pairs, ProxKey* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const kdop_contact& x = c[i];
+  const uint2 bp = pairs[x.pair];
+  keys[i] = prox_key(bp.x, bp.y, x.p, x.n, x.depth, i);
+}
+__global__ void k_prox_keys_points(const kdop_prox_contact* __restrict__ c, uint32_t n, ProxKey* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const kdop_prox_contact& x = c[i];
+  keys[i] = prox_key(x.body_i, x.body_j, x.p, x.n, x.depth, i);
+}
+
+// tiny::working_precision<float>() (tiny_precision.h:21-24): epsilon * 10.
+constexpr float kProxPrecision = FLT_EPSILON * 10.0f;
+
+// ContactPoint::operator== (prox_contact_point.h:91-138): the same bodies and
+// every compared float within the working precision (|difference| > it fails).
+__device__ inline bool prox_equal(const ProxKey& a, const ProxKey& b) {
+  if (a.w[0] != b.w[0] || a.w[1] != b.w[1]) return false;
+#pragma unroll
+  for (int i = 2; i < 9; ++i)
+    if (fabsf(ordered_float(b.w[i]) - ordered_float(a.w[i])) > kProxPrecision) return false;
+  return true;
+}
+// b follows a in sorted order and can equal no point before it: other bodies
+// (the body pair's points are contiguous), or p.y more than the precision
+// above a's (the points of one body pair are sorted by p.y, and a correctly
+// rounded difference only grows as the subtrahend falls, so every earlier
+// point of the pair is at least as far below).
+__device__ inline bool prox_break(const ProxKey& a, const ProxKey& b) {
+  return a.w[0] != b.w[0] || a.w[1] != b.w[1] || fabsf(ordered_float(b.w[2]) - ordered_float(a.w[2])) > kProxPrecision;
+}
+
+// std::unique as libstdc++ runs it: a point is kept unless it equals the last
+// point kept (not the point before it: the tolerance makes equality
+// intransitive, so the difference shows).  A point after a break is kept
+// whatever came before, and what follows it depends on nothing before it, so
+// the sorted points split at the breaks into segments handled independently:
+// the thread at a segment's start walks its segment and writes keep (1/0) for
+// each of its points.  Segments are short unless many points share p.y within
+// the precision.
+__global__ __launch_bounds__(256) void k_prox_unique(const ProxKey* __restrict__ keys, uint32_t n, uint32_t* __restrict__ keep) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ProxKey prev = keys[i];
+  if (i > 0u && !prox_break(keys[i - 1u], prev)) return;
+  ProxKey last = prev;
+  keep[i] = 1u;
+  for (uint32_t j = i + 1u; j < n; ++j) {
+    const ProxKey k = keys[j];
+    if (prox_break(prev, k)) break;
+    const bool same = prox_equal(last, k);
+    keep[j] = same ? 0u : 1u;
+    if (!same) last = k;
+    prev = k;
+  }
+}
+
+// out[off[i]] = in[keys[i].idx] for the kept points, in sorted order.
+template <class Rec>
+__global__ void k_prox_compact(const Rec* __restrict__ in, const ProxKey* __restrict__ keys, const uint32_t* __restrict__ keep,
+                               const uint32_t* __restrict__ off, uint32_t n, Rec* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && keep[i]) out[off[i]] = in[keys[i].idx];
 }
 
 // -------------------------------------------------------- Morton reorder

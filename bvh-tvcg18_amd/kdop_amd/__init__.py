@@ -295,6 +295,25 @@ class Context:
         code = {"sequential": _abi.KDOP_ORDER_SEQUENTIAL, "opencl": _abi.KDOP_ORDER_OPENCL}[order]
         self._check(self._lib.kdop_sort_contacts(self._h, code))
 
+    def reduce_contacts(self):
+        """PROX contact reduction of the last traversal's contacts on the device
+        (kdop_reduce_contacts): sorted by (body i, body j, p.y, p.x, p.z, n.y, n.x,
+        n.z, depth), contacts within the working precision of the last one kept
+        dropped.  Returns the count left; contacts() then returns them."""
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_reduce_contacts(self._h, ctypes.byref(n)))
+        return n.value
+
+    def reduce_prox_contacts(self, contacts):
+        """The same reduction over an array of PROX_CONTACT_DTYPE records
+        (kdop_reduce_prox_contacts); returns the reduced records."""
+        c = np.ascontiguousarray(contacts, _abi.PROX_CONTACT_DTYPE)
+        out = np.zeros(c.shape[0], _abi.PROX_CONTACT_DTYPE)
+        n = ctypes.c_uint64()
+        self._check(self._lib.kdop_reduce_prox_contacts(self._h, c.shape[0], ptr(c) if c.shape[0] else None,
+                                                        ptr(out) if c.shape[0] else None, 0, ctypes.byref(n)))
+        return out[:n.value]
+
     def mesh_reorder(self, X, Y, Z, tets):
         """Morton reorder of a mesh on the device (kdop_mesh_reorder_gpu): (X, Y, Z, tets) reordered."""
         X = np.ascontiguousarray(X, np.float32)

@@ -61,10 +61,15 @@ class Counters(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
-# numpy view of kdop_contact (44 bytes)
+# numpy view of kdop_sphere_contact (36 bytes)
 SPHERE_CONTACT_DTYPE = np.dtype([
     ("sphere", np.uint32), ("tet", np.uint32), ("p", np.float32, (3,)), ("n", np.float32, (3,)), ("depth", np.float32),
 ])
+# numpy view of kdop_prox_contact (36 bytes)
+PROX_CONTACT_DTYPE = np.dtype([
+    ("body_i", np.uint32), ("body_j", np.uint32), ("p", np.float32, (3,)), ("n", np.float32, (3,)), ("depth", np.float32),
+])
+# numpy view of kdop_contact (44 bytes)
 CONTACT_DTYPE = np.dtype([
     ("pair", np.uint32), ("tet_a", np.uint32), ("tet_b", np.uint32), ("ordinal", np.uint32),
     ("p", np.float32, (3,)), ("n", np.float32, (3,)), ("depth", np.float32),
@@ -89,6 +94,8 @@ SIGNATURES = [
     ("kdop_get_pairs", ctypes.c_int, [_p, ctypes.c_int, _p, _u64, ctypes.c_int, _p]),
     ("kdop_get_contacts", ctypes.c_int, [_p, _p, _u64, ctypes.c_int, _p]),
     ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
+    ("kdop_reduce_contacts", ctypes.c_int, [_p, _p]),
+    ("kdop_reduce_prox_contacts", ctypes.c_int, [_p, _u64, _p, _p, ctypes.c_int, _p]),
     ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p]),
     ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),

@@ -87,6 +87,19 @@ typedef struct kdop_sphere_contact {
   float depth;        /* negative when penetrating                     */
 } kdop_sphere_contact;
 
+/* One contact as PROX holds it (prox::ContactPoint, SIMULATION/PROX/PROX/
+ * include/prox_contact_point.h:15-63): the two bodies, point, unit normal and
+ * penetration measure.  Bodies are numbered by the caller (PROX compares body
+ * pointers; a body's index in the caller's body array orders them the same way
+ * when the bodies sit in one array). */
+typedef struct kdop_prox_contact {
+  uint32_t body_i;
+  uint32_t body_j;
+  float p[3];
+  float n[3];
+  float depth;
+} kdop_prox_contact;
+
 /* Library defaults: K=8, chunk_bytes=8000, device 0, autotuned traversal. */
 void kdop_default_config(kdop_config* cfg);
 
@@ -142,7 +155,7 @@ int kdop_traverse(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, ui
 int kdop_get_pairs(kdop_ctx* ctx, int which, uint32_t* dst, uint64_t cap, int dst_is_device, uint64_t* n);
 
 /* Copy the contacts of the last traversal, in device order unless
- * kdop_sort_contacts ran after it. */
+ * kdop_sort_contacts or kdop_reduce_contacts ran after it. */
 int kdop_get_contacts(kdop_ctx* ctx, kdop_contact* dst, uint64_t cap, int dst_is_device, uint64_t* n);
 
 /* Orders of kdop_sort_contacts. */
@@ -160,6 +173,29 @@ int kdop_get_contacts(kdop_ctx* ctx, kdop_contact* dst, uint64_t cap, int dst_is
  * kdop_cl_tandem_traversal.h:61-104 compare_kernel_contact_points, applied at
  * :1772-1788), with the source position breaking ties that qsort leaves open. */
 int kdop_sort_contacts(kdop_ctx* ctx, int order);
+
+/* PROX contact reduction, on the device, of the contacts of the last
+ * KDOP_NARROW traversal, in place: body i and body j of a contact are its
+ * pair's bodies A and B (the ContactCallbackFunctor of
+ * SIMULATION/PROX/PROX/include/prox_collision_detection.h:38-110).  Replaces
+ * the std::sort + std::unique of prox_collision_detection.h:340-345 over
+ * prox::ContactPoint: sorted by (body i, body j, p.y, p.x, p.z, n.y, n.x,
+ * n.z, depth) with float comparisons (prox_contact_point.h:140-190), then a
+ * contact is dropped when it equals the last one kept, equal meaning the same
+ * bodies and every compared float within tiny::working_precision<float>()
+ * (prox_contact_point.h:91-138, tiny_precision.h:21-24).  Points equal in every
+ * compared field (which std::sort leaves in no particular order) keep their
+ * traversal order.  *n receives the count left; kdop_get_contacts then
+ * returns the reduced contacts in that sorted order (each keeps its pair,
+ * tets and ordinal). */
+int kdop_reduce_contacts(kdop_ctx* ctx, uint64_t* n);
+
+/* The same reduction over a caller's contact array (in: n contacts; out: room
+ * for n; both device pointers when is_device, else host; out must not alias
+ * in).  *n_out receives the count written.  For contacts of several narrow
+ * phases gathered into one array, as PROX gathers them. */
+int kdop_reduce_prox_contacts(kdop_ctx* ctx, uint64_t n, const kdop_prox_contact* in, kdop_prox_contact* out, int is_device,
+                              uint64_t* n_out);
 
 /* Morton reorder of a tetrahedral mesh on the device (host arrays in and
  * out; outputs must not alias inputs): tets sorted by the Morton code of their

@@ -12,6 +12,8 @@
  * refitted in one call, and a whole pair list is traversed in one call, after
  * which every pair's callback receives its contacts in the order the
  * sequential reference reports them (kdop_sort_contacts, KDOP_ORDER_SEQUENTIAL).
+ * reduce_contacts then gives what PROX's contact reduction makes of those
+ * contacts (prox_collision_detection.h:340-345), computed on the device.
  * Errors throw kdop_amd::Error (the reference exits the process instead).
  * One Context per host thread.
  */
@@ -117,6 +119,7 @@ class Context {
     }
     kdop_counters cnt;
     check(kdop_traverse(c_, (uint32_t)pool.size(), ids.data(), KDOP_NARROW, &cnt));
+    pair_bodies_ = ids;
     check(kdop_sort_contacts(c_, order));
     uint64_t n = 0;
     check(kdop_get_contacts(c_, nullptr, 0, 0, &n));
@@ -129,9 +132,36 @@ class Context {
     return cnt;
   }
 
+  /* PROX contact reduction of the last tandem_traversal's contacts (the
+   * std::sort + std::unique of prox_collision_detection.h:340-345 over the
+   * ContactPoints its callbacks would have collected, body i and body j being
+   * each pair's trees): sorted by (body i, body j, p.y, p.x, p.z, n.y, n.x,
+   * n.z, depth), contacts within the working precision of the last one kept
+   * dropped.  Bodies are the Tree::body numbers. */
+  std::vector<kdop_prox_contact> reduce_contacts() {
+    uint64_t n = 0;
+    check(kdop_reduce_contacts(c_, &n));
+    contacts_.resize((size_t)n);
+    if (n) check(kdop_get_contacts(c_, contacts_.data(), n, 0, &n));
+    std::vector<kdop_prox_contact> out((size_t)n);
+    for (size_t i = 0; i < out.size(); ++i) {
+      const kdop_contact& k = contacts_[i];
+      kdop_prox_contact& o = out[i];
+      o.body_i = pair_bodies_[2 * (size_t)k.pair];
+      o.body_j = pair_bodies_[2 * (size_t)k.pair + 1];
+      for (int j = 0; j < 3; ++j) {
+        o.p[j] = k.p[j];
+        o.n[j] = k.n[j];
+      }
+      o.depth = k.depth;
+    }
+    return out;
+  }
+
  private:
   kdop_ctx* c_ = nullptr;
   std::vector<kdop_contact> contacts_;
+  std::vector<uint32_t> pair_bodies_;  // body ids of the last traversal's pairs
   void check(int rc) const {
     if (rc != KDOP_OK) throw Error(rc, kdop_last_error(c_));
   }

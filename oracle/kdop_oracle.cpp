@@ -26,6 +26,10 @@
 //   SAT                LP feasibility (scipy) on random tetrahedron pairs
 //   contacts           analytic depth on axis-aligned configurations; FP64
 //                      restatement agrees to 1e-5 relative
+//   prox_reduce        hand-worked known answers (key order, the precision
+//                      boundary, std::unique's last-kept rule); numpy stable
+//                      lexsort + first of each run on grid-valued contacts
+//                      (tests/test_reduce_oracle_cpu.py)
 // The reference itself is unbuildable in this container (its TINY headers need
 // Boost, which is not installed), so no output of the compiled reference pins it.
 
@@ -1257,6 +1261,66 @@ uint64_t orc_single_traversal(void* h, const uint32_t* tets, const float* X, con
     single_traversal(t, m, mk(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3], flip != 0, i, all);
   for (size_t i = 0; i < all.size() && i < cap; ++i) out[i] = all[i];
   return all.size();
+}
+
+// PROX contact reduction (SIMULATION/PROX/PROX/include/prox_collision_detection.h:
+// 340-345): std::sort of prox::ContactPoint, then std::unique.  The record
+// is (body i, body j, p[3], n[3], depth); bodies are compared as numbers (the
+// reference compares body pointers).  Writes the input positions of the kept
+// contacts, in their sorted order, to out; returns their count.  The
+// reference's std::sort is unstable; std::stable_sort here leaves contacts that
+// are equal under operator< (equal in every compared field, up to the sign of
+// a zero) in input order.
+struct ProxPoint {
+  uint32_t bi, bj;
+  float p[3], n[3], d;
+  uint32_t pos;
+};
+// ContactPoint::operator< (prox_contact_point.h:140-190)
+static bool prox_less(const ProxPoint& a, const ProxPoint& b) {
+  if (a.bi < b.bi) return true;
+  if (a.bi > b.bi) return false;
+  if (a.bj < b.bj) return true;
+  if (a.bj > b.bj) return false;
+  const float av[7] = {a.p[1], a.p[0], a.p[2], a.n[1], a.n[0], a.n[2], a.d};
+  const float bv[7] = {b.p[1], b.p[0], b.p[2], b.n[1], b.n[0], b.n[2], b.d};
+  for (int i = 0; i < 7; ++i) {
+    if (av[i] < bv[i]) return true;
+    if (av[i] > bv[i]) return false;
+  }
+  return false;
+}
+// ContactPoint::operator== (prox_contact_point.h:91-138) with
+// tiny::working_precision<float>() = epsilon * 10 (tiny_precision.h:21-24);
+// `self` is the contact std::unique kept last, `pt` the candidate.
+static bool prox_equal(const ProxPoint& self, const ProxPoint& pt) {
+  const float precision = std::numeric_limits<float>::epsilon() * 10;
+  if (self.bi != pt.bi) return false;
+  if (self.bj != pt.bj) return false;
+  if (std::fabs(pt.p[1] - self.p[1]) > precision) return false;
+  if (std::fabs(pt.p[0] - self.p[0]) > precision) return false;
+  if (std::fabs(pt.p[2] - self.p[2]) > precision) return false;
+  if (std::fabs(pt.n[1] - self.n[1]) > precision) return false;
+  if (std::fabs(pt.n[0] - self.n[0]) > precision) return false;
+  if (std::fabs(pt.n[2] - self.n[2]) > precision) return false;
+  if (std::fabs(pt.d - self.d) > precision) return false;
+  return true;
+}
+uint64_t orc_prox_reduce(uint64_t n, const uint32_t* bi, const uint32_t* bj, const float* p, const float* nrm, const float* d, uint32_t* out) {
+  std::vector<ProxPoint> v(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    ProxPoint& x = v[i];
+    x.bi = bi[i];
+    x.bj = bj[i];
+    for (int k = 0; k < 3; ++k) { x.p[k] = p[3 * i + k]; x.n[k] = nrm[3 * i + k]; }
+    x.d = d[i];
+    x.pos = (uint32_t)i;
+  }
+  std::stable_sort(v.begin(), v.end(), prox_less);
+  auto last = std::unique(v.begin(), v.end(), prox_equal);
+  v.erase(last, v.end());
+  for (size_t i = 0; i < v.size(); ++i) out[i] = v[i].pos;
+  return v.size();
 }
 
 // Brute force: every (leaf of A, leaf of B) whose leaf DOPs overlap.  The leaf

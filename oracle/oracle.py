@@ -54,6 +54,8 @@ def lib():
         L.orc_raycast.argtypes = [_p, _p, _p, _p, _p, _p, _u32, _p, _p, _p, _p, _p]
         L.orc_single_traversal.restype = _u64
         L.orc_single_traversal.argtypes = [_p, _p, _p, _p, _p, _p, _u32, _p, ctypes.c_int, _p, _u64]
+        L.orc_prox_reduce.restype = _u64
+        L.orc_prox_reduce.argtypes = [_u64, _p, _p, _p, _p, _p, _p]
         L.orc_brute_pairs.restype = _u64
         L.orc_brute_pairs.argtypes = [_p, _p, _p, _u64]
         L.orc_contacts_pair_f32.argtypes = [_p, _p, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _p]
@@ -252,6 +254,21 @@ def single_traversal(tree, mesh, spheres, flip=False):
     if n:
         lib().orc_single_traversal(tree.h, _ptr(t), _ptr(X), _ptr(Y), _ptr(Z), _ptr(S), sp.shape[0], _ptr(sp), int(flip), _ptr(out), n)
     return out
+
+
+def prox_reduce(body_i, body_j, p, n, depth):
+    """PROX contact reduction (prox_collision_detection.h:340-345: std::sort, std::unique
+    over prox::ContactPoint): the input positions of the kept contacts, in sorted order."""
+    bi = np.ascontiguousarray(body_i, np.uint32)
+    bj = np.ascontiguousarray(body_j, np.uint32)
+    P = np.ascontiguousarray(np.asarray(p, np.float32).reshape(-1, 3))
+    N = np.ascontiguousarray(np.asarray(n, np.float32).reshape(-1, 3))
+    D = np.ascontiguousarray(depth, np.float32)
+    m = bi.shape[0]
+    assert bj.shape[0] == m and P.shape[0] == m and N.shape[0] == m and D.shape[0] == m
+    out = np.zeros(max(m, 1), np.uint32)
+    k = int(lib().orc_prox_reduce(m, _ptr(bi), _ptr(bj), _ptr(P), _ptr(N), _ptr(D), _ptr(out)))
+    return out[:k]
 
 
 def brute_pairs(tree_a, tree_b, cap):

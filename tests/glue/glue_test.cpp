@@ -2,8 +2,10 @@
 // three box meshes in the reference's call shapes (make_tree, a pool refit,
 // a pair-list tandem traversal with one ContactsCallback per pair), two pose
 // steps; every pair's callback must receive exactly the contacts the oracle
-// reports for that pair, in the oracle's order, bit for bit.  Exit status 0
-// when all match.  Build: python bvh-tvcg18_amd/build.py --oracle.
+// reports for that pair, in the oracle's order, bit for bit; and the device's
+// PROX contact reduction of them must keep what the oracle's std::sort +
+// std::unique keeps of the same contacts collected in callback order, as
+// PROX collects them.  Exit status 0 when all match.  Build: python bvh-tvcg18_amd/build.py --oracle.
 #include <stdint.h>
 
 #include <cstdio>
@@ -28,6 +30,7 @@ int orc_traverse(void* hA, int ntA, const uint32_t* tetsA, const float* XA, cons
                  int ntB, const uint32_t* tetsB, const float* XB, const float* YB, const float* ZB, const uint8_t* surfB, int do_narrow,
                  int threads, uint32_t* p1, uint64_t p1_cap, uint32_t* p2, uint64_t p2_cap, OrcContact* contacts, uint64_t c_cap,
                  uint64_t* counters, double* seconds);
+uint64_t orc_prox_reduce(uint64_t n, const uint32_t* bi, const uint32_t* bj, const float* p, const float* nrm, const float* d, uint32_t* out);
 }
 
 struct MeshData {
@@ -113,6 +116,31 @@ int main() {
         bad += same ? 0 : 1;
         total += ng;
       }
+      // PROX's contacts vector: every pair's callback contacts, pair by pair
+      std::vector<uint32_t> bi, bj;
+      std::vector<float> P, N, D;
+      for (int k = 0; k < 3; ++k)
+        for (size_t i = 0; i < got[k].v.size() / 7; ++i) {
+          const float* v = &got[k].v[7 * i];
+          bi.push_back(trees[pairs[k][0]].body);
+          bj.push_back(trees[pairs[k][1]].body);
+          P.insert(P.end(), v, v + 3);
+          N.insert(N.end(), v + 3, v + 6);
+          D.push_back(v[6]);
+        }
+      std::vector<uint32_t> keep(bi.size() + 1);
+      const size_t nk = (size_t)orc_prox_reduce(bi.size(), bi.data(), bj.data(), P.data(), N.data(), D.data(), keep.data());
+      const std::vector<kdop_prox_contact> red = ctx.reduce_contacts();
+      bool same = red.size() == nk;
+      for (size_t i = 0; same && i < nk; ++i) {
+        const uint32_t j = keep[i];
+        const kdop_prox_contact& r = red[i];
+        const float w7[7] = {P[3 * j], P[3 * j + 1], P[3 * j + 2], N[3 * j], N[3 * j + 1], N[3 * j + 2], D[j]};
+        const float g7[7] = {r.p[0], r.p[1], r.p[2], r.n[0], r.n[1], r.n[2], r.depth};
+        same = r.body_i == bi[j] && r.body_j == bj[j] && std::memcmp(w7, g7, sizeof(w7)) == 0;
+      }
+      std::printf("step %d reduction: %zu of %zu contacts kept (oracle %zu) %s\n", step, red.size(), bi.size(), nk, same ? "match" : "MISMATCH");
+      bad += same ? 0 : 1;
     }
     for (void* h : orc) orc_tree_free(h);
     if (total == 0) {

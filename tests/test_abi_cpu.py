@@ -67,3 +67,27 @@ def test_cpp_header_compiles():
         assert r.returncode == 0, r.stderr
     finally:
         os.unlink(src)
+
+
+def test_record_layouts_match_numpy_views():
+    """Each public record of include/kdop_amd.h has the size and field offsets of
+    its numpy view (static_asserts compiled by the host compiler)."""
+    import subprocess
+    import tempfile
+    from kdop_amd import _abi
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ['#include <stddef.h>', '#include "kdop_amd.h"']
+    for ctype, dt in (("kdop_contact", _abi.CONTACT_DTYPE), ("kdop_sphere_contact", _abi.SPHERE_CONTACT_DTYPE),
+                      ("kdop_prox_contact", _abi.PROX_CONTACT_DTYPE)):
+        lines.append(f'static_assert(sizeof({ctype}) == {dt.itemsize}, "{ctype} size");')
+        for name in dt.names:
+            lines.append(f'static_assert(offsetof({ctype}, {name}) == {dt.fields[name][1]}, "{ctype}.{name}");')
+    with tempfile.NamedTemporaryFile("w", suffix=".cpp", delete=False) as f:
+        f.write("\n".join(lines) + "\nint main() { return 0; }\n")
+        src = f.name
+    try:
+        r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(root, "include"), src],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    finally:
+        os.unlink(src)
