@@ -74,6 +74,9 @@ struct kdop_ctx {
   DevBuf<unsigned char> sortbuf;       // kdop_sort_contacts: two key arrays
   DevBuf<unsigned char> raybuf;        // kdop_raycast / kdop_single_traversal: inputs and counts
   DevBuf<kdop_sphere_contact> sphere_out;  // kdop_single_traversal: the contacts
+  DevBuf<uint2> sg_items[2];               // kdop_single_traversal: (sphere, node) frontiers
+  DevBuf<uint32_t> sg_scan;                // kdop_single_traversal: counts, offsets, block sums, total, flag
+  uint32_t sg_max_items = 1u << 28;        // KDOP_OPT_SINGLE_ITEMS
   DevBuf<uint32_t> redbuf;                 // contact reduction: keep flags, offsets, scan sums, total
   DevBuf<kdop_prox_contact> prox_io;       // kdop_reduce_prox_contacts: staged input and output
   uint32_t reduced_n = 0;                  // contacts left by kdop_reduce_contacts
@@ -280,11 +283,30 @@ struct Raycast {
   }
 };
 template <int K>
-struct Single {
-  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* spheres, int flip, int write, uint32_t* count, const uint32_t* offset,
-                  kdop_sphere_contact* out) {
-    hipLaunchKernelGGL(k_single<K>, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->dbodies.p, body, c->dtops.p, c->nodes.p, c->dt, n,
-                       spheres, flip, write, count, offset, out);
+struct SgSeedCount {
+  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* spheres, uint32_t* count) {
+    hipLaunchKernelGGL(k_sg_seed_count<K>, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, c->nodes.p, c->dt, n,
+                       spheres, count);
+  }
+};
+template <int K>
+struct SgExpandCount {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* spheres, uint32_t* count) {
+    hipLaunchKernelGGL(k_sg_expand_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, spheres, count);
+  }
+};
+template <int K>
+struct SgExpandWrite {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const uint32_t* count, const uint32_t* off, uint2* out) {
+    hipLaunchKernelGGL(k_sg_expand_write<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, count, off, out);
+  }
+};
+template <int K>
+struct SgLeaves {
+  static void run(kdop_ctx* c, uint32_t body, const uint2* items, uint32_t m, const float* spheres, uint32_t sphere_base, int flip,
+                  int write, uint32_t* flag, const uint32_t* off, kdop_sphere_contact* out, uint32_t* bad) {
+    hipLaunchKernelGGL(k_sg_leaves<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, items, m, c->nodes.p, c->dt,
+                       spheres, sphere_base, flip, write, flag, off, out, bad);
   }
 };
 template <int K>
@@ -504,7 +526,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
-  release(c->redbuf); release(c->prox_io);
+  release(c->redbuf); release(c->prox_io); release(c->sg_items[0]); release(c->sg_items[1]); release(c->sg_scan);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -570,6 +592,11 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       if (v == 0) v = 6;
       if (v < 1 || v > 62) break;
       c->bfs_max = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_SINGLE_ITEMS:
+      if (v == 0) v = 1ll << 28;
+      if (v < 1 || v > (1ll << 30)) break;
+      c->sg_max_items = (uint32_t)v;
       return KDOP_OK;
     default:
       return fail(c, KDOP_ERR_ARG, "kdop_set_option: unknown option %d", option);
@@ -650,6 +677,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_NARROW_PATH: *v = c->narrow_path; return KDOP_OK;
     case KDOP_OPT_GRAPHS: *v = c->use_graphs ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
+    case KDOP_OPT_SINGLE_ITEMS: *v = c->sg_max_items; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
 }
@@ -1540,32 +1568,96 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
   const BodyHost& b = c->bodies[body];
   uint32_t lg = 0;
   while ((1u << lg) < b.L) ++lg;
-  if (b.H * std::max(lg, 1u) + 2 > kRayStack) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: tree too deep for the stack");
+  const uint32_t steps = b.H * std::max(lg, 1u);  // below the top-level chunk roots, every item is a leaf after these
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
-  // [spheres 4n floats][count n][offset n][block sums nb][total 1] in raybuf
-  const size_t o_cnt = 16 * (size_t)n, o_off = o_cnt + 4 * (size_t)n, o_sums = o_off + 4 * (size_t)n, o_tot = o_sums + 4 * (size_t)nb;
-  int rc = ensure(c, c->raybuf, o_tot + 4);
+  int rc = ensure(c, c->raybuf, 16 * (size_t)n);
   if (rc) return rc;
   float* d_s = reinterpret_cast<float*>(c->raybuf.p);
-  uint32_t* d_cnt = reinterpret_cast<uint32_t*>(c->raybuf.p + o_cnt);
-  uint32_t* d_off = reinterpret_cast<uint32_t*>(c->raybuf.p + o_off);
-  uint32_t* d_sums = reinterpret_cast<uint32_t*>(c->raybuf.p + o_sums);
-  uint32_t* d_tot = reinterpret_cast<uint32_t*>(c->raybuf.p + o_tot);
   HIPCHK(c, hipMemcpyAsync(d_s, spheres, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
-  dispatch_K<Single>(c->K, c, body, n, (const float*)d_s, flip, 0, d_cnt, (const uint32_t*)nullptr, (kdop_sphere_contact*)nullptr);
-  HIPCHK(c, hipGetLastError());
-  if ((rc = device_scan(c, d_cnt, d_off, n, d_sums, d_tot))) return rc;
-  uint32_t total = 0;
-  HIPCHK(c, hipMemcpyAsync(&total, d_tot, 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // scan scratch for m values: [count m][offset m][block sums][total][bad]
+  auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
+  auto read_total = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
+    const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
+    HIPCHK(c, hipMemcpyAsync(host, c->sg_scan.p + 2 * (size_t)m + nb, 4 * (size_t)words, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return KDOP_OK;
+  };
+  auto scan = [&](uint32_t m) {
+    const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
+    uint32_t* cnt = c->sg_scan.p;
+    return device_scan(c, cnt, cnt + m, m, cnt + 2 * (size_t)m, cnt + 2 * (size_t)m + nb);
+  };
+  // Spheres go in batches whose frontier stays within kMaxItems items; the
+  // first batch is all of them, and a batch that outgrows it is halved.
+  const uint32_t kMaxItems = c->sg_max_items;
+  uint64_t total = 0;
+  uint32_t s0 = 0, batch = n;
+  while (s0 < n) {
+    const uint32_t sn = std::min(batch, n - s0);
+    const float* bs = d_s + 4 * (size_t)s0;
+    if ((rc = scratch(sn))) return rc;
+    dispatch_K<SgSeedCount>(c->K, c, body, sn, bs, c->sg_scan.p);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = scan(sn))) return rc;
+    uint32_t m = 0;
+    if ((rc = read_total(sn, &m, 1))) return rc;
+    bool too_big = m > kMaxItems;
+    if (m && !too_big) {
+      if ((rc = ensure(c, c->sg_items[0], m))) return rc;
+      hipLaunchKernelGGL(k_sg_seed_write, dim3((sn + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, c->dtops.p, sn,
+                         (const uint32_t*)c->sg_scan.p, (const uint32_t*)(c->sg_scan.p + sn), c->sg_items[0].p);
+      HIPCHK(c, hipGetLastError());
+    }
+    int cur = 0;
+    for (uint32_t step = 0; !too_big && m && step < steps; ++step) {
+      if ((rc = scratch(m))) return rc;
+      dispatch_K<SgExpandCount>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bs, c->sg_scan.p);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = scan(m))) return rc;
+      uint32_t m2 = 0;
+      if ((rc = read_total(m, &m2, 1))) return rc;
+      if (m2 > kMaxItems) { too_big = true; break; }
+      if (m2) {
+        if ((rc = ensure(c, c->sg_items[cur ^ 1], m2))) return rc;
+        dispatch_K<SgExpandWrite>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint32_t*)c->sg_scan.p,
+                                  (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
+        HIPCHK(c, hipGetLastError());
+      }
+      cur ^= 1;
+      m = m2;
+    }
+    if (too_big) {
+      if (sn == 1) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: sphere %u overlaps more than %u nodes of one level", s0, kMaxItems);
+      batch = std::max(1u, sn / 2);
+      continue;
+    }
+    if (m) {
+      if ((rc = scratch(m))) return rc;
+      const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
+      uint32_t* bad = c->sg_scan.p + 2 * (size_t)m + nb + 1;
+      HIPCHK(c, hipMemsetAsync(bad, 0, 4, c->stream));
+      dispatch_K<SgLeaves>(c->K, c, body, (const uint2*)c->sg_items[cur].p, m, bs, s0, flip, 0, c->sg_scan.p, (const uint32_t*)nullptr,
+                           (kdop_sphere_contact*)nullptr, bad);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = scan(m))) return rc;
+      uint32_t tb[2] = {0, 0};  // contacts of the batch, bad
+      if ((rc = read_total(m, tb, 2))) return rc;
+      if (tb[1]) return fail(c, KDOP_ERR_INTERNAL, "kdop_single_traversal: a node other than a leaf after %u steps", steps);
+      if (tb[0]) {
+        if (total + tb[0] >= (1ull << 32)) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: more than 2^32 contacts");
+        if ((rc = ensure(c, c->sphere_out, total + tb[0], true))) return rc;
+        dispatch_K<SgLeaves>(c->K, c, body, (const uint2*)c->sg_items[cur].p, m, bs, s0, flip, 1, c->sg_scan.p,
+                             (const uint32_t*)(c->sg_scan.p + m), c->sphere_out.p + total, (uint32_t*)nullptr);
+        HIPCHK(c, hipGetLastError());
+        total += tb[0];
+      }
+    }
+    s0 += sn;
+  }
   *count = total;
   if (!out) return KDOP_OK;
-  if (cap < total) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: need %u contacts", total);
+  if (cap < total) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: need %llu contacts", (unsigned long long)total);
   if (total == 0) return KDOP_OK;
-  if ((rc = ensure(c, c->sphere_out, total))) return rc;
-  dispatch_K<Single>(c->K, c, body, n, (const float*)d_s, flip, 1, d_cnt, (const uint32_t*)d_off, c->sphere_out.p);
-  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(out, c->sphere_out.p, (size_t)total * sizeof(kdop_sphere_contact), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;

@@ -2,19 +2,18 @@
 // (SIMULATION/KDOP/KDOP/include/kdop_single_traversal.h:41-161) with the
 // sphere-tetrahedron contact of contacts/geometry_contacts_sphere_tetrahedron.h.
 //
-// One thread per sphere.  At and below the branch-chunk roots every node gets
-// the reference's overlap test in the reference's depth-first order, and a
-// leaf the reference's contact routine, so each sphere's contacts come out in
-// the order the sequential code reports them.  The reference tests all C
-// branch roots per sphere; here the super levels above them are walked first
-// with the same overlap test, which can only fail for a branch root when it
-// fails for every super node above it (a super DOP contains the DOPs below).
-// Two passes: the first counts each sphere's contacts, the second writes them
-// at the sphere's offset, so the output is grouped by sphere without a sort.
+// All spheres are traversed together, breadth first, over (sphere, node)
+// items kept in the reference's visiting order, so each sphere's contacts come
+// out in the order the sequential code reports them and the output is grouped
+// by sphere without a sort.  At and below the branch-chunk roots every node
+// gets the reference's overlap test and a leaf the reference's contact
+// routine.  The reference tests all C branch roots per sphere; here the super
+// levels above them are tested first with the same overlap test, which can
+// only fail for a branch root when it fails for every super node above it (a
+// super DOP contains the DOPs below).
 #pragma once
 
 #include "kdop_kernels.h"
-#include "kdop_raycast.h"
 
 namespace kdop_amd {
 
@@ -125,69 +124,144 @@ __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uin
   return false;
 }
 
-// One thread per sphere; write = false counts its contacts into count[i],
-// write = true writes them from out[offset[i]] on.  geometry_convert.h:13-31
-// makes the sphere's DOP (o -/+ radius per direction), kdop_single_traversal.h:
-// 139-160 the root test and the walk over the branch chunks in order.
+// ----------------------------------------------------------- traversal
+// Breadth first over (sphere, node) items of every sphere at once, in an
+// order that is the reference's: the frontier holds, sphere by sphere, the
+// untested nodes and the tested leaves left to right, and every step replaces
+// each item in place by what it becomes (nothing when its test fails, the
+// tested leaf itself, or its two children left child first), with the places
+// taken from an exclusive scan of those counts.  Within one chunk tree the
+// depth-first walk of kdop_single_traversal.h:41-117 reaches leaves left to
+// right (it descends into the first child first), and the chunks are walked in
+// order (:139-160), so the leaves come out in the order the reference tests
+// them, which is increasing tet order.  Each node is tested once, with the
+// reference's overlap test, exactly when its parent passed, as in the
+// depth-first walk (a single traversal carries no state from one leaf to the
+// next).  Every level of chunk trees adds at most log2 L steps below the
+// top-level chunk roots (the leaves of a super chunk tree are the roots of
+// the chunks below), so H log2 L steps leave only leaves.
+
+constexpr uint32_t kSgTested = 0x80000000u;  // item node bit: a leaf whose test passed
+
+// geometry_convert.h:13-31: the sphere's DOP, o -/+ radius per direction.
 template <int K>
-__global__ __launch_bounds__(64) void k_single(const BodyDev* __restrict__ bodies, uint32_t body, const uint32_t* __restrict__ tops,
-                                               const float* __restrict__ nodes, DirTable dt, uint32_t n, const float* __restrict__ spheres,
-                                               int flip, int write, uint32_t* __restrict__ count, const uint32_t* __restrict__ offset,
-                                               kdop_sphere_contact* __restrict__ out) {
-  __shared__ uint32_t stk[64][kRayStack];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t* st = stk[threadIdx.x];
-  const BodyDev& B = bodies[body];
-  const f3 c = f3{spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]};
-  const float r = spheres[4 * i + 3];
-  float sd[K];
+__device__ inline void sphere_dop(const float* __restrict__ spheres, uint32_t i, const DirTable& dt, f3& c, float& r, float* sd) {
+  const float4 s = reinterpret_cast<const float4*>(spheres)[i];
+  c = f3{s.x, s.y, s.z};
+  r = s.w;
 #pragma unroll
   for (int k = 0; k < K / 2; ++k) {
     const float o = dot3(f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]}, c);
     sd[2 * k] = o - r;
     sd[2 * k + 1] = o + r;
   }
-  uint32_t found = 0;
-  const uint32_t base = write ? offset[i] : 0u;
+}
+
+// Seed: the root test per sphere (kdop_single_traversal.h:139-141), then the
+// top-level chunk roots of the spheres that pass.
+template <int K>
+__global__ __launch_bounds__(256) void k_sg_seed_count(const BodyDev* __restrict__ bodies, uint32_t body, const float* __restrict__ nodes,
+                                                       DirTable dt, uint32_t n, const float* __restrict__ spheres, uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BodyDev& B = bodies[body];
+  f3 c;
+  float r, sd[K];
+  sphere_dop<K>(spheres, i, dt, c, r, sd);
   NodeRec<K> rec;
   load_rec<K>(nodes, B.node_base, rec);
-  if (dop_overlap<K>(rec.v, sd)) {
-    for (uint32_t top = 0; top < B.ntop; ++top) {
-      uint32_t sp = 0;
-      st[sp++] = tops[B.top_base + top];
-      while (sp) {
-        const uint32_t s = st[--sp];
-        load_rec<K>(nodes, s, rec);
-        if (!dop_overlap<K>(rec.v, sd)) continue;  // kdop_single_traversal.h:63-64 (and the super-level cull)
-        const uint32_t c0 = rec.m0 & IDX_MASK, c1 = rec.m1 & IDX_MASK;
-        if ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) {  // :68-89
-          const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)c0);
-          const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
-          f3 P[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
-          f3 p, nn;
-          float depth;
-          if (sphere_tet_contact(c, r, P, B.surf[c0], flip != 0, p, nn, depth)) {
-            if (write) {
-              kdop_sphere_contact& o = out[base + found];
-              o.sphere = i;
-              o.tet = c0;
-              o.p[0] = p.x; o.p[1] = p.y; o.p[2] = p.z;
-              o.n[0] = nn.x; o.n[1] = nn.y; o.n[2] = nn.z;
-              o.depth = depth;
-            }
-            ++found;
-          }
-        } else {  // :91-110 children in order (super nodes: the lower chunks' roots)
-          st[sp++] = c1;
-          st[sp++] = c0;
-        }
-      }
+  count[i] = dop_overlap<K>(rec.v, sd) ? B.ntop : 0u;
+}
+__global__ __launch_bounds__(256) void k_sg_seed_write(const BodyDev* __restrict__ bodies, uint32_t body, const uint32_t* __restrict__ tops,
+                                                       uint32_t n, const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
+                                                       uint2* __restrict__ items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || count[i] == 0u) return;
+  const BodyDev& B = bodies[body];
+  const uint32_t o = off[i];
+  for (uint32_t t = 0; t < B.ntop; ++t) items[o + t] = make_uint2(i, tops[B.top_base + t]);
+}
+
+// One step: count[j] = what item j becomes (0, 1 or 2 items).
+template <int K>
+__global__ __launch_bounds__(256) void k_sg_expand_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                         DirTable dt, const float* __restrict__ spheres, uint32_t* __restrict__ count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint2 it = items[j];
+  if (it.y & kSgTested) { count[j] = 1u; return; }
+  f3 c;
+  float r, sd[K];
+  sphere_dop<K>(spheres, it.x, dt, c, r, sd);
+  NodeRec<K> rec;
+  load_rec<K>(nodes, it.y, rec);
+  if (!dop_overlap<K>(rec.v, sd)) { count[j] = 0u; return; }  // kdop_single_traversal.h:63-64
+  count[j] = ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) ? 1u : 2u;
+}
+template <int K>
+__global__ __launch_bounds__(256) void k_sg_expand_write(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                         const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
+                                                         uint2* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t k = count[j];
+  if (k == 0u) return;
+  const uint2 it = items[j];
+  const uint32_t o = off[j];
+  if (k == 1u) { out[o] = make_uint2(it.x, it.y | kSgTested); return; }
+  const uint2 mm = load_meta<K>(nodes, it.y);  // :91-110 (super nodes: the lower chunks' roots)
+  out[o] = make_uint2(it.x, mm.x & IDX_MASK);
+  out[o + 1] = make_uint2(it.x, mm.y & IDX_MASK);
+}
+
+// The leaves: the test of a leaf not yet tested, then the contact routine
+// (:68-89).  write = 0: flag[j] = 1 when leaf j gives a contact (and *bad is
+// set if an item is not a leaf, which the step count rules out); write = 1:
+// the flagged leaves write their contact at out[off[j]], numbering spheres
+// from sphere_base (the items number them within their batch).
+template <int K>
+__global__ __launch_bounds__(256) void k_sg_leaves(const BodyDev* __restrict__ bodies, uint32_t body, const uint2* __restrict__ items,
+                                                   uint32_t m, const float* __restrict__ nodes, DirTable dt, const float* __restrict__ spheres,
+                                                   uint32_t sphere_base, int flip, int write, uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                                                   kdop_sphere_contact* __restrict__ out, uint32_t* __restrict__ bad) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  if (write && flag[j] == 0u) return;
+  const uint2 it = items[j];
+  const uint32_t node = it.y & ~kSgTested;
+  f3 c;
+  float r, sd[K];
+  sphere_dop<K>(spheres, it.x, dt, c, r, sd);
+  uint32_t m0;
+  if (it.y & kSgTested) {
+    m0 = load_meta<K>(nodes, node).x;
+  } else {
+    NodeRec<K> rec;
+    load_rec<K>(nodes, node, rec);
+    m0 = rec.m0;
+    if (!((m0 & F_BRANCH) && (m0 & F_LEAF))) {
+      if (!write) { flag[j] = 0u; atomicOr(bad, 1u); }
+      return;
     }
+    if (!dop_overlap<K>(rec.v, sd)) { flag[j] = 0u; return; }
   }
-  if (!write) count[i] = found;
+  const BodyDev& B = bodies[body];
+  const uint32_t t = m0 & IDX_MASK;
+  const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)t);
+  const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
+  f3 P[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
+  f3 p, nn;
+  float depth;
+  const bool hit = sphere_tet_contact(c, r, P, B.surf[t], flip != 0, p, nn, depth);
+  if (!write) { flag[j] = hit ? 1u : 0u; return; }
+  kdop_sphere_contact& o = out[off[j]];
+  o.sphere = sphere_base + it.x;
+  o.tet = t;
+  o.p[0] = p.x; o.p[1] = p.y; o.p[2] = p.z;
+  o.n[0] = nn.x; o.n[1] = nn.y; o.n[2] = nn.z;
+  o.depth = depth;
 }
 
 }  // namespace kdop_amd

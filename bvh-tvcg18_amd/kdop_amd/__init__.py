@@ -345,12 +345,17 @@ class Context:
         grouped by sphere, each sphere's in the reference's report order."""
         sp = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1, 4))
         n = ctypes.c_uint64()
-        self._check(self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), None, 0, ctypes.byref(n)))
-        out = np.zeros(n.value, _abi.SPHERE_CONTACT_DTYPE)
-        if n.value:
-            self._check(self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), n.value,
-                                                        ctypes.byref(n)))
-        return out
+        # room for 64 contacts per sphere (or the last call's count) first, so
+        # that one call usually suffices; np.empty touches no memory
+        cap = max(64 * sp.shape[0], getattr(self, "_single_last", 0), 1)
+        out = np.empty(cap, _abi.SPHERE_CONTACT_DTYPE)
+        rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), cap, ctypes.byref(n))
+        if rc == _abi.KDOP_ERR_CAPACITY:
+            out = np.empty(n.value, _abi.SPHERE_CONTACT_DTYPE)
+            rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), n.value, ctypes.byref(n))
+        self._check(rc)
+        self._single_last = n.value
+        return out[:n.value]
 
     def world(self, body):
         nv = self.meshes[body].nv

@@ -28,6 +28,7 @@ extern "C" {
 #define KDOP_ERR_NOMEM (-3)    /* device allocation failed                                  */
 #define KDOP_ERR_STATE (-4)    /* call out of order (e.g. kdop_narrow before a traversal)   */
 #define KDOP_ERR_CAPACITY (-5) /* caller buffer too small; the required size is reported    */
+#define KDOP_ERR_INTERNAL (-6) /* an internal consistency check failed (a library bug)      */
 
 typedef struct kdop_ctx kdop_ctx;
 
@@ -225,7 +226,9 @@ int kdop_raycast(kdop_ctx* ctx, uint32_t body, uint32_t n, const float* origins,
  * holds x, y, z, radius per sphere).  Contacts go to out grouped by sphere in
  * sphere order, each sphere's in the order the reference reports them through
  * its ContactsCallback; *n receives the contact count, and KDOP_ERR_CAPACITY
- * (nothing written) means cap < *n.  flip: the reference's should_flip.
+ * (nothing written) means cap < *n.  The contacts are computed in full on
+ * every call, so a cap that is large enough the first time saves a second
+ * traversal.  flip: the reference's should_flip.
  * Replaces kdop::single_traversal(sphere, tree, mesh, X, Y, Z, surface_map,
  * callback, should_flip) (SIMULATION/KDOP/KDOP/include/kdop_single_traversal.h:
  * 123-161) for n spheres at once, with the same contacts in the same order. */
@@ -300,6 +303,10 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   queued on the context stream (later calls order after it; */
                                   /*   device pose buffers must stay valid until the next call   */
                                   /*   that synchronises, e.g. kdop_traverse)                    */
+#define KDOP_OPT_SINGLE_ITEMS 12  /* most (sphere, node) items in one breadth-first step of     */
+                                  /*   kdop_single_traversal before it splits the spheres into   */
+                                  /*   batches (0: the default, 2^28; 1 .. 2^30); bounds its     */
+                                  /*   device memory at 8 bytes x 2 x this, results unchanged    */
 
 /* Set / read one option (no counterpart in the reference, whose OpenCL
  * path fixes these sizes at compile time).  Returns KDOP_ERR_ARG for an

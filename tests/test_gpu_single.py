@@ -39,6 +39,36 @@ def test_single_traversal_matches_oracle(k, n, chunk_bytes, nsph, flip):
     assert got.tobytes() == want.tobytes()
 
 
+def test_single_traversal_in_batches_is_unchanged():
+    """A small frontier limit splits the spheres into batches (halving until
+    each fits): the same records; a limit no single sphere fits is an error."""
+    k, n = 26, 16
+    rng = np.random.default_rng(5)
+    mesh = K.box_mesh(n, size=1.0, jitter=0.1, seed=2)
+    ctx = K.Context(K=k)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], np.zeros((1, 3), F32), np.array([[0, 0, 0, 1]], F32))
+    c = rng.uniform(-0.7, 0.7, size=(3000, 3))
+    r = rng.uniform(0.01, 0.2, size=(3000, 1))
+    sp = np.concatenate([c, r], 1).astype(F32)
+    full = ctx.single_traversal(b, sp)
+    X, Y, Z = mesh.X, mesh.Y, mesh.Z
+    tree = O.Tree(k, 8000, mesh.tets, X, Y, Z)
+    tree.refit(X, Y, Z)
+    want = O.single_traversal(tree, (mesh.tets, X, Y, Z, mesh.surface), sp)
+    assert full.tobytes() == want.tobytes()
+    for limit in (20000, 5000):
+        ctx.set_option("single_items", limit)
+        got = ctx.single_traversal(b, sp)
+        print("limit", limit, "contacts", got.shape[0])
+        assert got.tobytes() == full.tobytes()
+    ctx.set_option("single_items", 1)
+    with pytest.raises(K.KdopError):
+        ctx.single_traversal(b, sp[:4])
+    ctx.set_option("single_items", 0)
+    assert ctx.get_option("single_items") == 1 << 28
+
+
 def test_single_traversal_capacity_and_errors():
     ctx = K.Context(K=8)
     b = ctx.make_tree(K.box_mesh(3))
