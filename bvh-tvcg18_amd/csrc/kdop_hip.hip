@@ -421,11 +421,15 @@ struct Seed {
     hipLaunchKernelGGL(k_seed<K>, dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[0].p, fill, cap);
   }
 };
+// A breadth-first level: the persistent strategy's short ramp keeps each level
+// one contiguous run (round 0 reads it so); the level-synchronous traversal
+// writes every level after the seeds in kSegs segments.
 template <int K>
 struct Bfs {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t level, uint32_t target, uint32_t last) {
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t level, uint32_t target, uint32_t last, bool segmented) {
+    const uint32_t nin = (segmented && level > 0) ? (uint32_t)kSegs : 1u, nout = segmented ? (uint32_t)kSegs : 1u;
     hipLaunchKernelGGL(k_bfs_level<K>, dim3((unsigned)(c->num_cus * 4)), dim3(256), 0, c->stream, T, c->front[0].p, c->front[1].p, cap,
-                       c->dstate, level, target, last);
+                       c->dstate, level, target, last, nin, nout);
   }
 };
 // A persistent round, launched with start / stop events attached to its
@@ -844,12 +848,15 @@ bool narrow_one_pass(const kdop_ctx* c, uint64_t p2_guess) {
 }
 int launch_narrow(kdop_ctx* c, const TravArgs& T, int pending_round, bool levels_done, bool one_pass) {
   const NarrowArgs A = narrow_args(c, T, pending_round, levels_done);
-  const dim3 grid((unsigned)(c->num_cus * 4)), block(256);
+  // grid-stride kernels whose iterations are chains of dependent gathers: as
+  // many resident blocks as their LDS allows (about 13 KB for the gate, 26 KB
+  // with the contact stage) hide the chains' latency
+  const dim3 block(256);
   if (one_pass) {
-    hipLaunchKernelGGL(k_narrow_wave, grid, block, 0, c->stream, A);
+    hipLaunchKernelGGL(k_narrow_wave, dim3((unsigned)(c->num_cus * 4)), block, 0, c->stream, A);
   } else {
-    hipLaunchKernelGGL(k_narrow_gate, grid, block, 0, c->stream, A);
-    hipLaunchKernelGGL(k_narrow_gated, grid, block, 0, c->stream, A);
+    hipLaunchKernelGGL(k_narrow_gate, dim3((unsigned)(c->num_cus * 12)), block, 0, c->stream, A);
+    hipLaunchKernelGGL(k_narrow_gated, dim3((unsigned)(c->num_cus * 6)), block, 0, c->stream, A);
   }
   HIPCHK(c, hipGetLastError());
   return KDOP_OK;
@@ -1084,14 +1091,14 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
         dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
         if (persistent) {
           // levels 0 .. last-1; round 0 takes level `last` when no launch stopped earlier
-          for (uint32_t l = 0; l < last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last);
+          for (uint32_t l = 0; l < last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last, false);
           if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
           for (uint32_t r = 0; r < rounds_async; ++r) dispatch_K<PersistRound>(c->K, c, T, cap, r, last);
         } else {
           // first the levels the deeper tree suggests; more below if the
           // frontier is not empty by then
           if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-          for (uint32_t l = 0; l <= levels_queued; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
+          for (uint32_t l = 0; l <= levels_queued; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels, true);
         }
       } else {
         if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
@@ -1156,7 +1163,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     if (!persistent && npairs && !grow_front && !c->hsum->final_set) {
       // the level-synchronous frontier outlived the first batch of launches
-      for (uint32_t l = levels_queued + 1; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels);
+      for (uint32_t l = levels_queued + 1; l <= levels; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, levels, true);
       HIPCHK(c, hipGetLastError());
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
       if (narrow && (rc = launch_narrow(c, T, -1, true, one_pass))) return rc;

@@ -935,15 +935,65 @@ __device__ inline float wave_max_f(float v) {
 }
 __device__ inline float readlane_f(float v, uint32_t l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
 
+// Narrow-phase output staged per block in LDS.  A reservation on one global
+// counter per block iteration (tens of thousands per call for many surface
+// pairs) serializes across the XCDs and sets the kernel's time; staged, a
+// block reserves once per kStageC - kIterC contacts and once at its end.
+constexpr uint32_t kStageC = 512, kStageR = 64;
+constexpr uint32_t kIterC = 4u * (uint32_t)kMaxCand;  // most contacts one block iteration (4 waves) adds
+struct NarrowStage {
+  kdop_contact c[kStageC];
+  NarrowRec r[kStageR];
+  uint32_t nc, nr, base_c, base_r;
+};
+
+// Copy n staged records to out[base ..), as 32-bit words, dropping those at
+// or past cap.
+template <class T>
+__device__ inline void stage_copy(const T* st, uint32_t n, T* __restrict__ out, uint32_t base, uint32_t cap) {
+  static_assert(sizeof(T) % 4 == 0, "records of whole words");
+  constexpr uint32_t W = sizeof(T) / 4;
+  const uint32_t m = base < cap ? min(n, cap - base) : 0u;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(out) + (size_t)base * W;
+  for (uint32_t w = threadIdx.x; w < m * W; w += blockDim.x) dst[w] = src[w];
+}
+
+// Flush the staged contacts / records to global memory when forced (the
+// block's end) or when the next iteration might not fit.  Every thread of
+// the block calls it.
+__device__ inline void narrow_stage_flush(const NarrowArgs& A, NarrowStage& S, bool force) {
+  __syncthreads();
+  const uint32_t nc = S.nc, nr = S.nr;
+  const bool fc = nc != 0u && (force || nc > kStageC - kIterC);
+  const bool fr = nr != 0u && (force || nr > kStageR - 4u);
+  if (!fc && !fr) return;
+  if (threadIdx.x == 0) {
+    S.base_c = fc ? atomicAdd(A.c_fill, nc) : 0u;
+    S.base_r = fr ? atomicAdd(A.r_fill, nr) : 0u;
+    if (fc && S.base_c + nc > A.c_cap) atomicOr(A.overflow, 8u);
+  }
+  __syncthreads();
+  if (fc) stage_copy(S.c, nc, A.contacts, S.base_c, A.c_cap);
+  if (fr) stage_copy(S.r, nr, A.recs, S.base_r, A.r_cap);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (fc) S.nc = 0u;
+    if (fr) S.nr = 0u;
+  }
+  __syncthreads();
+}
+
 // Contact points of one gated surface pair, one wave (contacts/geometry_
 // contacts_tetrahedron_tetrahedron.h:543-681, as :1312-1345 calls it): lanes
 // take the 56 contact candidates, each computed with the serial operation
 // order; the serial loops' choices (first minimum, first maximum, a candidate
 // dropped when within working precision of ANY earlier candidate) are made
 // with ballots over lanes in that order, so the results are those of the
-// serial code.  gate_word: P2 index | face << 30.  Every thread of the block
-// calls it (block-level output reservation); live is wave-uniform.
-__device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, uint32_t gate_word, bool live, uint32_t* red,
+// serial code.  gate_word: P2 index | face << 30; live is wave-uniform.  The
+// contacts and the pair's record go to the block's stage (room for them is
+// left by narrow_stage_flush between iterations).
+__device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, uint32_t gate_word, bool live, NarrowStage& S,
                                             unsigned long long& hits) {
   const uint32_t lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -1022,29 +1072,27 @@ __device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, u
     nu = (uint32_t)__popcll(km);
     ordinal = (uint32_t)__popcll(km & below);
   }
-  const uint32_t oc = block_reserve(A.c_fill, keep ? 1u : 0u, red);
-  const uint32_t orr = block_reserve(A.r_fill, (lane == 0u && nu) ? 1u : 0u, red);
+  uint32_t cb = 0, rb = 0;
+  if (lane == 0u && nu) {
+    cb = atomicAdd(&S.nc, nu);  // LDS atomics: this wave's places in the stage
+    rb = atomicAdd(&S.nr, 1u);
+  }
+  cb = __shfl(cb, 0, 64);
   if (keep) {
-    if (oc >= A.c_cap) {
-      atomicOr(A.overflow, 8u);
-    } else {
-      kdop_contact ct;
-      ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = ordinal;
-      ct.p[0] = q.x; ct.p[1] = q.y; ct.p[2] = q.z;
-      ct.n[0] = nrm.x; ct.n[1] = nrm.y; ct.n[2] = nrm.z;
-      ct.depth = depth;
-      A.contacts[oc] = ct;
-    }
+    kdop_contact ct;
+    ct.pair = lp.pair; ct.tet_a = lp.ta; ct.tet_b = lp.tb; ct.ordinal = ordinal;
+    ct.p[0] = q.x; ct.p[1] = q.y; ct.p[2] = q.z;
+    ct.n[0] = nrm.x; ct.n[1] = nrm.y; ct.n[2] = nrm.z;
+    ct.depth = depth;
+    S.c[cb + ordinal] = ct;
   }
   if (lane == 0u && nu) {
     ++hits;
-    if (orr < A.r_cap) {
-      NarrowRec rec;
-      rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
-      rec.face = (uint8_t)face; rec.nuniq = (uint8_t)nu; rec.cmin = (uint8_t)cmin; rec.cmax = (uint8_t)cmax;
-      rec.depth = depth; rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
-      A.recs[orr] = rec;
-    }
+    NarrowRec rec;
+    rec.pair = lp.pair; rec.ta = lp.ta; rec.tb = lp.tb;
+    rec.face = (uint8_t)face; rec.nuniq = (uint8_t)nu; rec.cmin = (uint8_t)cmin; rec.cmax = (uint8_t)cmax;
+    rec.depth = depth; rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
+    S.r[rb] = rec;
   }
   wave_lds_sync();  // this wave's LDS is rewritten by its next pair
 }
@@ -1054,7 +1102,7 @@ __device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, u
 // ballots), then narrow_contacts_wave on the same wave.
 __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
   __shared__ NarrowLds lds[4];
-  __shared__ uint32_t red[5];
+  __shared__ NarrowStage S;
   if (A.pending) {
     const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
     if (__any(v != 0u)) return;
@@ -1064,6 +1112,8 @@ __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
   NarrowLds& L = lds[w];
   unsigned long long hits = 0;
+  if (threadIdx.x == 0) S.nc = S.nr = 0u;
+  __syncthreads();
   for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
     const uint32_t i = base + w;
     bool live = i < n;  // wave-uniform
@@ -1107,18 +1157,24 @@ __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
     const unsigned long long pick = __ballot(ok && test == best);
     const uint32_t face = pick ? (uint32_t)(__ffsll(pick) - 1) >> 2 : 0u;
     wave_lds_sync();
-    narrow_contacts_wave(A, L, i | (face << 30), live && pick != 0ull, red, hits);
+    narrow_contacts_wave(A, L, i | (face << 30), live && pick != 0ull, S, hits);
+    narrow_stage_flush(A, S, false);
   }
+  narrow_stage_flush(A, S, true);
   wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
 }
 
 // Narrow phase for many surface pairs, pass 1: the SAT gate and face pick
 // with 8 lanes per pair (gate_group, 32 pairs per block at a time); the
 // survivors go to the gate list (P2 index | face << 30).
+// Survivors are staged in LDS and written with one reservation per
+// kGateStage - 32 of them (and one at the block's end).
+constexpr uint32_t kGateStage = 1024;
 __global__ __launch_bounds__(256) void k_narrow_gate(NarrowArgs A) {
   __shared__ f3 sv[32][8];
   __shared__ UPlane su[32][8];
-  __shared__ uint32_t red[5];
+  __shared__ uint32_t gst[kGateStage];
+  __shared__ uint32_t gn, gbase;
   if (A.pending) {
     const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
     if (__any(v != 0u)) return;
@@ -1126,12 +1182,27 @@ __global__ __launch_bounds__(256) void k_narrow_gate(NarrowArgs A) {
   if (A.done && *A.done == 0u) return;
   const uint32_t n = min(*A.p2_fill, A.p2_cap);
   const uint32_t lane = lane_id(), grp = threadIdx.x >> 3;
-  for (uint32_t base = blockIdx.x * 32u; base < n; base += gridDim.x * 32u) {
-    const uint32_t i = base + grp;
-    const int face = gate_group(A, sv[grp], su[grp], i, i < n);
-    const bool emit = (lane & 7u) == 0u && face >= 0;
-    const uint32_t o = block_reserve(A.g_fill, emit ? 1u : 0u, red);
-    if (emit) A.gate[o] = i | ((uint32_t)face << 30);  // o < n <= P2 capacity = gate capacity
+  if (threadIdx.x == 0) gn = 0u;
+  __syncthreads();
+  for (uint32_t base = blockIdx.x * 32u;; base += gridDim.x * 32u) {
+    const bool more = base < n;  // block-uniform
+    if (more) {
+      const uint32_t i = base + grp;
+      const int face = gate_group(A, sv[grp], su[grp], i, i < n);
+      if ((lane & 7u) == 0u && face >= 0) gst[atomicAdd(&gn, 1u)] = i | ((uint32_t)face << 30);  // <= 32 per iteration
+    }
+    __syncthreads();
+    const uint32_t m = gn;
+    if (m != 0u && (!more || m > kGateStage - 32u)) {
+      if (threadIdx.x == 0) gbase = atomicAdd(A.g_fill, m);
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < m; k += blockDim.x)
+        if (gbase + k < A.p2_cap) A.gate[gbase + k] = gst[k];  // the gate list holds at most n <= P2 capacity words
+      __syncthreads();
+      if (threadIdx.x == 0) gn = 0u;
+      __syncthreads();
+    }
+    if (!more) break;
   }
 }
 
@@ -1139,14 +1210,18 @@ __global__ __launch_bounds__(256) void k_narrow_gate(NarrowArgs A) {
 // pairs, one wave each.
 __global__ __launch_bounds__(256) void k_narrow_gated(NarrowArgs A) {
   __shared__ NarrowLds lds[4];
-  __shared__ uint32_t red[5];
+  __shared__ NarrowStage S;
   const uint32_t n = *A.g_fill;
   const uint32_t w = threadIdx.x >> 6;
   unsigned long long hits = 0;
+  if (threadIdx.x == 0) S.nc = S.nr = 0u;
+  __syncthreads();
   for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
     const uint32_t k = base + w;
-    narrow_contacts_wave(A, lds[w], k < n ? A.gate[k] : 0u, k < n, red, hits);
+    narrow_contacts_wave(A, lds[w], k < n ? A.gate[k] : 0u, k < n, S, hits);
+    narrow_stage_flush(A, S, false);
   }
+  narrow_stage_flush(A, S, true);
   wave_add64(&stripe_of(A.stripes, wave_global())->narrow_pairs, hits);
 }
 

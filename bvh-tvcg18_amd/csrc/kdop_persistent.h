@@ -54,6 +54,7 @@ struct CallState {
   alignas(256) Fills fills;
   alignas(256) uint32_t level_fill[64];     // items per BFS level (level 0 = seeds)
   uint32_t final_level, final_count, final_set;  // level whose buffer holds the frontier, its size
+  SegCtr lseg[3][kSegs];  // segmented levels: per-segment fills, level l in set l % 3
 };
 
 // One breadth-first level of the BVTT over the whole grid: expand every item
@@ -63,13 +64,44 @@ struct CallState {
 // launches of the same call return at once.  The host queues `last` + 1 of
 // them without reading any size back.  With target = ~0u and last = BVTT
 // depth this is the whole level-synchronous traversal.
+//
+// A level is one contiguous run (nseg 1: its size in level_fill[level]) or
+// kSegs segments of cap / kSegs slots, each filled through a counter of its
+// own (lseg[level % 3][s]; a block writes segment blockIdx % kSegs).  One
+// counter for a whole wide level is hit once per block iteration from every
+// XCD, and those atomics serialize: at tens of thousands per level they, not
+// the tests, set the level's time.  Item i of a segmented level is found from
+// the prefix sums of the segment fills.  Level l clears set (l + 2) % 3, the
+// previous level's input, for level l + 1 to write.
 template <int K>
 __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                                   CallState* __restrict__ C, uint32_t level, uint32_t target, uint32_t last) {
+                                                   CallState* __restrict__ C, uint32_t level, uint32_t target, uint32_t last,
+                                                   uint32_t nseg_in, uint32_t nseg_out) {
   __shared__ uint32_t red[5];
+  __shared__ uint32_t pre[kSegs + 1];
+  __shared__ uint32_t raw_total;
   if (C->final_set) return;
-  const uint32_t fill = C->level_fill[level];
-  const uint32_t n = min(fill, cap);
+  const uint32_t segcap_in = cap / nseg_in;
+  uint32_t n;
+  if (nseg_in == 1u) {
+    n = min(C->level_fill[level], cap);
+  } else {
+    if (threadIdx.x < 64u) {  // wave 0: prefix sums of the segment fills
+      const uint32_t lane = threadIdx.x;
+      const uint32_t f = C->lseg[level % 3u][lane].fill;
+      uint32_t x = min(f, segcap_in), t = f;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+        t += __shfl_xor(t, d, 64);
+      }
+      pre[lane + 1u] = x;
+      if (lane == 0u) { pre[0] = 0u; raw_total = t; }
+    }
+    __syncthreads();
+    n = pre[kSegs];
+  }
   if (n == 0 || n >= target || level >= last) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       C->final_level = level;
@@ -78,8 +110,15 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     }
     return;
   }
+  if (blockIdx.x == 0) {
+    if (nseg_in > 1u && threadIdx.x == 0) C->level_fill[level] = raw_total;  // the level's size (all that was reserved)
+    if (nseg_out > 1u && threadIdx.x < (uint32_t)kSegs) C->lseg[(level + 2u) % 3u][threadIdx.x].fill = 0u;
+  }
   const Item* __restrict__ in = (level & 1) ? buf1 : buf0;
   Item* __restrict__ out = (level & 1) ? buf0 : buf1;
+  const uint32_t so = blockIdx.x % nseg_out, segcap_out = cap / nseg_out;
+  uint32_t* out_fill = nseg_out == 1u ? &C->level_fill[level + 1] : &C->lseg[(level + 1u) % 3u][so].fill;
+  Item* __restrict__ out_seg = out + (size_t)so * segcap_out;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;
@@ -88,12 +127,21 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     x.tests = x.descent = x.broot = 0;
     Item it{0, 0, 0, 0, 0};
     if (i < n) {
-      it = in[i];
+      uint32_t idx = i;
+      if (nseg_in > 1u) {  // segment s with pre[s] <= i < pre[s + 1]
+        uint32_t lo = 0, hi = kSegs;
+        while (hi - lo > 1u) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (pre[mid] <= i) lo = mid; else hi = mid;
+        }
+        idx = lo * segcap_in + (i - pre[lo]);
+      }
+      it = in[idx];
       expand_item<K>(T.nodes, it, x);
     }
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
-    const uint32_t o = block_reserve(&C->level_fill[level + 1], __popc(x.push), red);
-    emit_items(x, out, o, cap, T.overflow);
+    const uint32_t o = block_reserve(out_fill, __popc(x.push), red);
+    emit_items(x, out_seg, o, segcap_out, T.overflow);
     emit_leaves(T, it.pair, x);
   }
   CounterStripe* st = stripe_of(T.stripes, wave_global());
@@ -417,6 +465,7 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
     sc.next = 0u;
   }
   if (tid < 64u) C->level_fill[tid] = 0u;
+  if (tid < 3u * (uint32_t)kSegs) C->lseg[tid / kSegs][tid % kSegs].fill = 0u;
   if (tid < (uint32_t)kMaxRounds) C->done[tid] = 0ull;
   if (tid == 0) {
     C->overflow = 0u;
