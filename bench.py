@@ -190,6 +190,9 @@ def batched_leg(args):
         kept = ctx.reduce_contacts()
         tt += time.perf_counter() - t0
     reduce_ms = tt / reps * 1e3
+    # the reduction's input as PROX would hold it, for the CPU baseline
+    bp = np.asarray(pairs, np.uint32)
+    prox_in = dict(bi=bp[con["pair"], 0], bj=bp[con["pair"], 1], p=con["p"].copy(), n=con["n"].copy(), d=con["depth"].copy())
     return dict(workload=wl.name, pairs=int(n), K=args.batched_k, tets_total=int(sum(wl.info["tets"])),
                 separated_pairs=wl.info["separated"],
                 pair_queries_per_s=n / t_batched, unbatched_pair_queries_per_s=n / t_single,
@@ -197,7 +200,8 @@ def batched_leg(args):
                 tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms,
                 contact_sort={"contacts": int(con.shape[0]), "device_ms": sort_ms,
                               "host_numpy_lexsort_opencl_order_ms": t_host * 1e3},
-                contact_reduction={"contacts": int(con.shape[0]), "kept": int(kept), "device_ms": reduce_ms})
+                contact_reduction={"contacts": int(con.shape[0]), "kept": int(kept), "device_ms": reduce_ms},
+                _prox_in=prox_in)
 
 
 def ray_set(n, seed=5):
@@ -273,10 +277,11 @@ def reorder_leg(args):
     return dict(tets=int(T.shape[0]), verts=int(X.shape[0]), device_ms=t_dev * 1e3, host_ms=t_host * 1e3, identical=same)
 
 
-def cpu_leg(args, head):
+def cpu_leg(args, head, batched=None):
     """Oracle (a restatement of the reference's sequential CPU traversal) on a
     bounded sample: the first R branch rows of body A against all of body B,
-    one thread; rate scaled to the full query."""
+    one thread; rate scaled to the full query.  Also the oracle's raycast,
+    single traversal and PROX contact reduction on the inputs of those legs."""
     import numpy as np
     from oracle import oracle as O
     wl = head["wl"]
@@ -331,13 +336,23 @@ def cpu_leg(args, head):
         ss = _t.perf_counter() - t0
         sph = dict(value=nr / ss, unit="spheres/s", cores=1, kind="port",
                    sample=f"oracle kdop::single_traversal, first {nr} spheres of the set ({ss:.2f} s)")
+    # PROX contact reduction (std::sort + std::unique) of the batched call's contacts, one thread
+    red = None
+    if batched and batched.get("_prox_in") is not None:
+        import time as _t
+        pi = batched["_prox_in"]
+        t0 = _t.perf_counter()
+        kept = O.prox_reduce(pi["bi"], pi["bj"], pi["p"], pi["n"], pi["d"])
+        sr = _t.perf_counter() - t0
+        red = dict(ms=sr * 1e3, kept=int(kept.shape[0]), cores=1, kind="port",
+                   sample=f"oracle std::stable_sort + std::unique of the batched call's {pi['d'].shape[0]} contacts")
     # the same query on every host core this job may use (rows taken one at a time), whole query
     cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
     cm, sm = O.traverse_rows_mt(trees[0], meshA, trees[1], meshB, 0, rows_total, cores)
     allc = dict(value=cm["tests"] / sm, unit="node-pair tests/s", cores=cores, kind="port",
                 sample=f"oracle tandem traversal + narrow phase, all {rows_total} branch rows of body A on {cores} threads "
                        f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm)
-    return one, allc, ray, sph
+    return one, allc, ray, sph, red
 
 
 def main():
@@ -408,7 +423,12 @@ def main():
     cpu = cpu_all = None
     if not args.no_cpu:
         log("[bench] cpu baseline sample")
-        cpu, cpu_all, cpu_ray, cpu_sph = cpu_leg(args, head)
+        cpu, cpu_all, cpu_ray, cpu_sph, cpu_red = cpu_leg(args, head, batched)
+        if cpu_red and batched:
+            cr = batched["contact_reduction"]
+            cr["cpu_baseline"] = cpu_red
+            cr["vs_cpu"] = cpu_red["ms"] / cr["device_ms"]
+            cr["identical_count"] = cpu_red["kept"] == cr["kept"]
         if cpu_sph and head.get("single"):
             head["single"]["cpu_baseline"] = cpu_sph
             head["single"]["vs_cpu"] = head["single"]["spheres_per_s"] / cpu_sph["value"]
@@ -417,6 +437,8 @@ def main():
             head["raycast"]["vs_cpu"] = head["raycast"]["rays_per_s"] / cpu_ray["value"]
         log(f"[bench] cpu {cpu['value']:.3e} tests/s, est. full query {cpu['est_full_query_s']:.1f} s; "
             f"{cpu_all['cores']} cores: {cpu_all['full_query_s']:.2f} s")
+    if batched:
+        batched.pop("_prox_in", None)
     steps_per_s = 1e3 / head["ms_per_step"]
     kernel = f"k_persist<{args.k},{head['sched']['queue_depth']}>"
     traffic = measured_traffic(kernel)

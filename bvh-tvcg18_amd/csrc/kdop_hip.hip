@@ -75,7 +75,9 @@ struct kdop_ctx {
   DevBuf<unsigned char> raybuf;        // kdop_raycast / kdop_single_traversal: inputs and counts
   DevBuf<kdop_sphere_contact> sphere_out;  // kdop_single_traversal: the contacts
   DevBuf<uint2> sg_items[2];               // kdop_single_traversal: (sphere, node) frontiers
-  DevBuf<uint32_t> sg_scan;                // kdop_single_traversal: counts, offsets, block sums, total, flag
+  DevBuf<uint32_t> sg_scan;                // kdop_single_traversal / kdop_raycast: counts, offsets, block sums, total, flag
+  DevBuf<float> rc_best;                   // kdop_raycast: each (ray, chunk) item's best length from FLT_MAX
+  DevBuf<uint32_t> rc_first;               // kdop_raycast: each ray's first item, then a flag
   uint32_t sg_max_items = 1u << 28;        // KDOP_OPT_SINGLE_ITEMS
   DevBuf<uint32_t> redbuf;                 // contact reduction: keep flags, offsets, scan sums, total
   DevBuf<kdop_prox_contact> prox_io;       // kdop_reduce_prox_contacts: staged input and output
@@ -276,10 +278,38 @@ struct RefitSuper {
   }
 };
 template <int K>
-struct Raycast {
-  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* o, const float* d, float* hits, float* lengths, uint8_t* flags) {
-    hipLaunchKernelGGL(k_raycast<K>, dim3((n + 63) / 64), dim3(64), 0, c->stream, c->dbodies.p, body, c->dtops.p, c->nodes.p, c->dt, n, o, d,
-                       hits, lengths, flags);
+struct RcSeedCount {
+  static void run(kdop_ctx* c, uint32_t body, uint32_t n, const float* o, const float* d, uint32_t* count) {
+    hipLaunchKernelGGL(k_rc_seed_count<K>, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, c->nodes.p, c->dt, n, o, d,
+                       count);
+  }
+};
+template <int K>
+struct RcExpandCount {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* o, const float* d, uint32_t* count) {
+    hipLaunchKernelGGL(k_rc_expand_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, o, d, count);
+  }
+};
+template <int K>
+struct RcRoots {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* o, const float* d, uint32_t* pass, float* best, uint32_t* bad) {
+    hipLaunchKernelGGL(k_rc_roots<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, o, d, pass, best, bad);
+  }
+};
+template <int K>
+struct RcChunks {
+  static void run(kdop_ctx* c, uint32_t body, const uint2* items, const uint32_t* list, uint32_t np, const float* o, const float* d,
+                  float* best) {
+    hipLaunchKernelGGL(k_rc_chunks<K>, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, items, list, np, c->nodes.p,
+                       c->dt, o, d, best);
+  }
+};
+template <int K>
+struct RcResolve {
+  static void run(kdop_ctx* c, uint32_t body, const uint2* items, uint32_t m, const float* best, const uint32_t* first, uint32_t n,
+                  const float* o, const float* d, uint32_t ray_base, float* hits, float* lengths, uint8_t* flags) {
+    hipLaunchKernelGGL(k_rc_resolve<K>, dim3((n + 3) / 4), dim3(256), 0, c->stream, c->dbodies.p, body, items, m, best, first, n,
+                       c->nodes.p, c->dt, o, d, ray_base, hits, lengths, flags);
   }
 };
 template <int K>
@@ -531,6 +561,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
   release(c->redbuf); release(c->prox_io); release(c->sg_items[0]); release(c->sg_items[1]); release(c->sg_scan);
+  release(c->rc_best); release(c->rc_first);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -1543,7 +1574,8 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   const BodyHost& b = c->bodies[body];
   uint32_t lg = 0;
   while ((1u << lg) < b.L) ++lg;
-  if (b.H * std::max(lg, 1u) + 2 > kRayStack) return fail(c, KDOP_ERR_ARG, "kdop_raycast: tree too deep for the ray stack");
+  if (lg + 1u > kChunkStack) return fail(c, KDOP_ERR_ARG, "kdop_raycast: chunk trees too deep (%u leaves)", b.L);
+  const uint32_t steps = (b.H - 1u) * std::max(lg, 1u);  // below the top-level roots, every item is a branch-chunk root after these
   HIPCHK(c, hipSetDevice(c->cfg.device));
   const size_t bytes_in = 6 * (size_t)n * sizeof(float), bytes_out = 4 * (size_t)n * sizeof(float);
   int rc = ensure(c, c->raybuf, bytes_in + bytes_out + n);
@@ -1555,8 +1587,97 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   uint8_t* d_f = reinterpret_cast<uint8_t*>(d_l + n);
   HIPCHK(c, hipMemcpyAsync(d_o, origins, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(d_d, directions, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  dispatch_K<Raycast>(c->K, c, body, n, (const float*)d_o, (const float*)d_d, d_h, d_l, d_f);
-  HIPCHK(c, hipGetLastError());
+  // scan scratch for m values: [count m][offset m][block sums][total][bad]
+  auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
+  auto read_words = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
+    const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
+    HIPCHK(c, hipMemcpyAsync(host, c->sg_scan.p + 2 * (size_t)m + nb, 4 * (size_t)words, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return KDOP_OK;
+  };
+  auto scan = [&](uint32_t m) {
+    const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
+    uint32_t* cnt = c->sg_scan.p;
+    return device_scan(c, cnt, cnt + m, m, cnt + 2 * (size_t)m, cnt + 2 * (size_t)m + nb);
+  };
+  // rays go in batches whose (ray, node) frontier stays within the item limit
+  const uint32_t max_items = c->sg_max_items;
+  uint32_t s0 = 0, batch = n;
+  while (s0 < n) {
+    const uint32_t sn = std::min(batch, n - s0);
+    const float* bo = d_o + 3 * (size_t)s0;
+    const float* bd = d_d + 3 * (size_t)s0;
+    if ((rc = scratch(sn))) return rc;
+    dispatch_K<RcSeedCount>(c->K, c, body, sn, bo, bd, c->sg_scan.p);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = scan(sn))) return rc;
+    uint32_t m = 0;
+    if ((rc = read_words(sn, &m, 1))) return rc;
+    bool too_big = m > max_items;
+    if (m && !too_big) {
+      if ((rc = ensure(c, c->sg_items[0], m))) return rc;
+      hipLaunchKernelGGL(k_sg_seed_write, dim3((sn + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body, c->dtops.p, sn,
+                         (const uint32_t*)c->sg_scan.p, (const uint32_t*)(c->sg_scan.p + sn), c->sg_items[0].p);
+      HIPCHK(c, hipGetLastError());
+    }
+    int cur = 0;
+    for (uint32_t step = 0; !too_big && m && step < steps; ++step) {
+      if ((rc = scratch(m))) return rc;
+      dispatch_K<RcExpandCount>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bo, bd, c->sg_scan.p);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = scan(m))) return rc;
+      uint32_t m2 = 0;
+      if ((rc = read_words(m, &m2, 1))) return rc;
+      if (m2 > max_items) { too_big = true; break; }
+      if (m2) {
+        if ((rc = ensure(c, c->sg_items[cur ^ 1], m2))) return rc;
+        dispatch_K<SgExpandWrite>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint32_t*)c->sg_scan.p,
+                                  (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
+        HIPCHK(c, hipGetLastError());
+      }
+      cur ^= 1;
+      m = m2;
+    }
+    if (too_big) {
+      if (sn == 1) return fail(c, KDOP_ERR_CAPACITY, "kdop_raycast: ray %u meets more than %u nodes of one level", s0, max_items);
+      batch = std::max(1u, sn / 2);
+      continue;
+    }
+    // per item: its chunk's best length; per ray: its first item; the bad
+    // flag; the passing roots' item list (the other frontier buffer)
+    if ((rc = scratch(m)) || (rc = ensure(c, c->rc_best, std::max<size_t>(m, 1))) || (rc = ensure(c, c->rc_first, (size_t)sn + 1)))
+      return rc;
+    float* best = c->rc_best.p;
+    uint32_t* first = c->rc_first.p;
+    uint32_t* bad = first + sn;
+    HIPCHK(c, hipMemsetAsync(first, 0xFF, 4 * (size_t)sn, c->stream));
+    HIPCHK(c, hipMemsetAsync(bad, 0, 4, c->stream));
+    if (m) {
+      const uint2* items = (const uint2*)c->sg_items[cur].p;
+      dispatch_K<RcRoots>(c->K, c, items, m, bo, bd, c->sg_scan.p, best, bad);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = scan(m))) return rc;
+      uint32_t np = 0;
+      if ((rc = read_words(m, &np, 1))) return rc;
+      if (np) {
+        if ((rc = ensure(c, c->sg_items[cur ^ 1], (np + 1) / 2))) return rc;
+        uint32_t* list = reinterpret_cast<uint32_t*>(c->sg_items[cur ^ 1].p);
+        hipLaunchKernelGGL(k_rc_gather, dim3((m + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)c->sg_scan.p,
+                           (const uint32_t*)(c->sg_scan.p + m), m, list);
+        dispatch_K<RcChunks>(c->K, c, body, items, (const uint32_t*)list, np, bo, bd, best);
+      }
+      hipLaunchKernelGGL(k_rc_first, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, first);
+      HIPCHK(c, hipGetLastError());
+    }
+    dispatch_K<RcResolve>(c->K, c, body, (const uint2*)c->sg_items[cur].p, m, (const float*)best, (const uint32_t*)first, sn, bo, bd, s0,
+                          d_h, d_l, d_f);
+    HIPCHK(c, hipGetLastError());
+    uint32_t bad_h = 0;
+    HIPCHK(c, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad_h) return fail(c, KDOP_ERR_INTERNAL, "kdop_raycast: a node other than a branch-chunk root after %u steps", steps);
+    s0 += sn;
+  }
   HIPCHK(c, hipMemcpyAsync(hit_points, d_h, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(lengths, d_l, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(hits, d_f, n, hipMemcpyDeviceToHost, c->stream));

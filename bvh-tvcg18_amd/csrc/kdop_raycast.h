@@ -1,20 +1,20 @@
 // kdop_raycast.h -- kdop::raycast (SIMULATION/KDOP/KDOP/include/kdop_raycast.h:19-155)
-// for a batch of rays against one body, one thread per ray.
+// for a batch of rays against one body.
 //
 // Below (and at) the branch-chunk roots every node gets the reference's own
 // test, in the reference's depth-first order, with the reference's running
 // best length, so hits, lengths and ties come out as the sequential code
 // gives them.  The reference walks all C branch roots of the tree per ray;
-// here the super levels above them cull ranges of branch chunks first, with
-// a test that can only drop chunks whose root the reference's test would
-// reject or prune (see k_raycast), so the results do not change.
+// here the super levels above them cull chunks first, with a test that can
+// only drop chunks whose root the reference's test would reject, and the
+// chunks are walked in parallel before each ray's running length is applied
+// (see "rays" below for why that keeps the result).
 #pragma once
 
 #include "kdop_kernels.h"
+#include "kdop_single.h"
 
 namespace kdop_amd {
-
-constexpr uint32_t kRayStack = 64;  // per-thread DFS stack (the host checks H log2 L + 2 <= it)
 
 // Entry and exit parameters of the ray through the slabs of a DOP
 // (raycast/geometry_raycast_dop.h:35-75; a slab the ray runs parallel to is
@@ -112,78 +112,216 @@ __device__ inline bool raycast_tet(const f3& o, const f3& r, const f3* P, uint32
   return status;
 }
 
-// One thread per ray (make_ray normalises the direction, types/geometry_ray.h:39-42).
-// The root test is the reference's (kdop_raycast.h:126-127).  Then the tree is
-// walked depth first from the top-level chunk roots in order; a super node's
-// children are the lower chunks' roots in order, so branch chunks come in
-// increasing order, as kdop_raycast.h:129-149 visits them.  A super node is
-// skipped when the ray's slab interval through it is empty (t_min > t_max),
-// behind the origin (t_max <= 0) or starts at or beyond the best length so
-// far: a DOP inside it gives, slab by slab, entry parameters no smaller and
-// exit parameters no larger (the subtractions and divisions are correctly
-// rounded, hence monotone), so each of its branch roots would fail the
-// reference's test or be pruned by `t >= length` (:55-57) without changing
-// the hit.  At branch nodes the reference's test and recursion run unchanged.
+// ---------------------------------------------------------------- rays
+// kdop::raycast (kdop_raycast.h:102-155): the root test, then every branch
+// chunk in order, each walked depth first from its root (:19-100) with the
+// running best length, which prunes nodes whose DOP entry is not below it.
+// Here the walk is cut at the chunks without changing its result:
+//   A. the branch-chunk roots each ray may reach, in chunk order, breadth
+//      first over (ray, node) items of all rays; a super node is dropped when
+//      the ray's slab interval through it is empty or behind the origin (a
+//      DOP inside it then gives entry parameters no smaller and exit
+//      parameters no larger, slab by slab, since correctly rounded
+//      subtraction and division are monotone, so every branch root below it
+//      fails the reference's DOP test);
+//   B. every (ray, chunk) walked on its own from length FLT_MAX, giving the
+//      chunk's best length h;
+//   C. per ray, its chunks in order with the running length L: a chunk with
+//      h >= L changes nothing, so only chunks with h < L are walked again,
+//      from L, exactly as the reference walks them.
+// Why C may skip a chunk with h >= L: walked from L, every node's prune test
+// compares with a length that is at most L <= h, while the walk from
+// FLT_MAX never went below h; so the walk from L prunes at least wherever the
+// walk from FLT_MAX did, reaches a subset of its leaves, and finds no hit
+// below h >= L.
+
+constexpr uint32_t kChunkStack = 8;  // a chunk tree is at most log2 32 = 5 deep
+
+// The reference's walk below one branch-chunk root (kdop_raycast.h:46-100)
+// from `length`: returns the best length, hit updated with it.
 template <int K>
-__global__ __launch_bounds__(64) void k_raycast(const BodyDev* __restrict__ bodies, uint32_t body, const uint32_t* __restrict__ tops,
-                                                const float* __restrict__ nodes, DirTable dt, uint32_t n, const float* __restrict__ orig,
-                                                const float* __restrict__ dir, float* __restrict__ hits, float* __restrict__ lengths,
-                                                uint8_t* __restrict__ flags) {
-  __shared__ uint32_t stk[64][kRayStack];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t* st = stk[threadIdx.x];
-  const BodyDev& B = bodies[body];
-  const f3 o =f3{orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
-  const f3 r = unit3(f3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]});
-  const float threshold = 0.01f;
-  f3 hit = f3{0.0f, 0.0f, 0.0f};
-  float length = FLT_MAX;
-  float t;
+__device__ inline float chunk_raycast(const float* __restrict__ nodes, const BodyDev& B, uint32_t root, const f3& o, const f3& r,
+                                      const DirTable& dt, float length, f3& hit, uint32_t* st) {
+  uint32_t sp = 0;
+  st[sp++] = root;
   NodeRec<K> rec;
-  load_rec<K>(nodes, B.node_base, rec);
-  if (raycast_dop<K>(o, r, rec.v, dt, threshold, t)) {
-    for (uint32_t top = 0; top < B.ntop; ++top) {
-      uint32_t sp = 0;
-      st[sp++] = tops[B.top_base + top];
-      while (sp) {
-        const uint32_t s = st[--sp];
-        load_rec<K>(nodes, s, rec);
-        const uint32_t c0 = rec.m0 & IDX_MASK, c1 = rec.m1 & IDX_MASK;
-        if (!(rec.m0 & F_BRANCH)) {  // super node: cull only
-          float t_min, t_max;
-          ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
-          if (t_min > t_max || t_max <= 0.0f || t_min >= length) continue;
-          st[sp++] = c1;
-          st[sp++] = c0;
-          continue;
-        }
-        if (!raycast_dop<K>(o, r, rec.v, dt, threshold, t)) continue;  // kdop_raycast.h:52-57
-        if (t >= length) continue;
-        if (rec.m0 & F_LEAF) {  // :61-79
-          const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)c0);
-          const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
-          f3 P[4];
+  float t;
+  while (sp) {
+    const uint32_t s = st[--sp];
+    load_rec<K>(nodes, s, rec);
+    if (!raycast_dop<K>(o, r, rec.v, dt, 0.01f, t)) continue;  // :52-57
+    if (t >= length) continue;
+    const uint32_t c0 = rec.m0 & IDX_MASK, c1 = rec.m1 & IDX_MASK;
+    if (rec.m0 & F_LEAF) {  // :61-79
+      const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)c0);
+      const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
+      f3 P[4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
-          f3 q;
-          float len;
-          if (raycast_tet(o, r, P, B.surf[c0], q, len)) {
-            hit = len < length ? q : hit;
-            length = len < length ? len : length;
-          }
-        } else {
-          st[sp++] = c1;
-          st[sp++] = c0;
-        }
+      for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
+      f3 q;
+      float len;
+      if (raycast_tet(o, r, P, B.surf[c0], q, len)) {
+        hit = len < length ? q : hit;
+        length = len < length ? len : length;
       }
+    } else {  // :81-99, first child first
+      st[sp++] = c1;
+      st[sp++] = c0;
     }
   }
-  hits[3 * i] = hit.x;
-  hits[3 * i + 1] = hit.y;
-  hits[3 * i + 2] = hit.z;
-  lengths[i] = length;
-  flags[i] = length < FLT_MAX ? 1u : 0u;
+  return length;
+}
+
+__device__ inline void load_ray(const float* __restrict__ orig, const float* __restrict__ dir, uint32_t i, f3& o, f3& r) {
+  o = f3{orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
+  r = unit3(f3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]});  // make_ray (types/geometry_ray.h:39-42)
+}
+
+// A: the root test (kdop_raycast.h:126-127), then the top-level chunk roots.
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_seed_count(const BodyDev* __restrict__ bodies, uint32_t body, const float* __restrict__ nodes,
+                                                       DirTable dt, uint32_t n, const float* __restrict__ orig, const float* __restrict__ dir,
+                                                       uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BodyDev& B = bodies[body];
+  f3 o, r;
+  load_ray(orig, dir, i, o, r);
+  NodeRec<K> rec;
+  load_rec<K>(nodes, B.node_base, rec);
+  float t;
+  count[i] = raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) ? B.ntop : 0u;
+}
+
+// A, one step: a super node becomes its two children or nothing; a
+// branch-chunk root (or one already marked) stays, marked.
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_expand_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                         DirTable dt, const float* __restrict__ orig, const float* __restrict__ dir,
+                                                         uint32_t* __restrict__ count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint2 it = items[j];
+  if (it.y & kSgTested) { count[j] = 1u; return; }
+  NodeRec<K> rec;
+  load_rec<K>(nodes, it.y, rec);
+  if (rec.m0 & F_BRANCH) { count[j] = 1u; return; }
+  f3 o, r;
+  load_ray(orig, dir, it.x, o, r);
+  float t_min, t_max;
+  ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
+  count[j] = (t_min > t_max || t_max <= 0.0f) ? 0u : 2u;
+}
+
+// B, first pass: the reference's test of each chunk root (from FLT_MAX);
+// pass[j] = 1 when the root passes, and best[j] = FLT_MAX for every item
+// (*bad set if an item is not a branch-chunk root, which the step count
+// rules out).  Most roots fail, so the walks below run on the passing items
+// only, gathered by a scan, instead of leaving most lanes of a wave idle.
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_roots(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes, DirTable dt,
+                                                  const float* __restrict__ orig, const float* __restrict__ dir, uint32_t* __restrict__ pass,
+                                                  float* __restrict__ best, uint32_t* __restrict__ bad) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint2 it = items[j];
+  const uint32_t root = it.y & ~kSgTested;
+  best[j] = FLT_MAX;
+  NodeRec<K> rec;
+  load_rec<K>(nodes, root, rec);
+  if (!(rec.m0 & F_BRANCH)) {
+    pass[j] = 0u;
+    atomicOr(bad, 1u);
+    return;
+  }
+  f3 o, r;
+  load_ray(orig, dir, it.x, o, r);
+  float t;
+  pass[j] = (raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) && t < FLT_MAX) ? 1u : 0u;  // kdop_raycast.h:52-57
+}
+__global__ __launch_bounds__(256) void k_rc_gather(const uint32_t* __restrict__ pass, const uint32_t* __restrict__ off, uint32_t m,
+                                                   uint32_t* __restrict__ list) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m && pass[j]) list[off[j]] = j;
+}
+// B, second pass: best[j] = the best length of chunk j's walk from FLT_MAX,
+// for the items in list.
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_chunks(const BodyDev* __restrict__ bodies, uint32_t body, const uint2* __restrict__ items,
+                                                   const uint32_t* __restrict__ list, uint32_t np, const float* __restrict__ nodes, DirTable dt,
+                                                   const float* __restrict__ orig, const float* __restrict__ dir, float* __restrict__ best) {
+  __shared__ uint32_t stk[256][kChunkStack];
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const uint32_t j = list[k];
+  const uint2 it = items[j];
+  f3 o, r, hit{0.0f, 0.0f, 0.0f};
+  load_ray(orig, dir, it.x, o, r);
+  best[j] = chunk_raycast<K>(nodes, bodies[body], it.y & ~kSgTested, o, r, dt, FLT_MAX, hit, stk[threadIdx.x]);
+}
+
+// first[ray] = its first item (items are grouped by ray; rays without items
+// keep the ~0u the host filled in).
+__global__ __launch_bounds__(256) void k_rc_first(const uint2* __restrict__ items, uint32_t m, uint32_t* __restrict__ first) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t ray = items[j].x;
+  if (j == 0u || items[j - 1u].x != ray) first[ray] = j;
+}
+
+// C: per ray, its chunks in order from length FLT_MAX (VT::infinity), each
+// with best < length walked again from length.  One wave per ray: the wave
+// reads 64 of the ray's chunk bests at a time and a ballot finds the next
+// chunk in order whose best is below the running length; lane 0 walks it
+// and the length is re-broadcast, so chunks are taken in the sequential
+// order against the length the sequential code would hold.  A miss leaves
+// the point (0, 0, 0), length FLT_MAX and flag 0.  Rays are numbered from
+// ray_base in the outputs (the items number them within their batch).
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_resolve(const BodyDev* __restrict__ bodies, uint32_t body, const uint2* __restrict__ items,
+                                                    uint32_t m, const float* __restrict__ best, const uint32_t* __restrict__ first, uint32_t n,
+                                                    const float* __restrict__ nodes, DirTable dt, const float* __restrict__ orig,
+                                                    const float* __restrict__ dir, uint32_t ray_base, float* __restrict__ hits,
+                                                    float* __restrict__ lengths, uint8_t* __restrict__ flags) {
+  __shared__ uint32_t stk[4][kChunkStack];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 4u + w;  // this wave's ray
+  if (i >= n) return;
+  f3 hit{0.0f, 0.0f, 0.0f};
+  float length = FLT_MAX;
+  const uint32_t j0 = first[i];
+  if (j0 != ~0u) {
+    const BodyDev& B = bodies[body];
+    f3 o, r;
+    load_ray(orig, dir, i, o, r);
+    for (uint32_t base = j0;; base += 64u) {
+      const uint32_t j = base + lane;
+      const bool mine = j < m && items[j].x == i;
+      const float bj = mine ? best[j] : FLT_MAX;
+      const unsigned long long in_ray = __ballot(mine);
+      unsigned long long done = 0ull;  // lanes already taken in this group
+      for (;;) {
+        const unsigned long long cand = __ballot(mine && bj < length) & ~done;
+        if (!cand) break;
+        const uint32_t k = (uint32_t)(__ffsll(cand) - 1);
+        done |= (2ull << k) - 1ull;  // k and every lane before it
+        if (lane == 0u) {
+          const uint32_t root = items[base + k].y & ~kSgTested;
+          length = chunk_raycast<K>(nodes, B, root, o, r, dt, length, hit, stk[w]);
+        }
+        length = readlane_f(length, 0);
+      }
+      if (in_ray != ~0ull) break;  // the ray's items end in this group
+    }
+  }
+  if (lane == 0u) {
+    const uint32_t g = ray_base + i;
+    hits[3 * g] = hit.x;
+    hits[3 * g + 1] = hit.y;
+    hits[3 * g + 2] = hit.z;
+    lengths[g] = length;
+    flags[g] = length < FLT_MAX ? 1u : 0u;
+  }
 }
 
 }  // namespace kdop_amd
