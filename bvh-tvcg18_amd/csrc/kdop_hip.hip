@@ -1662,7 +1662,7 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
       if (np) {
         if ((rc = ensure(c, c->sg_items[cur ^ 1], (np + 1) / 2))) return rc;
         uint32_t* list = reinterpret_cast<uint32_t*>(c->sg_items[cur ^ 1].p);
-        hipLaunchKernelGGL(k_rc_gather, dim3((m + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)c->sg_scan.p,
+        hipLaunchKernelGGL(k_gather_flagged, dim3((m + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)c->sg_scan.p,
                            (const uint32_t*)(c->sg_scan.p + m), m, list);
         dispatch_K<RcChunks>(c->K, c, body, items, (const uint32_t*)list, np, bo, bd, best);
       }
@@ -1774,8 +1774,13 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
       if (tb[0]) {
         if (total + tb[0] >= (1ull << 32)) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: more than 2^32 contacts");
         if ((rc = ensure(c, c->sphere_out, total + tb[0], true))) return rc;
-        dispatch_K<SgLeaves>(c->K, c, body, (const uint2*)c->sg_items[cur].p, m, bs, s0, flip, 1, c->sg_scan.p,
-                             (const uint32_t*)(c->sg_scan.p + m), c->sphere_out.p + total, (uint32_t*)nullptr);
+        // the flagged leaves in order (the other frontier buffer holds the list), then their contacts
+        if ((rc = ensure(c, c->sg_items[cur ^ 1], ((size_t)tb[0] + 1) / 2))) return rc;
+        uint32_t* list = reinterpret_cast<uint32_t*>(c->sg_items[cur ^ 1].p);
+        hipLaunchKernelGGL(k_gather_flagged, dim3((m + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)c->sg_scan.p,
+                           (const uint32_t*)(c->sg_scan.p + m), m, list);
+        dispatch_K<SgLeaves>(c->K, c, body, (const uint2*)c->sg_items[cur].p, tb[0], bs, s0, flip, 1, (uint32_t*)nullptr,
+                             (const uint32_t*)list, c->sphere_out.p + total, (uint32_t*)nullptr);
         HIPCHK(c, hipGetLastError());
         total += tb[0];
       }

@@ -239,11 +239,6 @@ __global__ __launch_bounds__(256) void k_rc_roots(const uint2* __restrict__ item
   float t;
   pass[j] = (raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) && t < FLT_MAX) ? 1u : 0u;  // kdop_raycast.h:52-57
 }
-__global__ __launch_bounds__(256) void k_rc_gather(const uint32_t* __restrict__ pass, const uint32_t* __restrict__ off, uint32_t m,
-                                                   uint32_t* __restrict__ list) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < m && pass[j]) list[off[j]] = j;
-}
 // B, second pass: best[j] = the best length of chunk j's walk from FLT_MAX,
 // for the items in list.
 template <int K>

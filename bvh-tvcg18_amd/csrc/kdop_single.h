@@ -143,6 +143,14 @@ __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uin
 
 constexpr uint32_t kSgTested = 0x80000000u;  // item node bit: a leaf whose test passed
 
+// list[off[j]] = j for every flagged j (off: the exclusive scan of the flags):
+// the flagged items in order, for a pass that works on them alone.
+__global__ __launch_bounds__(256) void k_gather_flagged(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, uint32_t m,
+                                                        uint32_t* __restrict__ list) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m && flag[j]) list[off[j]] = j;
+}
+
 // geometry_convert.h:13-31: the sphere's DOP, o -/+ radius per direction.
 template <int K>
 __device__ inline void sphere_dop(const float* __restrict__ spheres, uint32_t i, const DirTable& dt, f3& c, float& r, float* sd) {
@@ -217,16 +225,17 @@ __global__ __launch_bounds__(256) void k_sg_expand_write(const uint2* __restrict
 // The leaves: the test of a leaf not yet tested, then the contact routine
 // (:68-89).  write = 0: flag[j] = 1 when leaf j gives a contact (and *bad is
 // set if an item is not a leaf, which the step count rules out); write = 1:
-// the flagged leaves write their contact at out[off[j]], numbering spheres
-// from sphere_base (the items number them within their batch).
+// thread k takes the flagged leaf j = list[k] (the flagged leaves gathered
+// in order, so few lanes idle) and writes its contact at out[k], numbering
+// spheres from sphere_base (the items number them within their batch).
 template <int K>
 __global__ __launch_bounds__(256) void k_sg_leaves(const BodyDev* __restrict__ bodies, uint32_t body, const uint2* __restrict__ items,
                                                    uint32_t m, const float* __restrict__ nodes, DirTable dt, const float* __restrict__ spheres,
-                                                   uint32_t sphere_base, int flip, int write, uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                                                   uint32_t sphere_base, int flip, int write, uint32_t* __restrict__ flag, const uint32_t* __restrict__ list,
                                                    kdop_sphere_contact* __restrict__ out, uint32_t* __restrict__ bad) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
-  if (write && flag[j] == 0u) return;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t j = write ? list[k] : k;
   const uint2 it = items[j];
   const uint32_t node = it.y & ~kSgTested;
   f3 c;
@@ -243,7 +252,10 @@ __global__ __launch_bounds__(256) void k_sg_leaves(const BodyDev* __restrict__ b
       if (!write) { flag[j] = 0u; atomicOr(bad, 1u); }
       return;
     }
-    if (!dop_overlap<K>(rec.v, sd)) { flag[j] = 0u; return; }
+    if (!dop_overlap<K>(rec.v, sd)) {
+      if (!write) flag[j] = 0u;
+      return;
+    }
   }
   const BodyDev& B = bodies[body];
   const uint32_t t = m0 & IDX_MASK;
@@ -256,7 +268,7 @@ __global__ __launch_bounds__(256) void k_sg_leaves(const BodyDev* __restrict__ b
   float depth;
   const bool hit = sphere_tet_contact(c, r, P, B.surf[t], flip != 0, p, nn, depth);
   if (!write) { flag[j] = hit ? 1u : 0u; return; }
-  kdop_sphere_contact& o = out[off[j]];
+  kdop_sphere_contact& o = out[k];
   o.sphere = sphere_base + it.x;
   o.tet = t;
   o.p[0] = p.x; o.p[1] = p.y; o.p[2] = p.z;
