@@ -56,3 +56,58 @@ def test_raycast_errors_and_empty():
     assert h.shape == (0, 3)
     with pytest.raises(K.KdopError):
         ctx.raycast(b + 5, np.zeros((1, 3), F32), np.ones((1, 3), F32))
+
+
+def _axis_rays(rng, n, lo, hi):
+    """Rays along the coordinate axes and the face and cube diagonals (parallel
+    to many slabs of the direction tables, the f == 0 branch), aimed at points
+    of a grid of spacing (hi - lo) / 8 (along faces and edges of an unjittered
+    cube grid when the body is unposed), from outside the box."""
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (0, -1, 1), (1, 0, -1), (1, 1, 1),
+            (-1, 1, -1)]
+    d = np.array([dirs[i % len(dirs)] for i in range(n)], np.float64)
+    g = lo + (hi - lo) * rng.integers(0, 9, size=(n, 3)) / 8.0
+    o = g - 3.0 * d / np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(F32), d.astype(F32)
+
+
+@pytest.mark.parametrize("k,jitter,posed", [(26, 0.0, False), (26, 0.1, False), (14, 0.0, True), (8, 0.0, False), (6, 0.1, True)],
+                         ids=["k26_grid", "k26_jitter", "k14_posed", "k8_grid", "k6_posed"])
+def test_raycast_axis_and_diagonal_rays(k, jitter, posed):
+    rng = np.random.default_rng(k + (7 if posed else 0))
+    mesh = K.box_mesh(10, size=1.0, jitter=jitter, seed=3)
+    pose = W.Pose(np.array([0.05, 0.0, -0.1], F32), K.rng_quaternions(9, 0, 1)[0]) if posed else W.IDENTITY
+    X, Y, Z = O.transform_f32(mesh.X, mesh.Y, mesh.Z, pose.p, pose.q)
+    tree = O.Tree(k, 8000, mesh.tets, mesh.X, mesh.Y, mesh.Z)
+    tree.refit(X, Y, Z)
+    ctx = K.Context(K=k)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], pose.p[None, :], pose.q[None, :])
+    lo = np.array([X.min(), Y.min(), Z.min()], np.float64)
+    hi = np.array([X.max(), Y.max(), Z.max()], np.float64)
+    o, d = _axis_rays(rng, 3000, lo, hi)
+    h_d, l_d, f_d = ctx.raycast(b, o, d)
+    h_o, l_o, f_o = O.raycast(tree, (mesh.tets, X, Y, Z, mesh.surface), o, d)
+    print(k, jitter, posed, "hits", int(f_o.sum()), "of", o.shape[0])
+    assert f_o.sum() > o.shape[0] // 4
+    np.testing.assert_array_equal(f_d, f_o)
+    assert l_d.tobytes() == l_o.tobytes()
+    assert h_d.tobytes() == h_o.tobytes()
+
+
+def test_raycast_in_batches_is_unchanged():
+    """A small frontier limit splits the rays into batches: the same results."""
+    rng = np.random.default_rng(3)
+    mesh = K.box_mesh(16, size=1.0, jitter=0.1, seed=5)
+    ctx = K.Context(K=26)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], np.zeros((1, 3), F32), np.array([[0, 0, 0, 1]], F32))
+    o, d = _rays(rng, 3000, np.zeros(3), 2.0, 0.55)
+    full = ctx.raycast(b, o, d)
+    ctx.set_option("single_items", 20000)
+    part = ctx.raycast(b, o, d)
+    for a, c in zip(full, part):
+        assert a.tobytes() == c.tobytes()
+    ctx.set_option("single_items", 1)
+    with pytest.raises(K.KdopError):  # rays from outside, aimed at the box (the first ones start inside it)
+        ctx.raycast(b, o[-4:], d[-4:])
