@@ -204,6 +204,58 @@ def batched_leg(args):
                 _prox_in=prox_in)
 
 
+def adjoint_leg(args):
+    """The differentiable narrow phase on its configuration: 64 overlapping
+    pairs of ~53k-tet jittered boxes, K=14.  Times the traversal that fixes
+    the pair set and contacts, the adjoint of Phi (sum of contact depths per
+    pair) for every body's pose, and one torch.autograd forward + backward
+    (refit, traversal, adjoint).  The reference has no differentiable narrow
+    phase, so there is no CPU path to time beside it."""
+    import numpy as np
+    import torch
+    import kdop_amd as KA
+    from kdop_amd import workloads as W
+    from kdop_amd.autograd import penetration_measure
+    wl = W.batched_pairs(npairs=64, K=14, seed=29, tmin=5e4, tmax=5e4, separated=0.0)
+    ctx = KA.Context(K=14)
+    ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
+    ids = np.asarray(ids, np.uint32)
+    dp, dq = device_poses(pose_p, pose_q)
+    ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
+    reps = 5
+    for _ in range(2):
+        c = ctx.tandem_traversal(pairs, narrow=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c = ctx.tandem_traversal(pairs, narrow=True)
+    torch.cuda.synchronize()
+    t_trav = (time.perf_counter() - t0) / reps
+    ctx.narrow_adjoint(ids)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        phi, gp, gq = ctx.narrow_adjoint(ids)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / reps
+    p = torch.tensor(pose_p, device="cuda", requires_grad=True)
+    q = torch.tensor(pose_q, device="cuda", requires_grad=True)
+    penetration_measure(ctx, ids, pairs, p, q).sum().backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p.grad = None
+        q.grad = None
+        penetration_measure(ctx, ids, pairs, p, q).sum().backward()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / reps
+    return dict(workload="c5_64pairs_53k_k14", pairs=int(pairs.shape[0]), bodies=int(ids.shape[0]), tets_total=int(sum(wl.info["tets"])),
+                contacts=int(c["contacts"]), narrow_pairs=int(c["narrow_pairs"]), traversal_ms=t_trav * 1e3, adjoint_ms=t_adj * 1e3,
+                autograd_step_ms=t_step * 1e3,
+                note="adjoint: Phi and dPhi/d(p, q) of all 128 bodies in FP64 from the frozen records; autograd step: "
+                     "refit + traversal + narrow phase forward, adjoint backward")
+
+
 def ray_set(n, seed=5):
     """Rays at the headline body A: origins on a sphere of radius 2 around its
     pose, aimed at points of its box (a few start inside or point away)."""
@@ -379,6 +431,7 @@ def main():
     ap.add_argument("--no-reorder", action="store_true", help="skip the device Morton reorder leg")
     ap.add_argument("--rays", type=int, default=65536, help="rays of the raycast leg (0: skip)")
     ap.add_argument("--spheres", type=int, default=65536, help="spheres of the single-traversal leg (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable narrow phase leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
@@ -416,6 +469,11 @@ def main():
     if args.spheres:
         head["single"] = single_leg(args, head)
         log(f"[bench] single traversal {head['single']['spheres']} spheres: {head['single']['ms']:.2f} ms")
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(args)
+        log(f"[bench] adjoint (64 pairs, K=14): traversal {adjoint['traversal_ms']:.2f} ms, adjoint {adjoint['adjoint_ms']:.2f} ms, "
+            f"autograd step {adjoint['autograd_step_ms']:.2f} ms")
     reorder = None
     if not args.no_reorder:
         reorder = reorder_leg(args)
@@ -475,6 +533,7 @@ def main():
         "mesh_reorder": reorder,
         "raycast": head.get("raycast"),
         "single_traversal": head.get("single"),
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }
