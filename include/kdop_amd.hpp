@@ -14,6 +14,9 @@
  * sequential reference reports them (kdop_sort_contacts, KDOP_ORDER_SEQUENTIAL).
  * reduce_contacts then gives what PROX's contact reduction makes of those
  * contacts (prox_collision_detection.h:340-345), computed on the device.
+ * raycast and single_traversal are kdop::raycast (kdop_raycast.h:106-155) and
+ * kdop::single_traversal (kdop_single_traversal.h:123-161) on a tree in its
+ * last refitted pose, one query as the reference takes it or many at once.
  * Errors throw kdop_amd::Error (the reference exits the process instead).
  * One Context per host thread.
  */
@@ -22,6 +25,7 @@
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -132,6 +136,32 @@ class Context {
     return cnt;
   }
 
+  /* kdop::raycast: true and the nearest front-facing surface hit (hit_point,
+   * length) when the ray from p along r (normalised here, as make_ray does)
+   * hits the tree's mesh; else false, (0, 0, 0) and FLT_MAX. */
+  bool raycast(const Tree& tree, const float p[3], const float r[3], float hit_point[3], float& length) {
+    uint8_t hit = 0;
+    check(kdop_raycast(c_, tree.body, 1, p, r, hit_point, &length, &hit));
+    return hit != 0;
+  }
+  /* n rays at once (3 floats per origin, direction and hit point). */
+  void raycast(const Tree& tree, uint32_t n, const float* origins, const float* directions, float* hit_points, float* lengths,
+               uint8_t* hits) {
+    check(kdop_raycast(c_, tree.body, n, origins, directions, hit_points, lengths, hits));
+  }
+
+  /* kdop::single_traversal with a sphere: the callback receives the sphere's
+   * contacts with the tree's mesh in the reference's order. */
+  void single_traversal(const float center[3], float radius, const Tree& tree, ContactsCallback& callback, bool should_flip = false) {
+    const float sphere[4] = {center[0], center[1], center[2], radius};
+    for (const kdop_sphere_contact& k : spheres_(tree, 1, sphere, should_flip)) callback(k.p, k.n, k.depth);
+  }
+  /* n spheres at once (x, y, z, radius each): the contacts grouped by sphere,
+   * each sphere's in the reference's order. */
+  std::vector<kdop_sphere_contact> single_traversal(const Tree& tree, uint32_t n, const float* spheres, bool should_flip = false) {
+    return spheres_(tree, n, spheres, should_flip);
+  }
+
   /* PROX contact reduction of the last tandem_traversal's contacts (the
    * std::sort + std::unique of prox_collision_detection.h:340-345 over the
    * ContactPoints its callbacks would have collected, body i and body j being
@@ -159,6 +189,18 @@ class Context {
   }
 
  private:
+  std::vector<kdop_sphere_contact> spheres_(const Tree& tree, uint32_t n, const float* spheres, bool flip) {
+    std::vector<kdop_sphere_contact> out(std::max<size_t>(1, (size_t)n * 16));
+    uint64_t cnt = 0;
+    int rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt);
+    if (rc == KDOP_ERR_CAPACITY) {
+      out.resize((size_t)cnt);
+      rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt);
+    }
+    check(rc);
+    out.resize((size_t)cnt);
+    return out;
+  }
   kdop_ctx* c_ = nullptr;
   std::vector<kdop_contact> contacts_;
   std::vector<uint32_t> pair_bodies_;  // body ids of the last traversal's pairs

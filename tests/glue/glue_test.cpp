@@ -8,6 +8,7 @@
 // PROX collects them.  Exit status 0 when all match.  Build: python bvh-tvcg18_amd/build.py --oracle.
 #include <stdint.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -31,6 +32,14 @@ int orc_traverse(void* hA, int ntA, const uint32_t* tetsA, const float* XA, cons
                  int threads, uint32_t* p1, uint64_t p1_cap, uint32_t* p2, uint64_t p2_cap, OrcContact* contacts, uint64_t c_cap,
                  uint64_t* counters, double* seconds);
 uint64_t orc_prox_reduce(uint64_t n, const uint32_t* bi, const uint32_t* bj, const float* p, const float* nrm, const float* d, uint32_t* out);
+int orc_raycast(void* h, const uint32_t* tets, const float* X, const float* Y, const float* Z, const uint8_t* surf, uint32_t n,
+                const float* origins, const float* directions, float* hits, float* lengths, uint8_t* did_hit);
+struct OrcSphereContact {
+  uint32_t sphere, tet;
+  float p[3], n[3], d;
+};
+uint64_t orc_single_traversal(void* h, const uint32_t* tets, const float* X, const float* Y, const float* Z, const uint8_t* surf, uint32_t n,
+                              const float* spheres, int flip, OrcSphereContact* out, uint64_t cap);
 }
 
 struct MeshData {
@@ -141,6 +150,56 @@ int main() {
       }
       std::printf("step %d reduction: %zu of %zu contacts kept (oracle %zu) %s\n", step, red.size(), bi.size(), nk, same ? "match" : "MISMATCH");
       bad += same ? 0 : 1;
+      if (step == 1) {
+        // kdop::raycast and kdop::single_traversal on body 0 in this pose: one query at a time and in one batch
+        const MeshData& m0 = meshes[0];
+        const uint32_t nr = 64;
+        std::vector<float> o(3 * nr), d(3 * nr), sp(4 * nr);
+        for (uint32_t i = 0; i < nr; ++i) {
+          const float a = 0.37f * (float)i, z = -0.9f + 1.8f * (float)i / (float)nr;
+          const float c = std::sqrt(1.0f - z * z);
+          const float u[3] = {c * std::cos(a), c * std::sin(a), z};
+          for (int k = 0; k < 3; ++k) {
+            o[3 * i + k] = pool[0].p[k] + 2.0f * u[k];
+            d[3 * i + k] = -u[k] + 0.05f * (float)((i + k) % 5 - 2);
+            sp[4 * i + k] = pool[0].p[k] + 0.6f * u[k];
+          }
+          sp[4 * i + 3] = 0.02f + 0.002f * (float)(i % 50);
+        }
+        std::vector<float> hw(3 * nr), lw(nr), hg(3 * nr), lg(nr);
+        std::vector<uint8_t> fw(nr), fg(nr);
+        orc_raycast(orc[0], m0.tets.data(), world[0].X.data(), world[0].Y.data(), world[0].Z.data(), m0.surf.data(), nr, o.data(), d.data(),
+                    hw.data(), lw.data(), fw.data());
+        ctx.raycast(trees[0], nr, o.data(), d.data(), hg.data(), lg.data(), fg.data());
+        bool rsame = std::memcmp(hw.data(), hg.data(), hw.size() * 4) == 0 && std::memcmp(lw.data(), lg.data(), lw.size() * 4) == 0 &&
+                     fw == fg;
+        uint32_t nhit = 0;
+        for (uint32_t i = 0; i < nr && rsame; ++i) {
+          float h1[3], l1 = 0.0f;
+          const bool hit1 = ctx.raycast(trees[0], &o[3 * i], &d[3 * i], h1, l1);
+          rsame = hit1 == (fw[i] != 0) && std::memcmp(h1, &hw[3 * i], 12) == 0 && std::memcmp(&l1, &lw[i], 4) == 0;
+          nhit += hit1 ? 1u : 0u;
+        }
+        std::printf("raycast: %u of %u rays hit %s\n", nhit, nr, rsame ? "match" : "MISMATCH");
+        bad += rsame ? 0 : 1;
+        std::vector<OrcSphereContact> sw(1 << 16);
+        const size_t ns = (size_t)orc_single_traversal(orc[0], m0.tets.data(), world[0].X.data(), world[0].Y.data(), world[0].Z.data(),
+                                                       m0.surf.data(), nr, sp.data(), 0, sw.data(), sw.size());
+        const std::vector<kdop_sphere_contact> sg = ctx.single_traversal(trees[0], nr, sp.data());
+        bool ssame = ns == sg.size() && ns <= sw.size();
+        for (size_t i = 0; ssame && i < ns; ++i)
+          ssame = sw[i].sphere == sg[i].sphere && sw[i].tet == sg[i].tet && std::memcmp(sw[i].p, sg[i].p, 28) == 0;
+        size_t k = 0;  // the same contacts through a callback, one sphere at a time
+        for (uint32_t i = 0; i < nr && ssame; ++i) {
+          Collect cb;
+          ctx.single_traversal(&sp[4 * i], sp[4 * i + 3], trees[0], cb);
+          for (size_t j = 0; ssame && j < cb.v.size() / 7; ++j, ++k)
+            ssame = k < ns && sw[k].sphere == i && std::memcmp(sw[k].p, &cb.v[7 * j], 28) == 0;
+        }
+        ssame = ssame && k == ns;
+        std::printf("single traversal: %zu contacts %s\n", ns, ssame ? "match" : "MISMATCH");
+        bad += ssame ? 0 : 1;
+      }
     }
     for (void* h : orc) orc_tree_free(h);
     if (total == 0) {

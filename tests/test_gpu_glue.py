@@ -20,4 +20,4 @@ def test_cpp_call_shapes_match_oracle():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
-    assert r.stdout.count("match") == 8 and "MISMATCH" not in r.stdout
+    assert r.stdout.count("match") == 10 and "MISMATCH" not in r.stdout
