@@ -741,6 +741,56 @@ __device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expan
     }
 }
 
+// P1 part of emit_leaves (every lane of the wave calls it; nothing when P1 is
+// not kept).
+__device__ inline void emit_p1(const TravArgs& T, uint32_t pair, const Expansion& x) {
+  if (!T.p1) return;
+  const uint32_t o1 = wave_reserve(T.p1_fill, __popc(x.leaf));
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if ((x.leaf >> k) & 1u) {
+      const uint32_t o = o1 + __popc(x.leaf & ((1u << k) - 1u));
+      if (o < T.p1_cap) T.p1[o] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+      else atomicOr(T.overflow, 2u);
+    }
+}
+
+// Surface leaf pairs (P2) staged per block in LDS: a wave-wide reservation on
+// the one P2 counter for every wave that found some serializes across the
+// XCDs at the leaf levels of a wide traversal; staged, a block reserves once
+// per kP2Stage - 1024 pairs (an iteration of 256 lanes adds at most 1024).
+constexpr uint32_t kP2Stage = 2048;
+struct P2Stage {
+  LeafPair p[kP2Stage];
+  uint32_t n, base;
+};
+__device__ inline void stage_p2(P2Stage& S, uint32_t pair, const Expansion& x) {
+  const uint32_t ns = __popc(x.surf);
+  if (!ns) return;
+  uint32_t pos = atomicAdd(&S.n, ns);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if ((x.surf >> k) & 1u) S.p[pos++] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
+}
+// Write the stage to P2 when forced (the block's end) or when the next
+// iteration might not fit.  Every thread of the block calls it.
+__device__ inline void flush_p2(const TravArgs& T, P2Stage& S, bool force) {
+  __syncthreads();
+  const uint32_t m = S.n;
+  if (m == 0u || (!force && m <= kP2Stage - 1024u)) return;
+  if (threadIdx.x == 0) {
+    S.base = atomicAdd(T.p2_fill, m);
+    if (S.base + m > T.p2_cap) atomicOr(T.overflow, 4u);
+  }
+  __syncthreads();
+  const uint32_t b = S.base;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
+    if (b + i < T.p2_cap) T.p2[b + i] = S.p[i];
+  __syncthreads();
+  if (threadIdx.x == 0) S.n = 0u;
+  __syncthreads();
+}
+
 // Write this lane's new items at slot o of `out` (one wave-wide reservation).
 __device__ inline void emit_items(const Expansion& x, Item* __restrict__ out, uint32_t o, uint32_t cap, unsigned int* overflow) {
 #pragma unroll

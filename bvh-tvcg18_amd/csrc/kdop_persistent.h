@@ -80,7 +80,15 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   __shared__ uint32_t red[5];
   __shared__ uint32_t pre[kSegs + 1];
   __shared__ uint32_t raw_total;
-  if (C->final_set) return;
+  __shared__ P2Stage st2;
+  __shared__ uint32_t stopped;
+  // one read per block: block 0 of the launch that stops the levels sets
+  // final_set while other blocks of that launch start, and every wave of a
+  // block must take the same branch (wave 0 computes the prefix sums the
+  // others wait for)
+  if (threadIdx.x == 0) stopped = __hip_atomic_load(&C->final_set, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (stopped) return;
   const uint32_t segcap_in = cap / nseg_in;
   uint32_t n;
   if (nseg_in == 1u) {
@@ -120,6 +128,8 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   uint32_t* out_fill = nseg_out == 1u ? &C->level_fill[level + 1] : &C->lseg[(level + 1u) % 3u][so].fill;
   Item* __restrict__ out_seg = out + (size_t)so * segcap_out;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
+  if (threadIdx.x == 0) st2.n = 0u;
+  __syncthreads();
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;
     Expansion x;
@@ -142,8 +152,11 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
     const uint32_t o = block_reserve(out_fill, __popc(x.push), red);
     emit_items(x, out_seg, o, segcap_out, T.overflow);
-    emit_leaves(T, it.pair, x);
+    emit_p1(T, it.pair, x);
+    stage_p2(st2, it.pair, x);
+    flush_p2(T, st2, false);
   }
+  flush_p2(T, st2, true);
   CounterStripe* st = stripe_of(T.stripes, wave_global());
   wave_add64(&st->tests, tests);
   wave_add64(&st->descent, descent);
