@@ -103,11 +103,12 @@ __device__ inline bool raycast_tet(const f3& o, const f3& r, const f3* P, uint32
     if (dot3(n[i], q) - w[i] > 0.0f) continue;
     if (dot3(n[j], q) - w[j] > 0.0f) continue;
     if (dot3(n[k], q) - w[k] > 0.0f) continue;
-    if (t < length) {
-      length = t;
-      hit = q;
-      status = true;
-    }
+    const bool better = t < length;  // per-component selects keep q out of scratch memory
+    hit.x = better ? q.x : hit.x;
+    hit.y = better ? q.y : hit.y;
+    hit.z = better ? q.z : hit.z;
+    length = better ? t : length;
+    status = status || better;
   }
   return status;
 }
@@ -161,8 +162,11 @@ __device__ inline float chunk_raycast(const float* __restrict__ nodes, const Bod
       f3 q;
       float len;
       if (raycast_tet(o, r, P, B.surf[c0], q, len)) {
-        hit = len < length ? q : hit;
-        length = len < length ? len : length;
+        const bool better = len < length;
+        hit.x = better ? q.x : hit.x;
+        hit.y = better ? q.y : hit.y;
+        hit.z = better ? q.z : hit.z;
+        length = better ? len : length;
       }
     } else {  // :81-99, first child first
       st[sp++] = c1;

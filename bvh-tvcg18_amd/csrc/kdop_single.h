@@ -57,23 +57,42 @@ __device__ inline bool inside_triangle_walls(const f3& p, const f3* t) {
 // (true, with its point, normal and depth); surf bit v: the face opposite
 // vertex v is on the surface.
 __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uint32_t surf, bool flip, f3& p, f3& n, float& depth) {
+  // Every loop is unrolled so that each array index is a constant and the
+  // arrays stay in registers (a dynamically indexed array goes to scratch
+  // memory, which for tens of millions of lanes is gigabytes of traffic).
   constexpr int FACE[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};  // get_opposite_face
   Plane3 plane[4];
   float d[4];
+#pragma unroll
   for (int k = 0; k < 4; ++k) {  // :51-60, stopping at the first face too far away
     plane[k] = plane3(B[FACE[k][0]], B[FACE[k][1]], B[FACE[k][2]]);
     d[k] = signed_distance(c, plane[k]);
     if (d[k] > r) return false;
   }
   if (d[0] < 0.0f && d[1] < 0.0f && d[2] < 0.0f && d[3] < 0.0f) {  // :63-84 center inside
+    // the first face whose distance is below no other's, taken by selects
+    // (returning from inside the loop let the compiler index plane[] at run time)
+    bool found = false;
+    Plane3 P{f3{0.0f, 0.0f, 0.0f}, 0.0f};
+    float dk = 0.0f;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (d[k] < d[(k + 1) % 4] || d[k] < d[(k + 2) % 4] || d[k] < d[(k + 3) % 4]) continue;
-      n = flip ? plane[k].n : f3{-plane[k].n.x, -plane[k].n.y, -plane[k].n.z};
-      p = closest_on_plane(c, plane[k]);
-      depth = d[k];
+      const bool pick = !found && !(d[k] < d[(k + 1) % 4] || d[k] < d[(k + 2) % 4] || d[k] < d[(k + 3) % 4]);
+      P.n.x = pick ? plane[k].n.x : P.n.x;
+      P.n.y = pick ? plane[k].n.y : P.n.y;
+      P.n.z = pick ? plane[k].n.z : P.n.z;
+      P.w = pick ? plane[k].w : P.w;
+      dk = pick ? d[k] : dk;
+      found = found || pick;
+    }
+    if (found) {
+      n = flip ? P.n : f3{-P.n.x, -P.n.y, -P.n.z};
+      p = closest_on_plane(c, P);
+      depth = dk;
       return true;
     }
   }
+#pragma unroll
   for (int i = 0; i < 4; ++i) {  // :87-112 vertex regions
     if (!(((surf >> ((i + 1) & 3)) | (surf >> ((i + 2) & 3)) | (surf >> ((i + 3) & 3))) & 1u)) continue;
     const Plane3 v1 = plane3_np(unit3(sub3(B[i], B[(i + 1) & 3])), B[i]);
@@ -91,6 +110,7 @@ __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uin
     }
   }
   constexpr int EDGE[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 2, 0, 3}, {1, 3, 2, 0}, {2, 3, 0, 1}};
+#pragma unroll
   for (int v = 0; v < 6; ++v) {  // :115-160 edge regions
     const int i = EDGE[v][0], j = EDGE[v][1], k = EDGE[v][2], m = EDGE[v][3];
     if (!(((surf >> k) | (surf >> m)) & 1u)) continue;
@@ -109,6 +129,7 @@ __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uin
       return true;
     }
   }
+#pragma unroll
   for (int v = 0; v < 4; ++v) {  // :163-185 face regions
     if (!((surf >> v) & 1u)) continue;
     const float dist = signed_distance(c, plane[v]);
