@@ -227,3 +227,23 @@ def test_both_narrow_paths(path):
     ctx.set_option("narrow_path", path)
     dev = device_run(wl, ctx=ctx)
     _check_pair_results(wl, dev, orc)
+
+
+def test_levels_first_call_regrows_segmented_frontier():
+    """A fresh context starts with a 65,536-item frontier; a level-synchronous
+    call whose levels hold millions of items overflows its segments, grows the
+    buffers and runs again (counters['overflow'] > 0) with full parity; the
+    next call on the grown buffers does not overflow and gives the same sets."""
+    # a body against an identical copy in the same pose: every chunk overlaps
+    # its neighbours, so the levels far exceed the first estimate (4 per tet)
+    mesh = K.box_mesh(14, size=1.0, jitter=0.1, seed=6)
+    wl = W.PairWorkload("self14_k8", 8, [mesh, mesh], np.array([[0, 1]], np.uint32), [W.IDENTITY, W.IDENTITY])
+    orc, _, _ = oracle_run(wl)
+    dev = device_run(wl, traversal=K.KDOP_TRAVERSAL_LEVELS)
+    print("overflow", dev["counters"]["overflow"], "tests", dev["counters"]["tests"])
+    assert dev["counters"]["overflow"] > 0
+    _check_pair_results(wl, dev, orc)
+    again = device_run(wl, ctx=dev["ctx"], ids=dev["ids"])
+    assert again["counters"]["overflow"] == 0
+    np.testing.assert_array_equal(sort_pairs(again["p1"]), sort_pairs(dev["p1"]))
+    np.testing.assert_array_equal(sort_pairs(again["p2"]), sort_pairs(dev["p2"]))
