@@ -97,6 +97,9 @@ struct kdop_ctx {
   CallState* dstate = nullptr;    // counters and frontier segments of one call
   CallSummary* hsum = nullptr;    // call summary, pinned host memory written by k_sum_counters
   CallSummary* hsum_dev = nullptr;  // its device address
+  HostWords* hwords = nullptr;      // step sizes of the single traversal / raycast, pinned host memory
+  HostWords* hwords_dev = nullptr;  // its device address
+  uint32_t hwords_seq = 0;
   uint32_t last_npairs = 0;
   bool have_traversal = false, have_p1 = false, have_narrow = false;
   std::vector<uint2> last_pairs;
@@ -531,6 +534,12 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
   }
   std::memset(c->hsum, 0, sizeof(CallSummary));
   if (hipHostGetDevicePointer((void**)&c->hsum_dev, c->hsum, 0) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
+  if (hipHostMalloc(&c->hwords, sizeof(HostWords), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    kdop_destroy(c);
+    return KDOP_ERR_NOMEM;
+  }
+  std::memset(c->hwords, 0, sizeof(HostWords));
+  if (hipHostGetDevicePointer((void**)&c->hwords_dev, c->hwords, 0) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
@@ -564,6 +573,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->rc_best); release(c->rc_first);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
+  if (c->hwords) (void)hipHostFree(c->hwords);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
   if (c->tg_exec) (void)hipGraphExecDestroy(c->tg_exec);
@@ -921,6 +931,27 @@ int finish_summary(kdop_ctx* c) {
   c->state_clean = c->hsum->reset != 0;
   return KDOP_OK;
 }
+// Read n <= 4 device words once the work queued before is done (k_publish_words
+// into pinned host memory; the host spins on the sequence number and checks
+// the stream now and then, as finish_summary does).
+int read_device_words(kdop_ctx* c, const uint32_t* src, uint32_t n, uint32_t* out) {
+  const uint32_t seq = ++c->hwords_seq;
+  hipLaunchKernelGGL(k_publish_words, dim3(1), dim3(64), 0, c->stream, src, n, c->hwords_dev, seq);
+  HIPCHK(c, hipGetLastError());
+  for (uint64_t spin = 1;; ++spin) {
+    if (__atomic_load_n(&c->hwords->seq, __ATOMIC_ACQUIRE) == seq) break;
+    if ((spin & 4095u) == 0u) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) return fail(c, KDOP_ERR_HIP, "HIP error %d (%s)", (int)e, hipGetErrorString(e));
+      if (__atomic_load_n(&c->hwords->seq, __ATOMIC_ACQUIRE) != seq) return fail(c, KDOP_ERR_HIP, "device words were not published");
+      break;
+    }
+  }
+  for (uint32_t i = 0; i < n; ++i) out[i] = c->hwords->v[i];
+  return KDOP_OK;
+}
+
 int summarize(kdop_ctx* c, uint32_t cap, int final_round) {
   int rc = enqueue_summary(c, cap, final_round);
   return rc ? rc : finish_summary(c);
@@ -1591,9 +1622,7 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
   auto read_words = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
     const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
-    HIPCHK(c, hipMemcpyAsync(host, c->sg_scan.p + 2 * (size_t)m + nb, 4 * (size_t)words, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return KDOP_OK;
+    return read_device_words(c, c->sg_scan.p + 2 * (size_t)m + nb, words, host);
   };
   auto scan = [&](uint32_t m) {
     const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
@@ -1673,8 +1702,7 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
                           d_h, d_l, d_f);
     HIPCHK(c, hipGetLastError());
     uint32_t bad_h = 0;
-    HIPCHK(c, hipMemcpyAsync(&bad_h, bad, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = read_device_words(c, bad, 1, &bad_h))) return rc;
     if (bad_h) return fail(c, KDOP_ERR_INTERNAL, "kdop_raycast: a node other than a branch-chunk root after %u steps", steps);
     s0 += sn;
   }
@@ -1706,9 +1734,7 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
   auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
   auto read_total = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
     const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;
-    HIPCHK(c, hipMemcpyAsync(host, c->sg_scan.p + 2 * (size_t)m + nb, 4 * (size_t)words, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return KDOP_OK;
+    return read_device_words(c, c->sg_scan.p + 2 * (size_t)m + nb, words, host);
   };
   auto scan = [&](uint32_t m) {
     const uint32_t nb = (m + kScanBlock - 1) / kScanBlock;

@@ -164,6 +164,20 @@ __device__ inline bool sphere_tet_contact(const f3& c, float r, const f3* B, uin
 
 constexpr uint32_t kSgTested = 0x80000000u;  // item node bit: a leaf whose test passed
 
+// A few device words read by the host without a copy and a stream
+// synchronisation: one thread copies them into mapped, coherent host memory,
+// then releases `seq` at system scope; the host waits for that number.
+struct HostWords {
+  uint32_t v[4];
+  uint32_t seq;
+};
+__global__ void k_publish_words(const uint32_t* __restrict__ src, uint32_t n, HostWords* __restrict__ dst, uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  for (uint32_t i = 0; i < n; ++i) dst->v[i] = src[i];
+  __threadfence_system();
+  __hip_atomic_store(&dst->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // list[off[j]] = j for every flagged j (off: the exclusive scan of the flags):
 // the flagged items in order, for a pass that works on them alone.
 __global__ __launch_bounds__(256) void k_gather_flagged(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, uint32_t m,
