@@ -113,3 +113,25 @@ def test_raycast_in_batches_is_unchanged():
     ctx.set_option("single_items", 1)
     with pytest.raises(K.KdopError):  # rays from outside, aimed at the box (the first ones start inside it)
         ctx.raycast(b, o[-4:], d[-4:])
+
+
+@pytest.mark.parametrize("k,n", [(8, 1), (26, 1), (6, 2)], ids=["k8_cube", "k26_cube", "k6_2cubes"])
+def test_raycast_one_chunk_bodies(k, n):
+    """Bodies of one branch chunk (no super level, no breadth-first step): the
+    same hits as the oracle, including the single tetrahedron."""
+    rng = np.random.default_rng(k * 7 + n)
+    tet = K.Mesh(np.array([0, 1, 0, 0], F32), np.array([0, 0, 1, 0], F32), np.array([0, 0, 0, 1], F32),
+                 np.array([[0, 1, 2, 3]], np.uint32), np.array([15], np.uint8))
+    for mesh in (tet, K.box_mesh(n, size=1.0, jitter=0.0, seed=1)):
+        tree = O.Tree(k, 8000, mesh.tets, mesh.X, mesh.Y, mesh.Z)
+        tree.refit(mesh.X, mesh.Y, mesh.Z)
+        ctx = K.Context(K=k)
+        b = ctx.make_tree(mesh)
+        ctx.refit([b], np.zeros((1, 3), F32), np.array([[0, 0, 0, 1]], F32))
+        assert ctx.shape(b)["H"] == 1
+        o, d = _rays(rng, 512, np.full(3, 0.25), 2.0, 0.3)
+        got = ctx.raycast(b, o, d)
+        want = O.raycast(tree, (mesh.tets, mesh.X, mesh.Y, mesh.Z, mesh.surface), o, d)
+        assert want[2].sum() > 0
+        for a, w in zip(got, want):
+            assert np.asarray(a).tobytes() == np.asarray(w).tobytes()

@@ -92,3 +92,25 @@ def ctypes_count(ctx, b, sp):
                                             out.shape[0], ctypes.byref(cnt))
         assert rc == K._abi.KDOP_ERR_CAPACITY if out.size else rc == 0
     return cnt.value
+
+
+@pytest.mark.parametrize("k,n", [(8, 1), (26, 1), (6, 2)], ids=["k8_cube", "k26_cube", "k6_2cubes"])
+def test_single_traversal_one_chunk_bodies(k, n):
+    """Bodies of one branch chunk (no super level, no breadth-first step): the
+    same contacts as the oracle, including the single tetrahedron."""
+    rng = np.random.default_rng(k * 5 + n)
+    tet = K.Mesh(F32([0, 1, 0, 0]), F32([0, 0, 1, 0]), F32([0, 0, 0, 1]), np.array([[0, 1, 2, 3]], np.uint32), np.array([15], np.uint8))
+    for mesh in (tet, K.box_mesh(n, size=1.0, jitter=0.0, seed=1)):
+        tree = O.Tree(k, 8000, mesh.tets, mesh.X, mesh.Y, mesh.Z)
+        tree.refit(mesh.X, mesh.Y, mesh.Z)
+        ctx = K.Context(K=k)
+        b = ctx.make_tree(mesh)
+        ctx.refit([b], np.zeros((1, 3), F32), np.array([[0, 0, 0, 1]], F32))
+        assert ctx.shape(b)["H"] == 1
+        c = rng.uniform(-0.2, 0.8, size=(512, 3))
+        r = rng.uniform(0.02, 0.3, size=(512, 1))
+        sp = np.concatenate([c, r], 1).astype(F32)
+        got = ctx.single_traversal(b, sp)
+        want = O.single_traversal(tree, (mesh.tets, mesh.X, mesh.Y, mesh.Z, mesh.surface), sp)
+        assert want.shape[0] > 0
+        assert got.tobytes() == want.tobytes()
