@@ -1596,7 +1596,7 @@ int kdop_mesh_reorder_gpu(kdop_ctx* c, uint32_t nv, const float* X, const float*
 }
 
 int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, const float* directions, float* hit_points, float* lengths,
-                 uint8_t* hits) {
+                 uint8_t* hits, int is_device) {
   if (!c) return KDOP_ERR_ARG;
   if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_raycast: unknown body %u", body);
   if (n == 0) return KDOP_OK;
@@ -1608,16 +1608,29 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   if (lg + 1u > kChunkStack) return fail(c, KDOP_ERR_ARG, "kdop_raycast: chunk trees too deep (%u leaves)", b.L);
   const uint32_t steps = (b.H - 1u) * std::max(lg, 1u);  // below the top-level roots, every item is a branch-chunk root after these
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  const size_t bytes_in = 6 * (size_t)n * sizeof(float), bytes_out = 4 * (size_t)n * sizeof(float);
-  int rc = ensure(c, c->raybuf, bytes_in + bytes_out + n);
-  if (rc) return rc;
-  float* d_o = reinterpret_cast<float*>(c->raybuf.p);
-  float* d_d = d_o + 3 * (size_t)n;
-  float* d_h = d_d + 3 * (size_t)n;
-  float* d_l = d_h + 3 * (size_t)n;
-  uint8_t* d_f = reinterpret_cast<uint8_t*>(d_l + n);
-  HIPCHK(c, hipMemcpyAsync(d_o, origins, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(d_d, directions, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  if (is_device && ((reinterpret_cast<uintptr_t>(origins) | reinterpret_cast<uintptr_t>(directions) | reinterpret_cast<uintptr_t>(hit_points) |
+                     reinterpret_cast<uintptr_t>(lengths)) & 3u))
+    return fail(c, KDOP_ERR_ARG, "kdop_raycast: device float arrays must be 4-byte aligned");
+  // device arrays are read and written in place; host arrays go through raybuf
+  const float* d_o = origins;
+  const float* d_d = directions;
+  float* d_h = hit_points;
+  float* d_l = lengths;
+  uint8_t* d_f = hits;
+  int rc = KDOP_OK;
+  if (!is_device) {
+    const size_t bytes_in = 6 * (size_t)n * sizeof(float), bytes_out = 4 * (size_t)n * sizeof(float);
+    if ((rc = ensure(c, c->raybuf, bytes_in + bytes_out + n))) return rc;
+    float* s_o = reinterpret_cast<float*>(c->raybuf.p);
+    float* s_d = s_o + 3 * (size_t)n;
+    d_h = s_d + 3 * (size_t)n;
+    d_l = d_h + 3 * (size_t)n;
+    d_f = reinterpret_cast<uint8_t*>(d_l + n);
+    HIPCHK(c, hipMemcpyAsync(s_o, origins, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s_d, directions, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    d_o = s_o;
+    d_d = s_d;
+  }
   // scan scratch for m values: [count m][offset m][block sums][total][bad]
   auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
   auto read_words = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
@@ -1706,15 +1719,17 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
     if (bad_h) return fail(c, KDOP_ERR_INTERNAL, "kdop_raycast: a node other than a branch-chunk root after %u steps", steps);
     s0 += sn;
   }
-  HIPCHK(c, hipMemcpyAsync(hit_points, d_h, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(lengths, d_l, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hits, d_f, n, hipMemcpyDeviceToHost, c->stream));
+  if (!is_device) {
+    HIPCHK(c, hipMemcpyAsync(hit_points, d_h, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(lengths, d_l, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hits, d_f, n, hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
 }
 
 int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* spheres, int flip, kdop_sphere_contact* out, uint64_t cap,
-                          uint64_t* count) {
+                          uint64_t* count, int is_device) {
   if (!c || !count) return KDOP_ERR_ARG;
   *count = 0;
   if (body >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: unknown body %u", body);
@@ -1726,10 +1741,15 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
   while ((1u << lg) < b.L) ++lg;
   const uint32_t steps = b.H * std::max(lg, 1u);  // below the top-level chunk roots, every item is a leaf after these
   HIPCHK(c, hipSetDevice(c->cfg.device));
-  int rc = ensure(c, c->raybuf, 16 * (size_t)n);
-  if (rc) return rc;
-  float* d_s = reinterpret_cast<float*>(c->raybuf.p);
-  HIPCHK(c, hipMemcpyAsync(d_s, spheres, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  int rc = KDOP_OK;
+  if (is_device && (reinterpret_cast<uintptr_t>(spheres) & 15u))
+    return fail(c, KDOP_ERR_ARG, "kdop_single_traversal: device spheres must be 16-byte aligned");
+  const float* d_s = spheres;  // device spheres are read in place
+  if (!is_device) {
+    if ((rc = ensure(c, c->raybuf, 16 * (size_t)n))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->raybuf.p, spheres, 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    d_s = reinterpret_cast<const float*>(c->raybuf.p);
+  }
   // scan scratch for m values: [count m][offset m][block sums][total][bad]
   auto scratch = [&](uint32_t m) { return ensure(c, c->sg_scan, 2 * (size_t)m + (m + kScanBlock - 1) / kScanBlock + 2); };
   auto read_total = [&](uint32_t m, uint32_t* host, uint32_t words) -> int {
@@ -1817,7 +1837,8 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
   if (!out) return KDOP_OK;
   if (cap < total) return fail(c, KDOP_ERR_CAPACITY, "kdop_single_traversal: need %llu contacts", (unsigned long long)total);
   if (total == 0) return KDOP_OK;
-  HIPCHK(c, hipMemcpyAsync(out, c->sphere_out.p, (size_t)total * sizeof(kdop_sphere_contact), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->sphere_out.p, (size_t)total * sizeof(kdop_sphere_contact),
+                           is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
 }

@@ -28,6 +28,19 @@ class KdopError(RuntimeError):
         self.code = code
 
 
+def _is_gpu_tensor(x):
+    """A torch tensor in GPU memory (torch is not imported for host arrays)."""
+    return getattr(x, "is_cuda", False) is True
+
+
+def _gpu_f32(x, width):
+    """x as a contiguous float32 GPU tensor of shape (n, width)."""
+    import torch
+    if not _is_gpu_tensor(x):
+        raise ValueError("expected a GPU tensor")
+    return x.to(torch.float32).reshape(-1, width).contiguous()
+
+
 @dataclass
 class Mesh:
     """Tetrahedral mesh: SoA vertex coordinates, tets (nt x 4, vertices i j k m),
@@ -328,7 +341,11 @@ class Context:
         return Xo, Yo, Zo, To
 
     def raycast(self, body, origins, directions):
-        """kdop_raycast: (hit points (n, 3), lengths (n,), hit flags (n,) bool) of n rays at `body`."""
+        """kdop_raycast: (hit points (n, 3), lengths (n,), hit flags (n,) bool) of n rays at `body`.
+        With GPU tensors for origins and directions the call reads and writes
+        device memory in place and returns GPU tensors (hit flags uint8)."""
+        if _is_gpu_tensor(origins) or _is_gpu_tensor(directions):
+            return self._raycast_device(body, origins, directions)
         o = np.ascontiguousarray(np.asarray(origins, np.float32).reshape(-1, 3))
         d = np.ascontiguousarray(np.asarray(directions, np.float32).reshape(-1, 3))
         n = o.shape[0]
@@ -337,22 +354,61 @@ class Context:
         hits = np.zeros((n, 3), np.float32)
         lengths = np.zeros(n, np.float32)
         flags = np.zeros(n, np.uint8)
-        self._check(self._lib.kdop_raycast(self._h, body, n, ptr(o), ptr(d), ptr(hits), ptr(lengths), ptr(flags)))
+        self._check(self._lib.kdop_raycast(self._h, body, n, ptr(o), ptr(d), ptr(hits), ptr(lengths), ptr(flags), 0))
         return hits, lengths, flags.astype(bool)
+
+    def _raycast_device(self, body, origins, directions):
+        import torch
+        o = _gpu_f32(origins, 3)
+        d = _gpu_f32(directions, 3)
+        n = o.shape[0]
+        if d.shape[0] != n or d.device != o.device:
+            raise ValueError("origins and directions differ in length or device")
+        hits = torch.empty((n, 3), dtype=torch.float32, device=o.device)
+        lengths = torch.empty(n, dtype=torch.float32, device=o.device)
+        flags = torch.empty(n, dtype=torch.uint8, device=o.device)
+        if n:
+            torch.cuda.current_stream(o.device).synchronize()  # inputs written on torch's stream are complete
+            self._check(self._lib.kdop_raycast(self._h, body, n, o.data_ptr(), d.data_ptr(), hits.data_ptr(), lengths.data_ptr(),
+                                               flags.data_ptr(), 1))
+        return hits, lengths, flags
 
     def single_traversal(self, body, spheres, flip=False):
         """kdop_single_traversal: contacts of spheres (n, 4: x, y, z, radius) with `body`,
-        grouped by sphere, each sphere's in the reference's report order."""
+        grouped by sphere, each sphere's in the reference's report order.
+        With a GPU tensor of spheres the call reads it in place and returns
+        the records as a GPU uint8 tensor (count, record size) in device memory."""
+        if _is_gpu_tensor(spheres):
+            return self._single_traversal_device(body, spheres, flip)
         sp = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1, 4))
         n = ctypes.c_uint64()
         # room for 64 contacts per sphere (or the last call's count) first, so
         # that one call usually suffices; np.empty touches no memory
         cap = max(64 * sp.shape[0], getattr(self, "_single_last", 0), 1)
         out = np.empty(cap, _abi.SPHERE_CONTACT_DTYPE)
-        rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), cap, ctypes.byref(n))
+        rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), cap, ctypes.byref(n), 0)
         if rc == _abi.KDOP_ERR_CAPACITY:
             out = np.empty(n.value, _abi.SPHERE_CONTACT_DTYPE)
-            rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), n.value, ctypes.byref(n))
+            rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], ptr(sp), int(flip), ptr(out), n.value, ctypes.byref(n), 0)
+        self._check(rc)
+        self._single_last = n.value
+        return out[:n.value]
+
+    def _single_traversal_device(self, body, spheres, flip):
+        import torch
+        sp = _gpu_f32(spheres, 4)
+        rec = _abi.SPHERE_CONTACT_DTYPE.itemsize
+        n = ctypes.c_uint64()
+        cap = max(64 * sp.shape[0], getattr(self, "_single_last", 0), 1)
+        out = torch.empty((cap, rec), dtype=torch.uint8, device=sp.device)
+        if sp.shape[0]:
+            torch.cuda.current_stream(sp.device).synchronize()
+        rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], sp.data_ptr(), int(flip), out.data_ptr(), cap,
+                                             ctypes.byref(n), 1)
+        if rc == _abi.KDOP_ERR_CAPACITY:
+            out = torch.empty((n.value, rec), dtype=torch.uint8, device=sp.device)
+            rc = self._lib.kdop_single_traversal(self._h, body, sp.shape[0], sp.data_ptr(), int(flip), out.data_ptr(), n.value,
+                                                 ctypes.byref(n), 1)
         self._check(rc)
         self._single_last = n.value
         return out[:n.value]

@@ -97,8 +97,8 @@ SIGNATURES = [
     ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
     ("kdop_reduce_contacts", ctypes.c_int, [_p, _p]),
     ("kdop_reduce_prox_contacts", ctypes.c_int, [_p, _u64, _p, _p, ctypes.c_int, _p]),
-    ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p]),
-    ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p]),
+    ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p, ctypes.c_int]),
+    ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p, ctypes.c_int]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),
     ("kdop_get_world", ctypes.c_int, [_p, _u32, _p, _p, _p]),
     ("kdop_export_tree", ctypes.c_int, [_p, _u32, _u64, _p, _p, _p, _p, _p]),

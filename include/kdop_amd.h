@@ -209,9 +209,10 @@ int kdop_reduce_prox_contacts(kdop_ctx* ctx, uint64_t n, const kdop_prox_contact
 int kdop_mesh_reorder_gpu(kdop_ctx* ctx, uint32_t nv, const float* X, const float* Y, const float* Z, uint32_t nt,
                           const uint32_t* tets, float* Xo, float* Yo, float* Zo, uint32_t* tets_o);
 
-/* Cast n rays at one body in its last refitted pose (host arrays; origins and
- * directions 3 floats each, a direction need not be unit: it is normalised as
- * geometry::make_ray does).  Per ray: the nearest hit on a front-facing
+/* Cast n rays at one body in its last refitted pose (origins and directions
+ * 3 floats each, a direction need not be unit: it is normalised as
+ * geometry::make_ray does; is_device 0: host arrays, 1: device arrays of the
+ * context's GPU, read and written in place, 4-byte aligned).  Per ray: the nearest hit on a front-facing
  * surface face (hit_points, 3 floats), its ray length, and hits[i] = 1; a
  * miss gives the point (0, 0, 0), length FLT_MAX and 0.  Replaces
  * kdop::raycast(ray, tree, mesh, X, Y, Z, surface_map, hit_point, length)
@@ -220,10 +221,11 @@ int kdop_mesh_reorder_gpu(kdop_ctx* ctx, uint32_t nv, const float* X, const floa
  * the super levels culling only chunks whose root the reference's test would
  * reject or prune. */
 int kdop_raycast(kdop_ctx* ctx, uint32_t body, uint32_t n, const float* origins, const float* directions, float* hit_points,
-                 float* lengths, uint8_t* hits);
+                 float* lengths, uint8_t* hits, int is_device);
 
-/* Spheres against one body in its last refitted pose (host arrays: spheres
- * holds x, y, z, radius per sphere).  Contacts go to out grouped by sphere in
+/* Spheres against one body in its last refitted pose (spheres holds x, y, z,
+ * radius per sphere; is_device 0: spheres and out are host arrays, 1: device
+ * arrays of the context's GPU, spheres 16-byte aligned).  Contacts go to out grouped by sphere in
  * sphere order, each sphere's in the order the reference reports them through
  * its ContactsCallback; *n receives the contact count, and KDOP_ERR_CAPACITY
  * (nothing written) means cap < *n.  The contacts are computed in full on
@@ -233,7 +235,7 @@ int kdop_raycast(kdop_ctx* ctx, uint32_t body, uint32_t n, const float* origins,
  * callback, should_flip) (SIMULATION/KDOP/KDOP/include/kdop_single_traversal.h:
  * 123-161) for n spheres at once, with the same contacts in the same order. */
 int kdop_single_traversal(kdop_ctx* ctx, uint32_t body, uint32_t nspheres, const float* spheres, int flip,
-                          kdop_sphere_contact* out, uint64_t cap, uint64_t* n);
+                          kdop_sphere_contact* out, uint64_t cap, uint64_t* n, int is_device);
 
 /* World coordinates of a body after the last refit (3 arrays of nv floats). */
 int kdop_get_world(kdop_ctx* ctx, uint32_t body, float* X, float* Y, float* Z);

@@ -141,13 +141,13 @@ class Context {
    * hits the tree's mesh; else false, (0, 0, 0) and FLT_MAX. */
   bool raycast(const Tree& tree, const float p[3], const float r[3], float hit_point[3], float& length) {
     uint8_t hit = 0;
-    check(kdop_raycast(c_, tree.body, 1, p, r, hit_point, &length, &hit));
+    check(kdop_raycast(c_, tree.body, 1, p, r, hit_point, &length, &hit, 0));
     return hit != 0;
   }
   /* n rays at once (3 floats per origin, direction and hit point). */
   void raycast(const Tree& tree, uint32_t n, const float* origins, const float* directions, float* hit_points, float* lengths,
                uint8_t* hits) {
-    check(kdop_raycast(c_, tree.body, n, origins, directions, hit_points, lengths, hits));
+    check(kdop_raycast(c_, tree.body, n, origins, directions, hit_points, lengths, hits, 0));
   }
 
   /* kdop::single_traversal with a sphere: the callback receives the sphere's
@@ -192,10 +192,10 @@ class Context {
   std::vector<kdop_sphere_contact> spheres_(const Tree& tree, uint32_t n, const float* spheres, bool flip) {
     std::vector<kdop_sphere_contact> out(std::max<size_t>(1, (size_t)n * 16));
     uint64_t cnt = 0;
-    int rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt);
+    int rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt, 0);
     if (rc == KDOP_ERR_CAPACITY) {
       out.resize((size_t)cnt);
-      rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt);
+      rc = kdop_single_traversal(c_, tree.body, n, spheres, flip ? 1 : 0, out.data(), out.size(), &cnt, 0);
     }
     check(rc);
     out.resize((size_t)cnt);

@@ -135,3 +135,33 @@ def test_raycast_one_chunk_bodies(k, n):
         assert want[2].sum() > 0
         for a, w in zip(got, want):
             assert np.asarray(a).tobytes() == np.asarray(w).tobytes()
+
+
+def test_raycast_device_arrays_match_host_arrays():
+    """is_device = 1: rays read from and results written to device memory in
+    place give the host-array call's results byte for byte (and so the
+    oracle's); a misaligned device pointer is an argument error."""
+    import torch
+    rng = np.random.default_rng(41)
+    mesh = K.box_mesh(12, size=1.0, jitter=0.1, seed=7)
+    pose = W.Pose(np.array([0.3, -0.2, 0.1], F32), K.rng_quaternions(5, 1, 1)[0])
+    ctx = K.Context(K=26)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], pose.p[None, :], pose.q[None, :])
+    o, d = _rays(rng, 3000, pose.p.astype(np.float64), 2.0, 0.55)
+    h, l, f = ctx.raycast(b, o, d)
+    hd, ld, fd = ctx.raycast(b, torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda())
+    print("device raycast hits", int(fd.sum()), "of", o.shape[0], flush=True)
+    assert fd.is_cuda and ld.is_cuda and hd.is_cuda
+    assert fd.cpu().numpy().astype(bool).tolist() == f.tolist()
+    assert ld.cpu().numpy().tobytes() == l.tobytes()
+    assert hd.cpu().numpy().tobytes() == h.tobytes()
+    # the second call reuses the context's scratch after a host-array call
+    hd2, ld2, _ = ctx.raycast(b, torch.from_numpy(o[:700]).cuda(), torch.from_numpy(d[:700]).cuda())
+    assert ld2.cpu().numpy().tobytes() == l[:700].tobytes()
+    raw = torch.zeros(3 * 8 + 1, dtype=torch.float32, device="cuda")
+    import ctypes
+    rc = ctx._lib.kdop_raycast(ctx._h, b, 1, raw.data_ptr() + 2, raw.data_ptr(), raw.data_ptr() + 16, raw.data_ptr() + 64,
+                               raw.data_ptr() + 80, 1)
+    assert rc == K._abi.KDOP_ERR_ARG
+    ctx.close()

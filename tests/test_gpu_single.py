@@ -85,11 +85,11 @@ def test_single_traversal_capacity_and_errors():
 def ctypes_count(ctx, b, sp):
     import ctypes
     cnt = ctypes.c_uint64()
-    ctx._check(ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], K.ptr(sp), 0, None, 0, ctypes.byref(cnt)))
+    ctx._check(ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], K.ptr(sp), 0, None, 0, ctypes.byref(cnt), 0))
     out = np.zeros(max(cnt.value - 1, 0), K._abi.SPHERE_CONTACT_DTYPE)
     if cnt.value:  # one record short: a capacity error, nothing written
         rc = ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], K.ptr(sp), 0, K.ptr(out) if out.size else None,
-                                            out.shape[0], ctypes.byref(cnt))
+                                            out.shape[0], ctypes.byref(cnt), 0)
         assert rc == K._abi.KDOP_ERR_CAPACITY if out.size else rc == 0
     return cnt.value
 
@@ -114,3 +114,39 @@ def test_single_traversal_one_chunk_bodies(k, n):
         want = O.single_traversal(tree, (mesh.tets, mesh.X, mesh.Y, mesh.Z, mesh.surface), sp)
         assert want.shape[0] > 0
         assert got.tobytes() == want.tobytes()
+
+
+def test_single_traversal_device_arrays_match_host_arrays():
+    """is_device = 1: spheres read in place and records written to device
+    memory give the host-array call's records byte for byte and in the same
+    order; a device buffer one record short is a capacity error that reports
+    the count, and misaligned device spheres are an argument error."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(77)
+    mesh = K.box_mesh(12, size=1.0, jitter=0.1, seed=4)
+    pose = W.Pose(np.array([-0.2, 0.1, 0.3], F32), K.rng_quaternions(6, 2, 1)[0])
+    ctx = K.Context(K=14)
+    b = ctx.make_tree(mesh)
+    ctx.refit([b], pose.p[None, :], pose.q[None, :])
+    c = pose.p[None, :] + rng.uniform(-0.75, 0.75, size=(3000, 3))
+    r = rng.uniform(0.01, 0.15, size=(3000, 1))
+    sp = np.concatenate([c, r], 1).astype(F32)
+    for flip in (False, True):
+        want = ctx.single_traversal(b, sp, flip=flip)
+        got = ctx.single_traversal(b, torch.from_numpy(sp).cuda(), flip=flip)
+        print("device single traversal flip", flip, "contacts", want.shape[0], flush=True)
+        assert got.is_cuda and want.shape[0] > 1000
+        assert got.cpu().numpy().tobytes() == want.tobytes()
+    dsp = torch.from_numpy(sp).cuda()
+    rec = K._abi.SPHERE_CONTACT_DTYPE.itemsize
+    n = ctypes.c_uint64()
+    short = torch.zeros((want.shape[0] - 1, rec), dtype=torch.uint8, device="cuda")
+    rc = ctx._lib.kdop_single_traversal(ctx._h, b, sp.shape[0], dsp.data_ptr(), 1, short.data_ptr(), short.shape[0],
+                                        ctypes.byref(n), 1)
+    assert rc == K._abi.KDOP_ERR_CAPACITY and n.value == want.shape[0]
+    assert int(short.sum()) == 0  # nothing written
+    raw = torch.zeros(64, dtype=torch.float32, device="cuda")
+    rc = ctx._lib.kdop_single_traversal(ctx._h, b, 2, raw.data_ptr() + 4, 0, short.data_ptr(), short.shape[0], ctypes.byref(n), 1)
+    assert rc == K._abi.KDOP_ERR_ARG
+    ctx.close()
