@@ -76,6 +76,7 @@ struct kdop_ctx {
   DevBuf<kdop_sphere_contact> sphere_out;  // kdop_single_traversal: the contacts
   DevBuf<uint2> sg_items[2];               // kdop_single_traversal: (sphere, node) frontiers
   DevBuf<uint32_t> sg_scan;                // kdop_single_traversal / kdop_raycast: counts, offsets, block sums, total, flag
+  DevBuf<uint8_t> sg_kind;                 // kdop_single_traversal: what each item of a two-level step becomes
   DevBuf<float> rc_best;                   // kdop_raycast: each (ray, chunk) item's best length from FLT_MAX
   DevBuf<uint32_t> rc_first;               // kdop_raycast: each ray's first item, then a flag
   uint32_t sg_max_items = 1u << 28;        // KDOP_OPT_SINGLE_ITEMS
@@ -288,9 +289,10 @@ struct RcSeedCount {
   }
 };
 template <int K>
-struct RcExpandCount {
-  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* o, const float* d, uint32_t* count) {
-    hipLaunchKernelGGL(k_rc_expand_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, o, d, count);
+struct RcExpand2Count {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* o, const float* d, uint32_t* count, uint8_t* kind) {
+    hipLaunchKernelGGL(k_rc_expand2_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, o, d, count,
+                       kind);
   }
 };
 template <int K>
@@ -323,15 +325,16 @@ struct SgSeedCount {
   }
 };
 template <int K>
-struct SgExpandCount {
-  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* spheres, uint32_t* count) {
-    hipLaunchKernelGGL(k_sg_expand_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, spheres, count);
+struct SgExpand2Count {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const float* spheres, uint32_t* count, uint8_t* kind) {
+    hipLaunchKernelGGL(k_sg_expand2_count<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, c->dt, spheres, count,
+                       kind);
   }
 };
 template <int K>
-struct SgExpandWrite {
-  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const uint32_t* count, const uint32_t* off, uint2* out) {
-    hipLaunchKernelGGL(k_sg_expand_write<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, count, off, out);
+struct SgExpand2Write {
+  static void run(kdop_ctx* c, const uint2* items, uint32_t m, const uint8_t* kind, const uint32_t* off, uint2* out) {
+    hipLaunchKernelGGL(k_sg_expand2_write<K>, dim3((m + 255) / 256), dim3(256), 0, c->stream, items, m, c->nodes.p, kind, off, out);
   }
 };
 template <int K>
@@ -569,7 +572,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
-  release(c->redbuf); release(c->prox_io); release(c->sg_items[0]); release(c->sg_items[1]); release(c->sg_scan);
+  release(c->redbuf); release(c->prox_io); release(c->sg_items[0]); release(c->sg_items[1]); release(c->sg_scan); release(c->sg_kind);
   release(c->rc_best); release(c->rc_first);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
@@ -1606,7 +1609,9 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
   uint32_t lg = 0;
   while ((1u << lg) < b.L) ++lg;
   if (lg + 1u > kChunkStack) return fail(c, KDOP_ERR_ARG, "kdop_raycast: chunk trees too deep (%u leaves)", b.L);
-  const uint32_t steps = (b.H - 1u) * std::max(lg, 1u);  // below the top-level roots, every item is a branch-chunk root after these
+  // below the top-level roots every item is a branch-chunk root after (H - 1)
+  // log2 L levels, taken two per step
+  const uint32_t steps = ((b.H - 1u) * std::max(lg, 1u) + 1u) / 2u;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   if (is_device && ((reinterpret_cast<uintptr_t>(origins) | reinterpret_cast<uintptr_t>(directions) | reinterpret_cast<uintptr_t>(hit_points) |
                      reinterpret_cast<uintptr_t>(lengths)) & 3u))
@@ -1665,7 +1670,8 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
     int cur = 0;
     for (uint32_t step = 0; !too_big && m && step < steps; ++step) {
       if ((rc = scratch(m))) return rc;
-      dispatch_K<RcExpandCount>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bo, bd, c->sg_scan.p);
+      if ((rc = ensure(c, c->sg_kind, m))) return rc;
+      dispatch_K<RcExpand2Count>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bo, bd, c->sg_scan.p, c->sg_kind.p);
       HIPCHK(c, hipGetLastError());
       if ((rc = scan(m))) return rc;
       uint32_t m2 = 0;
@@ -1673,8 +1679,8 @@ int kdop_raycast(kdop_ctx* c, uint32_t body, uint32_t n, const float* origins, c
       if (m2 > max_items) { too_big = true; break; }
       if (m2) {
         if ((rc = ensure(c, c->sg_items[cur ^ 1], m2))) return rc;
-        dispatch_K<SgExpandWrite>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint32_t*)c->sg_scan.p,
-                                  (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
+        dispatch_K<SgExpand2Write>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint8_t*)c->sg_kind.p,
+                                   (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
         HIPCHK(c, hipGetLastError());
       }
       cur ^= 1;
@@ -1739,7 +1745,9 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
   const BodyHost& b = c->bodies[body];
   uint32_t lg = 0;
   while ((1u << lg) < b.L) ++lg;
-  const uint32_t steps = b.H * std::max(lg, 1u);  // below the top-level chunk roots, every item is a leaf after these
+  // below the top-level chunk roots every item is a leaf after H log2 L levels,
+  // taken two per step
+  const uint32_t steps = (b.H * std::max(lg, 1u) + 1u) / 2u;
   HIPCHK(c, hipSetDevice(c->cfg.device));
   int rc = KDOP_OK;
   if (is_device && (reinterpret_cast<uintptr_t>(spheres) & 15u))
@@ -1785,7 +1793,8 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
     int cur = 0;
     for (uint32_t step = 0; !too_big && m && step < steps; ++step) {
       if ((rc = scratch(m))) return rc;
-      dispatch_K<SgExpandCount>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bs, c->sg_scan.p);
+      if ((rc = ensure(c, c->sg_kind, m))) return rc;
+      dispatch_K<SgExpand2Count>(c->K, c, (const uint2*)c->sg_items[cur].p, m, bs, c->sg_scan.p, c->sg_kind.p);
       HIPCHK(c, hipGetLastError());
       if ((rc = scan(m))) return rc;
       uint32_t m2 = 0;
@@ -1793,8 +1802,8 @@ int kdop_single_traversal(kdop_ctx* c, uint32_t body, uint32_t n, const float* s
       if (m2 > kMaxItems) { too_big = true; break; }
       if (m2) {
         if ((rc = ensure(c, c->sg_items[cur ^ 1], m2))) return rc;
-        dispatch_K<SgExpandWrite>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint32_t*)c->sg_scan.p,
-                                  (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
+        dispatch_K<SgExpand2Write>(c->K, c, (const uint2*)c->sg_items[cur].p, m, (const uint8_t*)c->sg_kind.p,
+                                   (const uint32_t*)(c->sg_scan.p + m), c->sg_items[cur ^ 1].p);
         HIPCHK(c, hipGetLastError());
       }
       cur ^= 1;

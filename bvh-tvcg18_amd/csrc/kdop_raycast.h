@@ -197,24 +197,42 @@ __global__ __launch_bounds__(256) void k_rc_seed_count(const BodyDev* __restrict
   count[i] = raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) ? B.ntop : 0u;
 }
 
-// A, one step: a super node becomes its two children or nothing; a
-// branch-chunk root (or one already marked) stays, marked.
+// A, one step of two levels (as k_sg_expand2_count for the single traversal,
+// with the slab test culling super nodes and branch-chunk roots as the leaves):
+// a marked item stays; an unmarked branch-chunk root stays, marked; a super
+// node whose slab interval is empty or behind the origin becomes nothing, and
+// otherwise each child in order becomes nothing (its interval is empty or
+// behind), itself marked (a branch-chunk root), or its two children.  count /
+// kind as k_sg_expand2_write reads them.
 template <int K>
-__global__ __launch_bounds__(256) void k_rc_expand_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
-                                                         DirTable dt, const float* __restrict__ orig, const float* __restrict__ dir,
-                                                         uint32_t* __restrict__ count) {
+__device__ inline uint32_t rc_child_kind(const float* __restrict__ nodes, uint32_t node, const DirTable& dt, const f3& o, const f3& r) {
+  NodeRec<K> rec;
+  load_rec<K>(nodes, node, rec);
+  if (rec.m0 & F_BRANCH) return 1u;
+  float t_min, t_max;
+  ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
+  return (t_min > t_max || t_max <= 0.0f) ? 0u : 2u;
+}
+template <int K>
+__global__ __launch_bounds__(256) void k_rc_expand2_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                          DirTable dt, const float* __restrict__ orig, const float* __restrict__ dir,
+                                                          uint32_t* __restrict__ count, uint8_t* __restrict__ kind) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const uint2 it = items[j];
-  if (it.y & kSgTested) { count[j] = 1u; return; }
+  if (it.y & kSgTested) { count[j] = 1u; kind[j] = kSelf; return; }
   NodeRec<K> rec;
   load_rec<K>(nodes, it.y, rec);
-  if (rec.m0 & F_BRANCH) { count[j] = 1u; return; }
+  if (rec.m0 & F_BRANCH) { count[j] = 1u; kind[j] = kSelf; return; }
   f3 o, r;
   load_ray(orig, dir, it.x, o, r);
   float t_min, t_max;
   ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
-  count[j] = (t_min > t_max || t_max <= 0.0f) ? 0u : 2u;
+  if (t_min > t_max || t_max <= 0.0f) { count[j] = 0u; kind[j] = 0u; return; }
+  const uint32_t k0 = rc_child_kind<K>(nodes, rec.m0 & IDX_MASK, dt, o, r);
+  const uint32_t k1 = rc_child_kind<K>(nodes, rec.m1 & IDX_MASK, dt, o, r);
+  count[j] = k0 + k1;
+  kind[j] = (uint8_t)(k0 | (k1 << 2));
 }
 
 // B, first pass: the reference's test of each chunk root (from FLT_MAX);

@@ -225,36 +225,66 @@ __global__ __launch_bounds__(256) void k_sg_seed_write(const BodyDev* __restrict
   for (uint32_t t = 0; t < B.ntop; ++t) items[o + t] = make_uint2(i, tops[B.top_base + t]);
 }
 
-// One step: count[j] = what item j becomes (0, 1 or 2 items).
+// One step takes two levels of the walk: untested item j becomes nothing (its
+// test fails), the tested leaf itself, or, child by child in order (first
+// child first, :91-110), nothing (the child's test fails), the tested child
+// leaf, or the child's two children.  So a node is tested exactly when its
+// parent passed, and the frontier keeps the order described above.
+// count[j] is the number of items j becomes, kind[j] which: kSelf, or 2 bits
+// per child (0 nothing, 1 the tested leaf, 2 its children).  The level in
+// between is never written, scanned or read back.
+constexpr uint8_t kSelf = 0x10;
 template <int K>
-__global__ __launch_bounds__(256) void k_sg_expand_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
-                                                         DirTable dt, const float* __restrict__ spheres, uint32_t* __restrict__ count) {
+__device__ inline uint32_t child_kind(const float* __restrict__ nodes, uint32_t node, const float* sd) {
+  NodeRec<K> rec;
+  load_rec<K>(nodes, node, rec);
+  if (!dop_overlap<K>(rec.v, sd)) return 0u;  // kdop_single_traversal.h:63-64
+  return ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) ? 1u : 2u;
+}
+template <int K>
+__global__ __launch_bounds__(256) void k_sg_expand2_count(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                          DirTable dt, const float* __restrict__ spheres, uint32_t* __restrict__ count,
+                                                          uint8_t* __restrict__ kind) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const uint2 it = items[j];
-  if (it.y & kSgTested) { count[j] = 1u; return; }
+  if (it.y & kSgTested) { count[j] = 1u; kind[j] = kSelf; return; }
   f3 c;
   float r, sd[K];
   sphere_dop<K>(spheres, it.x, dt, c, r, sd);
   NodeRec<K> rec;
   load_rec<K>(nodes, it.y, rec);
-  if (!dop_overlap<K>(rec.v, sd)) { count[j] = 0u; return; }  // kdop_single_traversal.h:63-64
-  count[j] = ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) ? 1u : 2u;
+  if (!dop_overlap<K>(rec.v, sd)) { count[j] = 0u; kind[j] = 0u; return; }
+  if ((rec.m0 & F_BRANCH) && (rec.m0 & F_LEAF)) { count[j] = 1u; kind[j] = kSelf; return; }
+  const uint32_t k0 = child_kind<K>(nodes, rec.m0 & IDX_MASK, sd);  // :91-110, first child first
+  const uint32_t k1 = child_kind<K>(nodes, rec.m1 & IDX_MASK, sd);
+  count[j] = k0 + k1;
+  kind[j] = (uint8_t)(k0 | (k1 << 2));
 }
 template <int K>
-__global__ __launch_bounds__(256) void k_sg_expand_write(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
-                                                         const uint32_t* __restrict__ count, const uint32_t* __restrict__ off,
-                                                         uint2* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sg_expand2_write(const uint2* __restrict__ items, uint32_t m, const float* __restrict__ nodes,
+                                                          const uint8_t* __restrict__ kind, const uint32_t* __restrict__ off,
+                                                          uint2* __restrict__ out) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
-  const uint32_t k = count[j];
-  if (k == 0u) return;
+  const uint32_t kd = kind[j];
+  if (kd == 0u) return;
   const uint2 it = items[j];
-  const uint32_t o = off[j];
-  if (k == 1u) { out[o] = make_uint2(it.x, it.y | kSgTested); return; }
-  const uint2 mm = load_meta<K>(nodes, it.y);  // :91-110 (super nodes: the lower chunks' roots)
-  out[o] = make_uint2(it.x, mm.x & IDX_MASK);
-  out[o + 1] = make_uint2(it.x, mm.y & IDX_MASK);
+  uint32_t o = off[j];
+  if (kd == kSelf) { out[o] = make_uint2(it.x, it.y | kSgTested); return; }
+  const uint2 mm = load_meta<K>(nodes, it.y);
+  const uint32_t ch[2] = {mm.x & IDX_MASK, mm.y & IDX_MASK};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t kh = (kd >> (2 * h)) & 3u;
+    if (kh == 1u) {
+      out[o++] = make_uint2(it.x, ch[h] | kSgTested);
+    } else if (kh == 2u) {
+      const uint2 mc = load_meta<K>(nodes, ch[h]);
+      out[o++] = make_uint2(it.x, mc.x & IDX_MASK);
+      out[o++] = make_uint2(it.x, mc.y & IDX_MASK);
+    }
+  }
 }
 
 // The leaves: the test of a leaf not yet tested, then the contact routine
