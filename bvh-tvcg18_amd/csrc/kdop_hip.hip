@@ -109,6 +109,7 @@ struct kdop_ctx {
   DevBuf<float> poses;
   int num_cus = 0;
   int ls = 512, wpc = 8;
+  int bw = 2;                // waves per persistent workgroup (1: a stack per wave; 2, 4, 8: a shared stack)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
@@ -484,9 +485,25 @@ void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, 
   else
     hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep, last);
 }
+// A persistent round of W-wave workgroups sharing one stack of 512 entries per
+// wave (k_persist_blk); as many waves as the per-wave kernel launches.
+template <int K, int W>
+void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
+  const dim3 grid((unsigned)std::max(1, c->num_cus * c->wpc / W)), block(64 * W);
+  Item* b0 = c->front[0].p;
+  Item* b1 = c->front[1].p;
+  if (c->timing_level >= 2)
+    hipExtLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0,
+                          b1, cap, c->dstate, round, c->keep, last);
+  else
+    hipLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep, last);
+}
 template <int K>
 struct PersistRound {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
+    if (c->bw == 2) { launch_round_blk<K, 2>(c, T, cap, round, last); return; }
+    if (c->bw == 4) { launch_round_blk<K, 4>(c, T, cap, round, last); return; }
+    if (c->bw == 8) { launch_round_blk<K, 8>(c, T, cap, round, last); return; }
     switch (c->ls) {
       case 256: launch_round<K, 256>(c, T, cap, round, last); break;
       case 512: launch_round<K, 512>(c, T, cap, round, last); break;
@@ -641,6 +658,11 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       if (v < 1 || v > 62) break;
       c->bfs_max = (uint32_t)v;
       return KDOP_OK;
+    case KDOP_OPT_BLOCK_WAVES:
+      if (v == 0) v = 2;
+      if (v != 1 && v != 2 && v != 4 && v != 8) break;
+      c->bw = (int)v;
+      return KDOP_OK;
     case KDOP_OPT_SINGLE_ITEMS:
       if (v == 0) v = 1ll << 28;
       if (v < 1 || v > (1ll << 30)) break;
@@ -682,6 +704,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   };
   const std::vector<Knob> knobs = {
       {KDOP_OPT_TRAVERSAL, {KDOP_TRAVERSAL_PERSISTENT, KDOP_TRAVERSAL_LEVELS}},
+      {KDOP_OPT_BLOCK_WAVES, {1, 2, 4}},
       {KDOP_OPT_WAVES_PER_CU, {4, 8, 12}},
       {KDOP_OPT_QUEUE_DEPTH, {256, 512, 1024}},
       {KDOP_OPT_STACK_KEEP, {128, 512, 2048}},
@@ -726,6 +749,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_GRAPHS: *v = c->use_graphs ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
     case KDOP_OPT_SINGLE_ITEMS: *v = c->sg_max_items; return KDOP_OK;
+    case KDOP_OPT_BLOCK_WAVES: *v = c->bw; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
 }
@@ -1183,7 +1207,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dbodies.p, (uint64_t)(uintptr_t)c->dpairs.p, (uint64_t)(uintptr_t)c->dtops.p,
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
-          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus};
+          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus, (uint64_t)c->bw};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;

@@ -82,21 +82,19 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   __shared__ uint32_t raw_total;
   __shared__ P2Stage st2;
   __shared__ uint32_t stopped;
-  // one read per block: block 0 of the launch that stops the levels sets
-  // final_set while other blocks of that launch start, and every wave of a
-  // block must take the same branch (wave 0 computes the prefix sums the
-  // others wait for)
-  if (threadIdx.x == 0) stopped = __hip_atomic_load(&C->final_set, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  if (stopped) return;
+  // Wave 0 reads the stop flag and the level's size (its segment fills) at
+  // once, one round trip, and shares them through LDS.  One read per block:
+  // block 0 of the launch that stops the levels sets final_set while other
+  // blocks of that launch start, and every wave of a block must take the same
+  // branch.  The sizes are used only when the flag is clear.
   const uint32_t segcap_in = cap / nseg_in;
-  uint32_t n;
-  if (nseg_in == 1u) {
-    n = min(C->level_fill[level], cap);
-  } else {
-    if (threadIdx.x < 64u) {  // wave 0: prefix sums of the segment fills
-      const uint32_t lane = threadIdx.x;
-      const uint32_t f = C->lseg[level % 3u][lane].fill;
+  if (threadIdx.x < 64u) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t f = nseg_in == 1u ? (lane == 0u ? C->level_fill[level] : 0u) : C->lseg[level % 3u][lane].fill;
+    if (lane == 0u) stopped = __hip_atomic_load(&C->final_set, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nseg_in == 1u) {
+      if (lane == 0u) { pre[kSegs] = min(f, cap); raw_total = f; }
+    } else {  // prefix sums of the segment fills
       uint32_t x = min(f, segcap_in), t = f;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
@@ -107,9 +105,10 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
       pre[lane + 1u] = x;
       if (lane == 0u) { pre[0] = 0u; raw_total = t; }
     }
-    __syncthreads();
-    n = pre[kSegs];
   }
+  __syncthreads();
+  if (stopped) return;
+  const uint32_t n = pre[kSegs];
   if (n == 0 || n >= target || level >= last) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       C->final_level = level;
@@ -397,6 +396,232 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);
   if (lane == 0) {
+    atomicAdd(&st->iters, (unsigned long long)iters);
+    atomicAdd(&st->lanes, (unsigned long long)lanes);
+    atomicAdd(&st->grabs, (unsigned long long)grabs);
+    atomicMax(&st->max_iters, (unsigned long long)iters);
+  }
+}
+
+// The same round with W waves per workgroup sharing one LDS stack of W * LS
+// items.  Each iteration the block pops up to 64 W items from the top of the
+// stack, topped up with a guided chunk of the frontier, expands one per
+// thread, and pushes the children at block-wide prefix-sum places, so the
+// subtrees of the frontier items the block took are spread over all its
+// waves (with one stack per wave, the wave that took the largest subtrees
+// runs on alone).  Two barriers per iteration; every value that steers the
+// loop is read from LDS after a barrier, so all waves take the same path.
+// Wave 0 keeps the frontier state and decides the next grab between the
+// barriers.  Spills, the tail hand-off and the leaf staging work as in
+// k_persist, per wave.
+template <int K, int LS, int W>
+__global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+                                                        CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last) {
+  constexpr uint32_t NT = 64u * W;
+  constexpr uint32_t S = (uint32_t)LS * W;
+  static_assert((S & (S - 1)) == 0 && LS >= 256, "LDS stack: power of two");
+  static_assert(kSegs == 64, "one lane of wave 0 per segment");
+  constexpr uint32_t SB = 512;  // staged leaf pairs per wave; flushed above SB - 256
+  __shared__ uint32_t sa0[S], sa1[S], sb0[S], sb1[S], sp[S];
+  __shared__ uint32_t lp_pair[W][SB], lp_ta[W][SB], lp_tb[W][SB];
+  __shared__ uint32_t wtot[W];
+  __shared__ uint32_t ctl[3];  // next grab: first frontier index, count; frontier done
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  uint32_t final_level = C->final_level, final_count = 0;
+  if (round == 0) {
+    const bool recorded = C->final_set != 0u;
+    if (!recorded) final_level = last;
+    final_count = recorded ? C->final_count : min(C->level_fill[last], cap);
+    if (!recorded && blockIdx.x == 0 && tid == 0) {
+      C->final_level = final_level;
+      C->final_count = final_count;
+    }
+  }
+  const bool odd = ((final_level + round) & 1u) != 0u;
+  const Item* __restrict__ in = odd ? buf1 : buf0;
+  Item* __restrict__ spill = odd ? buf0 : buf1;
+  const uint32_t segcap = cap / kSegs;
+  const uint32_t bps = max(1u, gridDim.x / (uint32_t)kSegs);  // blocks per segment
+  const uint32_t home = blockIdx.x & (kSegs - 1);
+  // wave 0 only: lane s holds segment s of this round's frontier
+  uint32_t seg_base = 0, seg_len = 0, cur = home, next_est = 0;
+  unsigned long long known = ~0ull;
+  bool wfront_done = true;  // wave 0's view
+  if (w == 0) {
+    if (round == 0) {
+      const uint32_t n = min(final_count, cap);
+      seg_base = (uint32_t)(((unsigned long long)n * lane) / kSegs);
+      seg_len = (uint32_t)(((unsigned long long)n * (lane + 1)) / kSegs) - seg_base;
+    } else {
+      seg_base = lane * segcap;
+      seg_len = min(C->seg[round][lane].fill, segcap);
+    }
+    known = __ballot(seg_len == 0u);
+    wfront_done = known == ~0ull;
+    if (!wfront_done && ((known >> cur) & 1ull)) cur = next_open(known, cur);
+  }
+  uint32_t grabs = 0, wave0_iters = 0;
+  // wave 0: the grab for an iteration whose stack part is ns items; writes ctl
+  auto plan = [&](uint32_t ns) {
+    cur = uniform(cur); next_est = uniform(next_est);
+    uint32_t gstart = 0, ng = 0;
+    bool refresh = false;
+    if (ns < NT && !wfront_done) {
+      const uint32_t len = __builtin_amdgcn_readlane(seg_len, cur), sbase = __builtin_amdgcn_readlane(seg_base, cur);
+      const uint32_t remaining = len > next_est ? len - next_est : 0u;
+      uint32_t want = remaining / (2u * bps);
+      want = want < 1u ? 1u : (want > NT - ns ? NT - ns : want);
+      uint32_t start = 0;
+      if (lane == 0) start = atomicAdd(&C->seg[round][cur].next, want);
+      start = lane0(start);
+      if (start < len) {
+        ++grabs;
+        next_est = start + want;
+        ng = min(want, len - start);
+        gstart = sbase + start;
+        if (start + want >= len) {  // this block took the segment's last chunk
+          if (lane == 0) atomicOr(&C->done[round], 1ull << cur);
+          known |= 1ull << cur;
+        }
+      } else {
+        known |= 1ull << cur;
+        refresh = true;
+      }
+    } else if (!wfront_done && (wave0_iters & 15u) == 15u) {
+      refresh = true;  // a block living off its stack looks at the frontier now and then
+    }
+    if (refresh) {
+      unsigned long long d = 0;
+      if (lane == 0) d = __hip_atomic_load(&C->done[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      known |= ((unsigned long long)lane0((uint32_t)(d >> 32)) << 32) | lane0((uint32_t)d);
+    }
+    if (!wfront_done && ((known >> cur) & 1ull)) {
+      if (known == ~0ull) wfront_done = true;
+      else { cur = next_open(known, cur); next_est = 0; }
+    }
+    if (lane == 0) {
+      ctl[0] = gstart;
+      ctl[1] = ng;
+      ctl[2] = wfront_done ? 1u : 0u;
+    }
+  };
+  if (w == 0) plan(0u);
+  __syncthreads();
+  uint32_t top = 0, bot = 0;  // stack = [bot, top) modulo S, equal in every thread
+  uint32_t nstage = 0;        // this wave's staged leaf pairs
+  const bool keep_p1 = T.p1 != nullptr;
+  unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
+  uint32_t iters = 0, lanes = 0, nsp = 0;
+  for (;;) {
+    const uint32_t gstart = ctl[0], ng = ctl[1];
+    const bool front_done = ctl[2] != 0u;
+    // ---- tail: hand the bottom of a deep stack to the next round, 64 items per
+    // wave, beyond `keep` items per wave (before this iteration's pops and
+    // pushes, so it reads only entries written before the last barrier and
+    // never the slots being rewritten)
+    if (front_done && (top - bot) > keep * W + NT) {
+      const uint32_t s3 = (bot + tid) & (S - 1);
+      const Item mine = Item{sa0[s3], sa1[s3], sb0[s3], sb1[s3], sp[s3]};
+      const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, 64u);
+      b = lane0(b) + lane;
+      if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
+      else atomicOr(T.overflow, 1u);
+      bot += NT;
+    }
+    const uint32_t ns = min(NT, top - bot);
+    if (ns + ng == 0u && front_done) break;  // block-uniform: ctl and the stack bounds are equal in every thread
+    // ---- batch: the top ns stack items, then ng frontier items
+    Item it{0, 0, 0, 0, 0};
+    if (tid < ns) {
+      const uint32_t s = (top - ns + tid) & (S - 1);
+      it = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
+    } else if (tid < ns + ng) {
+      it = in[gstart + tid - ns];
+    }
+    top -= ns;
+    const uint32_t n = ns + ng;
+    Expansion x;
+    x.push = x.leaf = x.surf = 0;
+    x.tests = x.descent = x.broot = 0;
+    if (tid < n) expand_item<K>(T.nodes, it, x);
+    tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
+    ++iters;
+    lanes += n;
+    uint32_t wsum;
+    const uint32_t kpre = wave_prefix_small(__popc(x.push), &wsum);
+    if (lane == 0) wtot[w] = wsum;
+    // ---- leaf pairs: stage in this wave's LDS (all of P1 when it is kept, else only P2)
+    const uint32_t lm = keep_p1 ? x.leaf : x.surf;
+    uint32_t ltot;
+    const uint32_t lpre = wave_prefix_small(__popc(lm), &ltot);
+    if (ltot) {
+      uint32_t o = nstage + lpre;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if ((lm >> k) & 1u) {
+          lp_pair[w][o] = it.pair | (((x.surf >> k) & 1u) << 31);
+          lp_ta[w][o] = x.ta[k >> 1];
+          lp_tb[w][o] = x.tb[k & 1];
+          ++o;
+        }
+      nstage += ltot;
+    }
+    __syncthreads();  // A: the stack reads are done and every wave's push total is in wtot
+    uint32_t woff = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < W; ++v) {
+      const uint32_t t = wtot[v];
+      woff += (uint32_t)v < w ? t : 0u;
+      total += t;
+    }
+    // ---- children: the shared stack when they fit, else the next round (per wave)
+    if (total) {
+      if ((top - bot) + total <= S) {
+        uint32_t o = top + woff + kpre;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((x.push >> k) & 1u) {
+            const uint32_t s2 = (o++) & (S - 1);
+            const Item& t = x.kids[k];
+            sa0[s2] = t.a0; sa1[s2] = t.a1; sb0[s2] = t.b0; sb1[s2] = t.b1; sp[s2] = t.pair;
+          }
+        top += total;
+      } else {
+        const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
+        const uint32_t mine = wtot[w];
+        if (mine) {
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, mine);
+          b = lane0(b);
+          emit_items(x, spill + (size_t)sg * segcap, b + kpre, segcap, T.overflow);
+        }
+      }
+    }
+    if (nstage > SB - 256) {
+      wave_lds_sync();
+      flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
+      nstage = 0;
+    }
+    if (w == 0) {
+      ++wave0_iters;
+      plan(min(NT, top - bot));
+    }
+    __syncthreads();  // B: pushes, hand-offs and the next grab are visible to every wave
+  }
+  wave_lds_sync();
+  if (nstage) flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  const unsigned long long wtests = wave_sum64(tests);
+  if (lane == 0 && wtests) {
+    atomicAdd(&st->tests, wtests);
+    atomicAdd(&st->ptests, wtests);
+  }
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
+  if (tid == 0) {  // schedule statistics per block: iterations, items expanded, grabs
     atomicAdd(&st->iters, (unsigned long long)iters);
     atomicAdd(&st->lanes, (unsigned long long)lanes);
     atomicAdd(&st->grabs, (unsigned long long)grabs);

@@ -22,7 +22,8 @@ KDOP_TRAVERSAL_PERSISTENT = 2
 # kdop_set_option / kdop_get_option (include/kdop_amd.h)
 OPTIONS = {"traversal": 1, "queue_depth": 2, "waves_per_cu": 3, "ramp_target": 4, "stack_keep": 5,
            "seed_budget": 6, "kernel_timing": 7, "bfs_max": 8, "async_refit": 9,
-           "narrow_path": 10, "graphs": 11, "single_items": 12}
+           "narrow_path": 10, "graphs": 11, "single_items": 12,
+           "block_waves": 13}
 KDOP_NARROW = 1
 KDOP_ORDER_SEQUENTIAL = 0
 KDOP_ORDER_OPENCL = 1

@@ -309,6 +309,11 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   kdop_single_traversal before it splits the spheres into   */
                                   /*   batches (0: the default, 2^28; 1 .. 2^30); bounds its     */
                                   /*   device memory at 8 bytes x 2 x this, results unchanged    */
+#define KDOP_OPT_BLOCK_WAVES 13   /* persistent rounds: 2 (0: the default), 4 or 8 waves per     */
+                                  /*   workgroup share one LDS stack of 512 entries per wave      */
+                                  /*   (KDOP_OPT_QUEUE_DEPTH then unused); 1: one stack per wave   */
+                                  /*   of KDOP_OPT_QUEUE_DEPTH entries.  Results unchanged; the    */
+                                  /*   autotuner tries 1, 2 and 4 (not reported in `chosen`)       */
 
 /* Set / read one option (no counterpart in the reference, whose OpenCL
  * path fixes these sizes at compile time).  Returns KDOP_ERR_ARG for an

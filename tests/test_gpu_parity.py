@@ -127,12 +127,31 @@ def test_batched_pairs_one_launch(mode):
 
 @pytest.mark.parametrize("ls,wpc", [(256, 1), (512, 4), (2048, 16)])
 def test_persistent_queue_shapes(ls, wpc):
-    """Persistent kernel with tiny and large LDS stacks / few and many waves:
-    stack overflow into the ring and work stealing must not change results."""
+    """Persistent kernel with one stack per wave, tiny and large LDS stacks /
+    few and many waves: stack overflow into the next round must not change
+    results."""
     wl = W.large_pair(n=10, K=18, seed=13)
     orc, _, _ = oracle_run(wl)
-    dev = device_run(wl, traversal=K.KDOP_TRAVERSAL_PERSISTENT, queue_depth=ls, waves_per_cu=wpc)
+    ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT, queue_depth=ls, waves_per_cu=wpc)
+    ctx.set_option("block_waves", 1)
+    dev = device_run(wl, ctx=ctx)
     _check_pair_results(wl, dev, orc)
+
+
+@pytest.mark.parametrize("bw,wpc,keep", [(2, 8, 512), (4, 8, 512), (8, 8, 512), (2, 2, 16), (4, 16, 64), (8, 4, 1)],
+                         ids=["w2", "w4", "w8", "w2_few_waves_small_keep", "w4_many_waves", "w8_hand_off_all"])
+def test_persistent_shared_stack_blocks(bw, wpc, keep):
+    """Persistent rounds with bw waves per workgroup sharing one LDS stack:
+    the oracle's counts, pair sets and contacts, also when a small keep hands
+    most of each block's stack to the next round and with few or many waves."""
+    for wl in (W.large_pair(n=10, K=18, seed=13), W.batched_pairs(npairs=10, K=26, seed=8, tmin=300, tmax=5000)):
+        orc, _, _ = oracle_run(wl)
+        ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT, waves_per_cu=wpc, keep=keep)
+        ctx.set_option("block_waves", bw)
+        assert ctx.get_option("block_waves") == bw
+        dev = device_run(wl, ctx=ctx)
+        print("bw", bw, "wpc", wpc, "keep", keep, "rounds", ctx.traversal_stats()["rounds"], flush=True)
+        _check_pair_results(wl, dev, orc)
 
 
 def test_duplicate_body_pairs_fall_back_to_levels():
