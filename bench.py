@@ -83,6 +83,7 @@ def headline_leg(args):
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     sched = ctx.traversal_stats()  # of the last timed step
+    sched["block_waves"] = ctx.get_option("block_waves")
     # phase times and the dominant kernel (the persistent rounds): a second
     # timed pass of the same steps with HIP events on the context stream (phase
     # events, and one pair attached to each round's dispatch packet); events
@@ -529,7 +530,8 @@ def main():
     if batched:
         batched.pop("_prox_in", None)
     steps_per_s = 1e3 / head["ms_per_step"]
-    kernel = f"k_persist<{args.k},{head['sched']['queue_depth']}>"
+    bw = head["sched"]["block_waves"]
+    kernel = f"k_persist_blk<{args.k},512,{bw}>" if bw > 1 else f"k_persist<{args.k},{head['sched']['queue_depth']}>"
     traffic = measured_traffic(kernel)
     out = {
         "metric": METRIC,
