@@ -472,6 +472,14 @@ struct Bfs {
 // A persistent round, launched with start / stop events attached to its
 // dispatch packet (the events time exactly this kernel, with no extra packets
 // on the stream).
+// Persistent rounds queued before the first host sync.  Rounds after them
+// drain what is left and hand no stack tail on (only a full stack spills), so
+// a small stack keep cannot push work from round to round until the rounds
+// run out.
+constexpr uint32_t kQueuedRounds = 3;
+constexpr uint32_t kNoHandOff = 1u << 20;  // above any stack; small enough that keep x waves cannot overflow
+inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < kQueuedRounds ? c->keep : kNoHandOff; }
+
 // A persistent round.  With kernel timing on, start / stop events ride on its
 // dispatch packet and time exactly this kernel.
 template <int K, int LS>
@@ -481,9 +489,9 @@ void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, 
   Item* b1 = c->front[1].p;
   if (c->timing_level >= 2)
     hipExtLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0, b1,
-                          cap, c->dstate, round, c->keep, last);
+                          cap, c->dstate, round, round_keep(c, round), last);
   else
-    hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep, last);
+    hipLaunchKernelGGL((k_persist<K, LS>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, round_keep(c, round), last);
 }
 // A persistent round of W-wave workgroups sharing one stack of 512 entries per
 // wave (k_persist_blk); as many waves as the per-wave kernel launches.
@@ -494,9 +502,10 @@ void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t rou
   Item* b1 = c->front[1].p;
   if (c->timing_level >= 2)
     hipExtLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0,
-                          b1, cap, c->dstate, round, c->keep, last);
+                          b1, cap, c->dstate, round, round_keep(c, round), last);
   else
-    hipLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, c->keep, last);
+    hipLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, round_keep(c, round),
+                       last);
 }
 template <int K>
 struct PersistRound {
@@ -720,7 +729,9 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       if (elapsed_ms() > budget_ms) break;
       if (kdop_set_option(c, k.opt, v) != KDOP_OK) continue;
       double ms = 0.0;
-      if ((rc = measure(&ms))) { kdop_set_option(c, k.opt, cur); return rc; }
+      rc = measure(&ms);
+      if (rc == KDOP_ERR_CAPACITY) { rc = KDOP_OK; continue; }  // a candidate the call cannot run with: skip it
+      if (rc) { kdop_set_option(c, k.opt, cur); return rc; }
       if (ms < best) { best = ms; best_v = v; }
     }
     kdop_set_option(c, k.opt, best_v);
@@ -1149,7 +1160,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
   const uint32_t target = c->ramp_target ? c->ramp_target : 16 * pwaves;
-  const uint32_t rounds_async = 3;  // persistent rounds queued before the first host sync
+  const uint32_t rounds_async = kQueuedRounds;
   uint64_t overflowed = 0;
   CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {

@@ -321,11 +321,13 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 int kdop_set_option(kdop_ctx* ctx, int option, int64_t value);
 int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
 
-/* Autotuner: picks the traversal options (strategy, waves per CU, LDS queue
- * depth, stack keep, breadth-first target, seed budget) for this context by
- * timing kdop_traverse(npairs, body_pairs, flags) itself: coordinate descent
- * over the options in that order, each candidate measured as the median host
- * wall time of 3 calls, until budget_ms of tuning time is spent.  The best
+/* Autotuner: picks the traversal options (strategy, waves per persistent
+ * workgroup, waves per CU, LDS queue depth, stack keep, breadth-first target,
+ * seed budget) for this context by timing kdop_traverse(npairs, body_pairs,
+ * flags) itself: coordinate descent over the options in that order, each
+ * candidate measured as the median host wall time of 3 calls (a candidate
+ * that fails with KDOP_ERR_CAPACITY is skipped), until budget_ms of tuning
+ * time is spent.  The best
  * options stay set on the context, and the last traversal's results are those
  * of a run with them.  chosen (may be NULL) receives 8 values indexed by
  * KDOP_OPT_* - 1 (options 1..8); *best_ms (may be NULL) the median time of

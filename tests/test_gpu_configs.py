@@ -41,6 +41,30 @@ def test_c3_batched_subset_full_sizes():
         _check_pair_results(wl, dev, orc)
 
 
+def test_c3_small_stack_keep_completes():
+    """The full 512-pair batched draw under the persistent strategy with a
+    stack keep of 1 and 32 (most of every stack handed to the next round),
+    one stack per wave and two waves per stack: the call completes (the
+    rounds drained after the queued ones hand nothing on) and gives the
+    level-synchronous strategy's counts, surface pairs and contacts."""
+    wl = W.batched_pairs(npairs=512, K=8, seed=11)
+    ref = device_run(wl, keep_all=False, traversal=K.KDOP_TRAVERSAL_LEVELS)
+    rc = ref["counters"]
+    key = lambda a: sorted(a[i:i + 1].tobytes() for i in range(a.shape[0]))
+    ref_p2, ref_c = key(ref["p2"]), key(ref["ctx"].contacts())
+    for bw, keep in ((1, 1), (2, 1), (2, 32)):
+        ctx = K.Context(K=8, traversal=K.KDOP_TRAVERSAL_PERSISTENT, keep=keep)
+        ctx.set_option("block_waves", bw)
+        dev = device_run(wl, keep_all=False, ctx=ctx)
+        c = dev["counters"]
+        print("bw", bw, "keep", keep, "rounds", ctx.traversal_stats()["rounds"], "tests", c["tests"], flush=True)
+        for f in ("tests", "leaf_pairs", "surface_pairs", "contacts", "narrow_pairs", "overflow"):
+            assert c[f] == rc[f], f
+        assert key(dev["p2"]) == ref_p2
+        assert key(ctx.contacts()) == ref_c
+        ctx.close()
+
+
 def test_c4_refit_series():
     """64 overlapping pairs (20k-80k tets, K=18), 200 steps of rigid motion,
     refit (never rebuilt) and re-traversed every step on the device; the
