@@ -453,9 +453,13 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
 // --------------------------------------------------------------- traversal
 template <int K>
 struct Seed {
-  static void run(kdop_ctx* c, const TravArgs& T, uint32_t total, uint32_t* fill, uint32_t cap) {
+  // expand: write level 1 (the seeds' expansion, into front[1]) instead of level 0
+  static void run(kdop_ctx* c, const TravArgs& T, uint32_t total, uint32_t* fill, uint32_t cap, bool expand) {
     const uint32_t blocks = std::max(1u, std::min(4096u, (total + 255) / 256));
-    hipLaunchKernelGGL(k_seed<K>, dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[0].p, fill, cap);
+    if (expand)
+      hipLaunchKernelGGL((k_seed<K, true>), dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[1].p, fill, cap);
+    else
+      hipLaunchKernelGGL((k_seed<K, false>), dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[0].p, fill, cap);
   }
 };
 // A breadth-first level: the persistent strategy's short ramp keeps each level
@@ -1182,16 +1186,22 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       else levels_queued = std::min(levels, level_guess);
     }
     const bool one_pass = narrow_one_pass(c, p2_guess);
+    // the persistent ramp's first level goes into the seed kernel when a level
+    // is queued at all and the seed pairs are at most a few ramp widths (the
+    // tests made are the same either way; when level 0 alone would have ended
+    // the ramp, round 0 starts one level deeper)
+    const bool seed_expands = persistent && npairs && last >= 1u && total_seeds <= 4ull * target;
     // the launches up to the summary copy: recorded once into a HIP graph per
     // set of launch arguments and replayed (host submission of a dozen small
     // kernels otherwise leaves the GPU waiting); direct with timing events
     auto enqueue_first = [&]() -> int {
       if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
       if (npairs) {
-        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[0], cap);
+        dispatch_K<Seed>(c->K, c, T, (uint32_t)total_seeds, &d->level_fill[seed_expands ? 1 : 0], cap, seed_expands);
         if (persistent) {
-          // levels 0 .. last-1; round 0 takes level `last` when no launch stopped earlier
-          for (uint32_t l = 0; l < last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last, false);
+          // levels 0 (or 1 when the seeds expanded) .. last-1; round 0 takes
+          // level `last` when no launch stopped earlier
+          for (uint32_t l = seed_expands ? 1u : 0u; l < last; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, target, last, false);
           if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
           for (uint32_t r = 0; r < rounds_async; ++r) dispatch_K<PersistRound>(c->K, c, T, cap, r, last);
         } else {

@@ -813,12 +813,15 @@ struct SeedDesc {
 // test is kdop_tandem_traversal.h:164-165).  Seeding below the top level
 // only skips tests among super-chunk nodes, which the reference never
 // makes; the pairs found and every test at or below the branch roots are
-// unchanged.
-template <int K>
+// unchanged.  EXPAND: a seed pair that overlaps is expanded at once, as the
+// first breadth-first level would expand it (its child pairs tested, the
+// overlapping ones written as items of level 1), which saves that level's
+// launch; the tests made are the same.
+template <int K, bool EXPAND>
 __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __restrict__ sd, uint32_t total, Item* __restrict__ out,
                                              uint32_t* __restrict__ out_fill, uint32_t out_cap) {
   __shared__ uint32_t red[5];
-  unsigned long long tests = 0, broot = 0, p1s = 0, roots = 0, bpairs = 0;
+  unsigned long long tests = 0, broot = 0, p1s = 0, roots = 0, bpairs = 0, descent = 0;
   for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const uint32_t g = base + threadIdx.x;
     Expansion x;
@@ -866,14 +869,26 @@ __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __rest
         }
       }
     }
-    const uint32_t o = block_reserve(out_fill, x.push, red);
-    emit_items(x, out, o, out_cap, T.overflow);
     p1s += x.leaf;
     emit_leaves(T, pair, x);
+    if (EXPAND) {
+      Expansion x2;
+      x2.push = x2.leaf = x2.surf = 0;
+      x2.tests = x2.descent = x2.broot = 0;
+      if (x.push) expand_item<K>(T.nodes, x.kids[0], x2);
+      tests += x2.tests; descent += x2.descent; broot += x2.broot; p1s += __popc(x2.leaf);
+      const uint32_t o = block_reserve(out_fill, __popc(x2.push), red);
+      emit_items(x2, out, o, out_cap, T.overflow);
+      emit_leaves(T, pair, x2);
+    } else {
+      const uint32_t o = block_reserve(out_fill, x.push, red);
+      emit_items(x, out, o, out_cap, T.overflow);
+    }
   }
   CounterStripe* st = stripe_of(T.stripes, wave_global());
   wave_add64(&st->root_tests, roots);
   wave_add64(&st->tests, tests + roots);
+  wave_add64(&st->descent, descent);
   wave_add64(&st->branch_pairs, bpairs);
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);
