@@ -101,7 +101,14 @@ def headline_leg(args):
         ptests += st["persist_tests"]
         launches += st["persist_launches"]
         r = tm["round_ms"][:st["persist_launches"]]
-        round_ms = r if round_ms is None or len(round_ms) != len(r) else [a + b for a, b in zip(round_ms, r)]
+        if round_ms is None:
+            round_ms, round_n = [], []
+        for i, x in enumerate(r):  # each round's mean over the calls that launched it
+            if i == len(round_ms):
+                round_ms.append(0.0)
+                round_n.append(0)
+            round_ms[i] += x
+            round_n[i] += 1
     torch.cuda.synchronize()
     ctx.set_option("kernel_timing", 0)
     trav_ms /= args.steps
@@ -114,7 +121,7 @@ def headline_leg(args):
     return dict(wl=wl, ctx=ctx, ids=ids, counters=counters, ms_per_step=ms_per_step, trav_ms=trav_ms,
                 narrow_ms=narrow_ms, per_launch_bytes=per_launch_bytes, per_launch_ms=per_launch_ms,
                 launches_per_step=launches / args.steps, achieved_gbs=achieved, sched=sched,
-                round_ms=[x / args.steps for x in round_ms] if round_ms else None)
+                round_ms=[x / n for x, n in zip(round_ms, round_n)] if round_ms else None)
 
 
 def measured_traffic(kernel):

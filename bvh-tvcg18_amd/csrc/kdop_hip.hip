@@ -113,6 +113,9 @@ struct kdop_ctx {
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
+  uint32_t rounds_hint = 3;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
+  uint32_t rounds_calm = 0;    // consecutive calls that needed fewer rounds than rounds_hint
+  uint32_t rounds_queued = 3;  // persistent rounds queued by the current call
   bool async_refit = false;    // refit returns once queued (the next call orders after it)
   bool state_clean = false;    // CallState zeroed (by the last call's final fold)
   std::vector<uint32_t> plan_key;   // bodies of the refit plan in c->plan
@@ -476,13 +479,16 @@ struct Bfs {
 // A persistent round, launched with start / stop events attached to its
 // dispatch packet (the events time exactly this kernel, with no extra packets
 // on the stream).
-// Persistent rounds queued before the first host sync.  Rounds after them
-// drain what is left and hand no stack tail on (only a full stack spills), so
-// a small stack keep cannot push work from round to round until the rounds
-// run out.
+// Persistent rounds queued before the first host sync: as many as the
+// context's recent calls needed (at most kQueuedRounds; raised at once, lowered
+// after 8 calls in a row needed fewer), so that a query whose round 0 spills
+// nothing launches no empty rounds.  Rounds after them drain
+// what is left and hand no stack tail on (only a full stack spills), so a
+// small stack keep cannot push work from round to round until the rounds run
+// out.
 constexpr uint32_t kQueuedRounds = 3;
 constexpr uint32_t kNoHandOff = 1u << 20;  // above any stack; small enough that keep x waves cannot overflow
-inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < kQueuedRounds ? c->keep : kNoHandOff; }
+inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOff; }
 
 // A persistent round.  With kernel timing on, start / stop events ride on its
 // dispatch packet and time exactly this kernel.
@@ -1164,7 +1170,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
   const uint32_t target = c->ramp_target ? c->ramp_target : 16 * pwaves;
-  const uint32_t rounds_async = kQueuedRounds;
+  const uint32_t rounds_async = std::min(std::max(c->rounds_hint, 1u), kQueuedRounds);
+  c->rounds_queued = rounds_async;
   uint64_t overflowed = 0;
   CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {
@@ -1228,7 +1235,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dbodies.p, (uint64_t)(uintptr_t)c->dpairs.p, (uint64_t)(uintptr_t)c->dtops.p,
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
-          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus, (uint64_t)c->bw};
+          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus, (uint64_t)c->bw, rounds_async};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;
@@ -1331,6 +1338,19 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->last_persistent = persistent;
     c->last_npairs = npairs;
     c->last_p2 = f.p2;
+    if (persistent && npairs) {  // rounds this query needed: queue as many next time
+      uint32_t needed = 1;
+      for (uint32_t r = 1; r < (uint32_t)kMaxRounds; ++r)
+        if (h.round_total[r] > 0) needed = r + 1;
+      needed = std::min(needed, kQueuedRounds);
+      if (needed >= c->rounds_hint) {
+        c->rounds_hint = needed;
+        c->rounds_calm = 0;
+      } else if (++c->rounds_calm >= 8u) {
+        c->rounds_hint -= 1u;
+        c->rounds_calm = 0;
+      }
+    }
     c->last_pairs = pairs;
     c->have_traversal = true;
     c->have_p1 = keep_p1;

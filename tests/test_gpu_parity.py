@@ -266,3 +266,31 @@ def test_levels_first_call_regrows_segmented_frontier():
     assert again["counters"]["overflow"] == 0
     np.testing.assert_array_equal(sort_pairs(again["p1"]), sort_pairs(dev["p1"]))
     np.testing.assert_array_equal(sort_pairs(again["p2"]), sort_pairs(dev["p2"]))
+
+
+def test_queued_rounds_follow_recent_calls():
+    """The persistent strategy queues as many rounds as recent calls needed.
+    One context alternates between a pair that needs one round and calls
+    that spill over several (a stack keep of 1): every call, whether its
+    rounds were queued or drained one by one, gives the level-synchronous
+    strategy's counts, surface pairs and contacts, including the calls
+    right after the number of rounds changed and graph replays."""
+    wl = W.batched_pairs(npairs=16, K=14, seed=21, tmin=2000, tmax=40000)
+    key = lambda a: sorted(a[i:i + 1].tobytes() for i in range(a.shape[0]))
+    ref = device_run(wl, keep_all=False, traversal=K.KDOP_TRAVERSAL_LEVELS)
+    want = (ref["counters"], key(ref["p2"]), key(ref["ctx"].contacts()))
+    ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+    ids = [ctx.make_tree(m) for m in wl.meshes]
+    ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
+    pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
+    seen = set()
+    for keep in [512] * 12 + [1] * 3 + [512] * 12:
+        ctx.set_option("stack_keep", keep)
+        c = ctx.tandem_traversal(pairs, narrow=True)
+        seen.add(ctx.traversal_stats()["rounds"])
+        for f in ("tests", "leaf_pairs", "surface_pairs", "contacts", "narrow_pairs", "overflow"):
+            assert c[f] == want[0][f], (keep, f)
+        assert key(ctx.pairs(1)) == want[1]
+        assert key(ctx.contacts()) == want[2]
+    print("rounds seen", sorted(seen), flush=True)
+    assert len(seen) >= 2
