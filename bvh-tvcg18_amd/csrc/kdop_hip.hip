@@ -113,7 +113,7 @@ struct kdop_ctx {
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
-  uint32_t rounds_hint = 3;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
+  uint32_t rounds_hint = 1;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
   uint32_t rounds_calm = 0;    // consecutive calls that needed fewer rounds than rounds_hint
   uint32_t rounds_queued = 3;  // persistent rounds queued by the current call
   bool async_refit = false;    // refit returns once queued (the next call orders after it)
@@ -480,9 +480,10 @@ struct Bfs {
 // dispatch packet (the events time exactly this kernel, with no extra packets
 // on the stream).
 // Persistent rounds queued before the first host sync: as many as the
-// context's recent calls needed (at most kQueuedRounds; raised at once, lowered
-// after 8 calls in a row needed fewer), so that a query whose round 0 spills
-// nothing launches no empty rounds.  Rounds after them drain
+// context's recent calls needed (one at first; at most kQueuedRounds; raised
+// at once, lowered after 8 calls in a row needed fewer), so that a query whose
+// round 0 spills nothing launches no empty rounds.  A call that needs more
+// drains them one by one after a host sync.  Rounds after them drain
 // what is left and hand no stack tail on (only a full stack spills), so a
 // small stack keep cannot push work from round to round until the rounds run
 // out.
