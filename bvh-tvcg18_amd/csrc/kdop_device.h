@@ -109,9 +109,10 @@ struct alignas(256) CounterStripe {
   unsigned long long lanes;          // persistent: items expanded
   unsigned long long grabs;          // persistent: frontier chunks taken
   unsigned long long ptests;         // persistent: node-pair tests made by k_persist
+  unsigned long long handoffs;       // persistent: items the tail hand-off passed to the next round
   unsigned long long max_iters;      // persistent: most iterations of one wave (max, not sum; last field)
 };
-constexpr int kStripeFields = 12;
+constexpr int kStripeFields = 13;
 
 // Segments of the persistent traversal's frontier: each round's input is
 // split into kSegs segments with their own chunk counters (one lane of a

@@ -26,6 +26,10 @@
 
 using namespace kdop_amd;
 
+// Stack keep that never hands work on: above any stack, and small enough that
+// keep x waves per workgroup cannot overflow.
+constexpr uint32_t kNoHandOffKeep = 1u << 20;
+
 namespace {
 
 struct BodyHost {
@@ -108,14 +112,17 @@ struct kdop_ctx {
   DevBuf<uint32_t> plan;
   DevBuf<float> poses;
   int num_cus = 0;
-  int ls = 512, wpc = 8;
-  int bw = 2;                // waves per persistent workgroup (1: a stack per wave; 2, 4, 8: a shared stack)
+  // persistent launch shape as set (0: the default of the kernel that runs; see ls_of / wpc_of / bw_of)
+  int ls = 0, wpc = 0;
+  int bw = 0;                // waves per persistent workgroup (1: a stack per wave; 2, 4, 8: a shared stack)
+  int item_lanes = 4;        // lanes per work item in the persistent rounds (4: k_persist_quad; 1: k_persist / k_persist_blk)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
   uint32_t rounds_hint = 1;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
   uint32_t rounds_calm = 0;    // consecutive calls that needed fewer rounds than rounds_hint
   uint32_t rounds_queued = 3;  // persistent rounds queued by the current call
+  uint32_t max_rounds = kMaxRounds - 1;  // KDOP_OPT_MAX_ROUNDS: persistent rounds before the levels finish a spill
   bool async_refit = false;    // refit returns once queued (the next call orders after it)
   bool state_clean = false;    // CallState zeroed (by the last call's final fold)
   std::vector<uint32_t> plan_key;   // bodies of the refit plan in c->plan
@@ -125,16 +132,25 @@ struct kdop_ctx {
   size_t seed_key_bodies = 0;
   uint64_t seed_total = 0;
   std::vector<uint64_t> seed_plan_est;  // per pair: seeds tested
-  uint32_t keep = 512;       // items a wave keeps once the frontier is gone
+  uint32_t keep = kNoHandOffKeep;  // items a wave keeps once the frontier is gone (default: no hand-off)
   uint32_t last_rounds = 0;
   bool last_persistent = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t evr[2 * kMaxRounds] = {};  // start / end of each persistent round
   double timing[4 + kMaxRounds] = {};  // see kdop_last_timing
   uint32_t last_levels = 0;
+  uint32_t last_fallback_levels = 0;  // levels launched after the persistent rounds ran out (last call)
 };
 
 namespace {
+
+// Persistent launch shape in effect: the option's value, or the default of the
+// kernel the item-lane setting selects (quad kernel: 4-wave workgroups, 24
+// waves per CU, 256 stack entries per wave; one lane per item: 2-wave
+// workgroups, 8 waves per CU, 512 entries per wave).
+int ls_of(const kdop_ctx* c) { return c->ls ? c->ls : (c->item_lanes == 4 ? 256 : 512); }
+int wpc_of(const kdop_ctx* c) { return c->wpc ? c->wpc : (c->item_lanes == 4 ? 24 : 8); }
+int bw_of(const kdop_ctx* c) { return c->bw ? c->bw : (c->item_lanes == 4 ? 4 : 2); }
 
 int fail(kdop_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -488,14 +504,13 @@ struct Bfs {
 // small stack keep cannot push work from round to round until the rounds run
 // out.
 constexpr uint32_t kQueuedRounds = 3;
-constexpr uint32_t kNoHandOff = 1u << 20;  // above any stack; small enough that keep x waves cannot overflow
-inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOff; }
+inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOffKeep; }
 
 // A persistent round.  With kernel timing on, start / stop events ride on its
 // dispatch packet and time exactly this kernel.
 template <int K, int LS>
 void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
-  const dim3 grid((unsigned)(c->num_cus * c->wpc)), block(64);
+  const dim3 grid((unsigned)(c->num_cus * wpc_of(c))), block(64);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
   if (c->timing_level >= 2)
@@ -508,7 +523,7 @@ void launch_round(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, 
 // wave (k_persist_blk); as many waves as the per-wave kernel launches.
 template <int K, int W>
 void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
-  const dim3 grid((unsigned)std::max(1, c->num_cus * c->wpc / W)), block(64 * W);
+  const dim3 grid((unsigned)std::max(1, c->num_cus * wpc_of(c) / W)), block(64 * W);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
   if (c->timing_level >= 2)
@@ -518,13 +533,35 @@ void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t rou
     hipLaunchKernelGGL((k_persist_blk<K, 512, W>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, round_keep(c, round),
                        last);
 }
+// A persistent round of W-wave workgroups with 4 lanes per work item
+// (k_persist_quad) and a shared stack of 2^SL items.
+template <int K, int W, int SL>
+void launch_round_quad(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
+  const dim3 grid((unsigned)std::max(1, c->num_cus * wpc_of(c) / W)), block(64 * W);
+  Item* b0 = c->front[0].p;
+  Item* b1 = c->front[1].p;
+  if (c->timing_level >= 2)
+    hipExtLaunchKernelGGL((k_persist_quad<K, W, SL>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0,
+                          b1, cap, c->dstate, round, round_keep(c, round), last);
+  else
+    hipLaunchKernelGGL((k_persist_quad<K, W, SL>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, round_keep(c, round),
+                       last);
+}
 template <int K>
 struct PersistRound {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
-    if (c->bw == 2) { launch_round_blk<K, 2>(c, T, cap, round, last); return; }
-    if (c->bw == 4) { launch_round_blk<K, 4>(c, T, cap, round, last); return; }
-    if (c->bw == 8) { launch_round_blk<K, 8>(c, T, cap, round, last); return; }
-    switch (c->ls) {
+    const int bw = bw_of(c), ls = ls_of(c);
+    if (c->item_lanes == 4) {  // stack of KDOP_OPT_QUEUE_DEPTH (256 or 512) entries per wave
+      const bool deep = ls >= 512;
+      if (bw <= 2) { if (deep) launch_round_quad<K, 2, 10>(c, T, cap, round, last); else launch_round_quad<K, 2, 9>(c, T, cap, round, last); }
+      else if (bw == 4) { if (deep) launch_round_quad<K, 4, 11>(c, T, cap, round, last); else launch_round_quad<K, 4, 10>(c, T, cap, round, last); }
+      else launch_round_quad<K, 8, 11>(c, T, cap, round, last);
+      return;
+    }
+    if (bw == 2) { launch_round_blk<K, 2>(c, T, cap, round, last); return; }
+    if (bw == 4) { launch_round_blk<K, 4>(c, T, cap, round, last); return; }
+    if (bw == 8) { launch_round_blk<K, 8>(c, T, cap, round, last); return; }
+    switch (ls) {
       case 256: launch_round<K, 256>(c, T, cap, round, last); break;
       case 512: launch_round<K, 512>(c, T, cap, round, last); break;
       case 2048: launch_round<K, 2048>(c, T, cap, round, last); break;
@@ -634,13 +671,11 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->cfg.traversal = (int32_t)v;
       return KDOP_OK;
     case KDOP_OPT_QUEUE_DEPTH:
-      if (v == 0) v = 512;
-      if (v != 256 && v != 512 && v != 1024 && v != 2048) break;
+      if (v != 0 && v != 256 && v != 512 && v != 1024 && v != 2048) break;
       c->ls = (int)v;
       return KDOP_OK;
     case KDOP_OPT_WAVES_PER_CU:
-      if (v == 0) v = 8;
-      if (v < 1 || v > 32) break;
+      if (v < 0 || v > 32) break;
       c->wpc = (int)v;
       return KDOP_OK;
     case KDOP_OPT_RAMP_TARGET:
@@ -648,9 +683,8 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->ramp_target = (uint32_t)v;
       return KDOP_OK;
     case KDOP_OPT_STACK_KEEP:
-      if (v == 0) v = 512;
-      if (v < 1 || v > (1ll << 20)) break;
-      c->keep = (uint32_t)v;
+      if (v < 0 || v > (1ll << 20)) break;
+      c->keep = v == 0 ? kNoHandOffKeep : (uint32_t)v;
       return KDOP_OK;
     case KDOP_OPT_SEED_BUDGET:
       if (v == 0) v = 65536;
@@ -679,9 +713,18 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->bfs_max = (uint32_t)v;
       return KDOP_OK;
     case KDOP_OPT_BLOCK_WAVES:
-      if (v == 0) v = 2;
-      if (v != 1 && v != 2 && v != 4 && v != 8) break;
+      if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) break;
       c->bw = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_MAX_ROUNDS:
+      if (v == 0) v = kMaxRounds - 1;
+      if (v < 1 || v > kMaxRounds - 1) break;
+      c->max_rounds = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_ITEM_LANES:
+      if (v == 0) v = 4;
+      if (v != 1 && v != 4) break;
+      c->item_lanes = (int)v;
       return KDOP_OK;
     case KDOP_OPT_SINGLE_ITEMS:
       if (v == 0) v = 1ll << 28;
@@ -717,21 +760,31 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (rc) return rc;
   double best = 0.0;
   if ((rc = measure(&best))) return rc;
-  const int64_t waves = (int64_t)c->num_cus;
-  struct Knob {
-    int opt;
-    std::vector<int64_t> values;
+  // candidate values per option; the launch-shape ones depend on the lanes
+  // per item chosen before them (the quad kernel runs more, wider workgroups)
+  auto candidates = [&](int opt) -> std::vector<int64_t> {
+    const bool quad = c->item_lanes == 4;
+    const int64_t pw = (int64_t)c->num_cus * wpc_of(c);  // persistent waves in effect
+    const int64_t ls = ls_of(c);
+    switch (opt) {
+      case KDOP_OPT_TRAVERSAL: return {KDOP_TRAVERSAL_PERSISTENT, KDOP_TRAVERSAL_LEVELS};
+      case KDOP_OPT_ITEM_LANES: return {4, 1};
+      case KDOP_OPT_BLOCK_WAVES: return quad ? std::vector<int64_t>{2, 4, 8} : std::vector<int64_t>{1, 2, 4};
+      case KDOP_OPT_WAVES_PER_CU: return quad ? std::vector<int64_t>{12, 16, 24} : std::vector<int64_t>{4, 8, 12};
+      case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{256, 512, 1024};
+      // no hand-off, or a keep that hands on the bottom of deep stacks
+      case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
+      case KDOP_OPT_RAMP_TARGET: return {4 * pw, 16 * pw, 64 * pw};
+      default: return {16384, 65536, 262144};  // KDOP_OPT_SEED_BUDGET
+    }
   };
-  const std::vector<Knob> knobs = {
-      {KDOP_OPT_TRAVERSAL, {KDOP_TRAVERSAL_PERSISTENT, KDOP_TRAVERSAL_LEVELS}},
-      {KDOP_OPT_BLOCK_WAVES, {1, 2, 4}},
-      {KDOP_OPT_WAVES_PER_CU, {4, 8, 12}},
-      {KDOP_OPT_QUEUE_DEPTH, {256, 512, 1024}},
-      {KDOP_OPT_STACK_KEEP, {128, 512, 2048}},
-      {KDOP_OPT_RAMP_TARGET, {4 * waves * 8, 16 * waves * 8, 64 * waves * 8}},
-      {KDOP_OPT_SEED_BUDGET, {16384, 65536, 262144}},
-  };
-  for (const Knob& k : knobs) {
+  const int order[] = {KDOP_OPT_TRAVERSAL, KDOP_OPT_ITEM_LANES, KDOP_OPT_BLOCK_WAVES, KDOP_OPT_WAVES_PER_CU,
+                       KDOP_OPT_QUEUE_DEPTH, KDOP_OPT_STACK_KEEP, KDOP_OPT_RAMP_TARGET, KDOP_OPT_SEED_BUDGET};
+  for (const int opt : order) {
+    struct {
+      int opt;
+      std::vector<int64_t> values;
+    } k{opt, candidates(opt)};
     int64_t cur = 0;
     kdop_get_option(c, k.opt, &cur);
     int64_t best_v = cur;
@@ -751,7 +804,9 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   if ((rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr))) return rc;  // results of the chosen options
   if (chosen)
-    for (int o = 1; o <= 8; ++o) kdop_get_option(c, o, &chosen[o - 1]);  // the traversal options
+    for (int o = 1; o <= KDOP_AUTOTUNE_CHOSEN; ++o)  // every option by number (0 for an unknown number)
+      if (kdop_get_option(c, o, &chosen[o - 1]) != KDOP_OK) chosen[o - 1] = 0;
+  c->err.clear();
   if (best_ms) *best_ms = best;
   return KDOP_OK;
 }
@@ -760,10 +815,10 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
   if (!c || !v) return KDOP_ERR_ARG;
   switch (option) {
     case KDOP_OPT_TRAVERSAL: *v = c->cfg.traversal; return KDOP_OK;
-    case KDOP_OPT_QUEUE_DEPTH: *v = c->ls; return KDOP_OK;
-    case KDOP_OPT_WAVES_PER_CU: *v = c->wpc; return KDOP_OK;
+    case KDOP_OPT_QUEUE_DEPTH: *v = ls_of(c); return KDOP_OK;
+    case KDOP_OPT_WAVES_PER_CU: *v = wpc_of(c); return KDOP_OK;
     case KDOP_OPT_RAMP_TARGET: *v = c->ramp_target; return KDOP_OK;
-    case KDOP_OPT_STACK_KEEP: *v = c->keep; return KDOP_OK;
+    case KDOP_OPT_STACK_KEEP: *v = c->keep >= kNoHandOffKeep ? 0 : c->keep; return KDOP_OK;
     case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget; return KDOP_OK;
     case KDOP_OPT_KERNEL_TIMING: *v = c->timing_level; return KDOP_OK;
     case KDOP_OPT_ASYNC_REFIT: *v = c->async_refit ? 1 : 0; return KDOP_OK;
@@ -771,7 +826,9 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_GRAPHS: *v = c->use_graphs ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_BFS_MAX: *v = c->bfs_max; return KDOP_OK;
     case KDOP_OPT_SINGLE_ITEMS: *v = c->sg_max_items; return KDOP_OK;
-    case KDOP_OPT_BLOCK_WAVES: *v = c->bw; return KDOP_OK;
+    case KDOP_OPT_BLOCK_WAVES: *v = bw_of(c); return KDOP_OK;
+    case KDOP_OPT_ITEM_LANES: *v = c->item_lanes; return KDOP_OK;
+    case KDOP_OPT_MAX_ROUNDS: *v = c->max_rounds; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
 }
@@ -1169,11 +1226,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if ((rc = ensure(c, c->recs, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
   }
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
-  const uint32_t pwaves = (uint32_t)(c->num_cus * c->wpc);
+  const uint32_t pwaves = (uint32_t)(c->num_cus * wpc_of(c));
   const uint32_t target = c->ramp_target ? c->ramp_target : 16 * pwaves;
-  const uint32_t rounds_async = std::min(std::max(c->rounds_hint, 1u), kQueuedRounds);
+  const uint32_t rounds_async = std::min(std::min(std::max(c->rounds_hint, 1u), kQueuedRounds), c->max_rounds);
   c->rounds_queued = rounds_async;
   uint64_t overflowed = 0;
+  uint32_t fallback_levels = 0;  // level-synchronous launches that finished a persistent call
   CallState* d = c->dstate;
   for (int attempt = 0; attempt < 10; ++attempt) {
     if (!c->state_clean) HIPCHK(c, hipMemsetAsync(d, 0, sizeof(CallState), c->stream));
@@ -1236,7 +1294,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dbodies.p, (uint64_t)(uintptr_t)c->dpairs.p, (uint64_t)(uintptr_t)c->dtops.p,
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
-          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)c->wpc, (uint64_t)c->ls, c->keep, (uint64_t)c->num_cus, (uint64_t)c->bw, rounds_async};
+          (uint64_t)(uintptr_t)c->gate.p, (uint64_t)wpc_of(c), (uint64_t)ls_of(c), c->keep, (uint64_t)c->num_cus, (uint64_t)bw_of(c), rounds_async,
+          (uint64_t)c->item_lanes, c->max_rounds};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;
@@ -1263,21 +1322,40 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     if ((rc = finish_summary(c))) return rc;
     bool grow_front = (c->hsum->overflow & 1u) != 0;
+    fallback_levels = 0;
     if (persistent && !grow_front && npairs && c->hsum->round_total[rounds] > 0) {
       // spill left after the queued rounds (the narrow phase above saw it and
-      // did nothing): drain it, then run the narrow phase
+      // did nothing): drain it with further rounds, then run the narrow phase.
+      // When the rounds run out, the spill becomes a breadth-first level and
+      // the level-synchronous launches finish the traversal, however deep.
+      bool to_levels = false;
       while (c->hsum->round_total[rounds] > 0 && !(c->hsum->overflow & 1u)) {
-        if (rounds + 1 >= (uint32_t)kMaxRounds) return fail(c, KDOP_ERR_CAPACITY, "kdop_traverse: spill rounds exhausted");
+        if (rounds >= c->max_rounds) { to_levels = true; break; }
         dispatch_K<PersistRound>(c->K, c, T, cap, rounds, last);
         ++rounds;
         HIPCHK(c, hipGetLastError());
         if ((rc = summarize(c, cap, -2))) return rc;
       }
       grow_front = (c->hsum->overflow & 1u) != 0;
-      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      if (narrow && !grow_front && (rc = launch_narrow(c, T, (int)rounds, false, one_pass))) return rc;
-      if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-      if ((rc = summarize(c, cap, (int)rounds))) return rc;
+      if (to_levels && !grow_front) {
+        // round `rounds` would read buffer (final_level + rounds) & 1; level l
+        // reads buffer l & 1, and a level >= 1 reads segments
+        const uint32_t l0 = ((c->hsum->final_level + rounds) & 1u) ? 1u : 2u, lend = l0 + levels;
+        hipLaunchKernelGGL(k_spill_to_level, dim3(1), dim3(64), 0, c->stream, d, rounds, l0);
+        for (uint32_t l = l0; l <= lend; ++l) dispatch_K<Bfs>(c->K, c, T, cap, l, ~0u, lend, true);
+        HIPCHK(c, hipGetLastError());
+        fallback_levels = lend - l0 + 1;
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        if (narrow && (rc = launch_narrow(c, T, -1, true, one_pass))) return rc;
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        if ((rc = summarize(c, cap, -1))) return rc;
+      } else {
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        if (narrow && !grow_front && (rc = launch_narrow(c, T, (int)rounds, false, one_pass))) return rc;
+        if (c->timing_level >= 1) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+        if ((rc = summarize(c, cap, (int)rounds))) return rc;
+      }
+      grow_front = (c->hsum->overflow & 1u) != 0;
     }
     if (!persistent && npairs && !grow_front && !c->hsum->final_set) {
       // the level-synchronous frontier outlived the first batch of launches
@@ -1335,6 +1413,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       c->timing[3] = ms[3] / (double)(levels + 1);
     }
     c->last_levels = persistent ? 0 : levels;
+    c->last_fallback_levels = fallback_levels;
     c->last_rounds = persistent ? rounds : 0;
     c->last_persistent = persistent;
     c->last_npairs = npairs;
@@ -1458,14 +1537,18 @@ int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint3
 
 int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
   if (!c || !out) return KDOP_ERR_ARG;
-  std::memset(out, 0, 12 * sizeof(uint64_t));
-  out[10] = (uint64_t)c->ls;
-  out[11] = (uint64_t)c->wpc;
+  std::memset(out, 0, 16 * sizeof(uint64_t));
+  out[10] = (uint64_t)ls_of(c);
+  out[11] = (uint64_t)wpc_of(c);
+  out[13] = (uint64_t)c->item_lanes;
+  out[14] = (uint64_t)bw_of(c);
   if (!c->have_traversal || !c->last_persistent) return KDOP_OK;
   const CallSummary& S = *c->hsum;
   out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.total[7];
-  out[4] = S.total[8]; out[5] = S.total[11]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * c->wpc;
+  out[4] = S.total[8]; out[5] = S.total[12]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * wpc_of(c);
   out[8] = S.total[10]; out[9] = c->last_rounds;
+  out[12] = S.total[11];
+  out[15] = c->last_fallback_levels;
   return KDOP_OK;
 }
 

@@ -718,6 +718,123 @@ __device__ inline void expand_item(const float* __restrict__ nodes, const Item& 
   }
 }
 
+// ------------------------------------------- cooperative expansion (4 lanes)
+// A work item expanded by the 4 lanes of a quad: lane r of the quad loads the
+// 16-byte words r and r + 4 of every record it needs (so one load instruction
+// covers 16 records in whole 64-byte runs instead of 64 scattered records),
+// tests the slabs held in those words for all 4 child combinations, and the
+// quad ORs its separation bits; lane r then owns combination k = r (child
+// i = r >> 1 of side a, child j = r & 1 of side b): its test, its counters and
+// the one item or leaf pair it may produce.  The meta words come from the
+// lane that loaded them by a quad broadcast.
+template <int K>
+struct Quad {
+  static constexpr int QU = (K + 2 + 3) / 4;  // 16-byte words holding vol[K], m0, m1
+  static constexpr int NQ = (QU + 3) / 4;     // of them per lane
+  static constexpr int MQ = K / 4;            // the word holding m0, m1
+  static constexpr int ML = MQ & 3;           // its lane in the quad
+  static constexpr int MT = MQ >> 2;          // and that lane's load slot
+  static constexpr bool MZ = (K % 4) != 0;    // m0, m1 in .z .w (else .x .y)
+};
+
+template <int CTRL>
+__device__ inline uint32_t quad_dpp(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false); }
+
+// The quad's view of one record: this lane's words (slot t: word r + 4 t).
+template <int K>
+struct QuadRec {
+  float4 w[Quad<K>::NQ];
+};
+// A lane loads only its slots that hold part of the record (lanes past the
+// used words keep what they had); lane_base: the pool as 16-byte words,
+// offset by this lane's first word.
+template <int K>
+__device__ inline void load_quad(const float4* __restrict__ lane_base, uint32_t idx, uint32_t r, QuadRec<K>& q) {
+  constexpr uint32_t RW = rec_stride(K) / 4;  // 16-byte words per record
+  const float4* p = lane_base + (size_t)idx * RW;
+#pragma unroll
+  for (int t = 0; t < Quad<K>::NQ; ++t)
+    if (r + 4u * t < (uint32_t)Quad<K>::QU) q.w[t] = p[4 * t];
+}
+template <int K>
+__device__ inline const float4* quad_lane_base(const float* __restrict__ nodes, uint32_t r) {
+  return reinterpret_cast<const float4*>(nodes) + r;
+}
+// This lane's share of the strict separation test of X and Y: the slabs held
+// in its words (lo0 hi0 lo1 hi1 per word; lanes past the volume test none).
+template <int K>
+__device__ inline bool quad_sep(const QuadRec<K>& X, const QuadRec<K>& Y, uint32_t r) {
+  bool s = false;
+#pragma unroll
+  for (int t = 0; t < Quad<K>::NQ; ++t) {
+    const uint32_t q = r + 4u * t;
+    const float4 x = X.w[t], y = Y.w[t];
+    const bool v0 = 4u * q + 1u < (uint32_t)K, v1 = 4u * q + 3u < (uint32_t)K;
+    s |= v0 & ((x.y < y.x) | (y.y < x.x));
+    s |= v1 & ((x.w < y.z) | (y.w < x.z));
+  }
+  return s;
+}
+template <int K>
+__device__ inline uint2 quad_meta(const QuadRec<K>& X) {
+  const float4 m = X.w[Quad<K>::MT];
+  const uint32_t m0 = __float_as_uint(Quad<K>::MZ ? m.z : m.x), m1 = __float_as_uint(Quad<K>::MZ ? m.w : m.y);
+  constexpr int B = Quad<K>::ML * 0x55;  // quad_perm [ML, ML, ML, ML]
+  return make_uint2(quad_dpp<B>(m0), quad_dpp<B>(m1));
+}
+
+// What lane r of a quad makes of its child combination.
+struct QuadOut {
+  Item kid;             // valid when push
+  uint32_t ta, tb;      // tet ids when leaf
+  bool test, descent, broot, leaf, surf, push;
+};
+
+// Every lane of the wave must call it (quad-wide DPP); live: the quad holds
+// an item (quad-uniform).  A0 .. B1 are the caller's loop-carried registers:
+// records the item does not need (a side that does not expand, a dead quad)
+// are not loaded and keep stale values, which only feed combinations whose
+// test flag is false.
+template <int K>
+__device__ inline void expand_quad(const float4* __restrict__ lane_base, const Item& it, bool live, QuadRec<K>& A0, QuadRec<K>& A1,
+                                   QuadRec<K>& B0, QuadRec<K>& B1, QuadOut& o) {
+  const uint32_t r = (uint32_t)lane_id() & 3u, i = r >> 1, j = r & 1u;
+  const bool exp_a = (it.a0 & I_EXP) != 0u, exp_b = (it.b0 & I_EXP) != 0u;
+  const uint32_t ca0 = it.a0 & IDX_MASK, ca1 = it.a1, cb0 = it.b0 & IDX_MASK, cb1 = it.b1;
+  if (live) {
+    load_quad<K>(lane_base, ca0, r, A0);
+    if (exp_a) load_quad<K>(lane_base, ca1, r, A1);
+    load_quad<K>(lane_base, cb0, r, B0);
+    if (exp_b) load_quad<K>(lane_base, cb1, r, B1);
+  }
+  // separation bit of combination k in bit k, ORed over the quad
+  uint32_t sep = (quad_sep<K>(A0, B0, r) ? 1u : 0u) | (quad_sep<K>(A0, B1, r) ? 2u : 0u) | (quad_sep<K>(A1, B0, r) ? 4u : 0u) |
+                 (quad_sep<K>(A1, B1, r) ? 8u : 0u);
+  sep |= quad_dpp<0xB1>(sep);  // quad_perm [1, 0, 3, 2]
+  sep |= quad_dpp<0x4E>(sep);  // quad_perm [2, 3, 0, 1]
+  const uint2 ma0 = quad_meta<K>(A0), ma1 = quad_meta<K>(A1), mb0 = quad_meta<K>(B0), mb1 = quad_meta<K>(B1);
+  const uint2 X = i ? ma1 : ma0, Y = j ? mb1 : mb0;
+  const uint32_t xi = i ? ca1 : ca0, yi = j ? cb1 : cb0;
+  o.test = live && (i == 0u || exp_a) && (j == 0u || exp_b);
+  const bool xb = (X.x & F_BRANCH) != 0u, yb = (Y.x & F_BRANCH) != 0u;
+  const bool both_roots = xb && yb && (X.y & F_ROOT) && (Y.y & F_ROOT);
+  o.descent = o.test && xb && yb && !both_roots;
+  const bool ov = o.test && ((sep >> r) & 1u) == 0u;
+  o.broot = ov && both_roots;
+  o.leaf = ov && (X.x & F_LEAF) && (Y.x & F_LEAF);
+  o.surf = o.leaf && (X.y & F_SURF) && (Y.y & F_SURF);
+  o.push = ov && !o.leaf;
+  o.ta = X.x & IDX_MASK;
+  o.tb = Y.x & IDX_MASK;
+  bool ex, ey;
+  expand_rule(X.x, Y.x, ex, ey);
+  o.kid.a0 = ex ? (I_EXP | (X.x & IDX_MASK)) : xi;
+  o.kid.a1 = ex ? (X.y & IDX_MASK) : 0u;
+  o.kid.b0 = ey ? (I_EXP | (Y.x & IDX_MASK)) : yi;
+  o.kid.b1 = ey ? (Y.y & IDX_MASK) : 0u;
+  o.kid.pair = it.pair;
+}
+
 // Write the leaf pairs of one lane (P1 when kept, P2 for the surface subset).
 // Every lane of the wave must call it.
 __device__ inline void emit_leaves(const TravArgs& T, uint32_t pair, const Expansion& x) {

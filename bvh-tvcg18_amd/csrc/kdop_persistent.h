@@ -268,7 +268,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   uint32_t nstage = 0;        // staged leaf pairs
   const bool keep_p1 = T.p1 != nullptr;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
-  uint32_t iters = 0, lanes = 0, grabs = 0;
+  uint32_t iters = 0, lanes = 0, grabs = 0, handed = 0;
   for (;;) {
     top = uniform(top); bot = uniform(bot); cur = uniform(cur); next_est = uniform(next_est);
     // ---- batch: top of the stack, then a chunk of the current segment
@@ -360,6 +360,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       b = lane0(b) + lane;
       if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
       else atomicOr(T.overflow, 1u);
+      handed += 64u;
     }
     // ---- leaf pairs: stage in LDS (all of P1 when it is kept, else only P2)
     const uint32_t lm = keep_p1 ? x.leaf : x.surf;
@@ -399,6 +400,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     atomicAdd(&st->iters, (unsigned long long)iters);
     atomicAdd(&st->lanes, (unsigned long long)lanes);
     atomicAdd(&st->grabs, (unsigned long long)grabs);
+    if (handed) atomicAdd(&st->handoffs, (unsigned long long)handed);
     atomicMax(&st->max_iters, (unsigned long long)iters);
   }
 }
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
   uint32_t nstage = 0;        // this wave's staged leaf pairs
   const bool keep_p1 = T.p1 != nullptr;
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
-  uint32_t iters = 0, lanes = 0, nsp = 0;
+  uint32_t iters = 0, lanes = 0, nsp = 0, handed = 0;
   for (;;) {
     const uint32_t gstart = ctl[0], ng = ctl[1];
     const bool front_done = ctl[2] != 0u;
@@ -529,6 +531,7 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
       if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
       else atomicOr(T.overflow, 1u);
       bot += NT;
+      handed += NT;
     }
     const uint32_t ns = min(NT, top - bot);
     if (ns + ng == 0u && front_done) break;  // block-uniform: ctl and the stack bounds are equal in every thread
@@ -621,12 +624,247 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
   wave_add64(&st->descent, descent);
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);
-  if (tid == 0) {  // schedule statistics per block: iterations, items expanded, grabs
+  if (tid == 0) {  // schedule statistics per block: iterations, items expanded, grabs, hand-offs
     atomicAdd(&st->iters, (unsigned long long)iters);
     atomicAdd(&st->lanes, (unsigned long long)lanes);
     atomicAdd(&st->grabs, (unsigned long long)grabs);
+    if (handed) atomicAdd(&st->handoffs, (unsigned long long)handed);
     atomicMax(&st->max_iters, (unsigned long long)iters);
   }
+}
+
+// The shared-stack round with 4 lanes per work item (expand_quad): a block of
+// W waves expands up to 16 W items per iteration, each quad loading its
+// item's records in coalesced 64-byte runs.  The stack holds S = 2^SL items;
+// everything else (frontier grabs by wave 0, two barriers per iteration, the
+// tail hand-off, spills and the per-wave leaf staging) works as in
+// k_persist_blk.
+template <int K, int W, int SL>
+__global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+                                                         CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last) {
+  constexpr uint32_t NT = 64u * W;  // threads
+  constexpr uint32_t IPI = 16u * W; // items per iteration
+  constexpr uint32_t S = 1u << SL;
+  static_assert(S >= 2u * NT, "stack holds at least one hand-off beyond a full iteration");
+  static_assert(kSegs == 64, "one lane of wave 0 per segment");
+  constexpr uint32_t SB = 128;  // staged leaf pairs per wave; flushed above SB - 64 (one iteration adds <= 64)
+  __shared__ uint32_t sa0[S], sa1[S], sb0[S], sb1[S], sp[S];
+  __shared__ uint32_t lp_pair[W][SB], lp_ta[W][SB], lp_tb[W][SB];
+  __shared__ uint32_t wtot[W];
+  __shared__ uint32_t ctl[3];  // next grab: first frontier index, count; frontier done
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform(tid >> 6);
+  const uint32_t gi = w * 16u + (lane >> 2);  // this quad's item of an iteration
+  const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, lane & 3u);
+  uint32_t final_level = C->final_level, final_count = 0;
+  if (round == 0) {
+    const bool recorded = C->final_set != 0u;
+    if (!recorded) final_level = last;
+    final_count = recorded ? C->final_count : min(C->level_fill[last], cap);
+    if (!recorded && blockIdx.x == 0 && tid == 0) {
+      C->final_level = final_level;
+      C->final_count = final_count;
+    }
+  }
+  const bool odd = ((final_level + round) & 1u) != 0u;
+  const Item* __restrict__ in = odd ? buf1 : buf0;
+  Item* __restrict__ spill = odd ? buf0 : buf1;
+  const uint32_t segcap = cap / kSegs;
+  const uint32_t bps = max(1u, gridDim.x / (uint32_t)kSegs);  // blocks per segment
+  const uint32_t home = blockIdx.x & (kSegs - 1);
+  // wave 0 only: lane s holds segment s of this round's frontier
+  uint32_t seg_base = 0, seg_len = 0, cur = home, next_est = 0;
+  unsigned long long known = ~0ull;
+  bool wfront_done = true;  // wave 0's view
+  if (w == 0) {
+    if (round == 0) {
+      const uint32_t n = min(final_count, cap);
+      seg_base = (uint32_t)(((unsigned long long)n * lane) / kSegs);
+      seg_len = (uint32_t)(((unsigned long long)n * (lane + 1)) / kSegs) - seg_base;
+    } else {
+      seg_base = lane * segcap;
+      seg_len = min(C->seg[round][lane].fill, segcap);
+    }
+    known = __ballot(seg_len == 0u);
+    wfront_done = known == ~0ull;
+    if (!wfront_done && ((known >> cur) & 1ull)) cur = next_open(known, cur);
+  }
+  uint32_t grabs = 0, wave0_iters = 0;
+  // wave 0: the grab for an iteration whose stack part is ns items; writes ctl
+  auto plan = [&](uint32_t ns) {
+    cur = uniform(cur); next_est = uniform(next_est);
+    uint32_t gstart = 0, ng = 0;
+    bool refresh = false;
+    if (ns < IPI && !wfront_done) {
+      const uint32_t len = __builtin_amdgcn_readlane(seg_len, cur), sbase = __builtin_amdgcn_readlane(seg_base, cur);
+      const uint32_t remaining = len > next_est ? len - next_est : 0u;
+      uint32_t want = remaining / (2u * bps);
+      want = want < 1u ? 1u : (want > IPI - ns ? IPI - ns : want);
+      uint32_t start = 0;
+      if (lane == 0) start = atomicAdd(&C->seg[round][cur].next, want);
+      start = lane0(start);
+      if (start < len) {
+        ++grabs;
+        next_est = start + want;
+        ng = min(want, len - start);
+        gstart = sbase + start;
+        if (start + want >= len) {  // this block took the segment's last chunk
+          if (lane == 0) atomicOr(&C->done[round], 1ull << cur);
+          known |= 1ull << cur;
+        }
+      } else {
+        known |= 1ull << cur;
+        refresh = true;
+      }
+    } else if (!wfront_done && (wave0_iters & 15u) == 15u) {
+      refresh = true;  // a block living off its stack looks at the frontier now and then
+    }
+    if (refresh) {
+      unsigned long long d = 0;
+      if (lane == 0) d = __hip_atomic_load(&C->done[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      known |= ((unsigned long long)lane0((uint32_t)(d >> 32)) << 32) | lane0((uint32_t)d);
+    }
+    if (!wfront_done && ((known >> cur) & 1ull)) {
+      if (known == ~0ull) wfront_done = true;
+      else { cur = next_open(known, cur); next_est = 0; }
+    }
+    if (lane == 0) {
+      ctl[0] = gstart;
+      ctl[1] = ng;
+      ctl[2] = wfront_done ? 1u : 0u;
+    }
+  };
+  if (w == 0) plan(0u);
+  __syncthreads();
+  uint32_t top = 0, bot = 0;  // stack = [bot, top) modulo S, equal in every thread
+  uint32_t nstage = 0;        // this wave's staged leaf pairs
+  const bool keep_p1 = T.p1 != nullptr;
+  uint32_t tests = 0, descent = 0, broot = 0, p1s = 0;
+  uint32_t iters = 0, lanes = 0, nsp = 0, handed = 0;
+  QuadRec<K> A0, A1, B0, B1;  // this lane's words of the item's records (loop-carried: see expand_quad)
+#pragma unroll
+  for (int t = 0; t < Quad<K>::NQ; ++t) A0.w[t] = A1.w[t] = B0.w[t] = B1.w[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (;;) {
+    const uint32_t gstart = ctl[0], ng = ctl[1];
+    const bool front_done = ctl[2] != 0u;
+    // ---- tail: hand the bottom of a deep stack to the next round, 64 items per
+    // wave, beyond `keep` items per wave (before this iteration's pops and pushes)
+    if (front_done && (top - bot) > keep * W + NT) {
+      const uint32_t s3 = (bot + tid) & (S - 1);
+      const Item mine = Item{sa0[s3], sa1[s3], sb0[s3], sb1[s3], sp[s3]};
+      const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
+      uint32_t b = 0;
+      if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, 64u);
+      b = lane0(b) + lane;
+      if (b < segcap) spill[(size_t)sg * segcap + b] = mine;
+      else atomicOr(T.overflow, 1u);
+      bot += NT;
+      handed += NT;
+    }
+    const uint32_t ns = min(IPI, top - bot);
+    if (ns + ng == 0u && front_done) break;  // block-uniform
+    // ---- batch: the top ns stack items, then ng frontier items, one per quad
+    Item it{0, 0, 0, 0, 0};
+    const bool live = gi < ns + ng;
+    if (gi < ns) {
+      const uint32_t s = (top - ns + gi) & (S - 1);
+      it = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
+    } else if (live) {
+      it = in[gstart + gi - ns];
+    }
+    top -= ns;
+    QuadOut o;
+    expand_quad<K>(qbase, it, live, A0, A1, B0, B1, o);
+    tests += o.test; descent += o.descent; broot += o.broot; p1s += o.leaf;
+    ++iters;
+    lanes += ns + ng;
+    const unsigned long long pm = __ballot(o.push);
+    const uint32_t kpre = __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+    if (lane == 0) wtot[w] = (uint32_t)__popcll(pm);
+    // ---- leaf pairs: stage in this wave's LDS (all of P1 when it is kept, else only P2)
+    const bool lst = keep_p1 ? o.leaf : o.surf;
+    const unsigned long long lm = __ballot(lst);
+    if (lm) {
+      if (lst) {
+        const uint32_t o2 = nstage + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
+        lp_pair[w][o2] = o.kid.pair | (o.surf ? 0x80000000u : 0u);
+        lp_ta[w][o2] = o.ta;
+        lp_tb[w][o2] = o.tb;
+      }
+      nstage += (uint32_t)__popcll(lm);
+    }
+    __syncthreads();  // A: the stack reads are done and every wave's push total is in wtot
+    uint32_t woff = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < W; ++v) {
+      const uint32_t t = wtot[v];
+      woff += (uint32_t)v < w ? t : 0u;
+      total += t;
+    }
+    // ---- children: the shared stack when they fit, else the next round (per wave)
+    if (total) {
+      if ((top - bot) + total <= S) {
+        if (o.push) {
+          const uint32_t s2 = (top + woff + kpre) & (S - 1);
+          sa0[s2] = o.kid.a0; sa1[s2] = o.kid.a1; sb0[s2] = o.kid.b0; sb1[s2] = o.kid.b1; sp[s2] = o.kid.pair;
+        }
+        top += total;
+      } else {
+        const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
+        if (pm) {
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, (uint32_t)__popcll(pm));
+          b = lane0(b) + kpre;
+          if (o.push) {
+            if (b < segcap) spill[(size_t)sg * segcap + b] = o.kid;
+            else atomicOr(T.overflow, 1u);
+          }
+        }
+      }
+    }
+    if (nstage > SB - 64u) {
+      wave_lds_sync();
+      flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
+      nstage = 0;
+    }
+    if (w == 0) {
+      ++wave0_iters;
+      plan(min(IPI, top - bot));
+    }
+    __syncthreads();  // B: pushes, hand-offs and the next grab are visible to every wave
+  }
+  wave_lds_sync();
+  if (nstage) flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  const unsigned long long wtests = wave_sum64(tests);
+  if (lane == 0 && wtests) {
+    atomicAdd(&st->tests, wtests);
+    atomicAdd(&st->ptests, wtests);
+  }
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
+  if (tid == 0) {  // schedule statistics per block: iterations, items expanded, grabs, hand-offs
+    atomicAdd(&st->iters, (unsigned long long)iters);
+    atomicAdd(&st->lanes, (unsigned long long)lanes);
+    atomicAdd(&st->grabs, (unsigned long long)grabs);
+    if (handed) atomicAdd(&st->handoffs, (unsigned long long)handed);
+    atomicMax(&st->max_iters, (unsigned long long)iters);
+  }
+}
+
+// The spill left in round `round`'s segments becomes breadth-first level
+// `level` (one 64-lane block, before the level launches): the persistent
+// rounds spill into kSegs segments of cap / kSegs slots, the layout of a
+// segmented level, so the fills are copied over and the levels strategy
+// finishes the traversal from there.  level >= 1 (segmented input) and of the
+// parity that selects the buffer the spill is in; the level's output set and
+// the next are cleared, and the stop flag is reset for the levels to run.
+__global__ __launch_bounds__(64) void k_spill_to_level(CallState* __restrict__ C, uint32_t round, uint32_t level) {
+  const uint32_t lane = threadIdx.x;
+  C->lseg[level % 3u][lane].fill = C->seg[round][lane].fill;
+  C->lseg[(level + 1u) % 3u][lane].fill = 0u;
+  C->lseg[(level + 2u) % 3u][lane].fill = 0u;
+  if (lane == 0u) C->final_set = 0u;
 }
 
 // Fold the striped counters and the segment fills into the call summary, one
@@ -644,7 +882,7 @@ constexpr int kSumThreads = 1024;
 __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round,
                                                               CallSummary* __restrict__ host) {
   static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
-  static_assert(kStripeFields <= 12 && kMaxRounds == 8, "12 field waves, 4 round waves");
+  static_assert(kStripeFields == 13 && kMaxRounds == 8, "13 field waves, 3 round waves");
   static_assert(sizeof(CounterStripe) * kStripes == 16 * kSumThreads, "one 16-byte clear per thread");
   __shared__ uint32_t pending_s;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -659,11 +897,12 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
       x = (w == (uint32_t)kStripeFields - 1u) ? max(x, y) : x + y;
     }
     if (lane == 0) S.total[w] = x;
-  } else if (w >= 12u) {
+  } else {
     const uint32_t segcap = cap / kSegs;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = 2 * (int)(w - 12u) + h;
+    for (int h = 0; h < 3; ++h) {
+      const int r = 3 * (int)(w - 13u) + h;
+      if (r >= kMaxRounds) break;
       const uint32_t f = C->seg[r][lane].fill;
       uint32_t tot = min(f, segcap), mx = f;
 #pragma unroll
@@ -677,7 +916,7 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
         if (r == final_round) pending_s = tot;
       }
     }
-    if (w == 12u) {
+    if (w == 13u) {
       S.level_fill[lane] = C->level_fill[lane];
       if (lane == 0) {
         S.overflow = C->overflow;

@@ -208,14 +208,13 @@ class Context:
     def autotune(self, pairs, narrow=True, budget_ms=2000.0):
         """kdop_autotune on this pair list: returns (chosen options, best median ms)."""
         P = np.ascontiguousarray(np.asarray(pairs, np.uint32).reshape(-1, 2))
-        chosen = np.zeros(8, np.int64)
+        chosen = np.zeros(16, np.int64)
         best = ctypes.c_double()
         self._check(self._lib.kdop_autotune(self._h, P.shape[0], ptr(P), KDOP_NARROW if narrow else 0, float(budget_ms),
                                             ptr(chosen), ctypes.byref(best)))
         self._last_npairs = P.shape[0]
         self.generation += 1
-        names = sorted(_abi.OPTIONS, key=_abi.OPTIONS.get)
-        return dict(zip(names, (int(v) for v in chosen))), best.value
+        return {k: int(chosen[v - 1]) for k, v in _abi.OPTIONS.items()}, best.value
 
     # -- kdop::make_tree
     def make_tree(self, mesh):
@@ -433,10 +432,11 @@ class Context:
         return dict(vol=vol, range=rng, kind=kind, root=root)
 
     def traversal_stats(self):
-        out = np.zeros(12, np.uint64)
+        out = np.zeros(16, np.uint64)
         self._check(self._lib.kdop_traversal_stats(self._h, ptr(out)))
         keys = ("ramp_levels", "frontier", "rounds", "iters", "lanes", "max_iters", "grabs", "waves",
-                "persist_tests", "persist_launches", "queue_depth", "waves_per_cu")
+                "persist_tests", "persist_launches", "queue_depth", "waves_per_cu", "handoffs", "item_lanes",
+                "block_waves", "fallback_levels")
         return dict(zip(keys, (int(v) for v in out)))
 
     def last_timing(self):

@@ -10,6 +10,7 @@ same bits on any host of this image.
                                   overlap fraction is in [0.3, 0.6]
     batched_pairs(npairs=512)     meshes of log-uniform tet counts, ~40 % of the
                                   pairs separated
+    self_contact(n=20, K=26)      a jittered box against an identical copy at identity
     motion(...)                   seeded rigid motion sequence for refit series
 """
 from dataclasses import dataclass, field
@@ -79,6 +80,15 @@ def large_pair(n=40, K=26, seed=7, jitter=0.1, lo=0.3, hi=0.6):
         raise RuntimeError("no translation gave the requested overlap fraction")
     return PairWorkload(f"box{n}_jitter_pair_k{K}", K, [ma, mb], np.array([[0, 1]], np.uint32),
                         [IDENTITY, Pose(p, q)], dict(n=n, nt=ma.nt, nv=ma.nv, aabb_overlap=frac))
+
+
+def self_contact(n=20, K=26, seed=3, jitter=0.1):
+    """A jittered n^3-cube box against an identical copy at identity: every
+    tetrahedron overlaps itself and its neighbours, the densest traversal a
+    pair of these meshes can make."""
+    m = box_mesh(n, size=1.0, jitter=jitter, seed=seed)
+    return PairWorkload(f"box{n}_self_contact_k{K}", K, [m, m], np.array([[0, 1]], np.uint32), [IDENTITY, IDENTITY],
+                        dict(n=n, nt=m.nt, nv=m.nv))
 
 
 def batched_pairs(npairs=512, K=8, seed=11, tmin=1e3, tmax=1e5, separated=0.4, jitter=0.1, lo=0.3, hi=0.6):

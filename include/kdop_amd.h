@@ -274,23 +274,32 @@ int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies,
 int kdop_last_timing(kdop_ctx* ctx, double* out);
 
 /* Scheduling statistics of the last traversal (persistent strategy; zeros for
- * the level-synchronous one), 12 entries: out[0] breadth-first levels before
+ * the level-synchronous one), 16 entries: out[0] breadth-first levels before
  * the persistent rounds, out[1] frontier handed to them, out[2] rounds run,
- * out[3] wave loop iterations (sum), out[4] items expanded (sum of active
- * lanes), out[5] most iterations of one wave in one round, out[6] frontier
- * chunks taken, out[7] persistent waves, out[8] node-pair tests made by the
- * persistent rounds, out[9] persistent launches; out[10] LDS stack entries
- * per wave and out[11] waves per CU in effect (set for either strategy). */
+ * out[3] loop iterations (sum over waves, or over workgroups where waves
+ * share a stack), out[4] items expanded, out[5] most iterations of one wave
+ * or workgroup in one round, out[6] frontier chunks taken, out[7] persistent
+ * waves, out[8] node-pair tests made by the persistent rounds, out[9]
+ * persistent launches, out[12] items the tail hand-off passed on, out[15]
+ * breadth-first levels launched after the persistent rounds ran out (the
+ * spill finished level-synchronously); out[10] LDS stack entries per wave,
+ * out[11] waves per CU, out[13] lanes per work item and out[14] waves per
+ * workgroup in effect (set for either strategy). */
 int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 
 /* Options of a live context (kdop_set_option / kdop_get_option).  The
  * traversal knobs take effect at the next kdop_traverse; 0 restores the
  * library default of a knob. */
 #define KDOP_OPT_TRAVERSAL 1      /* KDOP_TRAVERSAL_*                                        */
-#define KDOP_OPT_QUEUE_DEPTH 2    /* LDS stack entries per wave: 256, 512, 1024 or 2048      */
-#define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32                           */
+#define KDOP_OPT_QUEUE_DEPTH 2    /* LDS stack entries per wave: 256, 512, 1024 or 2048 (the    */
+                                  /*   quad kernel: 256 or 512); 0: the kernel's default, 256    */
+                                  /*   (quad) or 512                                            */
+#define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32; 0: 24 (quad) or 8          */
 #define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels        */
-#define KDOP_OPT_STACK_KEEP 5     /* stack entries a wave keeps once the frontier is gone    */
+#define KDOP_OPT_STACK_KEEP 5     /* persistent rounds: once the frontier is gone, a workgroup  */
+                                  /*   whose stack holds more than this many entries per wave   */
+                                  /*   plus one hand-off hands its bottom 64 entries per wave  */
+                                  /*   to the next round; 0 (the default): never                */
 #define KDOP_OPT_SEED_BUDGET 6    /* chunk-root pairs tested to seed one call                */
 #define KDOP_OPT_KERNEL_TIMING 7  /* HIP events for kdop_last_timing: 0 none (default; the     */
                                   /*   timings read 0), 1 phase events, 2 also one pair on each */
@@ -309,11 +318,22 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   kdop_single_traversal before it splits the spheres into   */
                                   /*   batches (0: the default, 2^28; 1 .. 2^30); bounds its     */
                                   /*   device memory at 8 bytes x 2 x this, results unchanged    */
-#define KDOP_OPT_BLOCK_WAVES 13   /* persistent rounds: 2 (0: the default), 4 or 8 waves per     */
-                                  /*   workgroup share one LDS stack of 512 entries per wave      */
-                                  /*   (KDOP_OPT_QUEUE_DEPTH then unused); 1: one stack per wave   */
-                                  /*   of KDOP_OPT_QUEUE_DEPTH entries.  Results unchanged; the    */
-                                  /*   autotuner tries 1, 2 and 4 (not reported in `chosen`)       */
+#define KDOP_OPT_BLOCK_WAVES 13   /* persistent rounds: waves per workgroup sharing one LDS      */
+                                  /*   stack.  Quad kernel: 2, 4 (0: the default) or 8, stack of  */
+                                  /*   KDOP_OPT_QUEUE_DEPTH entries per wave.  One lane per item: */
+                                  /*   2 (0: the default), 4 or 8 with 512 entries per wave, or 1 */
+                                  /*   (one stack per wave of KDOP_OPT_QUEUE_DEPTH entries).      */
+                                  /*   Results unchanged                                          */
+#define KDOP_OPT_ITEM_LANES 14    /* persistent rounds: lanes per work item.  1: one lane loads    */
+                                  /*   and tests whole records; 4: the 4 lanes of a quad each load */
+                                  /*   a quarter of every record (coalesced 64-byte runs) and own   */
+                                  /*   one child combination; the stack then holds                 */
+                                  /*   KDOP_OPT_QUEUE_DEPTH (256 or 512) items per wave.  Results   */
+                                  /*   unchanged                                                   */
+#define KDOP_OPT_MAX_ROUNDS 15    /* persistent strategy: most persistent rounds per call, 1..7   */
+                                  /*   (0: 7); a spill left after them becomes a breadth-first     */
+                                  /*   level and the level-synchronous launches finish the        */
+                                  /*   traversal from it, however deep.  Results unchanged        */
 
 /* Set / read one option (no counterpart in the reference, whose OpenCL
  * path fixes these sizes at compile time).  Returns KDOP_ERR_ARG for an
@@ -321,18 +341,19 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 int kdop_set_option(kdop_ctx* ctx, int option, int64_t value);
 int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
 
-/* Autotuner: picks the traversal options (strategy, waves per persistent
- * workgroup, waves per CU, LDS queue depth, stack keep, breadth-first target,
- * seed budget) for this context by timing kdop_traverse(npairs, body_pairs,
- * flags) itself: coordinate descent over the options in that order, each
- * candidate measured as the median host wall time of 3 calls (a candidate
- * that fails with KDOP_ERR_CAPACITY is skipped), until budget_ms of tuning
- * time is spent.  The best
- * options stay set on the context, and the last traversal's results are those
- * of a run with them.  chosen (may be NULL) receives 8 values indexed by
- * KDOP_OPT_* - 1 (options 1..8); *best_ms (may be NULL) the median time of
- * the best.
+/* Autotuner: picks the traversal options (strategy, lanes per work item,
+ * waves per persistent workgroup, waves per CU, LDS queue depth, stack keep,
+ * breadth-first target, seed budget) for this context by timing
+ * kdop_traverse(npairs, body_pairs, flags) itself: coordinate descent over
+ * the options in that order, each candidate measured as the median host wall
+ * time of 3 calls (a candidate that fails with KDOP_ERR_CAPACITY is skipped),
+ * until budget_ms of tuning time is spent.  The best options stay set on the
+ * context, and the last traversal's results are those of a run with them.
+ * chosen (may be NULL) receives KDOP_AUTOTUNE_CHOSEN values indexed by
+ * KDOP_OPT_* - 1 (the value in effect of every option; 0 for numbers that
+ * name no option); *best_ms (may be NULL) the median time of the best.
  * No counterpart in the reference (its OpenCL sizes are compile-time). */
+#define KDOP_AUTOTUNE_CHOSEN 16
 int kdop_autotune(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, double budget_ms, int64_t* chosen,
                   double* best_ms);
 

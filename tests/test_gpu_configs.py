@@ -65,6 +65,41 @@ def test_c3_small_stack_keep_completes():
         ctx.close()
 
 
+def test_self_contact_completes():
+    """A jittered 20^3 box (40,000 tets, K=26) against an identical copy at
+    identity, the densest traversal these meshes make: under AUTO, under the
+    persistent strategy with a stack keep of 1 (the queued rounds hand the
+    bottom of every deep stack on), and with one persistent round only and a
+    keep of 1 (the hand-offs spill into a round that is not run, so the
+    spill is finished by the level-synchronous launches), with either item
+    mapping, every call completes with the oracle's counts, pair sets and
+    contacts."""
+    wl = W.self_contact(n=20, K=26, seed=3)
+    orc, _, _ = oracle_run(wl, threads=ORACLE_THREADS)
+    cases = [
+        (dict(traversal=K.KDOP_TRAVERSAL_AUTO), {}),
+        (dict(traversal=K.KDOP_TRAVERSAL_PERSISTENT, keep=1), {}),
+        (dict(traversal=K.KDOP_TRAVERSAL_PERSISTENT, keep=1), {"item_lanes": 1}),
+        (dict(traversal=K.KDOP_TRAVERSAL_PERSISTENT, keep=1, waves_per_cu=4), {"max_rounds": 1, "block_waves": 2}),
+        (dict(traversal=K.KDOP_TRAVERSAL_PERSISTENT, keep=1, queue_depth=256, waves_per_cu=2),
+         {"max_rounds": 1, "item_lanes": 1, "block_waves": 1}),
+    ]
+    for kw, opts in cases:
+        ctx = K.Context(K=26, **kw)
+        for name, v in opts.items():
+            ctx.set_option(name, v)
+        dev = device_run(wl, ctx=ctx)
+        st = ctx.traversal_stats()
+        print(kw, opts, "rounds", st["rounds"], "handoffs", st["handoffs"], "fallback levels", st["fallback_levels"],
+              "tests", dev["counters"]["tests"], flush=True)
+        _check_pair_results(wl, dev, orc)
+        if kw.get("keep") == 1:
+            assert st["handoffs"] > 0
+        if "max_rounds" in opts:
+            assert st["fallback_levels"] > 0 and st["rounds"] == opts["max_rounds"]
+        ctx.close()
+
+
 def test_c4_refit_series():
     """64 overlapping pairs (20k-80k tets, K=18), 200 steps of rigid motion,
     refit (never rebuilt) and re-traversed every step on the device; the
