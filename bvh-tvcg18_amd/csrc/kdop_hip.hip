@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -140,6 +141,8 @@ struct kdop_ctx {
   double timing[4 + kMaxRounds] = {};  // see kdop_last_timing
   uint32_t last_levels = 0;
   uint32_t last_fallback_levels = 0;  // levels launched after the persistent rounds ran out (last call)
+  void* scratch = nullptr;            // this device's address of the allocation-free entry point's array
+  std::string scratch_err;            // why it is null
 };
 
 namespace {
@@ -570,6 +573,58 @@ struct PersistRound {
   }
 };
 
+// Device scratch of a PROX reduction of n records: two key arrays (the keys
+// in the first), keep flags, their offsets, the scan's block sums and the
+// kept count.
+struct ReduceScratch {
+  ProxKey *ka, *kb;
+  uint32_t *keep, *off, *sums, *total;
+};
+// ------------------------------------------------ allocation-free reduction
+// The scratch of kdop_reduce_prox_contacts_noalloc: one fixed array per GPU,
+// held until the process ends, laid out for KDOP_NOALLOC_MAX_CONTACTS
+// records (two key arrays, keep flags, offsets, block sums, the count).
+constexpr size_t kNaKeyBytes = (size_t)KDOP_NOALLOC_MAX_CONTACTS * sizeof(ProxKey);
+constexpr size_t kNaWords = 2 * (size_t)KDOP_NOALLOC_MAX_CONTACTS + (KDOP_NOALLOC_MAX_CONTACTS + kScanBlock - 1) / kScanBlock + 1;
+constexpr size_t kNoAllocScratchBytes = 2 * kNaKeyBytes + kNaWords * sizeof(uint32_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` on each GPU (the host-side name is not one),
+// looked up once per device with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+// Called by kdop_create with the context's device current.  A failed lookup
+// leaves ctx->scratch null and its message in ctx->scratch_err; only the
+// allocation-free entry point depends on it.
+void setup_noalloc_scratch(kdop_ctx* c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  const size_t dev = (size_t)c->cfg.device;
+  if (g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  c->scratch = g_scratch_ptr[dev];
+}
+
+ReduceScratch noalloc_scratch_of(void* base) {
+  ReduceScratch r;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  r.ka = reinterpret_cast<ProxKey*>(b);
+  r.kb = reinterpret_cast<ProxKey*>(b + kNaKeyBytes);
+  r.keep = reinterpret_cast<uint32_t*>(b + 2 * kNaKeyBytes);
+  r.off = r.keep + KDOP_NOALLOC_MAX_CONTACTS;
+  r.sums = r.off + KDOP_NOALLOC_MAX_CONTACTS;
+  r.total = r.sums + (KDOP_NOALLOC_MAX_CONTACTS + kScanBlock - 1) / kScanBlock;
+  return r;
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -622,6 +677,7 @@ int kdop_create(kdop_ctx** out, const kdop_config* cfg) {
     if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) { kdop_destroy(c); return KDOP_ERR_HIP; }
     c->num_cus = prop.multiProcessorCount;
   }
+  setup_noalloc_scratch(c);
   {
     const int64_t opts[5][2] = {{KDOP_OPT_QUEUE_DEPTH, cfg->queue_depth}, {KDOP_OPT_WAVES_PER_CU, cfg->waves_per_cu},
                                 {KDOP_OPT_RAMP_TARGET, cfg->ramp_target}, {KDOP_OPT_STACK_KEEP, cfg->stack_keep},
@@ -1111,39 +1167,43 @@ uint32_t contact_count(const kdop_ctx* c) {
   return c->reduced ? c->reduced_n : (uint32_t)std::min<size_t>(c->hsum->fills.c, c->contacts.cap);
 }
 
-// PROX reduction of n records whose keys fill the first half of sortbuf: sort
-// the keys, mark the points std::unique keeps, and write them to out in
-// sorted order; *kept receives their count (the call ends synchronised).
-template <class Rec>
-int reduce_keys(kdop_ctx* c, const Rec* in, Rec* out, uint32_t n, uint32_t* kept) {
-  ProxKey* a = reinterpret_cast<ProxKey*>(c->sortbuf.p);
-  ProxKey* sorted = nullptr;
-  int rc = device_sort(c, a, a + n, n, &sorted);
-  if (rc) return rc;
-  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
-  uint32_t* keep = c->redbuf.p;
-  uint32_t* off = keep + n;
-  uint32_t* sums = off + n;
-  uint32_t* total = sums + nb;
-  const dim3 grid((n + 255) / 256);
-  hipLaunchKernelGGL(k_prox_unique, grid, dim3(256), 0, c->stream, (const ProxKey*)sorted, n, keep);
-  HIPCHK(c, hipGetLastError());
-  if ((rc = device_scan(c, keep, off, n, sums, total))) return rc;
-  hipLaunchKernelGGL(k_prox_compact<Rec>, grid, dim3(256), 0, c->stream, in, (const ProxKey*)sorted, (const uint32_t*)keep,
-                     (const uint32_t*)off, n, out);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(kept, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return KDOP_OK;
+// The scratch laid out from the context's buffers (sized by reduce_scratch).
+ReduceScratch reduce_scratch_of(kdop_ctx* c, uint32_t n) {
+  ReduceScratch r;
+  r.ka = reinterpret_cast<ProxKey*>(c->sortbuf.p);
+  r.kb = r.ka + n;
+  r.keep = c->redbuf.p;
+  r.off = r.keep + n;
+  r.sums = r.off + n;
+  r.total = r.sums + (n + kScanBlock - 1) / kScanBlock;
+  return r;
 }
 
-// Scratch of a reduction of n records: two key arrays, then keep, offsets,
-// block sums and the total.
+// PROX reduction of n records whose keys fill R.ka: sort the keys, mark the
+// points std::unique keeps, and write them to out in sorted order; *kept
+// receives their count through pinned host memory once the work is done.
+template <class Rec>
+int reduce_keys(kdop_ctx* c, const Rec* in, Rec* out, uint32_t n, const ReduceScratch& R, uint32_t* kept) {
+  ProxKey* sorted = nullptr;
+  int rc = device_sort(c, R.ka, R.kb, n, &sorted);
+  if (rc) return rc;
+  const dim3 grid((n + 255) / 256);
+  hipLaunchKernelGGL(k_prox_unique, grid, dim3(256), 0, c->stream, (const ProxKey*)sorted, n, R.keep);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = device_scan(c, R.keep, R.off, n, R.sums, R.total))) return rc;
+  hipLaunchKernelGGL(k_prox_compact<Rec>, grid, dim3(256), 0, c->stream, in, (const ProxKey*)sorted, (const uint32_t*)R.keep,
+                     (const uint32_t*)R.off, n, out);
+  HIPCHK(c, hipGetLastError());
+  return read_device_words(c, R.total, 1, kept);
+}
+
+// Scratch of a reduction of n records in the context's buffers.
 int reduce_scratch(kdop_ctx* c, uint32_t n) {
   int rc = ensure(c, c->sortbuf, 2 * (size_t)n * sizeof(ProxKey));
   if (!rc) rc = ensure(c, c->redbuf, 2 * (size_t)n + (n + kScanBlock - 1) / kScanBlock + 1);
   return rc;
 }
+
 
 }  // namespace
 
@@ -1624,7 +1684,7 @@ int kdop_reduce_contacts(kdop_ctx* c, uint64_t* n) {
   hipLaunchKernelGGL(k_prox_keys_pairs, dim3((nc + 255) / 256), dim3(256), 0, c->stream, (const kdop_contact*)c->contacts.p, nc,
                      (const uint2*)c->dpairs.p, reinterpret_cast<ProxKey*>(c->sortbuf.p));
   uint32_t kept = 0;
-  if ((rc = reduce_keys<kdop_contact>(c, c->contacts.p, c->contacts_tmp.p, nc, &kept))) return rc;
+  if ((rc = reduce_keys<kdop_contact>(c, c->contacts.p, c->contacts_tmp.p, nc, reduce_scratch_of(c, nc), &kept))) return rc;
   if (kept) HIPCHK(c, hipMemcpyAsync(c->contacts.p, c->contacts_tmp.p, (size_t)kept * sizeof(kdop_contact), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->reduced_n = kept;
@@ -1654,11 +1714,33 @@ int kdop_reduce_prox_contacts(kdop_ctx* c, uint64_t n, const kdop_prox_contact* 
   }
   hipLaunchKernelGGL(k_prox_keys_points, dim3((nc + 255) / 256), dim3(256), 0, c->stream, din, nc, reinterpret_cast<ProxKey*>(c->sortbuf.p));
   uint32_t kept = 0;
-  if ((rc = reduce_keys<kdop_prox_contact>(c, din, dout, nc, &kept))) return rc;
+  if ((rc = reduce_keys<kdop_prox_contact>(c, din, dout, nc, reduce_scratch_of(c, nc), &kept))) return rc;
   if (!is_device && kept) {
     HIPCHK(c, hipMemcpyAsync(out, dout, (size_t)kept * sizeof(kdop_prox_contact), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
+  *n_out = kept;
+  return KDOP_OK;
+}
+
+int kdop_reduce_prox_contacts_noalloc(kdop_ctx* c, uint64_t n, const kdop_prox_contact* in_dev, kdop_prox_contact* out_dev,
+                                      uint64_t* n_out) {
+  if (!c || !n_out) return KDOP_ERR_ARG;
+  *n_out = 0;
+  if (n == 0) return KDOP_OK;
+  if (!in_dev || !out_dev) return fail(c, KDOP_ERR_ARG, "kdop_reduce_prox_contacts_noalloc: null argument");
+  if (n > KDOP_NOALLOC_MAX_CONTACTS)
+    return fail(c, KDOP_ERR_CAPACITY, "kdop_reduce_prox_contacts_noalloc: %llu contacts, at most %u (KDOP_NOALLOC_MAX_CONTACTS)",
+                (unsigned long long)n, (unsigned)KDOP_NOALLOC_MAX_CONTACTS);
+  if (in_dev == out_dev) return fail(c, KDOP_ERR_ARG, "kdop_reduce_prox_contacts_noalloc: out aliases in");
+  if (!c->scratch) return fail(c, KDOP_ERR_HIP, "kdop_reduce_prox_contacts_noalloc: no scratch on this device (%s)", c->scratch_err.c_str());
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint32_t nc = (uint32_t)n;
+  const ReduceScratch R = noalloc_scratch_of(c->scratch);
+  hipLaunchKernelGGL(k_prox_keys_points, dim3((nc + 255) / 256), dim3(256), 0, c->stream, in_dev, nc, R.ka);
+  uint32_t kept = 0;
+  const int rc = reduce_keys<kdop_prox_contact>(c, in_dev, out_dev, nc, R, &kept);
+  if (rc) return rc;
   *n_out = kept;
   return KDOP_OK;
 }

@@ -15,6 +15,8 @@ KDOP_ERR_STATE = -4
 KDOP_ERR_CAPACITY = -5
 KDOP_ERR_INTERNAL = -6
 
+KDOP_NOALLOC_MAX_CONTACTS = 262144  # include/kdop_amd.h
+
 KDOP_TRAVERSAL_AUTO = 0
 KDOP_TRAVERSAL_LEVELS = 1
 KDOP_TRAVERSAL_PERSISTENT = 2
@@ -98,6 +100,7 @@ SIGNATURES = [
     ("kdop_sort_contacts", ctypes.c_int, [_p, ctypes.c_int]),
     ("kdop_reduce_contacts", ctypes.c_int, [_p, _p]),
     ("kdop_reduce_prox_contacts", ctypes.c_int, [_p, _u64, _p, _p, ctypes.c_int, _p]),
+    ("kdop_reduce_prox_contacts_noalloc", ctypes.c_int, [_p, _u64, _p, _p, _p]),
     ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p, ctypes.c_int]),
     ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p, ctypes.c_int]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),

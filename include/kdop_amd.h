@@ -198,6 +198,22 @@ int kdop_reduce_contacts(kdop_ctx* ctx, uint64_t* n);
 int kdop_reduce_prox_contacts(kdop_ctx* ctx, uint64_t n, const kdop_prox_contact* in, kdop_prox_contact* out, int is_device,
                               uint64_t* n_out);
 
+/* The same reduction with no allocation: in_dev and out_dev are device
+ * arrays of n records (not aliased), and the sort keys, flags and scans live
+ * in one fixed array of the library per GPU, held until the process ends,
+ * sized for at most KDOP_NOALLOC_MAX_CONTACTS records (about 23 MB); a larger
+ * n returns KDOP_ERR_CAPACITY and nothing runs.  The call allocates no device
+ * or host memory (no hipMalloc, hipFree or allocating library routine), so
+ * callers that cannot allocate, such as tight serving loops, can use it;
+ * like every call it completes before returning.  Every call on one GPU uses
+ * that GPU's one array: calls that overlap on the same GPU (from several
+ * contexts or threads) race.  Results equal kdop_reduce_prox_contacts' with
+ * is_device = 1.  No counterpart in the reference; it exists for the callers
+ * that cannot allocate. */
+#define KDOP_NOALLOC_MAX_CONTACTS 262144u
+int kdop_reduce_prox_contacts_noalloc(kdop_ctx* ctx, uint64_t n, const kdop_prox_contact* in_dev, kdop_prox_contact* out_dev,
+                                      uint64_t* n_out);
+
 /* Morton reorder of a tetrahedral mesh on the device (host arrays in and
  * out; outputs must not alias inputs): tets sorted by the Morton code of their
  * centroid, vertices renumbered in first-touch order, vertices no tet names
