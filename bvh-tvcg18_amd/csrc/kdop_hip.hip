@@ -43,6 +43,13 @@ struct BodyHost {
   float *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr;
   uint32_t* tets = nullptr;
   uint8_t* surf = nullptr;
+  // chunks wider than 32 leaves (topology built on the host): each tet's leaf
+  // slot, and the internal nodes ordered by height (union_off[h] .. union_off[h
+  // + 1]: the nodes of height h + 1)
+  uint32_t* leaf_slot = nullptr;
+  uint32_t* unions = nullptr;
+  std::vector<uint32_t> union_off;
+  bool wide() const { return L > 32; }
 };
 
 template <class T>
@@ -258,6 +265,80 @@ uint32_t host_resolved_root(const BodyHost& b, int h, uint32_t c) {
   }
 }
 
+// The median-split topology of a body whose chunks hold more than 32 leaves
+// (kdop_make_tree.h:57-121 branch chunks, :123-186 super chunks), in the
+// slot layout the device build writes for narrower chunks: per chunk, BFS
+// over element ranges; branch chunks store every range, super chunks only
+// ranges of two or more elements (a single element resolves to the child
+// chunk's root).  Fills meta (2 words per slot of the body), each tet's leaf
+// slot, and the internal nodes grouped by height for the refit.
+void build_wide_topology(const BodyHost& b, uint32_t body, const uint8_t* surf, std::vector<uint32_t>& meta,
+                         std::vector<uint32_t>& leaf_slot, std::vector<uint32_t>& unions, std::vector<uint32_t>& union_off) {
+  meta.assign(2 * (size_t)b.nnodes, 0xFFFFFFFFu);
+  meta[0] = meta[1] = 0u;  // the root record
+  leaf_slot.assign(b.nt, 0u);
+  std::vector<uint32_t> qlo, qhi;
+  for (uint32_t h = 0; h < b.H; ++h) {
+    const bool branch = h == b.H - 1;
+    for (uint32_t ch = 0; ch < b.C[h]; ++ch) {
+      const uint32_t n = host_chunk_elems(b, (int)h, ch);
+      if (!branch && n < 2) continue;
+      const uint32_t base = b.level_base[h] + ch * host_stride(b, (int)h) + 1;  // body-local slot of BFS node 0
+      const uint32_t first = ch * b.L;
+      qlo.assign(1, 0u);
+      qhi.assign(1, n - 1u);
+      for (size_t head = 0; head < qlo.size(); ++head) {
+        const uint32_t lo = qlo[head], hi = qhi[head], local = base + (uint32_t)head;
+        const uint32_t rootflag = (branch && head == 0) ? F_ROOT : 0u;
+        if (lo == hi) {  // branch leaf
+          const uint32_t tet = first + lo;
+          meta[2 * (size_t)local] = F_LEAF | F_BRANCH | tet;
+          meta[2 * (size_t)local + 1] = rootflag | ((surf[tet] & 15u) ? F_SURF : 0u) | body;
+          leaf_slot[tet] = b.node_base + local;
+          continue;
+        }
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t rlo[2] = {lo, mid + 1u}, rhi[2] = {mid, hi};
+        uint32_t child[2];
+        for (int s2 = 0; s2 < 2; ++s2) {
+          if (branch || rlo[s2] < rhi[s2]) {
+            child[s2] = b.node_base + base + (uint32_t)qlo.size();
+            qlo.push_back(rlo[s2]);
+            qhi.push_back(rhi[s2]);
+          } else {
+            child[s2] = host_resolved_root(b, (int)h + 1, first + rlo[s2]);
+          }
+        }
+        meta[2 * (size_t)local] = (branch ? F_BRANCH : 0u) | child[0];
+        meta[2 * (size_t)local + 1] = rootflag | child[1];
+      }
+    }
+  }
+  // heights: children sit in later slots (within a chunk in BFS order, across
+  // chunks on lower levels), so a reverse sweep sees them first
+  std::vector<uint8_t> height(b.nnodes, 0);
+  uint32_t hmax = 0;
+  std::vector<uint32_t> count(1, 0);
+  for (int64_t i = (int64_t)b.nnodes - 1; i >= 1; --i) {
+    const uint32_t m0 = meta[2 * (size_t)i], m1 = meta[2 * (size_t)i + 1];
+    if (m0 == 0xFFFFFFFFu || (m0 & F_LEAF)) continue;
+    const uint32_t c0 = (m0 & IDX_MASK) - b.node_base, c1 = (m1 & IDX_MASK) - b.node_base;
+    const uint32_t ht = 1u + std::max(height[c0], height[c1]);
+    height[i] = (uint8_t)ht;
+    if (ht > hmax) { hmax = ht; count.resize(hmax + 1, 0); }
+    ++count[ht];
+  }
+  union_off.assign(hmax + 1, 0);
+  for (uint32_t h = 1; h <= hmax; ++h) union_off[h] = union_off[h - 1] + count[h];
+  unions.assign(union_off[hmax], 0u);
+  std::vector<uint32_t> fill(union_off.begin(), union_off.end() - 1);
+  for (uint32_t i = 1; i < b.nnodes; ++i) {
+    const uint32_t m0 = meta[2 * (size_t)i];
+    if (m0 == 0xFFFFFFFFu || (m0 & F_LEAF)) continue;
+    unions[fill[height[i] - 1]++] = b.node_base + i;
+  }
+}
+
 int upload_bodies(kdop_ctx* c) {
   std::vector<BodyDev> h(c->bodies.size());
   for (size_t i = 0; i < c->bodies.size(); ++i) {
@@ -288,6 +369,21 @@ struct RefitLeaves {
   static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, uint32_t n) {
     hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, n, c->nodes.p,
                        c->dt);
+  }
+};
+// A body with chunks wider than 32 leaves: leaf DOPs at their slots, then
+// its internal nodes height by height.
+template <int K>
+struct RefitWide {
+  static void run(kdop_ctx* c, uint32_t body, const BodyHost& b) {
+    hipLaunchKernelGGL(k_refit_tet_slots<K>, dim3((b.nt + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, body,
+                       (const uint32_t*)b.leaf_slot, c->nodes.p, c->dt);
+    for (size_t h = 1; h < b.union_off.size(); ++h) {
+      const uint32_t lo = b.union_off[h - 1], cnt = b.union_off[h] - lo;
+      if (cnt)
+        hipLaunchKernelGGL(k_refit_union_list<K>, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)b.unions + lo, cnt,
+                           c->nodes.p);
+    }
   }
 };
 template <int K>
@@ -400,6 +496,10 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   std::vector<uint32_t> pbody, pchunk;
   for (uint32_t i = 0; i < n; ++i) {
     const BodyHost& b = c->bodies[bodies[i]];
+    if (b.wide()) {  // refitted from its own lists below
+      plan.push_back((uint32_t)acc);
+      continue;
+    }
     const uint64_t full = (uint64_t)(b.nt / b.L) * b.L;
     acc += (full + 63) / 64 * 64;
     plan.push_back((uint32_t)acc);
@@ -418,7 +518,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     uint32_t tot = 0;
     for (uint32_t i = 0; i < n; ++i) {
       const BodyHost& b = c->bodies[bodies[i]];
-      if (b.H < s + 1) continue;
+      if (b.H < s + 1 || b.wide()) continue;
       const int h = (int)b.H - 1 - (int)s;
       // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64)
       const uint32_t cf = (s == 1 && b.L * b.L <= 64) ? (b.nt / b.L) / b.L : 0u;
@@ -466,6 +566,10 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   for (const Step& st : steps)
     dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
                            st.m);
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    if (b.wide()) dispatch_K<RefitWide>(c->K, c, bodies[i], b);
+  }
   dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);
   HIPCHK(c, hipGetLastError());
   if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -697,6 +801,8 @@ void kdop_destroy(kdop_ctx* c) {
     (void)hipFree(b.X0); (void)hipFree(b.Y0); (void)hipFree(b.Z0);
     (void)hipFree(b.X); (void)hipFree(b.Y); (void)hipFree(b.Z);
     (void)hipFree(b.tets); (void)hipFree(b.surf);
+    if (b.leaf_slot) (void)hipFree(b.leaf_slot);
+    if (b.unions) (void)hipFree(b.unions);
   }
   release(c->nodes); release(c->dbodies); release(c->dposes); release(c->dtops); release(c->dpairs); release(c->dseeds);
   release(c->front[0]); release(c->front[1]); release(c->p1); release(c->p2);
@@ -899,15 +1005,20 @@ int kdop_tree_shape(int32_t K, uint64_t chunk_bytes, uint64_t nt, uint64_t* out)
 int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, const float* Z0, uint32_t nt, const uint32_t* tets,
                   const uint8_t* surface, uint32_t* body_id) {
   if (!c) return KDOP_ERR_ARG;
+  return kdop_add_body_ex(c, c->cfg.chunk_bytes, nv, X0, Y0, Z0, nt, tets, surface, body_id);
+}
+
+int kdop_add_body_ex(kdop_ctx* c, uint64_t mem_bytes, uint32_t nv, const float* X0, const float* Y0, const float* Z0, uint32_t nt,
+                     const uint32_t* tets, const uint8_t* surface, uint32_t* body_id) {
+  if (!c) return KDOP_ERR_ARG;
   if (!X0 || !Y0 || !Z0 || !tets || !surface || !body_id) return fail(c, KDOP_ERR_ARG, "kdop_add_body: null argument");
   if (nv == 0 || nt == 0) return fail(c, KDOP_ERR_ARG, "kdop_add_body: empty mesh (nv=%u nt=%u)", nv, nt);
   if (nt >= (1u << 30)) return fail(c, KDOP_ERR_ARG, "kdop_add_body: too many tetrahedra (%u)", nt);
   for (size_t i = 0; i < 4 * (size_t)nt; ++i)
     if (tets[i] >= nv) return fail(c, KDOP_ERR_ARG, "kdop_add_body: tet vertex index %u out of range (nv=%u)", tets[i], nv);
   Shape s;
-  if (!chunk_shape(c->K, c->cfg.chunk_bytes, nt, s))
-    return fail(c, KDOP_ERR_ARG, "kdop_add_body: chunk_bytes=%llu holds no two-leaf chunk for K=%d", (unsigned long long)c->cfg.chunk_bytes, c->K);
-  if (s.L > 32) return fail(c, KDOP_ERR_ARG, "kdop_add_body: %llu leaves per chunk; at most 32 supported (chunk_bytes <= 8191 gives at most 32)", (unsigned long long)s.L);
+  if (!chunk_shape(c->K, mem_bytes, nt, s))
+    return fail(c, KDOP_ERR_ARG, "kdop_add_body: mem_bytes=%llu holds no two-leaf chunk for K=%d", (unsigned long long)mem_bytes, c->K);
   if (s.H > (uint64_t)kMaxLevels) return fail(c, KDOP_ERR_ARG, "kdop_add_body: %llu chunk levels", (unsigned long long)s.H);
   HIPCHK(c, hipSetDevice(c->cfg.device));
   BodyHost b;
@@ -966,21 +1077,41 @@ int kdop_add_body(kdop_ctx* c, uint32_t nv, const float* X0, const float* Y0, co
   HIPCHK(c, hipMemcpyAsync(b.Z0, Z0, nv * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(b.tets, tets, 16 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(b.surf, surface, nt, hipMemcpyHostToDevice, c->stream));
-  c->nodes_used += b.nnodes;
   const uint32_t id = (uint32_t)c->bodies.size();
+  std::vector<uint32_t> hmeta, hleaf, hunions;
+  if (b.wide()) {  // the topology from the host, and the refit lists
+    build_wide_topology(b, id, surface, hmeta, hleaf, hunions, b.union_off);
+    if ((rc = alloc((void**)&b.leaf_slot, 4 * (size_t)nt)) || (rc = alloc((void**)&b.unions, 4 * std::max<size_t>(hunions.size(), 1)))) {
+      c->tops.resize(b.top_base);
+      return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(b.leaf_slot, hleaf.data(), 4 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
+    if (!hunions.empty()) HIPCHK(c, hipMemcpyAsync(b.unions, hunions.data(), 4 * hunions.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  c->nodes_used += b.nnodes;
   c->bodies.push_back(b);
   rc = upload_bodies(c);
   if (rc) return rc;
-  // device construction: topology, then refit to the material coordinates
-  TopoParams P;
-  std::memset(&P, 0, sizeof(P));
-  P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt; P.body = id;
-  uint32_t total_chunks = 0;
-  for (uint32_t h = 0; h < b.H; ++h) { P.C[h] = b.C[h]; P.E[h] = b.E[h]; P.level_base[h] = b.level_base[h]; total_chunks += b.C[h]; }
   // free slots read as meta 0xFFFFFFFF; the root record's meta is 0
   HIPCHK(c, hipMemsetAsync(c->nodes.p + (size_t)b.node_base * S, 0xFF, (size_t)b.nnodes * S * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->nodes.p + (size_t)b.node_base * S, 0, S * sizeof(float), c->stream));
-  hipLaunchKernelGGL(k_build_topology, dim3((total_chunks + 63) / 64), dim3(64), 0, c->stream, P, c->nodes.p, c->K, b.surf, total_chunks);
+  if (b.wide()) {
+    uint32_t* dmeta = nullptr;
+    if ((rc = alloc((void**)&dmeta, hmeta.size() * 4))) return rc;
+    HIPCHK(c, hipMemcpyAsync(dmeta, hmeta.data(), hmeta.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_write_meta, dim3((b.nnodes + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)dmeta, b.nnodes, b.node_base,
+                       c->K, c->nodes.p);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dmeta);
+    if (e != hipSuccess) return fail(c, KDOP_ERR_HIP, "kdop_add_body: topology upload: %s", hipGetErrorString(e));
+  } else {  // device construction of the topology
+    TopoParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.node_base = b.node_base; P.H = b.H; P.L = b.L; P.nt = nt; P.body = id;
+    uint32_t total_chunks = 0;
+    for (uint32_t h = 0; h < b.H; ++h) { P.C[h] = b.C[h]; P.E[h] = b.E[h]; P.level_base[h] = b.level_base[h]; total_chunks += b.C[h]; }
+    hipLaunchKernelGGL(k_build_topology, dim3((total_chunks + 63) / 64), dim3(64), 0, c->stream, P, c->nodes.p, c->K, b.surf, total_chunks);
+  }
   HIPCHK(c, hipGetLastError());
   rc = run_refit(c, 1, &id, nullptr, nullptr, true);
   if (rc) return rc;

@@ -609,6 +609,49 @@ __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ 
   }
 }
 
+// Bodies with chunks wider than 32 leaves (a per-body mem_bytes above about
+// 8 kB, or one chunk of every tet): the topology comes from the host, and the
+// refit runs from two host-built lists.  Leaves: one thread per tet writes
+// its DOP at leaf_slot[tet].
+template <int K>
+__global__ __launch_bounds__(256) void k_refit_tet_slots(const BodyDev* __restrict__ bodies, uint32_t body,
+                                                         const uint32_t* __restrict__ leaf_slot, float* __restrict__ nodes,
+                                                         DirTable dt) {
+  const BodyDev& B = bodies[body];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.nt) return;
+  float v[K];
+  tet_dop<K>(B, t, dt, v);
+  store_vol<K>(nodes, leaf_slot[t], v);
+}
+// Internal nodes of one height (1 + the larger height of the two children,
+// leaves 0), one thread each: the union of its children (left, right), in
+// the reference's order (geometry_make_union.h:9-25).  Launched height by
+// height, so every child is final before its parent reads it.
+template <int K>
+__global__ __launch_bounds__(256) void k_refit_union_list(const uint32_t* __restrict__ list, uint32_t n, float* __restrict__ nodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t slot = list[i];
+  const uint2 m = load_meta<K>(nodes, slot);
+  NodeRec<K> a, b;
+  load_rec<K>(nodes, m.x & IDX_MASK, a);
+  load_rec<K>(nodes, m.y & IDX_MASK, b);
+  float v[K];
+#pragma unroll
+  for (int f = 0; f < K; ++f) v[f] = (f & 1) ? smax(a.v[f], b.v[f]) : smin(a.v[f], b.v[f]);
+  store_vol<K>(nodes, slot, v);
+}
+// Meta words of a host-built topology: record i of the body gets m0, m1 =
+// meta[2 i], meta[2 i + 1] (the volumes are written by the refit).
+__global__ void k_write_meta(const uint32_t* __restrict__ meta, uint32_t nnodes, uint32_t node_base, int K, float* __restrict__ nodes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  uint32_t* m = reinterpret_cast<uint32_t*>(nodes + (size_t)(node_base + i) * rec_stride(K) + K);
+  m[0] = meta[2 * (size_t)i];
+  m[1] = meta[2 * (size_t)i + 1];
+}
+
 // Root volume of each body = union of its top-level chunk roots
 // (kdop_refit_tree.h:154-161).  One 64-lane block per batch item.
 template <int K>

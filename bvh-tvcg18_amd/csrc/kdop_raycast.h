@@ -136,7 +136,7 @@ __device__ inline bool raycast_tet(const f3& o, const f3& r, const f3* P, uint32
 // walk from FLT_MAX did, reaches a subset of its leaves, and finds no hit
 // below h >= L.
 
-constexpr uint32_t kChunkStack = 8;  // a chunk tree is at most log2 32 = 5 deep
+constexpr uint32_t kChunkStack = 16;  // a chunk walk holds at most depth + 1 <= log2 L + 1 entries: L <= 2^15
 
 // The reference's walk below one branch-chunk root (kdop_raycast.h:46-100)
 // from `length`: returns the best length, hit updated with it.

@@ -35,7 +35,9 @@ struct MortonKey {
 // two children when one is a leaf), then the order the contact generator
 // produced the points in (the contact's ordinal).
 struct ContactSeqKey {
-  uint32_t pair, ca, cb, path, ord, idx;
+  uint32_t pair, ca, cb;
+  unsigned long long path;
+  uint32_t ord, idx;
   __device__ bool operator<(const ContactSeqKey& o) const {
     if (pair != o.pair) return pair < o.pair;
     if (ca != o.ca) return ca < o.ca;
@@ -44,7 +46,7 @@ struct ContactSeqKey {
     if (ord != o.ord) return ord < o.ord;
     return idx < o.idx;
   }
-  __device__ static ContactSeqKey top() { return ContactSeqKey{~0u, ~0u, ~0u, ~0u, ~0u, ~0u}; }
+  __device__ static ContactSeqKey top() { return ContactSeqKey{~0u, ~0u, ~0u, ~0ull, ~0u, ~0u}; }
 };
 
 // Descent path to the leaf pair (ja, jb) of two branch chunks of na and nb
@@ -53,11 +55,12 @@ struct ContactSeqKey {
 // internal side's choice, left-aligned in 2 * kMaxPathSteps bits so that
 // integer order is the descent's order (two leaf pairs of one chunk pair
 // share their steps up to the node pair where they part, and no path is a
-// prefix of another).  A chunk holds at most 32 tets, so a path has at most
-// max(5, 5) steps.
-constexpr uint32_t kMaxPathSteps = 16;
-__device__ inline uint32_t descent_path(uint32_t ja, uint32_t na, uint32_t jb, uint32_t nb) {
-  uint32_t loA = 0, hiA = na - 1u, loB = 0, hiB = nb - 1u, key = 0, steps = 0;
+// prefix of another).  A path has as many steps as the deeper chunk tree,
+// at most ceil(log2 L) <= 30 for the node counts the library takes.
+constexpr uint32_t kMaxPathSteps = 32;
+__device__ inline unsigned long long descent_path(uint32_t ja, uint32_t na, uint32_t jb, uint32_t nb) {
+  uint32_t loA = 0, hiA = na - 1u, loB = 0, hiB = nb - 1u, steps = 0;
+  unsigned long long key = 0;
   while (loA < hiA || loB < hiB) {
     const bool ia = loA < hiA, ib = loB < hiB;
     uint32_t ca = 0, cb = 0;
@@ -74,7 +77,7 @@ __device__ inline uint32_t descent_path(uint32_t ja, uint32_t na, uint32_t jb, u
     key = (key << 2) | ((ia && ib) ? 2u * ca + cb : (ia ? ca : cb));
     ++steps;
   }
-  return steps ? key << (2u * (kMaxPathSteps - steps)) : 0u;
+  return steps ? key << (2u * (kMaxPathSteps - steps)) : 0ull;
 }
 
 // A float as a uint32 whose unsigned order is the float order, with -0 and +0
@@ -259,7 +262,7 @@ __global__ void k_contact_keys_seq(const kdop_contact* __restrict__ c, uint32_t 
   const BodyDev& A = bodies[bp.x];
   const BodyDev& B = bodies[bp.y];
   const uint32_t ca = x.tet_a / A.L, cb = x.tet_b / B.L;
-  const uint32_t path = descent_path(x.tet_a - ca * A.L, chunk_size(A, ca), x.tet_b - cb * B.L, chunk_size(B, cb));
+  const unsigned long long path = descent_path(x.tet_a - ca * A.L, chunk_size(A, ca), x.tet_b - cb * B.L, chunk_size(B, cb));
   keys[i] = ContactSeqKey{x.pair, ca, cb, path, x.ordinal, i};
 }
 

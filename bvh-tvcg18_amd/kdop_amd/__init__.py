@@ -217,15 +217,22 @@ class Context:
         return {k: int(chosen[v - 1]) for k, v in _abi.OPTIONS.items()}, best.value
 
     # -- kdop::make_tree
-    def make_tree(self, mesh):
+    def make_tree(self, mesh, mem_bytes=None):
+        """kdop::make_tree(mem_bytes, mesh, ...): mem_bytes None takes the
+        context's chunk_bytes (kdop_add_body), else this body's own
+        (kdop_add_body_ex; SIZE_MAX = 2**64 - 1 gives one chunk of every tet)."""
         X = np.ascontiguousarray(mesh.X, np.float32)
         Y = np.ascontiguousarray(mesh.Y, np.float32)
         Z = np.ascontiguousarray(mesh.Z, np.float32)
         T = np.ascontiguousarray(mesh.tets, np.uint32)
         S = np.ascontiguousarray(mesh.surface, np.uint8)
         bid = ctypes.c_uint32()
-        self._check(self._lib.kdop_add_body(self._h, X.shape[0], ptr(X), ptr(Y), ptr(Z), T.shape[0], ptr(T), ptr(S),
-                                            ctypes.byref(bid)))
+        if mem_bytes is None:
+            rc = self._lib.kdop_add_body(self._h, X.shape[0], ptr(X), ptr(Y), ptr(Z), T.shape[0], ptr(T), ptr(S), ctypes.byref(bid))
+        else:
+            rc = self._lib.kdop_add_body_ex(self._h, int(mem_bytes), X.shape[0], ptr(X), ptr(Y), ptr(Z), T.shape[0], ptr(T), ptr(S),
+                                            ctypes.byref(bid))
+        self._check(rc)
         self.meshes.append(mesh)
         return bid.value
 

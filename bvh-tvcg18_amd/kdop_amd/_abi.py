@@ -91,6 +91,7 @@ SIGNATURES = [
     ("kdop_destroy", None, [_p]),
     ("kdop_last_error", ctypes.c_char_p, [_p]),
     ("kdop_add_body", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p]),
+    ("kdop_add_body_ex", ctypes.c_int, [_p, _u64, _u32, _p, _p, _p, _u32, _p, _p, _p]),
     ("kdop_body_shape", ctypes.c_int, [_p, _u32, _p]),
     ("kdop_refit", ctypes.c_int, [_p, _u32, _p, _p, _p]),
     ("kdop_refit_device", ctypes.c_int, [_p, _u32, _p, _p, _p]),

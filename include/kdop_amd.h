@@ -113,12 +113,25 @@ const char* kdop_last_error(const kdop_ctx* ctx);
 /* Upload a tetrahedral mesh (material coordinates X0/Y0/Z0, tets as 4 vertex
  * indices i,j,k,m, surface bits: bit v set when the face opposite vertex v is on
  * the surface) and build its chunked kDOP tree on the device, refitted to the
- * material coordinates.
+ * material coordinates, with the context's chunk_bytes as mem_bytes.
  * Replaces kdop::make_tree<V,K,T>(mem_bytes, mesh, X, Y, Z, sequential)
  * (SIMULATION/KDOP/KDOP/include/kdop_make_tree.h:190-288) as called from
  * narrow::make_kdop_bvh (SIMULATION/NARROW/NARROW/include/narrow_object.h:96-134). */
 int kdop_add_body(kdop_ctx* ctx, uint32_t nv, const float* X0, const float* Y0, const float* Z0,
                   uint32_t nt, const uint32_t* tets, const uint8_t* surface, uint32_t* body_id);
+
+/* The same with the tree shape of this body's own mem_bytes, the first
+ * argument of kdop::make_tree (kdop_make_tree.h:190-229): the chunk size
+ * follows from how many kdop::Node<float,K> fit in mem_bytes / 2, so any
+ * value the reference accepts is accepted here, including chunks of more
+ * than 32 leaves (narrow_chunk_bytes from the PROX configuration,
+ * simulators_prox_parameters.cpp:54,394) and SIZE_MAX, one chunk of every
+ * tet (narrow_object.h:104-111).  Bodies of one context may differ in
+ * mem_bytes.  Chunks wider than 32 leaves get their topology from the host
+ * and a refit that unions one tree height per launch; the results are the
+ * same bits either way. */
+int kdop_add_body_ex(kdop_ctx* ctx, uint64_t mem_bytes, uint32_t nv, const float* X0, const float* Y0, const float* Z0,
+                     uint32_t nt, const uint32_t* tets, const uint8_t* surface, uint32_t* body_id);
 
 /* Tree shape of a body: out[0..4] = {L leaves per chunk, C branch chunks, H levels,
  * device node count, depth (longest root-to-leaf path)}. */

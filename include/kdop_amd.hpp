@@ -83,16 +83,20 @@ class Context {
   Context& operator=(const Context&) = delete;
   kdop_ctx* get() const { return c_; }
 
-  /* kdop::make_tree on the material coordinates: tets holds 4 vertex
-   * indices per tetrahedron, surface one byte per tetrahedron (bit v: the face
-   * opposite its v-th vertex has no neighbour, compute_surface_map's flags). */
+  /* kdop::make_tree(mem_bytes, ...) on the material coordinates: tets holds 4
+   * vertex indices per tetrahedron, surface one byte per tetrahedron (bit v:
+   * the face opposite its v-th vertex has no neighbour, compute_surface_map's
+   * flags).  mem_bytes 0: the context's chunk_bytes. */
   Tree make_tree(const std::vector<float>& X, const std::vector<float>& Y, const std::vector<float>& Z, const std::vector<uint32_t>& tets,
-                 const std::vector<uint8_t>& surface) {
+                 const std::vector<uint8_t>& surface, uint64_t mem_bytes = 0) {
     if (X.size() != Y.size() || X.size() != Z.size() || tets.size() % 4 != 0 || surface.size() != tets.size() / 4)
       throw Error(KDOP_ERR_ARG, "make_tree: inconsistent mesh arrays");
     Tree t;
-    check(kdop_add_body(c_, (uint32_t)X.size(), X.data(), Y.data(), Z.data(), (uint32_t)(tets.size() / 4), tets.data(), surface.data(),
-                        &t.body));
+    const uint32_t nv = (uint32_t)X.size(), nt = (uint32_t)(tets.size() / 4);
+    if (mem_bytes)
+      check(kdop_add_body_ex(c_, mem_bytes, nv, X.data(), Y.data(), Z.data(), nt, tets.data(), surface.data(), &t.body));
+    else
+      check(kdop_add_body(c_, nv, X.data(), Y.data(), Z.data(), nt, tets.data(), surface.data(), &t.body));
     return t;
   }
 

@@ -13,11 +13,13 @@ def world_coords(mesh, pose):
 
 def oracle_run(wl, narrow=True, threads=1, chunk_bytes=8000):
     """Oracle traversal of every body pair of a workload.  Returns a list of
-    (counters, p1, p2, contacts) per pair and the oracle trees per mesh."""
+    (counters, p1, p2, contacts) per pair and the oracle trees per mesh.
+    chunk_bytes: make_tree's mem_bytes, one value or one per mesh."""
     trees, worlds = [], []
-    for m, pose in zip(wl.meshes, wl.poses):
+    per_mesh = chunk_bytes if isinstance(chunk_bytes, (list, tuple)) else [chunk_bytes] * len(wl.meshes)
+    for m, pose, mb in zip(wl.meshes, wl.poses, per_mesh):
         X, Y, Z = world_coords(m, pose)
-        t = O.Tree(wl.K, chunk_bytes, m.tets, m.X, m.Y, m.Z)  # make_tree on material coordinates
+        t = O.Tree(wl.K, mb, m.tets, m.X, m.Y, m.Z)  # make_tree on material coordinates
         t.refit(X, Y, Z)                                 # update_kdop_bvh
         trees.append(t)
         worlds.append((X, Y, Z))

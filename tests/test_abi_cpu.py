@@ -51,6 +51,12 @@ def test_tree_shape_matches_quoted_workload_shapes():
         assert (s["L"], s["C"], s["H"]) == (16, 20000, 4)
     s = K.tree_shape(14, 8000, 50000)
     assert (s["L"], s["C"], s["H"]) == (16, 3125, 3)
+    # chunks wider than 32 leaves (a per-body mem_bytes): K=8 nodes are 56 B, so
+    # 16000 / 2 / 56 = 142 -> 128 slots -> 64 leaves; 32000 -> 256 slots -> 128
+    # leaves; SIZE_MAX: the smallest power of two >= 2 M slots, one chunk
+    assert K.tree_shape(8, 16000, 5000) == dict(node_bytes=56, n_perfect=127, L=64, C=79, H=2)
+    assert K.tree_shape(8, 32000, 5000) == dict(node_bytes=56, n_perfect=255, L=128, C=40, H=1)
+    assert K.tree_shape(8, 2 ** 64 - 1, 5000) == dict(node_bytes=56, n_perfect=16383, L=8192, C=1, H=1)
 
 
 def test_cpp_header_compiles():

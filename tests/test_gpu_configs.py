@@ -100,6 +100,41 @@ def test_self_contact_completes():
         ctx.close()
 
 
+SIZE_MAX = 2 ** 64 - 1
+
+
+@pytest.mark.parametrize("k,mem_a,mem_b", [(8, 16000, 16000), (8, 32000, 32000), (8, 32000, 8000), (26, 131072, 16000),
+                                           (14, SIZE_MAX, SIZE_MAX), (18, SIZE_MAX, 8000)],
+                         ids=["k8_L64", "k8_L128", "k8_L128_vs_L32", "k26_L256_vs_L16", "k14_one_chunk", "k18_one_chunk_vs_L16"])
+def test_per_body_mem_bytes(k, mem_a, mem_b):
+    """kdop_add_body_ex with each body's own mem_bytes (make_tree's first
+    argument, kdop_make_tree.h:190-229): chunks of 64 to 256 leaves, one chunk
+    of every tet (SIZE_MAX, the gProximity setting), and two bodies of one
+    context built with different values.  Tree shape, every tree volume, the
+    counts, pair sets and contacts equal the oracle's built with the same
+    mem_bytes, over a refit to a new pose and in both traversal strategies."""
+    wl = W.large_pair(n=10, K=k, seed=13)
+    orc, trees, worlds = oracle_run(wl, chunk_bytes=[mem_a, mem_b], threads=ORACLE_THREADS)
+    for mode in (K.KDOP_TRAVERSAL_PERSISTENT, K.KDOP_TRAVERSAL_LEVELS):
+        ctx = K.Context(K=k, traversal=mode)
+        ids = [ctx.make_tree(m, mem_bytes=mb) for m, mb in zip(wl.meshes, (mem_a, mem_b))]
+        for bid, mb, t in zip(ids, (mem_a, mem_b), trees):
+            sh, ref = ctx.shape(bid), K.tree_shape(k, mb, wl.meshes[0].nt)
+            assert (sh["L"], sh["C"], sh["H"]) == (ref["L"], ref["C"], ref["H"])
+            assert len(t.levels()) == ref["H"] and t.levels()[-1] == ref["C"]
+        dev = device_run(wl, ctx=ctx, ids=ids)
+        print(k, mem_a, mem_b, "mode", mode, "L", ctx.shape(ids[0])["L"], ctx.shape(ids[1])["L"], "tests",
+              dev["counters"]["tests"], flush=True)
+        _check_pair_results(wl, dev, orc)
+        for i, bid in enumerate(ids):
+            ex, te = trees[i].export(), ctx.export_tree(bid)
+            np.testing.assert_array_equal(te["root"], trees[i].root())
+            ref = {(int(f), int(l)): v for (f, l), v in zip(ex["range"], ex["vol"])}
+            for (f, l), v in zip(te["range"], te["vol"]):
+                np.testing.assert_array_equal(v, ref[(int(f), int(l))])
+        ctx.close()
+
+
 def test_c4_refit_series():
     """64 overlapping pairs (20k-80k tets, K=18), 200 steps of rigid motion,
     refit (never rebuilt) and re-traversed every step on the device; the

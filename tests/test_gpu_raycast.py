@@ -28,9 +28,10 @@ def _rays(rng, n, center, radius, spread):
 
 @pytest.mark.parametrize("k,n,chunk_bytes,nrays", [(8, 10, 8000, 4096), (26, 12, 8000, 4096), (14, 12, 8000, 2048),
                                                    (26, 12, 2000, 2048), (26, 40, 8000, 256), (6, 12, 8000, 2048),
-                                                   (18, 12, 8000, 2048), (12, 11, 8000, 1024), (20, 11, 8000, 1024)],
+                                                   (18, 12, 8000, 2048), (12, 11, 8000, 1024), (20, 11, 8000, 1024),
+                                                   (8, 10, 32000, 2048), (14, 10, 2 ** 64 - 1, 1024)],
                          ids=["k8_box10", "k26_box12", "k14_box12", "k26_L2", "k26_box40", "k6_box12", "k18_box12", "k12_box11",
-                              "k20_box11"])
+                              "k20_box11", "k8_L128", "k14_one_chunk"])
 def test_raycast_matches_oracle(k, n, chunk_bytes, nrays):
     rng = np.random.default_rng(n * 31 + k)
     mesh = K.box_mesh(n, size=1.0, jitter=0.1, seed=7)

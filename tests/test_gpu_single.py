@@ -18,9 +18,10 @@ F32 = np.float32
                                                        (14, 12, 8000, 2048, False), (26, 12, 2000, 2048, False),
                                                        (26, 40, 8000, 512, False), (6, 12, 8000, 2048, True),
                                                        (18, 12, 8000, 2048, False), (12, 11, 8000, 1024, False),
-                                                       (20, 11, 8000, 1024, True)],
+                                                       (20, 11, 8000, 1024, True), (8, 10, 32000, 2048, False),
+                                                       (26, 10, 2 ** 64 - 1, 1024, True)],
                          ids=["k8_box10", "k26_box12_flip", "k14_box12", "k26_L2", "k26_box40", "k6_box12_flip", "k18_box12",
-                              "k12_box11", "k20_box11_flip"])
+                              "k12_box11", "k20_box11_flip", "k8_L128", "k26_one_chunk_flip"])
 def test_single_traversal_matches_oracle(k, n, chunk_bytes, nsph, flip):
     rng = np.random.default_rng(n * 13 + k)
     mesh = K.box_mesh(n, size=1.0, jitter=0.1, seed=4)

@@ -30,14 +30,21 @@ def _oracle_report(orc):
     return out
 
 
-@pytest.mark.parametrize("wl_name", ["batched14", "pair26"])
+@pytest.mark.parametrize("wl_name", ["batched14", "pair26", "pair8_L128", "pair14_one_chunk"])
 def test_sort_contacts_sequential_is_the_report_order(wl_name):
+    """Also for chunks of 128 leaves and one chunk of every tet (mem_bytes
+    SIZE_MAX), whose descent paths run 7 and 13 steps deep."""
+    chunk_bytes = {"pair8_L128": 32000, "pair14_one_chunk": 2 ** 64 - 1}.get(wl_name, 8000)
     if wl_name == "batched14":
         wl = W.batched_pairs(npairs=16, K=14, seed=4, tmin=500, tmax=8000)
+    elif wl_name == "pair8_L128":
+        wl = W.large_pair(n=10, K=8, seed=3)
+    elif wl_name == "pair14_one_chunk":
+        wl = W.large_pair(n=10, K=14, seed=3)
     else:
         wl = W.large_pair(n=12, K=26, seed=3)
-    orc, _, _ = oracle_run(wl)
-    dev = device_run(wl)
+    orc, _, _ = oracle_run(wl, chunk_bytes=chunk_bytes)
+    dev = device_run(wl, chunk_bytes=chunk_bytes)
     ctx = dev["ctx"]
     ctx.sort_contacts("sequential")
     got = ctx.contacts()
