@@ -25,6 +25,11 @@ sys.path.insert(0, os.path.join(ROOT, "bvh-tvcg18_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def rec_stride(K):
+    """Floats per node record: the power of two >= K + 2 (8, 16 or 32)."""
+    return 8 if K + 2 <= 8 else (16 if K + 2 <= 16 else 32)
 METRIC = "BVTT node-pair tests/s + batched pair-queries/s; % HBM roofline; vs CPU"
 
 
@@ -115,15 +120,23 @@ def headline_leg(args):
     ctx.set_option("kernel_timing", 0)
     trav_ms /= args.steps
     narrow_ms /= args.steps
-    node_bytes = 4 * args.k + 8
-    # algorithmic bytes per launch of the dominant kernel: its node-pair tests x 2 (4K + 8)
-    per_launch_bytes = ptests * 2 * node_bytes / max(launches, 1)
     per_launch_ms = launch_ms / max(launches, 1)
-    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    # a record-counting call (measurement, not timed): the distinct node records
+    # the persistent rounds read; the set does not depend on the schedule
+    ctx.set_option("count_records", 1)
+    step()
+    torch.cuda.synchronize()
+    cst = ctx.traversal_stats()
+    ctx.set_option("count_records", 0)
+    # gather roofs of this node pool, loaded the way the expansion loads records:
+    # random records within 16,384 slots (2 MB: L2-resident) and over the whole pool
+    roof_l2 = ctx.gather_roof(window=16384)
+    roof_pool = ctx.gather_roof(window=0)
     return dict(wl=wl, ctx=ctx, ids=ids, counters=counters, ms_per_step=ms_per_step, trav_ms=trav_ms,
-                narrow_ms=narrow_ms, per_launch_bytes=per_launch_bytes, per_launch_ms=per_launch_ms,
-                launches_per_step=launches / args.steps, achieved_gbs=achieved, sched=sched,
-                round_ms=[x / n for x, n in zip(round_ms, round_n)] if round_ms else None)
+                narrow_ms=narrow_ms, per_launch_ms=per_launch_ms, launches_per_step=launches / args.steps, sched=sched,
+                ptests_per_launch=ptests / max(launches, 1), distinct_records=cst["distinct_records"],
+                count_launches=cst["persist_launches"], records_loaded=cst["records_loaded"], roof_l2=roof_l2,
+                roof_pool=roof_pool, round_ms=[x / n for x, n in zip(round_ms, round_n)] if round_ms else None)
 
 
 def measured_traffic(kernel):
@@ -163,6 +176,22 @@ def batched_leg(args):
         c = ctx.tandem_traversal(pairs, narrow=True)
     torch.cuda.synchronize()
     t_batched = (time.perf_counter() - t0) / reps
+    # the same batched call under each traversal strategy with the tuned options
+    # otherwise (the one-launch persistent design beside the level-synchronous one)
+    strategy_ms = {}
+    tuned_strategy = ctx.get_option("traversal")
+    for name, mode in (("persistent", KA.KDOP_TRAVERSAL_PERSISTENT), ("levels", KA.KDOP_TRAVERSAL_LEVELS)):
+        ctx.set_option("traversal", mode)
+        for _ in range(2):
+            ctx.tandem_traversal(pairs, narrow=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cs = ctx.tandem_traversal(pairs, narrow=True)
+        torch.cuda.synchronize()
+        strategy_ms[name] = (time.perf_counter() - t0) / reps * 1e3
+        assert cs["contacts"] == c["contacts"] and cs["tests"] == c["tests"], (name, cs, c)
+    ctx.set_option("traversal", tuned_strategy)
     # unbatched: one call per body pair
     for i in range(min(8, pairs.shape[0])):
         ctx.tandem_traversal(pairs[i:i + 1], narrow=True)
@@ -209,6 +238,8 @@ def batched_leg(args):
                 separated_pairs=wl.info["separated"],
                 pair_queries_per_s=n / t_batched, unbatched_pair_queries_per_s=n / t_single,
                 batched_speedup=t_single / t_batched, batched_ms=t_batched * 1e3, unbatched_ms=t_single * 1e3,
+                persistent_ms=strategy_ms["persistent"], levels_ms=strategy_ms["levels"],
+                persistent_vs_levels=strategy_ms["persistent"] / strategy_ms["levels"],
                 tests=c["tests"], contacts=c["contacts"], autotuned=tuned, autotune_best_ms=tuned_ms,
                 contact_sort={"contacts": int(con.shape[0]), "device_ms": sort_ms,
                               "host_numpy_lexsort_opencl_order_ms": t_host * 1e3},
@@ -438,13 +469,71 @@ def cpu_leg(args, head, batched=None):
         sr = _t.perf_counter() - t0
         red = dict(ms=sr * 1e3, kept=int(kept.shape[0]), cores=1, kind="port",
                    sample=f"oracle std::stable_sort + std::unique of the batched call's {pi['d'].shape[0]} contacts")
-    # the same query on every host core this job may use (rows taken one at a time), whole query
-    cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+    # the same query on every host core this process may run on (rows taken one
+    # at a time), whole query
+    host = host_cpu_info()
+    cores = host["affinity_cpus"]
     cm, sm = O.traverse_rows_mt(trees[0], meshA, trees[1], meshB, 0, rows_total, cores)
     allc = dict(value=cm["tests"] / sm, unit="node-pair tests/s", cores=cores, kind="port",
                 sample=f"oracle tandem traversal + narrow phase, all {rows_total} branch rows of body A on {cores} threads "
-                       f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm)
+                       f"({cm['tests']} tests in {sm:.2f} s)", full_query_s=sm, host=host)
+    one["host"] = host
     return one, allc, ray, sph, red
+
+
+def host_cpu_info():
+    """The host CPUs: nproc, the CPUs this process may run on, the cgroup CPU
+    quota (cpu.max) and the model name (lscpu)."""
+    import subprocess
+    info = dict(nproc=os.cpu_count(), affinity_cpus=len(os.sched_getaffinity(0)))
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        info["cgroup_cpu_quota"] = None if q == "max" else float(q) / float(per)
+    except (OSError, ValueError):
+        info["cgroup_cpu_quota"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def contact_fp64_error(head):
+    """Every contact of the headline query against the FP64 restatement of its
+    tet pair with the float32 run's discrete choices (face, candidates, extreme
+    points) frozen: the largest error of points, normal and depth relative to
+    max(|x|, 1).  Also checks that the device's float32 contacts are the
+    oracle's, bit for bit."""
+    import numpy as np
+    from oracle import oracle as O
+    wl, ctx = head["wl"], head["ctx"]
+    con = ctx.contacts()
+    ma, mb = wl.meshes
+    PA = np.stack(O.transform_f32(ma.X, ma.Y, ma.Z, wl.poses[0].p, wl.poses[0].q), 1)
+    PB = np.stack(O.transform_f32(mb.X, mb.Y, mb.Z, wl.poses[1].p, wl.poses[1].q), 1)
+    key = np.lexsort((con["ordinal"], con["tet_b"], con["tet_a"]))
+    con = con[key]
+    worst, pairs, i = 0.0, 0, 0
+    while i < con.shape[0]:
+        j = i
+        while j < con.shape[0] and con["tet_a"][j] == con["tet_a"][i] and con["tet_b"][j] == con["tet_b"][i]:
+            j += 1
+        ta, tb = int(con["tet_a"][i]), int(con["tet_b"][i])
+        r = O.contacts_pair_codes(PA[ma.tets[ta]], PB[mb.tets[tb]], ma.surface[ta], mb.surface[tb])
+        assert r["pts"].shape[0] == j - i and r["pts"].tobytes() == con["p"][i:j].tobytes(), (ta, tb)
+        n64, d64, p64 = O.frozen_pair_f64(PA[ma.tets[ta]], PB[mb.tets[tb]], r["face"], r["codes"], r["uniq"], r["imin"], r["imax"])
+        for got, want in ((con["p"][i:j], p64), (con["n"][i:j], np.broadcast_to(n64, (j - i, 3))),
+                          (con["depth"][i:j], np.full(j - i, d64))):
+            err = np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), 1.0)
+            worst = max(worst, float(err.max()))
+        pairs += 1
+        i = j
+    return dict(contacts=int(con.shape[0]), pairs=pairs, max_rel_err=worst,
+                note="device float32 contacts vs the FP64 restatement with the float32 choices frozen; "
+                     "relative to max(|x|, 1)")
 
 
 def main():
@@ -539,10 +628,25 @@ def main():
             f"{cpu_all['cores']} cores: {cpu_all['full_query_s']:.2f} s")
     if batched:
         batched.pop("_prox_in", None)
+    fp64 = None
+    if not args.no_cpu:
+        log("[bench] contacts vs the FP64 restatement")
+        fp64 = contact_fp64_error(head)
     steps_per_s = 1e3 / head["ms_per_step"]
-    bw = head["sched"]["block_waves"]
-    kernel = f"k_persist_blk<{args.k},512,{bw}>" if bw > 1 else f"k_persist<{args.k},{head['sched']['queue_depth']}>"
+    sc = head["sched"]
+    bw, ls = sc["block_waves"], sc["queue_depth"]
+    if sc["item_lanes"] == 4:
+        kernel = f"k_persist_quad<{args.k},{bw},{(ls * bw).bit_length() - 1}>"
+    else:
+        kernel = f"k_persist_blk<{args.k},512,{bw}>" if bw > 1 else f"k_persist<{args.k},{ls}>"
     traffic = measured_traffic(kernel)
+    launch_ms = head["per_launch_ms"]
+    rec_line = 4 * rec_stride(args.k)                    # bytes of one node record (one cache line at K >= 18)
+    rec_used = 16 * ((args.k + 2 + 3) // 4)              # the words a record load reads
+    cl = max(head["count_launches"], 1)
+    compulsory = head["distinct_records"] / cl * rec_line  # each distinct record read from memory once
+    gathered = head["records_loaded"] / cl * rec_used      # the record bytes the expansions load
+    gbs = lambda b: b / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": cnt["tests"] * steps_per_s,
@@ -559,14 +663,27 @@ def main():
         "config": {"workload": f"box{args.n}_jitter_pair_k{args.k}", "tets_per_body": head["wl"].info["nt"],
                    "K": args.k, "chunk_bytes": 8000, "aabb_overlap": round(head["wl"].info["aabb_overlap"], 3),
                    "step": "refit both bodies + tandem traversal + narrow phase"},
-        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": gbs(compulsory), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs(compulsory) / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_gbs": gbs(traffic["bytes"]) if traffic else None,
+                     "traffic_frac": gbs(traffic["bytes"]) / HBM_PEAK_GBS if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "kernel": kernel, "launch_ms": head["per_launch_ms"], "round_ms": head["round_ms"],
+                     "kernel": kernel, "launch_ms": launch_ms, "round_ms": head["round_ms"],
                      "launches_per_step": head["launches_per_step"],
-                     "algorithmic_bytes_per_launch": head["per_launch_bytes"],
-                     "bytes_per_test": 2 * (4 * args.k + 8)},
+                     "algorithmic_bytes_per_launch": compulsory,
+                     "algorithmic": f"distinct node records the launch reads ({head['distinct_records'] / cl:.0f}, "
+                                    f"counted on the device) x {rec_line} B: every one must come from memory at least once"},
+        "gather": {"bytes_per_launch": gathered, "achieved": gbs(gathered), "unit": "GB/s",
+                   "roof_l2": head["roof_l2"], "frac_l2": gbs(gathered) / head["roof_l2"] if head["roof_l2"] else None,
+                   "roof_pool": head["roof_pool"], "frac_pool": gbs(gathered) / head["roof_pool"] if head["roof_pool"] else None,
+                   "records_per_launch": head["records_loaded"] / cl, "bytes_per_record": rec_used,
+                   "tests_per_launch": head["ptests_per_launch"],
+                   "survey_bytes_per_test": 2 * (4 * args.k + 8),
+                   "note": "record bytes the expansions load (one load per record and item) per launch time, against the "
+                           "rate of the same loads of random records within 2 MB (roof_l2) and over the whole node pool "
+                           "(roof_pool), measured on this GPU by kdop_gather_roof"},
+        "contacts_fp64": fp64,
         "counters": cnt,
         "reference_tests_per_s": cnt["reference_tests"] * steps_per_s,
         "traversal_ms": head["trav_ms"],

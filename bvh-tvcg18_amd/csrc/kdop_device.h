@@ -110,9 +110,10 @@ struct alignas(256) CounterStripe {
   unsigned long long grabs;          // persistent: frontier chunks taken
   unsigned long long ptests;         // persistent: node-pair tests made by k_persist
   unsigned long long handoffs;       // persistent: items the tail hand-off passed to the next round
+  unsigned long long records;        // persistent: node records loaded by the expansions (one per record and item)
   unsigned long long max_iters;      // persistent: most iterations of one wave (max, not sum; last field)
 };
-constexpr int kStripeFields = 13;
+constexpr int kStripeFields = 14;
 
 // Segments of the persistent traversal's frontier: each round's input is
 // split into kSegs segments with their own chunk counters (one lane of a

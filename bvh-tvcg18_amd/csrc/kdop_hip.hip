@@ -30,6 +30,11 @@ using namespace kdop_amd;
 // Stack keep that never hands work on: above any stack, and small enough that
 // keep x waves per workgroup cannot overflow.
 constexpr uint32_t kNoHandOffKeep = 1u << 20;
+// Default iteration budget of a persistent round: above the longest chain of
+// a block on the 320k-tet pairs (38 iterations at K = 26, 52 at K = 14), so
+// such a query runs one round, while a batch whose heavy pairs would keep a
+// few blocks busy for hundreds of iterations is spread over later rounds.
+constexpr uint32_t kRoundItersDefault = 64;
 
 namespace {
 
@@ -124,6 +129,8 @@ struct kdop_ctx {
   int ls = 0, wpc = 0;
   int bw = 0;                // waves per persistent workgroup (1: a stack per wave; 2, 4, 8: a shared stack)
   int item_lanes = 4;        // lanes per work item in the persistent rounds (4: k_persist_quad; 1: k_persist / k_persist_blk)
+  int quad_items = 1;        // KDOP_OPT_QUAD_ITEMS: items per quad per iteration of k_persist_quad (1 or 2)
+  uint32_t round_iters = kRoundItersDefault;  // KDOP_OPT_ROUND_ITERS: a block's iteration budget per round once the frontier is gone (0: none)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
@@ -148,6 +155,9 @@ struct kdop_ctx {
   double timing[4 + kMaxRounds] = {};  // see kdop_last_timing
   uint32_t last_levels = 0;
   uint32_t last_fallback_levels = 0;  // levels launched after the persistent rounds ran out (last call)
+  bool count_records = false;         // KDOP_OPT_COUNT_RECORDS: mark the records the persistent rounds read
+  DevBuf<uint8_t> touched;            // ... one byte per node slot
+  uint64_t last_distinct_records = 0; // distinct records the last counting call's rounds read
   void* scratch = nullptr;            // this device's address of the allocation-free entry point's array
   std::string scratch_err;            // why it is null
 };
@@ -384,6 +394,12 @@ struct RefitWide {
         hipLaunchKernelGGL(k_refit_union_list<K>, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, (const uint32_t*)b.unions + lo, cnt,
                            c->nodes.p);
     }
+  }
+};
+template <int K>
+struct GatherProbe {
+  static void run(kdop_ctx* c, const uint32_t* idx, uint32_t nidx, uint32_t* out, uint32_t blocks) {
+    hipLaunchKernelGGL(k_gather_probe<K>, dim3(blocks), dim3(256), 0, c->stream, c->nodes.p, idx, nidx, out);
   }
 };
 template <int K>
@@ -642,22 +658,34 @@ void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t rou
 }
 // A persistent round of W-wave workgroups with 4 lanes per work item
 // (k_persist_quad) and a shared stack of 2^SL items.
-template <int K, int W, int SL>
+template <int K, int W, int SL, bool COUNT = false, int IPQ = 1>
 void launch_round_quad(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
   const dim3 grid((unsigned)std::max(1, c->num_cus * wpc_of(c) / W)), block(64 * W);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
+  // the budget applies in rounds that may spill (the last one allowed takes
+  // no budget: its spill would only go to the level-synchronous finish)
+  const uint32_t budget = (c->round_iters && round + 1 < c->max_rounds) ? c->round_iters : ~0u;
   if (c->timing_level >= 2)
-    hipExtLaunchKernelGGL((k_persist_quad<K, W, SL>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1], 0, T, b0,
-                          b1, cap, c->dstate, round, round_keep(c, round), last);
+    hipExtLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1],
+                          0, T, b0, b1, cap, c->dstate, round, round_keep(c, round), last, budget);
   else
-    hipLaunchKernelGGL((k_persist_quad<K, W, SL>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round, round_keep(c, round),
-                       last);
+    hipLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round,
+                       round_keep(c, round), last, budget);
 }
 template <int K>
 struct PersistRound {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
     const int bw = bw_of(c), ls = ls_of(c);
+    if (c->count_records) {  // record-counting calls: the quad kernel in its default shape
+      launch_round_quad<K, 4, 10, true>(c, T, cap, round, last);
+      return;
+    }
+    if (c->item_lanes == 4 && c->quad_items == 2) {  // two items per quad, 4-wave workgroups
+      if (ls >= 512) launch_round_quad<K, 4, 11, false, 2>(c, T, cap, round, last);
+      else launch_round_quad<K, 4, 10, false, 2>(c, T, cap, round, last);
+      return;
+    }
     if (c->item_lanes == 4) {  // stack of KDOP_OPT_QUEUE_DEPTH (256 or 512) entries per wave
       const bool deep = ls >= 512;
       if (bw <= 2) { if (deep) launch_round_quad<K, 2, 10>(c, T, cap, round, last); else launch_round_quad<K, 2, 9>(c, T, cap, round, last); }
@@ -809,7 +837,7 @@ void kdop_destroy(kdop_ctx* c) {
   release(c->contacts); release(c->recs); release(c->gate); release(c->adj_acc); release(c->adj_bodies); release(c->plan); release(c->poses);
   release(c->contacts_tmp); release(c->sortbuf); release(c->raybuf); release(c->sphere_out);
   release(c->redbuf); release(c->prox_io); release(c->sg_items[0]); release(c->sg_items[1]); release(c->sg_scan); release(c->sg_kind);
-  release(c->rc_best); release(c->rc_first);
+  release(c->rc_best); release(c->rc_first); release(c->touched);
   if (c->dstate) (void)hipFree(c->dstate);
   if (c->hsum) (void)hipHostFree(c->hsum);
   if (c->hwords) (void)hipHostFree(c->hwords);
@@ -878,6 +906,19 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) break;
       c->bw = (int)v;
       return KDOP_OK;
+    case KDOP_OPT_ROUND_ITERS:  // 0: no budget
+      if (v < 0 || v > (1ll << 20)) break;
+      c->round_iters = (uint32_t)v;
+      return KDOP_OK;
+    case KDOP_OPT_QUAD_ITEMS:
+      if (v == 0) v = 1;
+      if (v != 1 && v != 2) break;
+      c->quad_items = (int)v;
+      return KDOP_OK;
+    case KDOP_OPT_COUNT_RECORDS:
+      if (v != 0 && v != 1) break;
+      c->count_records = v != 0;
+      return KDOP_OK;
     case KDOP_OPT_MAX_ROUNDS:
       if (v == 0) v = kMaxRounds - 1;
       if (v < 1 || v > kMaxRounds - 1) break;
@@ -922,6 +963,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   if (rc) return rc;
   double best = 0.0;
   if ((rc = measure(&best))) return rc;
+  const bool start_persistent = c->last_persistent;  // the strategy the starting options ran (AUTO resolved)
   // candidate values per option; the launch-shape ones depend on the lanes
   // per item chosen before them (the quad kernel runs more, wider workgroups)
   auto candidates = [&](int opt) -> std::vector<int64_t> {
@@ -936,11 +978,12 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{256, 512, 1024};
       // no hand-off, or a keep that hands on the bottom of deep stacks
       case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
+      case KDOP_OPT_ROUND_ITERS: return {0, 16, 64};
       case KDOP_OPT_RAMP_TARGET: return {4 * pw, 16 * pw, 64 * pw};
       default: return {16384, 65536, 262144};  // KDOP_OPT_SEED_BUDGET
     }
   };
-  const int order[] = {KDOP_OPT_TRAVERSAL, KDOP_OPT_ITEM_LANES, KDOP_OPT_BLOCK_WAVES, KDOP_OPT_WAVES_PER_CU,
+  const int order[] = {KDOP_OPT_TRAVERSAL, KDOP_OPT_ITEM_LANES, KDOP_OPT_ROUND_ITERS, KDOP_OPT_BLOCK_WAVES, KDOP_OPT_WAVES_PER_CU,
                        KDOP_OPT_QUEUE_DEPTH, KDOP_OPT_STACK_KEEP, KDOP_OPT_RAMP_TARGET, KDOP_OPT_SEED_BUDGET};
   for (const int opt : order) {
     struct {
@@ -960,6 +1003,9 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       if (rc) { kdop_set_option(c, k.opt, cur); return rc; }
       if (ms < best) { best = ms; best_v = v; }
     }
+    // an AUTO start that stayed best is pinned to the strategy it ran
+    if (k.opt == KDOP_OPT_TRAVERSAL && best_v == KDOP_TRAVERSAL_AUTO)
+      best_v = start_persistent ? KDOP_TRAVERSAL_PERSISTENT : KDOP_TRAVERSAL_LEVELS;
     kdop_set_option(c, k.opt, best_v);
     // the levels strategy ignores the persistent knobs
     if (k.opt == KDOP_OPT_TRAVERSAL && best_v == KDOP_TRAVERSAL_LEVELS) break;
@@ -991,6 +1037,9 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_BLOCK_WAVES: *v = bw_of(c); return KDOP_OK;
     case KDOP_OPT_ITEM_LANES: *v = c->item_lanes; return KDOP_OK;
     case KDOP_OPT_MAX_ROUNDS: *v = c->max_rounds; return KDOP_OK;
+    case KDOP_OPT_COUNT_RECORDS: *v = c->count_records ? 1 : 0; return KDOP_OK;
+    case KDOP_OPT_QUAD_ITEMS: *v = c->quad_items; return KDOP_OK;
+    case KDOP_OPT_ROUND_ITERS: *v = c->round_iters; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
 }
@@ -1400,13 +1449,13 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
-  // AUTO: the persistent strategy where its depth-first waves beat one launch
-  // per level (tight DOPs, up to a few million tets per call); the
-  // level-synchronous one for wide frontiers (small K, big batches)
-  uint64_t call_tets = 0;
-  for (uint32_t i = 0; i < npairs; ++i) call_tets += (uint64_t)c->bodies[pairs[i].x].nt + c->bodies[pairs[i].y].nt;
+  // AUTO: the persistent strategy where its depth-first rounds beat one
+  // launch per level -- tight DOPs (K >= 12), and batches of many pairs at
+  // any K, whose heavy pairs the round budget spreads over later rounds; the
+  // level-synchronous one for a few pairs of loose DOPs (K = 6, 8), whose
+  // wide frontiers fill the GPU level by level
   const bool persistent = c->cfg.traversal == KDOP_TRAVERSAL_PERSISTENT ||
-                          (c->cfg.traversal == KDOP_TRAVERSAL_AUTO && c->K >= 14 && call_tets <= (2ull << 20));
+                          (c->cfg.traversal == KDOP_TRAVERSAL_AUTO && (c->K >= 12 || npairs >= 64));
   est = std::max<uint64_t>(est, 1 << 16);
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;
@@ -1435,6 +1484,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.p1_cap = keep_p1 ? (uint32_t)std::min<size_t>(c->p1.cap, 0xFFFFFFFFu) : 0u;
     T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
     T.npairs = npairs;
+    T.touched = nullptr;
+    if (c->count_records && persistent) {
+      if ((rc = ensure(c, c->touched, std::max<size_t>(c->nodes_used, 1)))) return rc;
+      HIPCHK(c, hipMemsetAsync(c->touched.p, 0, c->nodes_used, c->stream));
+      T.touched = c->touched.p;
+    }
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
     uint32_t rounds = 0, levels_queued = 0;
     const uint32_t last = std::min(levels, c->bfs_max);
@@ -1478,7 +1533,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       return enqueue_summary(c, cap, persistent ? (int)rounds : -1);
     };
     std::vector<uint64_t> sig;
-    if (c->use_graphs && c->timing_level == 0 && npairs) {
+    if (c->use_graphs && c->timing_level == 0 && !c->count_records && npairs) {
       sig = {
           (uint64_t)c->K, npairs, persistent, last, levels_queued, levels, target, cap, total_seeds, narrow, one_pass,
           (uint64_t)(uintptr_t)T.p1, (uint64_t)(uintptr_t)T.p2, T.p1_cap, T.p2_cap, (uint64_t)(uintptr_t)c->nodes.p,
@@ -1486,7 +1541,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
           (uint64_t)(uintptr_t)c->gate.p, (uint64_t)wpc_of(c), (uint64_t)ls_of(c), c->keep, (uint64_t)c->num_cus, (uint64_t)bw_of(c), rounds_async,
-          (uint64_t)c->item_lanes, c->max_rounds};
+          (uint64_t)c->item_lanes, c->max_rounds, (uint64_t)c->quad_items, c->round_iters};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;
@@ -1605,6 +1660,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     c->last_levels = persistent ? 0 : levels;
     c->last_fallback_levels = fallback_levels;
+    c->last_distinct_records = 0;
+    if (T.touched) {  // count the marked records (a measurement call; the copy is not timed)
+      std::vector<uint8_t> h(c->nodes_used);
+      HIPCHK(c, hipMemcpy(h.data(), c->touched.p, h.size(), hipMemcpyDeviceToHost));
+      for (const uint8_t v : h) c->last_distinct_records += v ? 1u : 0u;
+    }
     c->last_rounds = persistent ? rounds : 0;
     c->last_persistent = persistent;
     c->last_npairs = npairs;
@@ -1728,7 +1789,7 @@ int kdop_export_tree(kdop_ctx* c, uint32_t body, uint64_t cap, float* vol, uint3
 
 int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
   if (!c || !out) return KDOP_ERR_ARG;
-  std::memset(out, 0, 16 * sizeof(uint64_t));
+  std::memset(out, 0, 20 * sizeof(uint64_t));
   out[10] = (uint64_t)ls_of(c);
   out[11] = (uint64_t)wpc_of(c);
   out[13] = (uint64_t)c->item_lanes;
@@ -1736,16 +1797,59 @@ int kdop_traversal_stats(kdop_ctx* c, uint64_t* out) {
   if (!c->have_traversal || !c->last_persistent) return KDOP_OK;
   const CallSummary& S = *c->hsum;
   out[0] = S.final_level; out[1] = S.final_count; out[2] = c->last_rounds; out[3] = S.total[7];
-  out[4] = S.total[8]; out[5] = S.total[12]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * wpc_of(c);
+  out[4] = S.total[8]; out[5] = S.total[13]; out[6] = S.total[9]; out[7] = (uint64_t)c->num_cus * wpc_of(c);
   out[8] = S.total[10]; out[9] = c->last_rounds;
   out[12] = S.total[11];
   out[15] = c->last_fallback_levels;
+  out[16] = S.total[12];
+  out[17] = c->last_distinct_records;
   return KDOP_OK;
 }
 
 int kdop_last_timing(kdop_ctx* c, double* out) {
   if (!c || !out) return KDOP_ERR_ARG;
   for (int i = 0; i < 4 + kMaxRounds; ++i) out[i] = c->timing[i];
+  return KDOP_OK;
+}
+
+int kdop_gather_roof(kdop_ctx* c, uint64_t window, uint32_t nloads, uint32_t seed, double* gbytes_per_s) {
+  if (!c || !gbytes_per_s) return KDOP_ERR_ARG;
+  *gbytes_per_s = 0.0;
+  if (c->nodes_used < 2) return fail(c, KDOP_ERR_STATE, "kdop_gather_roof: no bodies");
+  if (nloads < 1024 || nloads > (1u << 26)) return fail(c, KDOP_ERR_ARG, "kdop_gather_roof: nloads %u not in [1024, 2^26]", nloads);
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  const uint64_t span = std::min<uint64_t>(window ? window : c->nodes_used, c->nodes_used);
+  std::vector<uint32_t> h(nloads);
+  uint64_t x = 0x9E3779B97F4A7C15ull ^ seed;
+  for (uint32_t i = 0; i < nloads; ++i) {  // xorshift64*, uniform slots of the window
+    x ^= x >> 12; x ^= x << 25; x ^= x >> 27;
+    h[i] = (uint32_t)(((x * 0x2545F4914F6CDD1Dull) >> 32) % span);
+  }
+  const uint32_t blocks = (uint32_t)c->num_cus * 16u;
+  uint32_t *didx = nullptr, *dout = nullptr;
+  if (hipMalloc(&didx, 4 * (size_t)nloads) != hipSuccess) return fail(c, KDOP_ERR_NOMEM, "kdop_gather_roof: hipMalloc");
+  if (hipMalloc(&dout, 4 * (size_t)blocks * 256) != hipSuccess) { (void)hipFree(didx); return fail(c, KDOP_ERR_NOMEM, "kdop_gather_roof: hipMalloc"); }
+  int rc = KDOP_OK;
+  std::vector<float> ms;
+  if (hipMemcpy(didx, h.data(), 4 * (size_t)nloads, hipMemcpyHostToDevice) != hipSuccess) rc = fail(c, KDOP_ERR_HIP, "kdop_gather_roof: copy");
+  for (int rep = 0; rep < 7 && rc == KDOP_OK; ++rep) {  // one warm-up, then the median of six
+    if (hipEventRecord(c->ev[0], c->stream) != hipSuccess) { rc = fail(c, KDOP_ERR_HIP, "kdop_gather_roof: event"); break; }
+    dispatch_K<GatherProbe>(c->K, c, (const uint32_t*)didx, nloads, dout, blocks);
+    if (hipEventRecord(c->ev[1], c->stream) != hipSuccess || hipEventSynchronize(c->ev[1]) != hipSuccess) {
+      rc = fail(c, KDOP_ERR_HIP, "kdop_gather_roof: %s", hipGetErrorString(hipGetLastError()));
+      break;
+    }
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+    if (rep) ms.push_back(t);
+  }
+  (void)hipFree(didx);
+  (void)hipFree(dout);
+  if (rc) return rc;
+  std::sort(ms.begin(), ms.end());
+  const double med = 0.5 * (ms[2] + ms[3]);
+  const double rec_bytes = 16.0 * ((c->K + 2 + 3) / 4);  // the words a record's load reads: K volumes and 2 meta words
+  *gbytes_per_s = (double)(nloads / 4 * 4) * rec_bytes / (med * 1e-3) / 1e9;
   return KDOP_OK;
 }
 

@@ -683,6 +683,7 @@ struct TravArgs {
   uint32_t* p2_fill;
   uint32_t p1_cap, p2_cap;
   uint32_t npairs;
+  uint8_t* touched;          // record-counting calls only: byte per node slot, set when a round reads the record
 };
 
 // Which sides of an overlapping node pair (x, y) expand: the reference's rule
@@ -791,17 +792,24 @@ struct QuadRec {
 // A lane loads only its slots that hold part of the record (lanes past the
 // used words keep what they had); lane_base: the pool as 16-byte words,
 // offset by this lane's first word.
+// Every lane loads all its slots, so the loads take no branches (measured
+// faster than predicating the lanes and records that are not needed): a word
+// past the used ones is the record's padding, or for K = 6 its last word
+// again; quad_sep masks the words that hold no slab.  lane_base: the pool as
+// 16-byte words, offset by this lane's first word.
 template <int K>
-__device__ inline void load_quad(const float4* __restrict__ lane_base, uint32_t idx, uint32_t r, QuadRec<K>& q) {
+__device__ inline void load_quad(const float4* __restrict__ lane_base, uint32_t idx, QuadRec<K>& q) {
   constexpr uint32_t RW = rec_stride(K) / 4;  // 16-byte words per record
   const float4* p = lane_base + (size_t)idx * RW;
 #pragma unroll
-  for (int t = 0; t < Quad<K>::NQ; ++t)
-    if (r + 4u * t < (uint32_t)Quad<K>::QU) q.w[t] = p[4 * t];
+  for (int t = 0; t < Quad<K>::NQ; ++t) q.w[t] = p[4 * t];
 }
+// This lane's first word of a record (lanes past the record's words for K = 6
+// read its last word; they test no slab).
 template <int K>
 __device__ inline const float4* quad_lane_base(const float* __restrict__ nodes, uint32_t r) {
-  return reinterpret_cast<const float4*>(nodes) + r;
+  constexpr uint32_t RW = rec_stride(K) / 4;
+  return reinterpret_cast<const float4*>(nodes) + (r < RW ? r : RW - 1u);
 }
 // This lane's share of the strict separation test of X and Y: the slabs held
 // in its words (lo0 hi0 lo1 hi1 per word; lanes past the volume test none).
@@ -834,22 +842,21 @@ struct QuadOut {
 };
 
 // Every lane of the wave must call it (quad-wide DPP); live: the quad holds
-// an item (quad-uniform).  A0 .. B1 are the caller's loop-carried registers:
-// records the item does not need (a side that does not expand, a dead quad)
-// are not loaded and keep stale values, which only feed combinations whose
-// test flag is false.
+// an item (quad-uniform; a dead quad's item is all zeros, so its loads read
+// record 0).  A side that does not expand loads its one record twice, which
+// keeps the loads free of branches.  A0 .. B1: the caller's registers for the
+// four records.
 template <int K>
 __device__ inline void expand_quad(const float4* __restrict__ lane_base, const Item& it, bool live, QuadRec<K>& A0, QuadRec<K>& A1,
                                    QuadRec<K>& B0, QuadRec<K>& B1, QuadOut& o) {
   const uint32_t r = (uint32_t)lane_id() & 3u, i = r >> 1, j = r & 1u;
   const bool exp_a = (it.a0 & I_EXP) != 0u, exp_b = (it.b0 & I_EXP) != 0u;
-  const uint32_t ca0 = it.a0 & IDX_MASK, ca1 = it.a1, cb0 = it.b0 & IDX_MASK, cb1 = it.b1;
-  if (live) {
-    load_quad<K>(lane_base, ca0, r, A0);
-    if (exp_a) load_quad<K>(lane_base, ca1, r, A1);
-    load_quad<K>(lane_base, cb0, r, B0);
-    if (exp_b) load_quad<K>(lane_base, cb1, r, B1);
-  }
+  const uint32_t ca0 = it.a0 & IDX_MASK, cb0 = it.b0 & IDX_MASK;
+  const uint32_t ca1 = exp_a ? it.a1 : ca0, cb1 = exp_b ? it.b1 : cb0;
+  load_quad<K>(lane_base, ca0, A0);
+  load_quad<K>(lane_base, ca1, A1);
+  load_quad<K>(lane_base, cb0, B0);
+  load_quad<K>(lane_base, cb1, B1);
   // separation bit of combination k in bit k, ORed over the quad
   uint32_t sep = (quad_sep<K>(A0, B0, r) ? 1u : 0u) | (quad_sep<K>(A0, B1, r) ? 2u : 0u) | (quad_sep<K>(A1, B0, r) ? 4u : 0u) |
                  (quad_sep<K>(A1, B1, r) ? 8u : 0u);
@@ -876,6 +883,35 @@ __device__ inline void expand_quad(const float4* __restrict__ lane_base, const I
   o.kid.b0 = ey ? (I_EXP | (Y.x & IDX_MASK)) : yi;
   o.kid.b1 = ey ? (Y.y & IDX_MASK) : 0u;
   o.kid.pair = it.pair;
+}
+
+// Gather roof (measurement only): quads load records idx[j] of the node pool
+// exactly as expand_quad loads an item's records (4 records per quad per
+// iteration, each lane its words r and r + 4 of the used ones), and fold them
+// into a checksum so no load is dropped.  The rate of record bytes this
+// reaches over random indices bounds what the traversal's loads can reach.
+template <int K>
+__global__ __launch_bounds__(256) void k_gather_probe(const float* __restrict__ nodes, const uint32_t* __restrict__ idx, uint32_t nidx,
+                                                      uint32_t* __restrict__ out) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, r = tid & 3u;
+  const uint32_t quad = tid >> 2, nquads = (gridDim.x * blockDim.x) >> 2;
+  const float4* __restrict__ base = quad_lane_base<K>(nodes, r);
+  QuadRec<K> q[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int t = 0; t < Quad<K>::NQ; ++t) q[u].w[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t acc = 0;
+  for (uint32_t j = 4u * quad; j + 3u < nidx; j += 4u * nquads) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) load_quad<K>(base, idx[j + u], q[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < Quad<K>::NQ; ++t)
+        acc ^= __float_as_uint(q[u].w[t].x) ^ __float_as_uint(q[u].w[t].y) ^ __float_as_uint(q[u].w[t].z) ^ __float_as_uint(q[u].w[t].w);
+  }
+  out[tid] = acc;
 }
 
 // Write the leaf pairs of one lane (P1 when kept, P2 for the surface subset).

@@ -634,26 +634,35 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
 }
 
 // The shared-stack round with 4 lanes per work item (expand_quad): a block of
-// W waves expands up to 16 W items per iteration, each quad loading its
-// item's records in coalesced 64-byte runs.  The stack holds S = 2^SL items;
+// W waves expands up to 16 W IPQ items per iteration, each quad IPQ items
+// (slot q of quad g takes the iteration's item g + 16 W q), loading their
+// records in coalesced 64-byte runs.  The stack holds S = 2^SL items;
 // everything else (frontier grabs by wave 0, two barriers per iteration, the
 // tail hand-off, spills and the per-wave leaf staging) works as in
 // k_persist_blk.
-template <int K, int W, int SL>
+//
+// COUNT (record-counting calls, never timed): the quad's first lane also marks
+// every record its item reads in T.touched, so the host can count the
+// distinct records a round reads.  budget: iterations after which a block
+// whose frontier is gone hands its whole stack on (~0u: never).
+template <int K, int W, int SL, bool COUNT, int IPQ>
 __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
-                                                         CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last) {
-  constexpr uint32_t NT = 64u * W;  // threads
-  constexpr uint32_t IPI = 16u * W; // items per iteration
+                                                         CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last,
+                                                         uint32_t budget) {
+  constexpr uint32_t NT = 64u * W;          // threads
+  constexpr uint32_t QPI = 16u * W;         // quads
+  constexpr uint32_t IPI = QPI * IPQ;       // items per iteration
   constexpr uint32_t S = 1u << SL;
-  static_assert(S >= 2u * NT, "stack holds at least one hand-off beyond a full iteration");
+  static_assert(S >= 2u * NT && S >= 2u * IPI, "stack holds a full iteration and a hand-off");
   static_assert(kSegs == 64, "one lane of wave 0 per segment");
-  constexpr uint32_t SB = 128;  // staged leaf pairs per wave; flushed above SB - 64 (one iteration adds <= 64)
+  static_assert(IPQ == 1 || IPQ == 2, "one or two items per quad");
+  constexpr uint32_t SB = 64u * IPQ + 64u;  // staged leaf pairs per wave; flushed above SB - 64 IPQ (one iteration adds <= 64 IPQ)
   __shared__ uint32_t sa0[S], sa1[S], sb0[S], sb1[S], sp[S];
   __shared__ uint32_t lp_pair[W][SB], lp_ta[W][SB], lp_tb[W][SB];
   __shared__ uint32_t wtot[W];
   __shared__ uint32_t ctl[3];  // next grab: first frontier index, count; frontier done
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform(tid >> 6);
-  const uint32_t gi = w * 16u + (lane >> 2);  // this quad's item of an iteration
+  const uint32_t gi = w * 16u + (lane >> 2);  // this quad's item of an iteration (slot 0)
   const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, lane & 3u);
   uint32_t final_level = C->final_level, final_count = 0;
   if (round == 0) {
@@ -739,10 +748,8 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
   uint32_t nstage = 0;        // this wave's staged leaf pairs
   const bool keep_p1 = T.p1 != nullptr;
   uint32_t tests = 0, descent = 0, broot = 0, p1s = 0;
-  uint32_t iters = 0, lanes = 0, nsp = 0, handed = 0;
-  QuadRec<K> A0, A1, B0, B1;  // this lane's words of the item's records (loop-carried: see expand_quad)
-#pragma unroll
-  for (int t = 0; t < Quad<K>::NQ; ++t) A0.w[t] = A1.w[t] = B0.w[t] = B1.w[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t iters = 0, lanes = 0, nsp = 0, handed = 0, recs = 0;
+  QuadRec<K> A0[IPQ], A1[IPQ], B0[IPQ], B1[IPQ];  // this lane's words of each slot's records
   for (;;) {
     const uint32_t gstart = ctl[0], ng = ctl[1];
     const bool front_done = ctl[2] != 0u;
@@ -760,37 +767,85 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
       bot += NT;
       handed += NT;
     }
+    // ---- budget: once the frontier is gone, a block that has run `budget`
+    // iterations hands its whole stack to the next round and stops, so a deep
+    // subtree is spread over every block there instead of running on here
+    if (front_done && iters >= budget && top != bot) {
+      for (uint32_t c0 = 0; c0 < top - bot; c0 += NT) {
+        const uint32_t n = min(NT, top - bot - c0);
+        const uint32_t nw = n > 64u * w ? min(64u, n - 64u * w) : 0u;  // this wave's share
+        if (nw) {
+          const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, nw);
+          b = lane0(b) + lane;
+          if (lane < nw) {
+            const uint32_t s3 = (bot + c0 + tid) & (S - 1);
+            if (b < segcap) spill[(size_t)sg * segcap + b] = Item{sa0[s3], sa1[s3], sb0[s3], sb1[s3], sp[s3]};
+            else atomicOr(T.overflow, 1u);
+          }
+        }
+      }
+      handed += top - bot;
+      bot = top;
+    }
     const uint32_t ns = min(IPI, top - bot);
     if (ns + ng == 0u && front_done) break;  // block-uniform
-    // ---- batch: the top ns stack items, then ng frontier items, one per quad
-    Item it{0, 0, 0, 0, 0};
-    const bool live = gi < ns + ng;
-    if (gi < ns) {
-      const uint32_t s = (top - ns + gi) & (S - 1);
-      it = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
-    } else if (live) {
-      it = in[gstart + gi - ns];
+    // ---- batch: the top ns stack items, then ng frontier items; slot q of this quad
+    Item it[IPQ];
+    bool live[IPQ];
+#pragma unroll
+    for (int q = 0; q < IPQ; ++q) {
+      const uint32_t g = gi + (uint32_t)q * QPI;
+      it[q] = Item{0, 0, 0, 0, 0};
+      live[q] = g < ns + ng;
+      if (g < ns) {
+        const uint32_t s = (top - ns + g) & (S - 1);
+        it[q] = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
+      } else if (live[q]) {
+        it[q] = in[gstart + g - ns];
+      }
+      if (live[q] && (lane & 3u) == 0u) {
+        const uint32_t na = (it[q].a0 & I_EXP) ? 2u : 1u, nb = (it[q].b0 & I_EXP) ? 2u : 1u;
+        recs += na + nb;
+        if (COUNT) {
+          T.touched[it[q].a0 & IDX_MASK] = 1u;
+          if (na == 2u) T.touched[it[q].a1] = 1u;
+          T.touched[it[q].b0 & IDX_MASK] = 1u;
+          if (nb == 2u) T.touched[it[q].b1] = 1u;
+        }
+      }
     }
     top -= ns;
-    QuadOut o;
-    expand_quad<K>(qbase, it, live, A0, A1, B0, B1, o);
-    tests += o.test; descent += o.descent; broot += o.broot; p1s += o.leaf;
+    QuadOut o[IPQ];
+#pragma unroll
+    for (int q = 0; q < IPQ; ++q) expand_quad<K>(qbase, it[q], live[q], A0[q], A1[q], B0[q], B1[q], o[q]);
     ++iters;
     lanes += ns + ng;
-    const unsigned long long pm = __ballot(o.push);
-    const uint32_t kpre = __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
-    if (lane == 0) wtot[w] = (uint32_t)__popcll(pm);
+    unsigned long long pm[IPQ];
+    uint32_t kpre[IPQ], wsum = 0;
+#pragma unroll
+    for (int q = 0; q < IPQ; ++q) {  // pushes in slot order, lanes in order within a slot
+      tests += o[q].test; descent += o[q].descent; broot += o[q].broot; p1s += o[q].leaf;
+      pm[q] = __ballot(o[q].push);
+      kpre[q] = wsum + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm[q] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm[q], 0u));
+      wsum += (uint32_t)__popcll(pm[q]);
+    }
+    if (lane == 0) wtot[w] = wsum;
     // ---- leaf pairs: stage in this wave's LDS (all of P1 when it is kept, else only P2)
-    const bool lst = keep_p1 ? o.leaf : o.surf;
-    const unsigned long long lm = __ballot(lst);
-    if (lm) {
-      if (lst) {
-        const uint32_t o2 = nstage + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
-        lp_pair[w][o2] = o.kid.pair | (o.surf ? 0x80000000u : 0u);
-        lp_ta[w][o2] = o.ta;
-        lp_tb[w][o2] = o.tb;
+#pragma unroll
+    for (int q = 0; q < IPQ; ++q) {
+      const bool lst = keep_p1 ? o[q].leaf : o[q].surf;
+      const unsigned long long lm = __ballot(lst);
+      if (lm) {
+        if (lst) {
+          const uint32_t o2 = nstage + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
+          lp_pair[w][o2] = o[q].kid.pair | (o[q].surf ? 0x80000000u : 0u);
+          lp_ta[w][o2] = o[q].ta;
+          lp_tb[w][o2] = o[q].tb;
+        }
+        nstage += (uint32_t)__popcll(lm);
       }
-      nstage += (uint32_t)__popcll(lm);
     }
     __syncthreads();  // A: the stack reads are done and every wave's push total is in wtot
     uint32_t woff = 0, total = 0;
@@ -803,25 +858,29 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
     // ---- children: the shared stack when they fit, else the next round (per wave)
     if (total) {
       if ((top - bot) + total <= S) {
-        if (o.push) {
-          const uint32_t s2 = (top + woff + kpre) & (S - 1);
-          sa0[s2] = o.kid.a0; sa1[s2] = o.kid.a1; sb0[s2] = o.kid.b0; sb1[s2] = o.kid.b1; sp[s2] = o.kid.pair;
-        }
+#pragma unroll
+        for (int q = 0; q < IPQ; ++q)
+          if (o[q].push) {
+            const uint32_t s2 = (top + woff + kpre[q]) & (S - 1);
+            sa0[s2] = o[q].kid.a0; sa1[s2] = o[q].kid.a1; sb0[s2] = o[q].kid.b0; sb1[s2] = o[q].kid.b1; sp[s2] = o[q].kid.pair;
+          }
         top += total;
       } else {
         const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
-        if (pm) {
+        if (wsum) {
           uint32_t b = 0;
-          if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, (uint32_t)__popcll(pm));
-          b = lane0(b) + kpre;
-          if (o.push) {
-            if (b < segcap) spill[(size_t)sg * segcap + b] = o.kid;
-            else atomicOr(T.overflow, 1u);
-          }
+          if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, wsum);
+          b = lane0(b);
+#pragma unroll
+          for (int q = 0; q < IPQ; ++q)
+            if (o[q].push) {
+              if (b + kpre[q] < segcap) spill[(size_t)sg * segcap + b + kpre[q]] = o[q].kid;
+              else atomicOr(T.overflow, 1u);
+            }
         }
       }
     }
-    if (nstage > SB - 64u) {
+    if (nstage > SB - 64u * IPQ) {
       wave_lds_sync();
       flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
       nstage = 0;
@@ -843,6 +902,7 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
   wave_add64(&st->descent, descent);
   wave_add64(&st->branch_overlaps, broot);
   wave_add64(&st->p1, p1s);
+  wave_add64(&st->records, recs);
   if (tid == 0) {  // schedule statistics per block: iterations, items expanded, grabs, hand-offs
     atomicAdd(&st->iters, (unsigned long long)iters);
     atomicAdd(&st->lanes, (unsigned long long)lanes);
@@ -882,7 +942,7 @@ constexpr int kSumThreads = 1024;
 __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restrict__ C, uint32_t cap, int final_round,
                                                               CallSummary* __restrict__ host) {
   static_assert(kStripes == 64 && kSegs == 64, "one lane per stripe and per segment");
-  static_assert(kStripeFields == 13 && kMaxRounds == 8, "13 field waves, 3 round waves");
+  static_assert(kStripeFields == 14 && kMaxRounds == 8, "14 field waves, 2 round waves");
   static_assert(sizeof(CounterStripe) * kStripes == 16 * kSumThreads, "one 16-byte clear per thread");
   __shared__ uint32_t pending_s;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -900,9 +960,8 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
   } else {
     const uint32_t segcap = cap / kSegs;
 #pragma unroll
-    for (int h = 0; h < 3; ++h) {
-      const int r = 3 * (int)(w - 13u) + h;
-      if (r >= kMaxRounds) break;
+    for (int h = 0; h < 4; ++h) {
+      const int r = 4 * (int)(w - 14u) + h;
       const uint32_t f = C->seg[r][lane].fill;
       uint32_t tot = min(f, segcap), mx = f;
 #pragma unroll
@@ -916,7 +975,7 @@ __global__ __launch_bounds__(kSumThreads) void k_sum_counters(CallState* __restr
         if (r == final_round) pending_s = tot;
       }
     }
-    if (w == 13u) {
+    if (w == 14u) {
       S.level_fill[lane] = C->level_fill[lane];
       if (lane == 0) {
         S.overflow = C->overflow;

@@ -208,7 +208,7 @@ class Context:
     def autotune(self, pairs, narrow=True, budget_ms=2000.0):
         """kdop_autotune on this pair list: returns (chosen options, best median ms)."""
         P = np.ascontiguousarray(np.asarray(pairs, np.uint32).reshape(-1, 2))
-        chosen = np.zeros(16, np.int64)
+        chosen = np.zeros(20, np.int64)
         best = ctypes.c_double()
         self._check(self._lib.kdop_autotune(self._h, P.shape[0], ptr(P), KDOP_NARROW if narrow else 0, float(budget_ms),
                                             ptr(chosen), ctypes.byref(best)))
@@ -439,12 +439,19 @@ class Context:
         return dict(vol=vol, range=rng, kind=kind, root=root)
 
     def traversal_stats(self):
-        out = np.zeros(16, np.uint64)
+        out = np.zeros(20, np.uint64)
         self._check(self._lib.kdop_traversal_stats(self._h, ptr(out)))
         keys = ("ramp_levels", "frontier", "rounds", "iters", "lanes", "max_iters", "grabs", "waves",
                 "persist_tests", "persist_launches", "queue_depth", "waves_per_cu", "handoffs", "item_lanes",
-                "block_waves", "fallback_levels")
+                "block_waves", "fallback_levels", "records_loaded", "distinct_records")
         return dict(zip(keys, (int(v) for v in out)))
+
+    def gather_roof(self, window=0, nloads=1 << 23, seed=1):
+        """kdop_gather_roof: GB/s of random record loads over the first `window`
+        node slots of this context's pool (0: all of it)."""
+        g = ctypes.c_double()
+        self._check(self._lib.kdop_gather_roof(self._h, int(window), int(nloads), int(seed), ctypes.byref(g)))
+        return g.value
 
     def last_timing(self):
         out = np.zeros(12, np.float64)

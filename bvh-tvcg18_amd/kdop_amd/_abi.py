@@ -25,7 +25,7 @@ KDOP_TRAVERSAL_PERSISTENT = 2
 OPTIONS = {"traversal": 1, "queue_depth": 2, "waves_per_cu": 3, "ramp_target": 4, "stack_keep": 5,
            "seed_budget": 6, "kernel_timing": 7, "bfs_max": 8, "async_refit": 9,
            "narrow_path": 10, "graphs": 11, "single_items": 12,
-           "block_waves": 13, "item_lanes": 14, "max_rounds": 15}
+           "block_waves": 13, "item_lanes": 14, "max_rounds": 15, "count_records": 16, "quad_items": 17, "round_iters": 18}
 KDOP_NARROW = 1
 KDOP_ORDER_SEQUENTIAL = 0
 KDOP_ORDER_OPENCL = 1
@@ -102,6 +102,7 @@ SIGNATURES = [
     ("kdop_reduce_contacts", ctypes.c_int, [_p, _p]),
     ("kdop_reduce_prox_contacts", ctypes.c_int, [_p, _u64, _p, _p, ctypes.c_int, _p]),
     ("kdop_reduce_prox_contacts_noalloc", ctypes.c_int, [_p, _u64, _p, _p, _p]),
+    ("kdop_gather_roof", ctypes.c_int, [_p, _u64, _u32, _u32, _p]),
     ("kdop_raycast", ctypes.c_int, [_p, _u32, _u32, _p, _p, _p, _p, _p, ctypes.c_int]),
     ("kdop_single_traversal", ctypes.c_int, [_p, _u32, _u32, _p, ctypes.c_int, _p, _u64, _p, ctypes.c_int]),
     ("kdop_mesh_reorder_gpu", ctypes.c_int, [_p, _u32, _p, _p, _p, _u32, _p, _p, _p, _p, _p]),

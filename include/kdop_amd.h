@@ -33,10 +33,11 @@ extern "C" {
 typedef struct kdop_ctx kdop_ctx;
 
 /* Traversal strategies (kdop_config.traversal). */
-#define KDOP_TRAVERSAL_AUTO 0        /* persistent for K >= 14 and <= 2M tets per call,    */
+#define KDOP_TRAVERSAL_AUTO 0        /* persistent for K >= 12 or >= 64 body pairs,        */
                                      /* level-synchronous otherwise (kdop_autotune refines) */
 #define KDOP_TRAVERSAL_LEVELS 1      /* level-synchronous frontier kernels                 */
-#define KDOP_TRAVERSAL_PERSISTENT 2  /* one persistent launch, wave stacks + work stealing */
+#define KDOP_TRAVERSAL_PERSISTENT 2  /* persistent rounds: LDS stacks per workgroup, guided */
+                                     /* frontier chunks, budgeted rounds re-spreading work  */
 
 typedef struct kdop_config {
   int32_t K;             /* slab count of the kDOP: 6, 8, 14, 18 or 26 (reference template K)  */
@@ -302,8 +303,18 @@ int kdop_narrow_adjoint(kdop_ctx* ctx, uint32_t nbodies, const uint32_t* bodies,
  * holds 12 doubles. */
 int kdop_last_timing(kdop_ctx* ctx, double* out);
 
+/* Gather roof of this context's node pool (measurement; no counterpart in
+ * the reference): nloads random records (uniform over the first `window`
+ * node slots, 0 = the whole pool; seeded by `seed`) loaded the way the
+ * traversal's 4-lane expansion loads them, 4 per quad at a time, with every
+ * resident wave of the GPU.  *gbytes_per_s receives the median rate over six
+ * runs, counting each record's used words (16 bytes x ceil((K + 2) / 4)).  A
+ * small window keeps the records in the L2 caches, the whole pool of a large
+ * body in the 256 MB last-level cache.  Allocates and frees its index list. */
+int kdop_gather_roof(kdop_ctx* ctx, uint64_t window, uint32_t nloads, uint32_t seed, double* gbytes_per_s);
+
 /* Scheduling statistics of the last traversal (persistent strategy; zeros for
- * the level-synchronous one), 16 entries: out[0] breadth-first levels before
+ * the level-synchronous one), 20 entries: out[0] breadth-first levels before
  * the persistent rounds, out[1] frontier handed to them, out[2] rounds run,
  * out[3] loop iterations (sum over waves, or over workgroups where waves
  * share a stack), out[4] items expanded, out[5] most iterations of one wave
@@ -311,9 +322,12 @@ int kdop_last_timing(kdop_ctx* ctx, double* out);
  * waves, out[8] node-pair tests made by the persistent rounds, out[9]
  * persistent launches, out[12] items the tail hand-off passed on, out[15]
  * breadth-first levels launched after the persistent rounds ran out (the
- * spill finished level-synchronously); out[10] LDS stack entries per wave,
- * out[11] waves per CU, out[13] lanes per work item and out[14] waves per
- * workgroup in effect (set for either strategy). */
+ * spill finished level-synchronously), out[16] node records the rounds'
+ * expansions loaded (one per record and item), out[17] distinct records the
+ * rounds read (KDOP_OPT_COUNT_RECORDS calls only, else 0); out[10] LDS stack
+ * entries per wave, out[11] waves per CU, out[13] lanes per work item and
+ * out[14] waves per workgroup in effect (set for either strategy); out[18],
+ * out[19] 0. */
 int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 
 /* Options of a live context (kdop_set_option / kdop_get_option).  The
@@ -359,6 +373,19 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   one child combination; the stack then holds                 */
                                   /*   KDOP_OPT_QUEUE_DEPTH (256 or 512) items per wave.  Results   */
                                   /*   unchanged                                                   */
+#define KDOP_OPT_ROUND_ITERS 18   /* quad kernel: once the frontier is gone, a workgroup that   */
+                                  /*   has run this many iterations in a round hands its whole  */
+                                  /*   stack to the next round (spread there over every         */
+                                  /*   workgroup) and stops; the default is 64, 0 means no      */
+                                  /*   budget.  The last round allowed (KDOP_OPT_MAX_ROUNDS)     */
+                                  /*   runs unbudgeted.  Results unchanged                      */
+#define KDOP_OPT_QUAD_ITEMS 17    /* quad kernel: items each quad expands per iteration, 1 (0:  */
+                                  /*   the default) or 2 (twice as wide an iteration, 4-wave     */
+                                  /*   workgroups).  Results unchanged                           */
+#define KDOP_OPT_COUNT_RECORDS 16 /* 1: measurement calls -- the persistent rounds (quad kernel,  */
+                                  /*   default shape) also mark each node record they read, and   */
+                                  /*   kdop_traversal_stats reports how many distinct ones; slower */
+                                  /*   (not for timing), results unchanged.  0: off (default)     */
 #define KDOP_OPT_MAX_ROUNDS 15    /* persistent strategy: most persistent rounds per call, 1..7   */
                                   /*   (0: 7); a spill left after them becomes a breadth-first     */
                                   /*   level and the level-synchronous launches finish the        */
@@ -371,8 +398,8 @@ int kdop_set_option(kdop_ctx* ctx, int option, int64_t value);
 int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
 
 /* Autotuner: picks the traversal options (strategy, lanes per work item,
- * waves per persistent workgroup, waves per CU, LDS queue depth, stack keep,
- * breadth-first target, seed budget) for this context by timing
+ * round iteration budget, waves per persistent workgroup, waves per CU, LDS
+ * queue depth, stack keep, breadth-first target, seed budget) for this context by timing
  * kdop_traverse(npairs, body_pairs, flags) itself: coordinate descent over
  * the options in that order, each candidate measured as the median host wall
  * time of 3 calls (a candidate that fails with KDOP_ERR_CAPACITY is skipped),
@@ -382,7 +409,7 @@ int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
  * KDOP_OPT_* - 1 (the value in effect of every option; 0 for numbers that
  * name no option); *best_ms (may be NULL) the median time of the best.
  * No counterpart in the reference (its OpenCL sizes are compile-time). */
-#define KDOP_AUTOTUNE_CHOSEN 16
+#define KDOP_AUTOTUNE_CHOSEN 20
 int kdop_autotune(kdop_ctx* ctx, uint32_t npairs, const uint32_t* body_pairs, uint32_t flags, double budget_ms, int64_t* chosen,
                   double* best_ms);
 
