@@ -481,11 +481,16 @@ struct SgLeaves {
   }
 };
 template <int K>
-struct RefitRoot {
-  static void run(kdop_ctx* c, uint32_t n, const uint32_t* bb) {
-    hipLaunchKernelGGL(k_refit_root<K>, dim3(n), dim3(64), 0, c->stream, c->dbodies.p, bb, c->dtops.p, c->nodes.p);
+struct RefitTop {
+  static void run(kdop_ctx* c, uint32_t n, const uint4* ent) {
+    hipLaunchKernelGGL(k_refit_top<K>, dim3(n), dim3(kTopThreads), 0, c->stream, ent, c->dtops.p, c->nodes.p);
   }
 };
+// Super levels of at most this many chunks go to the one-block-per-body
+// k_refit_top, a wave per chunk: its 16 waves take them in at most two
+// passes, against a launch of its own otherwise.  The block's levels together
+// hold at most kTopRoots chunks (its LDS table) and kTopMetas internal nodes.
+constexpr uint32_t kTopChunks = 32;
 
 // Transform (or copy, identity) + leaf/union passes for a batch of bodies.
 // pos/quat: device pointers (3n, 4n floats) or null with identity.
@@ -527,6 +532,36 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   plan.insert(plan.end(), pbody.begin(), pbody.end());
   const size_t o_pchunk = plan.size();
   plan.insert(plan.end(), pchunk.begin(), pchunk.end());
+  // super level h of body b: {slot of its first chunk, chunks, elements of
+  // the last chunk, first chunk to refit}; none when the leaves pass did it
+  auto super_level = [&](const BodyHost& b, int h, uint4& lv) -> bool {
+    // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64)
+    const uint32_t cf = (h == (int)b.H - 2 && b.L * b.L <= 64) ? (b.nt / b.L) / b.L : 0u;
+    const uint32_t nc = b.C[h], nl = host_chunk_elems(b, h, nc - 1);
+    if (cf >= nc) return false;
+    const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u);
+    if (!nfc && nl < 2) return false;
+    lv = make_uint4(b.node_base + b.level_base[h], nc, nl, cf);
+    return true;
+  };
+  // per body the top super levels (those of at most kTopChunks chunks and
+  // every level above them, within the LDS limits) go to k_refit_top; the
+  // wider ones below get a launch per step
+  std::vector<int> htop(n, -1);
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    if (b.wide()) continue;
+    uint32_t rows = 0, nlev = 0;
+    for (int h = 0; h + 2 <= (int)b.H; ++h) {
+      uint4 lv;
+      const bool has = super_level(b, h, lv);
+      const uint32_t r = has ? lv.y - lv.w : 0u;
+      if (r > kTopChunks || rows + r > kTopRoots || (rows + r) * (b.L - 1) > kTopMetas || nlev + (has ? 1u : 0u) > kTopEntries - 2) break;
+      rows += r;
+      nlev += has ? 1u : 0u;
+      htop[i] = h;
+    }
+  }
   struct Step { size_t o_bb, o_gpre, o_cfirst, o_ncnk, o_cbase, o_cn; uint32_t m, blocks; };
   std::vector<Step> steps;
   for (uint32_t s = 1; s < maxH; ++s) {
@@ -536,13 +571,12 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
       const BodyHost& b = c->bodies[bodies[i]];
       if (b.H < s + 1 || b.wide()) continue;
       const int h = (int)b.H - 1 - (int)s;
-      // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64)
-      const uint32_t cf = (s == 1 && b.L * b.L <= 64) ? (b.nt / b.L) / b.L : 0u;
-      const uint32_t nc = b.C[h], nl = host_chunk_elems(b, h, nc - 1);
-      if (cf >= nc) continue;
-      const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u), G = 64 / b.L;
-      const uint32_t blocks = (nfc + G - 1) / G + (nl < b.L && nl >= 2 ? 1u : 0u);
-      if (!blocks) continue;
+      if (h <= htop[i]) continue;
+      uint4 lv;
+      if (!super_level(b, h, lv)) continue;
+      const uint32_t cf = lv.w, nc = lv.y, nl = lv.z;
+      const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u);
+      const uint32_t blocks = nfc + (nl < b.L && nl >= 2 ? 1u : 0u);  // a wave per chunk
       bb.push_back(bodies[i]);
       tot += blocks;
       gpre.push_back(tot);
@@ -562,6 +596,25 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     st.o_cbase = plan.size(); plan.insert(plan.end(), cbase.begin(), cbase.end());
     st.o_cn = plan.size(); plan.insert(plan.end(), cn.begin(), cn.end());
     steps.push_back(st);
+  }
+  // k_refit_top's entries: kTopEntries 16-byte records per body, its levels
+  // deepest first
+  while (plan.size() % 4) plan.push_back(0);
+  const size_t o_top = plan.size();
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    std::vector<uint4> levs;
+    uint32_t rows = 0;
+    for (int h = htop[i]; h >= 0; --h) {
+      uint4 lv;
+      if (super_level(b, h, lv)) { levs.push_back(lv); rows += lv.y - lv.w; }
+    }
+    const uint32_t nmeta = rows * (b.L - 1);
+    plan.insert(plan.end(), {(uint32_t)levs.size(), b.L, b.ntop, b.top_base, b.node_base, rows, nmeta, 0u});
+    for (uint32_t e = 0; e < kTopEntries - 2; ++e) {
+      const uint4 lv = e < levs.size() ? levs[e] : make_uint4(0, 0, 0, 0);
+      plan.insert(plan.end(), {lv.x, lv.y, lv.z, lv.w});
+    }
   }
   const bool same = c->plan_key.size() == n && std::equal(bodies, bodies + n, c->plan_key.begin()) && c->plan_host == plan;
   if (!same) {
@@ -586,7 +639,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     const BodyHost& b = c->bodies[bodies[i]];
     if (b.wide()) dispatch_K<RefitWide>(c->K, c, bodies[i], b);
   }
-  dispatch_K<RefitRoot>(c->K, c, n, P + o_bb);
+  dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + o_top));
   HIPCHK(c, hipGetLastError());
   if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;

@@ -534,79 +534,309 @@ __global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__
   }
 }
 
+// Super chunks (kdop_refit_tree.h:82-115): their children outside the chunk
+// are resolved roots of the level below, refitted before.
+//
+// A full chunk (L = 2^LOGL elements) is the implicit heap over its elements
+// and takes one wave, transposed: lane = g L + e holds the volume floats f =
+// g + NG i (i < F) of element e, NG = 64 / L groups.  A butterfly level is
+// then F = ceil(K / NG) unions per lane instead of K (4 instead of 26 at K =
+// 26, L = 8), which is what a lone wave's chain of levels costs.  NG is even,
+// so the floats of a lane are all lower (g even) or all upper (g odd) bounds.
+// Lanes e and e ^ S hold the two children of a level (the left one without
+// bit S); both end with their union, in the reference's order
+// (geometry_make_union.h:9-25).
+template <int K, int LOGL>
+struct ChunkT {
+  static constexpr uint32_t L = 1u << LOGL, NG = 64u / L, F = (K + NG - 1) / NG;
+};
+template <int K, int LOGL, int T>
+__device__ inline void chunk_t_level(float* v, uint32_t e, uint32_t g, float* __restrict__ nodes, uint32_t base) {
+  using C = ChunkT<K, LOGL>;
+  constexpr uint32_t SX = 1u << (T - 1);
+  const bool left = (e & SX) == 0u, upper = (g & 1u) != 0u;
+#pragma unroll
+  for (uint32_t i = 0; i < C::F; ++i) {
+    const float y = lane_xor<SX>(v[i]);
+    const float a = left ? v[i] : y, b = left ? y : v[i];
+    v[i] = upper ? smax(a, b) : smin(a, b);
+  }
+  if ((e & ((2u << (T - 1)) - 1u)) == 0u) {  // node (2^(LOGL-T) - 1) + e / 2^T of the heap
+    float* r = nodes + (size_t)(base + (1u << (LOGL - T)) - 1u + (e >> T)) * rec_stride(K);
+#pragma unroll
+    for (uint32_t i = 0; i < C::F; ++i) {
+      const uint32_t f = g + C::NG * i;
+      if (f < (uint32_t)K) r[f] = v[i];
+    }
+  }
+}
+// One full chunk whose BFS node 0 is at slot base.  src(e) gives element e's
+// record {slot, row}: row ~0u reads the record from global memory, another
+// row reads row `row` of the LDS table tab.  The internal nodes are stored,
+// and root (an LDS row, or null) receives the chunk root.
+template <int K, int LOGL, class Src>
+__device__ inline void chunk_t(float* __restrict__ nodes, uint32_t base, uint32_t lane, Src&& src, const float (*tab)[K], float* root) {
+  using C = ChunkT<K, LOGL>;
+  const uint32_t e = lane & (C::L - 1u), g = lane >> LOGL;
+  const uint2 cr = src(e);
+  float v[C::F];
+  if (cr.y != ~0u) {
+#pragma unroll
+    for (uint32_t i = 0; i < C::F; ++i) {
+      const uint32_t f = g + C::NG * i;
+      v[i] = f < (uint32_t)K ? tab[cr.y][f] : 0.0f;
+    }
+  } else {
+    const float* r = nodes + (size_t)cr.x * rec_stride(K);
+#pragma unroll
+    for (uint32_t i = 0; i < C::F; ++i) {
+      const uint32_t f = g + C::NG * i;
+      v[i] = f < (uint32_t)K ? r[f] : 0.0f;
+    }
+  }
+  if constexpr (LOGL >= 1) chunk_t_level<K, LOGL, 1>(v, e, g, nodes, base);
+  if constexpr (LOGL >= 2) chunk_t_level<K, LOGL, 2>(v, e, g, nodes, base);
+  if constexpr (LOGL >= 3) chunk_t_level<K, LOGL, 3>(v, e, g, nodes, base);
+  if constexpr (LOGL >= 4) chunk_t_level<K, LOGL, 4>(v, e, g, nodes, base);
+  if constexpr (LOGL >= 5) chunk_t_level<K, LOGL, 5>(v, e, g, nodes, base);
+  if (root && e == 0u) {
+#pragma unroll
+    for (uint32_t i = 0; i < C::F; ++i) {
+      const uint32_t f = g + C::NG * i;
+      if (f < (uint32_t)K) root[f] = v[i];
+    }
+  }
+}
+// chunk_t for a chunk size known at run time (L = 2 .. 32).
+template <int K, class Src>
+__device__ inline void chunk_t_any(uint32_t logL, float* __restrict__ nodes, uint32_t base, uint32_t lane, Src&& src,
+                                   const float (*tab)[K], float* root) {
+  switch (logL) {
+    case 1: chunk_t<K, 1>(nodes, base, lane, src, tab, root); break;
+    case 2: chunk_t<K, 2>(nodes, base, lane, src, tab, root); break;
+    case 3: chunk_t<K, 3>(nodes, base, lane, src, tab, root); break;
+    case 4: chunk_t<K, 4>(nodes, base, lane, src, tab, root); break;
+    default: chunk_t<K, 5>(nodes, base, lane, src, tab, root); break;
+  }
+}
+
+// LDS of one wave's partial chunk: the children from outside the chunk staged
+// by (internal node, side), then the internal nodes' volumes.
+template <int K>
+struct PartialLds {
+  float ext[62][K];
+  float vol[31][K];
+  uint2 meta[31];
+};
+// The last chunk of a level when it is not full (n elements, 2 <= n < L; its
+// BFS node 0 at slot base), by one wave: the reference's reverse BFS sweep
+// over its n - 1 internal nodes (children always follow their parent).  The
+// children from outside the chunk are first staged in LDS, a lane per
+// (internal node, side), so their loads are in flight together (fetch(c, v):
+// the K volume floats of record c); then lane f < K sweeps, two LDS reads a
+// node.  meta(i): meta words of internal node i.  root: as for chunk_t.
+template <int K, class Meta, class Fetch>
+__device__ inline void partial_chunk(float* __restrict__ nodes, uint32_t base, uint32_t n, uint32_t lane, PartialLds<K>& sh,
+                                     Meta&& meta, Fetch&& fetch, float* root) {
+  constexpr int S = rec_stride(K);
+  const uint32_t nn = n - 1;  // at most 30
+  if (lane < nn) sh.meta[lane] = meta(lane);
+  wave_lds_sync();
+  if (lane < 2u * nn) {
+    const uint32_t i = lane >> 1;
+    const uint32_t c = ((lane & 1u) ? sh.meta[i].y : sh.meta[i].x) & IDX_MASK;
+    if (c < base || c >= base + nn) {
+      float v[K];
+      fetch(c, v);
+#pragma unroll
+      for (int f = 0; f < K; ++f) sh.ext[lane][f] = v[f];
+    }
+  }
+  wave_lds_sync();
+  if (lane < (uint32_t)K) {
+    for (int i = (int)nn - 1; i >= 0; --i) {
+      const uint2 m = sh.meta[i];
+      const uint32_t c0 = (m.x & IDX_MASK) - base, c1 = (m.y & IDX_MASK) - base;
+      const float a = c0 < nn ? sh.vol[c0][lane] : sh.ext[2 * i][lane];
+      const float b = c1 < nn ? sh.vol[c1][lane] : sh.ext[2 * i + 1][lane];
+      sh.vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
+    }
+    if (root) root[lane] = sh.vol[0][lane];
+  }
+  wave_lds_sync();
+  for (uint32_t q = lane; q < nn * K; q += 64) {
+    const uint32_t i = q / K, f = q - i * K;
+    nodes[(size_t)(base + i) * S + f] = sh.vol[i][f];
+  }
+  wave_lds_sync();
+}
+
 // Super chunks of level h (per body h = H-1-step), chunks [cfirst, C[h]) of
-// each batch item; their children outside the chunk are resolved roots of
-// level h+1, refitted before (kdop_refit_tree.h:82-115).  A full chunk (L
-// elements) is the implicit heap over its elements, so a 64-lane block takes
-// 64 / L of them at once: lane j loads element j (the child of bottom node
-// L/2 - 1 + j/2) and a butterfly builds the internal nodes, as in
-// k_refit_leaves.  The level's last chunk, when it is not full, gets a block
-// of its own and the reference's reverse BFS sweep.  gpre: prefix of blocks
-// per item.
+// each batch item, one wave per chunk: full chunks transposed (chunk_t), the
+// level's last chunk, when it is not full, by the sweep.  gpre: prefix of
+// blocks per item.
 template <int K>
 __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                     const uint32_t* __restrict__ gpre, const uint32_t* __restrict__ cfirst,
                                                     const uint32_t* __restrict__ ncnk, const uint32_t* __restrict__ chunk_base,
                                                     const uint32_t* __restrict__ chunk_n, uint32_t nbatch, float* __restrict__ nodes) {
-  constexpr int S = rec_stride(K);
-  __shared__ float vol[32][K];
-  __shared__ uint32_t meta0[32], meta1[32];
-  const uint32_t g = blockIdx.x;
-  if (g >= gpre[nbatch]) return;
-  const uint32_t item = find_item(gpre, nbatch, g);
-  const uint32_t k = g - gpre[item];
+  __shared__ PartialLds<K> sh;
+  const uint32_t blk = blockIdx.x;
+  if (blk >= gpre[nbatch]) return;
+  const uint32_t item = find_item(gpre, nbatch, blk);
+  const uint32_t k = blk - gpre[item];
   const BodyDev& B = bodies[batch_body[item]];
   const uint32_t L = B.L, logL = (uint32_t)(__ffs((int)L) - 1);
   const uint32_t cf = cfirst[item], nc = ncnk[item], nlast = chunk_n[item];
   const uint32_t nfc = (nc - 1u - cf) + (nlast == L ? 1u : 0u);  // full chunks from cf (cf < nc)
-  const uint32_t G = 64u >> logL;                                 // full chunks per block
   const uint32_t lane = threadIdx.x;
-  if (k < (nfc + G - 1u) / G) {
-    const uint32_t j = lane & (L - 1u), slot = k * G + (lane >> logL);
-    const bool ok = slot < nfc;
-    const uint32_t base = chunk_base[item] + (cf + slot) * L + 1u;  // slot of BFS node 0
-    NodeRec<K> r;
-#pragma unroll
-    for (int f = 0; f < K; ++f) r.v[f] = 0.0f;
-    if (ok) {
-      const uint2 m = load_meta<K>(nodes, base + (L >> 1) - 1u + (j >> 1));
-      load_rec<K>(nodes, ((j & 1u) ? m.y : m.x) & IDX_MASK, r);
-    }
-    for (uint32_t t = 1; t <= logL; ++t) {
-      switch (t) {
-        case 1: union_level<K, 1>(r.v, j); break;
-        case 2: union_level<K, 2>(r.v, j); break;
-        case 3: union_level<K, 4>(r.v, j); break;
-        case 4: union_level<K, 8>(r.v, j); break;
-        default: union_level<K, 16>(r.v, j); break;
-      }
-      if (ok && (j & ((2u << (t - 1u)) - 1u)) == 0u) store_vol<K>(nodes, base + (1u << (logL - t)) - 1u + (j >> t), r.v);
-    }
+  if (k < nfc) {
+    const uint32_t base = chunk_base[item] + (cf + k) * L + 1u;  // slot of BFS node 0
+    chunk_t_any<K>(logL, nodes, base, lane, [&](uint32_t e) {
+      const uint2 m = load_meta<K>(nodes, base + (L >> 1) - 1u + (e >> 1));
+      return make_uint2(((e & 1u) ? m.y : m.x) & IDX_MASK, ~0u);
+    }, sh.vol, nullptr);
     return;
   }
-  // the level's last chunk, not full
-  const uint32_t n = nlast;
-  if (n < 2) return;
-  const uint32_t nn = n - 1;
-  const uint32_t base = chunk_base[item] + (nc - 1u) * L + 1;  // slot of BFS node 0
-  for (uint32_t i = lane; i < nn; i += 64) {
-    const uint2 m = load_meta<K>(nodes, base + i);
-    meta0[i] = m.x; meta1[i] = m.y;
+  if (nlast < 2) return;
+  const uint32_t base = chunk_base[item] + (nc - 1u) * L + 1u;
+  partial_chunk<K>(nodes, base, nlast, lane, sh, [&](uint32_t i) { return load_meta<K>(nodes, base + i); },
+                   [&](uint32_t c, float* v) {
+                     NodeRec<K> r;
+                     load_rec<K>(nodes, c, r);
+#pragma unroll
+                     for (int f = 0; f < K; ++f) v[f] = r.v[f];
+                   }, nullptr);
+}
+
+// The upper levels of each batch item's tree and its root in one launch, one
+// 1024-thread block per item: the super levels of few chunks, deepest first,
+// then the root volume as the union of the top-level chunk roots
+// (kdop_refit_tree.h:154-161).  The launch is a chain of dependent memory
+// round trips, so it is built to have three: the item's entries (kTopEntries
+// uint4 at ent[item * kTopEntries]: {levels, L, top roots, first top root},
+// {root slot, LDS rows, meta words, 0}, then per level {slot of the first
+// chunk, chunks, elements of the last chunk, first chunk to refit}); then,
+// all at once, the meta words of every internal node of those levels' chunks
+// and the top-root slots, into LDS; then the first level's children.  Every
+// chunk root the block computes goes to an LDS table (a row per chunk of its
+// levels, chunks [cf, nc)) besides its record, and a later level or the root
+// reads a child from there when the block made it, so the levels are linked
+// by LDS alone: the barrier between them orders LDS only and never waits for
+// the records' global stores.  A level's chunks take one wave each.
+constexpr uint32_t kTopThreads = 1024;
+constexpr uint32_t kTopEntries = 8;    // 2 header entries + at most 6 levels per item
+constexpr uint32_t kTopRoots = 256;    // LDS rows: chunks of one item's levels
+constexpr uint32_t kTopMetas = 2048;   // LDS meta words: rows x (L - 1) internal nodes
+constexpr uint32_t kTopTops = 64;      // top-root slots staged in LDS (more are read from global memory)
+
+__device__ inline void block_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS row of record `slot` if it is the root of a chunk of the block's levels
+// [0, e) (lv: their entries), else ~0u (the record comes from global memory:
+// a leaf, a branch chunk, or a level refitted by an earlier launch).
+__device__ inline uint32_t top_row(const uint4* lv, uint32_t e, uint32_t logL, uint32_t slot) {
+  uint32_t row = 0;
+  for (uint32_t q = 0; q < e; ++q) {
+    const uint4 l = lv[q];
+    const uint32_t d = slot - (l.x + 1u), c = d >> logL;
+    if (slot > l.x && (d & ((1u << logL) - 1u)) == 0u && c >= l.w && c < l.y) return row + c - l.w;
+    row += l.y - l.w;
   }
-  __syncthreads();
-  for (int i = (int)nn - 1; i >= 0; --i) {
-    const uint32_t c0 = meta0[i] & IDX_MASK, c1 = meta1[i] & IDX_MASK;
-    if (lane < K) {
-      const float a = (c0 >= base && c0 < base + nn) ? vol[c0 - base][lane] : nodes[(size_t)c0 * S + lane];
-      const float b = (c1 >= base && c1 < base + nn) ? vol[c1 - base][lane] : nodes[(size_t)c1 * S + lane];
-      vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
+  return ~0u;
+}
+
+template <int K>
+struct TopLds {
+  float roots[kTopRoots][K];
+  uint2 meta[kTopMetas];
+  uint4 ent[kTopEntries];
+  uint32_t tops[kTopTops];
+  PartialLds<K> part;
+};
+
+template <int K>
+__global__ __launch_bounds__(kTopThreads) void k_refit_top(const uint4* __restrict__ ent, const uint32_t* __restrict__ tops,
+                                                          float* __restrict__ nodes) {
+  constexpr int S = rec_stride(K);
+  __shared__ TopLds<K> sh;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6, nw = blockDim.x >> 6;
+  // round trip 1: the item's entries
+  if (tid < kTopEntries) sh.ent[tid] = ent[(size_t)blockIdx.x * kTopEntries + tid];
+  block_lds_barrier();
+  const uint4 h0 = sh.ent[0], h1 = sh.ent[1];
+  const uint32_t nlev = h0.x, L = h0.y, ntop = h0.z, top_base = h0.w, root_slot = h1.x, nmeta = h1.z;
+  const uint32_t logL = (uint32_t)(__ffs((int)L) - 1), NI = L - 1u;  // NI: internal nodes per chunk
+  const uint4* lv = sh.ent + 2;
+  // round trip 2: meta words (row r of the levels' rows: its chunk's L - 1
+  // internal nodes) and the top-root slots, every load issued before any wait
+  {
+    uint2 m[kTopMetas / kTopThreads];
+#pragma unroll
+    for (uint32_t u = 0; u < kTopMetas / kTopThreads; ++u) {
+      const uint32_t g = tid + u * kTopThreads;
+      m[u] = make_uint2(0u, 0u);
+      if (g < nmeta) {
+        uint32_t row = g / NI, e = 0, r0 = 0;
+        while (row - r0 >= lv[e].y - lv[e].w) { r0 += lv[e].y - lv[e].w; ++e; }
+        const uint32_t slot = lv[e].x + (lv[e].w + row - r0) * L + 1u + (g - row * NI);
+        m[u] = load_meta<K>(nodes, slot);
+      }
     }
-    __syncthreads();
+    const uint32_t tv = tid < min(ntop, kTopTops) ? tops[top_base + tid] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kTopMetas / kTopThreads; ++u) {
+      const uint32_t g = tid + u * kTopThreads;
+      if (g < nmeta) sh.meta[g] = m[u];
+    }
+    if (tid < min(ntop, kTopTops)) sh.tops[tid] = tv;
   }
-  for (uint32_t w = lane; w < nn * K; w += 64) {
-    const uint32_t i = w / K, f = w - i * K;
-    nodes[(size_t)(base + i) * S + f] = vol[i][f];
+  block_lds_barrier();
+  uint32_t row0 = 0;  // LDS row of chunk cf of level e
+  for (uint32_t e = 0; e < nlev; ++e) {
+    const uint4 l = lv[e];  // cbase, nc, nlast, cf
+    const uint32_t nfc = (l.y - 1u - l.w) + (l.z == L ? 1u : 0u);
+    const uint32_t ntask = nfc + (l.z < L && l.z >= 2u ? 1u : 0u);
+    for (uint32_t k = w; k < ntask; k += nw) {
+      if (k < nfc) {
+        const uint32_t base = l.x + (l.w + k) * L + 1u, mrow = (row0 + k) * NI;
+        chunk_t_any<K>(logL, nodes, base, lane, [&](uint32_t el) {
+          const uint2 m = sh.meta[mrow + (L >> 1) - 1u + (el >> 1)];
+          const uint32_t c = ((el & 1u) ? m.y : m.x) & IDX_MASK;
+          return make_uint2(c, top_row(lv, e, logL, c));
+        }, sh.roots, sh.roots[row0 + k]);
+      } else {
+        const uint32_t base = l.x + (l.y - 1u) * L + 1u, prow = row0 + (l.y - 1u - l.w);
+        partial_chunk<K>(nodes, base, l.z, lane, sh.part, [&](uint32_t i) { return sh.meta[prow * NI + i]; },
+                         [&](uint32_t c, float* v) {
+                           const uint32_t row = top_row(lv, e, logL, c);
+                           if (row != ~0u) {
+#pragma unroll
+                             for (int f = 0; f < K; ++f) v[f] = sh.roots[row][f];
+                           } else {
+                             NodeRec<K> r;
+                             load_rec<K>(nodes, c, r);
+#pragma unroll
+                             for (int f = 0; f < K; ++f) v[f] = r.v[f];
+                           }
+                         }, sh.roots[prow]);
+      }
+    }
+    block_lds_barrier();
+    row0 += l.y - l.w;
   }
+  if (w != 0 || lane >= (uint32_t)K) return;
+  float acc = 0.0f;
+  for (uint32_t t = 0; t < ntop; ++t) {
+    const uint32_t c = t < kTopTops ? sh.tops[t] : tops[top_base + t], row = top_row(lv, nlev, logL, c);
+    const float x = row != ~0u ? sh.roots[row][lane] : nodes[(size_t)c * S + lane];
+    acc = t == 0 ? x : ((lane & 1) ? smax(acc, x) : smin(acc, x));
+  }
+  nodes[(size_t)root_slot * S + lane] = acc;
 }
 
 // Bodies with chunks wider than 32 leaves (a per-body mem_bytes above about
@@ -650,23 +880,6 @@ __global__ void k_write_meta(const uint32_t* __restrict__ meta, uint32_t nnodes,
   uint32_t* m = reinterpret_cast<uint32_t*>(nodes + (size_t)(node_base + i) * rec_stride(K) + K);
   m[0] = meta[2 * (size_t)i];
   m[1] = meta[2 * (size_t)i + 1];
-}
-
-// Root volume of each body = union of its top-level chunk roots
-// (kdop_refit_tree.h:154-161).  One 64-lane block per batch item.
-template <int K>
-__global__ __launch_bounds__(64) void k_refit_root(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                   const uint32_t* __restrict__ tops, float* __restrict__ nodes) {
-  constexpr int S = rec_stride(K);
-  const BodyDev& B = bodies[batch_body[blockIdx.x]];
-  const int lane = threadIdx.x;
-  if (lane >= K) return;
-  float acc = nodes[(size_t)tops[B.top_base] * S + lane];
-  for (uint32_t t = 1; t < B.ntop; ++t) {
-    const float x = nodes[(size_t)tops[B.top_base + t] * S + lane];
-    acc = (lane & 1) ? smax(acc, x) : smin(acc, x);
-  }
-  nodes[(size_t)B.node_base * S + lane] = acc;
 }
 
 // ============================================================== traversal

@@ -40,7 +40,7 @@ typedef struct kdop_ctx kdop_ctx;
                                      /* frontier chunks, budgeted rounds re-spreading work  */
 
 typedef struct kdop_config {
-  int32_t K;             /* slab count of the kDOP: 6, 8, 14, 18 or 26 (reference template K)  */
+  int32_t K;             /* slab count of the kDOP: 6, 8, 12, 14, 18, 20 or 26 (template K)    */
   int32_t device;        /* HIP device ordinal                                                  */
   uint64_t chunk_bytes;  /* reference make_tree mem_bytes (narrow_params.h:54 default 8000)     */
   int32_t traversal;     /* KDOP_TRAVERSAL_*                                                    */
