@@ -376,9 +376,10 @@ int upload_bodies(kdop_ctx* c) {
 // ------------------------------------------------------------------- refit
 template <int K>
 struct RefitLeaves {
-  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, uint32_t n) {
-    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, n, c->nodes.p,
-                       c->dt);
+  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, const uint32_t* vpre, uint32_t n,
+                  const float* pos, const float* quat, bool identity) {
+    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, vpre, n, pos,
+                       quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
   }
 };
 // A body with chunks wider than 32 leaves: leaf DOPs at their slots, then
@@ -528,6 +529,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   }
   const uint64_t total_leaf_threads = acc;
   if (total_leaf_threads >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many tetrahedra in one batch");
+  if (total_v >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many vertices in one batch");
   const size_t o_pbody = plan.size();
   plan.insert(plan.end(), pbody.begin(), pbody.end());
   const size_t o_pchunk = plan.size();
@@ -625,12 +627,10 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     c->plan_host = plan;
   }
   const uint32_t* P = c->plan.p;
-  if (total_v) {
-    const uint32_t blocks = (uint32_t)((total_v + 255) / 256);
-    hipLaunchKernelGGL(k_transform, dim3(blocks), dim3(256), 0, c->stream, c->dbodies.p, P + o_bb, P + o_vpre, n, pos_dev, quat_dev,
-                       identity ? 1 : 0, c->dposes.p);
-  }
-  if (total_leaf_threads) dispatch_K<RefitLeaves>(c->K, c, (uint32_t)total_leaf_threads, P + o_bb, P + o_tpre, n);
+  // leaves and the vertex transform in one launch, its grid over both ranges
+  const uint64_t leaves_grid = std::max<uint64_t>(total_leaf_threads, total_v);
+  if (leaves_grid)
+    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + o_bb, P + o_tpre, P + o_vpre, n, pos_dev, quat_dev, identity);
   if (!pbody.empty()) dispatch_K<RefitBranch>(c->K, c, (uint32_t)pbody.size(), P + o_pbody, P + o_pchunk);
   for (const Step& st : steps)
     dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,

@@ -197,45 +197,16 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int K,
 }
 
 // ============================================================== refit
-// Vertex transform r = rotate(q, r0) + p (narrow_update_kdop_bvh.h:154-165),
-// batched over bodies: item i of the batch covers vertices [vpre[i], vpre[i+1]).
-// The pose of every body is recorded in body_pose (8 floats per body: p,
-// q as x y z w, 0) for the adjoint.
-__global__ void k_transform(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                            const uint32_t* __restrict__ vpre, uint32_t nbatch, const float* __restrict__ pos,
-                            const float* __restrict__ quat, int identity, float* __restrict__ body_pose) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= vpre[nbatch]) return;
-  uint32_t lo = 0, hi = nbatch;  // find i with vpre[i] <= g < vpre[i+1]
-  while (hi - lo > 1) {
-    const uint32_t m = (lo + hi) >> 1;
-    if (vpre[m] <= g) lo = m; else hi = m;
-  }
-  const BodyDev& B = bodies[batch_body[lo]];
-  const uint32_t v = g - vpre[lo];
-  if (v == 0u) {
-    float* bp = body_pose + 8 * (size_t)batch_body[lo];
-    if (identity) {
-      bp[0] = bp[1] = bp[2] = 0.f;
-      bp[3] = bp[4] = bp[5] = 0.f;
-      bp[6] = 1.f;
-    } else {
-      bp[0] = pos[3 * lo]; bp[1] = pos[3 * lo + 1]; bp[2] = pos[3 * lo + 2];
-      bp[3] = quat[4 * lo]; bp[4] = quat[4 * lo + 1]; bp[5] = quat[4 * lo + 2]; bp[6] = quat[4 * lo + 3];
-    }
-    bp[7] = 0.f;
-  }
+// World position of vertex v of B under the pose (q, p): rotate(q, r0) + p
+// (narrow_update_kdop_bvh.h:154-165), or r0 itself for the identity, which
+// make_tree refits to.  Every kernel that needs a vertex of the pose being
+// refitted computes it through this function, so all of them get the bits
+// that the refit stores.
+__device__ inline f3 posed_vertex(const BodyDev& B, uint32_t v, const float* q, const float* p, bool identity) {
   const f3 r0 = f3{B.X0[v], B.Y0[v], B.Z0[v]};
-  if (identity) {  // make_tree refits to the material coordinates themselves
-    B.X[v] = r0.x; B.Y[v] = r0.y; B.Z[v] = r0.z;
-    return;
-  }
-  const float* q = quat + 4 * lo;
-  const float* p = pos + 3 * lo;
+  if (identity) return r0;
   const f3 r = rotate3(q[0], q[1], q[2], q[3], r0);
-  B.X[v] = r.x + p[0];
-  B.Y[v] = r.y + p[1];
-  B.Z[v] = r.z + p[2];
+  return f3{r.x + p[0], r.y + p[1], r.z + p[2]};
 }
 
 // Locate the batch item owning global chunk g (cpre: prefix of chunk counts).
@@ -259,13 +230,14 @@ __device__ inline void store_vol(float* __restrict__ nodes, uint32_t slot, const
 }
 // Leaf DOP of one tetrahedron (geometry_dop.h:48-74): per slab direction the
 // min / max projection of its four world vertices.
-template <int K>
-__device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& dt, float* v) {
+// pos(i): world position of vertex i.
+template <int K, class Pos>
+__device__ inline void tet_dop_at(const BodyDev& B, uint32_t tet, const DirTable& dt, Pos&& pos, float* v) {
   const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)tet);
   const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
   f3 P[4];
 #pragma unroll
-  for (int p = 0; p < 4; ++p) P[p] = f3{B.X[vi[p]], B.Y[vi[p]], B.Z[vi[p]]};
+  for (int p = 0; p < 4; ++p) P[p] = pos(vi[p]);
 #pragma unroll
   for (int k = 0; k < K / 2; ++k) {
     const f3 d = f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]};
@@ -279,6 +251,11 @@ __device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& d
     v[2 * k] = lo;
     v[2 * k + 1] = hi;
   }
+}
+// The same from the body's stored world vertices.
+template <int K>
+__device__ inline void tet_dop(const BodyDev& B, uint32_t tet, const DirTable& dt, float* v) {
+  tet_dop_at<K>(B, tet, dt, [&](uint32_t i) { return f3{B.X[i], B.Y[i], B.Z[i]}; }, v);
 }
 
 // Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16, 32}: DPP row moves within
@@ -375,18 +352,51 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
 // nodes, then the super chunks' nodes; each a contiguous run of records) and
 // written out with consecutive 16-byte words on consecutive lanes, so that a
 // store instruction covers whole lines instead of one word of 64 lines.
+//
+// The same launch is the vertex transform of the batch: thread g also writes
+// world vertex g of the batch (vpre: prefix of vertex counts) to the body's
+// X, Y, Z, which later kernels read, and the first thread of each body
+// records its pose in body_pose (8 floats per body: p, q as x y z w, 0, for
+// the adjoint).  A leaf does not wait for those stores: it computes its four
+// vertices from the rest positions with the same arithmetic (posed_vertex).
+// pos / quat: 3 and 4 floats per batch item (unused with identity).
 template <int K>
 __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                      const uint32_t* __restrict__ tpre, uint32_t nbatch, float* __restrict__ nodes,
-                                                      DirTable dt) {
+                                                      const uint32_t* __restrict__ tpre, const uint32_t* __restrict__ vpre, uint32_t nbatch,
+                                                      const float* __restrict__ pos, const float* __restrict__ quat, int identity,
+                                                      float* __restrict__ body_pose, float* __restrict__ nodes, DirTable dt) {
   constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
   __shared__ __align__(16) float stage[4][64 * RS];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < vpre[nbatch]) {
+    const uint32_t vi = find_item(vpre, nbatch, g);
+    const uint32_t bi = batch_body[vi];
+    const BodyDev& B = bodies[bi];
+    const uint32_t v = g - vpre[vi];
+    const float* q = quat + 4 * (size_t)vi;
+    const float* p = pos + 3 * (size_t)vi;
+    if (v == 0u) {
+      float* bp = body_pose + 8 * (size_t)bi;
+      if (identity) {
+        bp[0] = bp[1] = bp[2] = 0.f;
+        bp[3] = bp[4] = bp[5] = 0.f;
+        bp[6] = 1.f;
+      } else {
+        bp[0] = p[0]; bp[1] = p[1]; bp[2] = p[2];
+        bp[3] = q[0]; bp[4] = q[1]; bp[5] = q[2]; bp[6] = q[3];
+      }
+      bp[7] = 0.f;
+    }
+    const f3 r = posed_vertex(B, v, q, p, identity != 0);
+    B.X[v] = r.x; B.Y[v] = r.y; B.Z[v] = r.z;
+  }
   if (g >= tpre[nbatch]) return;  // whole waves (tpre[nbatch] is a multiple of 64)
   float* __restrict__ st = stage[threadIdx.x >> 6];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t item = find_item(tpre, nbatch, g);
   const BodyDev& B = bodies[batch_body[item]];
+  const float* iq = quat + 4 * (size_t)item;
+  const float* ip = pos + 3 * (size_t)item;
   const uint32_t logL = (uint32_t)(__ffs((int)B.L) - 1), L = B.L;
   const uint32_t tet = g - tpre[item];
   const uint32_t tet0 = tet - lane;                   // first tet of the wave
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     wave_lds_sync();
   };
   if (act) {
-    tet_dop<K>(B, tet, dt, v);
+    tet_dop_at<K>(B, tet, dt, [&](uint32_t i) { return posed_vertex(B, i, iq, ip, identity != 0); }, v);
     stage_row(lane);
   }
   flush(64u, [&](uint32_t r) -> float* {  // row r: leaf r & (L-1) of chunk r >> logL
