@@ -5,7 +5,9 @@
 // reports for that pair, in the oracle's order, bit for bit; and the device's
 // PROX contact reduction of them must keep what the oracle's std::sort +
 // std::unique keeps of the same contacts collected in callback order, as
-// PROX collects them.  Exit status 0 when all match.  Build: python bvh-tvcg18_amd/build.py --oracle.
+// PROX collects them.  With --series FILE: PROX's loop over a motion series
+// (run_series).  Exit status 0 when all match.  Build: python
+// bvh-tvcg18_amd/build.py --oracle.
 #include <stdint.h>
 
 #include <cmath>
@@ -69,7 +71,136 @@ struct Collect : kdop_amd::ContactsCallback {
   }
 };
 
-int main() {
+// PROX's collision-detection loop (prox_collision_detection.h:124-345) over a
+// motion series written by tests/test_gpu_glue.py: per step the pool refit of
+// every body (narrow::update_kdop_bvh, narrow_update_kdop_bvh.h:126-176), one
+// tandem traversal of the pair list with a ContactsCallback per pair
+// (dispatch_tetramesh_tetramesh, narrow_tetramesh_tetramesh.h:61-91), then the
+// contact reduction of everything the callbacks collected.  Every step: each
+// pair's contacts equal the oracle's (count, order, bits) and the reduction
+// keeps what the oracle's std::sort + std::unique keeps.
+// File: int32 K, nmeshes, npairs, steps; per mesh uint32 nv, nt, float X, Y, Z
+// [nv], uint32 tets [4 nt], uint8 surface [nt]; uint32 pairs [2 npairs]; float
+// P [steps][nmeshes][3], Q [steps][nmeshes][4].
+static int run_series(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { std::printf("cannot open %s\n", path); return 4; }
+  auto rd = [&](void* dst, size_t bytes) {
+    if (std::fread(dst, 1, bytes, f) != bytes) throw kdop_amd::Error(-1, "short series file");
+  };
+  int32_t hdr[4];
+  int bad = 0;
+  try {
+    rd(hdr, sizeof hdr);
+    const int K = hdr[0], nm = hdr[1], np = hdr[2], steps = hdr[3];
+    std::vector<MeshData> meshes(nm);
+    for (MeshData& m : meshes) {
+      uint32_t nv = 0, nt = 0;
+      rd(&nv, 4); rd(&nt, 4);
+      m.X.resize(nv); m.Y.resize(nv); m.Z.resize(nv); m.tets.resize(4 * (size_t)nt); m.surf.resize(nt);
+      rd(m.X.data(), 4 * (size_t)nv); rd(m.Y.data(), 4 * (size_t)nv); rd(m.Z.data(), 4 * (size_t)nv);
+      rd(m.tets.data(), 16 * (size_t)nt); rd(m.surf.data(), nt);
+    }
+    std::vector<uint32_t> pairs(2 * (size_t)np);
+    rd(pairs.data(), 8 * (size_t)np);
+    std::vector<float> P(3 * (size_t)steps * nm), Q(4 * (size_t)steps * nm);
+    rd(P.data(), 4 * P.size()); rd(Q.data(), 4 * Q.size());
+    std::fclose(f);
+    f = nullptr;
+    kdop_amd::Context ctx(K);
+    std::vector<kdop_amd::Tree> trees;
+    std::vector<void*> orc;
+    for (const MeshData& m : meshes) {
+      trees.push_back(ctx.make_tree(m.X, m.Y, m.Z, m.tets, m.surf));
+      orc.push_back(orc_tree_build(K, 8000, (uint32_t)m.surf.size(), m.tets.data(), m.X.data(), m.Y.data(), m.Z.data()));
+    }
+    size_t total = 0, kept_total = 0, live_steps = 0;
+    std::vector<MeshData> world(nm);
+    std::vector<OrcContact> want(1 << 14);
+    for (int step = 0; step < steps; ++step) {
+      std::vector<kdop_amd::UpdateWorkItem> pool(nm);
+      for (int i = 0; i < nm; ++i) {
+        pool[i].tree = &trees[i];
+        std::memcpy(pool[i].p, &P[3 * ((size_t)step * nm + i)], 12);
+        std::memcpy(pool[i].q, &Q[4 * ((size_t)step * nm + i)], 16);
+        const MeshData& m = meshes[i];
+        world[i].X.resize(m.X.size()); world[i].Y.resize(m.X.size()); world[i].Z.resize(m.X.size());
+        orc_transform_f32((uint32_t)m.X.size(), m.X.data(), m.Y.data(), m.Z.data(), pool[i].p, pool[i].q, world[i].X.data(),
+                          world[i].Y.data(), world[i].Z.data());
+        orc_tree_refit(orc[i], m.tets.data(), world[i].X.data(), world[i].Y.data(), world[i].Z.data());
+      }
+      ctx.update_kdop_bvh(pool);
+      std::vector<Collect> got(np);
+      std::vector<kdop_amd::TestPair> tps;
+      for (int k = 0; k < np; ++k) tps.push_back(kdop_amd::TestPair{&trees[pairs[2 * k]], &trees[pairs[2 * k + 1]], &got[k]});
+      ctx.tandem_traversal(tps);
+      const std::vector<kdop_prox_contact> red = ctx.reduce_contacts();
+      // the oracle, pair by pair, and PROX's contacts vector in callback order
+      size_t step_contacts = 0, pairs_ok = 0;
+      std::vector<uint32_t> bi, bj;
+      std::vector<float> Pc, Nc, Dc;
+      for (int k = 0; k < np; ++k) {
+        const uint32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+        uint64_t cnt[9];
+        for (int pass = 0; pass < 2; ++pass) {  // again with room for all when the buffer was short
+          orc_traverse(orc[a], 0, meshes[a].tets.data(), world[a].X.data(), world[a].Y.data(), world[a].Z.data(), meshes[a].surf.data(),
+                       orc[b], 0, meshes[b].tets.data(), world[b].X.data(), world[b].Y.data(), world[b].Z.data(), meshes[b].surf.data(), 1, 1,
+                       nullptr, 0, nullptr, 0, want.data(), want.size(), cnt, nullptr);
+          if ((size_t)cnt[6] <= want.size()) break;
+          want.resize((size_t)cnt[6]);
+        }
+        const size_t nw = (size_t)cnt[6], ng = got[k].v.size() / 7;
+        bool same = nw == ng;
+        for (size_t i = 0; same && i < nw; ++i) {
+          const float w7[7] = {want[i].p[0], want[i].p[1], want[i].p[2], want[i].n[0], want[i].n[1], want[i].n[2], want[i].d};
+          same = std::memcmp(w7, &got[k].v[7 * i], sizeof(w7)) == 0;
+        }
+        if (!same) std::printf("series step %d pair %d: %zu contacts (oracle %zu) MISMATCH\n", step, k, ng, nw);
+        pairs_ok += same ? 1 : 0;
+        step_contacts += ng;
+        for (size_t i = 0; i < ng; ++i) {
+          const float* v = &got[k].v[7 * i];
+          bi.push_back(trees[a].body);
+          bj.push_back(trees[b].body);
+          Pc.insert(Pc.end(), v, v + 3);
+          Nc.insert(Nc.end(), v + 3, v + 6);
+          Dc.push_back(v[6]);
+        }
+      }
+      std::vector<uint32_t> keep(bi.size() + 1);
+      const size_t nk = (size_t)orc_prox_reduce(bi.size(), bi.data(), bj.data(), Pc.data(), Nc.data(), Dc.data(), keep.data());
+      bool rsame = red.size() == nk;
+      for (size_t i = 0; rsame && i < nk; ++i) {
+        const uint32_t j = keep[i];
+        const kdop_prox_contact& r = red[i];
+        const float w7[7] = {Pc[3 * j], Pc[3 * j + 1], Pc[3 * j + 2], Nc[3 * j], Nc[3 * j + 1], Nc[3 * j + 2], Dc[j]};
+        const float g7[7] = {r.p[0], r.p[1], r.p[2], r.n[0], r.n[1], r.n[2], r.depth};
+        rsame = r.body_i == bi[j] && r.body_j == bj[j] && std::memcmp(w7, g7, sizeof(w7)) == 0;
+      }
+      const bool ok = pairs_ok == (size_t)np && rsame;
+      std::printf("series step %d: %zu contacts, %zu kept, %zu of %d pairs and the reduction %s\n", step, step_contacts, red.size(),
+                  pairs_ok, np, ok ? "match" : "MISMATCH");
+      bad += ok ? 0 : 1;
+      total += step_contacts;
+      kept_total += red.size();
+      live_steps += step_contacts ? 1 : 0;
+    }
+    for (void* h : orc) orc_tree_free(h);
+    std::printf("series: %d steps, %zu with contacts, %zu contacts, %zu kept\n", steps, live_steps, total, kept_total);
+    if (live_steps < (size_t)steps) {
+      std::printf("steps without contacts: the series does not test every step\n");
+      return 2;
+    }
+    return bad ? 1 : 0;
+  } catch (const kdop_amd::Error& e) {
+    if (f) std::fclose(f);
+    std::printf("error %d: %s\n", e.code, e.what());
+    return 3;
+  }
+}
+
+int main(int argc, char** argv) {
+  if (argc == 3 && std::strcmp(argv[1], "--series") == 0) return run_series(argv[2]);
   const int K = 26;
   std::vector<MeshData> meshes = {box(8, 1.0f, 0.1, 1), box(10, 1.2f, 0.1, 2), box(6, 0.8f, 0.1, 3)};
   const uint32_t pairs[3][2] = {{0, 1}, {1, 2}, {0, 2}};

@@ -16,7 +16,7 @@ import torch
 import kdop_amd as K
 from kdop_amd import workloads as W
 from kdop_amd.autograd import penetration_measure
-from helpers import narrow_records
+from helpers import narrow_records, oracle_run
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -155,3 +155,35 @@ def test_adjoint_c5_scale():
     np.testing.assert_allclose(phi.cpu().numpy(), d, rtol=1e-4, atol=1e-4)
     worst, scale = _fd_check(ctx, ids, pos, quat, pairs, [0, 1, 77], range(7))
     assert worst <= FD_RTOL * scale, (worst, scale)
+
+
+def test_adjoint_c5_every_body_vs_fp64_oracle():
+    """The C5 draw (64 overlapping pairs of ~53k-tet jittered boxes, K=14):
+    Phi of every pair and the pose gradients of all 128 bodies against the
+    oracle's FP64 frozen evaluation and reverse mode, on the records the
+    oracle's float32 narrow phase chooses for each pair's surface pairs; the
+    same bar as the single pair."""
+    wl = W.batched_pairs(npairs=64, K=14, seed=29, tmin=5e4, tmax=5e4, separated=0.0)
+    ctx, ids, pos, quat, pairs = _context(wl)
+    ctx.refit(ids, pos, quat)
+    ctx.tandem_traversal(pairs, narrow=True)
+    phi, gp, gq = (t.cpu().numpy() for t in ctx.narrow_adjoint(ids))
+    orc, _, worlds = oracle_run(wl, narrow=False, threads=8)
+    to64 = lambda v: np.asarray(v, np.float64)  # noqa: E731
+    checked = 0
+    for i, (a, b) in enumerate(wl.pairs):
+        ma, mb = wl.meshes[a], wl.meshes[b]
+        PA, PB = np.stack(worlds[a], 1), np.stack(worlds[b], 1)
+        recs = []
+        for ta_i, tb_i in orc[i][2]:
+            r = O.contacts_pair_codes(PA[ma.tets[ta_i]], PB[mb.tets[tb_i]], ma.surface[ta_i], mb.surface[tb_i])
+            if r["pts"].shape[0]:
+                recs.append((int(ta_i), int(tb_i), r))
+        want = O.phi_frozen_f64(_records(recs), ma, (to64(pos[a]), to64(quat[a])), mb, (to64(pos[b]), to64(quat[b])),
+                                grad=True)
+        g_o = np.concatenate(want[1:])
+        g_d = np.concatenate([gp[a], gq[a], gp[b], gq[b]])
+        assert abs(float(phi[i]) - want[0]) <= ORACLE_RTOL * max(1.0, abs(want[0])), (i, float(phi[i]), want[0])
+        assert np.abs(g_d - g_o).max() <= ORACLE_RTOL * max(np.abs(g_o).max(), 1e-6), (i, g_d, g_o)
+        checked += len(recs) > 0
+    assert checked >= 60

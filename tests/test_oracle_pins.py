@@ -278,6 +278,145 @@ def test_all_pairs_overlap_closed_form_count(k):
         assert _bvtt_count(_median_tree(32), _median_tree(32)) == (4 ** 6 - 1) // 3
 
 
+# ------------------------------------------------- sequential visit order
+def _leaf_tree(n, first):
+    """_median_tree over tets first .. first + n - 1, leaves = tet ids."""
+    if n == 1:
+        return first
+    left = (n - 1) // 2 + 1
+    return (_leaf_tree(left, first), _leaf_tree(n - left, first + left))
+
+
+def _branch_trees(nt, L):
+    out, first = [], 0
+    for s in _chunk_sizes(nt, L):
+        out.append(_leaf_tree(s, first))
+        first += s
+    return out
+
+
+def _aabb(tree, box):
+    """K = 6 volume of a subtree: leaves from box (nt, 6) = per-tet (lo, hi) per axis."""
+    if isinstance(tree, int):
+        return box[tree]
+    a, b = _aabb(tree[0], box), _aabb(tree[1], box)
+    return np.array([min(a[0], b[0]), max(a[1], b[1]), min(a[2], b[2]), max(a[3], b[3]), min(a[4], b[4]), max(a[5], b[5])])
+
+
+def _visit_order(sa, sb, box_a, box_b):
+    """Leaf pairs in the order the sequential traversal reaches them
+    (kdop_tandem_traversal.h:40-151, 160-196): branch pairs row-major; per
+    node pair the strict overlap test, then A's children outer and B's inner,
+    or the one side that is not a leaf."""
+    out = []
+
+    def overlap(x, y):
+        u, v = _aabb(x, box_a), _aabb(y, box_b)
+        return not any(u[2 * k + 1] < v[2 * k] or v[2 * k + 1] < u[2 * k] for k in range(3))
+
+    def rec(x, y):
+        if not overlap(x, y):
+            return
+        xl, yl = isinstance(x, int), isinstance(y, int)
+        if xl and yl:
+            out.append((x, y))
+        elif not xl and not yl:
+            for a in x:
+                for b in y:
+                    rec(a, b)
+        elif not xl:
+            for a in x:
+                rec(a, y)
+        else:
+            for b in y:
+                rec(x, b)
+
+    ra = [_aabb(t, box_a) for t in sa]
+    rb = [_aabb(t, box_b) for t in sb]
+    root_a = np.array([f(r[k] for r in ra) for k, f in enumerate([min, max] * 3)])
+    root_b = np.array([f(r[k] for r in rb) for k, f in enumerate([min, max] * 3)])
+    if any(root_a[2 * k + 1] < root_b[2 * k] or root_b[2 * k + 1] < root_a[2 * k] for k in range(3)):
+        return out
+    for x in sa:
+        for y in sb:
+            rec(x, y)
+    return out
+
+
+def _tet_boxes(mesh, X, Y, Z):
+    P = np.stack([X, Y, Z], 1)[mesh.tets.astype(np.int64)]  # (nt, 4, 3), exact float32 coordinates
+    lo, hi = P.min(1), P.max(1)
+    return np.stack([lo[:, 0], hi[:, 0], lo[:, 1], hi[:, 1], lo[:, 2], hi[:, 2]], 1)
+
+
+@pytest.mark.parametrize("case", ["all_overlap", "offset"])
+def test_sequential_leaf_pair_order_is_the_reference_recursion(case):
+    """The oracle's P1 list in sequential order (threads=1) equals an
+    enumeration written from the reference's recursion alone: chunk trees
+    from the median split, K = 6 volumes as exact per-tet AABBs and their
+    unions (test_leaf_dop_k6_is_exact_aabb), strict overlap, branch pairs
+    row-major.  A swapped loop nest, a reversed child order or a missing
+    pruning test reorders or changes the list."""
+    if case == "all_overlap":
+        ma, mb = _star_mesh(2), _star_mesh(3)
+        Xb, Yb, Zb = mb.X, mb.Y, mb.Z
+    else:
+        ma, mb = K.box_mesh(6, jitter=0.2, seed=5), K.box_mesh(5, jitter=0.2, seed=6)
+        Xb, Yb, Zb = (mb.X + F32(0.55)).astype(F32), (mb.Y + F32(0.3)).astype(F32), mb.Z
+    ta = O.Tree(6, 8000, ma.tets, ma.X, ma.Y, ma.Z)
+    tb = O.Tree(6, 8000, mb.tets, mb.X, mb.Y, mb.Z)
+    tb.refit(Xb, Yb, Zb)
+    c, p1, _, _, _ = O.traverse(ta, _mesh_tuple(ma), tb, _mesh_tuple(mb, Xb, Yb, Zb), narrow=False, threads=1)
+    La = _shape_closed_form(6, 8000, ma.nt)["L"]
+    Lb = _shape_closed_form(6, 8000, mb.nt)["L"]
+    want = _visit_order(_branch_trees(ma.nt, La), _branch_trees(mb.nt, Lb),
+                        _tet_boxes(ma, ma.X, ma.Y, ma.Z), _tet_boxes(mb, Xb, Yb, Zb))
+    assert len(want) > 500 and (case == "all_overlap" or len(want) < ma.nt * mb.nt // 4)
+    assert c["leaf_pairs"] == len(want)
+    assert [tuple(int(v) for v in r) for r in p1] == want
+
+
+# --------------------------------------------------------------- surface map
+def _surface_bits(S):
+    return [(t, v) for t in range(S.shape[0]) for v in range(4) if (int(S[t]) >> v) & 1]
+
+
+@pytest.mark.parametrize("n,jitter,reorder", [(1, 0.0, False), (2, 0.0, False), (3, 0.2, False), (4, 0.15, True)])
+def test_surface_map_closed_form_on_box(n, jitter, reorder):
+    """An n^3 box of 5-tet cubes has 6 n^2 boundary squares of 2 triangles:
+    12 n^2 surface bits.  Bit v of tet t (mesh_array_compute_surface_map.h:
+    41-76: the face opposite vertex v has no neighbour) must name a face
+    whose three vertices lie on one boundary plane while vertex v does not;
+    jitter moves interior vertices only, so the planes stay exact."""
+    if reorder:
+        m = K.box_mesh(n, jitter=jitter, seed=4)
+        X, Y, Z, T, S = m.X, m.Y, m.Z, m.tets, m.surface
+    else:
+        X, Y, Z, T = K.box_mesh_raw(n, 1.0, jitter, 4)
+        S = K.compute_surface_map(X.shape[0], T)
+    bits = _surface_bits(S)
+    assert len(bits) == 12 * n * n
+    P = np.stack([X, Y, Z], 1)
+    for t, v in bits:
+        face = [int(T[t][q]) for q in range(4) if q != v]
+        on = [ax for ax in range(3) for s in (-0.5, 0.5) if all(P[i][ax] == F32(s) for i in face)]
+        assert len(on) == 1, (t, v, P[face])
+        ax = on[0]
+        assert P[int(T[t][v])][ax] != P[face[0]][ax]
+
+
+def test_surface_map_two_tets_known_answer():
+    """Two tets sharing the face {1, 2, 3}: in (0, 1, 2, 3) it lies opposite
+    local vertex 0, in (4, 2, 1, 3) opposite local vertex 0 too, in
+    (1, 2, 3, 4) opposite local vertex 3; every other face is on the surface."""
+    T = np.array([[0, 1, 2, 3], [1, 2, 3, 4]], np.uint32)
+    assert list(K.compute_surface_map(5, T)) == [0b1110, 0b0111]
+    T = np.array([[0, 1, 2, 3], [4, 2, 1, 3]], np.uint32)
+    assert list(K.compute_surface_map(5, T)) == [0b1110, 0b1110]
+    T = np.array([[0, 1, 2, 3], [5, 6, 7, 4]], np.uint32)  # no shared face
+    assert list(K.compute_surface_map(8, T)) == [0b1111, 0b1111]
+
+
 # ------------------------------------------------------------------- rotate
 def test_rotate_matches_rotation_matrix():
     rng = np.random.default_rng(4)
