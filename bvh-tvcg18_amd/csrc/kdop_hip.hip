@@ -1487,7 +1487,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           if (prod > share || prod <= best) continue;
           best = prod; ha = x; hb = y;
         }
-      seeds[i] = SeedDesc{A.roots_off[ha], A.C[ha], B.roots_off[hb], B.C[hb], (uint32_t)total_seeds};
+      seeds[i] = SeedDesc{A.roots_off[ha], A.C[ha], B.roots_off[hb], B.C[hb], (uint32_t)total_seeds, A.node_base, B.node_base,
+                          A.C[A.H - 1], B.C[B.H - 1]};
       total_seeds += best;
       c->seed_plan_est[i] = best;
       est += best;

@@ -1223,8 +1223,14 @@ __device__ inline void emit_items(const Expansion& x, Item* __restrict__ out, ui
 
 // Seed descriptor of one body pair: every chunk root of level hA of body A
 // (tops[a_off .. a_off+na)) against every chunk root of level hB of body B.
+// One body pair's seeds: chunk roots [a_off, a_off + na) x [b_off, b_off +
+// nb) of the context's root table, seed indices from base; with the pair's
+// root records and branch-chunk counts, so that a seed thread needs no body
+// descriptor before its first record loads.
 struct SeedDesc {
-  uint32_t a_off, na, b_off, nb, base;  // base: first seed index of the pair
+  uint32_t a_off, na, b_off, nb, base;
+  uint32_t root_a, root_b;    // root records of body A and body B
+  uint32_t nbr_a, nbr_b;      // branch chunks C_A, C_B (the reference's C_A x C_B branch-pair loop)
 };
 
 // Seed: one thread per (chunk root of A, chunk root of B) pair of the chosen
@@ -1255,23 +1261,21 @@ __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __rest
       pair = lo;
       const SeedDesc d = sd[pair];
       const uint32_t local = g - d.base;
-      const uint2 bp = T.pairs[pair];
-      const BodyDev& BA = T.bodies[bp.x];
-      const BodyDev& BB = T.bodies[bp.y];
-      NodeRec<K> ra, rb;
-      load_rec<K>(T.nodes, BA.node_base, ra);
-      load_rec<K>(T.nodes, BB.node_base, rb);
+      // the root records and the seed's chunk roots load together; the seed
+      // counts only when the roots overlap
+      const uint32_t i = local / d.nb, j = local - i * d.nb;
+      const uint32_t xa = T.tops[d.a_off + i], yb = T.tops[d.b_off + j];
+      NodeRec<K> ra, rb, X, Y;
+      load_rec<K>(T.nodes, d.root_a, ra);
+      load_rec<K>(T.nodes, d.root_b, rb);
+      load_rec<K>(T.nodes, xa, X);
+      load_rec<K>(T.nodes, yb, Y);
       const bool ov = dop_overlap<K>(ra.v, rb.v);
       if (local == 0) {
         ++roots;
-        if (ov) bpairs += (unsigned long long)BA.nbranch * BB.nbranch;
+        if (ov) bpairs += (unsigned long long)d.nbr_a * d.nbr_b;
       }
       if (ov) {
-        const uint32_t i = local / d.nb, j = local - i * d.nb;
-        const uint32_t xa = T.tops[d.a_off + i], yb = T.tops[d.b_off + j];
-        NodeRec<K> X, Y;
-        load_rec<K>(T.nodes, xa, X);
-        load_rec<K>(T.nodes, yb, Y);
         x.ta[0] = X.m0 & IDX_MASK;
         x.tb[0] = Y.m0 & IDX_MASK;
         ++tests;

@@ -88,6 +88,14 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   // blocks of that launch start, and every wave of a block must take the same
   // branch.  The sizes are used only when the flag is clear.
   const uint32_t segcap_in = cap / nseg_in;
+  // a contiguous level's first item of this thread is loaded before the
+  // level's size is known (its slot exists whatever the size; the value is
+  // used only when the slot holds an item), so that load overlaps the size read
+  const Item* __restrict__ in = (level & 1) ? buf1 : buf0;
+  const uint32_t i_first = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool early = nseg_in == 1u && i_first < cap;
+  Item it_first{0, 0, 0, 0, 0};
+  if (early) it_first = in[i_first];
   if (threadIdx.x < 64u) {
     const uint32_t lane = threadIdx.x;
     const uint32_t f = nseg_in == 1u ? (lane == 0u ? C->level_fill[level] : 0u) : C->lseg[level % 3u][lane].fill;
@@ -121,7 +129,6 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     if (nseg_in > 1u && threadIdx.x == 0) C->level_fill[level] = raw_total;  // the level's size (all that was reserved)
     if (nseg_out > 1u && threadIdx.x < (uint32_t)kSegs) C->lseg[(level + 2u) % 3u][threadIdx.x].fill = 0u;
   }
-  const Item* __restrict__ in = (level & 1) ? buf1 : buf0;
   Item* __restrict__ out = (level & 1) ? buf0 : buf1;
   const uint32_t so = blockIdx.x % nseg_out, segcap_out = cap / nseg_out;
   uint32_t* out_fill = nseg_out == 1u ? &C->level_fill[level + 1] : &C->lseg[(level + 1u) % 3u][so].fill;
@@ -136,16 +143,20 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     x.tests = x.descent = x.broot = 0;
     Item it{0, 0, 0, 0, 0};
     if (i < n) {
-      uint32_t idx = i;
-      if (nseg_in > 1u) {  // segment s with pre[s] <= i < pre[s + 1]
-        uint32_t lo = 0, hi = kSegs;
-        while (hi - lo > 1u) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (pre[mid] <= i) lo = mid; else hi = mid;
+      if (early && i == i_first) {
+        it = it_first;
+      } else {
+        uint32_t idx = i;
+        if (nseg_in > 1u) {  // segment s with pre[s] <= i < pre[s + 1]
+          uint32_t lo = 0, hi = kSegs;
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= i) lo = mid; else hi = mid;
+          }
+          idx = lo * segcap_in + (i - pre[lo]);
         }
-        idx = lo * segcap_in + (i - pre[lo]);
+        it = in[idx];
       }
-      it = in[idx];
       expand_item<K>(T.nodes, it, x);
     }
     tests += x.tests; descent += x.descent; broot += x.broot; p1s += __popc(x.leaf);
