@@ -539,8 +539,8 @@ def contact_fp64_error(head):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--k", type=int, default=26)
     ap.add_argument("--n", type=int, default=40, help="cubes per box edge (5 n^3 tets per body)")
     ap.add_argument("--batched-pairs", type=int, default=512)
