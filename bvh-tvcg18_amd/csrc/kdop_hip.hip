@@ -63,6 +63,22 @@ struct DevBuf {
   size_t cap = 0;  // elements
 };
 
+// The launches of one refit batch, derived from its body list alone (the
+// plan's offsets in the device copy): kept on the context and reused while
+// the list repeats, so a repeated refit makes its first launch at once.
+struct RefitStep {
+  size_t o_bb, o_gpre, o_cfirst, o_ncnk, o_cbase, o_cn;
+  uint32_t m, blocks;
+};
+struct RefitLaunch {
+  bool valid = false;
+  size_t o_bb = 0, o_vpre = 0, o_tpre = 0, o_pbody = 0, o_pchunk = 0, o_top = 0;
+  uint64_t total_v = 0, total_leaf_threads = 0;
+  uint32_t npbody = 0;
+  std::vector<RefitStep> steps;
+  std::vector<uint32_t> wide;  // bodies refitted from their own lists (chunks over 32 leaves)
+};
+
 }  // namespace
 
 struct kdop_ctx {
@@ -141,7 +157,7 @@ struct kdop_ctx {
   bool async_refit = false;    // refit returns once queued (the next call orders after it)
   bool state_clean = false;    // CallState zeroed (by the last call's final fold)
   std::vector<uint32_t> plan_key;   // bodies of the refit plan in c->plan
-  std::vector<uint32_t> plan_host;  // that plan (host copy)
+  RefitLaunch rl;                   // its launches
   std::vector<uint2> seed_key;      // pair list whose pairs / seeds are on the device
   uint32_t seed_key_budget = 0;
   size_t seed_key_bodies = 0;
@@ -493,13 +509,13 @@ struct RefitTop {
 // hold at most kTopRoots chunks (its LDS table) and kTopMetas internal nodes.
 constexpr uint32_t kTopChunks = 32;
 
-// Transform (or copy, identity) + leaf/union passes for a batch of bodies.
-// pos/quat: device pointers (3n, 4n floats) or null with identity.
-int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
-  // plan layout (uint32): [bb n][vpre n+1][tpre n+1][partial chunks: body m,
-  // chunk m] then per super step s: [bb m][gpre m+1][cfirst m][ncnk m]
-  // [cbase m][cn m].  It only depends on the body list, so the device copy is
-  // reused while that repeats.
+// Build the refit plan of a body list into c->plan (device) and c->rl.
+// Plan layout (uint32): [bb n][vpre n+1][tpre n+1][partial chunks: body m,
+// chunk m] then per super step s: [bb m][gpre m+1][cfirst m][ncnk m]
+// [cbase m][cn m], then k_refit_top's entries.
+int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
+  c->rl.valid = false;
+  c->plan_key.clear();
   std::vector<uint32_t> plan;
   plan.reserve(8 * (size_t)n + 16);
   uint32_t maxH = 1;
@@ -564,7 +580,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
       htop[i] = h;
     }
   }
-  struct Step { size_t o_bb, o_gpre, o_cfirst, o_ncnk, o_cbase, o_cn; uint32_t m, blocks; };
+  using Step = RefitStep;
   std::vector<Step> steps;
   for (uint32_t s = 1; s < maxH; ++s) {
     std::vector<uint32_t> bb, gpre{0}, cfirst, ncnk, cbase, cn;
@@ -618,28 +634,44 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
       plan.insert(plan.end(), {lv.x, lv.y, lv.z, lv.w});
     }
   }
-  const bool same = c->plan_key.size() == n && std::equal(bodies, bodies + n, c->plan_key.begin()) && c->plan_host == plan;
+  int rc = ensure(c, c->plan, plan.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  RefitLaunch& R = c->rl;
+  R.o_bb = o_bb; R.o_vpre = o_vpre; R.o_tpre = o_tpre; R.o_pbody = o_pbody; R.o_pchunk = o_pchunk; R.o_top = o_top;
+  R.total_v = total_v;
+  R.total_leaf_threads = total_leaf_threads;
+  R.npbody = (uint32_t)pbody.size();
+  R.steps = steps;
+  R.wide.clear();
+  for (uint32_t i = 0; i < n; ++i)
+    if (c->bodies[bodies[i]].wide()) R.wide.push_back(bodies[i]);
+  R.valid = true;
+  c->plan_key.assign(bodies, bodies + n);
+  return KDOP_OK;
+}
+
+// Transform (or copy, identity) + leaf/union passes for a batch of bodies.
+// pos/quat: device pointers (3n, 4n floats) or null with identity.  The
+// plan depends on the body list alone, so a repeated list reuses it.
+int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_dev, const float* quat_dev, bool identity) {
+  const bool same = c->rl.valid && c->plan_key.size() == n && std::equal(bodies, bodies + n, c->plan_key.begin());
   if (!same) {
-    int rc = ensure(c, c->plan, plan.size());
+    const int rc = build_refit_plan(c, n, bodies);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-    c->plan_key.assign(bodies, bodies + n);
-    c->plan_host = plan;
   }
+  const RefitLaunch& R = c->rl;
   const uint32_t* P = c->plan.p;
   // leaves and the vertex transform in one launch, its grid over both ranges
-  const uint64_t leaves_grid = std::max<uint64_t>(total_leaf_threads, total_v);
+  const uint64_t leaves_grid = std::max<uint64_t>(R.total_leaf_threads, R.total_v);
   if (leaves_grid)
-    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + o_bb, P + o_tpre, P + o_vpre, n, pos_dev, quat_dev, identity);
-  if (!pbody.empty()) dispatch_K<RefitBranch>(c->K, c, (uint32_t)pbody.size(), P + o_pbody, P + o_pchunk);
-  for (const Step& st : steps)
+    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + R.o_bb, P + R.o_tpre, P + R.o_vpre, n, pos_dev, quat_dev, identity);
+  if (R.npbody) dispatch_K<RefitBranch>(c->K, c, R.npbody, P + R.o_pbody, P + R.o_pchunk);
+  for (const RefitStep& st : R.steps)
     dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
                            st.m);
-  for (uint32_t i = 0; i < n; ++i) {
-    const BodyHost& b = c->bodies[bodies[i]];
-    if (b.wide()) dispatch_K<RefitWide>(c->K, c, bodies[i], b);
-  }
-  dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + o_top));
+  for (const uint32_t id : R.wide) dispatch_K<RefitWide>(c->K, c, id, c->bodies[id]);
+  dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + R.o_top));
   HIPCHK(c, hipGetLastError());
   if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;
