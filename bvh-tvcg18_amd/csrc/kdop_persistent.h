@@ -647,10 +647,12 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
 // The shared-stack round with 4 lanes per work item (expand_quad): a block of
 // W waves expands up to 16 W IPQ items per iteration, each quad IPQ items
 // (slot q of quad g takes the iteration's item g + 16 W q), loading their
-// records in coalesced 64-byte runs.  The stack holds S = 2^SL items;
-// everything else (frontier grabs by wave 0, two barriers per iteration, the
-// tail hand-off, spills and the per-wave leaf staging) works as in
-// k_persist_blk.
+// records in coalesced 64-byte runs.  The stack holds S = 2^SL items.  Wave 0
+// claims frontier chunks one iteration ahead and puts the claimed items on
+// the stack, so an iteration only pops the stack: the claim's round trip and
+// the items' load overlap the expansion instead of preceding it.  The two
+// barriers per iteration, the tail hand-off, spills and the per-wave leaf
+// staging work as in k_persist_blk.
 //
 // COUNT (record-counting calls, never timed): the quad's first lane also marks
 // every record its item reads in T.touched, so the host can count the
@@ -665,13 +667,13 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
   constexpr uint32_t IPI = QPI * IPQ;       // items per iteration
   constexpr uint32_t S = 1u << SL;
   static_assert(S >= 2u * NT && S >= 2u * IPI, "stack holds a full iteration and a hand-off");
+  static_assert(S >= 6u * IPI, "stack holds its remainder, an iteration's children and a grab");
   static_assert(kSegs == 64, "one lane of wave 0 per segment");
   static_assert(IPQ == 1 || IPQ == 2, "one or two items per quad");
   constexpr uint32_t SB = 64u * IPQ + 64u;  // staged leaf pairs per wave; flushed above SB - 64 IPQ (one iteration adds <= 64 IPQ)
   __shared__ uint32_t sa0[S], sa1[S], sb0[S], sb1[S], sp[S];
   __shared__ uint32_t lp_pair[W][SB], lp_ta[W][SB], lp_tb[W][SB];
-  __shared__ uint32_t wtot[W];
-  __shared__ uint32_t ctl[3];  // next grab: first frontier index, count; frontier done
+  __shared__ uint32_t xch[W + 2];  // per iteration: each wave's pushes, then wave 0's grabbed items and frontier-done flag
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform(tid >> 6);
   const uint32_t gi = w * 16u + (lane >> 2);  // this quad's item of an iteration (slot 0)
   const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, lane & 3u);
@@ -709,25 +711,45 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
     if (!wfront_done && ((known >> cur) & 1ull)) cur = next_open(known, cur);
   }
   uint32_t grabs = 0, wave0_iters = 0;
-  // wave 0: the grab for an iteration whose stack part is ns items; writes ctl
-  auto plan = [&](uint32_t ns) {
+  // Wave 0 takes the frontier one iteration ahead.  grab_issue claims the next
+  // chunk of its current segment before the expansion's loads, so the claim's
+  // round trip overlaps them; grab_finish (after the tests) loads the claimed
+  // items, and they go on the shared stack on top of the children, where the
+  // next iteration takes them like any stack item.  rem: items left on the
+  // stack after this iteration's pops; a block claims only while rem < IPI,
+  // so the claims are guided by the stack it holds, and at most 64 items (one
+  // per lane of wave 0) at a time.
+  uint32_t g_want = 0, g_len = 0, g_base = 0, g_raw = 0;
+  bool g_on = false;
+  auto grab_issue = [&](uint32_t rem) {
     cur = uniform(cur); next_est = uniform(next_est);
-    uint32_t gstart = 0, ng = 0;
+    g_on = false;
+    if (rem < IPI && !wfront_done) {
+      g_len = __builtin_amdgcn_readlane(seg_len, cur);
+      g_base = __builtin_amdgcn_readlane(seg_base, cur);
+      const uint32_t remaining = g_len > next_est ? g_len - next_est : 0u;
+      const uint32_t want = remaining / (2u * bps);
+      // at most one item per lane of wave 0, which loads and stacks them
+      const uint32_t most = min(IPI - rem, 64u);
+      g_want = want < 1u ? 1u : (want > most ? most : want);
+      g_raw = 0;
+      if (lane == 0) g_raw = atomicAdd(&C->seg[round][cur].next, g_want);
+      g_on = true;
+    }
+  };
+  // The claimed items (lane i < ng loads item i into fit), the segment
+  // bookkeeping and the frontier state; returns ng.
+  auto grab_finish = [&](Item& fit) -> uint32_t {
+    uint32_t ng = 0;
     bool refresh = false;
-    if (ns < IPI && !wfront_done) {
-      const uint32_t len = __builtin_amdgcn_readlane(seg_len, cur), sbase = __builtin_amdgcn_readlane(seg_base, cur);
-      const uint32_t remaining = len > next_est ? len - next_est : 0u;
-      uint32_t want = remaining / (2u * bps);
-      want = want < 1u ? 1u : (want > IPI - ns ? IPI - ns : want);
-      uint32_t start = 0;
-      if (lane == 0) start = atomicAdd(&C->seg[round][cur].next, want);
-      start = lane0(start);
-      if (start < len) {
+    if (g_on) {
+      const uint32_t start = lane0(g_raw);
+      if (start < g_len) {
         ++grabs;
-        next_est = start + want;
-        ng = min(want, len - start);
-        gstart = sbase + start;
-        if (start + want >= len) {  // this block took the segment's last chunk
+        next_est = start + g_want;
+        ng = min(g_want, g_len - start);
+        if (lane < ng) fit = in[g_base + start + lane];
+        if (start + g_want >= g_len) {  // this block took the segment's last chunk
           if (lane == 0) atomicOr(&C->done[round], 1ull << cur);
           known |= 1ull << cur;
         }
@@ -747,23 +769,31 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
       if (known == ~0ull) wfront_done = true;
       else { cur = next_open(known, cur); next_est = 0; }
     }
-    if (lane == 0) {
-      ctl[0] = gstart;
-      ctl[1] = ng;
-      ctl[2] = wfront_done ? 1u : 0u;
-    }
+    ++wave0_iters;
+    return ng;
   };
-  if (w == 0) plan(0u);
+  auto put = [&](uint32_t s, const Item& t) {
+    sa0[s] = t.a0; sa1[s] = t.a1; sb0[s] = t.b0; sb1[s] = t.b1; sp[s] = t.pair;
+  };
+  if (w == 0) {  // the first chunk, straight onto the stack
+    grab_issue(0u);
+    Item fit{0, 0, 0, 0, 0};
+    const uint32_t ng = grab_finish(fit);
+    if (lane < ng) put(lane, fit);
+    if (lane == 0) {
+      xch[W] = ng;
+      xch[W + 1] = wfront_done ? 1u : 0u;
+    }
+  }
   __syncthreads();
-  uint32_t top = 0, bot = 0;  // stack = [bot, top) modulo S, equal in every thread
+  uint32_t top = xch[W], bot = 0;  // stack = [bot, top) modulo S, equal in every thread
+  bool front_done = xch[W + 1] != 0u;
   uint32_t nstage = 0;        // this wave's staged leaf pairs
   const bool keep_p1 = T.p1 != nullptr;
   uint32_t tests = 0, descent = 0, broot = 0, p1s = 0;
   uint32_t iters = 0, lanes = 0, nsp = 0, handed = 0, recs = 0;
   QuadRec<K> A0[IPQ], A1[IPQ], B0[IPQ], B1[IPQ];  // this lane's words of each slot's records
   for (;;) {
-    const uint32_t gstart = ctl[0], ng = ctl[1];
-    const bool front_done = ctl[2] != 0u;
     // ---- tail: hand the bottom of a deep stack to the next round, 64 items per
     // wave, beyond `keep` items per wave (before this iteration's pops and pushes)
     if (front_done && (top - bot) > keep * W + NT) {
@@ -801,20 +831,19 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
       bot = top;
     }
     const uint32_t ns = min(IPI, top - bot);
-    if (ns + ng == 0u && front_done) break;  // block-uniform
-    // ---- batch: the top ns stack items, then ng frontier items; slot q of this quad
+    if (ns == 0u && front_done) break;  // block-uniform
+    if (w == 0) grab_issue(top - bot - ns);
+    // ---- batch: the top ns stack items; slot q of this quad
     Item it[IPQ];
     bool live[IPQ];
 #pragma unroll
     for (int q = 0; q < IPQ; ++q) {
       const uint32_t g = gi + (uint32_t)q * QPI;
       it[q] = Item{0, 0, 0, 0, 0};
-      live[q] = g < ns + ng;
-      if (g < ns) {
+      live[q] = g < ns;
+      if (live[q]) {
         const uint32_t s = (top - ns + g) & (S - 1);
         it[q] = Item{sa0[s], sa1[s], sb0[s], sb1[s], sp[s]};
-      } else if (live[q]) {
-        it[q] = in[gstart + g - ns];
       }
       if (live[q] && (lane & 3u) == 0u) {
         const uint32_t na = (it[q].a0 & I_EXP) ? 2u : 1u, nb = (it[q].b0 & I_EXP) ? 2u : 1u;
@@ -832,7 +861,7 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
 #pragma unroll
     for (int q = 0; q < IPQ; ++q) expand_quad<K>(qbase, it[q], live[q], A0[q], A1[q], B0[q], B1[q], o[q]);
     ++iters;
-    lanes += ns + ng;
+    lanes += ns;
     unsigned long long pm[IPQ];
     uint32_t kpre[IPQ], wsum = 0;
 #pragma unroll
@@ -842,7 +871,6 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
       kpre[q] = wsum + __builtin_amdgcn_mbcnt_hi((uint32_t)(pm[q] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm[q], 0u));
       wsum += (uint32_t)__popcll(pm[q]);
     }
-    if (lane == 0) wtot[w] = wsum;
     // ---- leaf pairs: stage in this wave's LDS (all of P1 when it is kept, else only P2)
 #pragma unroll
     for (int q = 0; q < IPQ; ++q) {
@@ -858,23 +886,32 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
         nstage += (uint32_t)__popcll(lm);
       }
     }
-    __syncthreads();  // A: the stack reads are done and every wave's push total is in wtot
+    // ---- the next frontier chunk (wave 0): its claim has returned by now
+    Item fit{0, 0, 0, 0, 0};
+    if (w == 0) {
+      const uint32_t ng = grab_finish(fit);
+      if (lane == 0) {
+        xch[W] = ng;
+        xch[W + 1] = wfront_done ? 1u : 0u;
+      }
+    }
+    if (lane == 0) xch[w] = wsum;
+    __syncthreads();  // A: the stack reads are done; every wave's push total and the grab are in xch
     uint32_t woff = 0, total = 0;
 #pragma unroll
     for (int v = 0; v < W; ++v) {
-      const uint32_t t = wtot[v];
+      const uint32_t t = xch[v];
       woff += (uint32_t)v < w ? t : 0u;
       total += t;
     }
+    const uint32_t ng = xch[W];
+    front_done = xch[W + 1] != 0u;
     // ---- children: the shared stack when they fit, else the next round (per wave)
     if (total) {
       if ((top - bot) + total <= S) {
 #pragma unroll
         for (int q = 0; q < IPQ; ++q)
-          if (o[q].push) {
-            const uint32_t s2 = (top + woff + kpre[q]) & (S - 1);
-            sa0[s2] = o[q].kid.a0; sa1[s2] = o[q].kid.a1; sb0[s2] = o[q].kid.b0; sb1[s2] = o[q].kid.b1; sp[s2] = o[q].kid.pair;
-          }
+          if (o[q].push) put((top + woff + kpre[q]) & (S - 1), o[q].kid);
         top += total;
       } else {
         const uint32_t sg = (home + W * (nsp++) + w) & (kSegs - 1);
@@ -891,16 +928,30 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
         }
       }
     }
+    // ---- the grabbed frontier items on top of the children (they always fit:
+    // a block claims only while fewer than IPI items stay on its stack, and
+    // S >= 6 IPI; the spill is a guard)
+    if (ng) {
+      if ((top - bot) + ng <= S) {
+        if (w == 0 && lane < ng) put((top + lane) & (S - 1), fit);
+        top += ng;
+      } else if (w == 0) {
+        const uint32_t sg = (home + W * (nsp++)) & (kSegs - 1);
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&C->seg[round + 1][sg].fill, ng);
+        b = lane0(b) + lane;
+        if (lane < ng) {
+          if (b < segcap) spill[(size_t)sg * segcap + b] = fit;
+          else atomicOr(T.overflow, 1u);
+        }
+      }
+    }
     if (nstage > SB - 64u * IPQ) {
       wave_lds_sync();
       flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);
       nstage = 0;
     }
-    if (w == 0) {
-      ++wave0_iters;
-      plan(min(IPI, top - bot));
-    }
-    __syncthreads();  // B: pushes, hand-offs and the next grab are visible to every wave
+    __syncthreads();  // B: pushes, hand-offs and the grabbed items are visible to every wave
   }
   wave_lds_sync();
   if (nstage) flush_leaves(T, lp_pair[w], lp_ta[w], lp_tb[w], nstage);

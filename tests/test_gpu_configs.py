@@ -65,6 +65,34 @@ def test_c3_small_stack_keep_completes():
         ctx.close()
 
 
+def test_c3_wide_iterations_claim_frontier():
+    """The full 512-pair batched draw under the persistent strategy with
+    workgroups that expand 128 items per iteration (eight waves per stack,
+    or four waves with two items per quad): the frontier is wide enough that
+    a workgroup would claim more frontier items than one wave has lanes, and
+    the claims are capped at one item per lane of the wave that loads them.
+    Counts, surface pairs and contacts equal the level-synchronous
+    strategy's."""
+    wl = W.batched_pairs(npairs=512, K=8, seed=11)
+    ref = device_run(wl, keep_all=False, traversal=K.KDOP_TRAVERSAL_LEVELS)
+    rc = ref["counters"]
+    key = lambda a: sorted(a[i:i + 1].tobytes() for i in range(a.shape[0]))
+    ref_p2, ref_c = key(ref["p2"]), key(ref["ctx"].contacts())
+    for opts in ({"block_waves": 8}, {"block_waves": 4, "quad_items": 2}, {"block_waves": 2}):
+        ctx = K.Context(K=8, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+        for name, v in opts.items():
+            ctx.set_option(name, v)
+        dev = device_run(wl, keep_all=False, ctx=ctx)
+        c = dev["counters"]
+        st = ctx.traversal_stats()
+        print(opts, "rounds", st["rounds"], "grabs", st["grabs"], "tests", c["tests"], flush=True)
+        for f in ("tests", "leaf_pairs", "surface_pairs", "contacts", "narrow_pairs", "overflow"):
+            assert c[f] == rc[f], f
+        assert key(dev["p2"]) == ref_p2
+        assert key(ctx.contacts()) == ref_c
+        ctx.close()
+
+
 def test_self_contact_completes():
     """A jittered 20^3 box (40,000 tets, K=26) against an identical copy at
     identity, the densest traversal these meshes make: under AUTO, under the
