@@ -1191,6 +1191,16 @@ __device__ inline void stage_p2(P2Stage& S, uint32_t pair, const Expansion& x) {
   for (int k = 0; k < 4; ++k)
     if ((x.surf >> k) & 1u) S.p[pos++] = LeafPair{pair, x.ta[k >> 1], x.tb[k & 1]};
 }
+// One lane's surface pair (expand_quad: at most one per lane), with one LDS
+// reservation per wave.  Every lane of the wave must call it.
+__device__ inline void stage_p2_lane(P2Stage& S, uint32_t pair, bool surf, uint32_t ta, uint32_t tb) {
+  const unsigned long long m = __ballot(surf);
+  if (m == 0ull) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(&S.n, (uint32_t)__popcll(m));
+  base = lane0(base);
+  if (surf) S.p[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = LeafPair{pair, ta, tb};
+}
 // Write the stage to P2 when forced (the block's end) or when the next
 // iteration might not fit.  Every thread of the block calls it.
 __device__ inline void flush_p2(const TravArgs& T, P2Stage& S, bool force) {

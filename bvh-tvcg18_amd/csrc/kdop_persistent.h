@@ -73,7 +73,13 @@ struct CallState {
 // the tests, set the level's time.  Item i of a segmented level is found from
 // the prefix sums of the segment fills.  Level l clears set (l + 2) % 3, the
 // previous level's input, for level l + 1 to write.
-template <int K>
+//
+// QUAD: four lanes per item (expand_quad), so a block expands 64 items per
+// pass with 8 record loads and 32 slab compares per lane instead of 256 items
+// with 32 loads and up to 104 compares per lane.  The narrow levels of the
+// persistent strategy's ramp then spread over four times as many waves and
+// each lane's chain of loads and tests is a quarter as long.
+template <int K, bool QUAD>
 __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
                                                    CallState* __restrict__ C, uint32_t level, uint32_t target, uint32_t last,
                                                    uint32_t nseg_in, uint32_t nseg_out) {
@@ -92,7 +98,8 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   // level's size is known (its slot exists whatever the size; the value is
   // used only when the slot holds an item), so that load overlaps the size read
   const Item* __restrict__ in = (level & 1) ? buf1 : buf0;
-  const uint32_t i_first = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr uint32_t LPI = QUAD ? 4u : 1u;  // lanes per item
+  const uint32_t i_first = (blockIdx.x * blockDim.x + threadIdx.x) / LPI;
   const bool early = nseg_in == 1u && i_first < cap;
   Item it_first{0, 0, 0, 0, 0};
   if (early) it_first = in[i_first];
@@ -136,6 +143,49 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
   unsigned long long tests = 0, descent = 0, broot = 0, p1s = 0;
   if (threadIdx.x == 0) st2.n = 0u;
   __syncthreads();
+  if constexpr (QUAD) {
+    const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, threadIdx.x & 3u);
+    const uint32_t ipb = blockDim.x / 4u;  // items per block pass
+    for (uint32_t base = blockIdx.x * ipb; base < n; base += gridDim.x * ipb) {
+      const uint32_t i = base + (threadIdx.x >> 2);
+      const bool live = i < n;
+      Item it{0, 0, 0, 0, 0};
+      if (live) {
+        if (early && i == i_first) {
+          it = it_first;
+        } else {
+          uint32_t idx = i;
+          if (nseg_in > 1u) {  // segment s with pre[s] <= i < pre[s + 1]
+            uint32_t lo = 0, hi = kSegs;
+            while (hi - lo > 1u) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (pre[mid] <= i) lo = mid; else hi = mid;
+            }
+            idx = lo * segcap_in + (i - pre[lo]);
+          }
+          it = in[idx];
+        }
+      }
+      QuadRec<K> A0, A1, B0, B1;
+      QuadOut o;
+      expand_quad<K>(qbase, it, live, A0, A1, B0, B1, o);
+      tests += o.test; descent += o.descent; broot += o.broot; p1s += o.leaf;
+      const uint32_t slot = block_reserve(out_fill, o.push ? 1u : 0u, red);
+      if (o.push) {
+        if (slot < segcap_out) out_seg[slot] = o.kid;
+        else atomicOr(T.overflow, 1u);
+      }
+      if (T.p1) {
+        const uint32_t o1 = wave_reserve(T.p1_fill, o.leaf ? 1u : 0u);
+        if (o.leaf) {
+          if (o1 < T.p1_cap) T.p1[o1] = LeafPair{it.pair, o.ta, o.tb};
+          else atomicOr(T.overflow, 2u);
+        }
+      }
+      stage_p2_lane(st2, it.pair, o.surf, o.ta, o.tb);
+      flush_p2(T, st2, false);
+    }
+  } else {
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;
     Expansion x;
@@ -165,6 +215,7 @@ __global__ __launch_bounds__(256) void k_bfs_level(TravArgs T, Item* __restrict_
     emit_p1(T, it.pair, x);
     stage_p2(st2, it.pair, x);
     flush_p2(T, st2, false);
+  }
   }
   flush_p2(T, st2, true);
   CounterStripe* st = stripe_of(T.stripes, wave_global());

@@ -274,12 +274,15 @@ def test_queued_rounds_follow_recent_calls():
     that spill over several (a stack keep of 1): every call, whether its
     rounds were queued or drained one by one, gives the level-synchronous
     strategy's counts, surface pairs and contacts, including the calls
-    right after the number of rounds changed and graph replays."""
+    right after the number of rounds changed and graph replays.  The round
+    budget is off, so only the stack keep decides how many rounds a call
+    needs."""
     wl = W.batched_pairs(npairs=16, K=14, seed=21, tmin=2000, tmax=40000)
     key = lambda a: sorted(a[i:i + 1].tobytes() for i in range(a.shape[0]))
     ref = device_run(wl, keep_all=False, traversal=K.KDOP_TRAVERSAL_LEVELS)
     want = (ref["counters"], key(ref["p2"]), key(ref["ctx"].contacts()))
     ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+    ctx.set_option("round_iters", 0)
     ids = [ctx.make_tree(m) for m in wl.meshes]
     ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
     pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
