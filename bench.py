@@ -69,6 +69,8 @@ def headline_leg(args):
         ctx.set_option("block_waves", args.block_waves)
     if args.item_lanes:
         ctx.set_option("item_lanes", args.item_lanes)
+    if args.node_slab:
+        ctx.set_option("node_slab", args.node_slab)
     ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
     dp, dq = device_poses(pose_p, pose_q)
     # the step chains refit and traversal on the context stream: the refit
@@ -558,6 +560,7 @@ def main():
     ap.add_argument("--bfs-max", type=int, default=0, help="persistent strategy: most breadth-first levels first")
     ap.add_argument("--block-waves", type=int, default=0, help="persistent kernel: waves sharing one stack (1, 4, 8)")
     ap.add_argument("--item-lanes", type=int, default=0, help="persistent kernel: lanes per work item (1 or 4)")
+    ap.add_argument("--node-slab", type=int, default=0, help="persistent kernel: 1 stages node records in LDS node slabs")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-reorder", action="store_true", help="skip the device Morton reorder leg")
     ap.add_argument("--rays", type=int, default=65536, help="rays of the raycast leg (0: skip)")

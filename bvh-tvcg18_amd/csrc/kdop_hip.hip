@@ -146,6 +146,7 @@ struct kdop_ctx {
   int bw = 0;                // waves per persistent workgroup (1: a stack per wave; 2, 4, 8: a shared stack)
   int item_lanes = 4;        // lanes per work item in the persistent rounds (4: k_persist_quad; 1: k_persist / k_persist_blk)
   int quad_items = 1;        // KDOP_OPT_QUAD_ITEMS: items per quad per iteration of k_persist_quad (1 or 2)
+  int node_slab = 0;         // KDOP_OPT_NODE_SLAB: 1 stages each wave's records in an LDS node slab (k_persist_quad SLAB)
   uint32_t round_iters = kRoundItersDefault;  // KDOP_OPT_ROUND_ITERS: a block's iteration budget per round once the frontier is gone (0: none)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
@@ -751,19 +752,26 @@ void launch_round_blk(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t rou
 }
 // A persistent round of W-wave workgroups with 4 lanes per work item
 // (k_persist_quad) and a shared stack of 2^SL items.
-template <int K, int W, int SL, bool COUNT = false, int IPQ = 1>
+template <int K, int W, int SL, bool COUNT = false, int IPQ = 1, bool SLAB = false>
 void launch_round_quad(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
-  const dim3 grid((unsigned)std::max(1, c->num_cus * wpc_of(c) / W)), block(64 * W);
+  int per_cu = wpc_of(c) / W;
+  if (SLAB) {  // the node slabs' LDS holds fewer workgroups per CU: launch as many as are resident
+    static int occ = -1;
+    if (occ < 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_persist_quad<K, W, SL, COUNT, IPQ, SLAB>, 64 * W, 0) != hipSuccess)
+      occ = 1;
+    per_cu = std::min(per_cu, std::max(1, occ));
+  }
+  const dim3 grid((unsigned)std::max(1, c->num_cus * per_cu)), block(64 * W);
   Item* b0 = c->front[0].p;
   Item* b1 = c->front[1].p;
   // the budget applies in rounds that may spill (the last one allowed takes
   // no budget: its spill would only go to the level-synchronous finish)
   const uint32_t budget = (c->round_iters && round + 1 < c->max_rounds) ? c->round_iters : ~0u;
   if (c->timing_level >= 2)
-    hipExtLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1],
+    hipExtLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ, SLAB>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1],
                           0, T, b0, b1, cap, c->dstate, round, round_keep(c, round), last, budget);
   else
-    hipLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round,
+    hipLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ, SLAB>), grid, block, 0, c->stream, T, b0, b1, cap, c->dstate, round,
                        round_keep(c, round), last, budget);
 }
 template <int K>
@@ -772,6 +780,11 @@ struct PersistRound {
     const int bw = bw_of(c), ls = ls_of(c);
     if (c->count_records) {  // record-counting calls: the quad kernel in its default shape
       launch_round_quad<K, 4, 10, true>(c, T, cap, round, last);
+      return;
+    }
+    if (c->item_lanes == 4 && c->node_slab) {  // records staged in per-wave LDS node slabs
+      if (bw <= 2) launch_round_quad<K, 2, 9, false, 1, true>(c, T, cap, round, last);
+      else launch_round_quad<K, 4, 10, false, 1, true>(c, T, cap, round, last);
       return;
     }
     if (c->item_lanes == 4 && c->quad_items == 2) {  // two items per quad, 4-wave workgroups
@@ -1008,6 +1021,10 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       if (v != 1 && v != 2) break;
       c->quad_items = (int)v;
       return KDOP_OK;
+    case KDOP_OPT_NODE_SLAB:
+      if (v != 0 && v != 1) break;
+      c->node_slab = (int)v;
+      return KDOP_OK;
     case KDOP_OPT_COUNT_RECORDS:
       if (v != 0 && v != 1) break;
       c->count_records = v != 0;
@@ -1072,11 +1089,12 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       // no hand-off, or a keep that hands on the bottom of deep stacks
       case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
       case KDOP_OPT_ROUND_ITERS: return {0, 16, 64};
+      case KDOP_OPT_NODE_SLAB: return quad ? std::vector<int64_t>{0, 1} : std::vector<int64_t>{0};
       case KDOP_OPT_RAMP_TARGET: return {4 * pw, 16 * pw, 64 * pw};
       default: return {16384, 65536, 262144};  // KDOP_OPT_SEED_BUDGET
     }
   };
-  const int order[] = {KDOP_OPT_TRAVERSAL, KDOP_OPT_ITEM_LANES, KDOP_OPT_ROUND_ITERS, KDOP_OPT_BLOCK_WAVES, KDOP_OPT_WAVES_PER_CU,
+  const int order[] = {KDOP_OPT_TRAVERSAL, KDOP_OPT_ITEM_LANES, KDOP_OPT_NODE_SLAB, KDOP_OPT_ROUND_ITERS, KDOP_OPT_BLOCK_WAVES, KDOP_OPT_WAVES_PER_CU,
                        KDOP_OPT_QUEUE_DEPTH, KDOP_OPT_STACK_KEEP, KDOP_OPT_RAMP_TARGET, KDOP_OPT_SEED_BUDGET};
   for (const int opt : order) {
     struct {
@@ -1132,6 +1150,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_MAX_ROUNDS: *v = c->max_rounds; return KDOP_OK;
     case KDOP_OPT_COUNT_RECORDS: *v = c->count_records ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_QUAD_ITEMS: *v = c->quad_items; return KDOP_OK;
+    case KDOP_OPT_NODE_SLAB: *v = c->node_slab; return KDOP_OK;
     case KDOP_OPT_ROUND_ITERS: *v = c->round_iters; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
@@ -1635,7 +1654,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
           (uint64_t)(uintptr_t)c->gate.p, (uint64_t)wpc_of(c), (uint64_t)ls_of(c), c->keep, (uint64_t)c->num_cus, (uint64_t)bw_of(c), rounds_async,
-          (uint64_t)c->item_lanes, c->max_rounds, (uint64_t)c->quad_items, c->round_iters};
+          (uint64_t)c->item_lanes, c->max_rounds, (uint64_t)c->quad_items, c->round_iters, (uint64_t)c->node_slab};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;

@@ -228,8 +228,20 @@ __device__ inline void store_vol(float* __restrict__ nodes, uint32_t slot, const
   for (int f = 0; f + 4 <= K; f += 4) *reinterpret_cast<float4*>(r + f) = make_float4(v[f], v[f + 1], v[f + 2], v[f + 3]);
   if (K % 4) *reinterpret_cast<float2*>(r + (K & ~3)) = make_float2(v[K - 2], v[K - 1]);
 }
+// Minimum / maximum of slab bounds built from leaf projections.  A
+// projection dot3(d, P) starts from +0 (0 + d.x P.x), so in round-to-nearest
+// it is never -0; it is NaN only for a NaN or infinite vertex.  Over such
+// values the reference's std::min / std::max ((b < a) ? b : a) and IEEE
+// minNum / maxNum (v_min_f32 / v_max_f32, which skip a NaN operand) give the
+// same bits: equal values have equal bits, and a NaN projection is skipped by
+// both when the running bound (never NaN) is the other operand.  One VALU
+// instruction (three-operand forms fold chains) instead of a compare and a
+// select.
+__device__ inline float bmin(float a, float b) { return fminf(a, b); }
+__device__ inline float bmax(float a, float b) { return fmaxf(a, b); }
+
 // Leaf DOP of one tetrahedron (geometry_dop.h:48-74): per slab direction the
-// min / max projection of its four world vertices.
+// min / max projection of its four world vertices (bmin / bmax, above).
 // pos(i): world position of vertex i.
 template <int K, class Pos>
 __device__ inline void tet_dop_at(const BodyDev& B, uint32_t tet, const DirTable& dt, Pos&& pos, float* v) {
@@ -245,8 +257,8 @@ __device__ inline void tet_dop_at(const BodyDev& B, uint32_t tet, const DirTable
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const float proj = dot3(d, P[p]);
-      lo = smin(lo, proj);
-      hi = smax(hi, proj);
+      lo = bmin(lo, proj);
+      hi = bmax(hi, proj);
     }
     v[2 * k] = lo;
     v[2 * k + 1] = hi;
@@ -276,15 +288,16 @@ __device__ inline float lane_xor(float v) {
 }
 
 // One butterfly level of the chunk unions: lanes j and j ^ S hold the two
-// children (left = the one without bit S) and both end with their union.
+// children and both end with their union.  The union of bounds of leaf
+// projections does not depend on the operand order (bmin / bmax), so the
+// left child need not come first as in geometry_make_union.h:9-25.
 template <int K, int S>
 __device__ inline void union_level(float* v, uint32_t j) {
-  const bool left = (j & S) == 0u;
+  (void)j;
 #pragma unroll
   for (int f = 0; f < K; ++f) {
     const float y = lane_xor<S>(v[f]);
-    const float a = left ? v[f] : y, b = left ? y : v[f];
-    v[f] = (f & 1) ? smax(a, b) : smin(a, b);
+    v[f] = (f & 1) ? bmax(v[f], y) : bmin(v[f], y);
   }
 }
 
@@ -295,7 +308,8 @@ __device__ inline void union_level(float* v, uint32_t j) {
 // elements (lane xor L, 2L, ...) holds no copies; even components are
 // minima, odd ones maxima, and L is even, so a lane does one kind only.  The
 // super chunks' nodes go to rows sc * L + BFS index (sact: this lane's super
-// chunk is full), overwriting the roots.
+// chunk is full), overwriting the roots.  Unions by bmin / bmax (their
+// operand order does not matter).
 template <int K, int LOGL, uint32_t RS>
 __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bool sact) {
   constexpr uint32_t L = 1u << LOGL, NC = (K + L - 1) / L;
@@ -308,7 +322,6 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
 #pragma unroll
   for (uint32_t t = 1; t <= LOGL; ++t) {
     const uint32_t S = L << (t - 1u);
-    const bool left = (lane & S) == 0u;
 #pragma unroll
     for (uint32_t m = 0; m < NC; ++m) {
       float y;
@@ -319,8 +332,7 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
         case 16: y = lane_xor<16>(w[m]); break;
         default: y = lane_xor<32>(w[m]); break;
       }
-      const float a = left ? w[m] : y, b = left ? y : w[m];
-      w[m] = hi ? smax(a, b) : smin(a, b);
+      w[m] = hi ? bmax(w[m], y) : bmin(w[m], y);
     }
     if (sact && (ei & ((2u << (t - 1u)) - 1u)) == 0u) {
       float* row = st + (sc * L + (1u << (LOGL - t)) - 1u + (ei >> t)) * RS;
@@ -1069,10 +1081,21 @@ struct QuadOut {
 // record 0).  A side that does not expand loads its one record twice, which
 // keeps the loads free of branches.  A0 .. B1: the caller's registers for the
 // four records.
+// Record slot s (0: A's first, 1: A's second, 2: B's first, 3: B's second
+// node) that expand_quad loads for an item: a side that does not expand
+// names its one node twice.
+__device__ inline uint32_t quad_record(const Item& it, uint32_t s) {
+  const bool exp_a = (it.a0 & I_EXP) != 0u, exp_b = (it.b0 & I_EXP) != 0u;
+  const uint32_t ca0 = it.a0 & IDX_MASK, cb0 = it.b0 & IDX_MASK;
+  const uint32_t x0 = s < 2u ? ca0 : cb0, x1 = s < 2u ? (exp_a ? it.a1 : ca0) : (exp_b ? it.b1 : cb0);
+  return (s & 1u) ? x1 : x0;
+}
+template <int K>
+__device__ inline void expand_quad_test(const Item& it, bool live, const QuadRec<K>& A0, const QuadRec<K>& A1, const QuadRec<K>& B0,
+                                        const QuadRec<K>& B1, QuadOut& o);
 template <int K>
 __device__ inline void expand_quad(const float4* __restrict__ lane_base, const Item& it, bool live, QuadRec<K>& A0, QuadRec<K>& A1,
                                    QuadRec<K>& B0, QuadRec<K>& B1, QuadOut& o) {
-  const uint32_t r = (uint32_t)lane_id() & 3u, i = r >> 1, j = r & 1u;
   const bool exp_a = (it.a0 & I_EXP) != 0u, exp_b = (it.b0 & I_EXP) != 0u;
   const uint32_t ca0 = it.a0 & IDX_MASK, cb0 = it.b0 & IDX_MASK;
   const uint32_t ca1 = exp_a ? it.a1 : ca0, cb1 = exp_b ? it.b1 : cb0;
@@ -1080,6 +1103,16 @@ __device__ inline void expand_quad(const float4* __restrict__ lane_base, const I
   load_quad<K>(lane_base, ca1, A1);
   load_quad<K>(lane_base, cb0, B0);
   load_quad<K>(lane_base, cb1, B1);
+  expand_quad_test<K>(it, live, A0, A1, B0, B1, o);
+}
+// The tests of expand_quad on records already in this lane's words.
+template <int K>
+__device__ inline void expand_quad_test(const Item& it, bool live, const QuadRec<K>& A0, const QuadRec<K>& A1, const QuadRec<K>& B0,
+                                        const QuadRec<K>& B1, QuadOut& o) {
+  const uint32_t r = (uint32_t)lane_id() & 3u, i = r >> 1, j = r & 1u;
+  const bool exp_a = (it.a0 & I_EXP) != 0u, exp_b = (it.b0 & I_EXP) != 0u;
+  const uint32_t ca0 = it.a0 & IDX_MASK, cb0 = it.b0 & IDX_MASK;
+  const uint32_t ca1 = exp_a ? it.a1 : ca0, cb1 = exp_b ? it.b1 : cb0;
   // separation bit of combination k in bit k, ORed over the quad
   uint32_t sep = (quad_sep<K>(A0, B0, r) ? 1u : 0u) | (quad_sep<K>(A0, B1, r) ? 2u : 0u) | (quad_sep<K>(A1, B0, r) ? 4u : 0u) |
                  (quad_sep<K>(A1, B1, r) ? 8u : 0u);

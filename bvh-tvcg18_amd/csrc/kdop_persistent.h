@@ -709,7 +709,14 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
 // every record its item reads in T.touched, so the host can count the
 // distinct records a round reads.  budget: iterations after which a block
 // whose frontier is gone hands its whole stack on (~0u: never).
-template <int K, int W, int SL, bool COUNT, int IPQ>
+//
+// SLAB (KDOP_OPT_NODE_SLAB): a wave stages the records of its 16 items in a
+// node slab in LDS first -- eight consecutive lanes load one record as whole
+// 16-byte words, so a wave-level load covers 128 / rec bytes records in full
+// lines -- and each quad then reads its words from the slab.  The slab takes
+// 64 records x rec_stride(K) floats per wave (8 KB at K >= 18), which the LDS
+// holds for fewer workgroups per CU.
+template <int K, int W, int SL, bool COUNT, int IPQ, bool SLAB = false>
 __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
                                                          CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last,
                                                          uint32_t budget) {
@@ -725,6 +732,9 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
   __shared__ uint32_t sa0[S], sa1[S], sb0[S], sb1[S], sp[S];
   __shared__ uint32_t lp_pair[W][SB], lp_ta[W][SB], lp_tb[W][SB];
   __shared__ uint32_t xch[W + 2];  // per iteration: each wave's pushes, then wave 0's grabbed items and frontier-done flag
+  static_assert(!SLAB || IPQ == 1, "node slabs with one item per quad");
+  constexpr uint32_t RW = rec_stride(K) / 4;  // 16-byte words per record
+  __shared__ float4 slab[SLAB ? W : 1][SLAB ? 64u * RW : 1u];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform(tid >> 6);
   const uint32_t gi = w * 16u + (lane >> 2);  // this quad's item of an iteration (slot 0)
   const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, lane & 3u);
@@ -909,8 +919,34 @@ __global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __res
     }
     top -= ns;
     QuadOut o[IPQ];
+    if constexpr (SLAB) {
+      // lane 4 q + s names record slot s of quad q's item, so record rho of the
+      // wave's slab is named by lane rho
+      const uint32_t mine = quad_record(it[0], lane & 3u);
+      const float4* __restrict__ nodes4 = reinterpret_cast<const float4*>(T.nodes);
+      float4* __restrict__ sl = slab[w];
 #pragma unroll
-    for (int q = 0; q < IPQ; ++q) expand_quad<K>(qbase, it[q], live[q], A0[q], A1[q], B0[q], B1[q], o[q]);
+      for (uint32_t k = 0; k < RW; ++k) {
+        const uint32_t x = k * 64u + lane, rho = x / RW;
+        const uint32_t rec = (uint32_t)__shfl((int)mine, (int)rho, 64);
+        sl[x] = nodes4[(size_t)rec * RW + (x % RW)];
+      }
+      wave_lds_sync();
+      const uint32_t r = lane & 3u, qb = (lane & ~3u) * RW;  // this quad's first slab word
+#pragma unroll
+      for (int t = 0; t < Quad<K>::NQ; ++t) {
+        const uint32_t wd = min(r + 4u * (uint32_t)t, RW - 1u);
+        A0[0].w[t] = sl[qb + wd];
+        A1[0].w[t] = sl[qb + RW + wd];
+        B0[0].w[t] = sl[qb + 2u * RW + wd];
+        B1[0].w[t] = sl[qb + 3u * RW + wd];
+      }
+      wave_lds_sync();
+      expand_quad_test<K>(it[0], live[0], A0[0], A1[0], B0[0], B1[0], o[0]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < IPQ; ++q) expand_quad<K>(qbase, it[q], live[q], A0[q], A1[q], B0[q], B1[q], o[q]);
+    }
     ++iters;
     lanes += ns;
     unsigned long long pm[IPQ];

@@ -25,7 +25,7 @@ KDOP_TRAVERSAL_PERSISTENT = 2
 OPTIONS = {"traversal": 1, "queue_depth": 2, "waves_per_cu": 3, "ramp_target": 4, "stack_keep": 5,
            "seed_budget": 6, "kernel_timing": 7, "bfs_max": 8, "async_refit": 9,
            "narrow_path": 10, "graphs": 11, "single_items": 12,
-           "block_waves": 13, "item_lanes": 14, "max_rounds": 15, "count_records": 16, "quad_items": 17, "round_iters": 18}
+           "block_waves": 13, "item_lanes": 14, "max_rounds": 15, "count_records": 16, "quad_items": 17, "round_iters": 18, "node_slab": 19}
 KDOP_NARROW = 1
 KDOP_ORDER_SEQUENTIAL = 0
 KDOP_ORDER_OPENCL = 1

@@ -384,6 +384,11 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 #define KDOP_OPT_QUAD_ITEMS 17    /* quad kernel: items each quad expands per iteration, 1 (0:  */
                                   /*   the default) or 2 (twice as wide an iteration, 4-wave     */
                                   /*   workgroups).  Results unchanged                           */
+#define KDOP_OPT_NODE_SLAB 19     /* quad kernel: 1 stages each wave's node records in an LDS    */
+                                  /*   node slab with whole-line loads (8 lanes per record) and   */
+                                  /*   tests from LDS; fewer workgroups per CU fit.  0 (default): */
+                                  /*   the quads load their words straight into registers.        */
+                                  /*   Results unchanged                                           */
 #define KDOP_OPT_COUNT_RECORDS 16 /* 1: measurement calls -- the persistent rounds (quad kernel,  */
                                   /*   default shape) also mark each node record they read, and   */
                                   /*   kdop_traversal_stats reports how many distinct ones; slower */

@@ -154,6 +154,22 @@ def test_persistent_shared_stack_blocks(bw, wpc, keep):
         _check_pair_results(wl, dev, orc)
 
 
+@pytest.mark.parametrize("bw,k", [(4, 26), (2, 18), (4, 14), (4, 8), (2, 6)])
+def test_persistent_node_slabs(bw, k):
+    """Persistent rounds that stage each wave's node records in an LDS node
+    slab (KDOP_OPT_NODE_SLAB) before testing: the oracle's counts, pair sets
+    and contacts, for every record width (K = 6, 8-14, 18-26) and both slab
+    workgroup shapes, on a single pair and on a batch of uneven pairs."""
+    for wl in (W.large_pair(n=10, K=k, seed=13), W.batched_pairs(npairs=10, K=k, seed=8, tmin=300, tmax=5000)):
+        orc, _, _ = oracle_run(wl)
+        ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+        ctx.set_option("block_waves", bw)
+        ctx.set_option("node_slab", 1)
+        assert ctx.get_option("node_slab") == 1
+        dev = device_run(wl, ctx=ctx)
+        _check_pair_results(wl, dev, orc)
+
+
 def test_duplicate_body_pairs_fall_back_to_levels():
     """The same body pair twice in one call: both copies are reported."""
     wl = W.single_pair(n=5, K=8)
