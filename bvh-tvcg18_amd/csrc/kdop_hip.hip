@@ -684,7 +684,10 @@ struct Seed {
   // expand: write level 1 (the seeds' expansion, into front[1]) instead of level 0
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t total, uint32_t* fill, uint32_t cap, bool expand) {
     const uint32_t blocks = std::max(1u, std::min(4096u, (total + 255) / 256));
-    if (expand)
+    if (expand && c->item_lanes == 4)  // four lanes per seed pair, like the ramp levels
+      hipLaunchKernelGGL(k_seed_quad<K>, dim3(std::max(1u, std::min(4096u, (total + 63) / 64))), dim3(256), 0, c->stream, T, c->dseeds.p,
+                         total, c->front[1].p, fill, cap);
+    else if (expand)
       hipLaunchKernelGGL((k_seed<K, true>), dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[1].p, fill, cap);
     else
       hipLaunchKernelGGL((k_seed<K, false>), dim3(blocks), dim3(256), 0, c->stream, T, c->dseeds.p, total, c->front[0].p, fill, cap);

@@ -935,16 +935,22 @@ __device__ inline void expand_rule(uint32_t mx0, uint32_t my0, bool& ex, bool& e
 }
 // Work item for the overlapping pair (x at slot xi, y at slot yi).
 template <int K>
-__device__ inline Item make_item(const NodeRec<K>& X, uint32_t xi, const NodeRec<K>& Y, uint32_t yi, uint32_t pair) {
+__device__ inline Item make_item(const NodeRec<K>& X, uint32_t xi, const NodeRec<K>& Y, uint32_t yi, uint32_t pair);
+// The same from the two records' meta words (m0, m1).
+__device__ inline Item make_item_meta(uint2 mx, uint32_t xi, uint2 my, uint32_t yi, uint32_t pair) {
   bool ex, ey;
-  expand_rule(X.m0, Y.m0, ex, ey);
+  expand_rule(mx.x, my.x, ex, ey);
   Item it;
-  it.a0 = ex ? (I_EXP | (X.m0 & IDX_MASK)) : xi;
-  it.a1 = ex ? (X.m1 & IDX_MASK) : 0u;
-  it.b0 = ey ? (I_EXP | (Y.m0 & IDX_MASK)) : yi;
-  it.b1 = ey ? (Y.m1 & IDX_MASK) : 0u;
+  it.a0 = ex ? (I_EXP | (mx.x & IDX_MASK)) : xi;
+  it.a1 = ex ? (mx.y & IDX_MASK) : 0u;
+  it.b0 = ey ? (I_EXP | (my.x & IDX_MASK)) : yi;
+  it.b1 = ey ? (my.y & IDX_MASK) : 0u;
   it.pair = pair;
   return it;
+}
+template <int K>
+__device__ inline Item make_item(const NodeRec<K>& X, uint32_t xi, const NodeRec<K>& Y, uint32_t yi, uint32_t pair) {
+  return make_item_meta(make_uint2(X.m0, X.m1), xi, make_uint2(Y.m0, Y.m1), yi, pair);
 }
 
 // Result of expanding one work item: new items and three 4-bit masks over the
@@ -1349,6 +1355,122 @@ __global__ __launch_bounds__(256) void k_seed(TravArgs T, const SeedDesc* __rest
     } else {
       const uint32_t o = block_reserve(out_fill, x.push, red);
       emit_items(x, out, o, out_cap, T.overflow);
+    }
+  }
+  CounterStripe* st = stripe_of(T.stripes, wave_global());
+  wave_add64(&st->root_tests, roots);
+  wave_add64(&st->tests, tests + roots);
+  wave_add64(&st->descent, descent);
+  wave_add64(&st->branch_pairs, bpairs);
+  wave_add64(&st->branch_overlaps, broot);
+  wave_add64(&st->p1, p1s);
+}
+
+// k_seed<K, true> with four lanes per seed pair (expand_quad): each quad
+// loads the root records and the seed's chunk roots as 16-byte words (8 loads
+// per lane instead of 32), tests both pairs, and expands an overlapping seed
+// pair itself.  The quad's first lane counts and writes what belongs to the
+// seed pair; every lane its own child combination.  The same tests, counts,
+// pairs and items as k_seed<K, true>; the persistent strategy's seeding.
+template <int K>
+__global__ __launch_bounds__(256) void k_seed_quad(TravArgs T, const SeedDesc* __restrict__ sd, uint32_t total, Item* __restrict__ out,
+                                                  uint32_t* __restrict__ out_fill, uint32_t out_cap) {
+  __shared__ uint32_t red[5];
+  unsigned long long tests = 0, broot = 0, p1s = 0, roots = 0, bpairs = 0, descent = 0;
+  const uint32_t r = threadIdx.x & 3u;
+  const bool lead = r == 0u;
+  const float4* __restrict__ qbase = quad_lane_base<K>(T.nodes, r);
+  const uint32_t spb = blockDim.x / 4u;  // seed pairs per block pass
+  for (uint32_t base = blockIdx.x * spb; base < total; base += gridDim.x * spb) {
+    const uint32_t g = base + (threadIdx.x >> 2);
+    const bool live = g < total;
+    uint32_t pair = 0, xa = 0, yb = 0, ra = 0, rb = 0, local = 0;
+    unsigned long long nbr = 0;
+    if (live) {
+      uint32_t lo = 0, hi = T.npairs;
+      while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (sd[m].base <= g) lo = m; else hi = m;
+      }
+      pair = lo;
+      const SeedDesc d = sd[pair];
+      local = g - d.base;
+      const uint32_t i = local / d.nb, j = local - i * d.nb;
+      xa = T.tops[d.a_off + i];
+      yb = T.tops[d.b_off + j];
+      ra = d.root_a;
+      rb = d.root_b;
+      nbr = (unsigned long long)d.nbr_a * d.nbr_b;
+    }
+    // the root records and the seed's chunk roots load together (a dead quad reads record 0)
+    QuadRec<K> RA, RB, X, Y;
+    load_quad<K>(qbase, ra, RA);
+    load_quad<K>(qbase, rb, RB);
+    load_quad<K>(qbase, xa, X);
+    load_quad<K>(qbase, yb, Y);
+    uint32_t sep = (quad_sep<K>(RA, RB, r) ? 1u : 0u) | (quad_sep<K>(X, Y, r) ? 2u : 0u);
+    sep |= quad_dpp<0xB1>(sep);  // quad_perm [1, 0, 3, 2]
+    sep |= quad_dpp<0x4E>(sep);  // quad_perm [2, 3, 0, 1]
+    const uint2 mx = quad_meta<K>(X), my = quad_meta<K>(Y);
+    const bool ov = live && (sep & 1u) == 0u;  // the seed counts only when the roots overlap
+    if (lead && live && local == 0u) {
+      ++roots;
+      if (ov) bpairs += nbr;
+    }
+    bool push = false, leaf = false, surf = false;
+    Item kid{0, 0, 0, 0, 0};
+    if (ov) {
+      if (lead) ++tests;
+      const bool both_roots = (mx.x & F_BRANCH) && (my.x & F_BRANCH) && (mx.y & F_ROOT) && (my.y & F_ROOT);
+      if ((sep & 2u) == 0u) {
+        if (lead && both_roots) ++broot;
+        if ((mx.x & F_LEAF) && (my.x & F_LEAF)) {
+          leaf = true;
+          surf = (mx.y & F_SURF) && (my.y & F_SURF);
+        } else {
+          push = true;
+          kid = make_item_meta(mx, xa, my, yb, pair);
+        }
+      }
+    }
+    // a seed pair of two tet leaves (one-tet bodies): written by the quad's first lane
+    const bool sl = lead && leaf, ss = lead && surf;
+    p1s += sl ? 1u : 0u;
+    if (T.p1) {
+      const uint32_t o1 = wave_reserve(T.p1_fill, sl ? 1u : 0u);
+      if (sl) {
+        if (o1 < T.p1_cap) T.p1[o1] = LeafPair{pair, mx.x & IDX_MASK, my.x & IDX_MASK};
+        else atomicOr(T.overflow, 2u);
+      }
+    }
+    {
+      const uint32_t o2 = wave_reserve(T.p2_fill, ss ? 1u : 0u);
+      if (ss) {
+        if (o2 < T.p2_cap) T.p2[o2] = LeafPair{pair, mx.x & IDX_MASK, my.x & IDX_MASK};
+        else atomicOr(T.overflow, 4u);
+      }
+    }
+    // the overlapping seed pair's children (quad-uniform push)
+    QuadRec<K> A0, A1, B0, B1;
+    QuadOut o;
+    expand_quad<K>(qbase, kid, push, A0, A1, B0, B1, o);
+    tests += o.test; descent += o.descent; broot += o.broot; p1s += o.leaf;
+    const uint32_t slot = block_reserve(out_fill, o.push ? 1u : 0u, red);
+    if (o.push) {
+      if (slot < out_cap) out[slot] = o.kid;
+      else atomicOr(T.overflow, 1u);
+    }
+    if (T.p1) {
+      const uint32_t o1 = wave_reserve(T.p1_fill, o.leaf ? 1u : 0u);
+      if (o.leaf) {
+        if (o1 < T.p1_cap) T.p1[o1] = LeafPair{pair, o.ta, o.tb};
+        else atomicOr(T.overflow, 2u);
+      }
+    }
+    const uint32_t o2 = wave_reserve(T.p2_fill, o.surf ? 1u : 0u);
+    if (o.surf) {
+      if (o2 < T.p2_cap) T.p2[o2] = LeafPair{pair, o.ta, o.tb};
+      else atomicOr(T.overflow, 4u);
     }
   }
   CounterStripe* st = stripe_of(T.stripes, wave_global());
