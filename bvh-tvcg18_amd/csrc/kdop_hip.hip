@@ -684,7 +684,7 @@ struct Seed {
   // expand: write level 1 (the seeds' expansion, into front[1]) instead of level 0
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t total, uint32_t* fill, uint32_t cap, bool expand) {
     const uint32_t blocks = std::max(1u, std::min(4096u, (total + 255) / 256));
-    if (expand && c->item_lanes == 4)  // four lanes per seed pair, like the ramp levels
+    if (expand && c->item_lanes == 4 && K >= 12)  // four lanes per seed pair, like the ramp levels
       hipLaunchKernelGGL(k_seed_quad<K>, dim3(std::max(1u, std::min(4096u, (total + 63) / 64))), dim3(256), 0, c->stream, T, c->dseeds.p,
                          total, c->front[1].p, fill, cap);
     else if (expand)
@@ -700,11 +700,10 @@ template <int K>
 struct Bfs {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t level, uint32_t target, uint32_t last, bool segmented) {
     const uint32_t nin = (segmented && level > 0) ? (uint32_t)kSegs : 1u, nout = segmented ? (uint32_t)kSegs : 1u;
-    // four lanes per item (like the persistent rounds) for the persistent
-    // strategy's narrow ramp levels and for tight DOPs; loose DOPs (K <= 8)
-    // over the wide levels of the level-synchronous strategy run faster with
-    // one lane per item (C2 query: K=8 0.241 vs 0.251 ms, K=26 0.311 vs 0.260)
-    if (c->item_lanes == 4 && (!segmented || K >= 12))
+    // four lanes per item (like the persistent rounds) for tight DOPs; loose
+    // DOPs (K <= 8), whose levels are wide, run faster with one lane per item
+    // (C2 level-synchronous: K=8 0.241 vs 0.251 ms, K=26 0.311 vs 0.260)
+    if (c->item_lanes == 4 && K >= 12)
       hipLaunchKernelGGL((k_bfs_level<K, true>), dim3((unsigned)(c->num_cus * 4)), dim3(256), 0, c->stream, T, c->front[0].p, c->front[1].p,
                          cap, c->dstate, level, target, last, nin, nout);
     else

@@ -371,10 +371,10 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   and tests whole records; 4: the 4 lanes of a quad each load */
                                   /*   a quarter of every record (coalesced 64-byte runs) and own   */
                                   /*   one child combination; the stack then holds                 */
-                                  /*   KDOP_OPT_QUEUE_DEPTH (256 or 512) items per wave.  With 4,  */
-                                  /*   the breadth-first levels take 4 lanes per item too (the    */
-                                  /*   persistent strategy's ramp, and every level of the level-  */
-                                  /*   synchronous strategy for K >= 12).  Results unchanged       */
+                                  /*   KDOP_OPT_QUEUE_DEPTH (256 or 512) items per wave.  With 4   */
+                                  /*   and K >= 12, the breadth-first levels of both strategies   */
+                                  /*   and the persistent strategy's seeds take 4 lanes per item   */
+                                  /*   too.  Results unchanged                                     */
 #define KDOP_OPT_ROUND_ITERS 18   /* quad kernel: once the frontier is gone, a workgroup that   */
                                   /*   has run this many iterations in a round hands its whole  */
                                   /*   stack to the next round (spread there over every         */
