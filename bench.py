@@ -249,6 +249,53 @@ def batched_leg(args):
                 _prox_in=prox_in)
 
 
+def series_leg(args):
+    """Refit + re-traverse time series (64 overlapping pairs of 20k-80k tets,
+    K=18, 200 steps of seeded rigid motion): every step refits all bodies to
+    that step's poses (device arrays) and traverses all pairs with the narrow
+    phase, the loop a simulator runs.  The trees are built once and refitted,
+    never rebuilt.  Reports the time per step over all 200 steps after one
+    untimed pass of the first 10."""
+    import numpy as np
+    import torch
+    import kdop_amd as KA
+    from kdop_amd import workloads as W
+    wl, P, Q = W.refit_series(npairs=64, K=18, steps=200)
+    ctx = KA.Context(K=wl.K)
+    ids = [ctx.make_tree(m) for m in wl.meshes]
+    pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
+    Pd = torch.from_numpy(np.ascontiguousarray(P)).to("cuda")
+    Qd = torch.from_numpy(np.ascontiguousarray(Q)).to("cuda")
+    torch.cuda.synchronize()
+    ctx.set_option("async_refit", 1)
+
+    def step(s):
+        ctx.refit_device(ids, Pd[s].data_ptr(), Qd[s].data_ptr())
+        return ctx.tandem_traversal(pairs, narrow=True)
+
+    for s in range(10):
+        step(s)
+    torch.cuda.synchronize()
+    tests = contacts = surface = 0
+    t0 = time.perf_counter()
+    for s in range(P.shape[0]):
+        c = step(s)
+        tests += c["tests"]
+        contacts += c["contacts"]
+        surface += c["surface_pairs"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nsteps = P.shape[0]
+    out = dict(workload=wl.name, pairs=len(wl.pairs), bodies=len(wl.meshes), K=wl.K,
+               tets_total=int(sum(m.nt for m in wl.meshes)), steps=nsteps, ms_per_step=dt * 1e3 / nsteps,
+               pair_queries_per_s=len(wl.pairs) * nsteps / dt, device_tests_per_s=tests / dt,
+               tests_total=int(tests), surface_pairs_total=int(surface), contacts_total=int(contacts),
+               note="per step: refit of all 128 bodies to that step's poses (device arrays) + tandem traversal + narrow "
+                    "phase of all 64 pairs; trees built once, refitted every step")
+    ctx.close()
+    return out
+
+
 def adjoint_leg(args):
     """The differentiable narrow phase on its configuration: 64 overlapping
     pairs of ~53k-tet jittered boxes, K=14.  Times the traversal that fixes
@@ -566,6 +613,7 @@ def main():
     ap.add_argument("--rays", type=int, default=65536, help="rays of the raycast leg (0: skip)")
     ap.add_argument("--spheres", type=int, default=65536, help="spheres of the single-traversal leg (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable narrow phase leg")
+    ap.add_argument("--no-series", action="store_true", help="skip the refit + re-traverse time series leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB megabytes (reference for rocprof byte counters)")
     args = ap.parse_args()
@@ -603,6 +651,11 @@ def main():
     if args.spheres:
         head["single"] = single_leg(args, head)
         log(f"[bench] single traversal {head['single']['spheres']} spheres: {head['single']['ms']:.2f} ms")
+    series = None
+    if not args.no_series:
+        series = series_leg(args)
+        log(f"[bench] refit series (64 pairs, K=18, 200 steps): {series['ms_per_step']:.3f} ms per step, "
+            f"{series['pair_queries_per_s']:.0f} pair-queries/s")
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(args)
@@ -696,6 +749,7 @@ def main():
         "mesh_reorder": reorder,
         "raycast": head.get("raycast"),
         "single_traversal": head.get("single"),
+        "refit_series": series,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
