@@ -715,14 +715,17 @@ struct Bfs {
 // dispatch packet (the events time exactly this kernel, with no extra packets
 // on the stream).
 // Persistent rounds queued before the first host sync: as many as the
-// context's recent calls needed (one at first; at most kQueuedRounds; raised
-// at once, lowered after 8 calls in a row needed fewer), so that a query whose
-// round 0 spills nothing launches no empty rounds.  A call that needs more
-// drains them one by one after a host sync.  Rounds after them drain
-// what is left and hand no stack tail on (only a full stack spills), so a
-// small stack keep cannot push work from round to round until the rounds run
-// out.
-constexpr uint32_t kQueuedRounds = 3;
+// context's recent calls needed (one at first; up to every round a call may
+// run, kMaxRounds - 1; raised at once, lowered after 8 calls in a row needed
+// fewer), so that a query whose round 0 spills nothing launches no empty
+// rounds, and a batch whose round budget spreads heavy pairs over several
+// rounds makes no host round trip between them (512-pair batch: 7 rounds,
+// each drained round cost a summary and a host sync, 25-40 us).  A call that
+// needs more than were queued drains them one by one after a host sync.
+// Rounds after the queued ones hand no stack tail on (only a full stack
+// spills), so a small stack keep cannot push work from round to round until
+// the rounds run out.
+constexpr uint32_t kQueuedRounds = kMaxRounds - 1;
 inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOffKeep; }
 
 // A persistent round.  With kernel timing on, start / stop events ride on its
