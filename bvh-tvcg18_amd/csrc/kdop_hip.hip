@@ -1093,7 +1093,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{256, 512, 1024};
       // no hand-off, or a keep that hands on the bottom of deep stacks
       case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
-      case KDOP_OPT_ROUND_ITERS: return {0, 16, 64};
+      case KDOP_OPT_ROUND_ITERS: return {0, 16, 32, 64};
       case KDOP_OPT_NODE_SLAB: return quad ? std::vector<int64_t>{0, 1} : std::vector<int64_t>{0};
       case KDOP_OPT_RAMP_TARGET: return {4 * pw, 16 * pw, 64 * pw};
       default: return {16384, 65536, 262144};  // KDOP_OPT_SEED_BUDGET
