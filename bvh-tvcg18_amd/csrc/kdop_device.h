@@ -53,6 +53,7 @@ struct BodyDev {
   float* X;
   float* Y;
   float* Z;
+  float4* W;  // the same world vertices as (x, y, z, 0): one 16-byte load per vertex for the narrow phase and the ray/sphere tests
 };
 
 // Topology parameters of one body (kernel argument).

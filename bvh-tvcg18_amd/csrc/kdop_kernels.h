@@ -401,6 +401,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     }
     const f3 r = posed_vertex(B, v, q, p, identity != 0);
     B.X[v] = r.x; B.Y[v] = r.y; B.Z[v] = r.z;
+    B.W[v] = make_float4(r.x, r.y, r.z, 0.0f);
   }
   if (g >= tpre[nbatch]) return;  // whole waves (tpre[nbatch] is a multiple of 64)
   float* __restrict__ st = stage[threadIdx.x >> 6];
@@ -1510,11 +1511,16 @@ struct NarrowArgs {
   const uint32_t* done;
 };
 
+// World vertex v of B (the refit's (x, y, z, 0) copy: one 16-byte load).
+__device__ inline f3 world_vertex(const BodyDev& B, uint32_t v) {
+  const float4 w = B.W[v];
+  return f3{w.x, w.y, w.z};
+}
 __device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
   const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)t);
   const uint32_t v[4] = {tv.x, tv.y, tv.z, tv.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) P[q] = f3{B.X[v[q]], B.Y[v[q]], B.Z[v[q]]};
+  for (int q = 0; q < 4; ++q) P[q] = world_vertex(B, v[q]);
 }
 
 // SAT gate (overlap/geometry_overlap_tetrahedron_tetrahedron.h:50-290) and
@@ -1536,7 +1542,7 @@ __device__ inline int gate_group(const NarrowArgs& A, f3* V, UPlane* U, uint32_t
     sb = BB.surf[lp.tb] & 15u;
     const BodyDev& Bd = r < 4u ? BA : BB;
     const uint32_t vi = Bd.tets[4 * (size_t)(r < 4u ? lp.ta : lp.tb) + (r & 3u)];
-    V[r] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
+    V[r] = world_vertex(Bd, vi);
   }
   wave_lds_sync();
   if (live) {
@@ -1660,7 +1666,7 @@ __device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, u
       const BodyDev& Bd = A.bodies[lane < 4u ? bp.x : bp.y];
       const uint32_t t = lane < 4u ? lp.ta : lp.tb;
       const uint32_t vi = Bd.tets[4 * (size_t)t + (lane & 3u)];
-      L.v[lane] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
+      L.v[lane] = world_vertex(Bd, vi);
     }
   }
   wave_lds_sync();
@@ -1781,7 +1787,7 @@ __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
       if (lane < 8u) {
         const BodyDev& Bd = lane < 4u ? BA : BB;
         const uint32_t vi = Bd.tets[4 * (size_t)(lane < 4u ? lp.ta : lp.tb) + (lane & 3u)];
-        L.v[lane] = f3{Bd.X[vi], Bd.Y[vi], Bd.Z[vi]};
+        L.v[lane] = world_vertex(Bd, vi);
       }
     }
     wave_lds_sync();

@@ -158,7 +158,7 @@ __device__ inline float chunk_raycast(const float* __restrict__ nodes, const Bod
       const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
       f3 P[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
+      for (int v = 0; v < 4; ++v) P[v] = world_vertex(B, vi[v]);
       f3 q;
       float len;
       if (raycast_tet(o, r, P, B.surf[c0], q, len)) {

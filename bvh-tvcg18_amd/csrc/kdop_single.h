@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) void k_sg_leaves(const BodyDev* __restrict__ b
   const uint32_t vi[4] = {tv.x, tv.y, tv.z, tv.w};
   f3 P[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) P[v] = f3{B.X[vi[v]], B.Y[vi[v]], B.Z[vi[v]]};
+  for (int v = 0; v < 4; ++v) P[v] = world_vertex(B, vi[v]);
   f3 p, nn;
   float depth;
   const bool hit = sphere_tet_contact(c, r, P, B.surf[t], flip != 0, p, nn, depth);
