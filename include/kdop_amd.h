@@ -48,7 +48,8 @@ typedef struct kdop_config {
   int32_t waves_per_cu;  /* persistent kernel: resident waves per CU (0 = autotune)            */
   int32_t ramp_target;   /* persistent kernel: frontier size that ends the breadth-first ramp   */
   int32_t stack_keep;    /* persistent kernel: stack entries a wave keeps once the frontier is  */
-                         /*   exhausted; deeper stacks go to the next round (0 = default 512)  */
+                         /*   exhausted; deeper stacks go to the next round (0 = default: no  */
+                         /*   hand-off, KDOP_OPT_STACK_KEEP)                                  */
   int32_t seed_budget;   /* chunk-root pairs tested to seed one call (0 = default 65536)        */
   int32_t reserved[2];
 } kdop_config;
