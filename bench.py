@@ -249,6 +249,61 @@ def batched_leg(args):
                 _prox_in=prox_in)
 
 
+def c1_leg(args):
+    """The reference's own CPU-runnable case: two 10^3-cube boxes (5,000 tets
+    each, K=8), B rotated by a seeded quaternion and moved by (0.05, 0.05,
+    0.05).  Device: refit of both bodies + traversal + narrow phase per query,
+    poses in device memory.  CPU: the oracle's sequential query (every branch
+    row, one thread; the transform and refit are not timed on the CPU side)."""
+    import numpy as np
+    import torch
+    import kdop_amd as KA
+    from kdop_amd import workloads as W
+    wl = W.single_pair(n=10, K=8, seed=1)
+    ctx = KA.Context(K=wl.K)
+    ids, pose_p, pose_q, pairs = build_pair(ctx, wl)
+    dp, dq = device_poses(pose_p, pose_q)
+    ctx.set_option("async_refit", 1)
+
+    def step():
+        ctx.refit_device(ids, dp.data_ptr(), dq.data_ptr())
+        return ctx.tandem_traversal(pairs, narrow=True)
+
+    for _ in range(20):
+        step()
+    torch.cuda.synchronize()
+    n = 200
+    t0 = time.perf_counter()
+    for _ in range(n):
+        c = step()
+    torch.cuda.synchronize()
+    dev_ms = (time.perf_counter() - t0) * 1e3 / n
+    out = dict(workload=wl.name, tets_per_body=int(wl.meshes[0].nt), K=wl.K, device_ms_per_query=dev_ms,
+               reference_tests=int(c["reference_tests"]), leaf_pairs=int(c["leaf_pairs"]), contacts=int(c["contacts"]))
+    ctx.close()
+    if not args.no_cpu:
+        from oracle import oracle as O
+        trees, meshes = [], []
+        for m, pose in zip(wl.meshes, wl.poses):
+            X, Y, Z = O.transform_f32(m.X, m.Y, m.Z, pose.p, pose.q)
+            t = O.Tree(wl.K, 8000, m.tets, m.X, m.Y, m.Z)
+            t.refit(X, Y, Z)
+            trees.append(t)
+            meshes.append((m.tets, X, Y, Z, m.surface))
+        rows = trees[0].levels()[-1]
+        best = None
+        for _ in range(3):
+            co, so = O.traverse_rows(trees[0], meshes[0], trees[1], meshes[1], 0, rows)
+            best = so if best is None else min(best, so)
+        out["cpu_baseline"] = dict(ms_per_query=best * 1e3, cores=1, kind="port",
+                                   sample=f"oracle sequential tandem traversal + narrow phase, whole query ({co['tests']} tests), best of 3")
+        # the row traversal makes every branch-row test; the reference adds the one root test
+        out["same_counts"] = (co["tests"] + 1 == c["reference_tests"] and co["leaf_pairs"] == c["leaf_pairs"]
+                              and co["contacts"] == c["contacts"])
+        out["vs_cpu"] = best * 1e3 / dev_ms
+    return out
+
+
 def series_leg(args):
     """Refit + re-traverse time series (64 overlapping pairs of 20k-80k tets,
     K=18, 200 steps of seeded rigid motion): every step refits all bodies to
@@ -651,6 +706,9 @@ def main():
     if args.spheres:
         head["single"] = single_leg(args, head)
         log(f"[bench] single traversal {head['single']['spheres']} spheres: {head['single']['ms']:.2f} ms")
+    c1 = c1_leg(args)
+    log(f"[bench] C1 (5k-tet pair, K=8): {c1['device_ms_per_query']:.3f} ms per query on the device"
+        + (f", {c1['cpu_baseline']['ms_per_query']:.2f} ms on one core" if c1.get('cpu_baseline') else ""))
     series = None
     if not args.no_series:
         series = series_leg(args)
@@ -749,6 +807,7 @@ def main():
         "mesh_reorder": reorder,
         "raycast": head.get("raycast"),
         "single_traversal": head.get("single"),
+        "c1_reference_case": c1,
         "refit_series": series,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
