@@ -1515,6 +1515,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   uint32_t levels = 0, level_guess = 0;  // BVTT depth bound; likely depth (deeper tree + 4)
   uint64_t est = 0;
   std::vector<uint2> pairs(npairs);
+  uint64_t call_tets = 0;  // tets of both bodies summed over the call's pairs
   for (uint32_t i = 0; i < npairs; ++i) {
     const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
     if (a >= c->bodies.size() || b >= c->bodies.size()) return fail(c, KDOP_ERR_ARG, "kdop_traverse: unknown body in pair %u", i);
@@ -1522,6 +1523,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     levels = std::max(levels, c->bodies[a].depth + c->bodies[b].depth);
     level_guess = std::max(level_guess, std::max(c->bodies[a].depth, c->bodies[b].depth) + 4);
     est += 4ull * (c->bodies[a].nt + c->bodies[b].nt);
+    call_tets += (uint64_t)c->bodies[a].nt + c->bodies[b].nt;
   }
   if (levels > 62) return fail(c, KDOP_ERR_ARG, "kdop_traverse: trees too deep (%u levels)", levels);
   c->have_traversal = c->have_p1 = c->have_narrow = false;
@@ -1570,12 +1572,16 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
   // AUTO: the persistent strategy where its depth-first rounds beat one
-  // launch per level -- tight DOPs (K >= 12), and batches of many pairs at
-  // any K, whose heavy pairs the round budget spreads over later rounds; the
-  // level-synchronous one for a few pairs of loose DOPs (K = 6, 8), whose
-  // wide frontiers fill the GPU level by level
+  // launch per level -- tight DOPs (K >= 12), batches of many pairs at any K,
+  // whose heavy pairs the round budget spreads over later rounds, and calls
+  // of fewer than kAutoLevelsTets tets, whose BVTT is too narrow to fill the
+  // GPU level by level (one 5k-tet pair at K = 8: 0.103 against 0.122 ms per
+  // query, 40k tets: 0.164 against 0.199); the level-synchronous one for a
+  // few large pairs of loose DOPs (K = 6, 8), whose wide frontiers fill the
+  // GPU level by level (two 320k-tet bodies: 0.239-0.248 against 0.250-0.253)
+  constexpr uint64_t kAutoLevelsTets = 400000;
   const bool persistent = c->cfg.traversal == KDOP_TRAVERSAL_PERSISTENT ||
-                          (c->cfg.traversal == KDOP_TRAVERSAL_AUTO && (c->K >= 12 || npairs >= 64));
+                          (c->cfg.traversal == KDOP_TRAVERSAL_AUTO && (c->K >= 12 || npairs >= 64 || call_tets < kAutoLevelsTets));
   est = std::max<uint64_t>(est, 1 << 16);
   if (keep_p1 && (rc = ensure(c, c->p1, est))) return rc;
   if ((rc = ensure(c, c->p2, std::max<uint64_t>(est / 16, 1 << 14)))) return rc;

@@ -33,8 +33,9 @@ extern "C" {
 typedef struct kdop_ctx kdop_ctx;
 
 /* Traversal strategies (kdop_config.traversal). */
-#define KDOP_TRAVERSAL_AUTO 0        /* persistent for K >= 12 or >= 64 body pairs,        */
-                                     /* level-synchronous otherwise (kdop_autotune refines) */
+#define KDOP_TRAVERSAL_AUTO 0        /* persistent for K >= 12, >= 64 body pairs or fewer  */
+                                     /* than 400k tets in the call; level-synchronous      */
+                                     /* otherwise (kdop_autotune refines)                  */
 #define KDOP_TRAVERSAL_LEVELS 1      /* level-synchronous frontier kernels                 */
 #define KDOP_TRAVERSAL_PERSISTENT 2  /* persistent rounds: LDS stacks per workgroup, guided */
                                      /* frontier chunks, budgeted rounds re-spreading work  */
