@@ -150,7 +150,7 @@ struct kdop_ctx {
   int node_slab = 0;         // KDOP_OPT_NODE_SLAB: 1 stages each wave's records in an LDS node slab (k_persist_quad SLAB)
   uint32_t round_iters = kRoundItersDefault;  // KDOP_OPT_ROUND_ITERS: a block's iteration budget per round once the frontier is gone (0: none)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
-  uint32_t bfs_max = 6;      // most breadth-first levels before the persistent rounds
+  uint32_t bfs_max = 0;      // most breadth-first levels before the persistent rounds (0: by the call's size, bfs_auto)
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
   uint32_t rounds_hint = 1;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
   uint32_t rounds_calm = 0;    // consecutive calls that needed fewer rounds than rounds_hint
@@ -727,6 +727,14 @@ struct Bfs {
 // spills), so a small stack keep cannot push work from round to round until
 // the rounds run out.
 constexpr uint32_t kQueuedRounds = kMaxRounds - 1;
+// Breadth-first levels before the persistent rounds when the caller set none:
+// a call whose BVTT stays narrow gains little balance from a wider frontier
+// and pays a launch per level.  Per query (refit, traversal, narrow phase),
+// 1 / 2 / 4 / 6 levels: 5k-tet pair K=8 0.099 / 0.104 / 0.117 / 0.104 ms,
+// 1k-tet pair 0.080 / 0.087 / 0.106 / 0.094, 40k-tet pair K=8 0.189 / 0.190 /
+// 0.162 / 0.164, K=26 0.160 / 0.159 / 0.163 / 0.177; two 320k-tet bodies at
+// K=26 take the most (6).  call_tets: tets of both bodies over the call's pairs.
+inline uint32_t bfs_auto(uint64_t call_tets) { return call_tets < 32000 ? 1u : (call_tets < 400000 ? 4u : 6u); }
 inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOffKeep; }
 
 // A persistent round.  With kernel timing on, start / stop events ride on its
@@ -1010,9 +1018,8 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->async_refit = v != 0;
       return KDOP_OK;
     case KDOP_OPT_BFS_MAX:
-      if (v == 0) v = 6;
-      if (v < 1 || v > 62) break;
-      c->bfs_max = (uint32_t)v;
+      if (v < 0 || v > 62) break;
+      c->bfs_max = (uint32_t)v;  // 0: by the call's size
       return KDOP_OK;
     case KDOP_OPT_BLOCK_WAVES:
       if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) break;
@@ -1618,7 +1625,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
     uint32_t rounds = 0, levels_queued = 0;
-    const uint32_t last = std::min(levels, c->bfs_max);
+    const uint32_t last = std::min(levels, c->bfs_max ? c->bfs_max : bfs_auto(call_tets));
     if (npairs) {
       if (persistent) rounds = rounds_async;
       else levels_queued = std::min(levels, level_guess);
