@@ -50,6 +50,7 @@ struct BodyDev {
   const float* X0;
   const float* Y0;
   const float* Z0;
+  const float4* W0;  // the same rest positions as (x, y, z, 0): one 16-byte load per vertex for the refit
   float* X;
   float* Y;
   float* Z;

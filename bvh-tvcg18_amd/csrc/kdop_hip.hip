@@ -47,6 +47,7 @@ struct BodyHost {
   std::vector<uint32_t> roots_off;  // per level: offset of the level's chunk roots in the context's root table
   float *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr, *X = nullptr, *Y = nullptr, *Z = nullptr;
   float4* W = nullptr;  // world vertices as (x, y, z, 0)
+  float4* W0 = nullptr;  // rest positions as (x, y, z, 0)
   uint32_t* tets = nullptr;
   uint8_t* surf = nullptr;
   // chunks wider than 32 leaves (topology built on the host): each tet's leaf
@@ -378,7 +379,7 @@ int upload_bodies(kdop_ctx* c) {
     d.branch_base = b.node_base + b.level_base[b.H - 1];
     d.super_base = b.H >= 2 ? b.node_base + b.level_base[b.H - 2] : 0u;
     d.depth = b.depth;
-    d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.X = b.X; d.Y = b.Y; d.Z = b.Z; d.W = b.W;
+    d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.W0 = b.W0; d.X = b.X; d.Y = b.Y; d.Z = b.Z; d.W = b.W;
   }
   int rc = ensure(c, c->dbodies, h.size());
   if (!rc) rc = ensure(c, c->dposes, 8 * h.size(), true);
@@ -946,7 +947,7 @@ void kdop_destroy(kdop_ctx* c) {
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& b : c->bodies) {
-    (void)hipFree(b.X0); (void)hipFree(b.Y0); (void)hipFree(b.Z0);
+    (void)hipFree(b.X0); (void)hipFree(b.Y0); (void)hipFree(b.Z0); (void)hipFree(b.W0);
     (void)hipFree(b.X); (void)hipFree(b.Y); (void)hipFree(b.Z); (void)hipFree(b.W);
     (void)hipFree(b.tets); (void)hipFree(b.surf);
     if (b.leaf_slot) (void)hipFree(b.leaf_slot);
@@ -1242,7 +1243,7 @@ int kdop_add_body_ex(kdop_ctx* c, uint64_t mem_bytes, uint32_t nv, const float* 
   };
   if ((rc = alloc((void**)&b.X0, nv * 4)) || (rc = alloc((void**)&b.Y0, nv * 4)) || (rc = alloc((void**)&b.Z0, nv * 4)) ||
       (rc = alloc((void**)&b.X, nv * 4)) || (rc = alloc((void**)&b.Y, nv * 4)) || (rc = alloc((void**)&b.Z, nv * 4)) ||
-      (rc = alloc((void**)&b.W, nv * (size_t)16)) ||
+      (rc = alloc((void**)&b.W, nv * (size_t)16)) || (rc = alloc((void**)&b.W0, nv * (size_t)16)) ||
       (rc = alloc((void**)&b.tets, 16 * (size_t)nt)) || (rc = alloc((void**)&b.surf, nt))) {
     c->tops.resize(b.top_base);
     return rc;
@@ -1250,6 +1251,8 @@ int kdop_add_body_ex(kdop_ctx* c, uint64_t mem_bytes, uint32_t nv, const float* 
   HIPCHK(c, hipMemcpyAsync(b.X0, X0, nv * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(b.Y0, Y0, nv * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(b.Z0, Z0, nv * 4, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_interleave3, dim3((nv + 255) / 256), dim3(256), 0, c->stream, (const float*)b.X0, (const float*)b.Y0,
+                     (const float*)b.Z0, nv, b.W0);
   HIPCHK(c, hipMemcpyAsync(b.tets, tets, 16 * (size_t)nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(b.surf, surface, nt, hipMemcpyHostToDevice, c->stream));
   const uint32_t id = (uint32_t)c->bodies.size();

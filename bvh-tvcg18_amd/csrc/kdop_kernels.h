@@ -203,10 +203,18 @@ __global__ void k_build_topology(TopoParams P, float* __restrict__ nodes, int K,
 // refitted computes it through this function, so all of them get the bits
 // that the refit stores.
 __device__ inline f3 posed_vertex(const BodyDev& B, uint32_t v, const float* q, const float* p, bool identity) {
-  const f3 r0 = f3{B.X0[v], B.Y0[v], B.Z0[v]};
+  const float4 w0 = B.W0[v];
+  const f3 r0 = f3{w0.x, w0.y, w0.z};
   if (identity) return r0;
   const f3 r = rotate3(q[0], q[1], q[2], q[3], r0);
   return f3{r.x + p[0], r.y + p[1], r.z + p[2]};
+}
+
+// (x[i], y[i], z[i], 0) for i < n: the rest positions as one 16-byte word per vertex.
+__global__ void k_interleave3(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z, uint32_t n,
+                              float4* __restrict__ w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = make_float4(x[i], y[i], z[i], 0.0f);
 }
 
 // Locate the batch item owning global chunk g (cpre: prefix of chunk counts).
