@@ -187,8 +187,10 @@ namespace {
 // kernel the item-lane setting selects (quad kernel: 4-wave workgroups, 24
 // waves per CU, 256 stack entries per wave; one lane per item: 2-wave
 // workgroups, 8 waves per CU, 512 entries per wave).
-int ls_of(const kdop_ctx* c) { return c->ls ? c->ls : (c->item_lanes == 4 ? 256 : 512); }
-int wpc_of(const kdop_ctx* c) { return c->wpc ? c->wpc : (c->item_lanes == 4 ? 24 : 8); }
+// quad rounds: 512-item stacks shared by 4 waves (16.4 KB of LDS, 70 VGPRs at K = 26), 7 workgroups per CU
+// (C2: round 0 68.8 -> 65.3 us against 1,024-item stacks at 6 per CU)
+int ls_of(const kdop_ctx* c) { return c->ls ? c->ls : (c->item_lanes == 4 ? 128 : 512); }
+int wpc_of(const kdop_ctx* c) { return c->wpc ? c->wpc : (c->item_lanes == 4 ? 28 : 8); }
 int bw_of(const kdop_ctx* c) { return c->bw ? c->bw : (c->item_lanes == 4 ? 4 : 2); }
 
 int fail(kdop_ctx* c, int code, const char* fmt, ...) {
@@ -794,7 +796,7 @@ struct PersistRound {
   static void run(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t round, uint32_t last) {
     const int bw = bw_of(c), ls = ls_of(c);
     if (c->count_records) {  // record-counting calls: the quad kernel in its default shape
-      launch_round_quad<K, 4, 10, true>(c, T, cap, round, last);
+      launch_round_quad<K, 4, 9, true>(c, T, cap, round, last);
       return;
     }
     if (c->item_lanes == 4 && c->node_slab) {  // records staged in per-wave LDS node slabs
@@ -810,7 +812,11 @@ struct PersistRound {
     if (c->item_lanes == 4) {  // stack of KDOP_OPT_QUEUE_DEPTH (256 or 512) entries per wave
       const bool deep = ls >= 512;
       if (bw <= 2) { if (deep) launch_round_quad<K, 2, 10>(c, T, cap, round, last); else launch_round_quad<K, 2, 9>(c, T, cap, round, last); }
-      else if (bw == 4) { if (deep) launch_round_quad<K, 4, 11>(c, T, cap, round, last); else launch_round_quad<K, 4, 10>(c, T, cap, round, last); }
+      else if (bw == 4) {
+        if (deep) launch_round_quad<K, 4, 11>(c, T, cap, round, last);
+        else if (ls <= 128) launch_round_quad<K, 4, 9>(c, T, cap, round, last);
+        else launch_round_quad<K, 4, 10>(c, T, cap, round, last);
+      }
       else launch_round_quad<K, 8, 11>(c, T, cap, round, last);
       return;
     }
@@ -982,7 +988,7 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       c->cfg.traversal = (int32_t)v;
       return KDOP_OK;
     case KDOP_OPT_QUEUE_DEPTH:
-      if (v != 0 && v != 256 && v != 512 && v != 1024 && v != 2048) break;
+      if (v != 0 && v != 128 && v != 256 && v != 512 && v != 1024 && v != 2048) break;
       c->ls = (int)v;
       return KDOP_OK;
     case KDOP_OPT_WAVES_PER_CU:
@@ -1099,7 +1105,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       case KDOP_OPT_ITEM_LANES: return {4, 1};
       case KDOP_OPT_BLOCK_WAVES: return quad ? std::vector<int64_t>{2, 4, 8} : std::vector<int64_t>{1, 2, 4};
       case KDOP_OPT_WAVES_PER_CU: return quad ? std::vector<int64_t>{12, 16, 24} : std::vector<int64_t>{4, 8, 12};
-      case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{256, 512, 1024};
+      case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{128, 256, 512} : std::vector<int64_t>{256, 512, 1024};
       // no hand-off, or a keep that hands on the bottom of deep stacks
       case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
       case KDOP_OPT_ROUND_ITERS: return {0, 16, 32, 64};

@@ -717,7 +717,7 @@ __global__ __launch_bounds__(64 * W) void k_persist_blk(TravArgs T, Item* __rest
 // 64 records x rec_stride(K) floats per wave (8 KB at K >= 18), which the LDS
 // holds for fewer workgroups per CU.
 template <int K, int W, int SL, bool COUNT, int IPQ, bool SLAB = false>
-__global__ __launch_bounds__(64 * W) void k_persist_quad(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(7))) void k_persist_quad(TravArgs T, Item* __restrict__ buf0, Item* __restrict__ buf1, uint32_t cap,
                                                          CallState* __restrict__ C, uint32_t round, uint32_t keep, uint32_t last,
                                                          uint32_t budget) {
   constexpr uint32_t NT = 64u * W;          // threads

@@ -337,9 +337,9 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
  * library default of a knob. */
 #define KDOP_OPT_TRAVERSAL 1      /* KDOP_TRAVERSAL_*                                        */
 #define KDOP_OPT_QUEUE_DEPTH 2    /* LDS stack entries per wave: 256, 512, 1024 or 2048 (the    */
-                                  /*   quad kernel: 256 or 512); 0: the kernel's default, 256    */
-                                  /*   (quad) or 512                                            */
-#define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32; 0: 24 (quad) or 8          */
+                                  /*   quad kernel: 128, 256 or 512); 0: the kernel's default,   */
+                                  /*   128 (quad) or 512                                        */
+#define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32; 0: 28 (quad) or 8          */
 #define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels        */
 #define KDOP_OPT_STACK_KEEP 5     /* persistent rounds: once the frontier is gone, a workgroup  */
                                   /*   whose stack holds more than this many entries per wave   */
