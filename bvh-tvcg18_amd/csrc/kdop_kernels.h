@@ -371,7 +371,19 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
 // The volumes are staged in LDS (the leaves, the branch chunks' internal
 // nodes, then the super chunks' nodes; each a contiguous run of records) and
 // written out with consecutive 16-byte words on consecutive lanes, so that a
-// store instruction covers whole lines instead of one word of 64 lines.
+// store instruction covers whole lines instead of one word of 64 lines.  The
+// branch chunks' records are written whole: the DOP, the two meta words and
+// the padding, so that every line is written in full.  HBM3E has no write
+// masks, and a line written in part costs a read-modify-write beyond the L2
+// (104 of 128 bytes per line: 30.8 us for 1.28M records against 25.2 us for
+// the whole lines).  The meta words of a full chunk follow from its heap
+// layout (k_build_topology wrote the same words): internal node i of a branch
+// chunk holds F_BRANCH | its first child (BFS 2i + 1) and its second child
+// (BFS 2i + 2, F_ROOT on node 0); leaf j holds F_LEAF | F_BRANCH | its tet and
+// its surface flag and body (F_ROOT when L = 1); node i of a super chunk
+// holds its children, BFS 2i + 1 and 2i + 2 when they are below L - 1, else
+// the root of branch chunk 2i + 2 - L (or 2i + 3 - L) of the super chunk.  The
+// padding keeps the 0xFF bytes the node pool is filled with at kdop_add_body.
 //
 // The same launch is the vertex transform of the batch: thread g also writes
 // world vertex g of the batch (vpre: prefix of vertex counts) to the body's
@@ -386,6 +398,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
                                                       const float* __restrict__ pos, const float* __restrict__ quat, int identity,
                                                       float* __restrict__ body_pose, float* __restrict__ nodes, DirTable dt) {
   constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
+  static_assert(K % 4 == 0 || K % 4 == 2, "the meta words start a 16-byte word or end one");
   __shared__ __align__(16) float stage[4][64 * RS];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < vpre[nbatch]) {
@@ -438,27 +451,49 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     for (int f = 0; f + 4 <= K; f += 4) *reinterpret_cast<float4*>(r + f) = make_float4(v[f], v[f + 1], v[f + 2], v[f + 3]);
     if (K % 4) *reinterpret_cast<float2*>(r + (K & ~3)) = make_float2(v[K - 2], v[K - 1]);
   };
-  // staged rows [0, nrows) -> records; dst(r) is row r's record or null
-  auto flush = [&](uint32_t nrows, auto&& dst) {
+  // staged rows [0, nrows) -> whole records: the DOP from the row, the meta
+  // words meta(r), the padding; dst(r) is row r's record or null
+  const float pad = __uint_as_float(0xFFFFFFFFu);
+  auto flush = [&](uint32_t nrows, auto&& dst, auto&& meta) {
     wave_lds_sync();
     for (uint32_t w = lane; w < nrows * W; w += 64u) {
       const uint32_t r = w / W, p = w % W;
+      const uint2 m = meta(r);  // on every lane (it may read other lanes' registers)
       float* d = dst(r);
-      if (!d || 4u * p >= (uint32_t)K) continue;
-      const float* src = st + r * RS + 4u * p;
-      if (4u * p + 4u <= (uint32_t)K) *reinterpret_cast<float4*>(d + 4u * p) = *reinterpret_cast<const float4*>(src);
-      else *reinterpret_cast<float2*>(d + 4u * p) = *reinterpret_cast<const float2*>(src);
+      if (!d) continue;
+      float4 x;
+      if (4u * p + 4u <= (uint32_t)K) {
+        x = *reinterpret_cast<const float4*>(st + r * RS + 4u * p);
+      } else if (4u * p >= (uint32_t)K + 2u) {
+        x = make_float4(pad, pad, pad, pad);
+      } else {  // the word that holds the meta words (after the DOP's last two floats when K % 4 == 2)
+        if (K % 4) {
+          const float2 y = *reinterpret_cast<const float2*>(st + r * RS + 4u * p);
+          x = make_float4(y.x, y.y, __uint_as_float(m.x), __uint_as_float(m.y));
+        } else {
+          x = make_float4(__uint_as_float(m.x), __uint_as_float(m.y), pad, pad);
+        }
+      }
+      *reinterpret_cast<float4*>(d + 4u * p) = x;
     }
     wave_lds_sync();
   };
+  // the second meta word of this lane's leaf (row `lane` of the leaves), loaded
+  // before the DOP so that the flush does not wait for it
+  const uint32_t leaf_m1 = act ? (L == 1u ? F_ROOT : 0u) | ((B.surf[tet] & 15u) ? F_SURF : 0u) | batch_body[item] : 0u;
   if (act) {
     tet_dop_at<K>(B, tet, dt, [&](uint32_t i) { return posed_vertex(B, i, iq, ip, identity != 0); }, v);
     stage_row(lane);
   }
-  flush(64u, [&](uint32_t r) -> float* {  // row r: leaf r & (L-1) of chunk r >> logL
-    if (tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
-    return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + L - 1u + (r & (L - 1u))) * S;
-  });
+  flush(
+      64u,
+      [&](uint32_t r) -> float* {  // row r: leaf r & (L-1) of chunk r >> logL
+        if (tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
+        return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + L - 1u + (r & (L - 1u))) * S;
+      },
+      [&](uint32_t r) {  // its tet; its surface flag and body (lane r's leaf_m1)
+        return make_uint2(F_LEAF | F_BRANCH | (tet0 + r), (uint32_t)__shfl((int)leaf_m1, (int)r, 64));
+      });
   for (uint32_t t = 1; t <= logL; ++t) {
     switch (t) {
       case 1: union_level<K, 1>(v, j); break;
@@ -469,11 +504,17 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     }
     if (act && (j & ((2u << (t - 1u)) - 1u)) == 0u) stage_row((q << logL) + (1u << (logL - t)) - 1u + (j >> t));  // BFS 2^d - 1 + i
   }
-  flush(64u, [&](uint32_t r) -> float* {  // row r: internal node r & (L-1) (< L-1) of chunk r >> logL
-    const uint32_t i = r & (L - 1u);
-    if (i == L - 1u || tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
-    return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + i) * S;
-  });
+  flush(
+      64u,
+      [&](uint32_t r) -> float* {  // row r: internal node r & (L-1) (< L-1) of chunk r >> logL
+        const uint32_t i = r & (L - 1u);
+        if (i == L - 1u || tet0 + (r & ~(L - 1u)) >= nfull) return nullptr;
+        return chunk_nodes + ((size_t)(r >> logL) * (2u * L) + i) * S;
+      },
+      [&](uint32_t r) {  // its children, BFS 2i + 1 and 2i + 2 of the chunk
+        const uint32_t i = r & (L - 1u), c0 = B.branch_base + ((tet0 >> logL) + (r >> logL)) * (2u * L) + 1u;
+        return make_uint2(F_BRANCH | (c0 + 2u * i + 1u), (i == 0u ? F_ROOT : 0u) | (c0 + 2u * i + 2u));
+      });
   if (!fuse || tet0 + (1u << logLL) > nfull) return;  // no full super chunk in this wave (sact is false on every lane)
   if (j == 0u) stage_row(q);  // chunk roots -> rows 0 .. 64/L - 1
   wave_lds_sync();
@@ -482,11 +523,18 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     case 2: super_butterfly<K, 2, RS>(st, lane, sact); break;
     default: super_butterfly<K, 3, RS>(st, lane, sact); break;
   }
-  flush(64u >> logL, [&](uint32_t r) -> float* {  // row r: node r & (L-1) (< L-1) of super chunk r >> logL
-    const uint32_t i = r & (L - 1u), ks = r >> logL;
-    if (i == L - 1u || tet0 + ((ks + 1u) << logLL) > nfull) return nullptr;
-    return nodes + ((size_t)B.super_base + (size_t)((tet0 >> logLL) + ks) * L + 1u + i) * S;
-  });
+  flush(
+      64u >> logL,
+      [&](uint32_t r) -> float* {  // row r: node r & (L-1) (< L-1) of super chunk r >> logL
+        const uint32_t i = r & (L - 1u), ks = r >> logL;
+        if (i == L - 1u || tet0 + ((ks + 1u) << logLL) > nfull) return nullptr;
+        return nodes + ((size_t)B.super_base + (size_t)((tet0 >> logLL) + ks) * L + 1u + i) * S;
+      },
+      [&](uint32_t r) {  // its children: BFS 2i + 1 and 2i + 2 below L - 1 are nodes of the super chunk, the others its branch chunks' roots
+        const uint32_t i = r & (L - 1u), sc = (tet0 >> logLL) + (r >> logL), s0 = B.super_base + sc * L + 1u;
+        auto child = [&](uint32_t h) { return h < L - 1u ? s0 + h : B.branch_base + (sc * L + h - (L - 1u)) * (2u * L) + 1u; };
+        return make_uint2(child(2u * i + 1u), child(2u * i + 2u));
+      });
 }
 
 // Branch chunks that are not full (the last chunk of a body when L does not
