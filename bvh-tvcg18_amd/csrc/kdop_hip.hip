@@ -69,12 +69,12 @@ struct DevBuf {
 // plan's offsets in the device copy): kept on the context and reused while
 // the list repeats, so a repeated refit makes its first launch at once.
 struct RefitStep {
-  size_t o_bb, o_gpre, o_cfirst, o_ncnk, o_cbase, o_cn;
+  size_t o_bb, o_gpre, o_gitem, o_cfirst, o_ncnk, o_cbase, o_cn;
   uint32_t m, blocks;
 };
 struct RefitLaunch {
   bool valid = false;
-  size_t o_bb = 0, o_vpre = 0, o_tpre = 0, o_pbody = 0, o_pchunk = 0, o_top = 0;
+  size_t o_bb = 0, o_vpre = 0, o_tpre = 0, o_vwave = 0, o_twave = 0, o_pbody = 0, o_pchunk = 0, o_top = 0;
   uint64_t total_v = 0, total_leaf_threads = 0;
   uint32_t npbody = 0;
   std::vector<RefitStep> steps;
@@ -397,10 +397,10 @@ int upload_bodies(kdop_ctx* c) {
 // ------------------------------------------------------------------- refit
 template <int K>
 struct RefitLeaves {
-  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, const uint32_t* vpre, uint32_t n,
-                  const float* pos, const float* quat, bool identity) {
-    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, vpre, n, pos,
-                       quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
+  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, const uint32_t* vpre, const uint32_t* twave,
+                  const uint32_t* vwave, uint32_t n, const float* pos, const float* quat, bool identity) {
+    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, vpre, twave, vwave,
+                       n, pos, quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
   }
 };
 // A body with chunks wider than 32 leaves: leaf DOPs at their slots, then
@@ -432,9 +432,9 @@ struct RefitBranch {
 };
 template <int K>
 struct RefitSuper {
-  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* gpre, const uint32_t* cfirst, const uint32_t* ncnk,
-                  const uint32_t* cbase, const uint32_t* cn, uint32_t n) {
-    hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, gpre, cfirst, ncnk, cbase, cn, n,
+  static void run(kdop_ctx* c, uint32_t nblocks, const uint32_t* bb, const uint32_t* gpre, const uint32_t* gitem, const uint32_t* cfirst,
+                  const uint32_t* ncnk, const uint32_t* cbase, const uint32_t* cn, uint32_t n) {
+    hipLaunchKernelGGL(k_refit_super<K>, dim3(nblocks), dim3(64), 0, c->stream, c->dbodies.p, bb, gpre, gitem, cfirst, ncnk, cbase, cn, n,
                        c->nodes.p);
   }
 };
@@ -527,10 +527,11 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   for (uint32_t i = 0; i < n; ++i) maxH = std::max(maxH, c->bodies[bodies[i]].H);
   const size_t o_bb = 0;
   for (uint32_t i = 0; i < n; ++i) plan.push_back(bodies[i]);
+  // vertices: one thread per vertex, each body's share padded to 64
   const size_t o_vpre = plan.size();
   uint64_t acc = 0;
   plan.push_back(0);
-  for (uint32_t i = 0; i < n; ++i) { acc += c->bodies[bodies[i]].nv; plan.push_back((uint32_t)acc); }
+  for (uint32_t i = 0; i < n; ++i) { acc += (c->bodies[bodies[i]].nv + 63ull) / 64 * 64; plan.push_back((uint32_t)acc); }
   const uint64_t total_v = acc;
   // full branch chunks: one thread per tet, each body's share padded to 64
   const size_t o_tpre = plan.size();
@@ -551,6 +552,11 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   const uint64_t total_leaf_threads = acc;
   if (total_leaf_threads >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many tetrahedra in one batch");
   if (total_v >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many vertices in one batch");
+  // per wave of each range, its batch item
+  const size_t o_vwave = plan.size();
+  for (uint32_t i = 0; i < n; ++i) plan.insert(plan.end(), (plan[o_vpre + i + 1] - plan[o_vpre + i]) / 64, i);
+  const size_t o_twave = plan.size();
+  for (uint32_t i = 0; i < n; ++i) plan.insert(plan.end(), (plan[o_tpre + i + 1] - plan[o_tpre + i]) / 64, i);
   const size_t o_pbody = plan.size();
   plan.insert(plan.end(), pbody.begin(), pbody.end());
   const size_t o_pchunk = plan.size();
@@ -614,6 +620,8 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
     st.blocks = tot;
     st.o_bb = plan.size(); plan.insert(plan.end(), bb.begin(), bb.end());
     st.o_gpre = plan.size(); plan.insert(plan.end(), gpre.begin(), gpre.end());
+    st.o_gitem = plan.size();  // per block, its item of the step
+    for (uint32_t j = 0; j < st.m; ++j) plan.insert(plan.end(), gpre[j + 1] - gpre[j], j);
     st.o_cfirst = plan.size(); plan.insert(plan.end(), cfirst.begin(), cfirst.end());
     st.o_ncnk = plan.size(); plan.insert(plan.end(), ncnk.begin(), ncnk.end());
     st.o_cbase = plan.size(); plan.insert(plan.end(), cbase.begin(), cbase.end());
@@ -643,7 +651,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(c->plan.p, plan.data(), plan.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   RefitLaunch& R = c->rl;
-  R.o_bb = o_bb; R.o_vpre = o_vpre; R.o_tpre = o_tpre; R.o_pbody = o_pbody; R.o_pchunk = o_pchunk; R.o_top = o_top;
+  R.o_bb = o_bb; R.o_vpre = o_vpre; R.o_tpre = o_tpre; R.o_vwave = o_vwave; R.o_twave = o_twave; R.o_pbody = o_pbody; R.o_pchunk = o_pchunk; R.o_top = o_top;
   R.total_v = total_v;
   R.total_leaf_threads = total_leaf_threads;
   R.npbody = (uint32_t)pbody.size();
@@ -670,10 +678,11 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   // leaves and the vertex transform in one launch, its grid over both ranges
   const uint64_t leaves_grid = std::max<uint64_t>(R.total_leaf_threads, R.total_v);
   if (leaves_grid)
-    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + R.o_bb, P + R.o_tpre, P + R.o_vpre, n, pos_dev, quat_dev, identity);
+    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + R.o_bb, P + R.o_tpre, P + R.o_vpre, P + R.o_twave, P + R.o_vwave, n,
+                            pos_dev, quat_dev, identity);
   if (R.npbody) dispatch_K<RefitBranch>(c->K, c, R.npbody, P + R.o_pbody, P + R.o_pchunk);
   for (const RefitStep& st : R.steps)
-    dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
+    dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_gitem, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
                            st.m);
   for (const uint32_t id : R.wide) dispatch_K<RefitWide>(c->K, c, id, c->bodies[id]);
   dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + R.o_top));

@@ -217,16 +217,6 @@ __global__ void k_interleave3(const float* __restrict__ x, const float* __restri
   if (i < n) w[i] = make_float4(x[i], y[i], z[i], 0.0f);
 }
 
-// Locate the batch item owning global chunk g (cpre: prefix of chunk counts).
-__device__ inline uint32_t find_item(const uint32_t* __restrict__ cpre, uint32_t n, uint32_t g) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t m = (lo + hi) >> 1;
-    if (cpre[m] <= g) lo = m; else hi = m;
-  }
-  return lo;
-}
-
 // Store the K volume floats of one record (16-byte stores; K is even, so the
 // tail is one 8-byte store and the meta words after it stay untouched).
 template <int K>
@@ -386,23 +376,31 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
 // padding keeps the 0xFF bytes the node pool is filled with at kdop_add_body.
 //
 // The same launch is the vertex transform of the batch: thread g also writes
-// world vertex g of the batch (vpre: prefix of vertex counts) to the body's
-// X, Y, Z, which later kernels read, and the first thread of each body
-// records its pose in body_pose (8 floats per body: p, q as x y z w, 0, for
-// the adjoint).  A leaf does not wait for those stores: it computes its four
-// vertices from the rest positions with the same arithmetic (posed_vertex).
+// world vertex g - vpre[item] of its item (vpre: prefix of the vertex counts,
+// each rounded up to 64) to the body's X, Y, Z and W, which later kernels
+// read, and the first thread of each body records its pose in body_pose (8
+// floats per body: p, q as x y z w, 0, for the adjoint).  A leaf does not
+// wait for those stores: it computes its four vertices from the rest
+// positions with the same arithmetic (posed_vertex).  Every item's vertex and
+// tet ranges are whole waves, so a wave finds its items in the per-wave
+// tables vwave and twave with one load at a wave-uniform index, and the
+// body's descriptor with scalar loads (a binary search over the prefix sums
+// per thread is a chain of log2(batch) dependent loads: the refit of 128
+// bodies took 512 us with it and takes 379 us with the tables).
 // pos / quat: 3 and 4 floats per batch item (unused with identity).
 template <int K>
 __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                      const uint32_t* __restrict__ tpre, const uint32_t* __restrict__ vpre, uint32_t nbatch,
+                                                      const uint32_t* __restrict__ tpre, const uint32_t* __restrict__ vpre,
+                                                      const uint32_t* __restrict__ twave, const uint32_t* __restrict__ vwave, uint32_t nbatch,
                                                       const float* __restrict__ pos, const float* __restrict__ quat, int identity,
                                                       float* __restrict__ body_pose, float* __restrict__ nodes, DirTable dt) {
   constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
   static_assert(K % 4 == 0 || K % 4 == 2, "the meta words start a 16-byte word or end one");
   __shared__ __align__(16) float stage[4][64 * RS];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < vpre[nbatch]) {
-    const uint32_t vi = find_item(vpre, nbatch, g);
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(g >> 6);  // this thread's wave of the batch
+  if (gw < (vpre[nbatch] >> 6)) {
+    const uint32_t vi = vwave[gw];
     const uint32_t bi = batch_body[vi];
     const BodyDev& B = bodies[bi];
     const uint32_t v = g - vpre[vi];
@@ -420,14 +418,16 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
       }
       bp[7] = 0.f;
     }
-    const f3 r = posed_vertex(B, v, q, p, identity != 0);
-    B.X[v] = r.x; B.Y[v] = r.y; B.Z[v] = r.z;
-    B.W[v] = make_float4(r.x, r.y, r.z, 0.0f);
+    if (v < B.nv) {
+      const f3 r = posed_vertex(B, v, q, p, identity != 0);
+      B.X[v] = r.x; B.Y[v] = r.y; B.Z[v] = r.z;
+      B.W[v] = make_float4(r.x, r.y, r.z, 0.0f);
+    }
   }
-  if (g >= tpre[nbatch]) return;  // whole waves (tpre[nbatch] is a multiple of 64)
+  if (gw >= (tpre[nbatch] >> 6)) return;  // whole waves (tpre[nbatch] is a multiple of 64)
   float* __restrict__ st = stage[threadIdx.x >> 6];
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t item = find_item(tpre, nbatch, g);
+  const uint32_t item = twave[gw];
   const BodyDev& B = bodies[batch_body[item]];
   const float* iq = quat + 4 * (size_t)item;
   const float* ip = pos + 3 * (size_t)item;
@@ -753,16 +753,17 @@ __device__ inline void partial_chunk(float* __restrict__ nodes, uint32_t base, u
 // Super chunks of level h (per body h = H-1-step), chunks [cfirst, C[h]) of
 // each batch item, one wave per chunk: full chunks transposed (chunk_t), the
 // level's last chunk, when it is not full, by the sweep.  gpre: prefix of
-// blocks per item.
+// blocks per item; gitem: each block's item.
 template <int K>
 __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
-                                                    const uint32_t* __restrict__ gpre, const uint32_t* __restrict__ cfirst,
+                                                    const uint32_t* __restrict__ gpre, const uint32_t* __restrict__ gitem,
+                                                    const uint32_t* __restrict__ cfirst,
                                                     const uint32_t* __restrict__ ncnk, const uint32_t* __restrict__ chunk_base,
                                                     const uint32_t* __restrict__ chunk_n, uint32_t nbatch, float* __restrict__ nodes) {
   __shared__ PartialLds<K> sh;
   const uint32_t blk = blockIdx.x;
   if (blk >= gpre[nbatch]) return;
-  const uint32_t item = find_item(gpre, nbatch, blk);
+  const uint32_t item = gitem[blk];
   const uint32_t k = blk - gpre[item];
   const BodyDev& B = bodies[batch_body[item]];
   const uint32_t L = B.L, logL = (uint32_t)(__ffs((int)L) - 1);
