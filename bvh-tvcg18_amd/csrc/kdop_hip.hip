@@ -533,22 +533,34 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   plan.push_back(0);
   for (uint32_t i = 0; i < n; ++i) { acc += (c->bodies[bodies[i]].nv + 63ull) / 64 * 64; plan.push_back((uint32_t)acc); }
   const uint64_t total_v = acc;
-  // full branch chunks: one thread per tet, each body's share padded to 64
+  // full branch chunks: one thread per tet, each body's range padded to 64;
+  // bodies with L = 16 first, their ranges padded to 256 so that each starts
+  // on a block (the leaves launch refits their full super chunks of level
+  // H-2 per block).  tstart: each item's first thread, then the total.
   const size_t o_tpre = plan.size();
+  plan.insert(plan.end(), (size_t)n + 1, 0u);
+  std::vector<uint32_t> order;  // items in the order of their ranges
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t i = 0; i < n; ++i) {
+      const BodyHost& b = c->bodies[bodies[i]];
+      if (!b.wide() && (b.L == 16u) == (pass == 0)) order.push_back(i);
+    }
   acc = 0;
-  plan.push_back(0);
   std::vector<uint32_t> pbody, pchunk;
   for (uint32_t i = 0; i < n; ++i) {
     const BodyHost& b = c->bodies[bodies[i]];
-    if (b.wide()) {  // refitted from its own lists below
-      plan.push_back((uint32_t)acc);
-      continue;
-    }
-    const uint64_t full = (uint64_t)(b.nt / b.L) * b.L;
-    acc += (full + 63) / 64 * 64;
-    plan.push_back((uint32_t)acc);
-    if (full < b.nt) { pbody.push_back(bodies[i]); pchunk.push_back(b.C[b.H - 1] - 1); }
+    if (!b.wide() && b.nt % b.L) { pbody.push_back(bodies[i]); pchunk.push_back(b.C[b.H - 1] - 1); }
   }
+  std::vector<uint32_t> twave;
+  for (const uint32_t i : order) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    const uint64_t full = (uint64_t)(b.nt / b.L) * b.L, gran = b.L == 16u ? 256u : 64u;
+    plan[o_tpre + i] = (uint32_t)acc;
+    const uint64_t len = (full + gran - 1) / gran * gran;
+    acc += len;
+    twave.insert(twave.end(), (size_t)(len / 64), i);
+  }
+  plan[o_tpre + n] = (uint32_t)acc;
   const uint64_t total_leaf_threads = acc;
   if (total_leaf_threads >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many tetrahedra in one batch");
   if (total_v >= (1ull << 32)) return fail(c, KDOP_ERR_ARG, "kdop_refit: too many vertices in one batch");
@@ -556,7 +568,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   const size_t o_vwave = plan.size();
   for (uint32_t i = 0; i < n; ++i) plan.insert(plan.end(), (plan[o_vpre + i + 1] - plan[o_vpre + i]) / 64, i);
   const size_t o_twave = plan.size();
-  for (uint32_t i = 0; i < n; ++i) plan.insert(plan.end(), (plan[o_tpre + i + 1] - plan[o_tpre + i]) / 64, i);
+  plan.insert(plan.end(), twave.begin(), twave.end());
   const size_t o_pbody = plan.size();
   plan.insert(plan.end(), pbody.begin(), pbody.end());
   const size_t o_pchunk = plan.size();
@@ -564,8 +576,8 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   // super level h of body b: {slot of its first chunk, chunks, elements of
   // the last chunk, first chunk to refit}; none when the leaves pass did it
   auto super_level = [&](const BodyHost& b, int h, uint4& lv) -> bool {
-    // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64)
-    const uint32_t cf = (h == (int)b.H - 2 && b.L * b.L <= 64) ? (b.nt / b.L) / b.L : 0u;
+    // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64: per wave, L = 16: per block)
+    const uint32_t cf = (h == (int)b.H - 2 && (b.L * b.L <= 64 || b.L == 16)) ? (b.nt / b.L) / b.L : 0u;
     const uint32_t nc = b.C[h], nl = host_chunk_elems(b, h, nc - 1);
     if (cf >= nc) return false;
     const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u);

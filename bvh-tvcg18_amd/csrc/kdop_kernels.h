@@ -341,6 +341,43 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
   }
 }
 
+// Level H-2 of one full super chunk of L = 16 branch chunks in one wave: the
+// 16 chunk roots are rows 0..15 of `roots` (row stride RS); lane (element
+// e = lane >> 2, component group cg = lane & 3) takes the components cg,
+// cg + 4, ... of element e, and the butterfly over the elements (lane xor 4,
+// 8, 16, 32) leaves the 15 nodes in rows 0..14 (BFS order) of `st`.  Even
+// components are minima, odd ones maxima (bmin / bmax: the operand order does
+// not matter).
+template <int K, uint32_t RS>
+__device__ inline void super16_butterfly(const float* __restrict__ roots, float* __restrict__ st, uint32_t lane) {
+  constexpr uint32_t NC = (K + 3) / 4;
+  const uint32_t e = lane >> 2, cg = lane & 3u;
+  const bool hi = (cg & 1u) != 0u;
+  float w[NC];
+#pragma unroll
+  for (uint32_t m = 0; m < NC; ++m) w[m] = (cg + 4u * m < (uint32_t)K) ? roots[e * RS + cg + 4u * m] : 0.0f;
+#pragma unroll
+  for (uint32_t t = 1; t <= 4; ++t) {
+#pragma unroll
+    for (uint32_t m = 0; m < NC; ++m) {
+      float y;
+      switch (t) {
+        case 1: y = lane_xor<4>(w[m]); break;
+        case 2: y = lane_xor<8>(w[m]); break;
+        case 3: y = lane_xor<16>(w[m]); break;
+        default: y = lane_xor<32>(w[m]); break;
+      }
+      w[m] = hi ? bmax(w[m], y) : bmin(w[m], y);
+    }
+    if ((e & ((1u << t) - 1u)) == 0u) {
+      float* row = st + ((1u << (4u - t)) - 1u + (e >> t)) * RS;
+#pragma unroll
+      for (uint32_t m = 0; m < NC; ++m)
+        if (cg + 4u * m < (uint32_t)K) row[cg + 4u * m] = w[m];
+    }
+  }
+}
+
 // Full branch chunks, one thread per tetrahedron: the leaf DOP, then the
 // chunk's internal nodes by a butterfly over its L lanes.  A full chunk's
 // median-split tree over L = 2^logL leaves is the implicit heap (BFS node
@@ -355,8 +392,11 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
 // When L * L <= 64 a wave also holds whole super chunks of level H-2 (L full
 // branch chunks each, whose chunk roots are the super chunk's elements; a
 // full super chunk is the same heap), and the butterfly goes on over them.
-// Super chunks that are not full, or that take a chunk that is not, are left
-// to k_refit_super.
+// When L = 16 a block's 256 tets are one super chunk of level H-2 (a body
+// with L = 16 has a tet range of its own that starts on a block), so after
+// one barrier wave 0 takes the block's 16 chunk roots from LDS and refits
+// that chunk (super16_butterfly).  Super chunks that are not full, or that
+// take a chunk that is not, are left to k_refit_super.
 //
 // The volumes are staged in LDS (the leaves, the branch chunks' internal
 // nodes, then the super chunks' nodes; each a contiguous run of records) and
@@ -397,6 +437,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
   constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
   static_assert(K % 4 == 0 || K % 4 == 2, "the meta words start a 16-byte word or end one");
   __shared__ __align__(16) float stage[4][64 * RS];
+  __shared__ __align__(16) float roots16[16 * RS];  // L = 16: the block's chunk roots
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t gw = __builtin_amdgcn_readfirstlane(g >> 6);  // this thread's wave of the batch
   if (gw < (vpre[nbatch] >> 6)) {
@@ -515,6 +556,27 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
         const uint32_t i = r & (L - 1u), c0 = B.branch_base + ((tet0 >> logL) + (r >> logL)) * (2u * L) + 1u;
         return make_uint2(F_BRANCH | (c0 + 2u * i + 1u), (i == 0u ? F_ROOT : 0u) | (c0 + 2u * i + 2u));
       });
+  if (logL == 4u && B.H >= 2u) {  // L = 16: the block (one body, 256-aligned) is one super chunk of level H-2
+    const uint32_t bw = threadIdx.x >> 6, btet0 = tet0 - 64u * bw;
+    if (btet0 + 256u > nfull) return;  // not full (the same on every wave of the block)
+    if (j == 0u) {  // every lane of a chunk holds its root; the first writes it
+      float* r = roots16 + (4u * bw + q) * RS;
+#pragma unroll
+      for (int f = 0; f < K; ++f) r[f] = v[f];
+    }
+    __syncthreads();
+    if (bw != 0u) return;
+    super16_butterfly<K, RS>(roots16, st, lane);
+    const uint32_t sc = btet0 >> 8;  // the body's super chunk of level H-2
+    flush(
+        15u, [&](uint32_t r) -> float* { return nodes + ((size_t)B.super_base + (size_t)sc * 16u + 1u + r) * S; },
+        [&](uint32_t r) {  // its children: BFS 2r + 1 and 2r + 2 below 15 are nodes of the super chunk, the others its branch chunks' roots
+          const uint32_t s0 = B.super_base + sc * 16u + 1u;
+          auto child = [&](uint32_t h) { return h < 15u ? s0 + h : B.branch_base + (sc * 16u + h - 15u) * 32u + 1u; };
+          return make_uint2(child(2u * r + 1u), child(2u * r + 2u));
+        });
+    return;
+  }
   if (!fuse || tet0 + (1u << logLL) > nfull) return;  // no full super chunk in this wave (sact is false on every lane)
   if (j == 0u) stage_row(q);  // chunk roots -> rows 0 .. 64/L - 1
   wave_lds_sync();
