@@ -237,6 +237,24 @@ __device__ inline void store_vol(float* __restrict__ nodes, uint32_t slot, const
 // select.
 __device__ inline float bmin(float a, float b) { return fminf(a, b); }
 __device__ inline float bmax(float a, float b) { return fmaxf(a, b); }
+// Union of two slab bounds in the refit butterflies: max(a, b) for an upper
+// bound, min(a, b) for a lower one.  A bound is never NaN (a leaf's starts
+// from +-FLT_MAX and bmin / bmax skip NaN projections) and never -0, so the
+// median of (a, b, +inf) is max(a, b) and of (a, b, -inf) is min(a, b), the
+// reference's std::max / std::min bits: one v_med3_f32.  fminf / fmaxf cost
+// a canonicalisation of every operand that comes from a lane move or a load,
+// and a lane whose kind is known only at run time would compute both and
+// select; here the third operand picks the kind.  When the kind is known at
+// compile time the infinity is made opaque (an empty asm statement that
+// holds it in an SGPR: no instruction), or the compiler folds the median
+// back into a min / max with canonicalised operands.
+__device__ inline float ubound(float a, float b, bool upper) { return __builtin_amdgcn_fmed3f(a, b, upper ? INFINITY : -INFINITY); }
+template <bool UPPER>
+__device__ inline float ubound(float a, float b) {
+  float lim = UPPER ? INFINITY : -INFINITY;
+  asm("" : "+s"(lim));
+  return __builtin_amdgcn_fmed3f(a, b, lim);
+}
 
 // Leaf DOP of one tetrahedron (geometry_dop.h:48-74): per slab direction the
 // min / max projection of its four world vertices (bmin / bmax, above).
@@ -289,13 +307,27 @@ __device__ inline float lane_xor(float v) {
 // children and both end with their union.  The union of bounds of leaf
 // projections does not depend on the operand order (bmin / bmax), so the
 // left child need not come first as in geometry_make_union.h:9-25.
+// A lane of the other half of this lane's aligned group of 2S lanes.  When
+// the lanes of each aligned group of S hold the same value (a butterfly after
+// its first levels), any such lane is as good a partner as lane ^ S, and for
+// S = 4 and 8 one DPP mirror reaches one where lane_xor takes two row shifts
+// and a select.
+template <int S>
+__device__ inline float lane_other(float v) {
+  const int x = __float_as_int(v);
+  if constexpr (S == 4) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false));  // row_half_mirror: i <- 7 - i in 8 lanes
+  if constexpr (S == 8) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false));  // row_mirror: i <- 15 - i in 16 lanes
+  return lane_xor<S>(v);
+}
+// One level of the chunk butterfly: level t (S = 2^(t-1)) leaves every aligned
+// group of 2S lanes with the union of its lanes' volumes.
 template <int K, int S>
 __device__ inline void union_level(float* v, uint32_t j) {
   (void)j;
 #pragma unroll
   for (int f = 0; f < K; ++f) {
-    const float y = lane_xor<S>(v[f]);
-    v[f] = (f & 1) ? bmax(v[f], y) : bmin(v[f], y);
+    const float y = lane_other<S>(v[f]);
+    v[f] = (f & 1) ? ubound<true>(v[f], y) : ubound<false>(v[f], y);
   }
 }
 
@@ -330,7 +362,7 @@ __device__ inline void super_butterfly(float* __restrict__ st, uint32_t lane, bo
         case 16: y = lane_xor<16>(w[m]); break;
         default: y = lane_xor<32>(w[m]); break;
       }
-      w[m] = hi ? bmax(w[m], y) : bmin(w[m], y);
+      w[m] = ubound(w[m], y, hi);
     }
     if (sact && (ei & ((2u << (t - 1u)) - 1u)) == 0u) {
       float* row = st + (sc * L + (1u << (LOGL - t)) - 1u + (ei >> t)) * RS;
@@ -367,7 +399,7 @@ __device__ inline void super16_butterfly(const float* __restrict__ roots, float*
         case 3: y = lane_xor<16>(w[m]); break;
         default: y = lane_xor<32>(w[m]); break;
       }
-      w[m] = hi ? bmax(w[m], y) : bmin(w[m], y);
+      w[m] = ubound(w[m], y, hi);
     }
     if ((e & ((1u << t) - 1u)) == 0u) {
       float* row = st + ((1u << (4u - t)) - 1u + (e >> t)) * RS;
@@ -700,7 +732,7 @@ __device__ inline void chunk_t_level(float* v, uint32_t e, uint32_t g, float* __
   for (uint32_t i = 0; i < C::F; ++i) {
     const float y = lane_xor<SX>(v[i]);
     const float a = left ? v[i] : y, b = left ? y : v[i];
-    v[i] = upper ? smax(a, b) : smin(a, b);
+    v[i] = ubound(a, b, upper);
   }
   if ((e & ((2u << (T - 1)) - 1u)) == 0u) {  // node (2^(LOGL-T) - 1) + e / 2^T of the heap
     float* r = nodes + (size_t)(base + (1u << (LOGL - T)) - 1u + (e >> T)) * rec_stride(K);
