@@ -757,8 +757,11 @@ constexpr uint32_t kQueuedRounds = kMaxRounds - 1;
 // 1 / 2 / 4 / 6 levels: 5k-tet pair K=8 0.099 / 0.104 / 0.117 / 0.104 ms,
 // 1k-tet pair 0.080 / 0.087 / 0.106 / 0.094, 40k-tet pair K=8 0.189 / 0.190 /
 // 0.162 / 0.164, K=26 0.160 / 0.159 / 0.163 / 0.177; two 320k-tet bodies at
-// K=26 take the most (6).  call_tets: tets of both bodies over the call's pairs.
-inline uint32_t bfs_auto(uint64_t call_tets) { return call_tets < 32000 ? 1u : (call_tets < 400000 ? 4u : 6u); }
+// K=26: 3 / 4 / 5 / 6 / 7 levels 0.189 / 0.189 / 0.189-0.191 / 0.193-0.195 /
+// 0.197 ms per step, and 64 pairs of 20k-80k tets at K=18 (a refit series
+// step) 4 / 5 / 6 levels 0.883 / 0.868 / 0.878 ms: 5.  call_tets: tets of
+// both bodies over the call's pairs.
+inline uint32_t bfs_auto(uint64_t call_tets) { return call_tets < 32000 ? 1u : (call_tets < 400000 ? 4u : 5u); }
 inline uint32_t round_keep(const kdop_ctx* c, uint32_t round) { return round < c->rounds_queued ? c->keep : kNoHandOffKeep; }
 
 // A persistent round.  With kernel timing on, start / stop events ride on its

@@ -351,7 +351,7 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   persistent round's dispatch; each event costs a few us   */
 #define KDOP_OPT_BFS_MAX 8        /* most breadth-first levels before the persistent rounds; */
                                   /*   0 (the default): by the call's size (1 below 32k tets  */
-                                  /*   over the call's pairs, 4 below 400k, else 6)           */
+                                  /*   over the call's pairs, 4 below 400k, else 5)           */
 #define KDOP_OPT_NARROW_PATH 10   /* narrow phase: 0 auto (by the last surface-pair count), */
                                   /*   1 one wave per pair, 2 SAT gate (8 lanes per pair) then */
                                   /*   one wave per surviving pair; same results either way     */
