@@ -45,6 +45,7 @@ struct BodyDev {
   uint32_t branch_base;  // global index of the first branch-chunk node
   uint32_t depth;        // longest root-record-to-leaf path
   uint32_t super_base;   // H >= 2: global index of the first node slot of level H-2
+  uint32_t super3_base;  // H >= 3: global index of the first node slot of level H-3
   const uint32_t* tets;
   const uint8_t* surf;
   const float* X0;

@@ -77,6 +77,7 @@ struct RefitLaunch {
   size_t o_bb = 0, o_vpre = 0, o_tpre = 0, o_vwave = 0, o_twave = 0, o_pbody = 0, o_pchunk = 0, o_top = 0;
   uint64_t total_v = 0, total_leaf_threads = 0;
   uint32_t npbody = 0;
+  bool blk8 = false;  // every body has L = 8: 512-thread leaves blocks, which refit level H-3 too
   std::vector<RefitStep> steps;
   std::vector<uint32_t> wide;  // bodies refitted from their own lists (chunks over 32 leaves)
 };
@@ -380,6 +381,7 @@ int upload_bodies(kdop_ctx* c) {
     d.top_base = b.top_base; d.ntop = b.ntop; d.nbranch = b.C[b.H - 1]; d.H = b.H; d.L = b.L;
     d.branch_base = b.node_base + b.level_base[b.H - 1];
     d.super_base = b.H >= 2 ? b.node_base + b.level_base[b.H - 2] : 0u;
+    d.super3_base = b.H >= 3 ? b.node_base + b.level_base[b.H - 3] : 0u;
     d.depth = b.depth;
     d.tets = b.tets; d.surf = b.surf; d.X0 = b.X0; d.Y0 = b.Y0; d.Z0 = b.Z0; d.W0 = b.W0; d.X = b.X; d.Y = b.Y; d.Z = b.Z; d.W = b.W;
   }
@@ -397,10 +399,14 @@ int upload_bodies(kdop_ctx* c) {
 // ------------------------------------------------------------------- refit
 template <int K>
 struct RefitLeaves {
-  static void run(kdop_ctx* c, uint32_t nthreads, const uint32_t* bb, const uint32_t* tpre, const uint32_t* vpre, const uint32_t* twave,
-                  const uint32_t* vwave, uint32_t n, const float* pos, const float* quat, bool identity) {
-    hipLaunchKernelGGL(k_refit_leaves<K>, dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, vpre, twave, vwave,
-                       n, pos, quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
+  static void run(kdop_ctx* c, uint32_t nthreads, bool blk8, const uint32_t* bb, const uint32_t* tpre, const uint32_t* vpre,
+                  const uint32_t* twave, const uint32_t* vwave, uint32_t n, const float* pos, const float* quat, bool identity) {
+    if (blk8)
+      hipLaunchKernelGGL((k_refit_leaves<K, 8>), dim3((nthreads + 511) / 512), dim3(512), 0, c->stream, c->dbodies.p, bb, tpre, vpre, twave,
+                         vwave, n, pos, quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
+    else
+      hipLaunchKernelGGL((k_refit_leaves<K, 4>), dim3((nthreads + 255) / 256), dim3(256), 0, c->stream, c->dbodies.p, bb, tpre, vpre, twave,
+                         vwave, n, pos, quat, identity ? 1 : 0, c->dposes.p, c->nodes.p, c->dt);
   }
 };
 // A body with chunks wider than 32 leaves: leaf DOPs at their slots, then
@@ -536,7 +542,17 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   // full branch chunks: one thread per tet, each body's range padded to 64;
   // bodies with L = 16 first, their ranges padded to 256 so that each starts
   // on a block (the leaves launch refits their full super chunks of level
-  // H-2 per block).  tstart: each item's first thread, then the total.
+  // H-2 per block).  When every body has L = 8 the leaves launch takes
+  // 512-thread blocks, each range padded to 512, and a block refits its
+  // super chunk of level H-3.  tstart: each item's first thread, then the
+  // total.
+  bool blk8 = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    if (b.wide()) continue;
+    if (b.L != 8u) { blk8 = false; break; }
+    blk8 = true;
+  }
   const size_t o_tpre = plan.size();
   plan.insert(plan.end(), (size_t)n + 1, 0u);
   std::vector<uint32_t> order;  // items in the order of their ranges
@@ -554,7 +570,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   std::vector<uint32_t> twave;
   for (const uint32_t i : order) {
     const BodyHost& b = c->bodies[bodies[i]];
-    const uint64_t full = (uint64_t)(b.nt / b.L) * b.L, gran = b.L == 16u ? 256u : 64u;
+    const uint64_t full = (uint64_t)(b.nt / b.L) * b.L, gran = blk8 ? 512u : (b.L == 16u ? 256u : 64u);
     plan[o_tpre + i] = (uint32_t)acc;
     const uint64_t len = (full + gran - 1) / gran * gran;
     acc += len;
@@ -576,8 +592,11 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   // super level h of body b: {slot of its first chunk, chunks, elements of
   // the last chunk, first chunk to refit}; none when the leaves pass did it
   auto super_level = [&](const BodyHost& b, int h, uint4& lv) -> bool {
-    // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64: per wave, L = 16: per block)
-    const uint32_t cf = (h == (int)b.H - 2 && (b.L * b.L <= 64 || b.L == 16)) ? (b.nt / b.L) / b.L : 0u;
+    // level H-2 chunks over L full branch chunks were refitted with the leaves (L * L <= 64: per wave, L = 16: per
+    // block), and with 512-thread leaves blocks (every body L = 8) the full level H-3 chunks too
+    const uint32_t cf = (h == (int)b.H - 2 && (b.L * b.L <= 64 || b.L == 16)) ? (b.nt / b.L) / b.L
+                        : (h == (int)b.H - 3 && blk8) ? ((b.nt / b.L) / b.L) / b.L
+                                                      : 0u;
     const uint32_t nc = b.C[h], nl = host_chunk_elems(b, h, nc - 1);
     if (cf >= nc) return false;
     const uint32_t nfc = (nc - 1 - cf) + (nl == b.L ? 1u : 0u);
@@ -667,6 +686,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   R.total_v = total_v;
   R.total_leaf_threads = total_leaf_threads;
   R.npbody = (uint32_t)pbody.size();
+  R.blk8 = blk8;
   R.steps = steps;
   R.wide.clear();
   for (uint32_t i = 0; i < n; ++i)
@@ -690,8 +710,8 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
   // leaves and the vertex transform in one launch, its grid over both ranges
   const uint64_t leaves_grid = std::max<uint64_t>(R.total_leaf_threads, R.total_v);
   if (leaves_grid)
-    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, P + R.o_bb, P + R.o_tpre, P + R.o_vpre, P + R.o_twave, P + R.o_vwave, n,
-                            pos_dev, quat_dev, identity);
+    dispatch_K<RefitLeaves>(c->K, c, (uint32_t)leaves_grid, R.blk8, P + R.o_bb, P + R.o_tpre, P + R.o_vpre, P + R.o_twave, P + R.o_vwave,
+                            n, pos_dev, quat_dev, identity);
   if (R.npbody) dispatch_K<RefitBranch>(c->K, c, R.npbody, P + R.o_pbody, P + R.o_pchunk);
   for (const RefitStep& st : R.steps)
     dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_gitem, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,

@@ -427,8 +427,11 @@ __device__ inline void super16_butterfly(const float* __restrict__ roots, float*
 // When L = 16 a block's 256 tets are one super chunk of level H-2 (a body
 // with L = 16 has a tet range of its own that starts on a block), so after
 // one barrier wave 0 takes the block's 16 chunk roots from LDS and refits
-// that chunk (super16_butterfly).  Super chunks that are not full, or that
-// take a chunk that is not, are left to k_refit_super.
+// that chunk (super16_butterfly).  When every body of the batch has L = 8
+// the launch takes 512-thread blocks (NW = 8 waves), a block's 512 tets are
+// one super chunk of level H-3, and after one barrier wave 0 refits it from
+// its waves' level H-2 roots.  Super chunks that are not full, or that take a
+// chunk that is not, are left to k_refit_super.
 //
 // The volumes are staged in LDS (the leaves, the branch chunks' internal
 // nodes, then the super chunks' nodes; each a contiguous run of records) and
@@ -460,16 +463,17 @@ __device__ inline void super16_butterfly(const float* __restrict__ roots, float*
 // per thread is a chain of log2(batch) dependent loads: the refit of 128
 // bodies took 512 us with it and takes 379 us with the tables).
 // pos / quat: 3 and 4 floats per batch item (unused with identity).
-template <int K>
-__global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
+template <int K, int NW>
+__global__ __launch_bounds__(64 * NW) void k_refit_leaves(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ batch_body,
                                                       const uint32_t* __restrict__ tpre, const uint32_t* __restrict__ vpre,
                                                       const uint32_t* __restrict__ twave, const uint32_t* __restrict__ vwave, uint32_t nbatch,
                                                       const float* __restrict__ pos, const float* __restrict__ quat, int identity,
                                                       float* __restrict__ body_pose, float* __restrict__ nodes, DirTable dt) {
   constexpr uint32_t S = rec_stride(K), RS = (K + 3) & ~3, W = S / 4;  // RS: LDS row, W: 16-byte words per record
   static_assert(K % 4 == 0 || K % 4 == 2, "the meta words start a 16-byte word or end one");
-  __shared__ __align__(16) float stage[4][64 * RS];
-  __shared__ __align__(16) float roots16[16 * RS];  // L = 16: the block's chunk roots
+  static_assert(NW == 4 || NW == 8, "256- or 512-thread blocks");
+  __shared__ __align__(16) float stage[NW][64 * RS];
+  __shared__ __align__(16) float roots16[16 * RS];  // the block's chunk roots (L = 16: 16 of level H-1, L = 8: 8 of level H-2)
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t gw = __builtin_amdgcn_readfirstlane(g >> 6);  // this thread's wave of the batch
   if (gw < (vpre[nbatch] >> 6)) {
@@ -498,7 +502,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
     }
   }
   if (gw >= (tpre[nbatch] >> 6)) return;  // whole waves (tpre[nbatch] is a multiple of 64)
-  float* __restrict__ st = stage[threadIdx.x >> 6];
+  float* st = stage[threadIdx.x >> 6];  // this wave's rows (the block's roots for its last flush at L = 8)
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t item = twave[gw];
   const BodyDev& B = bodies[batch_body[item]];
@@ -588,7 +592,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
         const uint32_t i = r & (L - 1u), c0 = B.branch_base + ((tet0 >> logL) + (r >> logL)) * (2u * L) + 1u;
         return make_uint2(F_BRANCH | (c0 + 2u * i + 1u), (i == 0u ? F_ROOT : 0u) | (c0 + 2u * i + 2u));
       });
-  if (logL == 4u && B.H >= 2u) {  // L = 16: the block (one body, 256-aligned) is one super chunk of level H-2
+  if (NW == 4 && logL == 4u && B.H >= 2u) {  // L = 16: the block (one body, 256-aligned) is one super chunk of level H-2
     const uint32_t bw = threadIdx.x >> 6, btet0 = tet0 - 64u * bw;
     if (btet0 + 256u > nfull) return;  // not full (the same on every wave of the block)
     if (j == 0u) {  // every lane of a chunk holds its root; the first writes it
@@ -629,6 +633,26 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const BodyDev* __restrict_
         auto child = [&](uint32_t h) { return h < L - 1u ? s0 + h : B.branch_base + (sc * L + h - (L - 1u)) * (2u * L) + 1u; };
         return make_uint2(child(2u * i + 1u), child(2u * i + 2u));
       });
+  if constexpr (NW == 8) {
+    // L = 8 in 512-thread blocks: the block's eight waves hold the eight
+    // super chunks of level H-2 (one body, 512-aligned) of one super chunk
+    // of level H-3, whose roots are row 0 of each wave's stage
+    const uint32_t bw = threadIdx.x >> 6, btet0 = tet0 - 64u * bw;
+    if (logL != 3u || B.H < 3u || btet0 + 512u > nfull) return;  // the same on every wave of the block
+    if (lane < (uint32_t)K) roots16[bw * RS + lane] = st[lane];
+    __syncthreads();
+    if (bw != 0u) return;
+    super_butterfly<K, 3, RS>(roots16, lane, true);  // its 7 nodes -> rows 0..6
+    st = roots16;
+    const uint32_t sc = btet0 >> 9;  // the body's super chunk of level H-3
+    flush(
+        7u, [&](uint32_t r) -> float* { return nodes + ((size_t)B.super3_base + (size_t)sc * 8u + 1u + r) * S; },
+        [&](uint32_t r) {  // its children: BFS 2r + 1 and 2r + 2 below 7 are its nodes, the others the roots of its level H-2 chunks
+          const uint32_t s0 = B.super3_base + sc * 8u + 1u;
+          auto child = [&](uint32_t h) { return h < 7u ? s0 + h : B.super_base + (sc * 8u + h - 7u) * 8u + 1u; };
+          return make_uint2(child(2u * r + 1u), child(2u * r + 2u));
+        });
+  }
 }
 
 // Branch chunks that are not full (the last chunk of a body when L does not
