@@ -153,6 +153,12 @@ struct kdop_ctx {
   uint32_t round_iters = kRoundItersDefault;  // KDOP_OPT_ROUND_ITERS: a block's iteration budget per round once the frontier is gone (0: none)
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 0;      // most breadth-first levels before the persistent rounds (0: by the call's size, bfs_auto)
+  // the level at which the last persistent call's ramp stopped (an empty or
+  // target-wide frontier; ~0u: it ran to its last level), the seeds and
+  // target it ran with: a call from the same seeds queues no levels past it
+  uint64_t seed_gen = 0;  // bumped whenever the seed descriptors are rebuilt
+  uint64_t ramp_seed_gen = ~0ull;
+  uint32_t ramp_stop = 0, ramp_stop_target = 0;
   int timing_level = 0;        // 0 none, 1 phase events, 2 + per-round dispatch events
   uint32_t rounds_hint = 1;    // persistent rounds to queue next call (1..3), from the rounds recent calls needed
   uint32_t rounds_calm = 0;    // consecutive calls that needed fewer rounds than rounds_hint
@@ -1628,6 +1634,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     c->seed_key_budget = c->seed_budget;
     c->seed_key_bodies = c->bodies.size();
     c->seed_total = total_seeds;
+    ++c->seed_gen;
   }
   const bool keep_p1 = flags & KDOP_KEEP_ALL_PAIRS;
   const bool narrow = flags & KDOP_NARROW;
@@ -1678,7 +1685,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     const uint32_t cap = (uint32_t)std::min<size_t>(std::min(c->front[0].cap, c->front[1].cap), 0xFFFFFFFFu);
     uint32_t rounds = 0, levels_queued = 0;
-    const uint32_t last = std::min(levels, c->bfs_max ? c->bfs_max : bfs_auto(call_tets));
+    uint32_t last = std::min(levels, c->bfs_max ? c->bfs_max : bfs_auto(call_tets));
+    // levels past the one where the last call from the same seeds found its
+    // frontier would only find the stop flag (a launch each); round 0 takes
+    // whatever level the ramp reaches, so the results do not depend on it
+    if (persistent && c->ramp_seed_gen == c->seed_gen && c->ramp_stop_target == target && c->ramp_stop < last)
+      last = c->ramp_stop;
     if (npairs) {
       if (persistent) rounds = rounds_async;
       else levels_queued = std::min(levels, level_guess);
@@ -1846,6 +1858,11 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     }
     c->last_levels = persistent ? 0 : levels;
     c->last_fallback_levels = fallback_levels;
+    if (persistent && npairs) {
+      c->ramp_seed_gen = c->seed_gen;
+      c->ramp_stop_target = target;
+      c->ramp_stop = (h.final_set && fallback_levels == 0) ? h.final_level : ~0u;  // (levels after spilled rounds stop elsewhere)
+    }
     c->last_distinct_records = 0;
     if (T.touched) {  // count the marked records (a measurement call; the copy is not timed)
       std::vector<uint8_t> h(c->nodes_used);

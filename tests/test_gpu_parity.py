@@ -317,3 +317,32 @@ def test_queued_rounds_follow_recent_calls():
     # calls after the spilling ones queue as many rounds as those needed
     # until they see fewer are enough, so only the smallest counts compare
     assert min(seen[512]) <= 2 < min(seen[1])
+
+
+def test_ramp_stops_where_the_last_call_stopped():
+    """A call from the same seeds queues no breadth-first launches past the
+    level where the previous call's ramp found a target-wide frontier (they
+    would only find the stop flag); every call still gives the oracle's
+    results, and so do calls after the target changes (the history no longer
+    applies) and after the pair list changes (new seeds)."""
+    wl = W.batched_pairs(npairs=12, K=14, seed=4, tmin=200, tmax=6000)
+    orc, _, _ = oracle_run(wl)
+    ctx = K.Context(K=14, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+    ctx.set_option("bfs_max", 6)
+    ids = None
+    stops = []
+    for target in (256, 256, 256, 4096, 4096):
+        ctx.set_option("ramp_target", target)
+        dev = device_run(wl, ctx=ctx, ids=ids)
+        ids = dev["ids"]
+        stops.append(ctx.traversal_stats()["ramp_levels"])
+        _check_pair_results(wl, dev, orc)
+    print("ramp stop levels", stops, flush=True)
+    # the same seeds and target stop at the same level, whether the levels
+    # after it were queued (call 1) or not (calls 2, 3)
+    assert stops[0] == stops[1] == stops[2] < 6 and stops[3] == stops[4] >= stops[0]
+    # a shorter pair list (new seeds) from the same context
+    sub = W.PairWorkload(wl.name, wl.K, wl.meshes, wl.pairs[:5], wl.poses)
+    orc5 = orc[:5]
+    dev = device_run(sub, ctx=ctx, ids=ids)
+    _check_pair_results(sub, dev, orc5)
