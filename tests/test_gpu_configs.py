@@ -236,13 +236,16 @@ def test_chunk_boundary_tet_counts(k):
             np.testing.assert_array_equal(v, ref[(int(f), int(l))])
 
 
-@pytest.mark.parametrize("k,chunk_bytes", [(26, 2000), (26, 4000), (26, 8000), (14, 8000), (8, 8000)],
-                         ids=["L2", "L4", "L8", "L16", "L32"])
+@pytest.mark.parametrize("k,chunk_bytes", [(26, 2000), (26, 4000), (26, 8000), (14, 8000), (8, 8000), (6, 2000), (8, 2000),
+                                           (12, 4000), (20, 4000), (20, 8000)],
+                         ids=["L2", "L4", "L8", "L16", "L32", "k6_L8", "k8_L8", "k12_L8", "k20_L8", "k20_L16"])
 def test_refit_tree_bits_around_super_chunks(k, chunk_bytes):
     """Tree volumes bit for bit for every chunk width L, on tet counts around
     L and L*L: level H-2 is refitted together with the leaves where L*L <= 64
-    (whole super chunks in a wave) and by the super-chunk pass where a chunk
-    or super chunk is not full."""
+    (whole super chunks in a wave) or L = 16 (one per block), level H-3 too
+    where every body has L = 8 (512-thread blocks), and by the super-chunk
+    pass where a chunk or super chunk is not full.  Every record width (32,
+    64 and 128 bytes, meta words starting a 16-byte word or ending one)."""
     L = K.tree_shape(k, chunk_bytes, 4096)["L"]
     base = K.box_mesh(9, size=1.0, jitter=0.1, seed=11)
     counts = sorted({min(c, base.tets.shape[0]) for c in
