@@ -355,8 +355,9 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 #define KDOP_OPT_NARROW_PATH 10   /* narrow phase: 0 auto (by the last surface-pair count), */
                                   /*   1 one wave per pair, 2 SAT gate (8 lanes per pair) then */
                                   /*   one wave per surviving pair; same results either way     */
-#define KDOP_OPT_GRAPHS 11        /* 1 (default): kdop_traverse replays its launches from a HIP */
-                                  /*   graph recorded per set of launch arguments; 0: direct    */
+#define KDOP_OPT_GRAPHS 11        /* 1: kdop_traverse replays its launches from a HIP graph    */
+                                  /*   recorded per set of launch arguments; 0 (default):      */
+                                  /*   direct launches                                         */
 #define KDOP_OPT_ASYNC_REFIT 9    /* 1: kdop_refit / kdop_refit_device return once the work is */
                                   /*   queued on the context stream (later calls order after it; */
                                   /*   device pose buffers must stay valid until the next call   */

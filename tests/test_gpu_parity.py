@@ -236,6 +236,7 @@ def test_async_refit_graph_replay_and_state_reuse():
     P, Q = W.motion(2, 6, seed=5, dt=0.05, drift=0.01)
     ctxs = [K.Context(K=18), K.Context(K=18)]
     ctxs[0].set_option("graphs", 0)
+    ctxs[1].set_option("graphs", 1)
     ctxs[1].set_option("async_refit", 1)
     ids = [[c.make_tree(m) for m in wl.meshes] for c in ctxs]
     for s in range(6):
@@ -300,6 +301,7 @@ def test_queued_rounds_follow_recent_calls():
     ref = device_run(wl, keep_all=False, traversal=K.KDOP_TRAVERSAL_LEVELS)
     want = (ref["counters"], key(ref["p2"]), key(ref["ctx"].contacts()))
     ctx = K.Context(K=wl.K, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+    ctx.set_option("graphs", 1)
     ids = [ctx.make_tree(m) for m in wl.meshes]
     ctx.refit(ids, np.stack([p.p for p in wl.poses]), np.stack([p.q for p in wl.poses]))
     pairs = np.array([[ids[a], ids[b]] for a, b in wl.pairs], np.uint32)
