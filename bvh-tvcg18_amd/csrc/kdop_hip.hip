@@ -516,8 +516,8 @@ struct SgLeaves {
 };
 template <int K>
 struct RefitTop {
-  static void run(kdop_ctx* c, uint32_t n, const uint4* ent) {
-    hipLaunchKernelGGL(k_refit_top<K>, dim3(n), dim3(kTopThreads), 0, c->stream, ent, c->dtops.p, c->nodes.p);
+  static void run(kdop_ctx* c, uint32_t n, const uint4* ent, const uint32_t* bb) {
+    hipLaunchKernelGGL(k_refit_top<K>, dim3(n), dim3(kTopThreads), 0, c->stream, ent, c->dtops.p, c->nodes.p, c->dbodies.p, bb, c->dt);
   }
 };
 // Super levels of at most this many chunks go to the one-block-per-body
@@ -568,11 +568,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
       if (!b.wide() && (b.L == 16u) == (pass == 0)) order.push_back(i);
     }
   acc = 0;
-  std::vector<uint32_t> pbody, pchunk;
-  for (uint32_t i = 0; i < n; ++i) {
-    const BodyHost& b = c->bodies[bodies[i]];
-    if (!b.wide() && b.nt % b.L) { pbody.push_back(bodies[i]); pchunk.push_back(b.C[b.H - 1] - 1); }
-  }
+  std::vector<uint32_t> pbody, pchunk;  // partial branch chunks left to k_refit_branch (filled below)
   std::vector<uint32_t> twave;
   for (const uint32_t i : order) {
     const BodyHost& b = c->bodies[bodies[i]];
@@ -591,10 +587,6 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
   for (uint32_t i = 0; i < n; ++i) plan.insert(plan.end(), (plan[o_vpre + i + 1] - plan[o_vpre + i]) / 64, i);
   const size_t o_twave = plan.size();
   plan.insert(plan.end(), twave.begin(), twave.end());
-  const size_t o_pbody = plan.size();
-  plan.insert(plan.end(), pbody.begin(), pbody.end());
-  const size_t o_pchunk = plan.size();
-  plan.insert(plan.end(), pchunk.begin(), pchunk.end());
   // super level h of body b: {slot of its first chunk, chunks, elements of
   // the last chunk, first chunk to refit}; none when the leaves pass did it
   auto super_level = [&](const BodyHost& b, int h, uint4& lv) -> bool {
@@ -628,6 +620,21 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
       htop[i] = h;
     }
   }
+  // a body's partial branch chunk goes to k_refit_top when that block refits
+  // every level above it (level H-2 and up), else to k_refit_branch, before
+  // the super launches that need its root
+  std::vector<uint32_t> top_branch(n, 0u);  // partial branch chunk + 1 per item
+  for (uint32_t i = 0; i < n; ++i) {
+    const BodyHost& b = c->bodies[bodies[i]];
+    if (b.wide() || b.nt % b.L == 0) continue;
+    const uint32_t chunk = b.C[b.H - 1] - 1;
+    if (htop[i] >= (int)b.H - 2) top_branch[i] = chunk + 1u;
+    else { pbody.push_back(bodies[i]); pchunk.push_back(chunk); }
+  }
+  const size_t o_pbody = plan.size();
+  plan.insert(plan.end(), pbody.begin(), pbody.end());
+  const size_t o_pchunk = plan.size();
+  plan.insert(plan.end(), pchunk.begin(), pchunk.end());
   using Step = RefitStep;
   std::vector<Step> steps;
   for (uint32_t s = 1; s < maxH; ++s) {
@@ -678,7 +685,7 @@ int build_refit_plan(kdop_ctx* c, uint32_t n, const uint32_t* bodies) {
       if (super_level(b, h, lv)) { levs.push_back(lv); rows += lv.y - lv.w; }
     }
     const uint32_t nmeta = rows * (b.L - 1);
-    plan.insert(plan.end(), {(uint32_t)levs.size(), b.L, b.ntop, b.top_base, b.node_base, rows, nmeta, 0u});
+    plan.insert(plan.end(), {(uint32_t)levs.size(), b.L, b.ntop, b.top_base, b.node_base, rows, nmeta, top_branch[i]});
     for (uint32_t e = 0; e < kTopEntries - 2; ++e) {
       const uint4 lv = e < levs.size() ? levs[e] : make_uint4(0, 0, 0, 0);
       plan.insert(plan.end(), {lv.x, lv.y, lv.z, lv.w});
@@ -723,7 +730,7 @@ int run_refit(kdop_ctx* c, uint32_t n, const uint32_t* bodies, const float* pos_
     dispatch_K<RefitSuper>(c->K, c, st.blocks, P + st.o_bb, P + st.o_gpre, P + st.o_gitem, P + st.o_cfirst, P + st.o_ncnk, P + st.o_cbase, P + st.o_cn,
                            st.m);
   for (const uint32_t id : R.wide) dispatch_K<RefitWide>(c->K, c, id, c->bodies[id]);
-  dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + R.o_top));
+  dispatch_K<RefitTop>(c->K, c, n, reinterpret_cast<const uint4*>(P + R.o_top), P + R.o_bb);
   HIPCHK(c, hipGetLastError());
   if (!c->async_refit || identity) HIPCHK(c, hipStreamSynchronize(c->stream));
   return KDOP_OK;

@@ -655,34 +655,32 @@ __global__ __launch_bounds__(64 * NW) void k_refit_leaves(const BodyDev* __restr
   }
 }
 
-// Branch chunks that are not full (the last chunk of a body when L does not
-// divide its tet count), one 64-lane block per chunk of the list: leaf DOPs,
-// unions bottom-up in LDS, then one coalesced write of the chunk's volumes.
+// A branch chunk that is not full (the last chunk of a body when L does not
+// divide its tet count), by one wave: leaf DOPs from the stored world
+// vertices, unions bottom-up in LDS, then one coalesced write of the chunk's
+// volumes (the meta words stay).  c: the chunk.
 template <int K>
-__global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ lbody,
-                                                     const uint32_t* __restrict__ lchunk, uint32_t nlist, float* __restrict__ nodes,
-                                                     DirTable dt) {
+struct BranchLds {
+  float vol[63][K];
+  uint32_t meta0[63], meta1[63];
+};
+template <int K>
+__device__ inline void refit_partial_branch(const BodyDev& B, uint32_t c, float* __restrict__ nodes, const DirTable& dt, BranchLds<K>& sh,
+                                            uint32_t lane) {
   constexpr int N = K / 2, S = rec_stride(K);
-  __shared__ float vol[63][K];
-  __shared__ uint32_t meta0[63], meta1[63];
-  const uint32_t g = blockIdx.x;
-  if (g >= nlist) return;
-  const BodyDev& B = bodies[lbody[g]];
-  const uint32_t c = lchunk[g];
   const uint32_t L = B.L;
   const uint32_t n = (c + 1 < B.nbranch) ? L : B.nt - c * L;
   const uint32_t nn = 2 * n - 1;
   const uint32_t base = B.branch_base + c * (2 * L) + 1;  // slot of BFS node 0
-  const int lane = threadIdx.x;
   for (uint32_t i = lane; i < nn; i += 64) {
     const uint2 m = load_meta<K>(nodes, base + i);
-    meta0[i] = m.x; meta1[i] = m.y;
+    sh.meta0[i] = m.x; sh.meta1[i] = m.y;
   }
-  __syncthreads();
+  wave_lds_sync();
   // leaf slabs: lanes over (node, slab)
   for (uint32_t w = lane; w < nn * N; w += 64) {
     const uint32_t i = w / N, k = w - i * N;
-    const uint32_t m0 = meta0[i];
+    const uint32_t m0 = sh.meta0[i];
     if (!(m0 & F_LEAF)) continue;
     const uint32_t tet = m0 & IDX_MASK;
     const uint4 tv = *reinterpret_cast<const uint4*>(B.tets + 4 * (size_t)tet);
@@ -696,39 +694,51 @@ __global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__
       lo = smin(lo, proj);
       hi = smax(hi, proj);
     }
-    vol[i][2 * k] = lo;
-    vol[i][2 * k + 1] = hi;
+    sh.vol[i][2 * k] = lo;
+    sh.vol[i][2 * k + 1] = hi;
   }
-  __syncthreads();
+  wave_lds_sync();
   if (n == L && (L & (L - 1)) == 0) {
     // perfect chunk: BFS order is the implicit heap, children of i at 2i+1, 2i+2
     for (int first = (int)(L / 2) - 1, cnt = (int)(L / 2); cnt >= 1; cnt >>= 1, first = (first - 1) / 2) {
-      for (int w = lane; w < cnt * K; w += 64) {
+      for (int w = (int)lane; w < cnt * K; w += 64) {
         const int i = first + w / K, f = w % K;
-        const float a = vol[2 * i + 1][f], b = vol[2 * i + 2][f];
-        vol[i][f] = (f & 1) ? smax(a, b) : smin(a, b);
+        const float a = sh.vol[2 * i + 1][f], b = sh.vol[2 * i + 2][f];
+        sh.vol[i][f] = (f & 1) ? smax(a, b) : smin(a, b);
       }
-      __syncthreads();
+      wave_lds_sync();
       if (first == 0) break;
     }
   } else {
     // partial chunk: reverse BFS sweep (children always follow their parent)
     for (int i = (int)nn - 1; i >= 0; --i) {
-      const uint32_t m0 = meta0[i];
+      const uint32_t m0 = sh.meta0[i];
       if (!(m0 & F_LEAF)) {
-        const uint32_t c0 = (m0 & IDX_MASK) - base, c1 = (meta1[i] & IDX_MASK) - base;
-        if (lane < K) {
-          const float a = vol[c0][lane], b = vol[c1][lane];
-          vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
+        const uint32_t c0 = (m0 & IDX_MASK) - base, c1 = (sh.meta1[i] & IDX_MASK) - base;
+        if (lane < (uint32_t)K) {
+          const float a = sh.vol[c0][lane], b = sh.vol[c1][lane];
+          sh.vol[i][lane] = (lane & 1) ? smax(a, b) : smin(a, b);
         }
       }
-      __syncthreads();
+      wave_lds_sync();
     }
   }
   for (uint32_t w = lane; w < nn * K; w += 64) {
     const uint32_t i = w / K, f = w - i * K;
-    nodes[(size_t)(base + i) * S + f] = vol[i][f];
+    nodes[(size_t)(base + i) * S + f] = sh.vol[i][f];
   }
+  wave_lds_sync();
+}
+// The partial chunks of the list, one wave each (bodies whose upper levels
+// are refitted by launches that need the chunk's root before k_refit_top).
+template <int K>
+__global__ __launch_bounds__(64) void k_refit_branch(const BodyDev* __restrict__ bodies, const uint32_t* __restrict__ lbody,
+                                                     const uint32_t* __restrict__ lchunk, uint32_t nlist, float* __restrict__ nodes,
+                                                     DirTable dt) {
+  __shared__ BranchLds<K> sh;
+  const uint32_t g = blockIdx.x;
+  if (g >= nlist) return;
+  refit_partial_branch<K>(bodies[lbody[g]], lchunk[g], nodes, dt, sh, threadIdx.x);
 }
 
 // Super chunks (kdop_refit_tree.h:82-115): their children outside the chunk
@@ -913,10 +923,14 @@ __global__ __launch_bounds__(64) void k_refit_super(const BodyDev* __restrict__ 
 // (kdop_refit_tree.h:154-161).  The launch is a chain of dependent memory
 // round trips, so it is built to have three: the item's entries (kTopEntries
 // uint4 at ent[item * kTopEntries]: {levels, L, top roots, first top root},
-// {root slot, LDS rows, meta words, 0}, then per level {slot of the first
-// chunk, chunks, elements of the last chunk, first chunk to refit}); then,
-// all at once, the meta words of every internal node of those levels' chunks
-// and the top-root slots, into LDS; then the first level's children.  Every
+// {root slot, LDS rows, meta words, partial branch chunk + 1 or 0}, then per
+// level {slot of the first chunk, chunks, elements of the last chunk, first
+// chunk to refit}); then, all at once, the meta words of every internal node
+// of those levels' chunks and the top-root slots, into LDS; then the first
+// level's children.  When the block refits every level above the body's
+// partial branch chunk (the last chunk, when L does not divide the tet
+// count), its last wave refits that chunk first, which saves a launch; a
+// full barrier then makes its records visible to the levels.  Every
 // chunk root the block computes goes to an LDS table (a row per chunk of its
 // levels, chunks [cf, nc)) besides its record, and a later level or the root
 // reads a child from there when the block made it, so the levels are linked
@@ -954,12 +968,16 @@ struct TopLds {
   uint2 meta[kTopMetas];
   uint4 ent[kTopEntries];
   uint32_t tops[kTopTops];
-  PartialLds<K> part;
+  union {
+    PartialLds<K> part;     // the levels' partial chunks
+    BranchLds<K> branch;    // the body's partial branch chunk, before the levels
+  };
 };
 
 template <int K>
 __global__ __launch_bounds__(kTopThreads) void k_refit_top(const uint4* __restrict__ ent, const uint32_t* __restrict__ tops,
-                                                          float* __restrict__ nodes) {
+                                                          float* __restrict__ nodes, const BodyDev* __restrict__ bodies,
+                                                          const uint32_t* __restrict__ batch_body, DirTable dt) {
   constexpr int S = rec_stride(K);
   __shared__ TopLds<K> sh;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6, nw = blockDim.x >> 6;
@@ -994,6 +1012,10 @@ __global__ __launch_bounds__(kTopThreads) void k_refit_top(const uint4* __restri
     if (tid < min(ntop, kTopTops)) sh.tops[tid] = tv;
   }
   block_lds_barrier();
+  if (h1.w) {  // the body's partial branch chunk + 1, when this block refits every level above it
+    if (w == nw - 1u) refit_partial_branch<K>(bodies[batch_body[blockIdx.x]], h1.w - 1u, nodes, dt, sh.branch, lane);
+    __syncthreads();  // its records are in global memory for the levels (and its LDS free for sh.part)
+  }
   uint32_t row0 = 0;  // LDS row of chunk cf of level e
   for (uint32_t e = 0; e < nlev; ++e) {
     const uint4 l = lv[e];  // cbase, nc, nlast, cf
