@@ -1727,11 +1727,10 @@ __device__ inline void gather_tet(const BodyDev& B, uint32_t t, f3* P) {
 // the first face pair, in a-major order, with the smallest dot product) come
 // from ballots over the group.  V, U: the group's LDS for the two tets'
 // vertices and face planes.  Returns the A face, or -1 (same in all 8 lanes).
-__device__ inline int gate_group(const NarrowArgs& A, f3* V, UPlane* U, uint32_t i, bool live) {
+__device__ inline int gate_group(const NarrowArgs& A, f3* V, UPlane* U, const LeafPair& lp, bool live) {
   const uint32_t lane = lane_id(), r = lane & 7u, shift = lane & ~7u;
   uint32_t sa = 0, sb = 0;
   if (live) {
-    const LeafPair lp = A.p2[i];
     const uint2 bp = A.pairs[lp.pair];
     const BodyDev& BA = A.bodies[bp.x];
     const BodyDev& BB = A.bodies[bp.y];
@@ -1953,19 +1952,34 @@ __device__ inline void narrow_contacts_wave(const NarrowArgs& A, NarrowLds& L, u
   wave_lds_sync();  // this wave's LDS is rewritten by its next pair
 }
 
+// The launch's control words -- the fills of the round's segments (A.pending:
+// the narrow phase runs only when the persistent rounds left nothing), the
+// done flag and the surface-pair count -- and this group's first pair entry
+// (slot `first`, read before the count is known; used only below it), all in
+// one round trip.  Returns false when the launch has nothing to do; *n: the
+// pairs to take.
+__device__ inline bool narrow_start(const NarrowArgs& A, uint32_t first, uint32_t* n, LeafPair* lp) {
+  uint32_t pv = 0u, dn = 1u;
+  if (A.pending) pv = A.pending[(size_t)lane_id() * A.pending_stride];
+  if (A.done) dn = *A.done;
+  const uint32_t fill = *A.p2_fill;
+  *lp = LeafPair{0u, 0u, 0u};
+  if (first < A.p2_cap) *lp = A.p2[first];
+  if (__any(pv != 0u) || dn == 0u) return false;
+  *n = min(fill, A.p2_cap);
+  return true;
+}
+
 // Narrow phase for few surface pairs: one wave per pair.  Lanes 0..43 test the
 // SAT axes and lanes 0..15 the face pairs (the serial loops' results by
 // ballots), then narrow_contacts_wave on the same wave.
 __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
   __shared__ NarrowLds lds[4];
   __shared__ NarrowStage S;
-  if (A.pending) {
-    const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
-    if (__any(v != 0u)) return;
-  }
-  if (A.done && *A.done == 0u) return;
-  const uint32_t n = min(*A.p2_fill, A.p2_cap);
   const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t n = 0;
+  LeafPair lp_next;
+  if (!narrow_start(A, blockIdx.x * 4u + w, &n, &lp_next)) return;
   NarrowLds& L = lds[w];
   unsigned long long hits = 0;
   if (threadIdx.x == 0) S.nc = S.nr = 0u;
@@ -1973,9 +1987,10 @@ __global__ __launch_bounds__(256) void k_narrow_wave(NarrowArgs A) {
   for (uint32_t base = blockIdx.x * 4u; base < n; base += gridDim.x * 4u) {
     const uint32_t i = base + w;
     bool live = i < n;  // wave-uniform
+    const LeafPair lp = lp_next;  // loaded an iteration ahead
+    if (i + gridDim.x * 4u < n) lp_next = A.p2[i + gridDim.x * 4u];
     uint32_t sa = 0, sb = 0;
     if (live) {
-      const LeafPair lp = A.p2[i];
       const uint2 bp = A.pairs[lp.pair];
       const BodyDev& BA = A.bodies[bp.x];
       const BodyDev& BB = A.bodies[bp.y];
@@ -2031,20 +2046,19 @@ __global__ __launch_bounds__(256) void k_narrow_gate(NarrowArgs A) {
   __shared__ UPlane su[32][8];
   __shared__ uint32_t gst[kGateStage];
   __shared__ uint32_t gn, gbase;
-  if (A.pending) {
-    const uint32_t v = A.pending[(size_t)lane_id() * A.pending_stride];
-    if (__any(v != 0u)) return;
-  }
-  if (A.done && *A.done == 0u) return;
-  const uint32_t n = min(*A.p2_fill, A.p2_cap);
   const uint32_t lane = lane_id(), grp = threadIdx.x >> 3;
+  uint32_t n = 0;
+  LeafPair lp_next;
+  if (!narrow_start(A, blockIdx.x * 32u + grp, &n, &lp_next)) return;
   if (threadIdx.x == 0) gn = 0u;
   __syncthreads();
   for (uint32_t base = blockIdx.x * 32u;; base += gridDim.x * 32u) {
     const bool more = base < n;  // block-uniform
     if (more) {
       const uint32_t i = base + grp;
-      const int face = gate_group(A, sv[grp], su[grp], i, i < n);
+      const LeafPair lp = lp_next;  // loaded an iteration ahead
+      if (i + gridDim.x * 32u < n) lp_next = A.p2[i + gridDim.x * 32u];
+      const int face = gate_group(A, sv[grp], su[grp], lp, i < n);
       if ((lane & 7u) == 0u && face >= 0) gst[atomicAdd(&gn, 1u)] = i | ((uint32_t)face << 30);  // <= 32 per iteration
     }
     __syncthreads();
