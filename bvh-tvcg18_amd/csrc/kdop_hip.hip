@@ -35,6 +35,14 @@ constexpr uint32_t kNoHandOffKeep = 1u << 20;
 // such a query runs one round, while a batch whose heavy pairs would keep a
 // few blocks busy for hundreds of iterations is spread over later rounds.
 constexpr uint32_t kRoundItersDefault = 64;
+// ... and for calls of at least kRoundItersBatchPairs pairs, whose rounds
+// hand heavy stacks on anyway: 512-pair batch at 64 / 48 / 32 / 24 / 16
+// iterations 1.544-1.555 / 1.555-1.559 / 1.522-1.532 / 1.507-1.509 /
+// 1.518-1.519 ms per call, 64-pair refit series step 0.859 / 0.854-0.857 /
+// 0.856-0.857 ms at 64 / 32 / 24 (a one-pair query at 24 hands its tail on:
+// 0.186 -> 0.206 ms per step).
+constexpr uint32_t kRoundItersBatch = 24;
+constexpr uint32_t kRoundItersBatchPairs = 64;
 
 namespace {
 
@@ -151,6 +159,8 @@ struct kdop_ctx {
   int quad_items = 1;        // KDOP_OPT_QUAD_ITEMS: items per quad per iteration of k_persist_quad (1 or 2)
   int node_slab = 0;         // KDOP_OPT_NODE_SLAB: 1 stages each wave's records in an LDS node slab (k_persist_quad SLAB)
   uint32_t round_iters = kRoundItersDefault;  // KDOP_OPT_ROUND_ITERS: a block's iteration budget per round once the frontier is gone (0: none)
+  bool round_iters_set = false;               // the caller set it; else by the call's pair count
+  uint32_t round_iters_eff = kRoundItersDefault;  // the budget of the current (or last) call
   uint32_t ramp_target = 0;  // frontier width that ends the breadth-first levels (0: 16 x persistent waves)
   uint32_t bfs_max = 0;      // most breadth-first levels before the persistent rounds (0: by the call's size, bfs_auto)
   // the level at which the last persistent call's ramp stopped (an empty or
@@ -840,7 +850,7 @@ void launch_round_quad(kdop_ctx* c, const TravArgs& T, uint32_t cap, uint32_t ro
   Item* b1 = c->front[1].p;
   // the budget applies in rounds that may spill (the last one allowed takes
   // no budget: its spill would only go to the level-synchronous finish)
-  const uint32_t budget = (c->round_iters && round + 1 < c->max_rounds) ? c->round_iters : ~0u;
+  const uint32_t budget = (c->round_iters_eff && round + 1 < c->max_rounds) ? c->round_iters_eff : ~0u;
   if (c->timing_level >= 2)
     hipExtLaunchKernelGGL((k_persist_quad<K, W, SL, COUNT, IPQ, SLAB>), grid, block, 0, c->stream, c->evr[2 * round], c->evr[2 * round + 1],
                           0, T, b0, b1, cap, c->dstate, round, round_keep(c, round), last, budget);
@@ -1092,6 +1102,7 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
     case KDOP_OPT_ROUND_ITERS:  // 0: no budget
       if (v < 0 || v > (1ll << 20)) break;
       c->round_iters = (uint32_t)v;
+      c->round_iters_set = true;
       return KDOP_OK;
     case KDOP_OPT_QUAD_ITEMS:
       if (v == 0) v = 1;
@@ -1165,7 +1176,7 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       case KDOP_OPT_QUEUE_DEPTH: return quad ? std::vector<int64_t>{128, 256, 512} : std::vector<int64_t>{256, 512, 1024};
       // no hand-off, or a keep that hands on the bottom of deep stacks
       case KDOP_OPT_STACK_KEEP: return {0, ls / 2, ls / 8};
-      case KDOP_OPT_ROUND_ITERS: return {0, 16, 32, 64};
+      case KDOP_OPT_ROUND_ITERS: return {0, 16, 24, 32, 64};
       case KDOP_OPT_NODE_SLAB: return quad ? std::vector<int64_t>{0, 1} : std::vector<int64_t>{0};
       case KDOP_OPT_RAMP_TARGET: return {4 * pw, 16 * pw, 64 * pw};
       default: return {16384, 65536, 262144};  // KDOP_OPT_SEED_BUDGET
@@ -1228,7 +1239,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_COUNT_RECORDS: *v = c->count_records ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_QUAD_ITEMS: *v = c->quad_items; return KDOP_OK;
     case KDOP_OPT_NODE_SLAB: *v = c->node_slab; return KDOP_OK;
-    case KDOP_OPT_ROUND_ITERS: *v = c->round_iters; return KDOP_OK;
+    case KDOP_OPT_ROUND_ITERS: *v = c->round_iters_set ? c->round_iters : c->round_iters_eff; return KDOP_OK;
     default: return fail(c, KDOP_ERR_ARG, "kdop_get_option: unknown option %d", option);
   }
 }
@@ -1684,6 +1695,8 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     T.p1_cap = keep_p1 ? (uint32_t)std::min<size_t>(c->p1.cap, 0xFFFFFFFFu) : 0u;
     T.p2_cap = (uint32_t)std::min<size_t>(c->p2.cap, 0xFFFFFFFFu);
     T.npairs = npairs;
+    c->round_iters_eff = c->round_iters_set ? c->round_iters
+                                            : (npairs >= kRoundItersBatchPairs ? kRoundItersBatch : kRoundItersDefault);
     T.touched = nullptr;
     if (c->count_records && persistent) {
       if ((rc = ensure(c, c->touched, std::max<size_t>(c->nodes_used, 1)))) return rc;
@@ -1748,7 +1761,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
           (uint64_t)(uintptr_t)c->dseeds.p, (uint64_t)(uintptr_t)c->front[0].p, (uint64_t)(uintptr_t)c->front[1].p,
           (uint64_t)(uintptr_t)c->contacts.p, c->contacts.cap, (uint64_t)(uintptr_t)c->recs.p, c->recs.cap,
           (uint64_t)(uintptr_t)c->gate.p, (uint64_t)wpc_of(c), (uint64_t)ls_of(c), c->keep, (uint64_t)c->num_cus, (uint64_t)bw_of(c), rounds_async,
-          (uint64_t)c->item_lanes, c->max_rounds, (uint64_t)c->quad_items, c->round_iters, (uint64_t)c->node_slab};
+          (uint64_t)c->item_lanes, c->max_rounds, (uint64_t)c->quad_items, c->round_iters_eff, (uint64_t)c->node_slab};
     }
     const bool replay = !sig.empty() && ((c->tg_exec && sig == c->tg_sig) || sig == c->prev_sig);
     c->prev_sig = sig;

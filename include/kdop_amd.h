@@ -383,8 +383,10 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
 #define KDOP_OPT_ROUND_ITERS 18   /* quad kernel: once the frontier is gone, a workgroup that   */
                                   /*   has run this many iterations in a round hands its whole  */
                                   /*   stack to the next round (spread there over every         */
-                                  /*   workgroup) and stops; the default is 64, 0 means no      */
-                                  /*   budget.  The last round allowed (KDOP_OPT_MAX_ROUNDS)     */
+                                  /*   workgroup) and stops; by default 24 for calls of 64 pairs */
+                                  /*   or more and 64 below (kdop_get_option reports the last    */
+                                  /*   call's), 0 means no budget.  The last round allowed       */
+                                  /*   (KDOP_OPT_MAX_ROUNDS)                                      */
                                   /*   runs unbudgeted.  Results unchanged                      */
 #define KDOP_OPT_QUAD_ITEMS 17    /* quad kernel: items each quad expands per iteration, 1 (0:  */
                                   /*   the default) or 2 (twice as wide an iteration, 4-wave     */
