@@ -649,7 +649,7 @@ def main():
     ap.add_argument("--n", type=int, default=40, help="cubes per box edge (5 n^3 tets per body)")
     ap.add_argument("--batched-pairs", type=int, default=512)
     ap.add_argument("--batched-k", type=int, default=8)
-    ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--batched-reps", type=int, default=20)
     ap.add_argument("--tune-ms", type=float, default=3000.0, help="autotuner budget for the batched leg (0: off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

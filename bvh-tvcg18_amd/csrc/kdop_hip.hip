@@ -1144,17 +1144,17 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   using clk = std::chrono::steady_clock;
   const auto t_start = clk::now();
   auto elapsed_ms = [&]() { return std::chrono::duration<double, std::milli>(clk::now() - t_start).count(); };
-  // median wall time of 3 calls with the context's current options
+  // median wall time of 5 calls with the context's current options
   auto measure = [&](double* ms) -> int {
-    double t[3];
-    for (int r = 0; r < 3; ++r) {
+    double t[5];
+    for (int r = 0; r < 5; ++r) {
       const auto a = clk::now();
       const int rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr);
       if (rc) return rc;
       t[r] = std::chrono::duration<double, std::milli>(clk::now() - a).count();
     }
-    std::sort(t, t + 3);
-    *ms = t[1];
+    std::sort(t, t + 5);
+    *ms = t[2];
     return KDOP_OK;
   };
   int rc = kdop_traverse(c, npairs, body_pairs, flags, nullptr);  // warm up (buffers sized)
@@ -1200,7 +1200,8 @@ int kdop_autotune(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
       rc = measure(&ms);
       if (rc == KDOP_ERR_CAPACITY) { rc = KDOP_OK; continue; }  // a candidate the call cannot run with: skip it
       if (rc) { kdop_set_option(c, k.opt, cur); return rc; }
-      if (ms < best) { best = ms; best_v = v; }
+      // a candidate must beat the best by more than the calls' noise (1 %)
+      if (ms < 0.99 * best) { best = ms; best_v = v; }
     }
     // an AUTO start that stayed best is pinned to the strategy it ran
     if (k.opt == KDOP_OPT_TRAVERSAL && best_v == KDOP_TRAVERSAL_AUTO)
@@ -1678,7 +1679,12 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   }
   if ((rc = ensure(c, c->front[0], est)) || (rc = ensure(c, c->front[1], est))) return rc;
   const uint32_t pwaves = (uint32_t)(c->num_cus * wpc_of(c));
-  const uint32_t target = c->ramp_target ? c->ramp_target : 16 * pwaves;
+  // the frontier width that ends the ramp: 16 items per persistent wave, 64
+  // for calls of many pairs, whose small trees give each pair's subtree to
+  // a few blocks unless the frontier is wider (512-pair batch: 16 / 32 / 64
+  // / 128 per wave 1.518 / 1.510 / 1.460-1.462 / 1.463-1.466 ms per call; a
+  // 64-pair refit series step 0.857-0.860 at 16 against 0.864-0.866 at 64)
+  const uint32_t target = c->ramp_target ? c->ramp_target : (npairs >= 256 ? 64u : 16u) * pwaves;
   const uint32_t rounds_async = std::min(std::min(std::max(c->rounds_hint, 1u), kQueuedRounds), c->max_rounds);
   c->rounds_queued = rounds_async;
   uint64_t overflowed = 0;

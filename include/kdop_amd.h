@@ -340,7 +340,9 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   quad kernel: 128, 256 or 512); 0: the kernel's default,   */
                                   /*   128 (quad) or 512                                        */
 #define KDOP_OPT_WAVES_PER_CU 3   /* persistent waves per CU, 1..32; 0: 28 (quad) or 8          */
-#define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels        */
+#define KDOP_OPT_RAMP_TARGET 4    /* frontier width that ends the breadth-first levels (0:    */
+                                  /*   16 items per persistent wave, 64 for calls of 256 pairs */
+                                  /*   or more)                                                  */
 #define KDOP_OPT_STACK_KEEP 5     /* persistent rounds: once the frontier is gone, a workgroup  */
                                   /*   whose stack holds more than this many entries per wave   */
                                   /*   plus one hand-off hands its bottom 64 entries per wave  */
@@ -416,7 +418,8 @@ int kdop_get_option(kdop_ctx* ctx, int option, int64_t* value);
  * queue depth, stack keep, breadth-first target, seed budget) for this context by timing
  * kdop_traverse(npairs, body_pairs, flags) itself: coordinate descent over
  * the options in that order, each candidate measured as the median host wall
- * time of 3 calls (a candidate that fails with KDOP_ERR_CAPACITY is skipped),
+ * time of 5 calls and taken only when it beats the best so far by more than
+ * 1 % (a candidate that fails with KDOP_ERR_CAPACITY is skipped),
  * until budget_ms of tuning time is spent.  The best options stay set on the
  * context, and the last traversal's results are those of a run with them.
  * chosen (may be NULL) receives KDOP_AUTOTUNE_CHOSEN values indexed by
