@@ -43,6 +43,13 @@ constexpr uint32_t kRoundItersDefault = 64;
 // 0.186 -> 0.206 ms per step).
 constexpr uint32_t kRoundItersBatch = 24;
 constexpr uint32_t kRoundItersBatchPairs = 64;
+// Calls of at least kManyPairs pairs (many small trees) take a wider ramp
+// target and a larger seed budget by default: 512-pair batch with 65,536 /
+// 262,144 / 1,048,576 seeds 1.457-1.461 / 1.415-1.419 / 1.444-1.446 ms per
+// call, a 64-pair refit series step 0.856-0.857 against 0.931-0.933 ms and a
+// one-pair query 0.189 against 0.205 ms with 262,144.
+constexpr uint32_t kManyPairs = 256;
+constexpr uint32_t kSeedBudgetDefault = 65536, kSeedBudgetManyPairs = 262144;
 
 namespace {
 
@@ -110,7 +117,9 @@ struct kdop_ctx {
   // traversal state
   DevBuf<uint2> dpairs;
   DevBuf<SeedDesc> dseeds;
-  uint32_t seed_budget = 65536;  // seeds (chunk-root pairs tested first) per call
+  uint32_t seed_budget = kSeedBudgetDefault;  // seeds (chunk-root pairs tested first) per call, when set
+  bool seed_budget_set = false;                 // the caller set it; else by the call's pair count
+  uint32_t seed_budget_eff = kSeedBudgetDefault;  // the budget of the current (or last) call
   DevBuf<Item> front[2];
   DevBuf<LeafPair> p1, p2;
   DevBuf<kdop_contact> contacts;
@@ -1070,10 +1079,10 @@ int kdop_set_option(kdop_ctx* c, int option, int64_t v) {
       if (v < 0 || v > (1ll << 20)) break;
       c->keep = v == 0 ? kNoHandOffKeep : (uint32_t)v;
       return KDOP_OK;
-    case KDOP_OPT_SEED_BUDGET:
-      if (v == 0) v = 65536;
-      if (v < 1 || v > (1ll << 30)) break;
-      c->seed_budget = (uint32_t)v;
+    case KDOP_OPT_SEED_BUDGET:  // 0: by the call's pair count
+      if (v < 0 || v > (1ll << 30)) break;
+      c->seed_budget = v ? (uint32_t)v : kSeedBudgetDefault;
+      c->seed_budget_set = v != 0;
       return KDOP_OK;
     case KDOP_OPT_KERNEL_TIMING:
       if (v < 0 || v > 2) break;
@@ -1227,7 +1236,7 @@ int kdop_get_option(kdop_ctx* c, int option, int64_t* v) {
     case KDOP_OPT_WAVES_PER_CU: *v = wpc_of(c); return KDOP_OK;
     case KDOP_OPT_RAMP_TARGET: *v = c->ramp_target; return KDOP_OK;
     case KDOP_OPT_STACK_KEEP: *v = c->keep >= kNoHandOffKeep ? 0 : c->keep; return KDOP_OK;
-    case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget; return KDOP_OK;
+    case KDOP_OPT_SEED_BUDGET: *v = c->seed_budget_set ? c->seed_budget : c->seed_budget_eff; return KDOP_OK;
     case KDOP_OPT_KERNEL_TIMING: *v = c->timing_level; return KDOP_OK;
     case KDOP_OPT_ASYNC_REFIT: *v = c->async_refit ? 1 : 0; return KDOP_OK;
     case KDOP_OPT_NARROW_PATH: *v = c->narrow_path; return KDOP_OK;
@@ -1614,9 +1623,11 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   c->have_traversal = c->have_p1 = c->have_narrow = false;
   int rc = KDOP_OK;
   // pair list and seed plan on the device: uploaded again only when they change
+  const uint32_t seed_budget = c->seed_budget_set ? c->seed_budget : (npairs >= kManyPairs ? kSeedBudgetManyPairs : kSeedBudgetDefault);
+  c->seed_budget_eff = seed_budget;
   const bool same_pairs = c->seed_key.size() == pairs.size() &&
                           (pairs.empty() || std::memcmp(c->seed_key.data(), pairs.data(), pairs.size() * sizeof(uint2)) == 0) &&
-                          c->seed_key_budget == c->seed_budget && c->seed_key_bodies == c->bodies.size();
+                          c->seed_key_budget == seed_budget && c->seed_key_bodies == c->bodies.size();
   uint64_t total_seeds = 0;
   if (same_pairs) {
     total_seeds = c->seed_total;
@@ -1628,7 +1639,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     // pairs fit the per-pair share of the seed budget
     std::vector<SeedDesc> seeds(npairs);
     c->seed_plan_est.assign(npairs, 0);
-    const uint64_t share = std::max<uint64_t>(1, c->seed_budget / std::max<uint32_t>(npairs, 1));
+    const uint64_t share = std::max<uint64_t>(1, seed_budget / std::max<uint32_t>(npairs, 1));
     for (uint32_t i = 0; i < npairs; ++i) {
       const BodyHost& A = c->bodies[pairs[i].x];
       const BodyHost& B = c->bodies[pairs[i].y];
@@ -1650,7 +1661,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
     if ((rc = ensure(c, c->dseeds, std::max<uint32_t>(npairs, 1)))) return rc;
     if (npairs) HIPCHK(c, hipMemcpyAsync(c->dseeds.p, seeds.data(), npairs * sizeof(SeedDesc), hipMemcpyHostToDevice, c->stream));
     c->seed_key = pairs;
-    c->seed_key_budget = c->seed_budget;
+    c->seed_key_budget = seed_budget;
     c->seed_key_bodies = c->bodies.size();
     c->seed_total = total_seeds;
     ++c->seed_gen;
@@ -1684,7 +1695,7 @@ int kdop_traverse(kdop_ctx* c, uint32_t npairs, const uint32_t* body_pairs, uint
   // a few blocks unless the frontier is wider (512-pair batch: 16 / 32 / 64
   // / 128 per wave 1.518 / 1.510 / 1.460-1.462 / 1.463-1.466 ms per call; a
   // 64-pair refit series step 0.857-0.860 at 16 against 0.864-0.866 at 64)
-  const uint32_t target = c->ramp_target ? c->ramp_target : (npairs >= 256 ? 64u : 16u) * pwaves;
+  const uint32_t target = c->ramp_target ? c->ramp_target : (npairs >= kManyPairs ? 64u : 16u) * pwaves;
   const uint32_t rounds_async = std::min(std::min(std::max(c->rounds_hint, 1u), kQueuedRounds), c->max_rounds);
   c->rounds_queued = rounds_async;
   uint64_t overflowed = 0;

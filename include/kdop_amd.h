@@ -51,7 +51,7 @@ typedef struct kdop_config {
   int32_t stack_keep;    /* persistent kernel: stack entries a wave keeps once the frontier is  */
                          /*   exhausted; deeper stacks go to the next round (0 = default: no  */
                          /*   hand-off, KDOP_OPT_STACK_KEEP)                                  */
-  int32_t seed_budget;   /* chunk-root pairs tested to seed one call (0 = default 65536)        */
+  int32_t seed_budget;   /* chunk-root pairs tested to seed one call (0 = by the call's pairs)  */
   int32_t reserved[2];
 } kdop_config;
 
@@ -347,7 +347,8 @@ int kdop_traversal_stats(kdop_ctx* ctx, uint64_t* out);
                                   /*   whose stack holds more than this many entries per wave   */
                                   /*   plus one hand-off hands its bottom 64 entries per wave  */
                                   /*   to the next round; 0 (the default): never                */
-#define KDOP_OPT_SEED_BUDGET 6    /* chunk-root pairs tested to seed one call                */
+#define KDOP_OPT_SEED_BUDGET 6    /* chunk-root pairs tested to seed one call (0: 65,536, or  */
+                                  /*   262,144 for calls of 256 pairs or more)                 */
 #define KDOP_OPT_KERNEL_TIMING 7  /* HIP events for kdop_last_timing: 0 none (default; the     */
                                   /*   timings read 0), 1 phase events, 2 also one pair on each */
                                   /*   persistent round's dispatch; each event costs a few us   */
