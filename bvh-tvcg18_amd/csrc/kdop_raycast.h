@@ -19,19 +19,49 @@ namespace kdop_amd {
 // Entry and exit parameters of the ray through the slabs of a DOP
 // (raycast/geometry_raycast_dop.h:35-75; a slab the ray runs parallel to is
 // skipped, VT::lowest / VT::highest are -/+FLT_MAX).
+//
+// The reference divides (w - d.p) / (d.r) in float at every node.  The ray's
+// part of each axis is fixed for a whole walk, so it is computed once per ray
+// (RaySlabs): d0.p and d1.p as the reference computes them, and y = RN(1/f1),
+// the double reciprocal of f1 = d1.r (0 marks a slab the ray runs parallel
+// to; 1/f1 of a nonzero float is never 0 in double).  d0 = -d1 makes f0 = -f1
+// bit for bit (negation commutes with every rounded product and sum), so
+// 1/f0 = -y exactly.  Each quotient is then float(double(num) * y):
+//  - double(num) * y is within 2^-52 (relative) of num / f;
+//  - a quotient of two 24-bit floats is never a float rounding midpoint and
+//    lies at least 2^-49 (relative) from one: with num = A 2^i, f = B 2^j and
+//    a midpoint M 2^k (A, B < 2^24, M < 2^25 odd), A 2^i - B M 2^k' is a
+//    nonzero multiple of 2^k', so the distance is at least 1 / (B M);
+// so the double product rounds to the same float as the IEEE division,
+// including overflow to infinity and subnormal quotients.
 template <int K>
-__device__ inline void ray_slabs(const f3& p, const f3& r, const float* vol, const DirTable& dt, float& t_min, float& t_max) {
-  t_min = -FLT_MAX;
-  t_max = FLT_MAX;
+struct RaySlabs {
+  float dp0[K / 2], dp1[K / 2];
+  double y[K / 2];
+};
+template <int K>
+__device__ inline void ray_slab_setup(const f3& p, const f3& r, const DirTable& dt, RaySlabs<K>& S) {
 #pragma unroll
   for (int k = 0; k < K / 2; ++k) {
     const f3 d1 = f3{dt.d[3 * k], dt.d[3 * k + 1], dt.d[3 * k + 2]};
     const f3 d0 = f3{-d1.x, -d1.y, -d1.z};
+    const float f1 = dot3(d1, r);
+    S.dp0[k] = dot3(d0, p);
+    S.dp1[k] = dot3(d1, p);
+    S.y[k] = f1 == 0.0f ? 0.0 : 1.0 / (double)f1;
+  }
+}
+template <int K>
+__device__ inline void ray_slabs(const RaySlabs<K>& S, const float* vol, float& t_min, float& t_max) {
+  t_min = -FLT_MAX;
+  t_max = FLT_MAX;
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k) {
+    const double y = S.y[k];
+    if (y == 0.0) continue;  // f0 == 0
     const float w0 = -vol[2 * k], w1 = vol[2 * k + 1];
-    const float f0 = dot3(d0, r), f1 = dot3(d1, r);
-    if (f0 == 0.0f) continue;
-    float t_near = (w0 - dot3(d0, p)) / f0;
-    float t_far = (w1 - dot3(d1, p)) / f1;
+    float t_near = (float)((double)(w0 - S.dp0[k]) * -y);  // (w0 - d0.p) / f0
+    float t_far = (float)((double)(w1 - S.dp1[k]) * y);    // (w1 - d1.p) / f1
     if (t_near > t_far) {
       const float tmp = t_far;
       t_far = t_near;
@@ -45,9 +75,10 @@ __device__ inline void ray_slabs(const f3& p, const f3& r, const float* vol, con
 // compute_raycast_dop with a threshold (raycast/geometry_raycast_dop.h:19-99,
 // outside_dop of geometry_inside.h:19-40); length: the entry parameter.
 template <int K>
-__device__ inline bool raycast_dop(const f3& p, const f3& r, const float* vol, const DirTable& dt, float threshold, float& length) {
+__device__ inline bool raycast_dop(const f3& p, const f3& r, const RaySlabs<K>& S, const float* vol, const DirTable& dt, float threshold,
+                                   float& length) {
   float t_min, t_max;
-  ray_slabs<K>(p, r, vol, dt, t_min, t_max);
+  ray_slabs<K>(S, vol, t_min, t_max);
   length = FLT_MAX;
   if (!(t_min > 0.0f && t_min <= t_max)) return false;
   const f3 q = add3(p, mul3(r, t_min));
@@ -142,7 +173,7 @@ constexpr uint32_t kChunkStack = 16;  // a chunk walk holds at most depth + 1 <=
 // from `length`: returns the best length, hit updated with it.
 template <int K>
 __device__ inline float chunk_raycast(const float* __restrict__ nodes, const BodyDev& B, uint32_t root, const f3& o, const f3& r,
-                                      const DirTable& dt, float length, f3& hit, uint32_t* st) {
+                                      const RaySlabs<K>& S, const DirTable& dt, float length, f3& hit, uint32_t* st) {
   uint32_t sp = 0;
   st[sp++] = root;
   NodeRec<K> rec;
@@ -150,7 +181,7 @@ __device__ inline float chunk_raycast(const float* __restrict__ nodes, const Bod
   while (sp) {
     const uint32_t s = st[--sp];
     load_rec<K>(nodes, s, rec);
-    if (!raycast_dop<K>(o, r, rec.v, dt, 0.01f, t)) continue;  // :52-57
+    if (!raycast_dop<K>(o, r, S, rec.v, dt, 0.01f, t)) continue;  // :52-57
     if (t >= length) continue;
     const uint32_t c0 = rec.m0 & IDX_MASK, c1 = rec.m1 & IDX_MASK;
     if (rec.m0 & F_LEAF) {  // :61-79
@@ -193,8 +224,10 @@ __global__ __launch_bounds__(256) void k_rc_seed_count(const BodyDev* __restrict
   load_ray(orig, dir, i, o, r);
   NodeRec<K> rec;
   load_rec<K>(nodes, B.node_base, rec);
+  RaySlabs<K> S;
+  ray_slab_setup<K>(o, r, dt, S);
   float t;
-  count[i] = raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) ? B.ntop : 0u;
+  count[i] = raycast_dop<K>(o, r, S, rec.v, dt, 0.01f, t) ? B.ntop : 0u;
 }
 
 // A, one step of two levels (as k_sg_expand2_count for the single traversal,
@@ -205,12 +238,12 @@ __global__ __launch_bounds__(256) void k_rc_seed_count(const BodyDev* __restrict
 // behind), itself marked (a branch-chunk root), or its two children.  count /
 // kind as k_sg_expand2_write reads them.
 template <int K>
-__device__ inline uint32_t rc_child_kind(const float* __restrict__ nodes, uint32_t node, const DirTable& dt, const f3& o, const f3& r) {
+__device__ inline uint32_t rc_child_kind(const float* __restrict__ nodes, uint32_t node, const RaySlabs<K>& S) {
   NodeRec<K> rec;
   load_rec<K>(nodes, node, rec);
   if (rec.m0 & F_BRANCH) return 1u;
   float t_min, t_max;
-  ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
+  ray_slabs<K>(S, rec.v, t_min, t_max);
   return (t_min > t_max || t_max <= 0.0f) ? 0u : 2u;
 }
 template <int K>
@@ -226,11 +259,13 @@ __global__ __launch_bounds__(256) void k_rc_expand2_count(const uint2* __restric
   if (rec.m0 & F_BRANCH) { count[j] = 1u; kind[j] = kSelf; return; }
   f3 o, r;
   load_ray(orig, dir, it.x, o, r);
+  RaySlabs<K> S;
+  ray_slab_setup<K>(o, r, dt, S);
   float t_min, t_max;
-  ray_slabs<K>(o, r, rec.v, dt, t_min, t_max);
+  ray_slabs<K>(S, rec.v, t_min, t_max);
   if (t_min > t_max || t_max <= 0.0f) { count[j] = 0u; kind[j] = 0u; return; }
-  const uint32_t k0 = rc_child_kind<K>(nodes, rec.m0 & IDX_MASK, dt, o, r);
-  const uint32_t k1 = rc_child_kind<K>(nodes, rec.m1 & IDX_MASK, dt, o, r);
+  const uint32_t k0 = rc_child_kind<K>(nodes, rec.m0 & IDX_MASK, S);
+  const uint32_t k1 = rc_child_kind<K>(nodes, rec.m1 & IDX_MASK, S);
   count[j] = k0 + k1;
   kind[j] = (uint8_t)(k0 | (k1 << 2));
 }
@@ -258,8 +293,10 @@ __global__ __launch_bounds__(256) void k_rc_roots(const uint2* __restrict__ item
   }
   f3 o, r;
   load_ray(orig, dir, it.x, o, r);
+  RaySlabs<K> S;
+  ray_slab_setup<K>(o, r, dt, S);
   float t;
-  pass[j] = (raycast_dop<K>(o, r, rec.v, dt, 0.01f, t) && t < FLT_MAX) ? 1u : 0u;  // kdop_raycast.h:52-57
+  pass[j] = (raycast_dop<K>(o, r, S, rec.v, dt, 0.01f, t) && t < FLT_MAX) ? 1u : 0u;  // kdop_raycast.h:52-57
 }
 // B, second pass: best[j] = the best length of chunk j's walk from FLT_MAX,
 // for the items in list.
@@ -274,7 +311,9 @@ __global__ __launch_bounds__(256) void k_rc_chunks(const BodyDev* __restrict__ b
   const uint2 it = items[j];
   f3 o, r, hit{0.0f, 0.0f, 0.0f};
   load_ray(orig, dir, it.x, o, r);
-  best[j] = chunk_raycast<K>(nodes, bodies[body], it.y & ~kSgTested, o, r, dt, FLT_MAX, hit, stk[threadIdx.x]);
+  RaySlabs<K> S;
+  ray_slab_setup<K>(o, r, dt, S);
+  best[j] = chunk_raycast<K>(nodes, bodies[body], it.y & ~kSgTested, o, r, S, dt, FLT_MAX, hit, stk[threadIdx.x]);
 }
 
 // first[ray] = its first item (items are grouped by ray; rays without items
@@ -311,6 +350,8 @@ __global__ __launch_bounds__(256) void k_rc_resolve(const BodyDev* __restrict__ 
     const BodyDev& B = bodies[body];
     f3 o, r;
     load_ray(orig, dir, i, o, r);
+    RaySlabs<K> S;
+    ray_slab_setup<K>(o, r, dt, S);
     for (uint32_t base = j0;; base += 64u) {
       const uint32_t j = base + lane;
       const bool mine = j < m && items[j].x == i;
@@ -324,7 +365,7 @@ __global__ __launch_bounds__(256) void k_rc_resolve(const BodyDev* __restrict__ 
         done |= (2ull << k) - 1ull;  // k and every lane before it
         if (lane == 0u) {
           const uint32_t root = items[base + k].y & ~kSgTested;
-          length = chunk_raycast<K>(nodes, B, root, o, r, dt, length, hit, stk[w]);
+          length = chunk_raycast<K>(nodes, B, root, o, r, S, dt, length, hit, stk[w]);
         }
         length = readlane_f(length, 0);
       }
