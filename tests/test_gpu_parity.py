@@ -348,3 +348,29 @@ def test_ramp_stops_where_the_last_call_stopped():
     orc5 = orc[:5]
     dev = device_run(sub, ctx=ctx, ids=ids)
     _check_pair_results(sub, dev, orc5)
+
+
+def test_seed_budget_follows_the_pair_count():
+    """An unset seed budget is 262,144 chunk-root pairs for a call of 256
+    pairs or more and 65,536 below (the option reports the last call's); a
+    budget the caller sets holds for every call until it is set to 0.  Every
+    call gives the oracle's results."""
+    wl = W.batched_pairs(npairs=256, K=8, seed=11, tmin=20, tmax=400)
+    orc, _, _ = oracle_run(wl)
+    one = W.PairWorkload(wl.name, wl.K, wl.meshes, wl.pairs[:1], wl.poses)
+    ctx = K.Context(K=8, traversal=K.KDOP_TRAVERSAL_PERSISTENT)
+    dev = device_run(wl, ctx=ctx)
+    ids = dev["ids"]
+    _check_pair_results(wl, dev, orc)
+    assert ctx.get_option("seed_budget") == 262144
+    dev = device_run(one, ctx=ctx, ids=ids)
+    _check_pair_results(one, dev, orc[:1])
+    assert ctx.get_option("seed_budget") == 65536
+    ctx.set_option("seed_budget", 4096)
+    dev = device_run(wl, ctx=ctx, ids=ids)
+    _check_pair_results(wl, dev, orc)
+    assert ctx.get_option("seed_budget") == 4096
+    ctx.set_option("seed_budget", 0)
+    dev = device_run(wl, ctx=ctx, ids=ids)
+    _check_pair_results(wl, dev, orc)
+    assert ctx.get_option("seed_budget") == 262144
